@@ -1,0 +1,218 @@
+/*
+ * atm_mi355x.h -- C ABI of the MI355X ATMMetaForce evaluation library
+ * (libatm_mi355x.so).
+ *
+ * One atm_context_t owns a batch of independent ATM systems resident on one
+ * GPU.  A system is an orthorhombic periodic box of N atoms with per-atom
+ * charge, Lennard-Jones sigma/epsilon and an ATM displacement vector, plus the
+ * nine ATM global parameters.  One evaluation computes, for every system,
+ *
+ *   U1, F1 = cutoff LJ + reaction-field energy and forces at x        (state 1)
+ *   U2, F2 = the same at x + d                                        (state 2)
+ *   u      = direction > 0 ? U2 - U1 : U1 - U2, soft-core mapped -> u_sc, f'
+ *   W      = e0 + [alpha>0] (l2-l1)/alpha ln(1+exp(-alpha(u_sc-u0))) + l2 u_sc + w0
+ *   F     += (1-sp) F_e0 + sp F_other,   sp = f' ((l2-l1)/(1+exp(-alpha(u_sc-u0))) + l1)
+ *
+ * which is what the reference computes with two inner OpenMM contexts
+ * (openmmapi/src/ATMMetaForceImpl.cpp:90-128) and the kernel interface
+ * CalcATMMetaForceKernel (openmmapi/include/ATMMetaForceKernels.h:15-59).
+ *
+ * Conventions
+ *  - Units: nm, kJ/mol, e (elementary charge), kJ/mol/nm.
+ *  - Every call returns ATM_OK (0) or a negative status; atm_last_error()
+ *    returns the message of the last failure on the calling thread.  No C++
+ *    exception crosses this boundary.
+ *  - Host arrays are owned by the caller and only read/written during the call.
+ *    Device memory is owned by the context.
+ *  - A context is not thread-safe: calls on one context must not overlap,
+ *    matching the single-threaded use of an OpenMM Context by the reference.
+ *  - Forces accumulate in 64-bit fixed point with ATM_FORCE_SCALE units per
+ *    kJ/mol/nm; the accumulated ATM force of a system is the sum over
+ *    evaluations since its last atm_clear_forces() (ATM_EVAL_CLEAR_FORCES
+ *    clears it first), mirroring the reference's "+=" into the context force
+ *    buffer (platforms/reference/src/ReferenceATMMetaForceKernels.cpp:100-110).
+ */
+#ifndef ATM_MI355X_H_
+#define ATM_MI355X_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ATM_MI355X_ABI_VERSION 1
+
+/* status codes */
+#define ATM_OK 0
+#define ATM_ERR_INVALID_ARGUMENT (-1)
+#define ATM_ERR_DEVICE (-2)          /* HIP runtime failure */
+#define ATM_ERR_CAPACITY (-3)        /* a neighbour structure overflowed twice */
+#define ATM_ERR_STATE (-4)           /* call out of order (e.g. results before eval) */
+#define ATM_ERR_UNSUPPORTED (-5)
+
+/* fixed-point force units per kJ/mol/nm (2^24) */
+#define ATM_FORCE_SCALE 16777216.0
+
+/* Coulomb constant 1/(4 pi eps0) in kJ nm/(mol e^2), OpenMM 7.7's ONE_4PI_EPS0 */
+#define ATM_ONE_4PI_EPS0 138.935456
+
+/* atm_eval flags */
+#define ATM_EVAL_FORCES        0x1  /* accumulate the hybrid ATM force */
+#define ATM_EVAL_CLEAR_FORCES  0x2  /* zero the accumulated force first */
+#define ATM_EVAL_PLAIN         0x4  /* reference-structured path: one system and one
+                                       coordinate set per launch sequence, U1/U2 read
+                                       back to the host, soft-core/softplus on the host,
+                                       hybrid-force kernel with a host scalar */
+#define ATM_EVAL_NO_GRAPH      0x8  /* batched path launched kernel by kernel instead
+                                       of through the captured HIP graph */
+#define ATM_EVAL_FP32_PAIRS    0x10 /* scalar pair function in FP32 (default: FP64 scalars
+                                       from the exact FP64 r^2; coordinates, force vectors
+                                       and their accumulation are FP32 in both modes) */
+
+/* One system of the batch.  Box edges must satisfy cutoff <= box/2. */
+typedef struct atm_system_desc {
+    int32_t num_atoms;
+    double box[3];          /* orthorhombic periodic box edge lengths, nm */
+    double cutoff;          /* nm */
+    double rf_dielectric;   /* reaction-field dielectric constant */
+} atm_system_desc;
+
+/* The nine ATM global parameters, named as in ATMMetaForce.h:151-218. */
+typedef struct atm_params {
+    double lambda1, lambda2, alpha, u0, w0, umax, ubcore, acore, direction;
+} atm_params;
+
+/* Per-system results of the last evaluation. */
+typedef struct atm_result {
+    double energy;               /* W: the ATM alchemical potential energy */
+    double perturbation_energy;  /* u_sc: soft-core perturbation energy
+                                    (what ATMMetaForce::getPerturbationEnergy returns) */
+    double state1_energy;        /* U1 (variable-group energy at x) */
+    double state2_energy;        /* U2 (variable-group energy at x + d) */
+    double raw_perturbation;     /* u before the soft-core map */
+    double softcore_derivative;  /* f' = du_sc/du */
+    double softplus_derivative;  /* bfp = dW/du_sc */
+    double hybrid_weight;        /* sp = bfp f' (weight of the non-e0 state's force) */
+} atm_result;
+
+/* Autotuned launch configuration (read back with atm_get_tuning). */
+typedef struct atm_tuning {
+    double column_scale;      /* column edge / (8/density)^(1/3) */
+    int32_t pair_waves_per_block;
+    int32_t zbins_per_column; /* z-bins per column edge */
+    double best_ms;           /* measured time of the chosen configuration */
+} atm_tuning;
+
+/* Library version string and ABI number. */
+const char* atm_version(void);
+int32_t atm_abi_version(void);
+const char* atm_last_error(void);
+
+/* Number of visible GPUs (0 when none). */
+int atm_device_count(int32_t* count);
+
+/* Create a context holding `num_systems` systems on GPU `device`.
+ * Replaces CalcATMMetaForceKernel::initialize (ATMMetaForceKernels.h:28) and the
+ * inner-context setup of ATMMetaForceImpl::initialize / calcForcesAndEnergy
+ * (ATMMetaForceImpl.cpp:69-101). */
+int atm_create(const atm_system_desc* systems, int32_t num_systems, int32_t device,
+               void** out_context);
+
+int atm_destroy(void* context);
+
+/* Per-atom nonbonded parameters and ATM displacements of system `sys`
+ * (arrays of num_atoms; displacements num_atoms*3).  Entry i displaces atom i,
+ * as the reference does (ReferenceATMMetaForceKernels.cpp:120-123).
+ * Replaces CalcATMMetaForceKernel::copyParametersToContext (ATMMetaForceKernels.h:46). */
+int atm_set_particles(void* context, int32_t sys, const double* charges,
+                      const double* sigmas, const double* epsilons,
+                      const double* displacements);
+
+/* Displacements only (num_atoms*3), the part copyParametersToContext updates in
+ * the reference (CommonATMMetaForceKernels.cpp:229-251). */
+int atm_set_displacements(void* context, int32_t sys, const double* displacements);
+
+/* Positions (num_atoms*3, nm) of system `sys`; the device path stores them in FP32.
+ * Replaces the position half of CalcATMMetaForceKernel::copyState
+ * (ATMMetaForceKernels.h:55; ReferenceATMMetaForceKernels.cpp:116-124). */
+int atm_set_positions(void* context, int32_t sys, const double* positions);
+
+/* Positions of every system at once from a device array (float32, 3 per atom,
+ * systems concatenated in order); the copy is enqueued on the context stream. */
+int atm_set_positions_device(void* context, const float* positions_dev);
+
+/* New box edges for system `sys` (copyState copies box vectors, :130-133). */
+int atm_set_box(void* context, int32_t sys, const double* box3);
+
+/* The nine ATM global parameters of system `sys` (read by execute,
+ * ReferenceATMMetaForceKernels.cpp:66-80). */
+int atm_set_params(void* context, int32_t sys, const atm_params* params);
+
+/* Evaluate every system (ATM_EVAL_* flags).  Enqueues on the context stream and
+ * returns; atm_get_results/atm_get_forces synchronize.
+ * Replaces ATMMetaForceImpl::calcForcesAndEnergy (ATMMetaForceImpl.cpp:90-128)
+ * with CalcATMMetaForceKernel::copyState + execute (ATMMetaForceKernels.h:37-55). */
+int atm_eval(void* context, int32_t flags);
+
+/* Wait for all work enqueued on the context stream. */
+int atm_synchronize(void* context);
+
+/* Results of the last evaluation of system `sys`.
+ * perturbation_energy is CalcATMMetaForceKernel::getPerturbationEnergy
+ * (ATMMetaForceKernels.h:57). */
+int atm_get_results(void* context, int32_t sys, atm_result* out);
+
+/* Accumulated ATM force of system `sys` (num_atoms*3 doubles, kJ/mol/nm). */
+int atm_get_forces(void* context, int32_t sys, double* forces);
+
+/* Accumulated ATM force as raw fixed-point integers (num_atoms*3). */
+int atm_get_forces_fixed(void* context, int32_t sys, int64_t* forces);
+
+/* Zero the accumulated force of every system. */
+int atm_clear_forces(void* context);
+
+/* Parity: the in-cutoff pair set of system `sys`, coordinate set `state`
+ * (0: x, 1: x + d) found by the last evaluation.  Writes min(count, capacity)
+ * pairs as (i, j) atom indices with i < j, sorted, and the total count. */
+int atm_get_pairs(void* context, int32_t sys, int32_t state, int32_t* pairs,
+                  int64_t capacity, int64_t* count);
+
+/* Gradient of sum_s gW[s] W_s with respect to per-atom charges, sigmas and
+ * epsilons (each sum_s num_atoms doubles, systems concatenated) and the
+ * per-system softplus parameters (num_systems x 5 doubles in the order
+ * lambda1, lambda2, alpha, u0, w0).  Recomputes the pair terms at the last
+ * evaluated positions; any output pointer may be NULL. */
+int atm_backward(void* context, const double* grad_energy, double* d_charges,
+                 double* d_sigmas, double* d_epsilons, double* d_params);
+
+/* Use a caller-provided HIP stream (hipStream_t) instead of the context's own
+ * (NULL restores the context stream). */
+int atm_set_stream(void* context, void* hip_stream);
+
+/* Time the candidate launch configurations on the current inputs and keep the
+ * fastest (cell shape, z-binning, waves per block of the pair kernel). */
+int atm_autotune(void* context, int32_t repeats, atm_tuning* out);
+int atm_get_tuning(void* context, atm_tuning* out);
+int atm_set_tuning(void* context, const atm_tuning* tuning);
+
+/* Per-stage device times (ms, averaged over `reps` batched evaluations launched
+ * kernel by kernel with HIP events between stages on the context stream).
+ * build = clear + binning + scan + scatter + cell sort + cluster boxes. */
+typedef struct atm_stage_times {
+    double build_ms, pairlist_ms, pairs_ms, energy_ms, blend_ms, total_ms;
+    int32_t reps;
+} atm_stage_times;
+int atm_profile(void* context, int32_t flags, int32_t reps, atm_stage_times* out);
+
+/* Counters of the last evaluation, for measurement: number of in-cutoff pairs
+ * (both states, all systems), candidate pairs evaluated by the pair kernel,
+ * and neighbour-list entries. */
+int atm_get_stats(void* context, int64_t* pairs_in_cutoff, int64_t* pairs_evaluated,
+                  int64_t* list_entries);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ATM_MI355X_H_ */

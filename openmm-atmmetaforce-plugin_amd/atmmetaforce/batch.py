@@ -1,0 +1,215 @@
+"""Batched ATM evaluation on one MI355X through the C ABI (include/atm_mi355x.h).
+
+A batch holds independent systems (lambda replicas of one system, or systems of
+uneven size).  One evaluation computes, for every system, U1/F1 at x, U2/F2 at
+x + d, the soft-core/softplus combine and the hybrid force, in one captured HIP
+graph; the reference-structured path (one system and one coordinate set per
+launch sequence, energies read back to the host) is available with plain=True.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Sequence
+
+import numpy as np
+
+from . import _capi as C
+
+
+def _dptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _params(p) -> C.Params:
+    if isinstance(p, C.Params):
+        return p
+    if hasattr(p, "as_tuple"):
+        return C.Params(*p.as_tuple())
+    if isinstance(p, dict):
+        return C.Params(**p)
+    return C.Params(*p)
+
+
+class ATMBatch:
+    """A context of `len(systems)` ATM systems on GPU `device`.
+
+    `systems` are objects with positions (N,3), charges, sigmas, epsilons (N,),
+    displacements (N,3), box (3,), cutoff, rf_dielectric and params (nine ATM
+    parameters) -- e.g. atmmetaforce.workloads.System.
+    """
+
+    def __init__(self, systems: Sequence, device: int = 0):
+        self._lib = C.load()
+        self.systems = list(systems)
+        n = len(self.systems)
+        descs = (C.SystemDesc * n)()
+        for k, s in enumerate(self.systems):
+            descs[k].num_atoms = int(s.positions.shape[0])
+            for a in range(3):
+                descs[k].box[a] = float(s.box[a])
+            descs[k].cutoff = float(s.cutoff)
+            descs[k].rf_dielectric = float(s.rf_dielectric)
+        h = ctypes.c_void_p()
+        C.check(self._lib.atm_create(descs, n, device, ctypes.byref(h)))
+        self._h = h
+        self.num_atoms = [int(s.positions.shape[0]) for s in self.systems]
+        for k, s in enumerate(self.systems):
+            self.set_particles(k, s.charges, s.sigmas, s.epsilons, s.displacements)
+            self.set_positions(k, s.positions)
+            self.set_params(k, s.params)
+
+    # ------------------------------------------------------------ lifecycle
+    def close(self):
+        if getattr(self, "_h", None):
+            C.check(self._lib.atm_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    # ------------------------------------------------------------ inputs
+    def set_particles(self, k, charges, sigmas, epsilons, displacements=None):
+        q = np.ascontiguousarray(charges, dtype=np.float64)
+        s = np.ascontiguousarray(sigmas, dtype=np.float64)
+        e = np.ascontiguousarray(epsilons, dtype=np.float64)
+        n = self.num_atoms[k]
+        if q.shape != (n,) or s.shape != (n,) or e.shape != (n,):
+            raise ValueError(f"system {k}: particle arrays must have shape ({n},)")
+        d = None
+        if displacements is not None:
+            d = np.ascontiguousarray(displacements, dtype=np.float64).reshape(n, 3)
+        C.check(self._lib.atm_set_particles(self._h, k, _dptr(q), _dptr(s), _dptr(e),
+                                            _dptr(d) if d is not None else None))
+
+    def set_displacements(self, k, displacements):
+        d = np.ascontiguousarray(displacements, dtype=np.float64).reshape(self.num_atoms[k], 3)
+        C.check(self._lib.atm_set_displacements(self._h, k, _dptr(d)))
+
+    def set_positions(self, k, positions):
+        x = np.ascontiguousarray(positions, dtype=np.float64).reshape(self.num_atoms[k], 3)
+        C.check(self._lib.atm_set_positions(self._h, k, _dptr(x)))
+
+    def set_positions_device(self, ptr: int):
+        C.check(self._lib.atm_set_positions_device(self._h, ctypes.c_void_p(ptr)))
+
+    def set_box(self, k, box):
+        b = np.ascontiguousarray(box, dtype=np.float64).reshape(3)
+        C.check(self._lib.atm_set_box(self._h, k, _dptr(b)))
+
+    def set_params(self, k, params):
+        p = _params(params)
+        C.check(self._lib.atm_set_params(self._h, k, ctypes.byref(p)))
+
+    def set_stream(self, stream_ptr: int | None):
+        C.check(self._lib.atm_set_stream(self._h, ctypes.c_void_p(stream_ptr) if stream_ptr else None))
+
+    # ------------------------------------------------------------ evaluation
+    def evaluate(self, forces: bool = True, clear: bool = True, plain: bool = False, graph: bool = True,
+                 fp32_pairs: bool = False):
+        """Enqueue one evaluation of every system (asynchronous)."""
+        flags = C.ATM_EVAL_FP32_PAIRS if fp32_pairs else 0
+        if forces:
+            flags |= C.ATM_EVAL_FORCES
+        if clear:
+            flags |= C.ATM_EVAL_CLEAR_FORCES
+        if plain:
+            flags |= C.ATM_EVAL_PLAIN
+        if not graph:
+            flags |= C.ATM_EVAL_NO_GRAPH
+        C.check(self._lib.atm_eval(self._h, flags))
+
+    def synchronize(self):
+        C.check(self._lib.atm_synchronize(self._h))
+
+    def compute(self, forces: bool = True, plain: bool = False, graph: bool = True, retries: int = 3,
+                fp32_pairs: bool = False):
+        """Evaluate (clearing forces) and wait; re-runs if a neighbour list overflowed."""
+        for attempt in range(retries + 1):
+            self.evaluate(forces=forces, clear=True, plain=plain, graph=graph, fp32_pairs=fp32_pairs)
+            r = C.Result()
+            rc = self._lib.atm_get_results(self._h, 0, ctypes.byref(r))
+            if rc == C.ATM_ERR_CAPACITY and attempt < retries:
+                continue
+            C.check(rc)
+            return
+
+    def result(self, k) -> dict:
+        r = C.Result()
+        C.check(self._lib.atm_get_results(self._h, k, ctypes.byref(r)))
+        return r.as_dict()
+
+    def results(self):
+        return [self.result(k) for k in range(len(self.systems))]
+
+    def forces(self, k) -> np.ndarray:
+        out = np.zeros((self.num_atoms[k], 3))
+        C.check(self._lib.atm_get_forces(self._h, k, _dptr(out)))
+        return out
+
+    def forces_fixed(self, k) -> np.ndarray:
+        out = np.zeros((self.num_atoms[k], 3), dtype=np.int64)
+        C.check(self._lib.atm_get_forces_fixed(self._h, k, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))))
+        return out
+
+    def clear_forces(self):
+        C.check(self._lib.atm_clear_forces(self._h))
+
+    def pairs(self, k, state) -> np.ndarray:
+        """Sorted (i, j) in-cutoff pairs (i < j) of system k, state 0 (x) or 1 (x + d)."""
+        cnt = ctypes.c_int64()
+        C.check(self._lib.atm_get_pairs(self._h, k, state, None, 0, ctypes.byref(cnt)))
+        out = np.zeros((max(cnt.value, 1), 2), dtype=np.int32)
+        C.check(self._lib.atm_get_pairs(self._h, k, state, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                        cnt.value, ctypes.byref(cnt)))
+        return out[:cnt.value]
+
+    def stats(self) -> dict:
+        a, b, c = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        C.check(self._lib.atm_get_stats(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return {"pairs_in_cutoff": a.value, "pairs_evaluated": b.value, "list_entries": c.value}
+
+    def backward(self, grad_energy) -> dict:
+        g = np.ascontiguousarray(grad_energy, dtype=np.float64).reshape(len(self.systems))
+        tot = sum(self.num_atoms)
+        dq, ds, de = np.zeros(tot), np.zeros(tot), np.zeros(tot)
+        dp = np.zeros(5 * len(self.systems))
+        C.check(self._lib.atm_backward(self._h, _dptr(g), _dptr(dq), _dptr(ds), _dptr(de), _dptr(dp)))
+        return {"dq": dq, "dsigma": ds, "deps": de, "dparams": dp.reshape(-1, 5)}
+
+    def profile(self, reps: int = 10, forces: bool = True, fp32_pairs: bool = False) -> dict:
+        """Per-stage device times (ms) of the batched evaluation, HIP events between stages."""
+        t = C.StageTimes()
+        flags = C.ATM_EVAL_CLEAR_FORCES | (C.ATM_EVAL_FORCES if forces else 0) | \
+            (C.ATM_EVAL_FP32_PAIRS if fp32_pairs else 0)
+        C.check(self._lib.atm_profile(self._h, flags, reps, ctypes.byref(t)))
+        return t.as_dict()
+
+    # ------------------------------------------------------------ tuning
+    def autotune(self, repeats: int = 3) -> dict:
+        t = C.Tuning()
+        C.check(self._lib.atm_autotune(self._h, repeats, ctypes.byref(t)))
+        return {"column_scale": t.column_scale, "pair_waves_per_block": t.pair_waves_per_block,
+                "zbins_per_column": t.zbins_per_column, "best_ms": t.best_ms}
+
+    def tuning(self) -> dict:
+        t = C.Tuning()
+        C.check(self._lib.atm_get_tuning(self._h, ctypes.byref(t)))
+        return {"column_scale": t.column_scale, "pair_waves_per_block": t.pair_waves_per_block,
+                "zbins_per_column": t.zbins_per_column, "best_ms": t.best_ms}
+
+    def set_tuning(self, column_scale=1.0, pair_waves_per_block=4, zbins_per_column=4):
+        t = C.Tuning(column_scale, pair_waves_per_block, zbins_per_column, 0.0)
+        C.check(self._lib.atm_set_tuning(self._h, ctypes.byref(t)))

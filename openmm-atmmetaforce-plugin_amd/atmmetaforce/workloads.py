@@ -1,0 +1,172 @@
+"""Seeded synthetic ATM workloads: jittered-lattice LJ + reaction-field systems.
+
+Every coordinate (positions, displacements, box edges) is rounded to a 2^-20 nm
+grid.  Below 16 nm such values are exact in FP32, so are their differences and
+minimum-image shifts, and the squared distance of two of them is exact in FP64.
+That makes the in-cutoff pair set of a system a well-defined integer object that
+the FP32 device path and the FP64 CPU path must reproduce bit for bit.
+
+Generator (one system):
+  * n = ceil(N^(1/3)) lattice sites per edge, spacing a = L/n; site (ix, iy, iz)
+    sits at ((ix+0.5)a, (iy+0.5)a, (iz+0.5)a), sites ordered with ix slowest and
+    iz fastest, and the first N sites are used;
+  * every coordinate jittered by U(-0.03, 0.03) nm;
+  * sigma ~ U[0.25, 0.35] nm, epsilon ~ U[0.1, 0.8] kJ/mol,
+    q ~ N(0, 0.4) e shifted to zero net charge;
+  * the ligand is the first n_lig atoms; they carry the displacement d, every
+    other atom carries (0, 0, 0);
+  * draws, in order, from numpy.random.default_rng(seed): jitter (N x 3),
+    sigma (N), epsilon (N), q (N).
+
+The ATM parameter names and meanings follow the reference's global parameters
+(openmmapi/include/ATMMetaForce.h:151-218); the direction convention
+(> 0 means "from the reference state to the displaced state") follows
+platforms/reference/src/ReferenceATMMetaForceKernels.cpp:84-85.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+from typing import List
+
+import numpy as np
+
+GRID = 2.0 ** -20  # nm
+
+
+def quantize(v):
+    """Round to the 2^-20 nm grid (exact in FP32 for |v| < 16 nm)."""
+    return np.round(np.asarray(v, dtype=np.float64) / GRID) * GRID
+
+
+@dataclasses.dataclass
+class ATMParams:
+    """The nine ATM global parameters (ATMMetaForce.h:151-218)."""
+    lambda1: float = 0.5
+    lambda2: float = 0.5
+    alpha: float = 0.1      # (kJ/mol)^-1
+    u0: float = 110.0       # kJ/mol
+    w0: float = 0.0         # kJ/mol
+    umax: float = 200.0     # kJ/mol
+    ubcore: float = 100.0   # kJ/mol
+    acore: float = 0.0625
+    direction: float = 1.0
+
+    def as_tuple(self):
+        return (self.lambda1, self.lambda2, self.alpha, self.u0, self.w0,
+                self.umax, self.ubcore, self.acore, self.direction)
+
+
+@dataclasses.dataclass
+class System:
+    """One ATM system: N atoms in an orthorhombic periodic box."""
+    positions: np.ndarray      # (N, 3) float64, nm
+    charges: np.ndarray        # (N,) float64, e
+    sigmas: np.ndarray         # (N,) float64, nm
+    epsilons: np.ndarray       # (N,) float64, kJ/mol
+    displacements: np.ndarray  # (N, 3) float64, nm
+    box: np.ndarray            # (3,) float64, nm
+    cutoff: float              # nm
+    rf_dielectric: float = 78.5
+    params: ATMParams = dataclasses.field(default_factory=ATMParams)
+    seed: int = 0
+    num_ligand: int = 0
+
+    @property
+    def num_atoms(self) -> int:
+        return int(self.positions.shape[0])
+
+
+def lattice_system(num_atoms: int, box: float, cutoff: float, num_ligand: int,
+                   displacement, seed: int, params: ATMParams | None = None,
+                   rf_dielectric: float = 78.5) -> System:
+    rng = np.random.default_rng(seed)
+    n = int(math.ceil(num_atoms ** (1.0 / 3.0) - 1e-9))
+    while n ** 3 < num_atoms:
+        n += 1
+    L = float(quantize(box))
+    a = L / n
+    idx = np.arange(num_atoms)
+    ix, iy, iz = idx // (n * n), (idx // n) % n, idx % n
+    sites = (np.stack([ix, iy, iz], axis=1).astype(np.float64) + 0.5) * a
+    jitter = rng.uniform(-0.03, 0.03, size=(num_atoms, 3))
+    sig = rng.uniform(0.25, 0.35, size=num_atoms)
+    eps = rng.uniform(0.1, 0.8, size=num_atoms)
+    q = rng.normal(0.0, 0.4, size=num_atoms)
+    q = q - q.mean()
+    pos = quantize(sites + jitter)
+    d = np.zeros((num_atoms, 3))
+    d[:num_ligand] = np.asarray(displacement, dtype=np.float64)
+    d = quantize(d)
+    return System(positions=pos, charges=q, sigmas=sig, epsilons=eps,
+                  displacements=d, box=np.array([L, L, L]), cutoff=float(cutoff),
+                  rf_dielectric=rf_dielectric,
+                  params=params if params is not None else ATMParams(),
+                  seed=seed, num_ligand=num_ligand)
+
+
+def config1(seed: int = 1) -> System:
+    """3000 atoms, 3.1 nm box, 0.9 nm cutoff, 24-atom ligand, d=(1.5,1.5,1.5).
+
+    The lattice spacing is L/ceil(N^(1/3)) = 3.1/15 nm: a 0.31 nm lattice in a
+    3.1 nm box holds only 1000 sites, so N, L and the cutoff are kept.
+    """
+    return lattice_system(3000, 3.1, 0.9, 24, (1.5, 1.5, 1.5), seed,
+                          ATMParams(0.5, 0.5, 0.1, 110.0, 0.0, 200.0, 100.0, 0.0625, 1.0))
+
+
+def config2(seed: int = 2) -> System:
+    """23,000 atoms, 8.8 nm box, 1.0 nm cutoff, 40-atom ligand, d=(2.2,2.2,2.2)."""
+    return lattice_system(23000, 8.8, 1.0, 40, (2.2, 2.2, 2.2), seed,
+                          ATMParams(0.5, 0.5, 0.1, 110.0, 0.0, 200.0, 100.0, 0.0625, 1.0))
+
+
+def lambda_schedule(num_replicas: int = 22):
+    """lambda2_k = 0.5 k/(K-1); lambda1_k = lambda2_k for even k, lambda2_{k-1} for odd k."""
+    lam2 = [0.5 * k / (num_replicas - 1) for k in range(num_replicas)]
+    lam1 = [lam2[k] if k % 2 == 0 else lam2[k - 1] for k in range(num_replicas)]
+    return lam1, lam2
+
+
+def replica_batch(num_replicas: int = 22, base: System | None = None,
+                  noise: float = 0.01) -> List[System]:
+    """Copies of the 23k-atom system with N(0, noise) nm jitter and a lambda schedule.
+
+    Replica k draws its noise from default_rng(1000 + k).
+    """
+    base = base if base is not None else config2()
+    lam1, lam2 = lambda_schedule(num_replicas)
+    out = []
+    for k in range(num_replicas):
+        rng = np.random.default_rng(1000 + k)
+        pos = quantize(base.positions + rng.normal(0.0, noise, size=base.positions.shape))
+        p = dataclasses.replace(base.params, lambda1=lam1[k], lambda2=lam2[k])
+        out.append(dataclasses.replace(base, positions=pos, params=p, seed=1000 + k))
+    return out
+
+
+def uneven_batch(num_systems: int = 64, min_atoms: int = 5000, max_atoms: int = 60000,
+                 cutoff: float = 1.0, seed: int = 4000) -> List[System]:
+    """Systems with log-uniform atom counts at the 23k-system density (33.75 nm^-3).
+
+    Batch-level draws from default_rng(seed): per system, in order, the atom
+    count, the ligand size U{10..80} and two U[0, 0.5] lambdas (sorted).
+    System s's atoms are drawn from default_rng(seed + 1 + s).  d = (L/4)(1,1,1).
+    """
+    rho = 23000.0 / 8.8 ** 3
+    rng = np.random.default_rng(seed)
+    out = []
+    for s in range(num_systems):
+        n = int(round(math.exp(rng.uniform(math.log(min_atoms), math.log(max_atoms)))))
+        nlig = int(rng.integers(10, 81))
+        l1, l2 = sorted(rng.uniform(0.0, 0.5, size=2))
+        L = (n / rho) ** (1.0 / 3.0)
+        p = ATMParams(float(l1), float(l2), 0.1, 110.0, 0.0, 200.0, 100.0, 0.0625, 1.0)
+        out.append(lattice_system(n, L, cutoff, nlig, (L / 4, L / 4, L / 4), seed + 1 + s, p))
+    return out
+
+
+def tiny_system(num_atoms: int, box: float, cutoff: float, num_ligand: int,
+                displacement, seed: int, params: ATMParams | None = None) -> System:
+    """Small systems for exhaustive tests (same generator, any size)."""
+    return lattice_system(num_atoms, box, cutoff, num_ligand, displacement, seed, params)

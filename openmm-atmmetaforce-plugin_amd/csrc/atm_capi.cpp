@@ -1,0 +1,995 @@
+// Host side of the C ABI declared in include/atm_mi355x.h.
+//
+// A context owns every device buffer of a batch of systems, one HIP stream and,
+// per flag combination, one captured HIP graph of the batched evaluation.
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "atm_internal.h"
+#include "atm_kernels.h"
+#include "atm_math.h"
+#include "../../include/atm_mi355x.h"
+
+namespace atm {
+// defined in atm_backward.hip
+hipError_t launch_backward(const DevPtrs& P, long long totalSlots, int setBegin, int nsets, int maxGroups,
+                           const double* wset, hipStream_t st);
+hipError_t launch_backward_gather(const DevPtrs& P, int sysBegin, int nsys, int maxAtoms, double* dq, double* dsig,
+                                  double* deps, hipStream_t st);
+hipError_t launch_set_positions(DevPtrs P, float4* posq, const float* xyz, long long n, hipStream_t st);
+}
+
+using namespace atm;
+
+namespace {
+
+thread_local std::string g_lastError;
+
+struct AtmError : std::runtime_error {
+    int code;
+    AtmError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHK(expr)                                                                                   \
+    do {                                                                                               \
+        hipError_t e_ = (expr);                                                                        \
+        if (e_ != hipSuccess)                                                                          \
+            throw AtmError(ATM_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));         \
+    } while (0)
+
+void fail(int code, const std::string& msg) { throw AtmError(code, msg); }
+
+template <typename F>
+int guarded(F&& f) {
+    try {
+        f();
+        return ATM_OK;
+    } catch (const AtmError& e) {
+        g_lastError = e.what();
+        return e.code;
+    } catch (const std::exception& e) {
+        g_lastError = e.what();
+        return ATM_ERR_INVALID_ARGUMENT;
+    }
+}
+
+// Device allocation owned by a context.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    void alloc(size_t n) {
+        release();
+        bytes = std::max<size_t>(n, 256);
+        HIPCHK(hipMalloc(&p, bytes));
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    ~DevBuf() { release(); }
+    template <typename T> T* as() const { return static_cast<T*>(p); }
+};
+
+struct Tuning {
+    double colScale = 1.0;
+    int pairWaves = 4;
+    int zbinsPerColumn = 4;
+};
+
+struct Context {
+    int device = 0;
+    hipStream_t own = nullptr;
+    hipStream_t stream = nullptr;
+    int nsys = 0;
+    std::vector<atm_system_desc> sys;
+    std::vector<int> atomOff;
+    long long totalAtoms = 0;
+    int maxAtoms = 0;
+    std::vector<SetDesc> sets;
+    std::vector<int> entryCapOverride;  // per set, 0 = automatic
+    int maxClusters = 0, maxGroups = 0, maxCells = 0;
+    long long totalCells = 0, totalCols = 0, totalSlots = 0, totalClusters = 0, totalGroups = 0, totalEntries = 0;
+    Tuning tuning;
+    double tunedMs = 0.0;
+
+    // device buffers
+    DevBuf dDesc, dRt, dPosq, dDispl, dLjp, dPrm64, dParams, dRes, dOutF;
+    DevBuf dBpos, dBcell, dBrank, dSlotOf, dCellCount, dCellStart, dColStart, dColCount;
+    DevBuf dSpos, dSprm, dSprm64, dSidx, dSfx, dSfy, dSfz, dSdx, dSdy, dSdz, dBbc, dBbh, dGinfo, dGcount, dGenergy, dEntries, dSetEnergy;
+    DevBuf dGq, dGsig, dGeps, dWset, dDq, dDsig, dDeps;
+    DevPtrs P{};
+
+    // host mirrors
+    std::vector<float4> hPosq, hDispl;
+    std::vector<float2> hLjp;
+    std::vector<double4> hPrm64;
+    std::vector<atm_params> params;
+    std::vector<char> haveParticles;
+    std::vector<atm_result> results;
+    bool resultsValid = false;
+    bool resultsOnHost = false;  // reference-structured path computed results on the host
+    bool evaluated = false;
+
+    std::map<int, hipGraphExec_t> graphs;
+
+    ~Context() {
+        dropGraphs();
+        if (own) (void)hipStreamDestroy(own);
+    }
+    void dropGraphs() {
+        for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
+        graphs.clear();
+    }
+};
+
+void check_ctx(void* c) {
+    if (!c) fail(ATM_ERR_INVALID_ARGUMENT, "null context");
+}
+Context& ctx_of(void* c) {
+    check_ctx(c);
+    return *static_cast<Context*>(c);
+}
+void check_sys(const Context& C, int s) {
+    if (s < 0 || s >= C.nsys) fail(ATM_ERR_INVALID_ARGUMENT, "system index " + std::to_string(s) + " out of range");
+}
+
+void validate_desc(const atm_system_desc& d, int s) {
+    const std::string pre = "system " + std::to_string(s) + ": ";
+    if (d.num_atoms <= 0) fail(ATM_ERR_INVALID_ARGUMENT, pre + "num_atoms must be positive");
+    if (d.num_atoms > (1 << 23)) fail(ATM_ERR_INVALID_ARGUMENT, pre + "more than 2^23 atoms");
+    if (!(d.cutoff > 0.0)) fail(ATM_ERR_INVALID_ARGUMENT, pre + "cutoff must be positive");
+    for (int k = 0; k < 3; ++k) {
+        if (!(d.box[k] > 0.0) || !std::isfinite(d.box[k])) fail(ATM_ERR_INVALID_ARGUMENT, pre + "box edges must be positive");
+        if (d.cutoff > 0.5 * d.box[k])
+            fail(ATM_ERR_INVALID_ARGUMENT, pre + "the cutoff distance cannot be greater than half the periodic box size");
+    }
+    if (!(d.rf_dielectric > 0.0)) fail(ATM_ERR_INVALID_ARGUMENT, pre + "reaction-field dielectric must be positive");
+}
+
+// Geometry and capacities of one coordinate set.
+SetDesc make_set(const atm_system_desc& d, const Tuning& tu, int s, int state) {
+    SetDesc D{};
+    const double L[3] = {d.box[0], d.box[1], d.box[2]};
+    const double rc = d.cutoff;
+    const double rho = d.num_atoms / (L[0] * L[1] * L[2]);
+    const double ceT = tu.colScale * std::cbrt(kClusterSize / rho);
+    const int nx = std::max(1, (int)std::floor(L[0] / ceT));
+    const int ny = std::max(1, (int)std::floor(L[1] / ceT));
+    const double cex = L[0] / nx, cey = L[1] / ny;
+    const double zeT = std::min(cex, cey) / std::max(1, tu.zbinsPerColumn);
+    const int nz = std::max(1, std::min((int)std::floor(L[2] / zeT), 4096));
+    const double rcl = rc * (1.0 + 1e-5) + 1e-4;  // list cutoff: conservative w.r.t. FP32 box rounding
+    int Rx = (int)std::ceil(rcl / cex), Ry = (int)std::ceil(rcl / cey);
+    const int fullX = (2 * Rx + 1 > nx), fullY = (2 * Ry + 1 > ny);
+    if (fullX) Rx = nx;
+    if (fullY) Ry = ny;
+    const int shiftOK = !fullX && !fullY && (Rx + 1) * cex < L[0] - rc && (Ry + 1) * cey < L[1] - rc;
+    for (int k = 0; k < 3; ++k) {
+        D.L[k] = (float)L[k];
+        D.invL[k] = (float)(1.0 / L[k]);
+    }
+    D.ce[0] = (float)cex; D.ce[1] = (float)cey;
+    D.invCe[0] = (float)(nx / L[0]); D.invCe[1] = (float)(ny / L[1]);
+    D.ze = (float)(L[2] / nz); D.invZe = (float)(nz / L[2]);
+    D.rc = (float)rc;
+    D.rcList2 = (float)(rcl * rcl);
+    D.rc2 = rc * rc;
+    D.rc2lo = (float)(D.rc2 * (1.0 - 4e-6));
+    D.rc2hi = (float)(D.rc2 * (1.0 + 4e-6));
+    const double eps = d.rf_dielectric;
+    D.krf = (float)((eps - 1.0) / ((2.0 * eps + 1.0) * rc * rc * rc));
+    D.crf = (float)(3.0 * eps / ((2.0 * eps + 1.0) * rc));
+    D.krf64 = (eps - 1.0) / ((2.0 * eps + 1.0) * rc * rc * rc);
+    D.crf64 = 3.0 * eps / ((2.0 * eps + 1.0) * rc);
+    D.nx = nx; D.ny = ny; D.nz = nz;
+    D.Rx = Rx; D.Ry = Ry; D.fullX = fullX; D.fullY = fullY; D.shiftOK = shiftOK;
+    D.sys = s; D.state = state;
+    D.natoms = d.num_atoms;
+    const int ncols = nx * ny;
+    D.slotCap = d.num_atoms + 7 * ncols;
+    D.clusterCap = D.slotCap / kClusterSize;
+    D.groupCap = ncols + D.clusterCap / kGroupClusters + 1;
+    const long long ncand = (long long)(fullX ? nx : 2 * Rx + 1) * (fullY ? ny : 2 * Ry + 1);
+    const double m = (double)d.num_atoms / (ncols * (double)kClusterSize) + 1.0;
+    long long cap = (long long)(ncand * (1.5 * m + 8.0));
+    cap = std::max<long long>(64, (cap + 31) / 32 * 32);
+    cap = std::min<long long>(cap, (long long)D.clusterCap + 32);
+    D.entryCap = (int)cap;
+    return D;
+}
+
+void layout(Context& C) {
+    C.sets.resize(2 * C.nsys);
+    C.entryCapOverride.resize(2 * C.nsys, 0);
+    long long cellOff = 0, colOff = 0, slotOff = 0, clOff = 0, gOff = 0, eOff = 0, setAtomOff = 0;
+    C.maxClusters = C.maxGroups = C.maxCells = 0;
+    for (int s = 0; s < C.nsys; ++s) {
+        for (int st = 0; st < 2; ++st) {
+            const int c = 2 * s + st;
+            SetDesc D = make_set(C.sys[s], C.tuning, s, st);
+            if (C.entryCapOverride[c] > D.entryCap) D.entryCap = C.entryCapOverride[c];
+            D.atomOff = C.atomOff[s];
+            D.setAtomOff = (int)setAtomOff;
+            D.cellOff = (int)cellOff;
+            D.colOff = (int)colOff;
+            D.slotOff = (int)slotOff;
+            D.clusterOff = (int)clOff;
+            D.groupOff = (int)gOff;
+            D.entryOff = eOff;
+            if (D.clusterCap >= (1 << 20)) fail(ATM_ERR_UNSUPPORTED, "too many clusters in one system");
+            setAtomOff += D.natoms;
+            cellOff += (long long)D.nx * D.ny * D.nz;
+            colOff += (long long)D.nx * D.ny;
+            slotOff += D.slotCap;
+            clOff += D.clusterCap;
+            gOff += D.groupCap;
+            eOff += (long long)D.groupCap * D.entryCap;
+            C.maxClusters = std::max(C.maxClusters, D.clusterCap);
+            C.maxGroups = std::max(C.maxGroups, D.groupCap);
+            C.maxCells = std::max(C.maxCells, D.nx * D.ny * D.nz);
+            C.sets[c] = D;
+        }
+    }
+    if (slotOff >= (1ll << 31) || cellOff >= (1ll << 31)) fail(ATM_ERR_UNSUPPORTED, "batch too large for 32-bit slot indices");
+    C.totalCells = cellOff;
+    C.totalCols = colOff;
+    C.totalSlots = slotOff;
+    C.totalClusters = clOff;
+    C.totalGroups = gOff;
+    C.totalEntries = eOff;
+}
+
+void alloc_layout(Context& C) {
+    C.dropGraphs();
+    const long long S2 = 2ll * C.nsys;
+    const long long setAtoms = 2 * C.totalAtoms;
+    C.dDesc.alloc(sizeof(SetDesc) * S2);
+    C.dRt.alloc(sizeof(SetRuntime) * S2);
+    C.dBpos.alloc(sizeof(float4) * setAtoms);
+    C.dBcell.alloc(sizeof(int) * setAtoms);
+    C.dBrank.alloc(sizeof(int) * setAtoms);
+    C.dSlotOf.alloc(sizeof(int) * setAtoms);
+    C.dCellCount.alloc(sizeof(int) * C.totalCells);
+    C.dCellStart.alloc(sizeof(int) * C.totalCells);
+    C.dColStart.alloc(sizeof(int) * C.totalCols);
+    C.dColCount.alloc(sizeof(int) * C.totalCols);
+    C.dSpos.alloc(sizeof(float4) * C.totalSlots);
+    C.dSprm.alloc(sizeof(float2) * C.totalSlots);
+    C.dSprm64.alloc(sizeof(double4) * C.totalSlots);
+    C.dSidx.alloc(sizeof(int) * C.totalSlots);
+    C.dSfx.alloc(sizeof(long long) * C.totalSlots);
+    C.dSfy.alloc(sizeof(long long) * C.totalSlots);
+    C.dSfz.alloc(sizeof(long long) * C.totalSlots);
+    C.dSdx.alloc(sizeof(double) * C.totalSlots);
+    C.dSdy.alloc(sizeof(double) * C.totalSlots);
+    C.dSdz.alloc(sizeof(double) * C.totalSlots);
+    C.dBbc.alloc(sizeof(float4) * C.totalClusters);
+    C.dBbh.alloc(sizeof(float4) * C.totalClusters);
+    C.dGinfo.alloc(sizeof(int4) * C.totalGroups);
+    C.dGcount.alloc(sizeof(int) * C.totalGroups);
+    C.dGenergy.alloc(sizeof(double) * C.totalGroups);
+    C.dEntries.alloc(sizeof(uint32_t) * C.totalEntries);
+    C.dSetEnergy.alloc(sizeof(double) * S2);
+    C.dGq.release(); C.dGsig.release(); C.dGeps.release();
+    HIPCHK(hipMemcpy(C.dDesc.p, C.sets.data(), sizeof(SetDesc) * S2, hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(C.dRt.p, 0, sizeof(SetRuntime) * S2));
+    HIPCHK(hipMemset(C.dCellCount.p, 0, sizeof(int) * C.totalCells));
+    HIPCHK(hipMemset(C.dGcount.p, 0, sizeof(int) * C.totalGroups));
+
+    DevPtrs& P = C.P;
+    P.desc = C.dDesc.as<SetDesc>();
+    P.rt = C.dRt.as<SetRuntime>();
+    P.posq = C.dPosq.as<float4>();
+    P.displ = C.dDispl.as<float4>();
+    P.ljp = C.dLjp.as<float2>();
+    P.prm64 = C.dPrm64.as<double4>();
+    P.params = C.dParams.as<AtmParamsDev>();
+    P.res = C.dRes.as<SysResult>();
+    P.outF = C.dOutF.as<long long>();
+    P.bpos = C.dBpos.as<float4>();
+    P.bcell = C.dBcell.as<int>();
+    P.brank = C.dBrank.as<int>();
+    P.slotOf = C.dSlotOf.as<int>();
+    P.cellCount = C.dCellCount.as<int>();
+    P.cellStart = C.dCellStart.as<int>();
+    P.colStart = C.dColStart.as<int>();
+    P.colCount = C.dColCount.as<int>();
+    P.spos = C.dSpos.as<float4>();
+    P.sprm = C.dSprm.as<float2>();
+    P.sprm64 = C.dSprm64.as<double4>();
+    P.sidx = C.dSidx.as<int>();
+    P.sfx = C.dSfx.as<long long>();
+    P.sfy = C.dSfy.as<long long>();
+    P.sfz = C.dSfz.as<long long>();
+    P.sdx = C.dSdx.as<double>();
+    P.sdy = C.dSdy.as<double>();
+    P.sdz = C.dSdz.as<double>();
+    P.bbc = C.dBbc.as<float4>();
+    P.bbh = C.dBbh.as<float4>();
+    P.ginfo = C.dGinfo.as<int4>();
+    P.gcount = C.dGcount.as<int>();
+    P.genergy = C.dGenergy.as<double>();
+    P.entries = C.dEntries.as<uint32_t>();
+    P.setEnergy = C.dSetEnergy.as<double>();
+    P.gq = P.gsig = P.geps = nullptr;
+}
+
+void require_ready(const Context& C) {
+    for (int s = 0; s < C.nsys; ++s)
+        if (!C.haveParticles[s])
+            fail(ATM_ERR_STATE, "system " + std::to_string(s) + ": particles were never set (atm_set_particles)");
+}
+
+// Batched evaluation of every set and system, kernel by kernel.  With `ev`
+// (profiling), an event is recorded after each stage: ev[0] before the first
+// kernel, then after build, pair list, pair kernel, energy and blend.
+void enqueue_batched(Context& C, int flags, hipEvent_t* ev = nullptr) {
+    const hipStream_t st = C.stream;
+    const DevPtrs& P = C.P;
+    const int S2 = 2 * C.nsys;
+    if (ev) HIPCHK(hipEventRecord(ev[0], st));
+    if (flags & ATM_EVAL_CLEAR_FORCES) HIPCHK(launch_fill_i64(P.outF, 3 * C.totalAtoms, st));
+    HIPCHK(launch_zero_counts(P, 0, (int)C.totalCells, 0, S2, st));
+    HIPCHK(launch_build(P, 0, S2, C.maxAtoms, C.maxCells, C.maxClusters, st));
+    if (ev) HIPCHK(hipEventRecord(ev[1], st));
+    HIPCHK(launch_pairlist(P, 0, S2, C.maxGroups, 4, st));
+    if (ev) HIPCHK(hipEventRecord(ev[2], st));
+    HIPCHK(launch_pairs(P, 0, S2, C.maxGroups, C.tuning.pairWaves, (flags & ATM_EVAL_FORCES) != 0,
+                        (flags & ATM_EVAL_FP32_PAIRS) == 0, st));
+    if (ev) HIPCHK(hipEventRecord(ev[3], st));
+    HIPCHK(launch_energy(P, 0, C.nsys, st));
+    if (ev) HIPCHK(hipEventRecord(ev[4], st));
+    if (flags & ATM_EVAL_FORCES) HIPCHK(launch_blend(P, 0, C.nsys, C.maxAtoms, 0, 0.0, 0.0, st));
+    if (ev) HIPCHK(hipEventRecord(ev[5], st));
+}
+
+void eval_batched(Context& C, int flags) {
+    const int key = flags & (ATM_EVAL_FORCES | ATM_EVAL_CLEAR_FORCES | ATM_EVAL_FP32_PAIRS);
+    if (flags & ATM_EVAL_NO_GRAPH) {
+        enqueue_batched(C, key);
+        return;
+    }
+    auto it = C.graphs.find(key);
+    if (it == C.graphs.end()) {
+        hipGraph_t g = nullptr;
+        HIPCHK(hipStreamBeginCapture(C.stream, hipStreamCaptureModeThreadLocal));
+        try {
+            enqueue_batched(C, key);
+        } catch (...) {
+            hipGraph_t junk = nullptr;
+            (void)hipStreamEndCapture(C.stream, &junk);
+            if (junk) (void)hipGraphDestroy(junk);
+            throw;
+        }
+        HIPCHK(hipStreamEndCapture(C.stream, &g));
+        hipGraphExec_t ex = nullptr;
+        hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HIPCHK(e);
+        it = C.graphs.emplace(key, ex).first;
+    }
+    HIPCHK(hipGraphLaunch(it->second, C.stream));
+}
+
+// Reference-structured path: per system, state 1 then state 2 evaluated as
+// separate launch sequences with a blocking energy read-back each (the two
+// inner-context evaluations of ATMMetaForceImpl.cpp:113,116), the soft-core
+// and softplus on the host (ReferenceATMMetaForceKernels.cpp:83-98) and a
+// hybrid-force launch with a host scalar (CommonATMMetaForceKernels.cpp:199-201).
+void eval_plain(Context& C, int flags) {
+    const hipStream_t st = C.stream;
+    const DevPtrs& P = C.P;
+    const bool forces = (flags & ATM_EVAL_FORCES) != 0;
+    for (int s = 0; s < C.nsys; ++s) {
+        const SetDesc& D0 = C.sets[2 * s];
+        if (flags & ATM_EVAL_CLEAR_FORCES)
+            HIPCHK(launch_fill_i64(P.outF + 3ll * D0.atomOff, 3ll * D0.natoms, st));
+        double U[2];
+        for (int stt = 0; stt < 2; ++stt) {
+            const int c = 2 * s + stt;
+            const SetDesc& D = C.sets[c];
+            HIPCHK(launch_zero_counts(P, D.cellOff, D.cellOff + D.nx * D.ny * D.nz, c, c + 1, st));
+            HIPCHK(launch_build(P, c, 1, D.natoms, D.nx * D.ny * D.nz, D.clusterCap, st));
+            HIPCHK(launch_pairlist(P, c, 1, D.groupCap, 4, st));
+            HIPCHK(launch_pairs(P, c, 1, D.groupCap, C.tuning.pairWaves, true, (flags & ATM_EVAL_FP32_PAIRS) == 0, st));
+            HIPCHK(launch_set_energy(P, c, st));
+            HIPCHK(hipMemcpyAsync(&U[stt], P.setEnergy + c, sizeof(double), hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+        }
+        AtmParamsDev p;
+        std::memcpy(&p, &C.params[s], sizeof(p));
+        SysResult r;
+        atm_combine_host(p, U[0], U[1], r);
+        atm_result& o = C.results[s];
+        o.energy = r.W; o.perturbation_energy = r.usc; o.state1_energy = r.U1; o.state2_energy = r.U2;
+        o.raw_perturbation = r.u; o.softcore_derivative = r.fp; o.softplus_derivative = r.bfp; o.hybrid_weight = r.sp;
+        if (forces) HIPCHK(launch_blend(P, s, 1, D0.natoms, 1, r.sp, p.direction, st));
+    }
+}
+
+void fetch_results(Context& C) {
+    if (!C.evaluated) fail(ATM_ERR_STATE, "no evaluation has been run");
+    HIPCHK(hipStreamSynchronize(C.stream));
+    std::vector<SetRuntime> rt(2 * C.nsys);
+    HIPCHK(hipMemcpy(rt.data(), C.dRt.p, sizeof(SetRuntime) * rt.size(), hipMemcpyDeviceToHost));
+    bool overflow = false;
+    for (int c = 0; c < 2 * C.nsys; ++c) {
+        if (rt[c].overflow) {
+            overflow = true;
+            C.entryCapOverride[c] = C.sets[c].entryCap * 2;
+        }
+    }
+    if (overflow) {
+        layout(C);
+        alloc_layout(C);
+        C.evaluated = false;
+        C.resultsValid = false;
+        fail(ATM_ERR_CAPACITY, "neighbour list capacity exceeded; capacity doubled, re-run the evaluation");
+    }
+    if (!C.resultsValid) {
+        if (!C.resultsOnHost) {
+            std::vector<SysResult> r(C.nsys);
+            HIPCHK(hipMemcpy(r.data(), C.dRes.p, sizeof(SysResult) * C.nsys, hipMemcpyDeviceToHost));
+            for (int s = 0; s < C.nsys; ++s) {
+                atm_result& o = C.results[s];
+                o.energy = r[s].W; o.perturbation_energy = r[s].usc; o.state1_energy = r[s].U1; o.state2_energy = r[s].U2;
+                o.raw_perturbation = r[s].u; o.softcore_derivative = r[s].fp; o.softplus_derivative = r[s].bfp;
+                o.hybrid_weight = r[s].sp;
+            }
+        }
+        C.resultsValid = true;
+    }
+}
+
+void upload_system(Context& C, int s) {
+    const size_t off = C.atomOff[s], n = C.sys[s].num_atoms;
+    HIPCHK(hipMemcpyAsync(C.dPosq.as<float4>() + off, C.hPosq.data() + off, n * sizeof(float4), hipMemcpyHostToDevice, C.stream));
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* atm_version(void) { return "atm-mi355x 0.1.0 (ATMMetaForce 0.3.1 semantics)"; }
+int32_t atm_abi_version(void) { return ATM_MI355X_ABI_VERSION; }
+const char* atm_last_error(void) { return g_lastError.c_str(); }
+
+int atm_device_count(int32_t* count) {
+    return guarded([&] {
+        if (!count) fail(ATM_ERR_INVALID_ARGUMENT, "null count");
+        int n = 0;
+        hipError_t e = hipGetDeviceCount(&n);
+        *count = e == hipSuccess ? n : 0;
+    });
+}
+
+int atm_create(const atm_system_desc* systems, int32_t num_systems, int32_t device, void** out_context) {
+    return guarded([&] {
+        if (!out_context) fail(ATM_ERR_INVALID_ARGUMENT, "null output pointer");
+        *out_context = nullptr;
+        if (!systems || num_systems <= 0) fail(ATM_ERR_INVALID_ARGUMENT, "need at least one system");
+        for (int s = 0; s < num_systems; ++s) validate_desc(systems[s], s);
+        int ndev = 0;
+        HIPCHK(hipGetDeviceCount(&ndev));
+        if (device < 0 || device >= ndev) fail(ATM_ERR_INVALID_ARGUMENT, "no GPU with index " + std::to_string(device));
+        HIPCHK(hipSetDevice(device));
+        std::unique_ptr<Context> C(new Context());
+        C->device = device;
+        HIPCHK(hipStreamCreateWithFlags(&C->own, hipStreamNonBlocking));
+        C->stream = C->own;
+        C->nsys = num_systems;
+        C->sys.assign(systems, systems + num_systems);
+        C->atomOff.resize(num_systems);
+        long long off = 0;
+        for (int s = 0; s < num_systems; ++s) {
+            C->atomOff[s] = (int)off;
+            off += systems[s].num_atoms;
+            C->maxAtoms = std::max(C->maxAtoms, systems[s].num_atoms);
+        }
+        if (off >= (1ll << 31) / 3) fail(ATM_ERR_UNSUPPORTED, "batch too large");
+        C->totalAtoms = off;
+        C->hPosq.assign(off, make_float4(0.f, 0.f, 0.f, 0.f));
+        C->hDispl.assign(off, make_float4(0.f, 0.f, 0.f, 0.f));
+        C->hLjp.assign(off, make_float2(0.f, 0.f));
+        C->hPrm64.assign(off, make_double4(0.0, 0.0, 0.0, 0.0));
+        C->params.assign(num_systems, atm_params{0.5, 0.5, 0.0, 0.0, 0.0, 200.0, 100.0, 0.0625, 1.0});
+        C->haveParticles.assign(num_systems, 0);
+        C->results.assign(num_systems, atm_result{});
+        C->dPosq.alloc(sizeof(float4) * off);
+        C->dDispl.alloc(sizeof(float4) * off);
+        C->dLjp.alloc(sizeof(float2) * off);
+        C->dPrm64.alloc(sizeof(double4) * off);
+        C->dParams.alloc(sizeof(AtmParamsDev) * num_systems);
+        C->dRes.alloc(sizeof(SysResult) * num_systems);
+        C->dOutF.alloc(sizeof(long long) * 3 * off);
+        HIPCHK(hipMemset(C->dPosq.p, 0, sizeof(float4) * off));
+        HIPCHK(hipMemset(C->dDispl.p, 0, sizeof(float4) * off));
+        HIPCHK(hipMemset(C->dLjp.p, 0, sizeof(float2) * off));
+        HIPCHK(hipMemset(C->dPrm64.p, 0, sizeof(double4) * off));
+        HIPCHK(hipMemset(C->dOutF.p, 0, sizeof(long long) * 3 * off));
+        HIPCHK(hipMemcpy(C->dParams.p, C->params.data(), sizeof(AtmParamsDev) * num_systems, hipMemcpyHostToDevice));
+        layout(*C);
+        alloc_layout(*C);
+        *out_context = C.release();
+    });
+}
+
+int atm_destroy(void* context) {
+    return guarded([&] {
+        if (!context) return;
+        Context* C = static_cast<Context*>(context);
+        (void)hipSetDevice(C->device);
+        if (C->stream) (void)hipStreamSynchronize(C->stream);
+        delete C;
+    });
+}
+
+int atm_set_particles(void* context, int32_t sys, const double* charges, const double* sigmas, const double* epsilons,
+                      const double* displacements) {
+    return guarded([&] {
+        Context& C = ctx_of(context);
+        check_sys(C, sys);
+        if (!charges || !sigmas || !epsilons) fail(ATM_ERR_INVALID_ARGUMENT, "null particle array");
+        HIPCHK(hipSetDevice(C.device));
+        const int n = C.sys[sys].num_atoms, off = C.atomOff[sys];
+        const double sqk = std::sqrt(ATM_ONE_4PI_EPS0);
+        for (int i = 0; i < n; ++i) {
+            if (!(sigmas[i] >= 0.0) || !(epsilons[i] >= 0.0))
+                fail(ATM_ERR_INVALID_ARGUMENT, "sigma and epsilon must be non-negative (atom " + std::to_string(i) + ")");
+            C.hPosq[off + i].w = (float)(charges[i] * sqk);
+            C.hLjp[off + i] = make_float2((float)(0.5 * sigmas[i]), (float)(2.0 * std::sqrt(epsilons[i])));
+            C.hPrm64[off + i] = make_double4(0.5 * sigmas[i], 2.0 * std::sqrt(epsilons[i]), charges[i] * sqk, 0.0);
+            if (displacements)
+                C.hDispl[off + i] = make_float4((float)displacements[3 * i], (float)displacements[3 * i + 1],
+                                                (float)displacements[3 * i + 2], 0.f);
+        }
+        upload_system(C, sys);
+        HIPCHK(hipMemcpyAsync(C.dLjp.as<float2>() + off, C.hLjp.data() + off, n * sizeof(float2), hipMemcpyHostToDevice, C.stream));
+        HIPCHK(hipMemcpyAsync(C.dPrm64.as<double4>() + off, C.hPrm64.data() + off, n * sizeof(double4), hipMemcpyHostToDevice,
+                              C.stream));
+        HIPCHK(hipMemcpyAsync(C.dDispl.as<float4>() + off, C.hDispl.data() + off, n * sizeof(float4), hipMemcpyHostToDevice, C.stream));
+        HIPCHK(hipStreamSynchronize(C.stream));
+        C.haveParticles[sys] = 1;
+    });
+}
+
+int atm_set_displacements(void* context, int32_t sys, const double* d) {
+    return guarded([&] {
+        Context& C = ctx_of(context);
+        check_sys(C, sys);
+        if (!d) fail(ATM_ERR_INVALID_ARGUMENT, "null displacement array");
+        HIPCHK(hipSetDevice(C.device));
+        const int n = C.sys[sys].num_atoms, off = C.atomOff[sys];
+        for (int i = 0; i < n; ++i)
+            C.hDispl[off + i] = make_float4((float)d[3 * i], (float)d[3 * i + 1], (float)d[3 * i + 2], 0.f);
+        HIPCHK(hipMemcpyAsync(C.dDispl.as<float4>() + off, C.hDispl.data() + off, n * sizeof(float4), hipMemcpyHostToDevice, C.stream));
+        HIPCHK(hipStreamSynchronize(C.stream));
+    });
+}
+
+int atm_set_positions(void* context, int32_t sys, const double* x) {
+    return guarded([&] {
+        Context& C = ctx_of(context);
+        check_sys(C, sys);
+        if (!x) fail(ATM_ERR_INVALID_ARGUMENT, "null position array");
+        HIPCHK(hipSetDevice(C.device));
+        const int n = C.sys[sys].num_atoms, off = C.atomOff[sys];
+        for (int i = 0; i < n; ++i) {
+            if (!std::isfinite(x[3 * i]) || !std::isfinite(x[3 * i + 1]) || !std::isfinite(x[3 * i + 2]))
+                fail(ATM_ERR_INVALID_ARGUMENT, "non-finite position (atom " + std::to_string(i) + ")");
+            float4& p = C.hPosq[off + i];
+            p.x = (float)x[3 * i]; p.y = (float)x[3 * i + 1]; p.z = (float)x[3 * i + 2];
+        }
+        upload_system(C, sys);
+        HIPCHK(hipStreamSynchronize(C.stream));
+    });
+}
+
+int atm_set_positions_device(void* context, const float* positions_dev) {
+    return guarded([&] {
+        Context& C = ctx_of(context);
+        if (!positions_dev) fail(ATM_ERR_INVALID_ARGUMENT, "null device pointer");
+        HIPCHK(hipSetDevice(C.device));
+        HIPCHK(launch_set_positions(C.P, C.dPosq.as<float4>(), positions_dev, C.totalAtoms, C.stream));
+    });
+}
+
+int atm_set_box(void* context, int32_t sys, const double* box3) {
+    return guarded([&] {
+        Context& C = ctx_of(context);
+        check_sys(C, sys);
+        if (!box3) fail(ATM_ERR_INVALID_ARGUMENT, "null box");
+        atm_system_desc d = C.sys[sys];
+        for (int k = 0; k < 3; ++k) d.box[k] = box3[k];
+        validate_desc(d, sys);
+        HIPCHK(hipSetDevice(C.device));
+        HIPCHK(hipStreamSynchronize(C.stream));
+        C.sys[sys] = d;
+        layout(C);
+        alloc_layout(C);
+        C.evaluated = false;
+        C.resultsValid = false;
+    });
+}
+
+int atm_set_params(void* context, int32_t sys, const atm_params* p) {
+    return guarded([&] {
+        Context& C = ctx_of(context);
+        check_sys(C, sys);
+        if (!p) fail(ATM_ERR_INVALID_ARGUMENT, "null parameters");
+        HIPCHK(hipSetDevice(C.device));
+        C.params[sys] = *p;
+        HIPCHK(hipMemcpyAsync(C.dParams.as<AtmParamsDev>() + sys, &C.params[sys], sizeof(AtmParamsDev), hipMemcpyHostToDevice,
+                              C.stream));
+        HIPCHK(hipStreamSynchronize(C.stream));
+    });
+}
+
+int atm_eval(void* context, int32_t flags) {
+    return guarded([&] {
+        Context& C = ctx_of(context);
+        require_ready(C);
+        HIPCHK(hipSetDevice(C.device));
+        C.resultsValid = false;
+        if (flags & ATM_EVAL_PLAIN) {
+            eval_plain(C, flags);
+            C.resultsOnHost = true;
+        } else {
+            eval_batched(C, flags);
+            C.resultsOnHost = false;
+        }
+        C.evaluated = true;
+    });
+}
+
+int atm_synchronize(void* context) {
+    return guarded([&] {
+        Context& C = ctx_of(context);
+        HIPCHK(hipSetDevice(C.device));
+        HIPCHK(hipStreamSynchronize(C.stream));
+    });
+}
+
+int atm_get_results(void* context, int32_t sys, atm_result* out) {
+    return guarded([&] {
+        Context& C = ctx_of(context);
+        check_sys(C, sys);
+        if (!out) fail(ATM_ERR_INVALID_ARGUMENT, "null output");
+        HIPCHK(hipSetDevice(C.device));
+        fetch_results(C);
+        *out = C.results[sys];
+    });
+}
+
+int atm_get_forces_fixed(void* context, int32_t sys, int64_t* f) {
+    return guarded([&] {
+        Context& C = ctx_of(context);
+        check_sys(C, sys);
+        if (!f) fail(ATM_ERR_INVALID_ARGUMENT, "null output");
+        HIPCHK(hipSetDevice(C.device));
+        fetch_results(C);
+        const int n = C.sys[sys].num_atoms;
+        std::vector<long long> soa(3ll * n);
+        HIPCHK(hipMemcpy(soa.data(), C.dOutF.as<long long>() + 3ll * C.atomOff[sys], sizeof(long long) * 3 * n,
+                         hipMemcpyDeviceToHost));
+        for (int i = 0; i < n; ++i)
+            for (int k = 0; k < 3; ++k) f[3 * i + k] = soa[(size_t)k * n + i];
+    });
+}
+
+int atm_get_forces(void* context, int32_t sys, double* f) {
+    return guarded([&] {
+        Context& C = ctx_of(context);
+        check_sys(C, sys);
+        if (!f) fail(ATM_ERR_INVALID_ARGUMENT, "null output");
+        HIPCHK(hipSetDevice(C.device));
+        fetch_results(C);
+        const int n = C.sys[sys].num_atoms;
+        std::vector<long long> soa(3ll * n);
+        HIPCHK(hipMemcpy(soa.data(), C.dOutF.as<long long>() + 3ll * C.atomOff[sys], sizeof(long long) * 3 * n,
+                         hipMemcpyDeviceToHost));
+        const double inv = 1.0 / ATM_FORCE_SCALE;
+        for (int i = 0; i < n; ++i)
+            for (int k = 0; k < 3; ++k) f[3 * i + k] = (double)soa[(size_t)k * n + i] * inv;
+    });
+}
+
+int atm_clear_forces(void* context) {
+    return guarded([&] {
+        Context& C = ctx_of(context);
+        HIPCHK(hipSetDevice(C.device));
+        HIPCHK(launch_fill_i64(C.P.outF, 3 * C.totalAtoms, C.stream));
+        HIPCHK(hipStreamSynchronize(C.stream));
+    });
+}
+
+int atm_get_pairs(void* context, int32_t sys, int32_t state, int32_t* pairs, int64_t capacity, int64_t* count) {
+    return guarded([&] {
+        Context& C = ctx_of(context);
+        check_sys(C, sys);
+        if (state != 0 && state != 1) fail(ATM_ERR_INVALID_ARGUMENT, "state must be 0 or 1");
+        if (!count) fail(ATM_ERR_INVALID_ARGUMENT, "null count");
+        if (capacity > 0 && !pairs) fail(ATM_ERR_INVALID_ARGUMENT, "null pair buffer");
+        HIPCHK(hipSetDevice(C.device));
+        fetch_results(C);
+        const int c = 2 * sys + state;
+        const long long cap = std::max<long long>(capacity, 0);
+        DevBuf out, ctr;
+        out.alloc(sizeof(int2) * std::max<long long>(cap, 1));
+        ctr.alloc(sizeof(unsigned long long) * 4);
+        HIPCHK(hipMemset(ctr.p, 0, sizeof(unsigned long long) * 4));
+        HIPCHK(launch_pairs_dump(C.P, c, C.sets[c].groupCap, out.as<int2>(), cap, ctr.as<unsigned long long>(), C.stream));
+        HIPCHK(hipStreamSynchronize(C.stream));
+        unsigned long long h[4];
+        HIPCHK(hipMemcpy(h, ctr.p, sizeof(h), hipMemcpyDeviceToHost));
+        *count = (int64_t)h[0];
+        const long long nout = std::min<long long>((long long)h[0], cap);
+        if (nout > 0) {
+            std::vector<int2> v(nout);
+            HIPCHK(hipMemcpy(v.data(), out.p, sizeof(int2) * nout, hipMemcpyDeviceToHost));
+            std::sort(v.begin(), v.end(), [](const int2& a, const int2& b) { return a.x != b.x ? a.x < b.x : a.y < b.y; });
+            for (long long k = 0; k < nout; ++k) { pairs[2 * k] = v[k].x; pairs[2 * k + 1] = v[k].y; }
+        }
+    });
+}
+
+int atm_get_stats(void* context, int64_t* pairs_in_cutoff, int64_t* pairs_evaluated, int64_t* list_entries) {
+    return guarded([&] {
+        Context& C = ctx_of(context);
+        HIPCHK(hipSetDevice(C.device));
+        fetch_results(C);
+        DevBuf ctr;
+        ctr.alloc(sizeof(unsigned long long) * 4);
+        long long nin = 0, nev = 0, nent = 0;
+        for (int c = 0; c < 2 * C.nsys; ++c) {
+            HIPCHK(hipMemset(ctr.p, 0, sizeof(unsigned long long) * 4));
+            HIPCHK(launch_pairs_dump(C.P, c, C.sets[c].groupCap, nullptr, 0, ctr.as<unsigned long long>(), C.stream));
+            HIPCHK(hipStreamSynchronize(C.stream));
+            unsigned long long h[4];
+            HIPCHK(hipMemcpy(h, ctr.p, sizeof(h), hipMemcpyDeviceToHost));
+            nin += (long long)h[0];
+            nev += (long long)h[1];
+            std::vector<int> gc(C.sets[c].groupCap);
+            HIPCHK(hipMemcpy(gc.data(), C.dGcount.as<int>() + C.sets[c].groupOff, sizeof(int) * gc.size(), hipMemcpyDeviceToHost));
+            SetRuntime rt;
+            HIPCHK(hipMemcpy(&rt, C.dRt.as<SetRuntime>() + c, sizeof(rt), hipMemcpyDeviceToHost));
+            for (int g = 0; g < rt.ngroups; ++g) nent += gc[g];
+        }
+        if (pairs_in_cutoff) *pairs_in_cutoff = nin;
+        if (pairs_evaluated) *pairs_evaluated = nev;
+        if (list_entries) *list_entries = nent;
+    });
+}
+
+int atm_set_stream(void* context, void* hip_stream) {
+    return guarded([&] {
+        Context& C = ctx_of(context);
+        HIPCHK(hipSetDevice(C.device));
+        HIPCHK(hipStreamSynchronize(C.stream));
+        C.stream = hip_stream ? static_cast<hipStream_t>(hip_stream) : C.own;
+        C.dropGraphs();
+    });
+}
+
+int atm_get_tuning(void* context, atm_tuning* out) {
+    return guarded([&] {
+        Context& C = ctx_of(context);
+        if (!out) fail(ATM_ERR_INVALID_ARGUMENT, "null output");
+        out->column_scale = C.tuning.colScale;
+        out->pair_waves_per_block = C.tuning.pairWaves;
+        out->zbins_per_column = C.tuning.zbinsPerColumn;
+        out->best_ms = C.tunedMs;
+    });
+}
+
+int atm_set_tuning(void* context, const atm_tuning* t) {
+    return guarded([&] {
+        Context& C = ctx_of(context);
+        if (!t) fail(ATM_ERR_INVALID_ARGUMENT, "null tuning");
+        if (!(t->column_scale >= 0.5 && t->column_scale <= 2.0)) fail(ATM_ERR_INVALID_ARGUMENT, "column_scale must be in [0.5, 2]");
+        if (t->pair_waves_per_block != 1 && t->pair_waves_per_block != 2 && t->pair_waves_per_block != 4 &&
+            t->pair_waves_per_block != 8)
+            fail(ATM_ERR_INVALID_ARGUMENT, "pair_waves_per_block must be 1, 2, 4 or 8");
+        if (t->zbins_per_column < 1 || t->zbins_per_column > 16) fail(ATM_ERR_INVALID_ARGUMENT, "zbins_per_column must be in [1, 16]");
+        HIPCHK(hipSetDevice(C.device));
+        HIPCHK(hipStreamSynchronize(C.stream));
+        const bool relayout = t->column_scale != C.tuning.colScale || t->zbins_per_column != C.tuning.zbinsPerColumn;
+        C.tuning.colScale = t->column_scale;
+        C.tuning.zbinsPerColumn = t->zbins_per_column;
+        C.tuning.pairWaves = t->pair_waves_per_block;
+        if (relayout) {
+            layout(C);
+            alloc_layout(C);
+        } else {
+            C.dropGraphs();
+        }
+        C.evaluated = false;
+        C.resultsValid = false;
+    });
+}
+
+int atm_autotune(void* context, int32_t repeats, atm_tuning* out) {
+    return guarded([&] {
+        Context& C = ctx_of(context);
+        require_ready(C);
+        HIPCHK(hipSetDevice(C.device));
+        const int reps = std::max(1, (int)repeats);
+        const double scales[] = {0.8, 0.9, 1.0, 1.15, 1.3};
+        const int waves[] = {1, 2, 4};
+        const int zpcs[] = {2, 4};
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        Tuning best = C.tuning;
+        double bestMs = 1e300;
+        const int flags = ATM_EVAL_FORCES | ATM_EVAL_CLEAR_FORCES;
+        for (double sc : scales) {
+            for (int zp : zpcs) {
+                C.tuning.colScale = sc;
+                C.tuning.zbinsPerColumn = zp;
+                C.entryCapOverride.assign(2 * C.nsys, 0);
+                layout(C);
+                alloc_layout(C);
+                for (int w : waves) {
+                    C.tuning.pairWaves = w;
+                    C.dropGraphs();
+                    bool ok = true;
+                    for (int attempt = 0; attempt < 3; ++attempt) {  // grow list capacity if needed
+                        eval_batched(C, flags);
+                        C.evaluated = true;
+                        C.resultsOnHost = false;
+                        C.resultsValid = false;
+                        try {
+                            fetch_results(C);
+                            ok = true;
+                            break;
+                        } catch (const AtmError& e) {
+                            if (e.code != ATM_ERR_CAPACITY) throw;
+                            ok = false;
+                        }
+                    }
+                    if (!ok) continue;
+                    HIPCHK(hipEventRecord(e0, C.stream));
+                    for (int r = 0; r < reps; ++r) eval_batched(C, flags);
+                    HIPCHK(hipEventRecord(e1, C.stream));
+                    HIPCHK(hipEventSynchronize(e1));
+                    float ms = 0.f;
+                    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+                    const double per = ms / reps;
+                    if (per < bestMs) {
+                        bestMs = per;
+                        best = C.tuning;
+                    }
+                }
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        C.tuning = best;
+        C.tunedMs = bestMs;
+        C.entryCapOverride.assign(2 * C.nsys, 0);
+        layout(C);
+        alloc_layout(C);
+        C.evaluated = false;
+        C.resultsValid = false;
+        if (out) {
+            out->column_scale = best.colScale;
+            out->pair_waves_per_block = best.pairWaves;
+            out->zbins_per_column = best.zbinsPerColumn;
+            out->best_ms = bestMs;
+        }
+    });
+}
+
+int atm_profile(void* context, int32_t flags, int32_t reps, atm_stage_times* out) {
+    return guarded([&] {
+        Context& C = ctx_of(context);
+        if (!out) fail(ATM_ERR_INVALID_ARGUMENT, "null output");
+        if (reps < 1) fail(ATM_ERR_INVALID_ARGUMENT, "reps must be positive");
+        require_ready(C);
+        HIPCHK(hipSetDevice(C.device));
+        hipEvent_t ev[6];
+        for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+        double acc[5] = {0, 0, 0, 0, 0};
+        const int key = flags & (ATM_EVAL_FORCES | ATM_EVAL_CLEAR_FORCES | ATM_EVAL_FP32_PAIRS);
+        for (int r = 0; r < reps; ++r) {
+            enqueue_batched(C, key, ev);
+            HIPCHK(hipEventSynchronize(ev[5]));
+            for (int k = 0; k < 5; ++k) {
+                float ms = 0.f;
+                HIPCHK(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
+                acc[k] += ms;
+            }
+        }
+        for (auto& e : ev) (void)hipEventDestroy(e);
+        C.evaluated = true;
+        C.resultsOnHost = false;
+        C.resultsValid = false;
+        out->build_ms = acc[0] / reps;
+        out->pairlist_ms = acc[1] / reps;
+        out->pairs_ms = acc[2] / reps;
+        out->energy_ms = acc[3] / reps;
+        out->blend_ms = acc[4] / reps;
+        out->total_ms = (acc[0] + acc[1] + acc[2] + acc[3] + acc[4]) / reps;
+        out->reps = reps;
+    });
+}
+
+int atm_backward(void* context, const double* grad_energy, double* d_charges, double* d_sigmas, double* d_epsilons,
+                 double* d_params) {
+    return guarded([&] {
+        Context& C = ctx_of(context);
+        if (!grad_energy) fail(ATM_ERR_INVALID_ARGUMENT, "null grad_energy");
+        HIPCHK(hipSetDevice(C.device));
+        fetch_results(C);
+        const int S = C.nsys;
+        // per-set weights: dW/dU_state = gW * ((1-sp) for e0's state, sp for the other)
+        std::vector<double> wset(2 * S);
+        for (int s = 0; s < S; ++s) {
+            const atm_result& r = C.results[s];
+            const bool fwd = C.params[s].direction > 0.0;
+            const double w_e0 = 1.0 - r.hybrid_weight, w_o = r.hybrid_weight;
+            wset[2 * s] = grad_energy[s] * (fwd ? w_e0 : w_o);
+            wset[2 * s + 1] = grad_energy[s] * (fwd ? w_o : w_e0);
+        }
+        if (d_charges || d_sigmas || d_epsilons) {
+            if (!C.dGq.p) {
+                C.dGq.alloc(sizeof(double) * C.totalSlots);
+                C.dGsig.alloc(sizeof(double) * C.totalSlots);
+                C.dGeps.alloc(sizeof(double) * C.totalSlots);
+                C.P.gq = C.dGq.as<double>();
+                C.P.gsig = C.dGsig.as<double>();
+                C.P.geps = C.dGeps.as<double>();
+                C.dropGraphs();
+            }
+            C.dWset.alloc(sizeof(double) * 2 * S);
+            C.dDq.alloc(sizeof(double) * C.totalAtoms);
+            C.dDsig.alloc(sizeof(double) * C.totalAtoms);
+            C.dDeps.alloc(sizeof(double) * C.totalAtoms);
+            HIPCHK(hipMemcpyAsync(C.dWset.p, wset.data(), sizeof(double) * 2 * S, hipMemcpyHostToDevice, C.stream));
+            HIPCHK(launch_backward(C.P, C.totalSlots, 0, 2 * S, C.maxGroups, C.dWset.as<double>(), C.stream));
+            HIPCHK(launch_backward_gather(C.P, 0, S, C.maxAtoms, C.dDq.as<double>(), C.dDsig.as<double>(),
+                                          C.dDeps.as<double>(), C.stream));
+            HIPCHK(hipStreamSynchronize(C.stream));
+            const size_t nb = sizeof(double) * C.totalAtoms;
+            if (d_charges) HIPCHK(hipMemcpy(d_charges, C.dDq.p, nb, hipMemcpyDeviceToHost));
+            if (d_sigmas) HIPCHK(hipMemcpy(d_sigmas, C.dDsig.p, nb, hipMemcpyDeviceToHost));
+            if (d_epsilons) HIPCHK(hipMemcpy(d_epsilons, C.dDeps.p, nb, hipMemcpyDeviceToHost));
+        }
+        if (d_params) {
+            for (int s = 0; s < S; ++s) {
+                const atm_params& p = C.params[s];
+                const atm_result& r = C.results[s];
+                const double g = grad_energy[s];
+                const double usc = r.perturbation_energy;
+                const double x = -p.alpha * (usc - p.u0);
+                const double lnee = x > 0.0 ? x + std::log1p(std::exp(-x)) : std::log1p(std::exp(x));
+                const double inv_ee = x > 0.0 ? std::exp(-x) / (1.0 + std::exp(-x)) : 1.0 / (1.0 + std::exp(x));
+                double dl1 = 0.0, dl2 = usc, da = 0.0, du0 = 0.0;
+                if (p.alpha > 0.0) {
+                    dl1 = -lnee / p.alpha;
+                    dl2 += lnee / p.alpha;
+                    du0 = (p.lambda2 - p.lambda1) * (1.0 - inv_ee);
+                    da = (p.lambda2 - p.lambda1) * (-lnee / (p.alpha * p.alpha) - (usc - p.u0) * (1.0 - inv_ee) / p.alpha);
+                }
+                d_params[5 * s + 0] = g * dl1;
+                d_params[5 * s + 1] = g * dl2;
+                d_params[5 * s + 2] = g * da;
+                d_params[5 * s + 3] = g * du0;
+                d_params[5 * s + 4] = g * 1.0;
+            }
+        }
+    });
+}
+
+}  // extern "C"

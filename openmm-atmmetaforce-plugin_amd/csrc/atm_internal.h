@@ -1,0 +1,67 @@
+// Internal layout shared by the gfx950 kernels and the host side of the C ABI.
+//
+// Batch layout in HBM
+//   * A context holds S systems.  System s owns atoms [atomOff_s, atomOff_s+N_s)
+//     of the unsorted per-atom arrays (positions+charge, LJ parameters,
+//     displacements) and of the accumulated fixed-point ATM force (SoA x|y|z).
+//   * Every system has two coordinate sets ("states"): state 0 is x, state 1 is
+//     x + d.  Set c = 2 s + state is binned, sorted and paired independently;
+//     each set owns a slot range of the sorted arrays, a cell range, a column
+//     range, a cluster range, a group range and an entry range.
+//   * Sorting: atoms are binned into (column, z-bin) fine cells.  Columns tile
+//     the x-y plane of the box; each column's atom count is padded to a multiple
+//     of 8 with dummy slots (sidx = -1), so 8-slot clusters never straddle
+//     columns.  Up to 8 consecutive clusters of one column form a group; a group
+//     is the unit of work of the pair kernel (one wave64).
+#pragma once
+#include <stdint.h>
+
+namespace atm {
+
+constexpr int kClusterSize = 8;      // atoms per cluster
+constexpr int kGroupClusters = 8;    // clusters per group (one wave64 = 8 j x 8 i lanes)
+constexpr double kForceScale = 16777216.0;  // 2^24 fixed-point units per kJ/mol/nm
+// Partial forces at or above this magnitude (kJ/mol/nm) go to the FP64 side
+// accumulator instead of the fixed-point one, so the 64-bit sums cannot wrap.
+constexpr float kFixedMax = 2147483648.0f;  // 2^31
+
+struct SetDesc {
+    // geometry (FP32 copies of the FP64 host values)
+    float L[3], invL[3];
+    float ce[2], invCe[2];   // column edges in x, y
+    float ze, invZe;         // z-bin edge
+    float rc, rcList2;       // cutoff and the (slightly enlarged) list-build cutoff^2
+    float rc2lo, rc2hi;      // FP32 window around rc^2 where the FP64 test decides
+    float krf, crf;          // reaction field constants (k_e folded into the charges)
+    double rc2;              // exact FP64 cutoff^2
+    double krf64, crf64;     // reaction field constants in FP64 (applied to the energy sums)
+    int nx, ny, nz;          // columns in x, y; z-bins
+    int Rx, Ry;              // column search radius (columns) in x, y
+    int fullX, fullY;        // iterate every column once in that dimension
+    int shiftOK;             // x/y image shift known from the column offset
+    int sys, state;
+    int natoms, atomOff;     // system atoms in the unsorted arrays
+    int setAtomOff;          // this set's atoms in the per-set binned arrays
+    int cellOff, colOff;
+    int slotOff, slotCap;
+    int clusterOff, clusterCap;
+    int groupOff, groupCap;
+    long long entryOff;      // first entry of group 0 of this set
+    int entryCap;            // entries per group
+    int pad_;
+};
+
+struct SetRuntime {
+    int nslots, nclusters, ngroups, overflow;
+};
+
+// Per-system state of the last evaluation on the device (mirrors atm_result).
+struct SysResult {
+    double W, usc, U1, U2, u, fp, bfp, sp;
+};
+
+struct AtmParamsDev {
+    double lambda1, lambda2, alpha, u0, w0, umax, ubcore, acore, direction;
+};
+
+}  // namespace atm

@@ -1,0 +1,71 @@
+// Device buffer table and kernel launchers of the ATM evaluation.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include "atm_internal.h"
+
+namespace atm {
+
+// Every device array of a context, passed by value to the kernels.
+struct DevPtrs {
+    const SetDesc* desc;   // [2 S]
+    SetRuntime* rt;        // [2 S]
+    // per system, unsorted atom order
+    const float4* posq;    // (x, y, z, q sqrt(k_e))
+    const float4* displ;   // (dx, dy, dz, 0)
+    const float2* ljp;     // (sigma/2, 2 sqrt(eps))
+    const double4* prm64;  // (sigma/2, 2 sqrt(eps), q sqrt(k_e), 0) in FP64
+    const AtmParamsDev* params;  // [S]
+    SysResult* res;        // [S]
+    long long* outF;       // accumulated ATM force, SoA per system at 3*atomOff
+    // per set, unsorted order (setAtomOff)
+    float4* bpos;
+    int* bcell;
+    int* brank;
+    int* slotOf;
+    // cells and columns
+    int* cellCount;
+    int* cellStart;
+    int* colStart;
+    int* colCount;
+    // sorted slots
+    float4* spos;
+    float2* sprm;
+    double4* sprm64;
+    int* sidx;
+    long long* sfx;
+    long long* sfy;
+    long long* sfz;
+    double* sdx;           // FP64 side accumulators for partial forces >= kFixedMax
+    double* sdy;
+    double* sdz;
+    // clusters
+    float4* bbc;
+    float4* bbh;
+    // groups
+    int4* ginfo;
+    int* gcount;
+    double* genergy;
+    uint32_t* entries;
+    double* setEnergy;     // [2 S] (reference-structured path)
+    // backward
+    double* gq;            // [2 S sets x slots] gradient accumulators in slot order
+    double* gsig;
+    double* geps;
+};
+
+hipError_t launch_zero_counts(const DevPtrs& P, int cellBegin, int cellEnd, int setBegin, int setEnd, hipStream_t st);
+hipError_t launch_build(const DevPtrs& P, int setBegin, int nsets, int maxAtoms, int maxCells, int maxClusters,
+                        hipStream_t st);
+hipError_t launch_pairlist(const DevPtrs& P, int setBegin, int nsets, int maxGroups, int listWaves, hipStream_t st);
+hipError_t launch_pairs(const DevPtrs& P, int setBegin, int nsets, int maxGroups, int waves, bool forces, bool fp64,
+                        hipStream_t st);
+hipError_t launch_energy(const DevPtrs& P, int sysBegin, int nsys, hipStream_t st);
+hipError_t launch_set_energy(const DevPtrs& P, int c, hipStream_t st);
+hipError_t launch_blend(const DevPtrs& P, int sysBegin, int nsys, int maxAtoms, int hostMode, double sp, double dir,
+                        hipStream_t st);
+hipError_t launch_fill_i64(long long* p, long long n, hipStream_t st);
+hipError_t launch_pairs_dump(const DevPtrs& P, int c, int maxGroups, int2* out, long long cap,
+                             unsigned long long* counters, hipStream_t st);
+
+}  // namespace atm

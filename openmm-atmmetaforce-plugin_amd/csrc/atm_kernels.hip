@@ -1,0 +1,764 @@
+// gfx950 kernels of the ATM evaluation.
+//
+// Per evaluation, for every coordinate set c = 2*system + state of the batch:
+//   k_zero_counts  clear fine-cell counters and overflow flags
+//   k_bin          x (+ d for state 1), wrap into the box, fine-cell id and rank
+//   k_scan         per-set column/cell scans, dummy padding, group table
+//   k_scatter      counting-sort scatter into cluster order, clear set forces
+//   k_cellsort     order each fine cell by atom index (deterministic results)
+//   k_bbox         per-cluster bounding boxes
+//   k_pairlist     per group of 8 clusters: j-clusters whose boxes are within
+//                  the cutoff of at least one i-cluster (half shell, i <= j)
+//   k_pairs        LJ + reaction-field pair kernel, one wave64 per group,
+//                  lanes = 8 j-atoms x 8 i-atoms, FP32 pair math, FP64 energy,
+//                  64-bit fixed-point forces
+// and per system:
+//   k_energy       U1, U2 (fixed-order FP64 sums), soft-core, softplus -> W, sp
+//   k_blend        F += (1-sp) F1 + sp F2 (direction-aware), gathered back to
+//                  the system's atom order
+//
+// The ATM math follows platforms/reference/src/ReferenceATMMetaForceKernels.cpp
+// (SoftCoreF :26-37, execute :56-113); the pair term is OpenMM's NonbondedForce
+// with CutoffPeriodic + reaction field (U = 4 eps [(s/r)^12 - (s/r)^6]
+// + k_e q_i q_j (1/r + k_rf r^2 - c_rf), pairs with r < rc under minimum image).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "atm_internal.h"
+#include "atm_kernels.h"
+#include "atm_math.h"
+
+namespace atm {
+
+// ---------------------------------------------------------------- helpers
+
+__device__ __forceinline__ float wrap_coord(float x, float L, float invL) {
+    // x - L floor(x/L) is exact on the 2^-20 grid (single rounding of an
+    // exactly representable result); one correction step fixes floor() misses.
+    float w = fmaf(-L, floorf(x * invL), x);
+    if (w < 0.0f) w += L;
+    if (w >= L) w -= L;
+    return w;
+}
+
+__device__ __forceinline__ float min_image(float d, float L, float invL) {
+    return fmaf(-L, rintf(d * invL), d);
+}
+
+// float -> 64-bit fixed point (round to nearest).  |f * scale| < 2^51 takes the
+// magic-number path (one FP64 FMA + a 64-bit subtract); larger values use the
+// general conversion.
+__device__ __forceinline__ long long to_fixed(float f) {
+    double d = (double)f * kForceScale;
+    if (fabs(d) < 2251799813685248.0) {
+        double t = d + 6755399441055744.0;  // 1.5 * 2^52
+        return __double_as_longlong(t) - 0x4338000000000000LL;
+    }
+    return __double2ll_rn(d);
+}
+
+// kJ/mol/nm -> fixed point, saturating at the int64 range (blended output).
+__device__ __forceinline__ long long to_fixed_sat(double f) {
+    const double d = fmin(fmax(f * kForceScale, -9.2e18), 9.2e18);
+    return __double2ll_rn(d);
+}
+
+__device__ __forceinline__ void atomic_add_fixed(long long* p, long long v) {
+    atomicAdd(reinterpret_cast<unsigned long long*>(p), static_cast<unsigned long long>(v));
+}
+
+// Add a partial force to a slot: 64-bit fixed point below kFixedMax, FP64 side
+// accumulator above (clashing pairs of the displaced state).
+__device__ __forceinline__ void add_force(const DevPtrs& P, int slot, float fx, float fy, float fz) {
+    if (fabsf(fx) < kFixedMax) atomic_add_fixed(P.sfx + slot, to_fixed(fx)); else atomicAdd(P.sdx + slot, (double)fx);
+    if (fabsf(fy) < kFixedMax) atomic_add_fixed(P.sfy + slot, to_fixed(fy)); else atomicAdd(P.sdy + slot, (double)fy);
+    if (fabsf(fz) < kFixedMax) atomic_add_fixed(P.sfz + slot, to_fixed(fz)); else atomicAdd(P.sdz + slot, (double)fz);
+}
+
+// DPP-fused sum over each aligned group of 8 lanes (xor 1, xor 2, half-row mirror).
+__device__ __forceinline__ float oct_sum(float v) {
+    v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0xB1, 0xF, 0xF, true));
+    v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x4E, 0xF, 0xF, true));
+    v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x141, 0xF, 0xF, true));
+    return v;
+}
+
+// Sum over the 8 lanes {l & 7 + 8 k}: the same i-atom across the 8 j-columns.
+__device__ __forceinline__ float stride8_sum(float v) {
+    v += __shfl_xor(v, 8, 64);
+    v += __shfl_xor(v, 16, 64);
+    v += __shfl_xor(v, 32, 64);
+    return v;
+}
+
+__device__ __forceinline__ double wave_sum_d(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// In-cutoff decision: FP32 r^2 outside a narrow window around rc^2 decides;
+// inside it the exact FP64 r^2 (the squares of 2^-20-grid differences are
+// exact) decides, so the pair set equals the FP64 one bit for bit.
+__device__ __forceinline__ bool in_cutoff(const SetDesc& D, float dx, float dy, float dz, float r2) {
+    bool in = r2 < D.rc2lo;
+    if (r2 >= D.rc2lo && r2 < D.rc2hi) {
+        double r2d = (double)dx * dx + (double)dy * dy + (double)dz * dz;
+        in = r2d < D.rc2;
+    }
+    return in;
+}
+
+// 1/sqrt(x): hardware estimate plus one Newton step written on the residual
+// e = 1 - x y^2 (an FMA, exact before its rounding), which keeps the result
+// unbiased; the form y (1.5 - 0.5 x y^2) rounds near 1.0 and drops the
+// correction of low estimates.
+__device__ __forceinline__ float rsqrt_nr(float x) {
+    float y = __builtin_amdgcn_rsqf(x);
+    const float t = x * y;
+    const float e = fmaf(-t, y, 1.0f);
+    return fmaf(0.5f * y, e, y);
+}
+
+// Masked LJ + reaction-field pair term.  Particle data: pos.w = q sqrt(k_e);
+// prm = (sigma/2, 2 sqrt(eps)).  fs = F_i / (x_i - x_j); the energy is returned
+// as elj (LJ) and the Coulomb pieces qq/r, qq and qq r^2, so that
+// E_C = qq/r + k_rf qq r^2 - c_rf qq is formed in FP64 from the sums.  Lanes
+// with in == false return zeros (their r^2 may be 0: self pairs, dummies).
+struct PairE {
+    float fs, elj, qr, qq, qr2;
+};
+
+__device__ __forceinline__ PairE pair_term(bool in, float r2, float qq, float sig, float eps4, float krf) {
+    PairE o;
+    const float rinv = rsqrt_nr(r2);
+    const float r2inv = rinv * rinv;
+    const float s2 = sig * sig * r2inv;
+    const float s6 = s2 * s2 * s2;
+    const float elj = eps4 * fmaf(s6, s6, -s6);
+    const float flj = eps4 * s6 * fmaf(12.0f, s6, -6.0f);
+    const float fc = qq * fmaf(-2.0f * krf, r2, rinv);
+    const float fs = (flj + fc) * r2inv;
+    const float qqm = in ? qq : 0.f;
+    o.fs = in ? fs : 0.f;
+    o.elj = in ? elj : 0.f;
+    o.qr = qqm * (in ? rinv : 0.f);
+    o.qq = qqm;
+    o.qr2 = qqm * r2;
+    return o;
+}
+
+// FP64 scalar pair function from the exact FP64 r^2 and FP64 per-atom
+// parameters (h = sigma/2, e' = 2 sqrt(eps), q' = q sqrt(k_e)).  On CDNA4 the
+// FP64 FMA issues at the rate of a non-packed FP32 FMA.  Returns the force
+// magnitude over r (FP64), the energy without the reaction-field constants
+// (e = E_LJ + q'q'/r) and q'q'.
+struct PairE64 {
+    double fs, e, qq;
+};
+
+__device__ __forceinline__ PairE64 pair_term64(bool in, double r2, double sig, double eps4, double qq, double krf) {
+    double y = __builtin_amdgcn_rsq(r2);  // ~2^-22 estimate
+    {
+        const double t = r2 * y;
+        const double e = fma(-t, y, 1.0);
+        y = fma(0.5 * y, e, y);            // Newton on the residual: ~1e-13
+    }
+    const double r2inv = y * y;
+    const double s2 = sig * sig * r2inv;
+    const double s6 = s2 * s2 * s2;
+    const double elj = eps4 * fma(s6, s6, -s6);
+    const double flj = eps4 * s6 * fma(12.0, s6, -6.0);
+    const double fc = qq * fma(-2.0 * krf, r2, y);
+    PairE64 o;
+    o.fs = in ? (flj + fc) * r2inv : 0.0;
+    o.e = in ? fma(qq, y, elj) : 0.0;
+    o.qq = in ? qq : 0.0;
+    return o;
+}
+
+__device__ __forceinline__ uint64_t lanemask_lt() {
+    const int lane = threadIdx.x & 63;
+    return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
+// ---------------------------------------------------------------- binning
+
+__global__ void k_zero_counts(DevPtrs P, int cellBegin, int cellEnd, int setBegin, int setEnd) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    for (int k = cellBegin + i; k < cellEnd; k += gridDim.x * blockDim.x) P.cellCount[k] = 0;
+    if (i < setEnd - setBegin) P.rt[setBegin + i].overflow = 0;
+}
+
+__global__ void k_bin(DevPtrs P, int setBegin) {
+    const int c = setBegin + blockIdx.y;
+    const SetDesc& D = P.desc[c];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= D.natoms) return;
+    float4 p = P.posq[D.atomOff + i];
+    if (D.state) {
+        float4 d = P.displ[D.atomOff + i];
+        p.x += d.x; p.y += d.y; p.z += d.z;  // exact on the 2^-20 grid
+    }
+    p.x = wrap_coord(p.x, D.L[0], D.invL[0]);
+    p.y = wrap_coord(p.y, D.L[1], D.invL[1]);
+    p.z = wrap_coord(p.z, D.L[2], D.invL[2]);
+    // clamp in float before the conversion: every index stays inside the grid
+    const int cx = (int)fminf(fmaxf(p.x * D.invCe[0], 0.f), (float)(D.nx - 1));
+    const int cy = (int)fminf(fmaxf(p.y * D.invCe[1], 0.f), (float)(D.ny - 1));
+    const int cz = (int)fminf(fmaxf(p.z * D.invZe, 0.f), (float)(D.nz - 1));
+    int cell = (cx * D.ny + cy) * D.nz + cz;
+    int r = atomicAdd(P.cellCount + D.cellOff + cell, 1);
+    const int k = D.setAtomOff + i;
+    P.bpos[k] = p;
+    P.bcell[k] = cell;
+    P.brank[k] = r;
+}
+
+// One block of 256 threads per set.  Each thread scans a contiguous chunk of
+// columns; a block scan joins the chunks.
+__global__ void __launch_bounds__(256) k_scan(DevPtrs P, int setBegin) {
+    const int c = setBegin + blockIdx.x;
+    const SetDesc& D = P.desc[c];
+    const int t = threadIdx.x;
+    const int ncols = D.nx * D.ny, nz = D.nz;
+    const int chunk = (ncols + 255) / 256;
+    const int b = min(t * chunk, ncols), e = min(b + chunk, ncols);
+    __shared__ int sS[256], sG[256];
+    int accS = 0, accG = 0;
+    for (int col = b; col < e; ++col) {
+        const int* cnt = P.cellCount + D.cellOff + (size_t)col * nz;
+        int* st = P.cellStart + D.cellOff + (size_t)col * nz;
+        int s = 0;
+        for (int z = 0; z < nz; ++z) { st[z] = s; s += cnt[z]; }
+        P.colCount[D.colOff + col] = s;
+        int padded = (s + 7) & ~7;
+        int ncl = padded >> 3;
+        accS += padded;
+        accG += (ncl + kGroupClusters - 1) / kGroupClusters;
+    }
+    sS[t] = accS; sG[t] = accG;
+    __syncthreads();
+    for (int o = 1; o < 256; o <<= 1) {  // inclusive Hillis-Steele scan
+        int vs = t >= o ? sS[t - o] : 0, vg = t >= o ? sG[t - o] : 0;
+        __syncthreads();
+        sS[t] += vs; sG[t] += vg;
+        __syncthreads();
+    }
+    int offS = sS[t] - accS, offG = sG[t] - accG;
+    for (int col = b; col < e; ++col) {
+        int* st = P.cellStart + D.cellOff + (size_t)col * nz;
+        const int cnt = P.colCount[D.colOff + col];
+        const int padded = (cnt + 7) & ~7;
+        const int ncl = padded >> 3;
+        const int ng = (ncl + kGroupClusters - 1) / kGroupClusters;
+        P.colStart[D.colOff + col] = offS;
+        for (int z = 0; z < nz; ++z) st[z] += offS;
+        for (int s = offS + cnt; s < offS + padded; ++s) {
+            const int k = D.slotOff + s;
+            P.spos[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+            P.sprm[k] = make_float2(0.f, 0.f);
+            P.sprm64[k] = make_double4(0.0, 0.0, 0.0, 0.0);
+            P.sidx[k] = -1;
+            P.sfx[k] = 0; P.sfy[k] = 0; P.sfz[k] = 0;
+            P.sdx[k] = 0.0; P.sdy[k] = 0.0; P.sdz[k] = 0.0;
+        }
+        for (int gg = 0; gg < ng; ++gg)
+            P.ginfo[D.groupOff + offG + gg] =
+                make_int4((offS >> 3) + gg * kGroupClusters, min(kGroupClusters, ncl - gg * kGroupClusters), col, 0);
+        offS += padded;
+        offG += ng;
+    }
+    if (t == 255) {
+        P.rt[c].nslots = sS[255];
+        P.rt[c].nclusters = sS[255] >> 3;
+        P.rt[c].ngroups = sG[255];
+    }
+}
+
+__global__ void k_scatter(DevPtrs P, int setBegin) {
+    const int c = setBegin + blockIdx.y;
+    const SetDesc& D = P.desc[c];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= D.natoms) return;
+    const int k = D.setAtomOff + i;
+    const int slot = P.cellStart[D.cellOff + P.bcell[k]] + P.brank[k];
+    const int s = D.slotOff + slot;
+    P.spos[s] = P.bpos[k];
+    P.sprm[s] = P.ljp[D.atomOff + i];
+    P.sprm64[s] = P.prm64[D.atomOff + i];
+    P.sidx[s] = i;
+    P.sfx[s] = 0; P.sfy[s] = 0; P.sfz[s] = 0;
+    P.sdx[s] = 0.0; P.sdy[s] = 0.0; P.sdz[s] = 0.0;
+    P.slotOf[k] = slot;
+}
+
+// Order each fine cell's slots by original atom index (the atomic ranks of
+// k_bin are not deterministic); makes cluster composition, FP32 partial sums
+// and therefore every result of an evaluation bit-reproducible.
+__global__ void k_cellsort(DevPtrs P, int setBegin) {
+    const int c = setBegin + blockIdx.y;
+    const SetDesc& D = P.desc[c];
+    const int cell = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell >= D.nx * D.ny * D.nz) return;
+    const int cnt = P.cellCount[D.cellOff + cell];
+    if (cnt < 1) return;
+    const int start = P.cellStart[D.cellOff + cell];
+    const int base = D.slotOff + start;
+    for (int a = 1; a < cnt; ++a) {
+        const int key = P.sidx[base + a];
+        const float4 kp = P.spos[base + a];
+        const float2 kq = P.sprm[base + a];
+        const double4 kd = P.sprm64[base + a];
+        int b = a - 1;
+        while (b >= 0 && P.sidx[base + b] > key) {
+            P.sidx[base + b + 1] = P.sidx[base + b];
+            P.spos[base + b + 1] = P.spos[base + b];
+            P.sprm[base + b + 1] = P.sprm[base + b];
+            P.sprm64[base + b + 1] = P.sprm64[base + b];
+            --b;
+        }
+        P.sidx[base + b + 1] = key;
+        P.spos[base + b + 1] = kp;
+        P.sprm[base + b + 1] = kq;
+        P.sprm64[base + b + 1] = kd;
+    }
+    for (int a = 0; a < cnt; ++a) P.slotOf[D.setAtomOff + P.sidx[base + a]] = start + a;
+}
+
+__global__ void k_bbox(DevPtrs P, int setBegin) {
+    const int c = setBegin + blockIdx.y;
+    const SetDesc& D = P.desc[c];
+    const int cl = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cl >= P.rt[c].nclusters) return;
+    float lo[3] = {3.0e38f, 3.0e38f, 3.0e38f}, hi[3] = {-3.0e38f, -3.0e38f, -3.0e38f};
+    const int s0 = D.slotOff + cl * kClusterSize;
+    for (int a = 0; a < kClusterSize; ++a) {
+        if (P.sidx[s0 + a] < 0) continue;
+        float4 p = P.spos[s0 + a];
+        lo[0] = fminf(lo[0], p.x); hi[0] = fmaxf(hi[0], p.x);
+        lo[1] = fminf(lo[1], p.y); hi[1] = fmaxf(hi[1], p.y);
+        lo[2] = fminf(lo[2], p.z); hi[2] = fmaxf(hi[2], p.z);
+    }
+    // every cluster holds at least one real atom (padding < 8 per column)
+    P.bbc[D.clusterOff + cl] = make_float4(0.5f * (lo[0] + hi[0]), 0.5f * (lo[1] + hi[1]), 0.5f * (lo[2] + hi[2]), 0.f);
+    P.bbh[D.clusterOff + cl] = make_float4(0.5f * (hi[0] - lo[0]), 0.5f * (hi[1] - lo[1]), 0.5f * (hi[2] - lo[2]), 0.f);
+}
+
+// ---------------------------------------------------------------- pair list
+
+// One wave64 per group.  Entry = J (20 bits) | imask (8 bits) << 20 |
+// x-shift code << 28 | y-shift code << 30 (code 1: +L, 2: -L).
+template <int W>
+__global__ void __launch_bounds__(64 * W) k_pairlist(DevPtrs P, int setBegin) {
+    const int c = setBegin + blockIdx.y;
+    const SetDesc& D = P.desc[c];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int g = blockIdx.x * W + wave;
+    if (g >= P.rt[c].ngroups) return;
+    const int4 gi = P.ginfo[D.groupOff + g];
+    const int g0 = gi.x, ncl = gi.y, col = gi.z;
+    const int gx = col / D.ny, gy = col % D.ny;
+    float4 myc = make_float4(0.f, 0.f, 0.f, 0.f), myh = myc;
+    if (lane < ncl) { myc = P.bbc[D.clusterOff + g0 + lane]; myh = P.bbh[D.clusterOff + g0 + lane]; }
+    float icx[kGroupClusters], icy[kGroupClusters], icz[kGroupClusters];
+    float ihx[kGroupClusters], ihy[kGroupClusters], ihz[kGroupClusters];
+#pragma unroll
+    for (int q = 0; q < kGroupClusters; ++q) {
+        icx[q] = __shfl(myc.x, q, 64); icy[q] = __shfl(myc.y, q, 64); icz[q] = __shfl(myc.z, q, 64);
+        ihx[q] = __shfl(myh.x, q, 64); ihy[q] = __shfl(myh.y, q, 64); ihz[q] = __shfl(myh.z, q, 64);
+    }
+    uint32_t* out = P.entries + D.entryOff + (long long)g * D.entryCap;
+    int nEnt = 0;
+    const int xb = D.fullX ? 0 : -D.Rx, xe = D.fullX ? D.nx - 1 : D.Rx;
+    const int yb = D.fullY ? 0 : -D.Ry, ye = D.fullY ? D.ny - 1 : D.Ry;
+    for (int ox = xb; ox <= xe; ++ox) {
+        int jx = ox, sxc = 0;
+        float gapx = 0.f;
+        if (!D.fullX) {
+            jx = gx + ox;
+            if (jx < 0) { jx += D.nx; sxc = 2; } else if (jx >= D.nx) { jx -= D.nx; sxc = 1; }
+            gapx = fmaxf(0.f, (float)(abs(ox) - 1)) * D.ce[0];
+        }
+        for (int oy = yb; oy <= ye; ++oy) {
+            int jy = oy, syc = 0;
+            float gapy = 0.f;
+            if (!D.fullY) {
+                jy = gy + oy;
+                if (jy < 0) { jy += D.ny; syc = 2; } else if (jy >= D.ny) { jy -= D.ny; syc = 1; }
+                gapy = fmaxf(0.f, (float)(abs(oy) - 1)) * D.ce[1];
+            }
+            if (gapx * gapx + gapy * gapy >= D.rcList2) continue;
+            const int colJ = jx * D.ny + jy;
+            const int cs = P.colStart[D.colOff + colJ], cnt = P.colCount[D.colOff + colJ];
+            const int c0 = cs >> 3, c1 = (cs + ((cnt + 7) & ~7)) >> 3;
+            if (c1 <= g0) continue;
+            const float sx = D.shiftOK ? (sxc == 1 ? D.L[0] : (sxc == 2 ? -D.L[0] : 0.f)) : 0.f;
+            const float sy = D.shiftOK ? (syc == 1 ? D.L[1] : (syc == 2 ? -D.L[1] : 0.f)) : 0.f;
+            for (int J0 = max(c0, g0); J0 < c1; J0 += 64) {
+                const int J = J0 + lane;
+                uint32_t mask = 0;
+                if (J < c1) {
+                    const float4 jc = P.bbc[D.clusterOff + J], jh = P.bbh[D.clusterOff + J];
+#pragma unroll
+                    for (int q = 0; q < kGroupClusters; ++q) {
+                        if (q < ncl && g0 + q <= J) {
+                            float dx = jc.x + sx - icx[q], dy = jc.y + sy - icy[q], dz = jc.z - icz[q];
+                            if (!D.shiftOK) { dx = min_image(dx, D.L[0], D.invL[0]); dy = min_image(dy, D.L[1], D.invL[1]); }
+                            dz = min_image(dz, D.L[2], D.invL[2]);
+                            float ax = fmaxf(0.f, fabsf(dx) - ihx[q] - jh.x);
+                            float ay = fmaxf(0.f, fabsf(dy) - ihy[q] - jh.y);
+                            float az = fmaxf(0.f, fabsf(dz) - ihz[q] - jh.z);
+                            if (ax * ax + ay * ay + az * az < D.rcList2) mask |= 1u << q;
+                        }
+                    }
+                }
+                const uint64_t bal = __ballot(mask != 0);
+                const int pos = nEnt + __popcll(bal & lanemask_lt());
+                if (mask != 0 && pos < D.entryCap) {
+                    uint32_t codes = D.shiftOK ? ((uint32_t)sxc << 28) | ((uint32_t)syc << 30) : 0u;
+                    out[pos] = (uint32_t)J | (mask << 20) | codes;
+                }
+                nEnt += __popcll(bal);
+            }
+        }
+    }
+    if (lane == 0) {
+        P.gcount[D.groupOff + g] = min(nEnt, D.entryCap);
+        if (nEnt > D.entryCap) atomicOr(&P.rt[c].overflow, 1);
+    }
+}
+
+// ---------------------------------------------------------------- pair kernel
+
+// Decode an entry and load this lane's j-atom (lane = 8 jj + ii).
+struct JAtom {
+    float4 p;
+    float2 prm;
+    bool valid;
+    int slot;
+};
+
+__device__ __forceinline__ JAtom load_j(const DevPtrs& P, const SetDesc& D, uint32_t ent, int jj) {
+    JAtom a;
+    const int J = ent & 0xFFFFF;
+    a.slot = D.slotOff + J * kClusterSize + jj;
+    a.p = P.spos[a.slot];
+    a.prm = P.sprm[a.slot];
+    a.valid = P.sidx[a.slot] >= 0;
+    if (D.shiftOK) {
+        const uint32_t sxc = (ent >> 28) & 3u, syc = (ent >> 30) & 3u;
+        a.p.x += sxc == 1u ? D.L[0] : (sxc == 2u ? -D.L[0] : 0.f);
+        a.p.y += syc == 1u ? D.L[1] : (syc == 2u ? -D.L[1] : 0.f);
+    }
+    return a;
+}
+
+// One wave64 per group of up to 8 i-clusters; lane = 8 jj + ii pairs i-atom ii
+// of each interacting i-cluster with j-atom jj of the list entry's j-cluster.
+// F64: scalar pair function in FP64 (exact r^2, default); otherwise FP32.
+// Coordinates, force vectors and their per-lane accumulators are FP32 either
+// way; forces are flushed to 64-bit fixed point, energies summed in FP64.
+template <int W, bool FORCES, bool F64>
+__global__ void __launch_bounds__(64 * W) k_pairs(DevPtrs P, int setBegin) {
+    const int c = setBegin + blockIdx.y;
+    const SetDesc& D = P.desc[c];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int jj = lane >> 3, ii = lane & 7;
+    const int g = blockIdx.x * W + wave;
+    const bool active = g < P.rt[c].ngroups;
+    __shared__ float4 sPos[W][64];
+    __shared__ float2 sPrm[W][F64 ? 1 : 64];
+    __shared__ double4 sPrm64[W][F64 ? 64 : 1];
+    int g0 = 0, ncl = 0;
+    uint32_t ivm = 0;  // bit t: i-atom (t, ii) is a real atom
+    if (active) {
+        const int4 gi = P.ginfo[D.groupOff + g];
+        g0 = gi.x; ncl = gi.y;
+        float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
+        float2 q = make_float2(0.f, 0.f);
+        double4 qd = make_double4(0.0, 0.0, 0.0, 0.0);
+        bool v = false;
+        if (lane < ncl * kClusterSize) {
+            const int s = D.slotOff + g0 * kClusterSize + lane;
+            p = P.spos[s];
+            if (F64) qd = P.sprm64[s]; else q = P.sprm[s];
+            v = P.sidx[s] >= 0;
+        }
+        sPos[wave][lane] = p;
+        if (F64) sPrm64[wave][lane] = qd; else sPrm[wave][lane] = q;
+        const uint64_t vb = __ballot(v);
+#pragma unroll
+        for (int t = 0; t < kGroupClusters; ++t) ivm |= (uint32_t)((vb >> (t * kClusterSize + ii)) & 1ull) << t;
+    }
+    __syncthreads();
+    if (!active) return;
+
+    const float Lz = D.L[2], invLz = D.invL[2];
+    const float Lx = D.L[0], invLx = D.invL[0], Ly = D.L[1], invLy = D.invL[1];
+    const bool shiftOK = D.shiftOK;
+    const double rc2 = D.rc2;
+    float fi[kGroupClusters][3];
+#pragma unroll
+    for (int t = 0; t < kGroupClusters; ++t) fi[t][0] = fi[t][1] = fi[t][2] = 0.f;
+    double eE = 0.0, eQQ = 0.0, eQR2 = 0.0;  // sum(E_LJ + qq/r), sum(qq), sum(qq r^2)
+    const int ne = P.gcount[D.groupOff + g];
+    const uint32_t* E = P.entries + D.entryOff + (long long)g * D.entryCap;
+    for (int e = 0; e < ne; ++e) {
+        const uint32_t ent = E[e];
+        const int J = ent & 0xFFFFF;
+        const uint32_t imask = (ent >> 20) & 0xFFu;
+        const JAtom ja = load_j(P, D, ent, jj);
+        double4 jd = make_double4(0.0, 0.0, 0.0, 0.0);
+        if (F64) jd = P.sprm64[ja.slot];
+        float fjx = 0.f, fjy = 0.f, fjz = 0.f;
+#pragma unroll
+        for (int t = 0; t < kGroupClusters; ++t) {
+            if (imask & (1u << t)) {
+                const float4 pi = sPos[wave][t * kClusterSize + ii];
+                float dx = pi.x - ja.p.x, dy = pi.y - ja.p.y, dz = pi.z - ja.p.z;
+                if (!shiftOK) { dx = min_image(dx, Lx, invLx); dy = min_image(dy, Ly, invLy); }
+                dz = min_image(dz, Lz, invLz);
+                float fs;
+                if (F64) {
+                    const double4 id = sPrm64[wave][t * kClusterSize + ii];
+                    // exact: the squares of 2^-20-grid differences fit in 53 bits
+                    const double r2 = fma((double)dx, (double)dx, fma((double)dy, (double)dy, (double)dz * (double)dz));
+                    bool in = r2 < rc2 && ja.valid && ((ivm >> t) & 1u);
+                    if (g0 + t == J) in = in && (ii < jj);
+                    const PairE64 pe = pair_term64(in, r2, id.x + jd.x, id.y * jd.y, id.z * jd.z, D.krf64);
+                    eE += pe.e; eQQ += pe.qq; eQR2 += pe.qq * r2;
+                    fs = (float)pe.fs;
+                } else {
+                    const float2 qi = sPrm[wave][t * kClusterSize + ii];
+                    const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                    bool in = in_cutoff(D, dx, dy, dz, r2) && ja.valid && ((ivm >> t) & 1u);
+                    if (g0 + t == J) in = in && (ii < jj);
+                    const PairE pe = pair_term(in, r2, pi.w * ja.p.w, qi.x + ja.prm.x, qi.y * ja.prm.y, D.krf);
+                    eE += (double)(pe.elj + pe.qr); eQQ += (double)pe.qq; eQR2 += (double)pe.qr2;
+                    fs = pe.fs;
+                }
+                if (FORCES) {
+                    const float fx = fs * dx, fy = fs * dy, fz = fs * dz;
+                    fi[t][0] += fx; fi[t][1] += fy; fi[t][2] += fz;
+                    fjx -= fx; fjy -= fy; fjz -= fz;
+                }
+            }
+        }
+        if (FORCES) {
+            fjx = oct_sum(fjx); fjy = oct_sum(fjy); fjz = oct_sum(fjz);
+            if (ii == 0 && ja.valid) add_force(P, ja.slot, fjx, fjy, fjz);
+        }
+    }
+    if (FORCES) {
+#pragma unroll
+        for (int t = 0; t < kGroupClusters; ++t) {
+            if (t < ncl) {
+                const float fx = stride8_sum(fi[t][0]), fy = stride8_sum(fi[t][1]), fz = stride8_sum(fi[t][2]);
+                if (jj == 0 && ((ivm >> t) & 1u)) add_force(P, D.slotOff + (g0 + t) * kClusterSize + ii, fx, fy, fz);
+            }
+        }
+    }
+    double eacc = eE + D.krf64 * eQR2 - D.crf64 * eQQ;
+    eacc = wave_sum_d(eacc);
+    if (lane == 0) P.genergy[D.groupOff + g] = eacc;
+}
+
+// Parity/statistics: the same traversal, emitting in-cutoff pairs as original
+// atom indices (i < j) and counting evaluated candidates.
+__global__ void __launch_bounds__(64) k_pairs_dump(DevPtrs P, int c, int2* out, long long cap,
+                                                   unsigned long long* counters) {
+    const SetDesc& D = P.desc[c];
+    const int lane = threadIdx.x & 63, jj = lane >> 3, ii = lane & 7;
+    const int g = blockIdx.x;
+    if (g >= P.rt[c].ngroups) return;
+    const int4 gi = P.ginfo[D.groupOff + g];
+    const int g0 = gi.x, ncl = gi.y;
+    const int ne = P.gcount[D.groupOff + g];
+    const uint32_t* E = P.entries + D.entryOff + (long long)g * D.entryCap;
+    unsigned long long nin = 0, nev = 0;
+    for (int e = 0; e < ne; ++e) {
+        const uint32_t ent = E[e];
+        const int J = ent & 0xFFFFF;
+        const uint32_t imask = (ent >> 20) & 0xFFu;
+        const JAtom ja = load_j(P, D, ent, jj);
+        const int jidx = P.sidx[ja.slot];
+        for (int t = 0; t < ncl; ++t) {
+            if (!(imask & (1u << t))) continue;
+            nev += 1;
+            const int si = D.slotOff + (g0 + t) * kClusterSize + ii;
+            const float4 pi = P.spos[si];
+            const int iidx = P.sidx[si];
+            float dx = pi.x - ja.p.x, dy = pi.y - ja.p.y, dz = pi.z - ja.p.z;
+            if (!D.shiftOK) { dx = min_image(dx, D.L[0], D.invL[0]); dy = min_image(dy, D.L[1], D.invL[1]); }
+            dz = min_image(dz, D.L[2], D.invL[2]);
+            const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+            bool in = in_cutoff(D, dx, dy, dz, r2) && ja.valid && iidx >= 0;
+            if (g0 + t == J) in = in && (ii < jj);
+            if (in) {
+                nin += 1;
+                unsigned long long k = atomicAdd(counters, 1ull);
+                if ((long long)k < cap) out[k] = make_int2(min(iidx, jidx), max(iidx, jidx));
+            }
+        }
+    }
+    atomicAdd(counters + 1, nev);  // per lane: candidate pairs evaluated
+    atomicAdd(counters + 2, nin);  // per lane: in-cutoff pairs
+}
+
+// ---------------------------------------------------------------- ATM combine
+
+// One block of 256 threads per system: fixed-order FP64 sums of the group
+// energies of both states, then the scalar ATM combine.
+__global__ void __launch_bounds__(256) k_energy(DevPtrs P, int sysBegin) {
+    const int s = sysBegin + blockIdx.x;
+    __shared__ double red[256];
+    double U[2];
+    for (int st = 0; st < 2; ++st) {
+        const int c = 2 * s + st;
+        const SetDesc& D = P.desc[c];
+        const int ng = P.rt[c].ngroups;
+        double acc = 0.0;
+        for (int g = threadIdx.x; g < ng; g += 256) acc += P.genergy[D.groupOff + g];
+        red[threadIdx.x] = acc;
+        __syncthreads();
+        for (int o = 128; o > 0; o >>= 1) {
+            if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+            __syncthreads();
+        }
+        U[st] = red[0];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        SysResult r;
+        atm_combine(P.params[s], U[0], U[1], r);
+        P.res[s] = r;
+    }
+}
+
+// Fixed-order energy of one set into P.setEnergy[c] (reference-structured path).
+__global__ void __launch_bounds__(256) k_set_energy(DevPtrs P, int c) {
+    __shared__ double red[256];
+    const SetDesc& D = P.desc[c];
+    const int ng = P.rt[c].ngroups;
+    double acc = 0.0;
+    for (int g = threadIdx.x; g < ng; g += 256) acc += P.genergy[D.groupOff + g];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) P.setEnergy[c] = red[0];
+}
+
+// F_sys += w1 F1 + w2 F2 with (w1, w2) = (1-sp, sp) for direction > 0 and
+// (sp, 1-sp) otherwise (ReferenceATMMetaForceKernels.cpp:100-110).  In host mode
+// sp and direction come from kernel arguments (the reference's HybridForce
+// receives sp as a scalar argument, CommonATMMetaForceKernels.cpp:199-201).
+__global__ void k_blend(DevPtrs P, int sysBegin, int hostMode, double spHost, double dirHost) {
+    const int s = sysBegin + blockIdx.y;
+    const SetDesc& D0 = P.desc[2 * s];
+    const SetDesc& D1 = P.desc[2 * s + 1];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int N = D0.natoms;
+    if (i >= N) return;
+    const double sp = hostMode ? spHost : P.res[s].sp;
+    const double dir = hostMode ? dirHost : P.params[s].direction;
+    const double w2 = dir > 0.0 ? sp : 1.0 - sp;
+    const double w1 = dir > 0.0 ? 1.0 - sp : sp;
+    const int a = D0.slotOff + P.slotOf[D0.setAtomOff + i];
+    const int b = D1.slotOff + P.slotOf[D1.setAtomOff + i];
+    long long* F = P.outF + 3ll * D0.atomOff;
+    const double inv = 1.0 / kForceScale;
+    const double f1x = (double)P.sfx[a] * inv + P.sdx[a], f2x = (double)P.sfx[b] * inv + P.sdx[b];
+    const double f1y = (double)P.sfy[a] * inv + P.sdy[a], f2y = (double)P.sfy[b] * inv + P.sdy[b];
+    const double f1z = (double)P.sfz[a] * inv + P.sdz[a], f2z = (double)P.sfz[b] * inv + P.sdz[b];
+    F[i] += to_fixed_sat(w1 * f1x + w2 * f2x);
+    F[N + i] += to_fixed_sat(w1 * f1y + w2 * f2y);
+    F[2 * N + i] += to_fixed_sat(w1 * f1z + w2 * f2z);
+}
+
+__global__ void k_fill_i64(long long* p, long long n) {
+    for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < n; k += (long long)gridDim.x * blockDim.x) p[k] = 0;
+}
+
+// ---------------------------------------------------------------- launchers
+
+static inline unsigned cdiv(long long a, long long b) { return (unsigned)((a + b - 1) / b); }
+
+hipError_t launch_zero_counts(const DevPtrs& P, int cellBegin, int cellEnd, int setBegin, int setEnd, hipStream_t st) {
+    long long n = cellEnd - cellBegin;
+    unsigned blocks = (unsigned)std::min<long long>(std::max<long long>(cdiv(n, 256), cdiv(setEnd - setBegin, 256)), 4096);
+    if (blocks == 0) blocks = 1;
+    k_zero_counts<<<blocks, 256, 0, st>>>(P, cellBegin, cellEnd, setBegin, setEnd);
+    return hipGetLastError();
+}
+
+hipError_t launch_build(const DevPtrs& P, int setBegin, int nsets, int maxAtoms, int maxCells, int maxClusters,
+                        hipStream_t st) {
+    dim3 ga(cdiv(maxAtoms, 256), nsets);
+    k_bin<<<ga, 256, 0, st>>>(P, setBegin);
+    k_scan<<<nsets, 256, 0, st>>>(P, setBegin);
+    k_scatter<<<ga, 256, 0, st>>>(P, setBegin);
+    k_cellsort<<<dim3(cdiv(maxCells, 256), nsets), 256, 0, st>>>(P, setBegin);
+    dim3 gb(cdiv(maxClusters, 256), nsets);
+    k_bbox<<<gb, 256, 0, st>>>(P, setBegin);
+    return hipGetLastError();
+}
+
+hipError_t launch_pairlist(const DevPtrs& P, int setBegin, int nsets, int maxGroups, int listWaves, hipStream_t st) {
+    switch (listWaves) {
+        case 1: k_pairlist<1><<<dim3(cdiv(maxGroups, 1), nsets), 64, 0, st>>>(P, setBegin); break;
+        case 2: k_pairlist<2><<<dim3(cdiv(maxGroups, 2), nsets), 128, 0, st>>>(P, setBegin); break;
+        default: k_pairlist<4><<<dim3(cdiv(maxGroups, 4), nsets), 256, 0, st>>>(P, setBegin); break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_pairs(const DevPtrs& P, int setBegin, int nsets, int maxGroups, int waves, bool forces, bool fp64,
+                        hipStream_t st) {
+#define ATM_PAIRS(WV, F64)                                                                                 \
+    if (forces) k_pairs<WV, true, F64><<<dim3(cdiv(maxGroups, WV), nsets), 64 * WV, 0, st>>>(P, setBegin);  \
+    else k_pairs<WV, false, F64><<<dim3(cdiv(maxGroups, WV), nsets), 64 * WV, 0, st>>>(P, setBegin);
+#define ATM_PAIRS_W(F64)               \
+    switch (waves) {                   \
+        case 1: ATM_PAIRS(1, F64) break; \
+        case 2: ATM_PAIRS(2, F64) break; \
+        case 8: ATM_PAIRS(8, F64) break; \
+        default: ATM_PAIRS(4, F64) break; \
+    }
+    if (fp64) { ATM_PAIRS_W(true) } else { ATM_PAIRS_W(false) }
+#undef ATM_PAIRS_W
+#undef ATM_PAIRS
+    return hipGetLastError();
+}
+
+hipError_t launch_energy(const DevPtrs& P, int sysBegin, int nsys, hipStream_t st) {
+    k_energy<<<nsys, 256, 0, st>>>(P, sysBegin);
+    return hipGetLastError();
+}
+
+hipError_t launch_set_energy(const DevPtrs& P, int c, hipStream_t st) {
+    k_set_energy<<<1, 256, 0, st>>>(P, c);
+    return hipGetLastError();
+}
+
+hipError_t launch_blend(const DevPtrs& P, int sysBegin, int nsys, int maxAtoms, int hostMode, double sp, double dir,
+                        hipStream_t st) {
+    k_blend<<<dim3(cdiv(maxAtoms, 256), nsys), 256, 0, st>>>(P, sysBegin, hostMode, sp, dir);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_i64(long long* p, long long n, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    unsigned blocks = (unsigned)std::min<long long>(cdiv(n, 256), 4096);
+    k_fill_i64<<<blocks, 256, 0, st>>>(p, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_pairs_dump(const DevPtrs& P, int c, int maxGroups, int2* out, long long cap,
+                             unsigned long long* counters, hipStream_t st) {
+    k_pairs_dump<<<maxGroups, 64, 0, st>>>(P, c, out, cap, counters);
+    return hipGetLastError();
+}
+
+}  // namespace atm

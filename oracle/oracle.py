@@ -1,0 +1,168 @@
+"""ctypes front end of the FP64 oracle (oracle/atm_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the product package.  Each function
+names the reference code it restates in atm_oracle.c's header.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libatm_oracle.so")
+_lib = None
+
+
+class Params(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in
+                ("lambda1", "lambda2", "alpha", "u0", "w0", "umax", "ubcore", "acore", "direction")]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in ("W", "usc", "U1", "U2", "u", "fp", "bfp", "sp")] + \
+               [("npairs1", ctypes.c_longlong), ("npairs2", ctypes.c_longlong),
+                ("abs1", ctypes.c_double), ("abs2", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def build():
+    subprocess.run(["make", "-C", _HERE, "-s"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH) or \
+                os.path.getmtime(_LIB_PATH) < os.path.getmtime(os.path.join(_HERE, "atm_oracle.c")):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        dp = ctypes.POINTER(ctypes.c_double)
+        ip = ctypes.POINTER(ctypes.c_int)
+        L.oracle_softcore.restype = ctypes.c_double
+        L.oracle_softcore.argtypes = [ctypes.c_double] * 4 + [dp]
+        L.oracle_combine.argtypes = [ctypes.POINTER(Params), ctypes.c_double, ctypes.c_double, ctypes.POINTER(Result)]
+        L.oracle_blend.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, dp, dp, dp]
+        L.oracle_pairs.restype = ctypes.c_longlong
+        L.oracle_pairs.argtypes = [ctypes.c_int, dp, dp, ctypes.c_double, ctypes.c_int, ip, ctypes.c_longlong]
+        L.oracle_nonbonded.restype = ctypes.c_double
+        L.oracle_nonbonded.argtypes = [ctypes.c_int, dp, dp, dp, dp, dp, ctypes.c_double, ctypes.c_double,
+                                       ctypes.c_int, dp, ctypes.POINTER(ctypes.c_longlong), dp]
+        L.oracle_atm_eval.restype = ctypes.c_int
+        L.oracle_atm_eval.argtypes = [ctypes.c_int, dp, dp, dp, dp, dp, dp, ctypes.c_double, ctypes.c_double,
+                                      ctypes.POINTER(Params), ctypes.c_int, dp, ctypes.POINTER(Result)]
+        L.oracle_atm_grad.argtypes = [ctypes.c_int, dp, dp, dp, dp, dp, dp, ctypes.c_double, ctypes.c_double,
+                                      ctypes.POINTER(Params), ctypes.POINTER(Result), dp, dp, dp, dp]
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def params_of(p) -> Params:
+    if isinstance(p, Params):
+        return p
+    if hasattr(p, "as_tuple"):
+        return Params(*p.as_tuple())
+    if isinstance(p, dict):
+        return Params(**p)
+    return Params(*p)
+
+
+def softcore(u, umax, acore, ubcore):
+    fp = ctypes.c_double()
+    v = lib().oracle_softcore(u, umax, acore, ubcore, ctypes.byref(fp))
+    return v, fp.value
+
+
+def combine(params, U1, U2) -> dict:
+    r = Result()
+    p = params_of(params)
+    lib().oracle_combine(ctypes.byref(p), U1, U2, ctypes.byref(r))
+    return r.as_dict()
+
+
+def blend(sp, direction, F1, F2, F=None):
+    F1, p1 = _d(F1)
+    F2, p2 = _d(F2)
+    out = np.zeros_like(F1) if F is None else np.array(F, dtype=np.float64, copy=True)
+    out = np.ascontiguousarray(out)
+    lib().oracle_blend(F1.shape[0], sp, direction, p1, p2, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    return out
+
+
+def pairs(x, box, rc, brute=False) -> np.ndarray:
+    """Sorted (i, j) in-cutoff pairs, i < j, shape (P, 2)."""
+    x, px = _d(x)
+    box, pb = _d(box)
+    n = x.shape[0]
+    count = lib().oracle_pairs(n, px, pb, rc, int(brute), None, 0)
+    out = np.zeros((max(count, 1), 2), dtype=np.int32)
+    lib().oracle_pairs(n, px, pb, rc, int(brute), out.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), count)
+    out = out[:count]
+    order = np.lexsort((out[:, 1], out[:, 0]))
+    return out[order]
+
+
+def nonbonded(x, q, sig, eps, box, rc, eps_rf, brute=False, forces=True):
+    """(energy, forces (N,3) or None, npairs, (E_LJ, E_C, sum |e_ij|))."""
+    x, px = _d(x)
+    q, pq = _d(q)
+    sig, ps = _d(sig)
+    eps, pe = _d(eps)
+    box, pb = _d(box)
+    n = x.shape[0]
+    F = np.zeros((n, 3)) if forces else None
+    np_ = ctypes.c_longlong()
+    parts = np.zeros(3)
+    E = lib().oracle_nonbonded(n, px, pq, ps, pe, pb, rc, eps_rf, int(brute),
+                               F.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) if forces else None,
+                               ctypes.byref(np_), parts.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    return E, F, np_.value, (parts[0], parts[1], parts[2])
+
+
+def atm_eval(system, params=None, brute=False, forces=True):
+    """Oracle ATM evaluation of a workloads.System: (result dict, ATM force (N,3) or None)."""
+    p = params_of(params if params is not None else system.params)
+    x, px = _d(system.positions)
+    d, pd = _d(system.displacements)
+    q, pq = _d(system.charges)
+    s, ps = _d(system.sigmas)
+    e, pe = _d(system.epsilons)
+    b, pb = _d(system.box)
+    n = x.shape[0]
+    F = np.zeros((n, 3)) if forces else None
+    r = Result()
+    rc = lib().oracle_atm_eval(n, px, pd, pq, ps, pe, pb, system.cutoff, system.rf_dielectric, ctypes.byref(p),
+                               int(brute), F.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) if forces else None,
+                               ctypes.byref(r))
+    if rc != 0:
+        raise MemoryError("oracle_atm_eval failed")
+    return r.as_dict(), F
+
+
+def atm_grad(system, params=None):
+    """Analytic FP64 gradient: dict with dq, dsigma, deps (N,) and dparams (5,)."""
+    p = params_of(params if params is not None else system.params)
+    res, _ = atm_eval(system, p, forces=False)
+    r = Result(**{k: res[k] for k in ("W", "usc", "U1", "U2", "u", "fp", "bfp", "sp")})
+    x, px = _d(system.positions)
+    d, pd = _d(system.displacements)
+    q, pq = _d(system.charges)
+    s, ps = _d(system.sigmas)
+    e, pe = _d(system.epsilons)
+    b, pb = _d(system.box)
+    n = x.shape[0]
+    dq, ds, de, dp = np.zeros(n), np.zeros(n), np.zeros(n), np.zeros(5)
+    c = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    lib().oracle_atm_grad(n, px, pd, pq, ps, pe, pb, system.cutoff, system.rf_dielectric, ctypes.byref(p),
+                          ctypes.byref(r), c(dq), c(ds), c(de), c(dp))
+    return {"dq": dq, "dsigma": ds, "deps": de, "dparams": dp}

@@ -1,0 +1,185 @@
+"""Parity of the MI355X path against the FP64 oracle (through the C ABI).
+
+Bar for the benchmark configurations: in-cutoff pair sets identical to the
+oracle's; U1, U2 and W within 1e-6 relative; per-atom forces within 1e-5
+relative RMS (||F - F_ref|| / ||F_ref||).
+
+Small contrived systems whose energy is a small difference of large pair terms
+are held to the FP32 error model instead: |dU| <= 1e-6 max(|U|, sum_ij |e_ij|)
+(FP32 pair arithmetic: a few ulp per pair, up to ~16 ulp where (s/r)^12
+amplifies), and W to the bound propagated from U1 and U2.
+"""
+import numpy as np
+import pytest
+
+from atmmetaforce import ATMBatch, workloads as W
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+E_TOL = 1e-6
+F_TOL = 1e-5
+
+
+def rel(a, b):
+    return abs(a - b) / max(abs(b), 1e-300)
+
+
+def frms(F, Fref):
+    return float(np.linalg.norm(F - Fref) / max(np.linalg.norm(Fref), 1e-300))
+
+
+def check_system(batch, k, system, pairs=True, tag="", strict=True, forces=True, ref=None):
+    if ref is None:
+        ref, Fref = oracle.atm_eval(system, forces=forces)
+    else:
+        ref, Fref = ref
+    r = batch.result(k)
+    print(f"[{tag} sys {k} N={system.num_atoms}] U1 {r['state1_energy']:.10g} vs {ref['U1']:.10g}; "
+          f"U2 {r['state2_energy']:.10g} vs {ref['U2']:.10g}; W {r['energy']:.10g} vs {ref['W']:.10g}", flush=True)
+    dU1 = abs(r["state1_energy"] - ref["U1"])
+    dU2 = abs(r["state2_energy"] - ref["U2"])
+    dW = abs(r["energy"] - ref["W"])
+    if strict:
+        assert dU1 <= E_TOL * abs(ref["U1"])
+        assert dU2 <= E_TOL * abs(ref["U2"])
+        assert dW <= E_TOL * abs(ref["W"])
+    else:
+        b1 = E_TOL * max(abs(ref["U1"]), ref["abs1"])
+        b2 = E_TOL * max(abs(ref["U2"]), ref["abs2"])
+        w1, w2 = (1 - ref["sp"], ref["sp"]) if system.params.direction > 0 else (ref["sp"], 1 - ref["sp"])
+        assert dU1 <= b1
+        assert dU2 <= b2
+        assert dW <= E_TOL * abs(ref["W"]) + abs(w1) * b1 + abs(w2) * b2
+    assert abs(r["perturbation_energy"] - ref["usc"]) <= E_TOL * max(abs(ref["usc"]), 1.0)
+    if forces:
+        F = batch.forces(k)
+        err = frms(F, Fref)
+        print(f"    force rel RMS {err:.3e}", flush=True)
+        assert err < F_TOL
+    if pairs:
+        for state, x in ((0, system.positions), (1, system.positions + system.displacements)):
+            pg = batch.pairs(k, state)
+            po = oracle.pairs(x, system.box, system.cutoff)
+            assert pg.shape == po.shape, f"state {state}: {pg.shape[0]} pairs vs oracle {po.shape[0]}"
+            assert np.array_equal(pg, po)
+    return r, ref
+
+
+@pytest.mark.parametrize("n,box,rc,seed", [(64, 1.7, 0.8, 11), (500, 2.6, 0.9, 12), (3000, 3.1, 0.9, 1)])
+@pytest.mark.parametrize("plain", [False, True])
+def test_single_system(gpu, n, box, rc, seed, plain):
+    s = W.tiny_system(n, box, rc, max(4, n // 100), (box / 3, box / 4, box / 5), seed,
+                      W.ATMParams(0.3, 0.45, 0.1, 110.0, 1.5, 200.0, 100.0, 0.0625, 1.0))
+    with ATMBatch([s]) as b:
+        b.compute(plain=plain)
+        check_system(b, 0, s, tag=f"single plain={plain}", strict=False)
+
+
+@pytest.mark.parametrize("plain", [False, True])
+def test_config1(gpu, plain):
+    s = W.config1()
+    with ATMBatch([s]) as b:
+        b.compute(plain=plain)
+        check_system(b, 0, s, tag=f"cfg1 plain={plain}")
+
+
+def test_config2_full_size(gpu):
+    s = W.config2()
+    with ATMBatch([s]) as b:
+        b.compute()
+        check_system(b, 0, s, tag="cfg2")
+        b.compute(plain=True)
+        check_system(b, 0, s, pairs=False, tag="cfg2 plain")
+
+
+def test_config3_replica_batch(gpu):
+    """22 replicas of the 23k-atom system, lambda schedule 0..0.5, one launch."""
+    reps = W.replica_batch(22)
+    with ATMBatch(reps) as b:
+        b.compute()
+        for k, s in enumerate(reps):
+            check_system(b, k, s, pairs=(k in (0, 21)), tag="cfg3")
+
+
+def test_config4_uneven_batch(gpu):
+    """64 systems, 5k-60k atoms (log-uniform), ligand 10-80 atoms, one launch."""
+    systems = W.uneven_batch(64)
+    with ATMBatch(systems) as b:
+        b.compute()
+        for k, s in enumerate(systems):
+            check_system(b, k, s, pairs=(k % 16 == 0), forces=(k % 4 == 0), tag="cfg4")
+
+
+def test_small_uneven_batch(gpu):
+    systems = W.uneven_batch(5, min_atoms=800, max_atoms=6000, seed=77)
+    with ATMBatch(systems) as b:
+        b.compute()
+        for k, s in enumerate(systems):
+            check_system(b, k, s, pairs=True, tag="uneven-small", strict=False)
+
+
+def test_direction_reverse_and_zero_displacement(gpu):
+    # sparse box (7.6 atoms/nm^3): the displaced ligand does not clash, so the
+    # reverse-direction perturbation energy is moderate and sp = bfp is O(0.3)
+    ps = [W.ATMParams(0.2, 0.4, 0.05, 50.0, 0.0, 200.0, 100.0, 0.0625, -1.0),
+          W.ATMParams(0.2, 0.4, 0.05, 50.0, 3.0, 200.0, 100.0, 0.0625, 0.0)]  # direction 0 means reverse too
+    systems = [W.tiny_system(250, 3.2, 0.9, 8, (1.1, 0.7, 0.45), seed, p) for seed, p in zip((5, 6), ps)]
+    z = W.tiny_system(700, 2.8, 0.9, 0, (0, 0, 0), 6)
+    with ATMBatch(systems + [z]) as b:
+        b.compute()
+        for k, s in enumerate(systems):
+            r, ref = check_system(b, k, s, tag="reverse", strict=False)
+            assert r["hybrid_weight"] == pytest.approx(ref["sp"], rel=1e-6)
+        r = b.result(2)
+        assert r["state1_energy"] == r["state2_energy"]
+        assert r["raw_perturbation"] == 0.0
+
+
+def test_softplus_overflow_divergence(gpu):
+    """Where the reference's exp(-alpha (u_sc - u0)) overflows (u_sc << u0), the
+    reference returns W = inf; this implementation evaluates ln(1 + e^x) stably.
+    U1, U2 and the forces still match the oracle; W matches the stable closed form."""
+    p = W.ATMParams(0.2, 0.4, 0.05, 50.0, 0.0, 200.0, 100.0, 0.0625, -1.0)
+    s = W.tiny_system(700, 2.8, 0.9, 10, (1.0, 0.7, 0.4), 5, p)
+    ref, Fref = oracle.atm_eval(s)
+    assert np.isinf(ref["W"])  # the reference's own arithmetic overflows here
+    with ATMBatch([s]) as b:
+        b.compute()
+        r = b.result(0)
+        assert rel(r["state1_energy"], ref["U1"]) < E_TOL
+        assert rel(r["state2_energy"], ref["U2"]) < E_TOL
+        u = r["state1_energy"] - r["state2_energy"]
+        x = -p.alpha * (u - p.u0)           # u <= ubcore: u_sc = u
+        lnee = x + np.log1p(np.exp(-x))
+        W_stable = r["state2_energy"] + (p.lambda2 - p.lambda1) / p.alpha * lnee + p.lambda2 * u + p.w0
+        assert rel(r["energy"], W_stable) < 1e-9
+        assert frms(b.forces(0), Fref) < F_TOL
+
+
+def test_force_accumulation_and_energy_only(gpu):
+    s = W.tiny_system(400, 2.4, 0.8, 6, (0.9, 0.9, 0.9), 9)
+    with ATMBatch([s]) as b:
+        b.compute()
+        f1 = b.forces_fixed(0)
+        e1 = b.result(0)["energy"]
+        b.evaluate(forces=True, clear=False)
+        f2 = b.forces_fixed(0)
+        assert np.array_equal(f2, 2 * f1)
+        b.evaluate(forces=False, clear=True)
+        assert b.result(0)["energy"] == pytest.approx(e1, rel=1e-12)
+        assert not b.forces_fixed(0).any()
+
+
+def test_graph_replays_match_eager(gpu):
+    s = W.tiny_system(1500, 2.9, 0.9, 12, (1.2, 1.0, 0.8), 21)
+    with ATMBatch([s]) as b:
+        b.compute(graph=False)
+        e0 = b.result(0)
+        f0 = b.forces_fixed(0)
+        for _ in range(3):
+            b.compute(graph=True)
+            assert b.result(0)["state1_energy"] == pytest.approx(e0["state1_energy"], rel=1e-13)
+            assert b.result(0)["state2_energy"] == pytest.approx(e0["state2_energy"], rel=1e-13)
+            assert np.array_equal(b.forces_fixed(0), f0)
+            b.synchronize()
