@@ -57,7 +57,7 @@ def cpu_baseline(systems, gpu_result0, gpu_forces0, seconds=12.0):
         "U2_rel": abs(gpu_result0["state2_energy"] - ref["U2"]) / abs(ref["U2"]),
         "W_rel": abs(gpu_result0["energy"] - ref["W"]) / abs(ref["W"]),
         "force_rel_rms": float(np.linalg.norm(gpu_forces0 - F) / np.linalg.norm(F)),
-        "pairs_match": int(ref["npairs1"] + ref["npairs2"]),
+        "oracle_pairs_in_cutoff": int(ref["npairs1"] + ref["npairs2"]),
     }
     return {"value": n / dt, "unit": "evals/s", "cores": 1, "kind": "port",
             "sample": f"{n} energy+force evaluations of replica 0 (23,000 atoms, both states) by the "
@@ -74,6 +74,10 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--tuning", default=None,
+                    help="column_scale,pair_waves_per_block,zbins_per_column (skips the autotuner)")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -90,7 +94,12 @@ def main():
     batch = ATMBatch(reps)
     batch.compute()
     tuning = batch.tuning()
-    if not args.no_autotune:
+    if args.tuning:
+        cs, wv, zp = args.tuning.split(",")
+        batch.set_tuning(float(cs), int(wv), int(zp))
+        batch.compute()
+        tuning = batch.tuning()
+    elif not args.no_autotune:
         log("[bench] autotuning")
         tuning = batch.autotune(repeats=5)
         batch.compute()
@@ -112,6 +121,18 @@ def main():
     ms_per_step = dt * 1e3 / args.steps
     evals_per_s = args.replicas * args.steps / dt
 
+    fp32_eps = None
+    if not args.no_fp32:
+        log("[bench] FP32 scalar pair-function mode")
+        for _ in range(args.warmup):
+            batch.evaluate(forces=True, clear=True, fp32_pairs=True)
+        batch.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(args.steps):
+            batch.evaluate(forces=True, clear=True, fp32_pairs=True)
+        batch.synchronize()
+        fp32_eps = args.replicas * args.steps / (time.perf_counter() - t1)
+
     log("[bench] per-stage HIP-event timing")
     prof = batch.profile(reps=max(10, args.steps // 10))
     r0 = batch.result(0)
@@ -129,18 +150,20 @@ def main():
     blend_gbs = BLEND_BYTES_PER_ATOM * total_atoms / (prof["blend_ms"] * 1e-3) / 1e9
     batch.close()
 
-    log("[bench] plain one-system path")
-    plain = ATMBatch([base])
-    for _ in range(3):
-        plain.compute(plain=True)
-    plain.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.plain_steps):
-        plain.evaluate(forces=True, clear=True, plain=True)
-    plain.synchronize()
-    plain_dt = time.perf_counter() - t0
-    plain_eps = args.plain_steps / plain_dt
-    plain.close()
+    plain_eps = None
+    if not args.no_plain:
+        log("[bench] plain one-system path")
+        plain = ATMBatch([base])
+        for _ in range(3):
+            plain.compute(plain=True)
+        plain.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.plain_steps):
+            plain.evaluate(forces=True, clear=True, plain=True)
+        plain.synchronize()
+        plain_dt = time.perf_counter() - t0
+        plain_eps = args.plain_steps / plain_dt
+        plain.close()
 
     cpu = None
     err = None
@@ -152,7 +175,7 @@ def main():
         "metric": METRIC, "value": round(evals_per_s, 2), "unit": "evals/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "fp32", "data": "synthetic",
+        "dtype": "fp32+fp64", "data": "synthetic",
         "config": {"workload": f"{args.replicas}-replica batch of 23,000-atom LJ + reaction-field systems "
                                "(8.8 nm box, rc 1.0 nm, 40-atom ligand, d=(2.2,2.2,2.2) nm), "
                                "one HIP-graph launch per evaluation of all replicas",
@@ -161,13 +184,16 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "pair_interactions_per_s": round(pairs * args.steps / dt, 1),
-        "plain_evals_per_s": round(plain_eps, 2),
-        "speedup_vs_plain": round(evals_per_s / plain_eps, 2),
+        "plain_evals_per_s": round(plain_eps, 2) if plain_eps else None,
+        "speedup_vs_plain": round(evals_per_s / plain_eps, 2) if plain_eps else None,
+        "fp32_pair_mode_evals_per_s": round(fp32_eps, 2) if fp32_eps else None,
         "stage_ms": {k: round(v, 4) for k, v in prof.items() if k.endswith("_ms")},
         "blend_kernel": {"bound": "hbm", "achieved": round(blend_gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                          "frac": round(blend_gbs / PEAK_HBM_GBS, 4)},
         "error_vs_fp64_oracle_replica0": err,
-        "precision": "pair math FP32; U1/U2/u/W FP64; forces 64-bit fixed point (2^-24 kJ/mol/nm)",
+        "precision": "coordinates, displacement, cutoff geometry and force vectors FP32; scalar pair function "
+                     "(r^2, 1/r, (s/r)^6, energies, force magnitude) FP64; U1/U2/u/W FP64; forces 64-bit "
+                     "fixed point (2^-24 kJ/mol/nm)",
     }
     print(json.dumps(out), flush=True)
 

@@ -97,7 +97,7 @@ struct Context {
     int maxAtoms = 0;
     std::vector<SetDesc> sets;
     std::vector<int> entryCapOverride;  // per set, 0 = automatic
-    int maxClusters = 0, maxGroups = 0, maxCells = 0;
+    int maxClusters = 0, maxGroups = 0, maxCells = 0, maxSlots = 0;
     long long totalCells = 0, totalCols = 0, totalSlots = 0, totalClusters = 0, totalGroups = 0, totalEntries = 0;
     Tuning tuning;
     double tunedMs = 0.0;
@@ -212,7 +212,7 @@ void layout(Context& C) {
     C.sets.resize(2 * C.nsys);
     C.entryCapOverride.resize(2 * C.nsys, 0);
     long long cellOff = 0, colOff = 0, slotOff = 0, clOff = 0, gOff = 0, eOff = 0, setAtomOff = 0;
-    C.maxClusters = C.maxGroups = C.maxCells = 0;
+    C.maxClusters = C.maxGroups = C.maxCells = C.maxSlots = 0;
     for (int s = 0; s < C.nsys; ++s) {
         for (int st = 0; st < 2; ++st) {
             const int c = 2 * s + st;
@@ -237,6 +237,7 @@ void layout(Context& C) {
             C.maxClusters = std::max(C.maxClusters, D.clusterCap);
             C.maxGroups = std::max(C.maxGroups, D.groupCap);
             C.maxCells = std::max(C.maxCells, D.nx * D.ny * D.nz);
+            C.maxSlots = std::max(C.maxSlots, D.slotCap);
             C.sets[c] = D;
         }
     }
@@ -340,7 +341,7 @@ void enqueue_batched(Context& C, int flags, hipEvent_t* ev = nullptr) {
     if (ev) HIPCHK(hipEventRecord(ev[0], st));
     if (flags & ATM_EVAL_CLEAR_FORCES) HIPCHK(launch_fill_i64(P.outF, 3 * C.totalAtoms, st));
     HIPCHK(launch_zero_counts(P, 0, (int)C.totalCells, 0, S2, st));
-    HIPCHK(launch_build(P, 0, S2, C.maxAtoms, C.maxCells, C.maxClusters, st));
+    HIPCHK(launch_build(P, 0, S2, C.maxAtoms, C.maxCells, C.maxSlots, st));
     if (ev) HIPCHK(hipEventRecord(ev[1], st));
     HIPCHK(launch_pairlist(P, 0, S2, C.maxGroups, 4, st));
     if (ev) HIPCHK(hipEventRecord(ev[2], st));
@@ -399,7 +400,7 @@ void eval_plain(Context& C, int flags) {
             const int c = 2 * s + stt;
             const SetDesc& D = C.sets[c];
             HIPCHK(launch_zero_counts(P, D.cellOff, D.cellOff + D.nx * D.ny * D.nz, c, c + 1, st));
-            HIPCHK(launch_build(P, c, 1, D.natoms, D.nx * D.ny * D.nz, D.clusterCap, st));
+            HIPCHK(launch_build(P, c, 1, D.natoms, D.nx * D.ny * D.nz, D.slotCap, st));
             HIPCHK(launch_pairlist(P, c, 1, D.groupCap, 4, st));
             HIPCHK(launch_pairs(P, c, 1, D.groupCap, C.tuning.pairWaves, true, (flags & ATM_EVAL_FP32_PAIRS) == 0, st));
             HIPCHK(launch_set_energy(P, c, st));

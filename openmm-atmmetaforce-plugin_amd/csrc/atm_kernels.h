@@ -55,7 +55,7 @@ struct DevPtrs {
 };
 
 hipError_t launch_zero_counts(const DevPtrs& P, int cellBegin, int cellEnd, int setBegin, int setEnd, hipStream_t st);
-hipError_t launch_build(const DevPtrs& P, int setBegin, int nsets, int maxAtoms, int maxCells, int maxClusters,
+hipError_t launch_build(const DevPtrs& P, int setBegin, int nsets, int maxAtoms, int maxCells, int maxSlots,
                         hipStream_t st);
 hipError_t launch_pairlist(const DevPtrs& P, int setBegin, int nsets, int maxGroups, int listWaves, hipStream_t st);
 hipError_t launch_pairs(const DevPtrs& P, int setBegin, int nsets, int maxGroups, int waves, bool forces, bool fp64,

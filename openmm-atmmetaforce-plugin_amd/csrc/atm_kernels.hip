@@ -4,9 +4,10 @@
 //   k_zero_counts  clear fine-cell counters and overflow flags
 //   k_bin          x (+ d for state 1), wrap into the box, fine-cell id and rank
 //   k_scan         per-set column/cell scans, dummy padding, group table
-//   k_scatter      counting-sort scatter into cluster order, clear set forces
-//   k_cellsort     order each fine cell by atom index (deterministic results)
-//   k_bbox         per-cluster bounding boxes
+//   k_scatter_idx  counting-sort scatter of atom indices into cluster order
+//   k_cellsort_idx order each fine cell by atom index (deterministic results)
+//   k_gather_bbox  gather positions/parameters into cluster order, clear the
+//                  set's force accumulators, per-cluster bounding boxes
 //   k_pairlist     per group of 8 clusters: j-clusters whose boxes are within
 //                  the cutoff of at least one i-cluster (half shell, i <= j)
 //   k_pairs        LJ + reaction-field pair kernel, one wave64 per group,
@@ -252,15 +253,7 @@ __global__ void __launch_bounds__(256) k_scan(DevPtrs P, int setBegin) {
         const int ng = (ncl + kGroupClusters - 1) / kGroupClusters;
         P.colStart[D.colOff + col] = offS;
         for (int z = 0; z < nz; ++z) st[z] += offS;
-        for (int s = offS + cnt; s < offS + padded; ++s) {
-            const int k = D.slotOff + s;
-            P.spos[k] = make_float4(0.f, 0.f, 0.f, 0.f);
-            P.sprm[k] = make_float2(0.f, 0.f);
-            P.sprm64[k] = make_double4(0.0, 0.0, 0.0, 0.0);
-            P.sidx[k] = -1;
-            P.sfx[k] = 0; P.sfy[k] = 0; P.sfz[k] = 0;
-            P.sdx[k] = 0.0; P.sdy[k] = 0.0; P.sdz[k] = 0.0;
-        }
+        for (int s = offS + cnt; s < offS + padded; ++s) P.sidx[D.slotOff + s] = -1;
         for (int gg = 0; gg < ng; ++gg)
             P.ginfo[D.groupOff + offG + gg] =
                 make_int4((offS >> 3) + gg * kGroupClusters, min(kGroupClusters, ncl - gg * kGroupClusters), col, 0);
@@ -274,88 +267,116 @@ __global__ void __launch_bounds__(256) k_scan(DevPtrs P, int setBegin) {
     }
 }
 
-__global__ void k_scatter(DevPtrs P, int setBegin) {
+// Counting-sort scatter of atom indices only (4 bytes per atom).
+__global__ void k_scatter_idx(DevPtrs P, int setBegin) {
     const int c = setBegin + blockIdx.y;
     const SetDesc& D = P.desc[c];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= D.natoms) return;
     const int k = D.setAtomOff + i;
     const int slot = P.cellStart[D.cellOff + P.bcell[k]] + P.brank[k];
-    const int s = D.slotOff + slot;
-    P.spos[s] = P.bpos[k];
-    P.sprm[s] = P.ljp[D.atomOff + i];
-    P.sprm64[s] = P.prm64[D.atomOff + i];
-    P.sidx[s] = i;
-    P.sfx[s] = 0; P.sfy[s] = 0; P.sfz[s] = 0;
-    P.sdx[s] = 0.0; P.sdy[s] = 0.0; P.sdz[s] = 0.0;
-    P.slotOf[k] = slot;
+    P.sidx[D.slotOff + slot] = i;
 }
 
-// Order each fine cell's slots by original atom index (the atomic ranks of
-// k_bin are not deterministic); makes cluster composition, FP32 partial sums
-// and therefore every result of an evaluation bit-reproducible.
-__global__ void k_cellsort(DevPtrs P, int setBegin) {
+// Order each fine cell's atom indices ascending (the atomic ranks of k_bin are
+// not deterministic).  Fixed cluster composition makes the FP32 partial sums,
+// and so every result of an evaluation, bit-reproducible.
+__global__ void k_cellsort_idx(DevPtrs P, int setBegin) {
     const int c = setBegin + blockIdx.y;
     const SetDesc& D = P.desc[c];
     const int cell = blockIdx.x * blockDim.x + threadIdx.x;
     if (cell >= D.nx * D.ny * D.nz) return;
     const int cnt = P.cellCount[D.cellOff + cell];
-    if (cnt < 1) return;
-    const int start = P.cellStart[D.cellOff + cell];
-    const int base = D.slotOff + start;
+    if (cnt < 2) return;
+    int* idx = P.sidx + D.slotOff + P.cellStart[D.cellOff + cell];
     for (int a = 1; a < cnt; ++a) {
-        const int key = P.sidx[base + a];
-        const float4 kp = P.spos[base + a];
-        const float2 kq = P.sprm[base + a];
-        const double4 kd = P.sprm64[base + a];
+        const int key = idx[a];
         int b = a - 1;
-        while (b >= 0 && P.sidx[base + b] > key) {
-            P.sidx[base + b + 1] = P.sidx[base + b];
-            P.spos[base + b + 1] = P.spos[base + b];
-            P.sprm[base + b + 1] = P.sprm[base + b];
-            P.sprm64[base + b + 1] = P.sprm64[base + b];
+        while (b >= 0 && idx[b] > key) {
+            idx[b + 1] = idx[b];
             --b;
         }
-        P.sidx[base + b + 1] = key;
-        P.spos[base + b + 1] = kp;
-        P.sprm[base + b + 1] = kq;
-        P.sprm64[base + b + 1] = kd;
+        idx[b + 1] = key;
     }
-    for (int a = 0; a < cnt; ++a) P.slotOf[D.setAtomOff + P.sidx[base + a]] = start + a;
 }
 
-__global__ void k_bbox(DevPtrs P, int setBegin) {
+__device__ __forceinline__ float oct_min(float v) {
+    v = fminf(v, __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0xB1, 0xF, 0xF, true)));
+    v = fminf(v, __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x4E, 0xF, 0xF, true)));
+    v = fminf(v, __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x141, 0xF, 0xF, true)));
+    return v;
+}
+
+__device__ __forceinline__ float oct_max(float v) {
+    v = fmaxf(v, __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0xB1, 0xF, 0xF, true)));
+    v = fmaxf(v, __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x4E, 0xF, 0xF, true)));
+    v = fmaxf(v, __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x141, 0xF, 0xF, true)));
+    return v;
+}
+
+// One thread per slot: gather the atom's wrapped position and parameters into
+// cluster order (coalesced writes), clear the slot's force accumulators, and
+// reduce each cluster's bounding box over its 8 lanes.  Slot counts are
+// multiples of 8, so a cluster's lanes are all active or all gone.
+__global__ void __launch_bounds__(256) k_gather_bbox(DevPtrs P, int setBegin) {
     const int c = setBegin + blockIdx.y;
     const SetDesc& D = P.desc[c];
-    const int cl = blockIdx.x * blockDim.x + threadIdx.x;
-    if (cl >= P.rt[c].nclusters) return;
-    float lo[3] = {3.0e38f, 3.0e38f, 3.0e38f}, hi[3] = {-3.0e38f, -3.0e38f, -3.0e38f};
-    const int s0 = D.slotOff + cl * kClusterSize;
-    for (int a = 0; a < kClusterSize; ++a) {
-        if (P.sidx[s0 + a] < 0) continue;
-        float4 p = P.spos[s0 + a];
-        lo[0] = fminf(lo[0], p.x); hi[0] = fmaxf(hi[0], p.x);
-        lo[1] = fminf(lo[1], p.y); hi[1] = fmaxf(hi[1], p.y);
-        lo[2] = fminf(lo[2], p.z); hi[2] = fmaxf(hi[2], p.z);
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= P.rt[c].nslots) return;
+    const int k = D.slotOff + s;
+    const int i = P.sidx[k];
+    float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
+    float2 q = make_float2(0.f, 0.f);
+    double4 qd = make_double4(0.0, 0.0, 0.0, 0.0);
+    if (i >= 0) {
+        p = P.bpos[D.setAtomOff + i];
+        q = P.ljp[D.atomOff + i];
+        qd = P.prm64[D.atomOff + i];
+        P.slotOf[D.setAtomOff + i] = s;
     }
-    // every cluster holds at least one real atom (padding < 8 per column)
-    P.bbc[D.clusterOff + cl] = make_float4(0.5f * (lo[0] + hi[0]), 0.5f * (lo[1] + hi[1]), 0.5f * (lo[2] + hi[2]), 0.f);
-    P.bbh[D.clusterOff + cl] = make_float4(0.5f * (hi[0] - lo[0]), 0.5f * (hi[1] - lo[1]), 0.5f * (hi[2] - lo[2]), 0.f);
+    P.spos[k] = p;
+    P.sprm[k] = q;
+    P.sprm64[k] = qd;
+    P.sfx[k] = 0; P.sfy[k] = 0; P.sfz[k] = 0;
+    P.sdx[k] = 0.0; P.sdy[k] = 0.0; P.sdz[k] = 0.0;
+    const float inf = __builtin_huge_valf();
+    const float lox = oct_min(i >= 0 ? p.x : inf), hix = oct_max(i >= 0 ? p.x : -inf);
+    const float loy = oct_min(i >= 0 ? p.y : inf), hiy = oct_max(i >= 0 ? p.y : -inf);
+    const float loz = oct_min(i >= 0 ? p.z : inf), hiz = oct_max(i >= 0 ? p.z : -inf);
+    if ((s & 7) == 0) {
+        // every cluster holds at least one real atom (padding < 8 per column)
+        const int cl = D.clusterOff + (s >> 3);
+        P.bbc[cl] = make_float4(0.5f * (lox + hix), 0.5f * (loy + hiy), 0.5f * (loz + hiz), 0.f);
+        P.bbh[cl] = make_float4(0.5f * (hix - lox), 0.5f * (hiy - loy), 0.5f * (hiz - loz), 0.f);
+    }
 }
 
 // ---------------------------------------------------------------- pair list
 
+__device__ __forceinline__ int wave_incl_scan(int v) {
+    const int lane = threadIdx.x & 63;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int u = __shfl_up(v, o, 64);
+        if (lane >= o) v += u;
+    }
+    return v;
+}
+
 // One wave64 per group.  Entry = J (20 bits) | imask (8 bits) << 20 |
 // x-shift code << 28 | y-shift code << 30 (code 1: +L, 2: -L).
+// Candidate columns (up to 64 per pass, one per lane) are enumerated first; a
+// wave prefix sum then flattens their (column, cluster) candidates so that
+// all 64 lanes test clusters, whatever the column occupancy.
 template <int W>
 __global__ void __launch_bounds__(64 * W) k_pairlist(DevPtrs P, int setBegin) {
     const int c = setBegin + blockIdx.y;
-    const SetDesc& D = P.desc[c];
+    const SetDesc D = P.desc[c];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int g = blockIdx.x * W + wave;
     if (g >= P.rt[c].ngroups) return;
     const int4 gi = P.ginfo[D.groupOff + g];
-    const int g0 = gi.x, ncl = gi.y, col = gi.z;
+    const int g0 = __builtin_amdgcn_readfirstlane(gi.x), ncl = __builtin_amdgcn_readfirstlane(gi.y);
+    const int col = __builtin_amdgcn_readfirstlane(gi.z);
     const int gx = col / D.ny, gy = col % D.ny;
     float4 myc = make_float4(0.f, 0.f, 0.f, 0.f), myh = myc;
     if (lane < ncl) { myc = P.bbc[D.clusterOff + g0 + lane]; myh = P.bbh[D.clusterOff + g0 + lane]; }
@@ -368,57 +389,75 @@ __global__ void __launch_bounds__(64 * W) k_pairlist(DevPtrs P, int setBegin) {
     }
     uint32_t* out = P.entries + D.entryOff + (long long)g * D.entryCap;
     int nEnt = 0;
-    const int xb = D.fullX ? 0 : -D.Rx, xe = D.fullX ? D.nx - 1 : D.Rx;
-    const int yb = D.fullY ? 0 : -D.Ry, ye = D.fullY ? D.ny - 1 : D.Ry;
-    for (int ox = xb; ox <= xe; ++ox) {
-        int jx = ox, sxc = 0;
-        float gapx = 0.f;
-        if (!D.fullX) {
-            jx = gx + ox;
-            if (jx < 0) { jx += D.nx; sxc = 2; } else if (jx >= D.nx) { jx -= D.nx; sxc = 1; }
-            gapx = fmaxf(0.f, (float)(abs(ox) - 1)) * D.ce[0];
-        }
-        for (int oy = yb; oy <= ye; ++oy) {
-            int jy = oy, syc = 0;
-            float gapy = 0.f;
+    const int ncx = D.fullX ? D.nx : 2 * D.Rx + 1, ncy = D.fullY ? D.ny : 2 * D.Ry + 1;
+    const int ncand = ncx * ncy;
+    for (int cb = 0; cb < ncand; cb += 64) {
+        // this lane's candidate column: first cluster, cluster count, shift codes
+        int cnt_l = 0, c0_l = 0, codes_l = 0;
+        const int ci = cb + lane;
+        if (ci < ncand) {
+            const int a = ci / ncy, b = ci % ncy;
+            int jx = a, jy = b, sxc = 0, syc = 0;
+            float gapx = 0.f, gapy = 0.f;
+            if (!D.fullX) {
+                const int ox = a - D.Rx;
+                jx = gx + ox;
+                if (jx < 0) { jx += D.nx; sxc = 2; } else if (jx >= D.nx) { jx -= D.nx; sxc = 1; }
+                gapx = fmaxf(0.f, (float)(abs(ox) - 1)) * D.ce[0];
+            }
             if (!D.fullY) {
+                const int oy = b - D.Ry;
                 jy = gy + oy;
                 if (jy < 0) { jy += D.ny; syc = 2; } else if (jy >= D.ny) { jy -= D.ny; syc = 1; }
                 gapy = fmaxf(0.f, (float)(abs(oy) - 1)) * D.ce[1];
             }
-            if (gapx * gapx + gapy * gapy >= D.rcList2) continue;
-            const int colJ = jx * D.ny + jy;
-            const int cs = P.colStart[D.colOff + colJ], cnt = P.colCount[D.colOff + colJ];
-            const int c0 = cs >> 3, c1 = (cs + ((cnt + 7) & ~7)) >> 3;
-            if (c1 <= g0) continue;
-            const float sx = D.shiftOK ? (sxc == 1 ? D.L[0] : (sxc == 2 ? -D.L[0] : 0.f)) : 0.f;
-            const float sy = D.shiftOK ? (syc == 1 ? D.L[1] : (syc == 2 ? -D.L[1] : 0.f)) : 0.f;
-            for (int J0 = max(c0, g0); J0 < c1; J0 += 64) {
-                const int J = J0 + lane;
-                uint32_t mask = 0;
-                if (J < c1) {
-                    const float4 jc = P.bbc[D.clusterOff + J], jh = P.bbh[D.clusterOff + J];
+            if (gapx * gapx + gapy * gapy < D.rcList2) {
+                const int colJ = jx * D.ny + jy;
+                const int cs = P.colStart[D.colOff + colJ], cnt = P.colCount[D.colOff + colJ];
+                const int cl0 = max(cs >> 3, g0), cl1 = (cs + ((cnt + 7) & ~7)) >> 3;
+                if (cl1 > cl0) { cnt_l = cl1 - cl0; c0_l = cl0; codes_l = sxc | (syc << 2); }
+            }
+        }
+        const int incl = wave_incl_scan(cnt_l);
+        const int excl = incl - cnt_l;
+        const int total = __shfl(incl, 63, 64);
+        for (int f0 = 0; f0 < total; f0 += 64) {
+            const int f = f0 + lane;
+            // the candidate column holding flat index f: the last lane with excl <= f
+            int lo = 0;
+            for (int step = 32; step > 0; step >>= 1) {
+                const int probe = min(lo + step, 63);
+                const int ex = __shfl(excl, probe, 64);
+                if (lo + step <= 63 && ex <= f) lo = probe;
+            }
+            const int J = __shfl(c0_l, lo, 64) + (f - __shfl(excl, lo, 64));
+            const int codes = __shfl(codes_l, lo, 64);
+            uint32_t mask = 0;
+            if (f < total) {
+                const int sxc = codes & 3, syc = codes >> 2;
+                const float sx = D.shiftOK ? (sxc == 1 ? D.L[0] : (sxc == 2 ? -D.L[0] : 0.f)) : 0.f;
+                const float sy = D.shiftOK ? (syc == 1 ? D.L[1] : (syc == 2 ? -D.L[1] : 0.f)) : 0.f;
+                const float4 jc = P.bbc[D.clusterOff + J], jh = P.bbh[D.clusterOff + J];
 #pragma unroll
-                    for (int q = 0; q < kGroupClusters; ++q) {
-                        if (q < ncl && g0 + q <= J) {
-                            float dx = jc.x + sx - icx[q], dy = jc.y + sy - icy[q], dz = jc.z - icz[q];
-                            if (!D.shiftOK) { dx = min_image(dx, D.L[0], D.invL[0]); dy = min_image(dy, D.L[1], D.invL[1]); }
-                            dz = min_image(dz, D.L[2], D.invL[2]);
-                            float ax = fmaxf(0.f, fabsf(dx) - ihx[q] - jh.x);
-                            float ay = fmaxf(0.f, fabsf(dy) - ihy[q] - jh.y);
-                            float az = fmaxf(0.f, fabsf(dz) - ihz[q] - jh.z);
-                            if (ax * ax + ay * ay + az * az < D.rcList2) mask |= 1u << q;
-                        }
+                for (int q = 0; q < kGroupClusters; ++q) {
+                    if (q < ncl && g0 + q <= J) {
+                        float dx = jc.x + sx - icx[q], dy = jc.y + sy - icy[q], dz = jc.z - icz[q];
+                        if (!D.shiftOK) { dx = min_image(dx, D.L[0], D.invL[0]); dy = min_image(dy, D.L[1], D.invL[1]); }
+                        dz = min_image(dz, D.L[2], D.invL[2]);
+                        const float ax = fmaxf(0.f, fabsf(dx) - ihx[q] - jh.x);
+                        const float ay = fmaxf(0.f, fabsf(dy) - ihy[q] - jh.y);
+                        const float az = fmaxf(0.f, fabsf(dz) - ihz[q] - jh.z);
+                        if (ax * ax + ay * ay + az * az < D.rcList2) mask |= 1u << q;
                     }
                 }
-                const uint64_t bal = __ballot(mask != 0);
-                const int pos = nEnt + __popcll(bal & lanemask_lt());
-                if (mask != 0 && pos < D.entryCap) {
-                    uint32_t codes = D.shiftOK ? ((uint32_t)sxc << 28) | ((uint32_t)syc << 30) : 0u;
-                    out[pos] = (uint32_t)J | (mask << 20) | codes;
-                }
-                nEnt += __popcll(bal);
             }
+            const uint64_t bal = __ballot(mask != 0);
+            const int pos = nEnt + __popcll(bal & lanemask_lt());
+            if (mask != 0 && pos < D.entryCap) {
+                const uint32_t sh = D.shiftOK ? ((uint32_t)(codes & 3) << 28) | ((uint32_t)(codes >> 2) << 30) : 0u;
+                out[pos] = (uint32_t)J | (mask << 20) | sh;
+            }
+            nEnt += __popcll(bal);
         }
     }
     if (lane == 0) {
@@ -452,15 +491,29 @@ __device__ __forceinline__ JAtom load_j(const DevPtrs& P, const SetDesc& D, uint
     return a;
 }
 
+// Flush one partial force vector to a slot.  The common case (all three
+// components below kFixedMax) takes the branch-free fixed-point path.
+__device__ __forceinline__ void flush_force(const DevPtrs& P, int slot, float fx, float fy, float fz) {
+    if (fmaxf(fabsf(fx), fmaxf(fabsf(fy), fabsf(fz))) < 134217728.0f) {  // 2^27: |f scale| < 2^51
+        atomic_add_fixed(P.sfx + slot, __double_as_longlong((double)fx * kForceScale + 6755399441055744.0) - 0x4338000000000000LL);
+        atomic_add_fixed(P.sfy + slot, __double_as_longlong((double)fy * kForceScale + 6755399441055744.0) - 0x4338000000000000LL);
+        atomic_add_fixed(P.sfz + slot, __double_as_longlong((double)fz * kForceScale + 6755399441055744.0) - 0x4338000000000000LL);
+    } else {
+        add_force(P, slot, fx, fy, fz);
+    }
+}
+
 // One wave64 per group of up to 8 i-clusters; lane = 8 jj + ii pairs i-atom ii
 // of each interacting i-cluster with j-atom jj of the list entry's j-cluster.
 // F64: scalar pair function in FP64 (exact r^2, default); otherwise FP32.
 // Coordinates, force vectors and their per-lane accumulators are FP32 either
 // way; forces are flushed to 64-bit fixed point, energies summed in FP64.
+// List entries are fetched 64 at a time (one per lane) and broadcast with
+// readlane, so the cluster mask and the diagonal test are scalar branches.
 template <int W, bool FORCES, bool F64>
 __global__ void __launch_bounds__(64 * W) k_pairs(DevPtrs P, int setBegin) {
     const int c = setBegin + blockIdx.y;
-    const SetDesc& D = P.desc[c];
+    const SetDesc D = P.desc[c];  // by value: lives in SGPRs for the whole kernel
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int jj = lane >> 3, ii = lane & 7;
     const int g = blockIdx.x * W + wave;
@@ -472,7 +525,8 @@ __global__ void __launch_bounds__(64 * W) k_pairs(DevPtrs P, int setBegin) {
     uint32_t ivm = 0;  // bit t: i-atom (t, ii) is a real atom
     if (active) {
         const int4 gi = P.ginfo[D.groupOff + g];
-        g0 = gi.x; ncl = gi.y;
+        g0 = __builtin_amdgcn_readfirstlane(gi.x);
+        ncl = __builtin_amdgcn_readfirstlane(gi.y);
         float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
         float2 q = make_float2(0.f, 0.f);
         double4 qd = make_double4(0.0, 0.0, 0.0, 0.0);
@@ -495,57 +549,71 @@ __global__ void __launch_bounds__(64 * W) k_pairs(DevPtrs P, int setBegin) {
     const float Lz = D.L[2], invLz = D.invL[2];
     const float Lx = D.L[0], invLx = D.invL[0], Ly = D.L[1], invLy = D.invL[1];
     const bool shiftOK = D.shiftOK;
-    const double rc2 = D.rc2;
+    const double rc2 = D.rc2, krf64 = D.krf64;
+    const float krf = D.krf;
     float fi[kGroupClusters][3];
 #pragma unroll
     for (int t = 0; t < kGroupClusters; ++t) fi[t][0] = fi[t][1] = fi[t][2] = 0.f;
     double eE = 0.0, eQQ = 0.0, eQR2 = 0.0;  // sum(E_LJ + qq/r), sum(qq), sum(qq r^2)
-    const int ne = P.gcount[D.groupOff + g];
+    const int ne = __builtin_amdgcn_readfirstlane(P.gcount[D.groupOff + g]);
     const uint32_t* E = P.entries + D.entryOff + (long long)g * D.entryCap;
-    for (int e = 0; e < ne; ++e) {
-        const uint32_t ent = E[e];
-        const int J = ent & 0xFFFFF;
-        const uint32_t imask = (ent >> 20) & 0xFFu;
-        const JAtom ja = load_j(P, D, ent, jj);
-        double4 jd = make_double4(0.0, 0.0, 0.0, 0.0);
-        if (F64) jd = P.sprm64[ja.slot];
-        float fjx = 0.f, fjy = 0.f, fjz = 0.f;
+    for (int e0 = 0; e0 < ne; e0 += 64) {
+        const uint32_t ebuf = (e0 + lane < ne) ? E[e0 + lane] : 0u;
+        const int nb = min(64, ne - e0);
+        for (int k = 0; k < nb; ++k) {
+            const uint32_t ent = __builtin_amdgcn_readlane(ebuf, k);
+            const int J = ent & 0xFFFFF;
+            const uint32_t imask = (ent >> 20) & 0xFFu;
+            const int js = D.slotOff + J * kClusterSize + jj;
+            float4 pj = P.spos[js];
+            const bool jv = P.sidx[js] >= 0;
+            float2 qj = make_float2(0.f, 0.f);
+            double4 jd = make_double4(0.0, 0.0, 0.0, 0.0);
+            if (F64) jd = P.sprm64[js]; else qj = P.sprm[js];
+            if (shiftOK) {
+                const uint32_t sxc = (ent >> 28) & 3u, syc = (ent >> 30) & 3u;
+                pj.x += sxc == 1u ? D.L[0] : (sxc == 2u ? -D.L[0] : 0.f);
+                pj.y += syc == 1u ? D.L[1] : (syc == 2u ? -D.L[1] : 0.f);
+            }
+            const int tdiag = J - g0;  // the i-cluster equal to J, if any
+            float fjx = 0.f, fjy = 0.f, fjz = 0.f;
 #pragma unroll
-        for (int t = 0; t < kGroupClusters; ++t) {
-            if (imask & (1u << t)) {
-                const float4 pi = sPos[wave][t * kClusterSize + ii];
-                float dx = pi.x - ja.p.x, dy = pi.y - ja.p.y, dz = pi.z - ja.p.z;
-                if (!shiftOK) { dx = min_image(dx, Lx, invLx); dy = min_image(dy, Ly, invLy); }
-                dz = min_image(dz, Lz, invLz);
-                float fs;
-                if (F64) {
-                    const double4 id = sPrm64[wave][t * kClusterSize + ii];
-                    // exact: the squares of 2^-20-grid differences fit in 53 bits
-                    const double r2 = fma((double)dx, (double)dx, fma((double)dy, (double)dy, (double)dz * (double)dz));
-                    bool in = r2 < rc2 && ja.valid && ((ivm >> t) & 1u);
-                    if (g0 + t == J) in = in && (ii < jj);
-                    const PairE64 pe = pair_term64(in, r2, id.x + jd.x, id.y * jd.y, id.z * jd.z, D.krf64);
-                    eE += pe.e; eQQ += pe.qq; eQR2 += pe.qq * r2;
-                    fs = (float)pe.fs;
-                } else {
-                    const float2 qi = sPrm[wave][t * kClusterSize + ii];
-                    const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-                    bool in = in_cutoff(D, dx, dy, dz, r2) && ja.valid && ((ivm >> t) & 1u);
-                    if (g0 + t == J) in = in && (ii < jj);
-                    const PairE pe = pair_term(in, r2, pi.w * ja.p.w, qi.x + ja.prm.x, qi.y * ja.prm.y, D.krf);
-                    eE += (double)(pe.elj + pe.qr); eQQ += (double)pe.qq; eQR2 += (double)pe.qr2;
-                    fs = pe.fs;
-                }
-                if (FORCES) {
-                    const float fx = fs * dx, fy = fs * dy, fz = fs * dz;
-                    fi[t][0] += fx; fi[t][1] += fy; fi[t][2] += fz;
-                    fjx -= fx; fjy -= fy; fjz -= fz;
+            for (int t = 0; t < kGroupClusters; ++t) {
+                if (imask & (1u << t)) {
+                    const float4 pi = sPos[wave][t * kClusterSize + ii];
+                    float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+                    if (!shiftOK) { dx = min_image(dx, Lx, invLx); dy = min_image(dy, Ly, invLy); }
+                    dz = min_image(dz, Lz, invLz);
+                    bool ok = jv && ((ivm >> t) & 1u);
+                    if (t == tdiag) ok = ok && (ii < jj);
+                    float fs;
+                    if (F64) {
+                        const double4 id = sPrm64[wave][t * kClusterSize + ii];
+                        // exact: the squares of 2^-20-grid differences fit in 53 bits
+                        const double r2 = fma((double)dx, (double)dx, fma((double)dy, (double)dy, (double)dz * (double)dz));
+                        const bool in = ok && r2 < rc2;
+                        const PairE64 pe = pair_term64(in, r2, id.x + jd.x, id.y * jd.y, id.z * jd.z, krf64);
+                        eE += pe.e; eQQ += pe.qq; eQR2 += pe.qq * r2;
+                        fs = (float)pe.fs;
+                    } else {
+                        const float2 qi = sPrm[wave][t * kClusterSize + ii];
+                        const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                        const bool in = ok && in_cutoff(D, dx, dy, dz, r2);
+                        const PairE pe = pair_term(in, r2, pi.w * pj.w, qi.x + qj.x, qi.y * qj.y, krf);
+                        eE += (double)(pe.elj + pe.qr); eQQ += (double)pe.qq; eQR2 += (double)pe.qr2;
+                        fs = pe.fs;
+                    }
+                    if (FORCES) {
+                        const float fx = fs * dx, fy = fs * dy, fz = fs * dz;
+                        fi[t][0] += fx; fi[t][1] += fy; fi[t][2] += fz;
+                        fjx -= fx; fjy -= fy; fjz -= fz;
+                    }
                 }
             }
-        }
-        if (FORCES) {
-            fjx = oct_sum(fjx); fjy = oct_sum(fjy); fjz = oct_sum(fjz);
-            if (ii == 0 && ja.valid) add_force(P, ja.slot, fjx, fjy, fjz);
+            if (FORCES) {
+                fjx = oct_sum(fjx); fjy = oct_sum(fjy); fjz = oct_sum(fjz);
+                if (ii == 0 && jv) flush_force(P, js, fjx, fjy, fjz);
+            }
         }
     }
     if (FORCES) {
@@ -553,7 +621,7 @@ __global__ void __launch_bounds__(64 * W) k_pairs(DevPtrs P, int setBegin) {
         for (int t = 0; t < kGroupClusters; ++t) {
             if (t < ncl) {
                 const float fx = stride8_sum(fi[t][0]), fy = stride8_sum(fi[t][1]), fz = stride8_sum(fi[t][2]);
-                if (jj == 0 && ((ivm >> t) & 1u)) add_force(P, D.slotOff + (g0 + t) * kClusterSize + ii, fx, fy, fz);
+                if (jj == 0 && ((ivm >> t) & 1u)) flush_force(P, D.slotOff + (g0 + t) * kClusterSize + ii, fx, fy, fz);
             }
         }
     }
@@ -693,15 +761,14 @@ hipError_t launch_zero_counts(const DevPtrs& P, int cellBegin, int cellEnd, int 
     return hipGetLastError();
 }
 
-hipError_t launch_build(const DevPtrs& P, int setBegin, int nsets, int maxAtoms, int maxCells, int maxClusters,
+hipError_t launch_build(const DevPtrs& P, int setBegin, int nsets, int maxAtoms, int maxCells, int maxSlots,
                         hipStream_t st) {
     dim3 ga(cdiv(maxAtoms, 256), nsets);
     k_bin<<<ga, 256, 0, st>>>(P, setBegin);
     k_scan<<<nsets, 256, 0, st>>>(P, setBegin);
-    k_scatter<<<ga, 256, 0, st>>>(P, setBegin);
-    k_cellsort<<<dim3(cdiv(maxCells, 256), nsets), 256, 0, st>>>(P, setBegin);
-    dim3 gb(cdiv(maxClusters, 256), nsets);
-    k_bbox<<<gb, 256, 0, st>>>(P, setBegin);
+    k_scatter_idx<<<ga, 256, 0, st>>>(P, setBegin);
+    k_cellsort_idx<<<dim3(cdiv(maxCells, 256), nsets), 256, 0, st>>>(P, setBegin);
+    k_gather_bbox<<<dim3(cdiv(maxSlots, 256), nsets), 256, 0, st>>>(P, setBegin);
     return hipGetLastError();
 }
 
