@@ -69,6 +69,9 @@ extern "C" {
 #define ATM_EVAL_FP32_PAIRS    0x10 /* scalar pair function in FP32 (default: FP64 scalars
                                        from the exact FP64 r^2; coordinates, force vectors
                                        and their accumulation are FP32 in both modes) */
+#define ATM_EVAL_STATE1_ONLY   0x20 /* plain cutoff LJ + reaction-field evaluation at x:
+                                       energy = state1_energy = U1, forces += F1; no ATM
+                                       combine (a NonbondedForce outside the ATM force) */
 
 /* One system of the batch.  Box edges must satisfy cutoff <= box/2. */
 typedef struct atm_system_desc {

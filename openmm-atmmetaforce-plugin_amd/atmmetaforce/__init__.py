@@ -1,13 +1,34 @@
 """atmmetaforce -- MI355X-native ATMMetaForce.
 
 Mirrors the Python surface of the reference's SWIG module `atmmetaforce`
-(python/atmmetaforceplugin.i): ATMMetaForce, ATMMETAFORCE_VERSION.  The batched
-evaluation API (ATMBatch) and the autograd entry point (ATMEnergyFunction) have
-no reference counterpart.
+(python/atmmetaforceplugin.i): ATMMetaForce (constructor, particles, the nine
+global-parameter names as static methods, default-value getters,
+getPerturbationEnergy, updateParametersInContext, cast/isinstance) and
+ATMMETAFORCE_VERSION.  Values are plain floats in OpenMM's units (nm, kJ/mol);
+openmm.unit is not installed, so no Quantity wrapping is applied.
+
+`atmmetaforce.openmm` holds the minimal OpenMM-compatible classes the force is
+used with (System, NonbondedForce, Context, State, Platform, VerletIntegrator,
+XmlSerializer); the only platform is "MI355X".
+
+Beyond the reference: ATMBatch (many systems/replicas per launch through the C
+ABI), ATMEnergyFunction (torch.autograd), and the synthetic workloads.
 """
 ATMMETAFORCE_VERSION = "0.3.1"
 
 from . import workloads  # noqa: E402
 from .batch import ATMBatch  # noqa: E402
 
-__all__ = ["ATMMETAFORCE_VERSION", "ATMBatch", "workloads"]
+try:
+    from ._atmmetaforce import ATMMetaForce, OpenMMException  # noqa: E402,F401
+    from . import openmm  # noqa: E402,F401
+except ImportError as _e:  # the C++ layer is not built: fail when it is used
+    _import_error = _e
+
+    def __getattr__(name):
+        if name in ("ATMMetaForce", "OpenMMException", "openmm"):
+            raise ImportError(f"atmmetaforce C++ module not built ({_import_error}); "
+                              "run __graft_entry__.build()")
+        raise AttributeError(name)
+
+__all__ = ["ATMMETAFORCE_VERSION", "ATMMetaForce", "ATMBatch", "OpenMMException", "workloads"]

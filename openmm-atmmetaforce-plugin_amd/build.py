@@ -62,12 +62,64 @@ def build_hip_library(jobs: int = 4) -> str:
     return out
 
 
+CXX = os.environ.get("CXX", "g++")
+CXX_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-parameter", "-fvisibility=hidden"]
+CORE_INC = ["-I" + os.path.join(PKG, "openmm_min", "include")]
+ATM_INC = CORE_INC + ["-I" + os.path.join(PKG, d) for d in
+                      ("openmmapi/include", "serialization/include", "platforms/mi355x/include")]
+
+
+def _sources(*dirs):
+    out = []
+    for d in dirs:
+        full = os.path.join(PKG, d)
+        out += sorted(os.path.join(full, f) for f in os.listdir(full) if f.endswith(".cpp"))
+    return out
+
+
+def build_cpp_layer(jobs: int = 4) -> None:
+    """libOpenMMMin.so (minimal OpenMM-compatible core) and libATMMetaForcePlugin.so
+    (ATMMetaForce API, XML proxy, MI355X platform) -- host C++ built with g++."""
+    os.makedirs(BUILD, exist_ok=True)
+
+    def compile_one(args):
+        src, inc = args
+        obj = os.path.join(BUILD, os.path.basename(os.path.dirname(os.path.dirname(src))) + "_" +
+                           os.path.splitext(os.path.basename(src))[0] + ".o")
+        _run([CXX] + CXX_FLAGS + inc + ["-c", src, "-o", obj])
+        return obj
+
+    core_src = _sources("openmm_min/src")
+    atm_src = _sources("openmmapi/src", "serialization/src", "platforms/mi355x/src")
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        core_obj = list(ex.map(compile_one, [(s, CORE_INC) for s in core_src]))
+        atm_obj = list(ex.map(compile_one, [(s, ATM_INC) for s in atm_src]))
+    core = os.path.join(LIB, "libOpenMMMin.so")
+    _run([CXX, "-shared", "-o", core + ".tmp"] + core_obj + ["-ldl"])
+    os.replace(core + ".tmp", core)
+    plugin = os.path.join(LIB, "libATMMetaForcePlugin.so")
+    _run([CXX, "-shared", "-o", plugin + ".tmp"] + atm_obj +
+         ["-L" + LIB, "-lOpenMMMin", "-latm_mi355x", "-Wl,-rpath,$ORIGIN"])
+    os.replace(plugin + ".tmp", plugin)
+
+
+def build_python_module() -> str:
+    """pybind11 module atmmetaforce/_atmmetaforce*.so (links the two libraries above)."""
+    import pybind11
+    suffix = sysconfig.get_config_var("EXT_SUFFIX")
+    out = os.path.join(PKG, "atmmetaforce", "_atmmetaforce" + suffix)
+    inc = ATM_INC + ["-I" + pybind11.get_include(), "-I" + sysconfig.get_paths()["include"]]
+    src = os.path.join(PKG, "python", "atmmetaforce_module.cpp")
+    _run([CXX] + CXX_FLAGS + inc + ["-shared", src, "-o", out + ".tmp", "-L" + LIB, "-lATMMetaForcePlugin",
+                                    "-lOpenMMMin", "-latm_mi355x", "-Wl,-rpath,$ORIGIN/../lib"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_all(jobs: int = 4) -> None:
     build_hip_library(jobs)
-    try:
-        import plugin_build  # noqa: F401  (C++ API layer, optional until present)
-    except ImportError:
-        pass
+    build_cpp_layer(jobs)
+    build_python_module()
 
 
 if __name__ == "__main__":

@@ -418,6 +418,33 @@ void eval_plain(Context& C, int flags) {
     }
 }
 
+// State 1 only (a plain cutoff LJ + reaction-field evaluation at x): per
+// system, the state-1 set's launch sequence, its energy read back to the host,
+// F += F1.  Serves NonbondedForce evaluations outside the ATM force.
+void eval_state1(Context& C, int flags) {
+    const hipStream_t st = C.stream;
+    const DevPtrs& P = C.P;
+    for (int s = 0; s < C.nsys; ++s) {
+        const SetDesc& D = C.sets[2 * s];
+        const int c = 2 * s;
+        if (flags & ATM_EVAL_CLEAR_FORCES) HIPCHK(launch_fill_i64(P.outF + 3ll * D.atomOff, 3ll * D.natoms, st));
+        HIPCHK(launch_zero_counts(P, D.cellOff, D.cellOff + D.nx * D.ny * D.nz, c, c + 1, st));
+        HIPCHK(launch_build(P, c, 1, D.natoms, D.nx * D.ny * D.nz, D.slotCap, st));
+        HIPCHK(launch_pairlist(P, c, 1, D.groupCap, 4, st));
+        HIPCHK(launch_pairs(P, c, 1, D.groupCap, C.tuning.pairWaves, (flags & ATM_EVAL_FORCES) != 0,
+                            (flags & ATM_EVAL_FP32_PAIRS) == 0, st));
+        HIPCHK(launch_set_energy(P, c, st));
+        double U = 0.0;
+        HIPCHK(hipMemcpyAsync(&U, P.setEnergy + c, sizeof(double), hipMemcpyDeviceToHost, st));
+        if (flags & ATM_EVAL_FORCES) HIPCHK(launch_blend(P, s, 1, D.natoms, 2, 0.0, 1.0, st));
+        HIPCHK(hipStreamSynchronize(st));
+        atm_result& o = C.results[s];
+        o = atm_result{};
+        o.energy = U;
+        o.state1_energy = U;
+    }
+}
+
 void fetch_results(Context& C) {
     if (!C.evaluated) fail(ATM_ERR_STATE, "no evaluation has been run");
     HIPCHK(hipStreamSynchronize(C.stream));
@@ -642,7 +669,10 @@ int atm_eval(void* context, int32_t flags) {
         require_ready(C);
         HIPCHK(hipSetDevice(C.device));
         C.resultsValid = false;
-        if (flags & ATM_EVAL_PLAIN) {
+        if (flags & ATM_EVAL_STATE1_ONLY) {
+            eval_state1(C, flags);
+            C.resultsOnHost = true;
+        } else if (flags & ATM_EVAL_PLAIN) {
             eval_plain(C, flags);
             C.resultsOnHost = true;
         } else {

@@ -719,7 +719,8 @@ __global__ void __launch_bounds__(256) k_set_energy(DevPtrs P, int c) {
 }
 
 // F_sys += w1 F1 + w2 F2 with (w1, w2) = (1-sp, sp) for direction > 0 and
-// (sp, 1-sp) otherwise (ReferenceATMMetaForceKernels.cpp:100-110).  In host mode
+// (sp, 1-sp) otherwise (ReferenceATMMetaForceKernels.cpp:100-110).  hostMode 2
+// adds F1 alone (state-1-only evaluation).  In host mode 1
 // sp and direction come from kernel arguments (the reference's HybridForce
 // receives sp as a scalar argument, CommonATMMetaForceKernels.cpp:199-201).
 __global__ void k_blend(DevPtrs P, int sysBegin, int hostMode, double spHost, double dirHost) {
@@ -734,9 +735,15 @@ __global__ void k_blend(DevPtrs P, int sysBegin, int hostMode, double spHost, do
     const double w2 = dir > 0.0 ? sp : 1.0 - sp;
     const double w1 = dir > 0.0 ? 1.0 - sp : sp;
     const int a = D0.slotOff + P.slotOf[D0.setAtomOff + i];
-    const int b = D1.slotOff + P.slotOf[D1.setAtomOff + i];
     long long* F = P.outF + 3ll * D0.atomOff;
     const double inv = 1.0 / kForceScale;
+    if (hostMode == 2) {  // state 1 only: F += F1 (the state-2 set was not evaluated)
+        F[i] += to_fixed_sat((double)P.sfx[a] * inv + P.sdx[a]);
+        F[N + i] += to_fixed_sat((double)P.sfy[a] * inv + P.sdy[a]);
+        F[2 * N + i] += to_fixed_sat((double)P.sfz[a] * inv + P.sdz[a]);
+        return;
+    }
+    const int b = D1.slotOff + P.slotOf[D1.setAtomOff + i];
     const double f1x = (double)P.sfx[a] * inv + P.sdx[a], f2x = (double)P.sfx[b] * inv + P.sdx[b];
     const double f1y = (double)P.sfy[a] * inv + P.sdy[a], f2y = (double)P.sfy[b] * inv + P.sdy[b];
     const double f1z = (double)P.sfz[a] * inv + P.sdz[a], f2z = (double)P.sfz[b] * inv + P.sdz[b];

@@ -1,0 +1,13 @@
+// Minimal OpenMM-compatible core: umbrella header.
+#pragma once
+#include "openmm/Context.h"
+#include "openmm/Force.h"
+#include "openmm/Integrator.h"
+#include "openmm/NonbondedForce.h"
+#include "openmm/OpenMMException.h"
+#include "openmm/Platform.h"
+#include "openmm/State.h"
+#include "openmm/System.h"
+#include "openmm/Vec3.h"
+#include "openmm/VerletIntegrator.h"
+#include "openmm/serialization/XmlSerializer.h"

@@ -1,0 +1,432 @@
+// Minimal OpenMM-compatible core: Force, System, NonbondedForce, State,
+// Integrator, Context and ContextImpl.
+#include <cmath>
+#include <cstdlib>
+#include <set>
+#include <sstream>
+
+#include "openmm/Context.h"
+#include "openmm/Force.h"
+#include "openmm/Integrator.h"
+#include "openmm/NonbondedForce.h"
+#include "openmm/OpenMMException.h"
+#include "openmm/Platform.h"
+#include "openmm/State.h"
+#include "openmm/System.h"
+#include "openmm/internal/ContextImpl.h"
+#include "openmm/internal/ForceImpl.h"
+#include "openmm/kernels.h"
+
+namespace OpenMM {
+
+// ------------------------------------------------------------------ Force
+
+void Force::setForceGroup(int group) {
+    if (group < 0 || group > 31) throw OpenMMException("Force group must be between 0 and 31");
+    forceGroup = group;
+}
+
+const std::string& Force::getName() const {
+    if (!name.empty()) return name;
+    defaultNameCache = defaultName();
+    return defaultNameCache;
+}
+
+ForceImpl& Force::getImplInContext(Context& context) {
+    for (ForceImpl* impl : context.getImpl().getForceImpls())
+        if (&impl->getOwner() == this) return *impl;
+    throw OpenMMException("getImplInContext: This Force is not present in the Context");
+}
+
+const ForceImpl& Force::getImplInContext(const Context& context) const {
+    for (ForceImpl* impl : context.getImpl().getForceImpls())
+        if (&impl->getOwner() == this) return *impl;
+    throw OpenMMException("getImplInContext: This Force is not present in the Context");
+}
+
+ContextImpl& Force::getContextImpl(Context& context) { return context.getImpl(); }
+
+// ------------------------------------------------------------------ System
+
+System::System() {
+    box[0] = Vec3(2, 0, 0);
+    box[1] = Vec3(0, 2, 0);
+    box[2] = Vec3(0, 0, 2);
+}
+
+System::~System() {
+    for (Force* f : forces) delete f;
+}
+
+double System::getParticleMass(int index) const {
+    if (index < 0 || index >= (int)masses.size()) throw OpenMMException("Index out of range");
+    return masses[index];
+}
+
+void System::setParticleMass(int index, double mass) {
+    if (index < 0 || index >= (int)masses.size()) throw OpenMMException("Index out of range");
+    masses[index] = mass;
+}
+
+int System::addConstraint(int particle1, int particle2, double distance) {
+    constraints.push_back(Constraint{particle1, particle2, distance});
+    return (int)constraints.size() - 1;
+}
+
+void System::getConstraintParameters(int index, int& particle1, int& particle2, double& distance) const {
+    if (index < 0 || index >= (int)constraints.size()) throw OpenMMException("Index out of range");
+    particle1 = constraints[index].p1;
+    particle2 = constraints[index].p2;
+    distance = constraints[index].d;
+}
+
+int System::addForce(Force* force) {
+    forces.push_back(force);
+    return (int)forces.size() - 1;
+}
+
+Force& System::getForce(int index) {
+    if (index < 0 || index >= (int)forces.size()) throw OpenMMException("Index out of range");
+    return *forces[index];
+}
+
+const Force& System::getForce(int index) const {
+    if (index < 0 || index >= (int)forces.size()) throw OpenMMException("Index out of range");
+    return *forces[index];
+}
+
+void System::removeForce(int index) {
+    if (index < 0 || index >= (int)forces.size()) throw OpenMMException("Index out of range");
+    delete forces[index];
+    forces.erase(forces.begin() + index);
+}
+
+void System::getDefaultPeriodicBoxVectors(Vec3& a, Vec3& b, Vec3& c) const {
+    a = box[0];
+    b = box[1];
+    c = box[2];
+}
+
+void System::setDefaultPeriodicBoxVectors(const Vec3& a, const Vec3& b, const Vec3& c) {
+    if (a[1] != 0.0 || a[2] != 0.0 || b[0] != 0.0 || b[2] != 0.0 || c[0] != 0.0 || c[1] != 0.0)
+        throw OpenMMException("Only rectangular periodic boxes are supported");
+    if (!(a[0] > 0 && b[1] > 0 && c[2] > 0)) throw OpenMMException("Periodic box vectors must be positive");
+    box[0] = a;
+    box[1] = b;
+    box[2] = c;
+}
+
+bool System::usesPeriodicBoundaryConditions() const {
+    for (Force* f : forces)
+        if (f->usesPeriodicBoundaryConditions()) return true;
+    return false;
+}
+
+// ------------------------------------------------------------------ NonbondedForce
+
+class NonbondedForceImpl : public ForceImpl {
+public:
+    explicit NonbondedForceImpl(const NonbondedForce& owner) : owner(owner) {}
+    void initialize(ContextImpl& context) override {
+        if (owner.getNumParticles() != context.getSystem().getNumParticles())
+            throw OpenMMException("NonbondedForce must have exactly as many particles as the System it belongs to.");
+        kernel = context.getPlatform().createKernel(CalcNonbondedForceKernel::Name(), context);
+        kernel.getAs<CalcNonbondedForceKernel>().initialize(context.getSystem(), owner);
+    }
+    const Force& getOwner() const override { return owner; }
+    void updateContextState(ContextImpl&, bool&) override {}
+    double calcForcesAndEnergy(ContextImpl& context, bool includeForces, bool includeEnergy, int groups) override {
+        if ((groups & (1 << owner.getForceGroup())) == 0) return 0.0;
+        return kernel.getAs<CalcNonbondedForceKernel>().execute(context, includeForces, includeEnergy);
+    }
+    std::map<std::string, double> getDefaultParameters() override { return {}; }
+    std::vector<std::string> getKernelNames() override { return {CalcNonbondedForceKernel::Name()}; }
+    void updateParametersInContext(ContextImpl& context) {
+        kernel.getAs<CalcNonbondedForceKernel>().copyParametersToContext(context, owner);
+    }
+
+private:
+    const NonbondedForce& owner;
+    Kernel kernel;
+};
+
+NonbondedForce::NonbondedForce()
+    : method(NoCutoff), cutoff(1.0), rfDielectric(78.3), useDispersionCorrection(true), recipGroup(-1) {}
+
+void NonbondedForce::setNonbondedMethod(NonbondedMethod m) {
+    if (m < NoCutoff || m > LJPME) throw OpenMMException("NonbondedForce: Illegal value for nonbonded method");
+    method = m;
+}
+
+void NonbondedForce::setCutoffDistance(double d) { cutoff = d; }
+void NonbondedForce::setReactionFieldDielectric(double d) { rfDielectric = d; }
+
+void NonbondedForce::setReciprocalSpaceForceGroup(int group) {
+    if (group < -1 || group > 31) throw OpenMMException("Force group must be between -1 and 31");
+    recipGroup = group;
+}
+
+int NonbondedForce::addParticle(double charge, double sigma, double epsilon) {
+    particles.push_back(Particle{charge, sigma, epsilon});
+    return (int)particles.size() - 1;
+}
+
+void NonbondedForce::getParticleParameters(int index, double& charge, double& sigma, double& epsilon) const {
+    if (index < 0 || index >= (int)particles.size()) throw OpenMMException("Index out of range");
+    charge = particles[index].q;
+    sigma = particles[index].sigma;
+    epsilon = particles[index].eps;
+}
+
+void NonbondedForce::setParticleParameters(int index, double charge, double sigma, double epsilon) {
+    if (index < 0 || index >= (int)particles.size()) throw OpenMMException("Index out of range");
+    particles[index] = Particle{charge, sigma, epsilon};
+}
+
+int NonbondedForce::addException(int p1, int p2, double qq, double sigma, double eps, bool replace) {
+    for (size_t k = 0; k < exceptions.size(); ++k) {
+        Exception& e = exceptions[k];
+        if ((e.p1 == p1 && e.p2 == p2) || (e.p1 == p2 && e.p2 == p1)) {
+            if (!replace) throw OpenMMException("NonbondedForce: There is already an exception for those particles");
+            e = Exception{p1, p2, qq, sigma, eps};
+            return (int)k;
+        }
+    }
+    exceptions.push_back(Exception{p1, p2, qq, sigma, eps});
+    return (int)exceptions.size() - 1;
+}
+
+void NonbondedForce::getExceptionParameters(int index, int& p1, int& p2, double& qq, double& sigma, double& eps) const {
+    if (index < 0 || index >= (int)exceptions.size()) throw OpenMMException("Index out of range");
+    const Exception& e = exceptions[index];
+    p1 = e.p1;
+    p2 = e.p2;
+    qq = e.qq;
+    sigma = e.sigma;
+    eps = e.eps;
+}
+
+void NonbondedForce::updateParametersInContext(Context& context) {
+    dynamic_cast<NonbondedForceImpl&>(getImplInContext(context)).updateParametersInContext(getContextImpl(context));
+}
+
+ForceImpl* NonbondedForce::createImpl() const { return new NonbondedForceImpl(*this); }
+
+// ------------------------------------------------------------------ State
+
+const std::vector<Vec3>& State::getPositions() const {
+    if (!(types & Positions)) throw OpenMMException("Invoked getPositions() on a State which does not contain positions.");
+    return positions;
+}
+const std::vector<Vec3>& State::getVelocities() const {
+    if (!(types & Velocities)) throw OpenMMException("Invoked getVelocities() on a State which does not contain velocities.");
+    return velocities;
+}
+const std::vector<Vec3>& State::getForces() const {
+    if (!(types & Forces)) throw OpenMMException("Invoked getForces() on a State which does not contain forces.");
+    return forces;
+}
+double State::getPotentialEnergy() const {
+    if (!(types & Energy)) throw OpenMMException("Invoked getPotentialEnergy() on a State which does not contain energies.");
+    return potentialEnergy;
+}
+double State::getKineticEnergy() const {
+    if (!(types & Energy)) throw OpenMMException("Invoked getKineticEnergy() on a State which does not contain energies.");
+    return kineticEnergy;
+}
+const std::map<std::string, double>& State::getParameters() const {
+    if (!(types & Parameters)) throw OpenMMException("Invoked getParameters() on a State which does not contain parameters.");
+    return parameters;
+}
+
+// ------------------------------------------------------------------ Integrator
+
+void Integrator::setIntegrationForceGroups(const std::set<int>& s) {
+    int mask = 0;
+    for (int g : s) {
+        if (g < 0 || g > 31) throw OpenMMException("Force group must be between 0 and 31");
+        mask |= 1 << g;
+    }
+    groups = mask;
+}
+
+void VerletIntegrator::step(int steps) {
+    if (!context) throw OpenMMException("This Integrator is not bound to a context!");
+    ContextImpl& c = *context;
+    const System& sys = c.getSystem();
+    const int n = sys.getNumParticles();
+    std::vector<Vec3>& x = c.positionsRef();
+    std::vector<Vec3>& v = c.velocitiesRef();
+    for (int s = 0; s < steps; ++s) {
+        c.calcForcesAndEnergy(true, false, groups);
+        const std::vector<Vec3>& f = c.forcesRef();
+        for (int i = 0; i < n; ++i) {
+            const double m = sys.getParticleMass(i);
+            if (m == 0.0) continue;
+            v[i] += f[i] * (stepSize / m);
+            x[i] += v[i] * stepSize;
+        }
+        c.setTime(c.getTime() + stepSize);
+    }
+}
+
+// ------------------------------------------------------------------ ContextImpl
+
+ContextImpl::ContextImpl(Context& owner, const System& system, Integrator& integrator, Platform* platform,
+                         const std::map<std::string, std::string>& properties)
+    : owner(owner), system(system), integrator(integrator), platform(platform), properties(properties), time(0.0),
+      platformData(nullptr) {
+    const int n = system.getNumParticles();
+    if (n == 0) throw OpenMMException("Cannot create a Context for a System with no particles");
+    positions.assign(n, Vec3());
+    velocities.assign(n, Vec3());
+    forces.assign(n, Vec3());
+    system.getDefaultPeriodicBoxVectors(box[0], box[1], box[2]);
+    platform->contextCreated(*this, properties);
+    try {
+        for (int i = 0; i < system.getNumForces(); ++i) forceImpls.push_back(system.getForce(i).createImpl());
+        for (ForceImpl* f : forceImpls)
+            for (auto& kv : f->getDefaultParameters()) parameters[kv.first] = kv.second;
+        for (ForceImpl* f : forceImpls) f->initialize(*this);
+        integrator.initialize(*this);
+    } catch (...) {
+        for (ForceImpl* f : forceImpls) delete f;
+        forceImpls.clear();
+        platform->contextDestroyed(*this);
+        throw;
+    }
+}
+
+ContextImpl::~ContextImpl() {
+    for (ForceImpl* f : forceImpls) delete f;
+    platform->contextDestroyed(*this);
+}
+
+void ContextImpl::setPositions(const std::vector<Vec3>& p) {
+    if (p.size() != positions.size())
+        throw OpenMMException("Called setPositions() on a Context with the wrong number of positions");
+    positions = p;
+}
+
+void ContextImpl::setVelocities(const std::vector<Vec3>& v) {
+    if (v.size() != velocities.size())
+        throw OpenMMException("Called setVelocities() on a Context with the wrong number of velocities");
+    velocities = v;
+}
+
+double ContextImpl::getParameter(const std::string& name) const {
+    auto it = parameters.find(name);
+    if (it == parameters.end()) throw OpenMMException("Called getParameter() with invalid parameter name: " + name);
+    return it->second;
+}
+
+void ContextImpl::setParameter(const std::string& name, double value) {
+    auto it = parameters.find(name);
+    if (it == parameters.end()) throw OpenMMException("Called setParameter() with invalid parameter name: " + name);
+    it->second = value;
+}
+
+void ContextImpl::setPeriodicBoxVectors(const Vec3& a, const Vec3& b, const Vec3& c) {
+    if (a[1] != 0.0 || a[2] != 0.0 || b[0] != 0.0 || b[2] != 0.0 || c[0] != 0.0 || c[1] != 0.0)
+        throw OpenMMException("Only rectangular periodic boxes are supported");
+    box[0] = a;
+    box[1] = b;
+    box[2] = c;
+}
+
+double ContextImpl::calcForcesAndEnergy(bool includeForces, bool includeEnergy, int groups) {
+    for (Vec3& f : forces) f = Vec3();
+    double energy = 0.0;
+    for (ForceImpl* f : forceImpls) energy += f->calcForcesAndEnergy(*this, includeForces, includeEnergy, groups);
+    return energy;
+}
+
+double ContextImpl::calcKineticEnergy() const {
+    double k = 0.0;
+    for (int i = 0; i < system.getNumParticles(); ++i) k += 0.5 * system.getParticleMass(i) * velocities[i].dot(velocities[i]);
+    return k;
+}
+
+// ------------------------------------------------------------------ Context
+
+static Platform& defaultPlatform() {
+    if (Platform::getNumPlatforms() == 0) throw OpenMMException("No Platform is registered");
+    Platform* best = &Platform::getPlatform(0);
+    for (int i = 1; i < Platform::getNumPlatforms(); ++i)
+        if (Platform::getPlatform(i).getSpeed() > best->getSpeed()) best = &Platform::getPlatform(i);
+    return *best;
+}
+
+Context::Context(const System& system, Integrator& integrator) : impl(nullptr) {
+    impl = new ContextImpl(*this, system, integrator, &defaultPlatform(), {});
+}
+
+Context::Context(const System& system, Integrator& integrator, Platform& platform) : impl(nullptr) {
+    impl = new ContextImpl(*this, system, integrator, &platform, {});
+}
+
+Context::Context(const System& system, Integrator& integrator, Platform& platform,
+                 const std::map<std::string, std::string>& properties)
+    : impl(nullptr) {
+    impl = new ContextImpl(*this, system, integrator, &platform, properties);
+}
+
+Context::~Context() { delete impl; }
+
+const System& Context::getSystem() const { return impl->getSystem(); }
+Integrator& Context::getIntegrator() { return impl->getIntegrator(); }
+Platform& Context::getPlatform() { return impl->getPlatform(); }
+
+State Context::getState(int types, bool enforcePeriodicBox, int groups) const {
+    State s;
+    s.types = types;
+    s.time = impl->getTime();
+    impl->getPeriodicBoxVectors(s.box[0], s.box[1], s.box[2]);
+    if (types & (State::Forces | State::Energy)) {
+        const double e = impl->calcForcesAndEnergy((types & State::Forces) != 0, (types & State::Energy) != 0, groups);
+        if (types & State::Energy) {
+            s.potentialEnergy = e;
+            s.kineticEnergy = impl->calcKineticEnergy();
+        }
+        if (types & State::Forces) impl->getForces(s.forces);
+    }
+    if (types & State::Parameters) s.parameters = impl->getParameters();
+    if (types & State::Positions) {
+        impl->getPositions(s.positions);
+        if (enforcePeriodicBox)
+            for (Vec3& p : s.positions)
+                for (int k = 0; k < 3; ++k) p[k] -= s.box[k][k] * std::floor(p[k] / s.box[k][k]);
+    }
+    if (types & State::Velocities) impl->getVelocities(s.velocities);
+    return s;
+}
+
+State Context::getState(int types, bool enforcePeriodicBox, const std::set<int>& groups) const {
+    int mask = 0;
+    for (int g : groups) {
+        if (g < 0 || g > 31) throw OpenMMException("Force group must be between 0 and 31");
+        mask |= 1 << g;
+    }
+    return getState(types, enforcePeriodicBox, mask);
+}
+
+void Context::setTime(double t) { impl->setTime(t); }
+void Context::setPositions(const std::vector<Vec3>& positions) { impl->setPositions(positions); }
+void Context::setVelocities(const std::vector<Vec3>& velocities) { impl->setVelocities(velocities); }
+double Context::getParameter(const std::string& name) const { return impl->getParameter(name); }
+void Context::setParameter(const std::string& name, double value) { impl->setParameter(name, value); }
+const std::map<std::string, double>& Context::getParameters() const { return impl->getParameters(); }
+void Context::setPeriodicBoxVectors(const Vec3& a, const Vec3& b, const Vec3& c) { impl->setPeriodicBoxVectors(a, b, c); }
+
+void Context::setState(const State& state) {
+    setTime(state.getTime());
+    Vec3 a, b, c;
+    state.getPeriodicBoxVectors(a, b, c);
+    setPeriodicBoxVectors(a, b, c);
+    if (state.getDataTypes() & State::Positions) setPositions(state.getPositions());
+    if (state.getDataTypes() & State::Velocities) setVelocities(state.getVelocities());
+    if (state.getDataTypes() & State::Parameters)
+        for (auto& kv : state.getParameters()) setParameter(kv.first, kv.second);
+}
+
+}  // namespace OpenMM

@@ -1,0 +1,78 @@
+#ifndef ATM_MI355X_KERNELS_H_
+#define ATM_MI355X_KERNELS_H_
+// MI355X implementations of the CalcATMMetaForce and CalcNonbondedForce
+// kernels.  Each owns one libatm_mi355x context (include/atm_mi355x.h) holding
+// one system; status codes of the C ABI are rethrown as OpenMMException.
+#include <string>
+#include <vector>
+#include "ATMMetaForceKernels.h"
+#include "openmm/NonbondedForce.h"
+#include "openmm/Vec3.h"
+#include "openmm/kernels.h"
+
+namespace ATMMetaForcePlugin {
+
+// One system of the C ABI built from a NonbondedForce (+ displacements).
+class MI355XSystemContext {
+public:
+    ~MI355XSystemContext();
+    void create(int device, const OpenMM::System& system, const OpenMM::NonbondedForce& nb,
+                const std::vector<OpenMM::Vec3>* displacements);
+    void setParticles(const OpenMM::NonbondedForce& nb, const std::vector<OpenMM::Vec3>* displacements);
+    void setDisplacements(const std::vector<OpenMM::Vec3>& displacements);
+    // Upload box (re-laying out the cell grid when it changed) and positions.
+    void syncState(OpenMM::ContextImpl& context);
+    // Evaluate with the given C-ABI flags; re-runs once if a list overflowed.
+    void evaluate(int flags);
+    void* handle() const { return ctx; }
+    int numAtoms() const { return n; }
+    double dispersionEnergy(const OpenMM::Vec3 box[3]) const;
+
+private:
+    void* ctx = nullptr;
+    int n = 0;
+    double box[3] = {0, 0, 0};
+    double cutoff = 0, rfDielectric = 0;
+    double dispersionCoefficient = 0;  // E_disp = coefficient / volume
+};
+
+class MI355XCalcATMMetaForceKernel : public CalcATMMetaForceKernel {
+public:
+    MI355XCalcATMMetaForceKernel(std::string name, const OpenMM::Platform& platform, OpenMM::ContextImpl& context);
+    void initialize(const OpenMM::System& system, const ATMMetaForce& force) override;
+    double execute(OpenMM::ContextImpl& context, OpenMM::ContextImpl& innerContext1, OpenMM::ContextImpl& innerContext2,
+                   double State1Energy, double State2Energy, bool includeForces, bool includeEnergy) override;
+    void copyParametersToContext(OpenMM::ContextImpl& context, const ATMMetaForce& force) override;
+    void copyState(OpenMM::ContextImpl& context, OpenMM::ContextImpl& innerContext1,
+                   OpenMM::ContextImpl& innerContext2) override;
+    double getPerturbationEnergy(void) override { return PerturbationEnergy; }
+    bool computesInnerStates() const override { return true; }
+    double executeFused(OpenMM::ContextImpl& context, bool includeForces, bool includeEnergy) override;
+    // U1, U2 of the last evaluation (for diagnostics and tests)
+    double lastState1Energy() const { return U1; }
+    double lastState2Energy() const { return U2; }
+
+private:
+    OpenMM::ContextImpl& context;
+    MI355XSystemContext sys;
+    bool hasVariableForces = false;
+    std::vector<int> particles;
+    double PerturbationEnergy = 0.0, U1 = 0.0, U2 = 0.0;
+};
+
+class MI355XCalcNonbondedForceKernel : public OpenMM::CalcNonbondedForceKernel {
+public:
+    MI355XCalcNonbondedForceKernel(std::string name, const OpenMM::Platform& platform, OpenMM::ContextImpl& context)
+        : OpenMM::CalcNonbondedForceKernel(name, platform), context(context) {}
+    void initialize(const OpenMM::System& system, const OpenMM::NonbondedForce& force) override;
+    double execute(OpenMM::ContextImpl& context, bool includeForces, bool includeEnergy) override;
+    void copyParametersToContext(OpenMM::ContextImpl& context, const OpenMM::NonbondedForce& force) override;
+
+private:
+    OpenMM::ContextImpl& context;
+    MI355XSystemContext sys;
+};
+
+}  // namespace ATMMetaForcePlugin
+
+#endif

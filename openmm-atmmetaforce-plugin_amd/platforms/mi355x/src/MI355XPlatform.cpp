@@ -1,0 +1,376 @@
+// MI355X platform, its kernel factories and the plugin entry points.
+#include <cmath>
+#include <cstdlib>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../../../include/atm_mi355x.h"
+#include "ATMMetaForce.h"
+#include "ATMMetaForceKernels.h"
+#include "MI355XATMMetaForceKernels.h"
+#include "MI355XPlatform.h"
+#include "openmm/KernelFactory.h"
+#include "openmm/NonbondedForce.h"
+#include "openmm/OpenMMException.h"
+#include "openmm/internal/ContextImpl.h"
+
+using namespace ATMMetaForcePlugin;
+using namespace OpenMM;
+
+static_assert(sizeof(Vec3) == 3 * sizeof(double), "Vec3 must be three packed doubles");
+
+namespace {
+
+void check(int rc, const char* what) {
+    if (rc != ATM_OK) throw OpenMMException(std::string("MI355X: ") + what + ": " + atm_last_error());
+}
+
+// First NonbondedForce among the forces whose group is in `mask`; every other
+// force in those groups is rejected (the device path evaluates only this term).
+const NonbondedForce* variableNonbonded(const System& system, int mask, const Force* skip) {
+    const NonbondedForce* nb = nullptr;
+    for (int i = 0; i < system.getNumForces(); ++i) {
+        const Force& f = system.getForce(i);
+        if (&f == skip || !(mask & (1 << f.getForceGroup()))) continue;
+        const NonbondedForce* n = dynamic_cast<const NonbondedForce*>(&f);
+        if (!n)
+            throw OpenMMException("MI355X ATMMetaForce: only a NonbondedForce may be in the variable force groups (found '" +
+                                  f.getName() + "')");
+        if (nb) throw OpenMMException("MI355X ATMMetaForce: more than one NonbondedForce in the variable force groups");
+        nb = n;
+    }
+    return nb;
+}
+
+void validateNonbonded(const NonbondedForce& nb) {
+    if (nb.getNonbondedMethod() != NonbondedForce::CutoffPeriodic)
+        throw OpenMMException("MI355X platform: NonbondedForce must use CutoffPeriodic (reaction-field) electrostatics");
+    if (nb.getNumExceptions() > 0)
+        throw OpenMMException("MI355X platform: NonbondedForce exceptions are not supported");
+}
+
+class NonbondedFactory : public KernelFactory {
+public:
+    KernelImpl* createKernelImpl(std::string name, const Platform& platform, ContextImpl& context) const override {
+        if (name == CalcNonbondedForceKernel::Name()) return new MI355XCalcNonbondedForceKernel(name, platform, context);
+        throw OpenMMException("Tried to create kernel with illegal kernel name '" + name + "'");
+    }
+};
+
+class ATMFactory : public KernelFactory {
+public:
+    KernelImpl* createKernelImpl(std::string name, const Platform& platform, ContextImpl& context) const override {
+        if (name == CalcATMMetaForceKernel::Name()) return new MI355XCalcATMMetaForceKernel(name, platform, context);
+        throw OpenMMException("Tried to create kernel with illegal kernel name '" + name + "'");
+    }
+};
+
+}  // namespace
+
+// ------------------------------------------------------------------ platform
+
+MI355XPlatform::MI355XPlatform() {
+    platformProperties = {"DeviceIndex", "Precision", "ATMEvaluation"};
+    setPropertyDefaultValue("DeviceIndex", "0");
+    setPropertyDefaultValue("Precision", "mixed");
+    setPropertyDefaultValue("ATMEvaluation", "fused");
+    registerKernelFactory(CalcNonbondedForceKernel::Name(), new NonbondedFactory());
+}
+
+const std::string& MI355XPlatform::getPropertyValue(const Context& context, const std::string& property) const {
+    return Platform::getPropertyValue(context, property);
+}
+
+void MI355XPlatform::contextCreated(ContextImpl& context, const std::map<std::string, std::string>& properties) const {
+    PlatformData* d = new PlatformData();
+    for (const std::string& p : platformProperties) d->properties[p] = getPropertyDefaultValue(p);
+    for (auto& kv : properties) {
+        bool known = false;
+        for (const std::string& p : platformProperties) known |= p == kv.first;
+        if (!known) {
+            delete d;
+            throw OpenMMException("Illegal property name: " + kv.first);
+        }
+        d->properties[kv.first] = kv.second;
+    }
+    d->device = std::atoi(d->properties["DeviceIndex"].c_str());
+    const std::string& prec = d->properties["Precision"];
+    if (prec != "mixed" && prec != "double" && prec != "single") {
+        delete d;
+        throw OpenMMException("Illegal value for Precision: " + prec);
+    }
+    d->fp32Pairs = prec == "single";
+    const std::string& ev = d->properties["ATMEvaluation"];
+    if (ev != "fused" && ev != "plain") {
+        delete d;
+        throw OpenMMException("Illegal value for ATMEvaluation: " + ev);
+    }
+    d->plainATM = ev == "plain";
+    context.setPlatformData(d);
+}
+
+void MI355XPlatform::contextDestroyed(ContextImpl& context) const {
+    delete static_cast<PlatformData*>(context.getPlatformData());
+    context.setPlatformData(nullptr);
+}
+
+MI355XPlatform::PlatformData& MI355XPlatform::data(ContextImpl& context) {
+    return *static_cast<PlatformData*>(context.getPlatformData());
+}
+
+// ------------------------------------------------------------------ one system of the C ABI
+
+MI355XSystemContext::~MI355XSystemContext() {
+    if (ctx) atm_destroy(ctx);
+}
+
+void MI355XSystemContext::create(int device, const System& system, const NonbondedForce& nb,
+                                 const std::vector<Vec3>* displacements) {
+    validateNonbonded(nb);
+    n = system.getNumParticles();
+    if (nb.getNumParticles() != n)
+        throw OpenMMException("NonbondedForce must have exactly as many particles as the System it belongs to.");
+    Vec3 a, b, c;
+    system.getDefaultPeriodicBoxVectors(a, b, c);
+    atm_system_desc d;
+    d.num_atoms = n;
+    d.box[0] = a[0];
+    d.box[1] = b[1];
+    d.box[2] = c[2];
+    d.cutoff = nb.getCutoffDistance();
+    d.rf_dielectric = nb.getReactionFieldDielectric();
+    cutoff = d.cutoff;
+    rfDielectric = d.rf_dielectric;
+    for (int k = 0; k < 3; ++k) box[k] = d.box[k];
+    check(atm_create(&d, 1, device, &ctx), "atm_create");
+    setParticles(nb, displacements);
+    std::vector<double> zeros(3 * (size_t)n, 0.0);
+    check(atm_set_positions(ctx, 0, zeros.data()), "atm_set_positions");
+}
+
+void MI355XSystemContext::setParticles(const NonbondedForce& nb, const std::vector<Vec3>* displacements) {
+    std::vector<double> q(n), s(n), e(n);
+    for (int i = 0; i < n; ++i) nb.getParticleParameters(i, q[i], s[i], e[i]);
+    const double* dptr = displacements ? reinterpret_cast<const double*>(displacements->data()) : nullptr;
+    check(atm_set_particles(ctx, 0, q.data(), s.data(), e.data(), dptr), "atm_set_particles");
+    // Long-range dispersion correction: E = 8 pi N^2 / V * (<eps s^12>/(9 rc^9) - <eps s^6>/(3 rc^3)),
+    // averaged over unordered pairs i <= j of particle classes with equal (sigma, epsilon).
+    dispersionCoefficient = 0.0;
+    if (nb.getUseDispersionCorrection()) {
+        std::map<std::pair<double, double>, long long> classes;
+        for (int i = 0; i < n; ++i) classes[{s[i], e[i]}]++;
+        std::vector<std::pair<std::pair<double, double>, long long>> cl(classes.begin(), classes.end());
+        double s12 = 0.0, s6 = 0.0;
+        for (size_t a = 0; a < cl.size(); ++a)
+            for (size_t b = a; b < cl.size(); ++b) {
+                const double sig = 0.5 * (cl[a].first.first + cl[b].first.first);
+                const double eps = std::sqrt(cl[a].first.second * cl[b].first.second);
+                const double w = a == b ? 0.5 * (double)cl[a].second * (cl[a].second + 1) : (double)cl[a].second * cl[b].second;
+                const double sig6 = std::pow(sig, 6.0);
+                s12 += w * eps * sig6 * sig6;
+                s6 += w * eps * sig6;
+            }
+        const double npairs = 0.5 * (double)n * (n + 1);
+        const double rc3 = cutoff * cutoff * cutoff;
+        dispersionCoefficient = 8.0 * M_PI * (double)n * n * (s12 / npairs / (9.0 * rc3 * rc3 * rc3) - s6 / npairs / (3.0 * rc3));
+    }
+}
+
+void MI355XSystemContext::setDisplacements(const std::vector<Vec3>& displacements) {
+    check(atm_set_displacements(ctx, 0, reinterpret_cast<const double*>(displacements.data())), "atm_set_displacements");
+}
+
+double MI355XSystemContext::dispersionEnergy(const Vec3 b[3]) const {
+    return dispersionCoefficient == 0.0 ? 0.0 : dispersionCoefficient / (b[0][0] * b[1][1] * b[2][2]);
+}
+
+void MI355XSystemContext::syncState(ContextImpl& context) {
+    Vec3 a, b, c;
+    context.getPeriodicBoxVectors(a, b, c);
+    if (a[0] != box[0] || b[1] != box[1] || c[2] != box[2]) {
+        const double nb[3] = {a[0], b[1], c[2]};
+        check(atm_set_box(ctx, 0, nb), "atm_set_box");
+        box[0] = nb[0];
+        box[1] = nb[1];
+        box[2] = nb[2];
+    }
+    check(atm_set_positions(ctx, 0, reinterpret_cast<const double*>(context.positionsRef().data())), "atm_set_positions");
+}
+
+void MI355XSystemContext::evaluate(int flags) {
+    for (int attempt = 0; attempt < 3; ++attempt) {
+        check(atm_eval(ctx, flags), "atm_eval");
+        atm_result r;
+        const int rc = atm_get_results(ctx, 0, &r);
+        if (rc == ATM_ERR_CAPACITY) continue;  // the list capacity was doubled: evaluate again
+        check(rc, "atm_get_results");
+        return;
+    }
+    throw OpenMMException("MI355X: neighbour list capacity could not be satisfied");
+}
+
+static void addForces(void* ctx, int n, ContextImpl& context) {
+    std::vector<double> f(3 * (size_t)n);
+    check(atm_get_forces(ctx, 0, f.data()), "atm_get_forces");
+    std::vector<Vec3>& F = context.forcesRef();
+    for (int i = 0; i < n; ++i) F[i] += Vec3(f[3 * i], f[3 * i + 1], f[3 * i + 2]);
+}
+
+// ------------------------------------------------------------------ NonbondedForce kernel
+
+void MI355XCalcNonbondedForceKernel::initialize(const System& system, const NonbondedForce& force) {
+    sys.create(MI355XPlatform::data(context).device, system, force, nullptr);
+}
+
+double MI355XCalcNonbondedForceKernel::execute(ContextImpl& ctx, bool includeForces, bool includeEnergy) {
+    sys.syncState(ctx);
+    const MI355XPlatform::PlatformData& pd = MI355XPlatform::data(ctx);
+    int flags = ATM_EVAL_STATE1_ONLY | ATM_EVAL_CLEAR_FORCES | (includeForces ? ATM_EVAL_FORCES : 0) |
+                (pd.fp32Pairs ? ATM_EVAL_FP32_PAIRS : 0);
+    sys.evaluate(flags);
+    atm_result r;
+    check(atm_get_results(sys.handle(), 0, &r), "atm_get_results");
+    if (includeForces) addForces(sys.handle(), sys.numAtoms(), ctx);
+    Vec3 b[3];
+    ctx.getPeriodicBoxVectors(b[0], b[1], b[2]);
+    return includeEnergy ? r.state1_energy + sys.dispersionEnergy(b) : 0.0;
+}
+
+void MI355XCalcNonbondedForceKernel::copyParametersToContext(ContextImpl& ctx, const NonbondedForce& force) {
+    (void)ctx;
+    if (force.getNumParticles() != sys.numAtoms())
+        throw OpenMMException("updateParametersInContext: The number of particles has changed");
+    validateNonbonded(force);
+    sys.setParticles(force, nullptr);
+}
+
+// ------------------------------------------------------------------ ATMMetaForce kernel
+
+MI355XCalcATMMetaForceKernel::MI355XCalcATMMetaForceKernel(std::string name, const Platform& platform,
+                                                           ContextImpl& context)
+    : CalcATMMetaForceKernel(name, platform), context(context) {}
+
+// Entry i of the force displaces particle i (the reference's convention,
+// ReferenceATMMetaForceKernels.cpp:120-123), so every particle of the System
+// must be added, in order.
+static std::vector<Vec3> displacementTable(const System& system, const ATMMetaForce& force, std::vector<int>* particles) {
+    const int n = system.getNumParticles();
+    if (force.getNumParticles() != n)
+        throw OpenMMException("ATMMetaForce: every particle of the System must be added to the force (" +
+                              std::to_string(force.getNumParticles()) + " of " + std::to_string(n) + ")");
+    std::vector<Vec3> d(n);
+    if (particles) particles->resize(n);
+    for (int i = 0; i < n; ++i) {
+        int p;
+        double dx, dy, dz;
+        force.getParticleParameters(i, p, dx, dy, dz);
+        d[i] = Vec3(dx, dy, dz);
+        if (particles) (*particles)[i] = p;
+    }
+    return d;
+}
+
+void MI355XCalcATMMetaForceKernel::initialize(const System& system, const ATMMetaForce& force) {
+    int mask = 0;
+    for (int g : force.getVariableForceGroups()) mask |= 1 << g;
+    const NonbondedForce* nb = variableNonbonded(system, mask, &force);
+    std::vector<Vec3> d = displacementTable(system, force, &particles);
+    hasVariableForces = nb != nullptr;
+    if (hasVariableForces) sys.create(MI355XPlatform::data(context).device, system, *nb, &d);
+}
+
+double MI355XCalcATMMetaForceKernel::executeFused(ContextImpl& ctx, bool includeForces, bool includeEnergy) {
+    atm_params p;
+    p.lambda1 = ctx.getParameter(ATMMetaForce::Lambda1());
+    p.lambda2 = ctx.getParameter(ATMMetaForce::Lambda2());
+    p.alpha = ctx.getParameter(ATMMetaForce::Alpha());
+    p.u0 = ctx.getParameter(ATMMetaForce::U0());
+    p.w0 = ctx.getParameter(ATMMetaForce::W0());
+    p.umax = ctx.getParameter(ATMMetaForce::Umax());
+    p.ubcore = ctx.getParameter(ATMMetaForce::Ubcore());
+    p.acore = ctx.getParameter(ATMMetaForce::Acore());
+    p.direction = ctx.getParameter(ATMMetaForce::Direction());
+    if (!hasVariableForces) {
+        // nothing varies: U1 = U2 = 0; the soft-core and softplus of u = 0
+        U1 = U2 = 0.0;
+        const double x = -p.alpha * (0.0 - p.u0);
+        double w = p.lambda2 * 0.0 + p.w0;
+        if (p.alpha > 0.0) w += (p.lambda2 - p.lambda1) / p.alpha * (x > 0.0 ? x + std::log1p(std::exp(-x)) : std::log1p(std::exp(x)));
+        PerturbationEnergy = 0.0;
+        return includeEnergy ? w : 0.0;
+    }
+    sys.syncState(ctx);
+    check(atm_set_params(sys.handle(), 0, &p), "atm_set_params");
+    const MI355XPlatform::PlatformData& pd = MI355XPlatform::data(ctx);
+    const int flags = ATM_EVAL_CLEAR_FORCES | (includeForces ? ATM_EVAL_FORCES : 0) | (pd.plainATM ? ATM_EVAL_PLAIN : 0) |
+                      (pd.fp32Pairs ? ATM_EVAL_FP32_PAIRS : 0);
+    sys.evaluate(flags);
+    atm_result r;
+    check(atm_get_results(sys.handle(), 0, &r), "atm_get_results");
+    if (includeForces) addForces(sys.handle(), sys.numAtoms(), ctx);
+    Vec3 b[3];
+    ctx.getPeriodicBoxVectors(b[0], b[1], b[2]);
+    const double edisp = sys.dispersionEnergy(b);  // present in both states: shifts e0 only
+    U1 = r.state1_energy + edisp;
+    U2 = r.state2_energy + edisp;
+    PerturbationEnergy = r.perturbation_energy;
+    return includeEnergy ? r.energy + edisp : 0.0;
+}
+
+double MI355XCalcATMMetaForceKernel::execute(ContextImpl&, ContextImpl&, ContextImpl&, double, double, bool, bool) {
+    throw OpenMMException("MI355X CalcATMMetaForce evaluates both states itself; use executeFused");
+}
+
+void MI355XCalcATMMetaForceKernel::copyState(ContextImpl&, ContextImpl&, ContextImpl&) {
+    throw OpenMMException("MI355X CalcATMMetaForce uses no inner contexts");
+}
+
+// Displacements are replaced (the behaviour of CommonATMMetaForceKernels.cpp:229-251;
+// the reference CPU kernel appends instead, ReferenceATMMetaForceKernels.cpp:155).
+void MI355XCalcATMMetaForceKernel::copyParametersToContext(ContextImpl& ctx, const ATMMetaForce& force) {
+    (void)ctx;
+    if (force.getNumParticles() != (int)particles.size())
+        throw OpenMMException("copyParametersToContext: The number of ATMMetaForce particles has changed");
+    std::vector<Vec3> d(particles.size());
+    for (int i = 0; i < force.getNumParticles(); ++i) {
+        int p;
+        double dx, dy, dz;
+        force.getParticleParameters(i, p, dx, dy, dz);
+        if (p != particles[i])
+            throw OpenMMException("MI355XCalcATMMetaForceKernel::copyParametersToContext: A particle index has changed");
+        d[i] = Vec3(dx, dy, dz);
+    }
+    if (hasVariableForces) sys.setDisplacements(d);
+}
+
+// ------------------------------------------------------------------ plugin entry points
+
+extern "C" __attribute__((visibility("default"))) void registerPlatforms() {
+    try {
+        Platform::getPlatformByName("MI355X");
+    } catch (...) {
+        Platform::registerPlatform(new MI355XPlatform());
+    }
+}
+
+// Registers the ATMMetaForce kernel factory with the MI355X platform; errors are
+// ignored, as the reference's factories do (HipATMMetaForceKernelFactory.cpp:18-27).
+extern "C" __attribute__((visibility("default"))) void registerKernelFactories() {
+    try {
+        Platform& platform = Platform::getPlatformByName("MI355X");
+        platform.registerKernelFactory(CalcATMMetaForceKernel::Name(), new ATMFactory());
+    } catch (...) {
+    }
+}
+
+extern "C" __attribute__((visibility("default"))) void registerATMMetaForceMI355XKernelFactories() {
+    registerPlatforms();
+    registerKernelFactories();
+}
+
+namespace {
+struct AutoRegister {
+    AutoRegister() { registerATMMetaForceMI355XKernelFactories(); }
+} autoRegister;
+}  // namespace

@@ -1,0 +1,285 @@
+// Python module _atmmetaforce: the reference's SWIG surface
+// (python/atmmetaforceplugin.i:64-124) -- ATMMetaForce with its static
+// parameter-name methods, cast/isinstance and ATMMETAFORCE_VERSION -- plus the
+// minimal OpenMM-compatible classes a script needs around it (System,
+// NonbondedForce, Context, State, Platform, VerletIntegrator, XmlSerializer).
+// C++ exceptions surface as Python exceptions (atmmetaforceplugin.i:54-61).
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+
+#include <map>
+#include <set>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "ATMMetaForce.h"
+#include "ATMMetaForceVersion.h"
+#include "internal/ATMMetaForceImpl.h"
+#include "openmm/Context.h"
+#include "openmm/Integrator.h"
+#include "openmm/NonbondedForce.h"
+#include "openmm/OpenMMException.h"
+#include "openmm/Platform.h"
+#include "openmm/State.h"
+#include "openmm/System.h"
+#include "openmm/serialization/XmlSerializer.h"
+
+namespace py = pybind11;
+using namespace OpenMM;
+using ATMMetaForcePlugin::ATMMetaForce;
+
+namespace {
+
+Vec3 toVec3(const py::handle& h) {
+    py::sequence s = py::reinterpret_borrow<py::sequence>(h);
+    if (py::len(s) != 3) throw OpenMMException("expected a 3-vector");
+    return Vec3(s[0].cast<double>(), s[1].cast<double>(), s[2].cast<double>());
+}
+
+std::vector<Vec3> toVec3List(const py::object& o) {
+    std::vector<Vec3> v;
+    if (py::isinstance<py::array>(o)) {
+        py::array_t<double, py::array::c_style | py::array::forcecast> a(o);
+        if (a.ndim() != 2 || a.shape(1) != 3) throw OpenMMException("expected an (N, 3) array");
+        auto r = a.unchecked<2>();
+        v.reserve(a.shape(0));
+        for (py::ssize_t i = 0; i < a.shape(0); ++i) v.emplace_back(r(i, 0), r(i, 1), r(i, 2));
+        return v;
+    }
+    for (py::handle h : o) v.push_back(toVec3(h));
+    return v;
+}
+
+py::object fromVec3List(const std::vector<Vec3>& v, bool asNumpy) {
+    if (asNumpy) {
+        py::array_t<double> a({(py::ssize_t)v.size(), (py::ssize_t)3});
+        auto w = a.mutable_unchecked<2>();
+        for (size_t i = 0; i < v.size(); ++i)
+            for (int k = 0; k < 3; ++k) w(i, k) = v[i][k];
+        return a;
+    }
+    py::list l;
+    for (const Vec3& x : v) l.append(py::make_tuple(x[0], x[1], x[2]));
+    return l;
+}
+
+int groupMask(const py::object& groups) {
+    if (groups.is_none()) return -1;
+    if (py::isinstance<py::int_>(groups)) return groups.cast<int>();
+    int mask = 0;
+    for (py::handle h : groups) {
+        int g = h.cast<int>();
+        if (g < 0 || g > 31) throw OpenMMException("Force group must be between 0 and 31");
+        mask |= 1 << g;
+    }
+    return mask;
+}
+
+}  // namespace
+
+PYBIND11_MODULE(_atmmetaforce, m) {
+    m.doc() = "MI355X-native ATMMetaForce (C++ API binding)";
+    py::register_exception<OpenMMException>(m, "OpenMMException", PyExc_Exception);
+    m.attr("ATMMETAFORCE_VERSION") = ATMMETAFORCE_VERSION;
+
+    // ---------------------------------------------------------------- core
+    py::class_<Platform, std::unique_ptr<Platform, py::nodelete>>(m, "Platform")
+        .def_static("getNumPlatforms", &Platform::getNumPlatforms)
+        .def_static("getPlatform", &Platform::getPlatform, py::return_value_policy::reference)
+        .def_static("getPlatformByName", &Platform::getPlatformByName, py::return_value_policy::reference)
+        .def_static("loadPluginsFromDirectory", &Platform::loadPluginsFromDirectory)
+        .def_static("getPluginLoadFailures", &Platform::getPluginLoadFailures)
+        .def_static("getDefaultPluginsDirectory", &Platform::getDefaultPluginsDirectory)
+        .def_static("getOpenMMVersion", &Platform::getOpenMMVersion)
+        .def("getName", &Platform::getName)
+        .def("getSpeed", &Platform::getSpeed)
+        .def("supportsDoublePrecision", &Platform::supportsDoublePrecision)
+        .def("getPropertyNames", &Platform::getPropertyNames)
+        .def("getPropertyDefaultValue", &Platform::getPropertyDefaultValue)
+        .def("setPropertyDefaultValue", &Platform::setPropertyDefaultValue);
+
+    py::class_<Force, std::unique_ptr<Force, py::nodelete>>(m, "Force")
+        .def("getForceGroup", &Force::getForceGroup)
+        .def("setForceGroup", &Force::setForceGroup)
+        .def("getName", &Force::getName)
+        .def("setName", &Force::setName)
+        .def("usesPeriodicBoundaryConditions", &Force::usesPeriodicBoundaryConditions);
+
+    py::class_<NonbondedForce, Force, std::unique_ptr<NonbondedForce, py::nodelete>> nb(m, "NonbondedForce");
+    nb.def(py::init([]() { return new NonbondedForce(); }))
+        .def("getNumParticles", &NonbondedForce::getNumParticles)
+        .def("getNumExceptions", &NonbondedForce::getNumExceptions)
+        .def("addParticle", &NonbondedForce::addParticle, py::arg("charge"), py::arg("sigma"), py::arg("epsilon"))
+        .def("getParticleParameters", [](const NonbondedForce& f, int i) {
+            double q, s, e;
+            f.getParticleParameters(i, q, s, e);
+            return py::make_tuple(q, s, e);
+        })
+        .def("setParticleParameters", &NonbondedForce::setParticleParameters)
+        .def("addException", &NonbondedForce::addException, py::arg("particle1"), py::arg("particle2"),
+             py::arg("chargeProd"), py::arg("sigma"), py::arg("epsilon"), py::arg("replace") = false)
+        .def("getNonbondedMethod", &NonbondedForce::getNonbondedMethod)
+        .def("setNonbondedMethod", &NonbondedForce::setNonbondedMethod)
+        .def("getCutoffDistance", &NonbondedForce::getCutoffDistance)
+        .def("setCutoffDistance", &NonbondedForce::setCutoffDistance)
+        .def("getReactionFieldDielectric", &NonbondedForce::getReactionFieldDielectric)
+        .def("setReactionFieldDielectric", &NonbondedForce::setReactionFieldDielectric)
+        .def("getUseDispersionCorrection", &NonbondedForce::getUseDispersionCorrection)
+        .def("setUseDispersionCorrection", &NonbondedForce::setUseDispersionCorrection)
+        .def("getReciprocalSpaceForceGroup", &NonbondedForce::getReciprocalSpaceForceGroup)
+        .def("setReciprocalSpaceForceGroup", &NonbondedForce::setReciprocalSpaceForceGroup)
+        .def("updateParametersInContext", &NonbondedForce::updateParametersInContext);
+    nb.attr("NoCutoff") = (int)NonbondedForce::NoCutoff;
+    nb.attr("CutoffNonPeriodic") = (int)NonbondedForce::CutoffNonPeriodic;
+    nb.attr("CutoffPeriodic") = (int)NonbondedForce::CutoffPeriodic;
+    nb.attr("Ewald") = (int)NonbondedForce::Ewald;
+    nb.attr("PME") = (int)NonbondedForce::PME;
+    nb.attr("LJPME") = (int)NonbondedForce::LJPME;
+    py::enum_<NonbondedForce::NonbondedMethod>(nb, "NonbondedMethod")
+        .value("NoCutoff", NonbondedForce::NoCutoff)
+        .value("CutoffNonPeriodic", NonbondedForce::CutoffNonPeriodic)
+        .value("CutoffPeriodic", NonbondedForce::CutoffPeriodic)
+        .value("Ewald", NonbondedForce::Ewald)
+        .value("PME", NonbondedForce::PME)
+        .value("LJPME", NonbondedForce::LJPME);
+
+    py::class_<System>(m, "System")
+        .def(py::init<>())
+        .def("getNumParticles", &System::getNumParticles)
+        .def("addParticle", &System::addParticle)
+        .def("getParticleMass", &System::getParticleMass)
+        .def("getNumConstraints", &System::getNumConstraints)
+        .def("addConstraint", &System::addConstraint)
+        .def("getNumForces", &System::getNumForces)
+        // the System takes ownership of the force, as in OpenMM
+        .def("addForce", [](System& s, Force* f) { return s.addForce(f); }, py::keep_alive<1, 2>())
+        .def("getForce", (Force & (System::*)(int)) & System::getForce, py::return_value_policy::reference_internal)
+        .def("getDefaultPeriodicBoxVectors", [](const System& s) {
+            Vec3 a, b, c;
+            s.getDefaultPeriodicBoxVectors(a, b, c);
+            return py::make_tuple(py::make_tuple(a[0], a[1], a[2]), py::make_tuple(b[0], b[1], b[2]),
+                                  py::make_tuple(c[0], c[1], c[2]));
+        })
+        .def("setDefaultPeriodicBoxVectors", [](System& s, py::object a, py::object b, py::object c) {
+            s.setDefaultPeriodicBoxVectors(toVec3(a), toVec3(b), toVec3(c));
+        })
+        .def("usesPeriodicBoundaryConditions", &System::usesPeriodicBoundaryConditions);
+
+    py::class_<Integrator>(m, "Integrator")
+        .def("step", &Integrator::step)
+        .def("getStepSize", &Integrator::getStepSize)
+        .def("setStepSize", &Integrator::setStepSize)
+        .def("getIntegrationForceGroups", &Integrator::getIntegrationForceGroups)
+        .def("setIntegrationForceGroups", [](Integrator& i, py::object g) { i.setIntegrationForceGroups(groupMask(g)); });
+    py::class_<VerletIntegrator, Integrator>(m, "VerletIntegrator").def(py::init<double>());
+
+    py::class_<State>(m, "State")
+        .def("getTime", &State::getTime)
+        .def("getPotentialEnergy", &State::getPotentialEnergy)
+        .def("getKineticEnergy", &State::getKineticEnergy)
+        .def("getForces", [](const State& s, bool asNumpy) { return fromVec3List(s.getForces(), asNumpy); },
+             py::arg("asNumpy") = false)
+        .def("getPositions", [](const State& s, bool asNumpy) { return fromVec3List(s.getPositions(), asNumpy); },
+             py::arg("asNumpy") = false)
+        .def("getVelocities", [](const State& s, bool asNumpy) { return fromVec3List(s.getVelocities(), asNumpy); },
+             py::arg("asNumpy") = false)
+        .def("getParameters", &State::getParameters)
+        .def("getPeriodicBoxVectors", [](const State& s) {
+            Vec3 a, b, c;
+            s.getPeriodicBoxVectors(a, b, c);
+            return py::make_tuple(py::make_tuple(a[0], a[1], a[2]), py::make_tuple(b[0], b[1], b[2]),
+                                  py::make_tuple(c[0], c[1], c[2]));
+        });
+
+    py::class_<Context>(m, "Context")
+        .def(py::init([](const System& s, Integrator& i) { return new Context(s, i); }), py::keep_alive<1, 2>(),
+             py::keep_alive<1, 3>())
+        .def(py::init([](const System& s, Integrator& i, Platform& p) { return new Context(s, i, p); }),
+             py::keep_alive<1, 2>(), py::keep_alive<1, 3>())
+        .def(py::init([](const System& s, Integrator& i, Platform& p, const std::map<std::string, std::string>& props) {
+                 return new Context(s, i, p, props);
+             }),
+             py::keep_alive<1, 2>(), py::keep_alive<1, 3>())
+        .def("getSystem", &Context::getSystem, py::return_value_policy::reference_internal)
+        .def("getIntegrator", &Context::getIntegrator, py::return_value_policy::reference_internal)
+        .def("getPlatform", &Context::getPlatform, py::return_value_policy::reference)
+        .def("getState",
+             [](const Context& c, bool getPositions, bool getVelocities, bool getForces, bool getEnergy,
+                bool getParameters, bool enforcePeriodicBox, py::object groups) {
+                 int types = (getPositions ? State::Positions : 0) | (getVelocities ? State::Velocities : 0) |
+                             (getForces ? State::Forces : 0) | (getEnergy ? State::Energy : 0) |
+                             (getParameters ? State::Parameters : 0);
+                 return c.getState(types, enforcePeriodicBox, groupMask(groups));
+             },
+             py::arg("getPositions") = false, py::arg("getVelocities") = false, py::arg("getForces") = false,
+             py::arg("getEnergy") = false, py::arg("getParameters") = false, py::arg("enforcePeriodicBox") = false,
+             py::arg("groups") = py::none())
+        .def("setPositions", [](Context& c, py::object x) { c.setPositions(toVec3List(x)); })
+        .def("setVelocities", [](Context& c, py::object v) { c.setVelocities(toVec3List(v)); })
+        .def("setTime", &Context::setTime)
+        .def("getParameter", &Context::getParameter)
+        .def("setParameter", &Context::setParameter)
+        .def("getParameters", &Context::getParameters)
+        .def("setState", &Context::setState)
+        .def("setPeriodicBoxVectors", [](Context& c, py::object a, py::object b, py::object d) {
+            c.setPeriodicBoxVectors(toVec3(a), toVec3(b), toVec3(d));
+        });
+
+    // ---------------------------------------------------------------- ATMMetaForce
+    py::class_<ATMMetaForce, Force, std::unique_ptr<ATMMetaForce, py::nodelete>>(m, "ATMMetaForce")
+        .def(py::init([](double l1, double l2, double a, double u0, double w0, double umax, double ub, double ac,
+                         double dir, const std::vector<int>& groups) {
+                 return new ATMMetaForce(l1, l2, a, u0, w0, umax, ub, ac, dir, groups);
+             }),
+             py::arg("Lambda1"), py::arg("Lambda2"), py::arg("Alpha"), py::arg("U0"), py::arg("W0"), py::arg("Umax"),
+             py::arg("Ubcore"), py::arg("Acore"), py::arg("direction"), py::arg("VariableForceGroups"))
+        .def("getNumParticles", &ATMMetaForce::getNumParticles)
+        .def("addParticle", &ATMMetaForce::addParticle)
+        .def("setParticleParameters", &ATMMetaForce::setParticleParameters)
+        .def("getParticleParameters", [](const ATMMetaForce& f, int i) {
+            int p;
+            double dx, dy, dz;
+            f.getParticleParameters(i, p, dx, dy, dz);
+            return py::make_tuple(p, dx, dy, dz);
+        })
+        .def("updateParametersInContext", &ATMMetaForce::updateParametersInContext)
+        .def("getPerturbationEnergy", &ATMMetaForce::getPerturbationEnergy)
+        .def_static("Lambda1", &ATMMetaForce::Lambda1)
+        .def_static("Lambda2", &ATMMetaForce::Lambda2)
+        .def_static("Alpha", &ATMMetaForce::Alpha)
+        .def_static("U0", &ATMMetaForce::U0)
+        .def_static("W0", &ATMMetaForce::W0)
+        .def_static("Umax", &ATMMetaForce::Umax)
+        .def_static("Ubcore", &ATMMetaForce::Ubcore)
+        .def_static("Acore", &ATMMetaForce::Acore)
+        .def_static("Direction", &ATMMetaForce::Direction)
+        .def_static("Version", &ATMMetaForce::Version)
+        .def("getDefaultLambda1", &ATMMetaForce::getDefaultLambda1)
+        .def("getDefaultLambda2", &ATMMetaForce::getDefaultLambda2)
+        .def("getDefaultAlpha", &ATMMetaForce::getDefaultAlpha)
+        .def("getDefaultU0", &ATMMetaForce::getDefaultU0)
+        .def("getDefaultW0", &ATMMetaForce::getDefaultW0)
+        .def("getDefaultUmax", &ATMMetaForce::getDefaultUmax)
+        .def("getDefaultUbcore", &ATMMetaForce::getDefaultUbcore)
+        .def("getDefaultAcore", &ATMMetaForce::getDefaultAcore)
+        .def("getDefaultDirection", &ATMMetaForce::getDefaultDirection)
+        .def("getVariableForceGroups", &ATMMetaForce::getVariableForceGroups)
+        .def("usesPeriodicBoundaryConditions", &ATMMetaForce::usesPeriodicBoundaryConditions)
+        .def_static("cast", [](Force& f) -> ATMMetaForce& { return dynamic_cast<ATMMetaForce&>(f); },
+                    py::return_value_policy::reference)
+        .def_static("isinstance", [](Force& f) { return dynamic_cast<ATMMetaForce*>(&f) != nullptr; });
+
+    // ---------------------------------------------------------------- XML
+    py::class_<XmlSerializer>(m, "XmlSerializer")
+        .def_static("serialize", [](const ATMMetaForce& f) {
+            std::stringstream s;
+            XmlSerializer::serialize<ATMMetaForce>(&f, "Force", s);
+            return s.str();
+        })
+        .def_static("deserialize", [](const std::string& xml) {
+            std::stringstream s(xml);
+            return XmlSerializer::deserialize<ATMMetaForce>(s);
+        }, py::return_value_policy::take_ownership);
+}

@@ -1,0 +1,108 @@
+"""The drop-in boundary without a GPU: the ATMMetaForce API, its XML format,
+the plugin entry points and the platform registry."""
+import ctypes
+import os
+import subprocess
+
+import pytest
+
+import atmmetaforce as atm
+from atmmetaforce import openmm as mm
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(ROOT, "openmm-atmmetaforce-plugin_amd")
+LIB = os.path.join(PKG, "lib")
+
+
+def make_force():
+    f = atm.ATMMetaForce(0.0, 0.1, 0.25, 0.5, 0.6, 200.0, 100.0, 0.07, 0.0, [1])
+    f.setForceGroup(30)
+    f.setName("MyATMMetaForce")
+    f.addParticle(0, 0.1, 0.2, 0.3)
+    f.addParticle(2, 0.4, 0.5, 0.6)
+    return f
+
+
+def test_cpp_serialization_roundtrip(tmp_path):
+    """The reference's TestSerializeATMMetaForce values, as a C++ program."""
+    exe = tmp_path / "TestSerializeATMMetaForce"
+    inc = [f"-I{PKG}/{d}" for d in ("openmm_min/include", "openmmapi/include", "serialization/include")]
+    subprocess.run(["g++", "-std=c++17", "-O1", *inc, os.path.join(ROOT, "tests/cpp/TestSerializeATMMetaForce.cpp"),
+                    "-o", str(exe), f"-L{LIB}", "-lATMMetaForcePlugin", "-lOpenMMMin", f"-Wl,-rpath,{LIB}"],
+                   check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert out.stdout.strip() == "Done"
+
+
+def test_python_serialization_roundtrip():
+    f = make_force()
+    xml = mm.XmlSerializer.serialize(f)
+    assert xml.startswith('<?xml version="1.0" ?>\n<Force ')
+    assert 'type="ATMMetaForce"' in xml and 'version="0"' in xml
+    g = mm.XmlSerializer.deserialize(xml)
+    assert g.getName() == "MyATMMetaForce" and g.getForceGroup() == 30
+    assert [g.getDefaultLambda1(), g.getDefaultLambda2(), g.getDefaultAlpha(), g.getDefaultU0(), g.getDefaultW0(),
+            g.getDefaultUmax(), g.getDefaultUbcore(), g.getDefaultAcore(), g.getDefaultDirection()] == \
+        [0.0, 0.1, 0.25, 0.5, 0.6, 200.0, 100.0, 0.07, 0.0]
+    assert g.getVariableForceGroups() == [1]
+    assert g.getParticleParameters(0) == (0, 0.1, 0.2, 0.3)
+    assert g.getParticleParameters(1) == (2, 0.4, 0.5, 0.6)
+    assert mm.XmlSerializer.serialize(g) == xml
+
+
+def test_xml_layout_matches_reference_proxy():
+    """Element and attribute names of serialization/src/ATMMetaForceProxy.cpp:11-40."""
+    xml = mm.XmlSerializer.serialize(make_force())
+    for attr in ("forceGroup", "name", "lambda1", "lambda2", "alpha", "u0", "w0", "uMax", "ubCore", "aCore",
+                 "direction"):
+        assert f' {attr}="' in xml
+    assert "<VariableForceGroups>" in xml and '<Parameter group="1"/>' in xml
+    assert '<Particle dx="0.10000000000000001" dy="0.20000000000000001" dz="0.29999999999999999" particle="0"/>' in xml
+
+
+def test_names_version_and_defaults():
+    f = make_force()
+    assert [f.Lambda1(), f.Lambda2(), f.Alpha(), f.U0(), f.W0(), f.Umax(), f.Ubcore(), f.Acore(), f.Direction()] == \
+        ["ATMLambda1", "ATMLambda2", "ATMAlpha", "ATMU0", "ATMW0", "ATMUmax", "ATMUbcore", "ATMAcore", "ATMDirection"]
+    assert atm.ATMMetaForce.Version() == atm.ATMMETAFORCE_VERSION == "0.3.1"
+    assert f.getNumParticles() == 2
+    assert f.usesPeriodicBoundaryConditions() is False
+    f.setParticleParameters(1, 2, 1.0, 2.0, 3.0)
+    assert f.getParticleParameters(1) == (2, 1.0, 2.0, 3.0)
+    with pytest.raises(Exception, match="Index out of range"):
+        f.getParticleParameters(2)
+    with pytest.raises(Exception):
+        f.setParticleParameters(-1, 0, 0, 0, 0)
+
+
+def test_cast_and_isinstance():
+    s = mm.System()
+    s.addParticle(1.0)
+    nb = mm.NonbondedForce()
+    nb.addParticle(0.0, 0.3, 0.5)
+    s.addForce(nb)
+    s.addForce(make_force())
+    assert not atm.ATMMetaForce.isinstance(s.getForce(0))
+    assert atm.ATMMetaForce.isinstance(s.getForce(1))
+    assert atm.ATMMetaForce.cast(s.getForce(1)).getDefaultAcore() == 0.07
+
+
+def test_bad_xml_and_versions():
+    with pytest.raises(Exception, match="Unsupported version"):
+        mm.XmlSerializer.deserialize('<?xml version="1.0" ?>\n<Force type="ATMMetaForce" version="3"/>')
+    with pytest.raises(Exception):
+        mm.XmlSerializer.deserialize("<Force type=")
+
+
+def test_plugin_entry_points_and_platform():
+    lib = ctypes.CDLL(os.path.join(LIB, "libATMMetaForcePlugin.so"))
+    for name in ("registerPlatforms", "registerKernelFactories", "registerATMMetaForceMI355XKernelFactories",
+                 "registerSerializationProxies"):
+        assert hasattr(lib, name), name
+    p = mm.Platform.getPlatformByName("MI355X")
+    assert p.getName() == "MI355X"
+    assert set(p.getPropertyNames()) == {"DeviceIndex", "Precision", "ATMEvaluation"}
+    assert p.getPropertyDefaultValue("Precision") == "mixed"
+    with pytest.raises(Exception, match="no registered Platform"):
+        mm.Platform.getPlatformByName("CUDA")
