@@ -24,11 +24,17 @@ try:
     from . import openmm  # noqa: E402,F401
 except ImportError as _e:  # the C++ layer is not built: fail when it is used
     _import_error = _e
+else:
+    _import_error = None
 
-    def __getattr__(name):
-        if name in ("ATMMetaForce", "OpenMMException", "openmm"):
-            raise ImportError(f"atmmetaforce C++ module not built ({_import_error}); "
-                              "run __graft_entry__.build()")
-        raise AttributeError(name)
 
-__all__ = ["ATMMETAFORCE_VERSION", "ATMMetaForce", "ATMBatch", "OpenMMException", "workloads"]
+def __getattr__(name):
+    if name in ("ATMEnergyFunction", "atm_energy"):  # imports torch on first use
+        from . import autograd
+        return getattr(autograd, name)
+    if name in ("ATMMetaForce", "OpenMMException", "openmm") and _import_error is not None:
+        raise ImportError(f"atmmetaforce C++ module not built ({_import_error}); run __graft_entry__.build()")
+    raise AttributeError(name)
+
+__all__ = ["ATMMETAFORCE_VERSION", "ATMMetaForce", "ATMBatch", "ATMEnergyFunction", "OpenMMException", "atm_energy",
+           "workloads"]

@@ -52,6 +52,7 @@ class ATMBatch:
         h = ctypes.c_void_p()
         C.check(self._lib.atm_create(descs, n, device, ctypes.byref(h)))
         self._h = h
+        self._eval_id = 0
         self.num_atoms = [int(s.positions.shape[0]) for s in self.systems]
         for k, s in enumerate(self.systems):
             self.set_particles(k, s.charges, s.sigmas, s.epsilons, s.displacements)
@@ -129,6 +130,7 @@ class ATMBatch:
             flags |= C.ATM_EVAL_PLAIN
         if not graph:
             flags |= C.ATM_EVAL_NO_GRAPH
+        self._eval_id += 1
         C.check(self._lib.atm_eval(self._h, flags))
 
     def synchronize(self):

@@ -592,6 +592,7 @@ __global__ void __launch_bounds__(64 * W) k_pairs(DevPtrs P, int setBegin) {
                         // exact: the squares of 2^-20-grid differences fit in 53 bits
                         const double r2 = fma((double)dx, (double)dx, fma((double)dy, (double)dy, (double)dz * (double)dz));
                         const bool in = ok && r2 < rc2;
+                        if (__ballot(in) == 0ull) continue;  // no pair of this 8x8 step is inside the cutoff
                         const PairE64 pe = pair_term64(in, r2, id.x + jd.x, id.y * jd.y, id.z * jd.z, krf64);
                         eE += pe.e; eQQ += pe.qq; eQR2 += pe.qq * r2;
                         fs = (float)pe.fs;
@@ -599,6 +600,7 @@ __global__ void __launch_bounds__(64 * W) k_pairs(DevPtrs P, int setBegin) {
                         const float2 qi = sPrm[wave][t * kClusterSize + ii];
                         const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
                         const bool in = ok && in_cutoff(D, dx, dy, dz, r2);
+                        if (__ballot(in) == 0ull) continue;
                         const PairE pe = pair_term(in, r2, pi.w * pj.w, qi.x + qj.x, qi.y * qj.y, krf);
                         eE += (double)(pe.elj + pe.qr); eQQ += (double)pe.qq; eQR2 += (double)pe.qr2;
                         fs = pe.fs;
