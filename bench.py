@@ -78,6 +78,9 @@ def main():
                     help="column_scale,pair_waves_per_block,zbins_per_column (skips the autotuner)")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="first run a device-to-device copy of this many MiB five times (a kernel of known "
+                         "traffic for scaling the FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -87,6 +90,15 @@ def main():
     import numpy as np
     import torch
     from atmmetaforce import ATMBatch, workloads as W
+
+    if args.calibrate_copy_mib > 0:
+        n = args.calibrate_copy_mib * 1024 * 1024 // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"[bench] calibration copy: {args.calibrate_copy_mib} MiB read + {args.calibrate_copy_mib} MiB written, x5")
 
     log("[bench] generating workload")
     base = W.config2()
@@ -141,8 +153,13 @@ def main():
     F0 = batch.forces(0)
 
     pair_tflops = FLOP_PER_PAIR * pairs / (prof["pairs_ms"] * 1e-3) / 1e12
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tpath):  # HBM bytes per launch from the committed rocprofv3 --pmc summaries
+        with open(tpath) as fh:
+            traffic = json.load(fh).get("k_pairs", {}).get("hbm_bytes_per_launch")
     roofline = {"bound": "valu", "achieved": round(pair_tflops, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(pair_tflops / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                "frac": round(pair_tflops / PEAK_FP32_TFLOPS, 4), "traffic": traffic,
                 "kernel": "k_pairs (LJ + reaction-field pair kernel, all 44 coordinate sets per launch)",
                 "per_launch_flop": FLOP_PER_PAIR * pairs, "pairs_per_launch": pairs,
                 "avg_launch_ms": round(prof["pairs_ms"], 4)}

@@ -26,8 +26,10 @@
  *    Device memory is owned by the context.
  *  - A context is not thread-safe: calls on one context must not overlap,
  *    matching the single-threaded use of an OpenMM Context by the reference.
- *  - Forces accumulate in 64-bit fixed point with ATM_FORCE_SCALE units per
- *    kJ/mol/nm; the accumulated ATM force of a system is the sum over
+ *  - Pair forces accumulate in 64-bit fixed point (ATM_FORCE_SCALE units per
+ *    kJ/mol/nm; partial forces beyond 2^31 kJ/mol/nm in an FP64 side
+ *    accumulator), so their sums do not depend on summation order.  The
+ *    blended ATM force of a system is accumulated in FP64: it is the sum over
  *    evaluations since its last atm_clear_forces() (ATM_EVAL_CLEAR_FORCES
  *    clears it first), mirroring the reference's "+=" into the context force
  *    buffer (platforms/reference/src/ReferenceATMMetaForceKernels.cpp:100-110).
@@ -169,7 +171,8 @@ int atm_get_results(void* context, int32_t sys, atm_result* out);
 /* Accumulated ATM force of system `sys` (num_atoms*3 doubles, kJ/mol/nm). */
 int atm_get_forces(void* context, int32_t sys, double* forces);
 
-/* Accumulated ATM force as raw fixed-point integers (num_atoms*3). */
+/* Accumulated ATM force in fixed point (num_atoms*3, ATM_FORCE_SCALE units,
+ * rounded to nearest, saturated at the int64 range). */
 int atm_get_forces_fixed(void* context, int32_t sys, int64_t* forces);
 
 /* Zero the accumulated force of every system. */

@@ -296,7 +296,7 @@ void alloc_layout(Context& C) {
     P.prm64 = C.dPrm64.as<double4>();
     P.params = C.dParams.as<AtmParamsDev>();
     P.res = C.dRes.as<SysResult>();
-    P.outF = C.dOutF.as<long long>();
+    P.outF = C.dOutF.as<double>();
     P.bpos = C.dBpos.as<float4>();
     P.bcell = C.dBcell.as<int>();
     P.brank = C.dBrank.as<int>();
@@ -339,7 +339,7 @@ void enqueue_batched(Context& C, int flags, hipEvent_t* ev = nullptr) {
     const DevPtrs& P = C.P;
     const int S2 = 2 * C.nsys;
     if (ev) HIPCHK(hipEventRecord(ev[0], st));
-    if (flags & ATM_EVAL_CLEAR_FORCES) HIPCHK(launch_fill_i64(P.outF, 3 * C.totalAtoms, st));
+    if (flags & ATM_EVAL_CLEAR_FORCES) HIPCHK(launch_fill_zero(P.outF, 3 * C.totalAtoms, st));
     HIPCHK(launch_zero_counts(P, 0, (int)C.totalCells, 0, S2, st));
     HIPCHK(launch_build(P, 0, S2, C.maxAtoms, C.maxCells, C.maxSlots, st));
     if (ev) HIPCHK(hipEventRecord(ev[1], st));
@@ -394,7 +394,7 @@ void eval_plain(Context& C, int flags) {
     for (int s = 0; s < C.nsys; ++s) {
         const SetDesc& D0 = C.sets[2 * s];
         if (flags & ATM_EVAL_CLEAR_FORCES)
-            HIPCHK(launch_fill_i64(P.outF + 3ll * D0.atomOff, 3ll * D0.natoms, st));
+            HIPCHK(launch_fill_zero(P.outF + 3ll * D0.atomOff, 3ll * D0.natoms, st));
         double U[2];
         for (int stt = 0; stt < 2; ++stt) {
             const int c = 2 * s + stt;
@@ -427,7 +427,7 @@ void eval_state1(Context& C, int flags) {
     for (int s = 0; s < C.nsys; ++s) {
         const SetDesc& D = C.sets[2 * s];
         const int c = 2 * s;
-        if (flags & ATM_EVAL_CLEAR_FORCES) HIPCHK(launch_fill_i64(P.outF + 3ll * D.atomOff, 3ll * D.natoms, st));
+        if (flags & ATM_EVAL_CLEAR_FORCES) HIPCHK(launch_fill_zero(P.outF + 3ll * D.atomOff, 3ll * D.natoms, st));
         HIPCHK(launch_zero_counts(P, D.cellOff, D.cellOff + D.nx * D.ny * D.nz, c, c + 1, st));
         HIPCHK(launch_build(P, c, 1, D.natoms, D.nx * D.ny * D.nz, D.slotCap, st));
         HIPCHK(launch_pairlist(P, c, 1, D.groupCap, 4, st));
@@ -539,12 +539,12 @@ int atm_create(const atm_system_desc* systems, int32_t num_systems, int32_t devi
         C->dPrm64.alloc(sizeof(double4) * off);
         C->dParams.alloc(sizeof(AtmParamsDev) * num_systems);
         C->dRes.alloc(sizeof(SysResult) * num_systems);
-        C->dOutF.alloc(sizeof(long long) * 3 * off);
+        C->dOutF.alloc(sizeof(double) * 3 * off);
         HIPCHK(hipMemset(C->dPosq.p, 0, sizeof(float4) * off));
         HIPCHK(hipMemset(C->dDispl.p, 0, sizeof(float4) * off));
         HIPCHK(hipMemset(C->dLjp.p, 0, sizeof(float2) * off));
         HIPCHK(hipMemset(C->dPrm64.p, 0, sizeof(double4) * off));
-        HIPCHK(hipMemset(C->dOutF.p, 0, sizeof(long long) * 3 * off));
+        HIPCHK(hipMemset(C->dOutF.p, 0, sizeof(double) * 3 * off));
         HIPCHK(hipMemcpy(C->dParams.p, C->params.data(), sizeof(AtmParamsDev) * num_systems, hipMemcpyHostToDevice));
         layout(*C);
         alloc_layout(*C);
@@ -710,11 +710,14 @@ int atm_get_forces_fixed(void* context, int32_t sys, int64_t* f) {
         HIPCHK(hipSetDevice(C.device));
         fetch_results(C);
         const int n = C.sys[sys].num_atoms;
-        std::vector<long long> soa(3ll * n);
-        HIPCHK(hipMemcpy(soa.data(), C.dOutF.as<long long>() + 3ll * C.atomOff[sys], sizeof(long long) * 3 * n,
+        std::vector<double> soa(3ll * n);
+        HIPCHK(hipMemcpy(soa.data(), C.dOutF.as<double>() + 3ll * C.atomOff[sys], sizeof(double) * 3 * n,
                          hipMemcpyDeviceToHost));
         for (int i = 0; i < n; ++i)
-            for (int k = 0; k < 3; ++k) f[3 * i + k] = soa[(size_t)k * n + i];
+            for (int k = 0; k < 3; ++k) {
+                const double v = std::fmin(std::fmax(soa[(size_t)k * n + i] * ATM_FORCE_SCALE, -9.2e18), 9.2e18);
+                f[3 * i + k] = (int64_t)std::llrint(v);
+            }
     });
 }
 
@@ -726,12 +729,11 @@ int atm_get_forces(void* context, int32_t sys, double* f) {
         HIPCHK(hipSetDevice(C.device));
         fetch_results(C);
         const int n = C.sys[sys].num_atoms;
-        std::vector<long long> soa(3ll * n);
-        HIPCHK(hipMemcpy(soa.data(), C.dOutF.as<long long>() + 3ll * C.atomOff[sys], sizeof(long long) * 3 * n,
+        std::vector<double> soa(3ll * n);
+        HIPCHK(hipMemcpy(soa.data(), C.dOutF.as<double>() + 3ll * C.atomOff[sys], sizeof(double) * 3 * n,
                          hipMemcpyDeviceToHost));
-        const double inv = 1.0 / ATM_FORCE_SCALE;
         for (int i = 0; i < n; ++i)
-            for (int k = 0; k < 3; ++k) f[3 * i + k] = (double)soa[(size_t)k * n + i] * inv;
+            for (int k = 0; k < 3; ++k) f[3 * i + k] = soa[(size_t)k * n + i];
     });
 }
 
@@ -739,7 +741,7 @@ int atm_clear_forces(void* context) {
     return guarded([&] {
         Context& C = ctx_of(context);
         HIPCHK(hipSetDevice(C.device));
-        HIPCHK(launch_fill_i64(C.P.outF, 3 * C.totalAtoms, C.stream));
+        HIPCHK(launch_fill_zero(C.P.outF, 3 * C.totalAtoms, C.stream));
         HIPCHK(hipStreamSynchronize(C.stream));
     });
 }

@@ -17,7 +17,7 @@ struct DevPtrs {
     const double4* prm64;  // (sigma/2, 2 sqrt(eps), q sqrt(k_e), 0) in FP64
     const AtmParamsDev* params;  // [S]
     SysResult* res;        // [S]
-    long long* outF;       // accumulated ATM force, SoA per system at 3*atomOff
+    double* outF;          // accumulated ATM force (kJ/mol/nm), SoA per system at 3*atomOff
     // per set, unsorted order (setAtomOff)
     float4* bpos;
     int* bcell;
@@ -65,6 +65,7 @@ hipError_t launch_set_energy(const DevPtrs& P, int c, hipStream_t st);
 hipError_t launch_blend(const DevPtrs& P, int sysBegin, int nsys, int maxAtoms, int hostMode, double sp, double dir,
                         hipStream_t st);
 hipError_t launch_fill_i64(long long* p, long long n, hipStream_t st);
+hipError_t launch_fill_zero(void* p, long long n8, hipStream_t st);
 hipError_t launch_pairs_dump(const DevPtrs& P, int c, int maxGroups, int2* out, long long cap,
                              unsigned long long* counters, hipStream_t st);
 

@@ -57,12 +57,6 @@ __device__ __forceinline__ long long to_fixed(float f) {
     return __double2ll_rn(d);
 }
 
-// kJ/mol/nm -> fixed point, saturating at the int64 range (blended output).
-__device__ __forceinline__ long long to_fixed_sat(double f) {
-    const double d = fmin(fmax(f * kForceScale, -9.2e18), 9.2e18);
-    return __double2ll_rn(d);
-}
-
 __device__ __forceinline__ void atomic_add_fixed(long long* p, long long v) {
     atomicAdd(reinterpret_cast<unsigned long long*>(p), static_cast<unsigned long long>(v));
 }
@@ -214,27 +208,49 @@ __global__ void k_bin(DevPtrs P, int setBegin) {
     P.brank[k] = r;
 }
 
-// One block of 256 threads per set.  Each thread scans a contiguous chunk of
-// columns; a block scan joins the chunks.
+__device__ __forceinline__ int wave_incl_scan(int v) {
+    const int lane = threadIdx.x & 63;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int u = __shfl_up(v, o, 64);
+        if (lane >= o) v += u;
+    }
+    return v;
+}
+
+// One block of 256 threads (4 waves) per set.
+//  A: one wave per column, wave prefix sums over its z-bins -> cell starts
+//     relative to the column and the column count;
+//  B: each thread scans a contiguous chunk of columns (padded slot counts and
+//     group counts), a block scan joins the chunks; column starts, dummy
+//     slots and the group table are written;
+//  C: one wave per column adds the column start to its cell starts.
 __global__ void __launch_bounds__(256) k_scan(DevPtrs P, int setBegin) {
     const int c = setBegin + blockIdx.x;
     const SetDesc& D = P.desc[c];
-    const int t = threadIdx.x;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int ncols = D.nx * D.ny, nz = D.nz;
+    for (int col = wave; col < ncols; col += 4) {
+        const int* cnt = P.cellCount + D.cellOff + (size_t)col * nz;
+        int* st = P.cellStart + D.cellOff + (size_t)col * nz;
+        int run = 0;
+        for (int z0 = 0; z0 < nz; z0 += 64) {
+            const int z = z0 + lane;
+            const int v = z < nz ? cnt[z] : 0;
+            const int incl = wave_incl_scan(v);
+            if (z < nz) st[z] = run + incl - v;
+            run += __shfl(incl, 63, 64);
+        }
+        if (lane == 0) P.colCount[D.colOff + col] = run;
+    }
+    __syncthreads();
     const int chunk = (ncols + 255) / 256;
     const int b = min(t * chunk, ncols), e = min(b + chunk, ncols);
     __shared__ int sS[256], sG[256];
     int accS = 0, accG = 0;
     for (int col = b; col < e; ++col) {
-        const int* cnt = P.cellCount + D.cellOff + (size_t)col * nz;
-        int* st = P.cellStart + D.cellOff + (size_t)col * nz;
-        int s = 0;
-        for (int z = 0; z < nz; ++z) { st[z] = s; s += cnt[z]; }
-        P.colCount[D.colOff + col] = s;
-        int padded = (s + 7) & ~7;
-        int ncl = padded >> 3;
+        const int padded = (P.colCount[D.colOff + col] + 7) & ~7;
         accS += padded;
-        accG += (ncl + kGroupClusters - 1) / kGroupClusters;
+        accG += ((padded >> 3) + kGroupClusters - 1) / kGroupClusters;
     }
     sS[t] = accS; sG[t] = accG;
     __syncthreads();
@@ -246,13 +262,11 @@ __global__ void __launch_bounds__(256) k_scan(DevPtrs P, int setBegin) {
     }
     int offS = sS[t] - accS, offG = sG[t] - accG;
     for (int col = b; col < e; ++col) {
-        int* st = P.cellStart + D.cellOff + (size_t)col * nz;
         const int cnt = P.colCount[D.colOff + col];
         const int padded = (cnt + 7) & ~7;
         const int ncl = padded >> 3;
         const int ng = (ncl + kGroupClusters - 1) / kGroupClusters;
         P.colStart[D.colOff + col] = offS;
-        for (int z = 0; z < nz; ++z) st[z] += offS;
         for (int s = offS + cnt; s < offS + padded; ++s) P.sidx[D.slotOff + s] = -1;
         for (int gg = 0; gg < ng; ++gg)
             P.ginfo[D.groupOff + offG + gg] =
@@ -264,6 +278,12 @@ __global__ void __launch_bounds__(256) k_scan(DevPtrs P, int setBegin) {
         P.rt[c].nslots = sS[255];
         P.rt[c].nclusters = sS[255] >> 3;
         P.rt[c].ngroups = sG[255];
+    }
+    __syncthreads();
+    for (int col = wave; col < ncols; col += 4) {
+        const int base = P.colStart[D.colOff + col];
+        int* st = P.cellStart + D.cellOff + (size_t)col * nz;
+        for (int z = lane; z < nz; z += 64) st[z] += base;
     }
 }
 
@@ -353,14 +373,6 @@ __global__ void __launch_bounds__(256) k_gather_bbox(DevPtrs P, int setBegin) {
 
 // ---------------------------------------------------------------- pair list
 
-__device__ __forceinline__ int wave_incl_scan(int v) {
-    const int lane = threadIdx.x & 63;
-    for (int o = 1; o < 64; o <<= 1) {
-        const int u = __shfl_up(v, o, 64);
-        if (lane >= o) v += u;
-    }
-    return v;
-}
 
 // One wave64 per group.  Entry = J (20 bits) | imask (8 bits) << 20 |
 // x-shift code << 28 | y-shift code << 30 (code 1: +L, 2: -L).
@@ -737,21 +749,21 @@ __global__ void k_blend(DevPtrs P, int sysBegin, int hostMode, double spHost, do
     const double w2 = dir > 0.0 ? sp : 1.0 - sp;
     const double w1 = dir > 0.0 ? 1.0 - sp : sp;
     const int a = D0.slotOff + P.slotOf[D0.setAtomOff + i];
-    long long* F = P.outF + 3ll * D0.atomOff;
+    double* F = P.outF + 3ll * D0.atomOff;
     const double inv = 1.0 / kForceScale;
     if (hostMode == 2) {  // state 1 only: F += F1 (the state-2 set was not evaluated)
-        F[i] += to_fixed_sat((double)P.sfx[a] * inv + P.sdx[a]);
-        F[N + i] += to_fixed_sat((double)P.sfy[a] * inv + P.sdy[a]);
-        F[2 * N + i] += to_fixed_sat((double)P.sfz[a] * inv + P.sdz[a]);
+        F[i] += (double)P.sfx[a] * inv + P.sdx[a];
+        F[N + i] += (double)P.sfy[a] * inv + P.sdy[a];
+        F[2 * N + i] += (double)P.sfz[a] * inv + P.sdz[a];
         return;
     }
     const int b = D1.slotOff + P.slotOf[D1.setAtomOff + i];
     const double f1x = (double)P.sfx[a] * inv + P.sdx[a], f2x = (double)P.sfx[b] * inv + P.sdx[b];
     const double f1y = (double)P.sfy[a] * inv + P.sdy[a], f2y = (double)P.sfy[b] * inv + P.sdy[b];
     const double f1z = (double)P.sfz[a] * inv + P.sdz[a], f2z = (double)P.sfz[b] * inv + P.sdz[b];
-    F[i] += to_fixed_sat(w1 * f1x + w2 * f2x);
-    F[N + i] += to_fixed_sat(w1 * f1y + w2 * f2y);
-    F[2 * N + i] += to_fixed_sat(w1 * f1z + w2 * f2z);
+    F[i] += w1 * f1x + w2 * f2x;
+    F[N + i] += w1 * f1y + w2 * f2y;
+    F[2 * N + i] += w1 * f1z + w2 * f2z;
 }
 
 __global__ void k_fill_i64(long long* p, long long n) {
@@ -829,6 +841,11 @@ hipError_t launch_fill_i64(long long* p, long long n, hipStream_t st) {
     unsigned blocks = (unsigned)std::min<long long>(cdiv(n, 256), 4096);
     k_fill_i64<<<blocks, 256, 0, st>>>(p, n);
     return hipGetLastError();
+}
+
+// Zero n8 8-byte words (a fill kernel: safe inside captured graphs).
+hipError_t launch_fill_zero(void* p, long long n8, hipStream_t st) {
+    return launch_fill_i64(static_cast<long long*>(p), n8, st);
 }
 
 hipError_t launch_pairs_dump(const DevPtrs& P, int c, int maxGroups, int2* out, long long cap,

@@ -120,8 +120,11 @@ PYBIND11_MODULE(_atmmetaforce, m) {
         .def("setParticleParameters", &NonbondedForce::setParticleParameters)
         .def("addException", &NonbondedForce::addException, py::arg("particle1"), py::arg("particle2"),
              py::arg("chargeProd"), py::arg("sigma"), py::arg("epsilon"), py::arg("replace") = false)
-        .def("getNonbondedMethod", &NonbondedForce::getNonbondedMethod)
-        .def("setNonbondedMethod", &NonbondedForce::setNonbondedMethod)
+        // methods are plain ints, as NonbondedForce.CutoffPeriodic etc. are in OpenMM's Python API
+        .def("getNonbondedMethod", [](const NonbondedForce& f) { return (int)f.getNonbondedMethod(); })
+        .def("setNonbondedMethod", [](NonbondedForce& f, int m) {
+            f.setNonbondedMethod((NonbondedForce::NonbondedMethod)m);
+        })
         .def("getCutoffDistance", &NonbondedForce::getCutoffDistance)
         .def("setCutoffDistance", &NonbondedForce::setCutoffDistance)
         .def("getReactionFieldDielectric", &NonbondedForce::getReactionFieldDielectric)
@@ -137,13 +140,6 @@ PYBIND11_MODULE(_atmmetaforce, m) {
     nb.attr("Ewald") = (int)NonbondedForce::Ewald;
     nb.attr("PME") = (int)NonbondedForce::PME;
     nb.attr("LJPME") = (int)NonbondedForce::LJPME;
-    py::enum_<NonbondedForce::NonbondedMethod>(nb, "NonbondedMethod")
-        .value("NoCutoff", NonbondedForce::NoCutoff)
-        .value("CutoffNonPeriodic", NonbondedForce::CutoffNonPeriodic)
-        .value("CutoffPeriodic", NonbondedForce::CutoffPeriodic)
-        .value("Ewald", NonbondedForce::Ewald)
-        .value("PME", NonbondedForce::PME)
-        .value("LJPME", NonbondedForce::LJPME);
 
     py::class_<System>(m, "System")
         .def(py::init<>())

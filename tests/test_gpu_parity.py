@@ -161,11 +161,11 @@ def test_force_accumulation_and_energy_only(gpu):
     s = W.tiny_system(400, 2.4, 0.8, 6, (0.9, 0.9, 0.9), 9)
     with ATMBatch([s]) as b:
         b.compute()
-        f1 = b.forces_fixed(0)
+        f1 = b.forces(0)
         e1 = b.result(0)["energy"]
         b.evaluate(forces=True, clear=False)
-        f2 = b.forces_fixed(0)
-        assert np.array_equal(f2, 2 * f1)
+        f2 = b.forces(0)
+        assert np.array_equal(f2, 2 * f1)  # x + x is exact in FP64
         b.evaluate(forces=False, clear=True)
         assert b.result(0)["energy"] == pytest.approx(e1, rel=1e-12)
         assert not b.forces_fixed(0).any()
@@ -176,10 +176,10 @@ def test_graph_replays_match_eager(gpu):
     with ATMBatch([s]) as b:
         b.compute(graph=False)
         e0 = b.result(0)
-        f0 = b.forces_fixed(0)
+        f0 = b.forces(0)
         for _ in range(3):
             b.compute(graph=True)
             assert b.result(0)["state1_energy"] == pytest.approx(e0["state1_energy"], rel=1e-13)
             assert b.result(0)["state2_energy"] == pytest.approx(e0["state2_energy"], rel=1e-13)
-            assert np.array_equal(b.forces_fixed(0), f0)
+            assert np.array_equal(b.forces(0), f0)
             b.synchronize()
