@@ -75,7 +75,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--tuning", default=None,
-                    help="column_scale,pair_waves_per_block,zbins_per_column (skips the autotuner)")
+                    help="column_scale,pair_waves_per_block,zbins_per_column[,pair_kernel] (skips the autotuner)")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
@@ -107,8 +107,8 @@ def main():
     batch.compute()
     tuning = batch.tuning()
     if args.tuning:
-        cs, wv, zp = args.tuning.split(",")
-        batch.set_tuning(float(cs), int(wv), int(zp))
+        f = args.tuning.split(",")
+        batch.set_tuning(float(f[0]), int(f[1]), int(f[2]), int(f[3]) if len(f) > 3 else 2)
         batch.compute()
         tuning = batch.tuning()
     elif not args.no_autotune:

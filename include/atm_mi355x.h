@@ -43,7 +43,7 @@
 extern "C" {
 #endif
 
-#define ATM_MI355X_ABI_VERSION 1
+#define ATM_MI355X_ABI_VERSION 2
 
 /* status codes */
 #define ATM_OK 0
@@ -107,6 +107,10 @@ typedef struct atm_tuning {
     int32_t pair_waves_per_block;
     int32_t zbins_per_column; /* z-bins per column edge */
     double best_ms;           /* measured time of the chosen configuration */
+    int32_t pair_kernel;      /* 0: 8x8 cluster-pair tiles; 1, 2, 3: compacted in-cutoff pairs
+                                 (FP64 pair function only) with 8, 16, 16 list entries per
+                                 batch, 3 drawing each chunk from 4 x 64 pending pairs */
+    int32_t reserved;
 } atm_tuning;
 
 /* Library version string and ABI number. */
@@ -197,7 +201,7 @@ int atm_backward(void* context, const double* grad_energy, double* d_charges,
 int atm_set_stream(void* context, void* hip_stream);
 
 /* Time the candidate launch configurations on the current inputs and keep the
- * fastest (cell shape, z-binning, waves per block of the pair kernel). */
+ * fastest (cell shape, z-binning, pair-kernel variant and waves per block). */
 int atm_autotune(void* context, int32_t repeats, atm_tuning* out);
 int atm_get_tuning(void* context, atm_tuning* out);
 int atm_set_tuning(void* context, const atm_tuning* tuning);

@@ -203,15 +203,20 @@ class ATMBatch:
     def autotune(self, repeats: int = 3) -> dict:
         t = C.Tuning()
         C.check(self._lib.atm_autotune(self._h, repeats, ctypes.byref(t)))
-        return {"column_scale": t.column_scale, "pair_waves_per_block": t.pair_waves_per_block,
-                "zbins_per_column": t.zbins_per_column, "best_ms": t.best_ms}
+        return _tuning_dict(t)
 
     def tuning(self) -> dict:
         t = C.Tuning()
         C.check(self._lib.atm_get_tuning(self._h, ctypes.byref(t)))
-        return {"column_scale": t.column_scale, "pair_waves_per_block": t.pair_waves_per_block,
-                "zbins_per_column": t.zbins_per_column, "best_ms": t.best_ms}
+        return _tuning_dict(t)
 
-    def set_tuning(self, column_scale=1.0, pair_waves_per_block=4, zbins_per_column=4):
-        t = C.Tuning(column_scale, pair_waves_per_block, zbins_per_column, 0.0)
+    def set_tuning(self, column_scale=1.0, pair_waves_per_block=2, zbins_per_column=4, pair_kernel=2):
+        """pair_kernel 0: 8x8 cluster-pair tiles; 1, 2, 3: compacted in-cutoff pairs with 8, 16, 16
+        list entries per batch (3: each chunk drawn from 4 x 64 pending pairs)."""
+        t = C.Tuning(column_scale, pair_waves_per_block, zbins_per_column, 0.0, pair_kernel, 0)
         C.check(self._lib.atm_set_tuning(self._h, ctypes.byref(t)))
+
+
+def _tuning_dict(t) -> dict:
+    return {"column_scale": t.column_scale, "pair_waves_per_block": t.pair_waves_per_block,
+            "zbins_per_column": t.zbins_per_column, "pair_kernel": t.pair_kernel, "best_ms": t.best_ms}

@@ -1,12 +1,13 @@
 """Minimal OpenMM-compatible classes (the subset the ATM force is used with).
 
 Names follow the `openmm` package so scripts read the same: System,
-NonbondedForce, Context, State, Platform, VerletIntegrator, XmlSerializer.
+NonbondedForce, MonteCarloBarostat (a holder of its two global parameters),
+Context, State, Platform, VerletIntegrator, XmlSerializer.
 Positions and forces are (N, 3) arrays or lists of 3-tuples in nm and
 kJ/mol/nm; `groups` arguments accept a set of force-group indices or a bit mask.
 """
-from ._atmmetaforce import (Context, Force, Integrator, NonbondedForce, OpenMMException,  # noqa: F401
-                            Platform, State, System, VerletIntegrator, XmlSerializer)
+from ._atmmetaforce import (Context, Force, Integrator, MonteCarloBarostat, NonbondedForce,  # noqa: F401
+                            OpenMMException, Platform, State, System, VerletIntegrator, XmlSerializer)
 
-__all__ = ["Context", "Force", "Integrator", "NonbondedForce", "OpenMMException", "Platform", "State", "System",
-           "VerletIntegrator", "XmlSerializer"]
+__all__ = ["Context", "Force", "Integrator", "MonteCarloBarostat", "NonbondedForce", "OpenMMException", "Platform",
+           "State", "System", "VerletIntegrator", "XmlSerializer"]

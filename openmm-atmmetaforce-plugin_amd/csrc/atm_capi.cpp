@@ -82,8 +82,9 @@ struct DevBuf {
 
 struct Tuning {
     double colScale = 1.0;
-    int pairWaves = 4;
+    int pairWaves = 2;
     int zbinsPerColumn = 4;
+    int pairKernel = 2;
 };
 
 struct Context {
@@ -105,7 +106,7 @@ struct Context {
     // device buffers
     DevBuf dDesc, dRt, dPosq, dDispl, dLjp, dPrm64, dParams, dRes, dOutF;
     DevBuf dBpos, dBcell, dBrank, dSlotOf, dCellCount, dCellStart, dColStart, dColCount;
-    DevBuf dSpos, dSprm, dSprm64, dSidx, dSfx, dSfy, dSfz, dSdx, dSdy, dSdz, dBbc, dBbh, dGinfo, dGcount, dGenergy, dEntries, dSetEnergy;
+    DevBuf dSpos, dSprm, dSprm64, dSidx, dSf, dSd, dBbc, dBbh, dGinfo, dGcount, dGenergy, dEntries, dSetEnergy;
     DevBuf dGq, dGsig, dGeps, dWset, dDq, dDsig, dDeps;
     DevPtrs P{};
 
@@ -268,12 +269,9 @@ void alloc_layout(Context& C) {
     C.dSprm.alloc(sizeof(float2) * C.totalSlots);
     C.dSprm64.alloc(sizeof(double4) * C.totalSlots);
     C.dSidx.alloc(sizeof(int) * C.totalSlots);
-    C.dSfx.alloc(sizeof(long long) * C.totalSlots);
-    C.dSfy.alloc(sizeof(long long) * C.totalSlots);
-    C.dSfz.alloc(sizeof(long long) * C.totalSlots);
-    C.dSdx.alloc(sizeof(double) * C.totalSlots);
-    C.dSdy.alloc(sizeof(double) * C.totalSlots);
-    C.dSdz.alloc(sizeof(double) * C.totalSlots);
+    C.dSf.alloc(sizeof(long long) * 3 * C.totalSlots);
+    C.dSd.alloc(sizeof(double) * 3 * C.totalSlots);
+    HIPCHK(hipMemset(C.dSd.p, 0, sizeof(double) * 3 * C.totalSlots));  // side sums start clear
     C.dBbc.alloc(sizeof(float4) * C.totalClusters);
     C.dBbh.alloc(sizeof(float4) * C.totalClusters);
     C.dGinfo.alloc(sizeof(int4) * C.totalGroups);
@@ -309,12 +307,8 @@ void alloc_layout(Context& C) {
     P.sprm = C.dSprm.as<float2>();
     P.sprm64 = C.dSprm64.as<double4>();
     P.sidx = C.dSidx.as<int>();
-    P.sfx = C.dSfx.as<long long>();
-    P.sfy = C.dSfy.as<long long>();
-    P.sfz = C.dSfz.as<long long>();
-    P.sdx = C.dSdx.as<double>();
-    P.sdy = C.dSdy.as<double>();
-    P.sdz = C.dSdz.as<double>();
+    P.sf = C.dSf.as<long long>();
+    P.sd = C.dSd.as<double>();
     P.bbc = C.dBbc.as<float4>();
     P.bbh = C.dBbh.as<float4>();
     P.ginfo = C.dGinfo.as<int4>();
@@ -346,7 +340,7 @@ void enqueue_batched(Context& C, int flags, hipEvent_t* ev = nullptr) {
     HIPCHK(launch_pairlist(P, 0, S2, C.maxGroups, 4, st));
     if (ev) HIPCHK(hipEventRecord(ev[2], st));
     HIPCHK(launch_pairs(P, 0, S2, C.maxGroups, C.tuning.pairWaves, (flags & ATM_EVAL_FORCES) != 0,
-                        (flags & ATM_EVAL_FP32_PAIRS) == 0, st));
+                        (flags & ATM_EVAL_FP32_PAIRS) == 0, C.tuning.pairKernel, st));
     if (ev) HIPCHK(hipEventRecord(ev[3], st));
     HIPCHK(launch_energy(P, 0, C.nsys, st));
     if (ev) HIPCHK(hipEventRecord(ev[4], st));
@@ -402,7 +396,8 @@ void eval_plain(Context& C, int flags) {
             HIPCHK(launch_zero_counts(P, D.cellOff, D.cellOff + D.nx * D.ny * D.nz, c, c + 1, st));
             HIPCHK(launch_build(P, c, 1, D.natoms, D.nx * D.ny * D.nz, D.slotCap, st));
             HIPCHK(launch_pairlist(P, c, 1, D.groupCap, 4, st));
-            HIPCHK(launch_pairs(P, c, 1, D.groupCap, C.tuning.pairWaves, true, (flags & ATM_EVAL_FP32_PAIRS) == 0, st));
+            HIPCHK(launch_pairs(P, c, 1, D.groupCap, C.tuning.pairWaves, true, (flags & ATM_EVAL_FP32_PAIRS) == 0,
+                                C.tuning.pairKernel, st));
             HIPCHK(launch_set_energy(P, c, st));
             HIPCHK(hipMemcpyAsync(&U[stt], P.setEnergy + c, sizeof(double), hipMemcpyDeviceToHost, st));
             HIPCHK(hipStreamSynchronize(st));
@@ -432,7 +427,7 @@ void eval_state1(Context& C, int flags) {
         HIPCHK(launch_build(P, c, 1, D.natoms, D.nx * D.ny * D.nz, D.slotCap, st));
         HIPCHK(launch_pairlist(P, c, 1, D.groupCap, 4, st));
         HIPCHK(launch_pairs(P, c, 1, D.groupCap, C.tuning.pairWaves, (flags & ATM_EVAL_FORCES) != 0,
-                            (flags & ATM_EVAL_FP32_PAIRS) == 0, st));
+                            (flags & ATM_EVAL_FP32_PAIRS) == 0, C.tuning.pairKernel, st));
         HIPCHK(launch_set_energy(P, c, st));
         double U = 0.0;
         HIPCHK(hipMemcpyAsync(&U, P.setEnergy + c, sizeof(double), hipMemcpyDeviceToHost, st));
@@ -822,6 +817,8 @@ int atm_get_tuning(void* context, atm_tuning* out) {
         out->pair_waves_per_block = C.tuning.pairWaves;
         out->zbins_per_column = C.tuning.zbinsPerColumn;
         out->best_ms = C.tunedMs;
+        out->pair_kernel = C.tuning.pairKernel;
+        out->reserved = 0;
     });
 }
 
@@ -834,12 +831,16 @@ int atm_set_tuning(void* context, const atm_tuning* t) {
             t->pair_waves_per_block != 8)
             fail(ATM_ERR_INVALID_ARGUMENT, "pair_waves_per_block must be 1, 2, 4 or 8");
         if (t->zbins_per_column < 1 || t->zbins_per_column > 16) fail(ATM_ERR_INVALID_ARGUMENT, "zbins_per_column must be in [1, 16]");
+        if (t->pair_kernel < 0 || t->pair_kernel > 3) fail(ATM_ERR_INVALID_ARGUMENT, "pair_kernel must be 0, 1, 2 or 3");
+        if (t->pair_kernel != 0 && t->pair_waves_per_block == 8)
+            fail(ATM_ERR_INVALID_ARGUMENT, "the compacted pair kernel takes 1, 2 or 4 waves per block");
         HIPCHK(hipSetDevice(C.device));
         HIPCHK(hipStreamSynchronize(C.stream));
         const bool relayout = t->column_scale != C.tuning.colScale || t->zbins_per_column != C.tuning.zbinsPerColumn;
         C.tuning.colScale = t->column_scale;
         C.tuning.zbinsPerColumn = t->zbins_per_column;
         C.tuning.pairWaves = t->pair_waves_per_block;
+        C.tuning.pairKernel = t->pair_kernel;
         if (relayout) {
             layout(C);
             alloc_layout(C);
@@ -859,6 +860,7 @@ int atm_autotune(void* context, int32_t repeats, atm_tuning* out) {
         const int reps = std::max(1, (int)repeats);
         const double scales[] = {0.8, 0.9, 1.0, 1.15, 1.3};
         const int waves[] = {1, 2, 4};
+        const int kernels[] = {0, 1, 2, 3};
         const int zpcs[] = {2, 4};
         hipEvent_t e0, e1;
         HIPCHK(hipEventCreate(&e0));
@@ -873,8 +875,9 @@ int atm_autotune(void* context, int32_t repeats, atm_tuning* out) {
                 C.entryCapOverride.assign(2 * C.nsys, 0);
                 layout(C);
                 alloc_layout(C);
-                for (int w : waves) {
-                    C.tuning.pairWaves = w;
+                for (int kv = 0; kv < 4 * 3; ++kv) {
+                    C.tuning.pairKernel = kernels[kv / 3];
+                    C.tuning.pairWaves = waves[kv % 3];
                     C.dropGraphs();
                     bool ok = true;
                     for (int attempt = 0; attempt < 3; ++attempt) {  // grow list capacity if needed
@@ -920,6 +923,8 @@ int atm_autotune(void* context, int32_t repeats, atm_tuning* out) {
             out->pair_waves_per_block = best.pairWaves;
             out->zbins_per_column = best.zbinsPerColumn;
             out->best_ms = bestMs;
+            out->pair_kernel = best.pairKernel;
+            out->reserved = 0;
         }
     });
 }

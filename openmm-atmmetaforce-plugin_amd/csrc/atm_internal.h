@@ -53,6 +53,8 @@ struct SetDesc {
 
 struct SetRuntime {
     int nslots, nclusters, ngroups, overflow;
+    int big;   // a partial force of this evaluation went to the FP64 side sums
+    int pad_[3];
 };
 
 // Per-system state of the last evaluation on the device (mirrors atm_result).

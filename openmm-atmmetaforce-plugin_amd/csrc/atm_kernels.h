@@ -33,12 +33,9 @@ struct DevPtrs {
     float2* sprm;
     double4* sprm64;
     int* sidx;
-    long long* sfx;
-    long long* sfy;
-    long long* sfz;
-    double* sdx;           // FP64 side accumulators for partial forces >= kFixedMax
-    double* sdy;
-    double* sdz;
+    long long* sf;         // 64-bit fixed-point pair-force sums, (x, y, z) per slot
+    double* sd;            // FP64 side sums, (x, y, z) per slot, of partial forces beyond the
+                           // fixed-point range; read and cleared only when rt[c].big is set
     // clusters
     float4* bbc;
     float4* bbh;
@@ -58,8 +55,10 @@ hipError_t launch_zero_counts(const DevPtrs& P, int cellBegin, int cellEnd, int 
 hipError_t launch_build(const DevPtrs& P, int setBegin, int nsets, int maxAtoms, int maxCells, int maxSlots,
                         hipStream_t st);
 hipError_t launch_pairlist(const DevPtrs& P, int setBegin, int nsets, int maxGroups, int listWaves, hipStream_t st);
+// variant 0: 8x8 tile kernel; compacted kernel (FP64 pairs only) with 1: 8 list entries per batch,
+// 2: 16 entries, 3: 16 entries and chunks strided over 4 x 64 pending pairs
 hipError_t launch_pairs(const DevPtrs& P, int setBegin, int nsets, int maxGroups, int waves, bool forces, bool fp64,
-                        hipStream_t st);
+                        int variant, hipStream_t st);
 hipError_t launch_energy(const DevPtrs& P, int sysBegin, int nsys, hipStream_t st);
 hipError_t launch_set_energy(const DevPtrs& P, int c, hipStream_t st);
 hipError_t launch_blend(const DevPtrs& P, int sysBegin, int nsys, int maxAtoms, int hostMode, double sp, double dir,

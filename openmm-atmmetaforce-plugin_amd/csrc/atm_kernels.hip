@@ -63,10 +63,14 @@ __device__ __forceinline__ void atomic_add_fixed(long long* p, long long v) {
 
 // Add a partial force to a slot: 64-bit fixed point below kFixedMax, FP64 side
 // accumulator above (clashing pairs of the displaced state).
-__device__ __forceinline__ void add_force(const DevPtrs& P, int slot, float fx, float fy, float fz) {
-    if (fabsf(fx) < kFixedMax) atomic_add_fixed(P.sfx + slot, to_fixed(fx)); else atomicAdd(P.sdx + slot, (double)fx);
-    if (fabsf(fy) < kFixedMax) atomic_add_fixed(P.sfy + slot, to_fixed(fy)); else atomicAdd(P.sdy + slot, (double)fy);
-    if (fabsf(fz) < kFixedMax) atomic_add_fixed(P.sfz + slot, to_fixed(fz)); else atomicAdd(P.sdz + slot, (double)fz);
+__device__ __forceinline__ void add_force(const DevPtrs& P, int c, int slot, float fx, float fy, float fz) {
+    long long* f = P.sf + 3ll * slot;
+    double* d = P.sd + 3ll * slot;
+    bool big = false;
+    if (fabsf(fx) < kFixedMax) atomic_add_fixed(f, to_fixed(fx)); else { atomicAdd(d, (double)fx); big = true; }
+    if (fabsf(fy) < kFixedMax) atomic_add_fixed(f + 1, to_fixed(fy)); else { atomicAdd(d + 1, (double)fy); big = true; }
+    if (fabsf(fz) < kFixedMax) atomic_add_fixed(f + 2, to_fixed(fz)); else { atomicAdd(d + 2, (double)fz); big = true; }
+    if (big) P.rt[c].big = 1;
 }
 
 // DPP-fused sum over each aligned group of 8 lanes (xor 1, xor 2, half-row mirror).
@@ -217,19 +221,20 @@ __device__ __forceinline__ int wave_incl_scan(int v) {
     return v;
 }
 
-// One block of 256 threads (4 waves) per set.
-//  A: one wave per column, wave prefix sums over its z-bins -> cell starts
-//     relative to the column and the column count;
+// One block of 1024 threads (16 waves) per set.
+//  A: one wave per column (16 columns in flight), wave prefix sums over its
+//     z-bins -> cell starts relative to their column, and the column count;
 //  B: each thread scans a contiguous chunk of columns (padded slot counts and
 //     group counts), a block scan joins the chunks; column starts, dummy
-//     slots and the group table are written;
-//  C: one wave per column adds the column start to its cell starts.
-__global__ void __launch_bounds__(256) k_scan(DevPtrs P, int setBegin) {
+//     slots and the group table are written.
+// A cell's first slot is colStart[column] + cellStart[cell].
+__global__ void __launch_bounds__(1024) k_scan(DevPtrs P, int setBegin) {
+    constexpr int NT = 1024, NW = NT / 64;
     const int c = setBegin + blockIdx.x;
     const SetDesc& D = P.desc[c];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int ncols = D.nx * D.ny, nz = D.nz;
-    for (int col = wave; col < ncols; col += 4) {
+    for (int col = wave; col < ncols; col += NW) {
         const int* cnt = P.cellCount + D.cellOff + (size_t)col * nz;
         int* st = P.cellStart + D.cellOff + (size_t)col * nz;
         int run = 0;
@@ -243,47 +248,43 @@ __global__ void __launch_bounds__(256) k_scan(DevPtrs P, int setBegin) {
         if (lane == 0) P.colCount[D.colOff + col] = run;
     }
     __syncthreads();
-    const int chunk = (ncols + 255) / 256;
+    const int chunk = (ncols + NT - 1) / NT;
     const int b = min(t * chunk, ncols), e = min(b + chunk, ncols);
-    __shared__ int sS[256], sG[256];
+    __shared__ int sS[NW], sG[NW];
     int accS = 0, accG = 0;
     for (int col = b; col < e; ++col) {
         const int padded = (P.colCount[D.colOff + col] + 7) & ~7;
         accS += padded;
         accG += ((padded >> 3) + kGroupClusters - 1) / kGroupClusters;
     }
-    sS[t] = accS; sG[t] = accG;
+    // block exclusive scan: wave scans, then a scan of the 16 wave totals
+    const int inS = wave_incl_scan(accS), inG = wave_incl_scan(accG);
+    if (lane == 63) { sS[wave] = inS; sG[wave] = inG; }
     __syncthreads();
-    for (int o = 1; o < 256; o <<= 1) {  // inclusive Hillis-Steele scan
-        int vs = t >= o ? sS[t - o] : 0, vg = t >= o ? sG[t - o] : 0;
-        __syncthreads();
-        sS[t] += vs; sG[t] += vg;
-        __syncthreads();
+    int baseS = 0, baseG = 0, totS = 0, totG = 0;
+    for (int w = 0; w < NW; ++w) {
+        const int vs = sS[w], vg = sG[w];
+        if (w < wave) { baseS += vs; baseG += vg; }
+        totS += vs; totG += vg;
     }
-    int offS = sS[t] - accS, offG = sG[t] - accG;
+    int offS = baseS + inS - accS, offG = baseG + inG - accG;
     for (int col = b; col < e; ++col) {
         const int cnt = P.colCount[D.colOff + col];
         const int padded = (cnt + 7) & ~7;
         const int ncl = padded >> 3;
         const int ng = (ncl + kGroupClusters - 1) / kGroupClusters;
         P.colStart[D.colOff + col] = offS;
-        for (int s = offS + cnt; s < offS + padded; ++s) P.sidx[D.slotOff + s] = -1;
+        for (int s2 = offS + cnt; s2 < offS + padded; ++s2) P.sidx[D.slotOff + s2] = -1;
         for (int gg = 0; gg < ng; ++gg)
             P.ginfo[D.groupOff + offG + gg] =
                 make_int4((offS >> 3) + gg * kGroupClusters, min(kGroupClusters, ncl - gg * kGroupClusters), col, 0);
         offS += padded;
         offG += ng;
     }
-    if (t == 255) {
-        P.rt[c].nslots = sS[255];
-        P.rt[c].nclusters = sS[255] >> 3;
-        P.rt[c].ngroups = sG[255];
-    }
-    __syncthreads();
-    for (int col = wave; col < ncols; col += 4) {
-        const int base = P.colStart[D.colOff + col];
-        int* st = P.cellStart + D.cellOff + (size_t)col * nz;
-        for (int z = lane; z < nz; z += 64) st[z] += base;
+    if (t == 0) {
+        P.rt[c].nslots = totS;
+        P.rt[c].nclusters = totS >> 3;
+        P.rt[c].ngroups = totG;
     }
 }
 
@@ -294,7 +295,8 @@ __global__ void k_scatter_idx(DevPtrs P, int setBegin) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= D.natoms) return;
     const int k = D.setAtomOff + i;
-    const int slot = P.cellStart[D.cellOff + P.bcell[k]] + P.brank[k];
+    const int cell = P.bcell[k];
+    const int slot = P.colStart[D.colOff + cell / D.nz] + P.cellStart[D.cellOff + cell] + P.brank[k];
     P.sidx[D.slotOff + slot] = i;
 }
 
@@ -308,7 +310,7 @@ __global__ void k_cellsort_idx(DevPtrs P, int setBegin) {
     if (cell >= D.nx * D.ny * D.nz) return;
     const int cnt = P.cellCount[D.cellOff + cell];
     if (cnt < 2) return;
-    int* idx = P.sidx + D.slotOff + P.cellStart[D.cellOff + cell];
+    int* idx = P.sidx + D.slotOff + P.colStart[D.colOff + cell / D.nz] + P.cellStart[D.cellOff + cell];
     for (int a = 1; a < cnt; ++a) {
         const int key = idx[a];
         int b = a - 1;
@@ -342,6 +344,12 @@ __global__ void __launch_bounds__(256) k_gather_bbox(DevPtrs P, int setBegin) {
     const int c = setBegin + blockIdx.y;
     const SetDesc& D = P.desc[c];
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    // FP64 side sums: cleared over the whole slot range after an evaluation
+    // that used them (k_pairlist then resets the flag)
+    if (P.rt[c].big && s < D.slotCap) {
+        double* d = P.sd + 3ll * (D.slotOff + s);
+        d[0] = 0.0; d[1] = 0.0; d[2] = 0.0;
+    }
     if (s >= P.rt[c].nslots) return;
     const int k = D.slotOff + s;
     const int i = P.sidx[k];
@@ -357,8 +365,7 @@ __global__ void __launch_bounds__(256) k_gather_bbox(DevPtrs P, int setBegin) {
     P.spos[k] = p;
     P.sprm[k] = q;
     P.sprm64[k] = qd;
-    P.sfx[k] = 0; P.sfy[k] = 0; P.sfz[k] = 0;
-    P.sdx[k] = 0.0; P.sdy[k] = 0.0; P.sdz[k] = 0.0;
+    P.sf[3ll * k] = 0; P.sf[3ll * k + 1] = 0; P.sf[3ll * k + 2] = 0;
     const float inf = __builtin_huge_valf();
     const float lox = oct_min(i >= 0 ? p.x : inf), hix = oct_max(i >= 0 ? p.x : -inf);
     const float loy = oct_min(i >= 0 ? p.y : inf), hiy = oct_max(i >= 0 ? p.y : -inf);
@@ -385,6 +392,7 @@ __global__ void __launch_bounds__(64 * W) k_pairlist(DevPtrs P, int setBegin) {
     const SetDesc D = P.desc[c];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int g = blockIdx.x * W + wave;
+    if (g == 0 && lane == 0) P.rt[c].big = 0;  // the side sums were cleared by k_gather_bbox
     if (g >= P.rt[c].ngroups) return;
     const int4 gi = P.ginfo[D.groupOff + g];
     const int g0 = __builtin_amdgcn_readfirstlane(gi.x), ncl = __builtin_amdgcn_readfirstlane(gi.y);
@@ -505,13 +513,14 @@ __device__ __forceinline__ JAtom load_j(const DevPtrs& P, const SetDesc& D, uint
 
 // Flush one partial force vector to a slot.  The common case (all three
 // components below kFixedMax) takes the branch-free fixed-point path.
-__device__ __forceinline__ void flush_force(const DevPtrs& P, int slot, float fx, float fy, float fz) {
+__device__ __forceinline__ void flush_force(const DevPtrs& P, int c, int slot, float fx, float fy, float fz) {
     if (fmaxf(fabsf(fx), fmaxf(fabsf(fy), fabsf(fz))) < 134217728.0f) {  // 2^27: |f scale| < 2^51
-        atomic_add_fixed(P.sfx + slot, __double_as_longlong((double)fx * kForceScale + 6755399441055744.0) - 0x4338000000000000LL);
-        atomic_add_fixed(P.sfy + slot, __double_as_longlong((double)fy * kForceScale + 6755399441055744.0) - 0x4338000000000000LL);
-        atomic_add_fixed(P.sfz + slot, __double_as_longlong((double)fz * kForceScale + 6755399441055744.0) - 0x4338000000000000LL);
+        long long* f = P.sf + 3ll * slot;
+        atomic_add_fixed(f, __double_as_longlong((double)fx * kForceScale + 6755399441055744.0) - 0x4338000000000000LL);
+        atomic_add_fixed(f + 1, __double_as_longlong((double)fy * kForceScale + 6755399441055744.0) - 0x4338000000000000LL);
+        atomic_add_fixed(f + 2, __double_as_longlong((double)fz * kForceScale + 6755399441055744.0) - 0x4338000000000000LL);
     } else {
-        add_force(P, slot, fx, fy, fz);
+        add_force(P, c, slot, fx, fy, fz);
     }
 }
 
@@ -626,7 +635,7 @@ __global__ void __launch_bounds__(64 * W) k_pairs(DevPtrs P, int setBegin) {
             }
             if (FORCES) {
                 fjx = oct_sum(fjx); fjy = oct_sum(fjy); fjz = oct_sum(fjz);
-                if (ii == 0 && jv) flush_force(P, js, fjx, fjy, fjz);
+                if (ii == 0 && jv) flush_force(P, c, js, fjx, fjy, fjz);
             }
         }
     }
@@ -635,8 +644,242 @@ __global__ void __launch_bounds__(64 * W) k_pairs(DevPtrs P, int setBegin) {
         for (int t = 0; t < kGroupClusters; ++t) {
             if (t < ncl) {
                 const float fx = stride8_sum(fi[t][0]), fy = stride8_sum(fi[t][1]), fz = stride8_sum(fi[t][2]);
-                if (jj == 0 && ((ivm >> t) & 1u)) flush_force(P, D.slotOff + (g0 + t) * kClusterSize + ii, fx, fy, fz);
+                if (jj == 0 && ((ivm >> t) & 1u)) flush_force(P, c, D.slotOff + (g0 + t) * kClusterSize + ii, fx, fy, fz);
             }
+        }
+    }
+    double eacc = eE + D.krf64 * eQR2 - D.crf64 * eQQ;
+    eacc = wave_sum_d(eacc);
+    if (lane == 0) P.genergy[D.groupOff + g] = eacc;
+}
+
+// ------------------------------------------------- compacted pair kernel
+
+// Order LDS accesses of the lanes of one wave (LDS executes a wave's
+// operations in issue order; this keeps the compiler from moving them).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// FP64 -> 64-bit fixed point (round to nearest) for |f| < 2^27 kJ/mol/nm.
+__device__ __forceinline__ unsigned long long fixed64(double f) {
+    const double t = fma(f, kForceScale, 6755399441055744.0);  // 1.5 * 2^52
+    return (unsigned long long)(__double_as_longlong(t) - 0x4338000000000000LL);
+}
+
+__device__ __forceinline__ void lds_add_u64(unsigned long long* p, unsigned long long v) {
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+}
+
+__device__ __forceinline__ void lds_sub_u64(unsigned long long* p, unsigned long long v) {
+    __hip_atomic_fetch_sub(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+}
+
+// One wave64 per group, two passes per batch of JB list entries.
+//  test:  lane = 8 jj + ii tests i-atom (t, ii) against j-atom jj of each
+//         entry's j-cluster for every i-cluster t in the entry's mask, with an
+//         FP32 r^2 and a margin above rc^2 (it never drops a pair inside the
+//         cutoff), and appends the passing pairs as (i, j) codes to a ring in
+//         LDS (ballot + mbcnt: order = entry, t, lane);
+//  dense: whenever 64 DR pairs are pending (and at the end of the batch) the
+//         wave drains them in chunks of 64: lane l of chunk q takes pending
+//         pair l m + q (m chunks), decides r < rc on the exact FP64 r^2 and
+//         evaluates the FP64 pair function, so the expensive part runs on
+//         in-cutoff pairs instead of whole 8x8 tiles.  The stride DR spreads
+//         a chunk over about DR list entries, so that fewer of its lanes add
+//         into the same j-atom's LDS accumulator (same-address atomics
+//         serialise).
+// Forces: per pair FP64 components -> 64-bit fixed point -> integer LDS
+// atomics into per-atom accumulators of the group (i) and of the batch (j),
+// flushed to the global fixed-point accumulators with integer atomics.
+// Integer sums do not depend on the order of the atomics, so results are
+// bit-reproducible.  Components >= 2^27 kJ/mol/nm take FP64 global atomics.
+template <int W, int JB, int DR, bool FORCES>
+__global__ void __launch_bounds__(64 * W) k_pairs_q(DevPtrs P, int setBegin) {
+    constexpr int NJ = JB * kClusterSize;  // j-atoms per batch
+    constexpr int QCAP = 1024;             // pending pairs < 64 DR + 8 * 64
+    static_assert(64 * DR + 512 <= QCAP, "pair ring too small");
+    const int c = setBegin + blockIdx.y;
+    const SetDesc D = P.desc[c];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int jj = lane >> 3, ii = lane & 7;
+    const int g = blockIdx.x * W + wave;
+    if (g >= P.rt[c].ngroups) return;  // waves synchronise only within themselves
+
+    __shared__ float4 sIPos[W][64];
+    __shared__ double4 sIPrm[W][64];
+    __shared__ float4 sJPos[W][NJ];    // w = 1: real atom, 0: dummy or past the list
+    __shared__ double4 sJPrm[W][NJ];
+    __shared__ int sJSlot[W][NJ];
+    __shared__ unsigned long long sIF[W][3][FORCES ? 64 : 1];
+    __shared__ unsigned long long sJF[W][3][FORCES ? NJ : 1];
+    __shared__ unsigned short sQ[W][QCAP];
+
+    const int4 gi = P.ginfo[D.groupOff + g];
+    const int g0 = __builtin_amdgcn_readfirstlane(gi.x), ncl = __builtin_amdgcn_readfirstlane(gi.y);
+    const int sBase = D.slotOff + g0 * kClusterSize;
+    bool iv = false;
+    {
+        float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
+        double4 q = make_double4(0.0, 0.0, 0.0, 0.0);
+        if (lane < ncl * kClusterSize) {
+            p = P.spos[sBase + lane];
+            q = P.sprm64[sBase + lane];
+            iv = P.sidx[sBase + lane] >= 0;
+        }
+        sIPos[wave][lane] = p;
+        sIPrm[wave][lane] = q;
+        if (FORCES) sIF[wave][0][lane] = sIF[wave][1][lane] = sIF[wave][2][lane] = 0ull;
+    }
+    const uint64_t ivb = __ballot(iv);
+    uint32_t ivm = 0;  // bit t: i-atom (t, ii) is a real atom
+#pragma unroll
+    for (int t = 0; t < kGroupClusters; ++t) ivm |= (uint32_t)((ivb >> (t * kClusterSize + ii)) & 1ull) << t;
+    wave_sync();
+    float pix[kGroupClusters], piy[kGroupClusters], piz[kGroupClusters];
+#pragma unroll
+    for (int t = 0; t < kGroupClusters; ++t) {
+        const float4 a = sIPos[wave][t * kClusterSize + ii];
+        pix[t] = a.x; piy[t] = a.y; piz[t] = a.z;
+    }
+
+    const float Lx = D.L[0], invLx = D.invL[0], Ly = D.L[1], invLy = D.invL[1], Lz = D.L[2], invLz = D.invL[2];
+    const bool shiftOK = D.shiftOK;
+    const float rc2hi = D.rc2hi;
+    const double rc2 = D.rc2, krf64 = D.krf64;
+    double eE = 0.0, eQQ = 0.0, eQR2 = 0.0;
+    const int ne = __builtin_amdgcn_readfirstlane(P.gcount[D.groupOff + g]);
+    const uint32_t* E = P.entries + D.entryOff + (long long)g * D.entryCap;
+
+    for (int b0 = 0; b0 < ne; b0 += JB) {
+        const int nb = min(JB, ne - b0);
+        uint32_t entReg[JB / 8];
+        // load the batch's j-clusters: lane l of round r holds atom (l & 7) of entry r * 8 + (l >> 3)
+#pragma unroll
+        for (int r = 0; r < JB / 8; ++r) {
+            const int k = r * 8 + jj;
+            uint32_t ent = 0u;
+            float4 pj = make_float4(0.f, 0.f, 0.f, 0.f);
+            double4 prm = make_double4(0.0, 0.0, 0.0, 0.0);
+            int js = 0;
+            bool jv = false;
+            if (k < nb) {
+                ent = E[b0 + k];
+                js = D.slotOff + (int)(ent & 0xFFFFFu) * kClusterSize + ii;
+                pj = P.spos[js];
+                prm = P.sprm64[js];
+                jv = P.sidx[js] >= 0;
+                if (shiftOK) {
+                    const uint32_t sxc = (ent >> 28) & 3u, syc = (ent >> 30) & 3u;
+                    pj.x += sxc == 1u ? Lx : (sxc == 2u ? -Lx : 0.f);
+                    pj.y += syc == 1u ? Ly : (syc == 2u ? -Ly : 0.f);
+                }
+            }
+            pj.w = jv ? 1.f : 0.f;
+            entReg[r] = ent;
+            sJPos[wave][r * 64 + lane] = pj;
+            sJPrm[wave][r * 64 + lane] = prm;
+            sJSlot[wave][r * 64 + lane] = js;
+            if (FORCES) sJF[wave][0][r * 64 + lane] = sJF[wave][1][r * 64 + lane] = sJF[wave][2][r * 64 + lane] = 0ull;
+        }
+        wave_sync();
+        int qh = 0, qn = 0;  // ring head and tail (wave-uniform)
+        for (int k = 0; k < nb; ++k) {
+            const uint32_t ent = __builtin_amdgcn_readlane(JB == 8 ? entReg[0] : ((k >> 3) ? entReg[JB / 8 - 1] : entReg[0]),
+                                                           (k & 7) * 8);
+            const int td = (int)(ent & 0xFFFFFu) - g0;  // the i-cluster equal to J, if any
+            const uint32_t imask = (ent >> 20) & 0xFFu;
+            const float4 pj = sJPos[wave][k * 8 + jj];
+            uint32_t ok = pj.w != 0.f ? ivm : 0u;
+            if (td >= 0 && td < kGroupClusters && ii >= jj) ok &= ~(1u << td);
+            const unsigned short code0 = (unsigned short)(ii | ((k * 8 + jj) << 6));
+#pragma unroll
+            for (int t = 0; t < kGroupClusters; ++t) {
+                if (!(imask & (1u << t))) continue;
+                float dx = pix[t] - pj.x, dy = piy[t] - pj.y, dz = piz[t] - pj.z;
+                if (!shiftOK) { dx = min_image(dx, Lx, invLx); dy = min_image(dy, Ly, invLy); }
+                dz = min_image(dz, Lz, invLz);
+                const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                const bool in = r2 < rc2hi && ((ok >> t) & 1u);
+                const uint64_t m = __ballot(in);
+                if (m == 0ull) continue;
+                if (in) {
+                    const int pos = qn + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+                    sQ[wave][pos & (QCAP - 1)] = (unsigned short)(code0 + t * 8);
+                }
+                qn += __popcll(m);
+            }
+            const bool last = k == nb - 1;
+            while (qn - qh >= 64 * DR || (last && qn > qh)) {
+              const int n = min(64 * DR, qn - qh);
+              const int m = (n + 63) >> 6;  // chunks of this drain
+              for (int q = 0; q < m; ++q) {
+                wave_sync();
+                const int e = lane * m + q;
+                const bool act = e < n;
+                const int code = act ? (int)sQ[wave][(qh + e) & (QCAP - 1)] : 0;
+                const int i = code & 63, j = code >> 6;
+                const float4 pi = sIPos[wave][i], pq = sJPos[wave][j];
+                float dx = pi.x - pq.x, dy = pi.y - pq.y, dz = pi.z - pq.z;
+                if (!shiftOK) { dx = min_image(dx, Lx, invLx); dy = min_image(dy, Ly, invLy); }
+                dz = min_image(dz, Lz, invLz);
+                const double dxd = dx, dyd = dy, dzd = dz;
+                // exact: the squares of 2^-20-grid differences fit in 53 bits
+                const double r2 = fma(dxd, dxd, fma(dyd, dyd, dzd * dzd));
+                const bool in = act && r2 < rc2;
+                const double4 a = sIPrm[wave][i], b = sJPrm[wave][j];
+                const PairE64 pe = pair_term64(in, r2, a.x + b.x, a.y * b.y, a.z * b.z, krf64);
+                eE += pe.e; eQQ += pe.qq; eQR2 += pe.qq * r2;
+                if (FORCES && in) {
+                    const double fx = pe.fs * dxd, fy = pe.fs * dyd, fz = pe.fs * dzd;
+                    if (fmax(fabs(fx), fmax(fabs(fy), fabs(fz))) < 134217728.0) {  // 2^27
+                        const unsigned long long vx = fixed64(fx), vy = fixed64(fy), vz = fixed64(fz);
+                        lds_add_u64(&sIF[wave][0][i], vx);
+                        lds_add_u64(&sIF[wave][1][i], vy);
+                        lds_add_u64(&sIF[wave][2][i], vz);
+                        lds_sub_u64(&sJF[wave][0][j], vx);
+                        lds_sub_u64(&sJF[wave][1][j], vy);
+                        lds_sub_u64(&sJF[wave][2][j], vz);
+                    } else {
+                        double* di = P.sd + 3ll * (sBase + i);
+                        double* dj = P.sd + 3ll * sJSlot[wave][j];
+                        atomicAdd(di, fx); atomicAdd(di + 1, fy); atomicAdd(di + 2, fz);
+                        atomicAdd(dj, -fx); atomicAdd(dj + 1, -fy); atomicAdd(dj + 2, -fz);
+                        P.rt[c].big = 1;
+                    }
+                }
+              }
+              qh += n;
+            }
+        }
+        wave_sync();
+        if (FORCES) {
+#pragma unroll
+            for (int r = 0; r < JB / 8; ++r) {
+                const int x = r * 64 + lane;
+                if (sJPos[wave][x].w != 0.f) {
+                    const unsigned long long vx = sJF[wave][0][x], vy = sJF[wave][1][x], vz = sJF[wave][2][x];
+                    unsigned long long* f = reinterpret_cast<unsigned long long*>(P.sf) + 3ll * sJSlot[wave][x];
+                    if (vx | vy | vz) {
+                        atomicAdd(f, vx);
+                        atomicAdd(f + 1, vy);
+                        atomicAdd(f + 2, vz);
+                    }
+                }
+            }
+        }
+        wave_sync();  // the next batch overwrites the j arrays
+    }
+    if (FORCES && iv) {
+        const unsigned long long vx = sIF[wave][0][lane], vy = sIF[wave][1][lane], vz = sIF[wave][2][lane];
+        if (vx | vy | vz) {
+            unsigned long long* f = reinterpret_cast<unsigned long long*>(P.sf) + 3ll * (sBase + lane);
+            atomicAdd(f, vx);
+            atomicAdd(f + 1, vy);
+            atomicAdd(f + 2, vz);
         }
     }
     double eacc = eE + D.krf64 * eQR2 - D.crf64 * eQQ;
@@ -748,19 +991,28 @@ __global__ void k_blend(DevPtrs P, int sysBegin, int hostMode, double spHost, do
     const double dir = hostMode ? dirHost : P.params[s].direction;
     const double w2 = dir > 0.0 ? sp : 1.0 - sp;
     const double w1 = dir > 0.0 ? 1.0 - sp : sp;
-    const int a = D0.slotOff + P.slotOf[D0.setAtomOff + i];
+    const long long a = D0.slotOff + P.slotOf[D0.setAtomOff + i];
     double* F = P.outF + 3ll * D0.atomOff;
     const double inv = 1.0 / kForceScale;
+    const long long* fa = P.sf + 3 * a;
+    double f1x = (double)fa[0] * inv, f1y = (double)fa[1] * inv, f1z = (double)fa[2] * inv;
+    if (P.rt[2 * s].big) {
+        const double* d = P.sd + 3 * a;
+        f1x += d[0]; f1y += d[1]; f1z += d[2];
+    }
     if (hostMode == 2) {  // state 1 only: F += F1 (the state-2 set was not evaluated)
-        F[i] += (double)P.sfx[a] * inv + P.sdx[a];
-        F[N + i] += (double)P.sfy[a] * inv + P.sdy[a];
-        F[2 * N + i] += (double)P.sfz[a] * inv + P.sdz[a];
+        F[i] += f1x;
+        F[N + i] += f1y;
+        F[2 * N + i] += f1z;
         return;
     }
-    const int b = D1.slotOff + P.slotOf[D1.setAtomOff + i];
-    const double f1x = (double)P.sfx[a] * inv + P.sdx[a], f2x = (double)P.sfx[b] * inv + P.sdx[b];
-    const double f1y = (double)P.sfy[a] * inv + P.sdy[a], f2y = (double)P.sfy[b] * inv + P.sdy[b];
-    const double f1z = (double)P.sfz[a] * inv + P.sdz[a], f2z = (double)P.sfz[b] * inv + P.sdz[b];
+    const long long b = D1.slotOff + P.slotOf[D1.setAtomOff + i];
+    const long long* fb = P.sf + 3 * b;
+    double f2x = (double)fb[0] * inv, f2y = (double)fb[1] * inv, f2z = (double)fb[2] * inv;
+    if (P.rt[2 * s + 1].big) {
+        const double* d = P.sd + 3 * b;
+        f2x += d[0]; f2y += d[1]; f2z += d[2];
+    }
     F[i] += w1 * f1x + w2 * f2x;
     F[N + i] += w1 * f1y + w2 * f2y;
     F[2 * N + i] += w1 * f1z + w2 * f2z;
@@ -786,7 +1038,7 @@ hipError_t launch_build(const DevPtrs& P, int setBegin, int nsets, int maxAtoms,
                         hipStream_t st) {
     dim3 ga(cdiv(maxAtoms, 256), nsets);
     k_bin<<<ga, 256, 0, st>>>(P, setBegin);
-    k_scan<<<nsets, 256, 0, st>>>(P, setBegin);
+    k_scan<<<nsets, 1024, 0, st>>>(P, setBegin);
     k_scatter_idx<<<ga, 256, 0, st>>>(P, setBegin);
     k_cellsort_idx<<<dim3(cdiv(maxCells, 256), nsets), 256, 0, st>>>(P, setBegin);
     k_gather_bbox<<<dim3(cdiv(maxSlots, 256), nsets), 256, 0, st>>>(P, setBegin);
@@ -803,7 +1055,22 @@ hipError_t launch_pairlist(const DevPtrs& P, int setBegin, int nsets, int maxGro
 }
 
 hipError_t launch_pairs(const DevPtrs& P, int setBegin, int nsets, int maxGroups, int waves, bool forces, bool fp64,
-                        hipStream_t st) {
+                        int variant, hipStream_t st) {
+    if (fp64 && variant != 0) {
+#define ATM_PAIRSQ(WV, JB, DR)                                                                                     \
+    if (forces) k_pairs_q<WV, JB, DR, true><<<dim3(cdiv(maxGroups, WV), nsets), 64 * WV, 0, st>>>(P, setBegin);  \
+    else k_pairs_q<WV, JB, DR, false><<<dim3(cdiv(maxGroups, WV), nsets), 64 * WV, 0, st>>>(P, setBegin);
+#define ATM_PAIRSQ_W(JB, DR)                 \
+    switch (waves) {                         \
+        case 1: ATM_PAIRSQ(1, JB, DR) break; \
+        case 2: ATM_PAIRSQ(2, JB, DR) break; \
+        default: ATM_PAIRSQ(4, JB, DR) break; \
+    }
+        if (variant == 1) { ATM_PAIRSQ_W(8, 1) } else if (variant == 2) { ATM_PAIRSQ_W(16, 1) } else { ATM_PAIRSQ_W(16, 4) }
+#undef ATM_PAIRSQ_W
+#undef ATM_PAIRSQ
+        return hipGetLastError();
+    }
 #define ATM_PAIRS(WV, F64)                                                                                 \
     if (forces) k_pairs<WV, true, F64><<<dim3(cdiv(maxGroups, WV), nsets), 64 * WV, 0, st>>>(P, setBegin);  \
     else k_pairs<WV, false, F64><<<dim3(cdiv(maxGroups, WV), nsets), 64 * WV, 0, st>>>(P, setBegin);

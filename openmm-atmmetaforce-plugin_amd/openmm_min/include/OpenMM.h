@@ -3,6 +3,7 @@
 #include "openmm/Context.h"
 #include "openmm/Force.h"
 #include "openmm/Integrator.h"
+#include "openmm/MonteCarloBarostat.h"
 #include "openmm/NonbondedForce.h"
 #include "openmm/OpenMMException.h"
 #include "openmm/Platform.h"

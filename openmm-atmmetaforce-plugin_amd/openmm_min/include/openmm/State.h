@@ -26,6 +26,7 @@ public:
 private:
     friend class Context;
     friend class StateBuilder;
+    friend class StateProxy;
     int types;
     double time, potentialEnergy, kineticEnergy;
     std::vector<Vec3> positions, velocities, forces;

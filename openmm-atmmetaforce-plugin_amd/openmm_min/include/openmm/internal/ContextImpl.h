@@ -45,6 +45,8 @@ public:
     // Zero the forces, then let every ForceImpl add its contribution for the
     // requested groups; returns the potential energy of those groups.
     double calcForcesAndEnergy(bool includeForces, bool includeEnergy, int groups = 0xFFFFFFFF);
+    // Lets every force update the state before a step (barostats, thermostats).
+    void updateContextState();
     double calcKineticEnergy() const;
     void* getPlatformData() { return platformData; }
     const void* getPlatformData() const { return platformData; }

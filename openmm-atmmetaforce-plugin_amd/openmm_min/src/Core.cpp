@@ -8,6 +8,7 @@
 #include "openmm/Context.h"
 #include "openmm/Force.h"
 #include "openmm/Integrator.h"
+#include "openmm/MonteCarloBarostat.h"
 #include "openmm/NonbondedForce.h"
 #include "openmm/OpenMMException.h"
 #include "openmm/Platform.h"
@@ -150,6 +151,42 @@ private:
     Kernel kernel;
 };
 
+// ------------------------------------------------------------------ MonteCarloBarostat
+
+class MonteCarloBarostatImpl : public ForceImpl {
+public:
+    explicit MonteCarloBarostatImpl(const MonteCarloBarostat& owner) : owner(owner) {}
+    void initialize(ContextImpl&) override { step = 0; }
+    const Force& getOwner() const override { return owner; }
+    void updateContextState(ContextImpl&, bool&) override {
+        if (owner.getFrequency() == 0 || ++step < owner.getFrequency()) return;
+        throw OpenMMException("MonteCarloBarostat: volume moves are not implemented here; keep its frequency above "
+                              "the number of steps (or 0) to use it as a holder of its parameters");
+    }
+    double calcForcesAndEnergy(ContextImpl&, bool, bool, int) override { return 0.0; }
+    std::map<std::string, double> getDefaultParameters() override {
+        return {{MonteCarloBarostat::Pressure(), owner.getDefaultPressure()},
+                {MonteCarloBarostat::Temperature(), owner.getDefaultTemperature()}};
+    }
+    std::vector<std::string> getKernelNames() override { return {}; }
+
+private:
+    const MonteCarloBarostat& owner;
+    long long step = 0;
+};
+
+MonteCarloBarostat::MonteCarloBarostat(double defaultPressure, double defaultTemperature, int frequency)
+    : pressure(defaultPressure), temperature(defaultTemperature), frequency(0), seed(0) {
+    setFrequency(frequency);
+}
+
+void MonteCarloBarostat::setFrequency(int freq) {
+    if (freq < 0) throw OpenMMException("Frequency cannot be negative");
+    frequency = freq;
+}
+
+ForceImpl* MonteCarloBarostat::createImpl() const { return new MonteCarloBarostatImpl(*this); }
+
 NonbondedForce::NonbondedForce()
     : method(NoCutoff), cutoff(1.0), rfDielectric(78.3), useDispersionCorrection(true), recipGroup(-1) {}
 
@@ -258,6 +295,7 @@ void VerletIntegrator::step(int steps) {
     std::vector<Vec3>& x = c.positionsRef();
     std::vector<Vec3>& v = c.velocitiesRef();
     for (int s = 0; s < steps; ++s) {
+        c.updateContextState();
         c.calcForcesAndEnergy(true, false, groups);
         const std::vector<Vec3>& f = c.forcesRef();
         for (int i = 0; i < n; ++i) {
@@ -332,6 +370,11 @@ void ContextImpl::setPeriodicBoxVectors(const Vec3& a, const Vec3& b, const Vec3
     box[0] = a;
     box[1] = b;
     box[2] = c;
+}
+
+void ContextImpl::updateContextState() {
+    bool forcesInvalid = false;
+    for (ForceImpl* f : forceImpls) f->updateContextState(*this, forcesInvalid);
 }
 
 double ContextImpl::calcForcesAndEnergy(bool includeForces, bool includeEnergy, int groups) {

@@ -11,6 +11,8 @@
 #include <typeindex>
 
 #include "openmm/OpenMMException.h"
+#include "openmm/Platform.h"
+#include "openmm/State.h"
 #include "openmm/serialization/SerializationNode.h"
 #include "openmm/serialization/SerializationProxy.h"
 #include "openmm/serialization/XmlSerializer.h"
@@ -281,6 +283,99 @@ struct Reader {
     }
 };
 
+}  // namespace
+
+// ------------------------------------------------------------------ State
+
+// State checkpoints in OpenMM's State XML layout (version 1): root attributes
+// openmmVersion and time; children PeriodicBoxVectors (A, B, C), Parameters
+// (one attribute per global parameter, e.g. the ATM parameters), Energies,
+// Positions/Velocities/Forces (one x, y, z element per particle) and
+// IntegratorParameters.  Reads files such as the ATM examples' equilibrated
+// checkpoints (example/abfe/temoa-g1-equil.xml).
+class StateProxy : public SerializationProxy {
+public:
+    StateProxy() : SerializationProxy("State") {}
+    void serialize(const void* object, SerializationNode& node) const override {
+        const State& s = *reinterpret_cast<const State*>(object);
+        node.setIntProperty("version", 1);
+        node.setStringProperty("openmmVersion", Platform::getOpenMMVersion());
+        node.setDoubleProperty("time", s.time);
+        SerializationNode& box = node.createChildNode("PeriodicBoxVectors");
+        const char* names[3] = {"A", "B", "C"};
+        for (int k = 0; k < 3; ++k)
+            box.createChildNode(names[k]).setDoubleProperty("x", s.box[k][0]).setDoubleProperty("y", s.box[k][1])
+                .setDoubleProperty("z", s.box[k][2]);
+        if (s.types & State::Parameters) {
+            SerializationNode& pn = node.createChildNode("Parameters");
+            for (auto& kv : s.parameters) pn.setDoubleProperty(kv.first, kv.second);
+        }
+        if (s.types & State::Energy)
+            node.createChildNode("Energies").setDoubleProperty("KineticEnergy", s.kineticEnergy)
+                .setDoubleProperty("PotentialEnergy", s.potentialEnergy);
+        auto list = [&](const char* outer, const char* inner, const std::vector<Vec3>& v) {
+            SerializationNode& n = node.createChildNode(outer);
+            for (const Vec3& x : v)
+                n.createChildNode(inner).setDoubleProperty("x", x[0]).setDoubleProperty("y", x[1]).setDoubleProperty("z", x[2]);
+        };
+        if (s.types & State::Positions) list("Positions", "Position", s.positions);
+        if (s.types & State::Velocities) list("Velocities", "Velocity", s.velocities);
+        if (s.types & State::Forces) list("Forces", "Force", s.forces);
+        node.createChildNode("IntegratorParameters");
+    }
+    void* deserialize(const SerializationNode& node) const override {
+        if (node.getIntProperty("version") != 1) throw OpenMMException("Unsupported version number");
+        State* s = new State();
+        try {
+            s->time = node.getDoubleProperty("time");
+            int types = 0;
+            for (const SerializationNode& c : node.getChildren()) {
+                const std::string& n = c.getName();
+                auto vecs = [&](std::vector<Vec3>& v) {
+                    v.clear();
+                    for (const SerializationNode& e : c.getChildren())
+                        v.emplace_back(e.getDoubleProperty("x"), e.getDoubleProperty("y"), e.getDoubleProperty("z"));
+                };
+                if (n == "PeriodicBoxVectors") {
+                    const char* names[3] = {"A", "B", "C"};
+                    for (int k = 0; k < 3; ++k) {
+                        const SerializationNode& b = c.getChildNode(names[k]);
+                        s->box[k] = Vec3(b.getDoubleProperty("x"), b.getDoubleProperty("y"), b.getDoubleProperty("z"));
+                    }
+                } else if (n == "Parameters") {
+                    for (auto& kv : c.getProperties()) s->parameters[kv.first] = c.getDoubleProperty(kv.first);
+                    types |= State::Parameters;
+                } else if (n == "Energies") {
+                    s->kineticEnergy = c.getDoubleProperty("KineticEnergy");
+                    s->potentialEnergy = c.getDoubleProperty("PotentialEnergy");
+                    types |= State::Energy;
+                } else if (n == "Positions") {
+                    vecs(s->positions);
+                    types |= State::Positions;
+                } else if (n == "Velocities") {
+                    vecs(s->velocities);
+                    types |= State::Velocities;
+                } else if (n == "Forces") {
+                    vecs(s->forces);
+                    types |= State::Forces;
+                }
+            }
+            s->types = types;
+        } catch (...) {
+            delete s;
+            throw;
+        }
+        return s;
+    }
+};
+
+namespace {
+struct RegisterCoreProxies {
+    RegisterCoreProxies() {
+        static StateProxy state;
+        SerializationProxy::registerProxy(typeid(State), &state);
+    }
+} registerCoreProxies;
 }  // namespace
 
 void XmlSerializer::readXml(std::istream& stream, SerializationNode& node) {

@@ -19,6 +19,7 @@
 #include "internal/ATMMetaForceImpl.h"
 #include "openmm/Context.h"
 #include "openmm/Integrator.h"
+#include "openmm/MonteCarloBarostat.h"
 #include "openmm/NonbondedForce.h"
 #include "openmm/OpenMMException.h"
 #include "openmm/Platform.h"
@@ -106,6 +107,20 @@ PYBIND11_MODULE(_atmmetaforce, m) {
         .def("getName", &Force::getName)
         .def("setName", &Force::setName)
         .def("usesPeriodicBoundaryConditions", &Force::usesPeriodicBoundaryConditions);
+
+    py::class_<MonteCarloBarostat, Force, std::unique_ptr<MonteCarloBarostat, py::nodelete>>(m, "MonteCarloBarostat")
+        .def(py::init([](double p, double t, int freq) { return new MonteCarloBarostat(p, t, freq); }),
+             py::arg("defaultPressure"), py::arg("defaultTemperature"), py::arg("frequency") = 25)
+        .def_static("Pressure", &MonteCarloBarostat::Pressure)
+        .def_static("Temperature", &MonteCarloBarostat::Temperature)
+        .def("getDefaultPressure", &MonteCarloBarostat::getDefaultPressure)
+        .def("setDefaultPressure", &MonteCarloBarostat::setDefaultPressure)
+        .def("getDefaultTemperature", &MonteCarloBarostat::getDefaultTemperature)
+        .def("setDefaultTemperature", &MonteCarloBarostat::setDefaultTemperature)
+        .def("getFrequency", &MonteCarloBarostat::getFrequency)
+        .def("setFrequency", &MonteCarloBarostat::setFrequency)
+        .def("getRandomNumberSeed", &MonteCarloBarostat::getRandomNumberSeed)
+        .def("setRandomNumberSeed", &MonteCarloBarostat::setRandomNumberSeed);
 
     py::class_<NonbondedForce, Force, std::unique_ptr<NonbondedForce, py::nodelete>> nb(m, "NonbondedForce");
     nb.def(py::init([]() { return new NonbondedForce(); }))
@@ -274,8 +289,23 @@ PYBIND11_MODULE(_atmmetaforce, m) {
             XmlSerializer::serialize<ATMMetaForce>(&f, "Force", s);
             return s.str();
         })
-        .def_static("deserialize", [](const std::string& xml) {
+        .def_static("serialize", [](const State& st) {
+            std::stringstream s;
+            XmlSerializer::serialize<State>(&st, "State", s);
+            return s.str();
+        })
+        // The root element's type attribute selects the class, as in OpenMM.
+        .def_static("deserialize", [](const std::string& xml) -> py::object {
             std::stringstream s(xml);
-            return XmlSerializer::deserialize<ATMMetaForce>(s);
-        }, py::return_value_policy::take_ownership);
+            SerializationNode node;
+            XmlSerializer::readXml(s, node);
+            const std::string type = node.getStringProperty("type");
+            const SerializationProxy& proxy = SerializationProxy::getProxy(type);
+            if (type == "State")
+                return py::cast(static_cast<State*>(proxy.deserialize(node)), py::return_value_policy::take_ownership);
+            if (type == "ATMMetaForce")
+                return py::cast(static_cast<ATMMetaForce*>(proxy.deserialize(node)),
+                                py::return_value_policy::take_ownership);
+            throw OpenMMException("XmlSerializer.deserialize: unsupported type " + type);
+        });
 }

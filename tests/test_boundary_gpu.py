@@ -174,3 +174,73 @@ def test_verlet_steps_with_atm_force(gpu):
     x1 = ctx.getState(getPositions=True).getPositions(asNumpy=True)
     assert np.isfinite(x1).all() and not np.array_equal(x0, x1)
     assert ctx.getState(getEnergy=True).getPotentialEnergy() != 0.0
+
+
+def test_state_checkpoint_restores_atm_parameters(gpu):
+    """Simulation.loadState's path (example/abfe/abfe.py:121): the reference's
+    checkpoint (cut to 16 particles) restored with Context.setState sets the
+    box, positions, velocities and the ATM global parameters; the evaluation
+    equals one set up by hand with the same values, and a checkpoint written
+    from the context restores it again."""
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "temoa-g1-equil-excerpt.xml")
+    st = mm.XmlSerializer.deserialize(open(path).read())
+    x = st.getPositions(asNumpy=True)
+    n = len(x)
+    rng = np.random.default_rng(3)
+    q = rng.normal(0, 0.4, n); q -= q.mean()
+    sg = rng.uniform(0.25, 0.35, n); ep = rng.uniform(0.1, 0.8, n)
+    d = np.zeros((n, 3)); d[:4] = (1.1, 0.9, 0.7)
+
+    def make():
+        system = mm.System()
+        for _ in range(n):
+            system.addParticle(12.0)
+        system.setDefaultPeriodicBoxVectors((4.0, 0, 0), (0, 4.0, 0), (0, 0, 4.0))
+        nb = mm.NonbondedForce()
+        nb.setNonbondedMethod(mm.NonbondedForce.CutoffPeriodic)
+        nb.setCutoffDistance(1.2)
+        nb.setUseDispersionCorrection(False)
+        for i in range(n):
+            nb.addParticle(float(q[i]), float(sg[i]), float(ep[i]))
+        nb.setForceGroup(1)
+        system.addForce(nb)
+        f = atm.ATMMetaForce(0.2, 0.3, 0.1, 10.0, 0.0, 200.0, 100.0, 0.0625, 1.0, [1])
+        for i in range(n):
+            f.addParticle(i, *map(float, d[i]))
+        f.setForceGroup(2)
+        system.addForce(f)
+        system.addForce(mm.MonteCarloBarostat(1.0, 300.0, 10000000))
+        ctx = mm.Context(system, mm.VerletIntegrator(0.001), mm.Platform.getPlatformByName("MI355X"))
+        return system, f, ctx
+
+    _, f1, c1 = make()
+    c1.setState(st)
+    for k, v in st.getParameters().items():
+        assert c1.getParameter(k) == v
+    assert c1.getParameter("ATMDirection") == 1.0          # absent from the checkpoint: unchanged
+    assert [tuple(v) for v in c1.getState(getPositions=True).getPeriodicBoxVectors()] == \
+        [tuple(v) for v in st.getPeriodicBoxVectors()]
+    s1 = c1.getState(getEnergy=True, getForces=True, groups={2})
+
+    _, f2, c2 = make()
+    a, b, c = st.getPeriodicBoxVectors()
+    c2.setPeriodicBoxVectors(a, b, c)
+    c2.setPositions(x)
+    for k, v in st.getParameters().items():
+        c2.setParameter(k, v)
+    s2 = c2.getState(getEnergy=True, getForces=True, groups={2})
+    assert s1.getPotentialEnergy() == s2.getPotentialEnergy()
+    assert np.array_equal(s1.getForces(asNumpy=True), s2.getForces(asNumpy=True))
+    assert f1.getPerturbationEnergy(c1) == f2.getPerturbationEnergy(c2)
+
+    # write a checkpoint from c1, restore it into a fresh context
+    c1.setParameter(f1.Lambda2(), 0.7)
+    ck = mm.XmlSerializer.serialize(c1.getState(getPositions=True, getVelocities=True, getParameters=True))
+    _, f3, c3 = make()
+    c3.setState(mm.XmlSerializer.deserialize(ck))
+    assert c3.getParameter("ATMLambda2") == 0.7
+    assert np.array_equal(c3.getState(getVelocities=True).getVelocities(asNumpy=True), st.getVelocities(asNumpy=True))
+    e1 = c1.getState(getEnergy=True, groups={2}).getPotentialEnergy()
+    e3 = c3.getState(getEnergy=True, groups={2}).getPotentialEnergy()
+    assert e1 == e3

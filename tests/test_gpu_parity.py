@@ -183,3 +183,71 @@ def test_graph_replays_match_eager(gpu):
             assert b.result(0)["state2_energy"] == pytest.approx(e0["state2_energy"], rel=1e-13)
             assert np.array_equal(b.forces(0), f0)
             b.synchronize()
+
+
+VARIANT_SYSTEMS = None
+
+
+def _variant_systems():
+    global VARIANT_SYSTEMS
+    if VARIANT_SYSTEMS is None:
+        p = W.ATMParams(0.3, 0.45, 0.1, 110.0, 1.5, 200.0, 100.0, 0.0625, 1.0)
+        VARIANT_SYSTEMS = [
+            W.tiny_system(64, 1.7, 0.8, 4, (0.6, 0.4, 0.3), 11, p),     # box < 2 (rc + column): no image shifts
+            W.tiny_system(3000, 3.1, 0.9, 30, (1.03, 0.77, 0.62), 1, p),
+            W.tiny_system(2500, 2.9, 0.9, 25, (0.21, 0.17, 0.13), 3, p),  # displaced ligand clashes
+        ] + W.uneven_batch(3, min_atoms=5000, max_atoms=12000, seed=5)
+    return VARIANT_SYSTEMS
+
+
+@pytest.mark.parametrize("kernel,waves", [(0, 1), (0, 2), (0, 4), (0, 8), (1, 1), (1, 2), (1, 4),
+                                          (2, 1), (2, 2), (2, 4), (3, 1), (3, 2), (3, 4)])
+def test_pair_kernel_variants(gpu, kernel, waves):
+    """Every pair-kernel variant (8x8 tiles; compacted, 8 or 16 entries per batch)
+    and block shape against the oracle; energy-only evaluation agrees."""
+    systems = _variant_systems()
+    with ATMBatch(systems) as b:
+        b.set_tuning(1.0, waves, 4, kernel)
+        b.compute()
+        for k, s in enumerate(systems):
+            check_system(b, k, s, pairs=(k < 2), tag=f"kernel {kernel} waves {waves}")
+        e = [b.result(k)["energy"] for k in range(len(systems))]
+        b.compute(forces=False)
+        for k in range(len(systems)):
+            assert b.result(k)["energy"] == pytest.approx(e[k], rel=1e-12)
+
+
+def test_compacted_kernel_forces_independent_of_batching(gpu):
+    """The compacted kernel sums 64-bit fixed-point pair forces with integer atomics:
+    the force sums do not depend on the batch size or block shape, bit for bit."""
+    systems = _variant_systems()[:2] + _variant_systems()[3:]   # no clash: every pair force < 2^27
+    ref = None
+    with ATMBatch(systems) as b:
+        for kernel, waves in ((1, 1), (1, 4), (2, 2), (2, 4), (3, 1), (3, 4)):
+            b.set_tuning(1.0, waves, 4, kernel)
+            b.compute()
+            f = [b.forces_fixed(k) for k in range(len(systems))]
+            if ref is None:
+                ref = f
+            else:
+                for a, r in zip(f, ref):
+                    assert np.array_equal(a, r)
+
+
+@pytest.mark.parametrize("kernel", [0, 2, 3])
+def test_clash_side_sums_cleared_between_evaluations(gpu, kernel):
+    """Pair forces beyond the fixed-point range go to FP64 side sums that are
+    cleared only after an evaluation used them.  The displacement alternates
+    between a small one and one that puts ligand atoms almost on top of others
+    (state-2 pair forces ~1e17 kJ/mol/nm); every evaluation matches."""
+    import dataclasses
+    p = W.ATMParams(0.3, 0.45, 0.1, 110.0, 1.5, 200.0, 100.0, 0.0625, 1.0)
+    small = W.tiny_system(2000, 2.8, 0.9, 20, (0.031, 0.027, 0.019), 8, p)
+    clash = dataclasses.replace(small, displacements=np.where(small.displacements != 0,
+                                                              W.quantize([1.3, 1.1, 0.9]), 0.0))
+    with ATMBatch([small]) as b:
+        b.set_tuning(1.0, 2, 4, kernel)
+        for s in (small, clash, small, clash):
+            b.set_displacements(0, s.displacements)
+            b.compute()
+            check_system(b, 0, s, pairs=False, tag=f"side sums kernel {kernel}")
