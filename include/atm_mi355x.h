@@ -107,9 +107,10 @@ typedef struct atm_tuning {
     int32_t pair_waves_per_block;
     int32_t zbins_per_column; /* z-bins per column edge */
     double best_ms;           /* measured time of the chosen configuration */
-    int32_t pair_kernel;      /* 0: 8x8 cluster-pair tiles; 1, 2, 3: compacted in-cutoff pairs
-                                 (FP64 pair function only) with 8, 16, 16 list entries per
-                                 batch, 3 drawing each chunk from 4 x 64 pending pairs */
+    int32_t pair_kernel;      /* 0: 8x8 cluster-pair tiles; 1-4: compacted in-cutoff pairs
+                                 (FP64 pair function only) with (list entries per batch,
+                                 copies of the j-force sums, drain after every test step)
+                                 = (4,2,yes), (4,2,no), (4,4,yes), (8,2,yes) */
     int32_t reserved;
 } atm_tuning;
 

@@ -82,9 +82,9 @@ struct DevBuf {
 
 struct Tuning {
     double colScale = 1.0;
-    int pairWaves = 2;
+    int pairWaves = 1;
     int zbinsPerColumn = 4;
-    int pairKernel = 2;
+    int pairKernel = 1;
 };
 
 struct Context {
@@ -142,6 +142,19 @@ Context& ctx_of(void* c) {
 }
 void check_sys(const Context& C, int s) {
     if (s < 0 || s >= C.nsys) fail(ATM_ERR_INVALID_ARGUMENT, "system index " + std::to_string(s) + " out of range");
+}
+
+// Every copy and clear is ordered on the context's stream: the stream is
+// created non-blocking, so work on the legacy default stream (plain hipMemset,
+// hipMemcpy) would not be ordered with the kernels.
+void zero_on_stream(Context& C, void* p, size_t bytes) { HIPCHK(hipMemsetAsync(p, 0, bytes, C.stream)); }
+void to_device(Context& C, void* dst, const void* src, size_t bytes) {
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, C.stream));
+    HIPCHK(hipStreamSynchronize(C.stream));
+}
+void to_host(Context& C, void* dst, const void* src, size_t bytes) {
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, C.stream));
+    HIPCHK(hipStreamSynchronize(C.stream));
 }
 
 void validate_desc(const atm_system_desc& d, int s) {
@@ -271,7 +284,7 @@ void alloc_layout(Context& C) {
     C.dSidx.alloc(sizeof(int) * C.totalSlots);
     C.dSf.alloc(sizeof(long long) * 3 * C.totalSlots);
     C.dSd.alloc(sizeof(double) * 3 * C.totalSlots);
-    HIPCHK(hipMemset(C.dSd.p, 0, sizeof(double) * 3 * C.totalSlots));  // side sums start clear
+    zero_on_stream(C, C.dSd.p, sizeof(double) * 3 * C.totalSlots);  // side sums start clear
     C.dBbc.alloc(sizeof(float4) * C.totalClusters);
     C.dBbh.alloc(sizeof(float4) * C.totalClusters);
     C.dGinfo.alloc(sizeof(int4) * C.totalGroups);
@@ -280,10 +293,10 @@ void alloc_layout(Context& C) {
     C.dEntries.alloc(sizeof(uint32_t) * C.totalEntries);
     C.dSetEnergy.alloc(sizeof(double) * S2);
     C.dGq.release(); C.dGsig.release(); C.dGeps.release();
-    HIPCHK(hipMemcpy(C.dDesc.p, C.sets.data(), sizeof(SetDesc) * S2, hipMemcpyHostToDevice));
-    HIPCHK(hipMemset(C.dRt.p, 0, sizeof(SetRuntime) * S2));
-    HIPCHK(hipMemset(C.dCellCount.p, 0, sizeof(int) * C.totalCells));
-    HIPCHK(hipMemset(C.dGcount.p, 0, sizeof(int) * C.totalGroups));
+    zero_on_stream(C, C.dRt.p, sizeof(SetRuntime) * S2);
+    zero_on_stream(C, C.dCellCount.p, sizeof(int) * C.totalCells);
+    zero_on_stream(C, C.dGcount.p, sizeof(int) * C.totalGroups);
+    to_device(C, C.dDesc.p, C.sets.data(), sizeof(SetDesc) * S2);
 
     DevPtrs& P = C.P;
     P.desc = C.dDesc.as<SetDesc>();
@@ -308,6 +321,7 @@ void alloc_layout(Context& C) {
     P.sprm64 = C.dSprm64.as<double4>();
     P.sidx = C.dSidx.as<int>();
     P.sf = C.dSf.as<long long>();
+    P.sfn = C.totalSlots;
     P.sd = C.dSd.as<double>();
     P.bbc = C.dBbc.as<float4>();
     P.bbh = C.dBbh.as<float4>();
@@ -444,7 +458,7 @@ void fetch_results(Context& C) {
     if (!C.evaluated) fail(ATM_ERR_STATE, "no evaluation has been run");
     HIPCHK(hipStreamSynchronize(C.stream));
     std::vector<SetRuntime> rt(2 * C.nsys);
-    HIPCHK(hipMemcpy(rt.data(), C.dRt.p, sizeof(SetRuntime) * rt.size(), hipMemcpyDeviceToHost));
+    to_host(C, rt.data(), C.dRt.p, sizeof(SetRuntime) * rt.size());
     bool overflow = false;
     for (int c = 0; c < 2 * C.nsys; ++c) {
         if (rt[c].overflow) {
@@ -462,7 +476,7 @@ void fetch_results(Context& C) {
     if (!C.resultsValid) {
         if (!C.resultsOnHost) {
             std::vector<SysResult> r(C.nsys);
-            HIPCHK(hipMemcpy(r.data(), C.dRes.p, sizeof(SysResult) * C.nsys, hipMemcpyDeviceToHost));
+            to_host(C, r.data(), C.dRes.p, sizeof(SysResult) * C.nsys);
             for (int s = 0; s < C.nsys; ++s) {
                 atm_result& o = C.results[s];
                 o.energy = r[s].W; o.perturbation_energy = r[s].usc; o.state1_energy = r[s].U1; o.state2_energy = r[s].U2;
@@ -535,12 +549,12 @@ int atm_create(const atm_system_desc* systems, int32_t num_systems, int32_t devi
         C->dParams.alloc(sizeof(AtmParamsDev) * num_systems);
         C->dRes.alloc(sizeof(SysResult) * num_systems);
         C->dOutF.alloc(sizeof(double) * 3 * off);
-        HIPCHK(hipMemset(C->dPosq.p, 0, sizeof(float4) * off));
-        HIPCHK(hipMemset(C->dDispl.p, 0, sizeof(float4) * off));
-        HIPCHK(hipMemset(C->dLjp.p, 0, sizeof(float2) * off));
-        HIPCHK(hipMemset(C->dPrm64.p, 0, sizeof(double4) * off));
-        HIPCHK(hipMemset(C->dOutF.p, 0, sizeof(double) * 3 * off));
-        HIPCHK(hipMemcpy(C->dParams.p, C->params.data(), sizeof(AtmParamsDev) * num_systems, hipMemcpyHostToDevice));
+        zero_on_stream(*C, C->dPosq.p, sizeof(float4) * off);
+        zero_on_stream(*C, C->dDispl.p, sizeof(float4) * off);
+        zero_on_stream(*C, C->dLjp.p, sizeof(float2) * off);
+        zero_on_stream(*C, C->dPrm64.p, sizeof(double4) * off);
+        zero_on_stream(*C, C->dOutF.p, sizeof(double) * 3 * off);
+        to_device(*C, C->dParams.p, C->params.data(), sizeof(AtmParamsDev) * num_systems);
         layout(*C);
         alloc_layout(*C);
         *out_context = C.release();
@@ -706,8 +720,7 @@ int atm_get_forces_fixed(void* context, int32_t sys, int64_t* f) {
         fetch_results(C);
         const int n = C.sys[sys].num_atoms;
         std::vector<double> soa(3ll * n);
-        HIPCHK(hipMemcpy(soa.data(), C.dOutF.as<double>() + 3ll * C.atomOff[sys], sizeof(double) * 3 * n,
-                         hipMemcpyDeviceToHost));
+        to_host(C, soa.data(), C.dOutF.as<double>() + 3ll * C.atomOff[sys], sizeof(double) * 3 * n);
         for (int i = 0; i < n; ++i)
             for (int k = 0; k < 3; ++k) {
                 const double v = std::fmin(std::fmax(soa[(size_t)k * n + i] * ATM_FORCE_SCALE, -9.2e18), 9.2e18);
@@ -725,8 +738,7 @@ int atm_get_forces(void* context, int32_t sys, double* f) {
         fetch_results(C);
         const int n = C.sys[sys].num_atoms;
         std::vector<double> soa(3ll * n);
-        HIPCHK(hipMemcpy(soa.data(), C.dOutF.as<double>() + 3ll * C.atomOff[sys], sizeof(double) * 3 * n,
-                         hipMemcpyDeviceToHost));
+        to_host(C, soa.data(), C.dOutF.as<double>() + 3ll * C.atomOff[sys], sizeof(double) * 3 * n);
         for (int i = 0; i < n; ++i)
             for (int k = 0; k < 3; ++k) f[3 * i + k] = soa[(size_t)k * n + i];
     });
@@ -755,16 +767,15 @@ int atm_get_pairs(void* context, int32_t sys, int32_t state, int32_t* pairs, int
         DevBuf out, ctr;
         out.alloc(sizeof(int2) * std::max<long long>(cap, 1));
         ctr.alloc(sizeof(unsigned long long) * 4);
-        HIPCHK(hipMemset(ctr.p, 0, sizeof(unsigned long long) * 4));
+        zero_on_stream(C, ctr.p, sizeof(unsigned long long) * 4);
         HIPCHK(launch_pairs_dump(C.P, c, C.sets[c].groupCap, out.as<int2>(), cap, ctr.as<unsigned long long>(), C.stream));
-        HIPCHK(hipStreamSynchronize(C.stream));
         unsigned long long h[4];
-        HIPCHK(hipMemcpy(h, ctr.p, sizeof(h), hipMemcpyDeviceToHost));
+        to_host(C, h, ctr.p, sizeof(h));
         *count = (int64_t)h[0];
         const long long nout = std::min<long long>((long long)h[0], cap);
         if (nout > 0) {
             std::vector<int2> v(nout);
-            HIPCHK(hipMemcpy(v.data(), out.p, sizeof(int2) * nout, hipMemcpyDeviceToHost));
+            to_host(C, v.data(), out.p, sizeof(int2) * nout);
             std::sort(v.begin(), v.end(), [](const int2& a, const int2& b) { return a.x != b.x ? a.x < b.x : a.y < b.y; });
             for (long long k = 0; k < nout; ++k) { pairs[2 * k] = v[k].x; pairs[2 * k + 1] = v[k].y; }
         }
@@ -780,17 +791,16 @@ int atm_get_stats(void* context, int64_t* pairs_in_cutoff, int64_t* pairs_evalua
         ctr.alloc(sizeof(unsigned long long) * 4);
         long long nin = 0, nev = 0, nent = 0;
         for (int c = 0; c < 2 * C.nsys; ++c) {
-            HIPCHK(hipMemset(ctr.p, 0, sizeof(unsigned long long) * 4));
+            zero_on_stream(C, ctr.p, sizeof(unsigned long long) * 4);
             HIPCHK(launch_pairs_dump(C.P, c, C.sets[c].groupCap, nullptr, 0, ctr.as<unsigned long long>(), C.stream));
-            HIPCHK(hipStreamSynchronize(C.stream));
             unsigned long long h[4];
-            HIPCHK(hipMemcpy(h, ctr.p, sizeof(h), hipMemcpyDeviceToHost));
+            to_host(C, h, ctr.p, sizeof(h));
             nin += (long long)h[0];
             nev += (long long)h[1];
             std::vector<int> gc(C.sets[c].groupCap);
-            HIPCHK(hipMemcpy(gc.data(), C.dGcount.as<int>() + C.sets[c].groupOff, sizeof(int) * gc.size(), hipMemcpyDeviceToHost));
+            to_host(C, gc.data(), C.dGcount.as<int>() + C.sets[c].groupOff, sizeof(int) * gc.size());
             SetRuntime rt;
-            HIPCHK(hipMemcpy(&rt, C.dRt.as<SetRuntime>() + c, sizeof(rt), hipMemcpyDeviceToHost));
+            to_host(C, &rt, C.dRt.as<SetRuntime>() + c, sizeof(rt));
             for (int g = 0; g < rt.ngroups; ++g) nent += gc[g];
         }
         if (pairs_in_cutoff) *pairs_in_cutoff = nin;
@@ -831,7 +841,7 @@ int atm_set_tuning(void* context, const atm_tuning* t) {
             t->pair_waves_per_block != 8)
             fail(ATM_ERR_INVALID_ARGUMENT, "pair_waves_per_block must be 1, 2, 4 or 8");
         if (t->zbins_per_column < 1 || t->zbins_per_column > 16) fail(ATM_ERR_INVALID_ARGUMENT, "zbins_per_column must be in [1, 16]");
-        if (t->pair_kernel < 0 || t->pair_kernel > 3) fail(ATM_ERR_INVALID_ARGUMENT, "pair_kernel must be 0, 1, 2 or 3");
+        if (t->pair_kernel < 0 || t->pair_kernel > 4) fail(ATM_ERR_INVALID_ARGUMENT, "pair_kernel must be in [0, 4]");
         if (t->pair_kernel != 0 && t->pair_waves_per_block == 8)
             fail(ATM_ERR_INVALID_ARGUMENT, "the compacted pair kernel takes 1, 2 or 4 waves per block");
         HIPCHK(hipSetDevice(C.device));
@@ -858,9 +868,9 @@ int atm_autotune(void* context, int32_t repeats, atm_tuning* out) {
         require_ready(C);
         HIPCHK(hipSetDevice(C.device));
         const int reps = std::max(1, (int)repeats);
-        const double scales[] = {0.8, 0.9, 1.0, 1.15, 1.3};
+        const double scales[] = {0.9, 1.0, 1.15, 1.3, 1.5};
         const int waves[] = {1, 2, 4};
-        const int kernels[] = {0, 1, 2, 3};
+        const int kernels[] = {0, 1, 2};
         const int zpcs[] = {2, 4};
         hipEvent_t e0, e1;
         HIPCHK(hipEventCreate(&e0));
@@ -875,7 +885,7 @@ int atm_autotune(void* context, int32_t repeats, atm_tuning* out) {
                 C.entryCapOverride.assign(2 * C.nsys, 0);
                 layout(C);
                 alloc_layout(C);
-                for (int kv = 0; kv < 4 * 3; ++kv) {
+                for (int kv = 0; kv < 3 * 3; ++kv) {
                     C.tuning.pairKernel = kernels[kv / 3];
                     C.tuning.pairWaves = waves[kv % 3];
                     C.dropGraphs();
@@ -1000,9 +1010,9 @@ int atm_backward(void* context, const double* grad_energy, double* d_charges, do
                                           C.dDeps.as<double>(), C.stream));
             HIPCHK(hipStreamSynchronize(C.stream));
             const size_t nb = sizeof(double) * C.totalAtoms;
-            if (d_charges) HIPCHK(hipMemcpy(d_charges, C.dDq.p, nb, hipMemcpyDeviceToHost));
-            if (d_sigmas) HIPCHK(hipMemcpy(d_sigmas, C.dDsig.p, nb, hipMemcpyDeviceToHost));
-            if (d_epsilons) HIPCHK(hipMemcpy(d_epsilons, C.dDeps.p, nb, hipMemcpyDeviceToHost));
+            if (d_charges) to_host(C, d_charges, C.dDq.p, nb);
+            if (d_sigmas) to_host(C, d_sigmas, C.dDsig.p, nb);
+            if (d_epsilons) to_host(C, d_epsilons, C.dDeps.p, nb);
         }
         if (d_params) {
             for (int s = 0; s < S; ++s) {

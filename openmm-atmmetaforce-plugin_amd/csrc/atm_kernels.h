@@ -33,7 +33,9 @@ struct DevPtrs {
     float2* sprm;
     double4* sprm64;
     int* sidx;
-    long long* sf;         // 64-bit fixed-point pair-force sums, (x, y, z) per slot
+    long long* sf;         // 64-bit fixed-point pair-force sums: x of slot s at sf[s], y at
+    long long sfn;         // sf[sfn + s], z at sf[2 sfn + s] (one array per component keeps
+                           // the atomics of consecutive slots on shared cache lines)
     double* sd;            // FP64 side sums, (x, y, z) per slot, of partial forces beyond the
                            // fixed-point range; read and cleared only when rt[c].big is set
     // clusters
@@ -55,8 +57,9 @@ hipError_t launch_zero_counts(const DevPtrs& P, int cellBegin, int cellEnd, int 
 hipError_t launch_build(const DevPtrs& P, int setBegin, int nsets, int maxAtoms, int maxCells, int maxSlots,
                         hipStream_t st);
 hipError_t launch_pairlist(const DevPtrs& P, int setBegin, int nsets, int maxGroups, int listWaves, hipStream_t st);
-// variant 0: 8x8 tile kernel; compacted kernel (FP64 pairs only) with 1: 8 list entries per batch,
-// 2: 16 entries, 3: 16 entries and chunks strided over 4 x 64 pending pairs
+// variant 0: 8x8 tile kernel; 1-4: compacted kernel (FP64 pairs only) with (list entries per batch,
+// copies of the j-force sums, drain after every test step) = (4, 2, yes), (4, 2, no), (4, 4, yes),
+// (8, 2, yes)
 hipError_t launch_pairs(const DevPtrs& P, int setBegin, int nsets, int maxGroups, int waves, bool forces, bool fp64,
                         int variant, hipStream_t st);
 hipError_t launch_energy(const DevPtrs& P, int sysBegin, int nsys, hipStream_t st);

@@ -64,12 +64,12 @@ __device__ __forceinline__ void atomic_add_fixed(long long* p, long long v) {
 // Add a partial force to a slot: 64-bit fixed point below kFixedMax, FP64 side
 // accumulator above (clashing pairs of the displaced state).
 __device__ __forceinline__ void add_force(const DevPtrs& P, int c, int slot, float fx, float fy, float fz) {
-    long long* f = P.sf + 3ll * slot;
+    long long* f = P.sf + slot;
     double* d = P.sd + 3ll * slot;
     bool big = false;
     if (fabsf(fx) < kFixedMax) atomic_add_fixed(f, to_fixed(fx)); else { atomicAdd(d, (double)fx); big = true; }
-    if (fabsf(fy) < kFixedMax) atomic_add_fixed(f + 1, to_fixed(fy)); else { atomicAdd(d + 1, (double)fy); big = true; }
-    if (fabsf(fz) < kFixedMax) atomic_add_fixed(f + 2, to_fixed(fz)); else { atomicAdd(d + 2, (double)fz); big = true; }
+    if (fabsf(fy) < kFixedMax) atomic_add_fixed(f + P.sfn, to_fixed(fy)); else { atomicAdd(d + 1, (double)fy); big = true; }
+    if (fabsf(fz) < kFixedMax) atomic_add_fixed(f + 2 * P.sfn, to_fixed(fz)); else { atomicAdd(d + 2, (double)fz); big = true; }
     if (big) P.rt[c].big = 1;
 }
 
@@ -225,8 +225,9 @@ __device__ __forceinline__ int wave_incl_scan(int v) {
 //  A: one wave per column (16 columns in flight), wave prefix sums over its
 //     z-bins -> cell starts relative to their column, and the column count;
 //  B: each thread scans a contiguous chunk of columns (padded slot counts and
-//     group counts), a block scan joins the chunks; column starts, dummy
-//     slots and the group table are written.
+//     group counts; it sums the cell counts of its columns itself, so no
+//     value passes between waves through global memory), a block scan joins
+//     the chunks; column starts, dummy slots and the group table are written.
 // A cell's first slot is colStart[column] + cellStart[cell].
 __global__ void __launch_bounds__(1024) k_scan(DevPtrs P, int setBegin) {
     constexpr int NT = 1024, NW = NT / 64;
@@ -247,13 +248,15 @@ __global__ void __launch_bounds__(1024) k_scan(DevPtrs P, int setBegin) {
         }
         if (lane == 0) P.colCount[D.colOff + col] = run;
     }
-    __syncthreads();
     const int chunk = (ncols + NT - 1) / NT;
     const int b = min(t * chunk, ncols), e = min(b + chunk, ncols);
     __shared__ int sS[NW], sG[NW];
     int accS = 0, accG = 0;
     for (int col = b; col < e; ++col) {
-        const int padded = (P.colCount[D.colOff + col] + 7) & ~7;
+        const int* cnt = P.cellCount + D.cellOff + (size_t)col * nz;
+        int n = 0;
+        for (int z = 0; z < nz; ++z) n += cnt[z];
+        const int padded = (n + 7) & ~7;
         accS += padded;
         accG += ((padded >> 3) + kGroupClusters - 1) / kGroupClusters;
     }
@@ -269,7 +272,9 @@ __global__ void __launch_bounds__(1024) k_scan(DevPtrs P, int setBegin) {
     }
     int offS = baseS + inS - accS, offG = baseG + inG - accG;
     for (int col = b; col < e; ++col) {
-        const int cnt = P.colCount[D.colOff + col];
+        const int* cc = P.cellCount + D.cellOff + (size_t)col * nz;
+        int cnt = 0;
+        for (int z = 0; z < nz; ++z) cnt += cc[z];
         const int padded = (cnt + 7) & ~7;
         const int ncl = padded >> 3;
         const int ng = (ncl + kGroupClusters - 1) / kGroupClusters;
@@ -353,7 +358,11 @@ __global__ void __launch_bounds__(256) k_gather_bbox(DevPtrs P, int setBegin) {
     if (s >= P.rt[c].nslots) return;
     const int k = D.slotOff + s;
     const int i = P.sidx[k];
-    float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
+    // dummy slots: zero parameters, and positions far outside the box and at
+    // least 4 nm from each other (x = 1e6 + 4 (s mod 2^20), y = 1e6 + 4 (s >> 20),
+    // exact in FP32), so no pair with a dummy passes a cutoff test unless x/y
+    // are minimum-imaged (sets without explicit image shifts test validity)
+    float4 p = make_float4(1.0e6f + 4.0f * (float)(s & 0xFFFFF), 1.0e6f + 4.0f * (float)(s >> 20), 0.f, 0.f);
     float2 q = make_float2(0.f, 0.f);
     double4 qd = make_double4(0.0, 0.0, 0.0, 0.0);
     if (i >= 0) {
@@ -365,7 +374,7 @@ __global__ void __launch_bounds__(256) k_gather_bbox(DevPtrs P, int setBegin) {
     P.spos[k] = p;
     P.sprm[k] = q;
     P.sprm64[k] = qd;
-    P.sf[3ll * k] = 0; P.sf[3ll * k + 1] = 0; P.sf[3ll * k + 2] = 0;
+    P.sf[k] = 0; P.sf[P.sfn + k] = 0; P.sf[2 * P.sfn + k] = 0;
     const float inf = __builtin_huge_valf();
     const float lox = oct_min(i >= 0 ? p.x : inf), hix = oct_max(i >= 0 ? p.x : -inf);
     const float loy = oct_min(i >= 0 ? p.y : inf), hiy = oct_max(i >= 0 ? p.y : -inf);
@@ -515,10 +524,10 @@ __device__ __forceinline__ JAtom load_j(const DevPtrs& P, const SetDesc& D, uint
 // components below kFixedMax) takes the branch-free fixed-point path.
 __device__ __forceinline__ void flush_force(const DevPtrs& P, int c, int slot, float fx, float fy, float fz) {
     if (fmaxf(fabsf(fx), fmaxf(fabsf(fy), fabsf(fz))) < 134217728.0f) {  // 2^27: |f scale| < 2^51
-        long long* f = P.sf + 3ll * slot;
+        long long* f = P.sf + slot;
         atomic_add_fixed(f, __double_as_longlong((double)fx * kForceScale + 6755399441055744.0) - 0x4338000000000000LL);
-        atomic_add_fixed(f + 1, __double_as_longlong((double)fy * kForceScale + 6755399441055744.0) - 0x4338000000000000LL);
-        atomic_add_fixed(f + 2, __double_as_longlong((double)fz * kForceScale + 6755399441055744.0) - 0x4338000000000000LL);
+        atomic_add_fixed(f + P.sfn, __double_as_longlong((double)fy * kForceScale + 6755399441055744.0) - 0x4338000000000000LL);
+        atomic_add_fixed(f + 2 * P.sfn, __double_as_longlong((double)fz * kForceScale + 6755399441055744.0) - 0x4338000000000000LL);
     } else {
         add_force(P, c, slot, fx, fy, fz);
     }
@@ -571,7 +580,7 @@ __global__ void __launch_bounds__(64 * W) k_pairs(DevPtrs P, int setBegin) {
     const float Lx = D.L[0], invLx = D.invL[0], Ly = D.L[1], invLy = D.invL[1];
     const bool shiftOK = D.shiftOK;
     const double rc2 = D.rc2, krf64 = D.krf64;
-    const float krf = D.krf;
+    const float krf = D.krf, rc2hi = D.rc2hi;
     float fi[kGroupClusters][3];
 #pragma unroll
     for (int t = 0; t < kGroupClusters; ++t) fi[t][0] = fi[t][1] = fi[t][2] = 0.f;
@@ -587,7 +596,9 @@ __global__ void __launch_bounds__(64 * W) k_pairs(DevPtrs P, int setBegin) {
             const uint32_t imask = (ent >> 20) & 0xFFu;
             const int js = D.slotOff + J * kClusterSize + jj;
             float4 pj = P.spos[js];
-            const bool jv = P.sidx[js] >= 0;
+            // with explicit x/y image shifts, dummy slots sit far outside the box
+            // (k_gather_bbox), so the FP64 path needs no validity test there
+            const bool jv = (F64 && shiftOK) ? true : P.sidx[js] >= 0;
             float2 qj = make_float2(0.f, 0.f);
             double4 jd = make_double4(0.0, 0.0, 0.0, 0.0);
             if (F64) jd = P.sprm64[js]; else qj = P.sprm[js];
@@ -605,19 +616,24 @@ __global__ void __launch_bounds__(64 * W) k_pairs(DevPtrs P, int setBegin) {
                     float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
                     if (!shiftOK) { dx = min_image(dx, Lx, invLx); dy = min_image(dy, Ly, invLy); }
                     dz = min_image(dz, Lz, invLz);
-                    bool ok = jv && ((ivm >> t) & 1u);
-                    if (t == tdiag) ok = ok && (ii < jj);
                     float fs;
                     if (F64) {
+                        // FP32 r^2 with a margin: a superset of the in-cutoff pairs, decides the early-out
+                        const float r2f = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                        bool cand = r2f < rc2hi;
+                        if (!shiftOK) cand = cand && jv && ((ivm >> t) & 1u);
+                        if (t == tdiag) cand = cand && (ii < jj);
+                        if (__builtin_amdgcn_ballot_w64(cand) == 0ull) continue;  // no pair of this 8x8 step is inside
                         const double4 id = sPrm64[wave][t * kClusterSize + ii];
                         // exact: the squares of 2^-20-grid differences fit in 53 bits
                         const double r2 = fma((double)dx, (double)dx, fma((double)dy, (double)dy, (double)dz * (double)dz));
-                        const bool in = ok && r2 < rc2;
-                        if (__ballot(in) == 0ull) continue;  // no pair of this 8x8 step is inside the cutoff
+                        const bool in = cand && r2 < rc2;
                         const PairE64 pe = pair_term64(in, r2, id.x + jd.x, id.y * jd.y, id.z * jd.z, krf64);
                         eE += pe.e; eQQ += pe.qq; eQR2 += pe.qq * r2;
                         fs = (float)pe.fs;
                     } else {
+                        bool ok = jv && ((ivm >> t) & 1u);
+                        if (t == tdiag) ok = ok && (ii < jj);
                         const float2 qi = sPrm[wave][t * kClusterSize + ii];
                         const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
                         const bool in = ok && in_cutoff(D, dx, dy, dz, r2);
@@ -635,7 +651,7 @@ __global__ void __launch_bounds__(64 * W) k_pairs(DevPtrs P, int setBegin) {
             }
             if (FORCES) {
                 fjx = oct_sum(fjx); fjy = oct_sum(fjy); fjz = oct_sum(fjz);
-                if (ii == 0 && jv) flush_force(P, c, js, fjx, fjy, fjz);
+                if (ii == 0 && jv && (fjx != 0.f || fjy != 0.f || fjz != 0.f)) flush_force(P, c, js, fjx, fjy, fjz);
             }
         }
     }
@@ -683,24 +699,30 @@ __device__ __forceinline__ void lds_sub_u64(unsigned long long* p, unsigned long
 //         FP32 r^2 and a margin above rc^2 (it never drops a pair inside the
 //         cutoff), and appends the passing pairs as (i, j) codes to a ring in
 //         LDS (ballot + mbcnt: order = entry, t, lane);
-//  dense: whenever 64 DR pairs are pending (and at the end of the batch) the
-//         wave drains them in chunks of 64: lane l of chunk q takes pending
-//         pair l m + q (m chunks), decides r < rc on the exact FP64 r^2 and
-//         evaluates the FP64 pair function, so the expensive part runs on
-//         in-cutoff pairs instead of whole 8x8 tiles.  The stride DR spreads
-//         a chunk over about DR list entries, so that fewer of its lanes add
-//         into the same j-atom's LDS accumulator (same-address atomics
-//         serialise).
+//  dense: whenever 64 pairs are pending each lane takes one queued pair,
+//         decides r < rc on the exact FP64 r^2 and evaluates the FP64 pair
+//         function, so the expensive part runs on in-cutoff pairs instead of
+//         whole 8x8 tiles.
+// The j-atoms of a batch are staged in one half of a two-half LDS buffer, so
+// a batch's pairs can still be pending while the next batch loads: chunks stay
+// full across batch boundaries.
 // Forces: per pair FP64 components -> 64-bit fixed point -> integer LDS
-// atomics into per-atom accumulators of the group (i) and of the batch (j),
-// flushed to the global fixed-point accumulators with integer atomics.
-// Integer sums do not depend on the order of the atomics, so results are
-// bit-reproducible.  Components >= 2^27 kJ/mol/nm take FP64 global atomics.
-template <int W, int JB, int DR, bool FORCES>
+// atomics into per-atom accumulators of the group (i) and of the batches (j),
+// flushed to the global fixed-point accumulators with integer atomics.  The j
+// accumulators exist in NCOPY copies (copy = lane % NCOPY, rows in different
+// banks): a chunk holds few distinct j-atoms, and same-address LDS atomics
+// serialise.  Integer sums do not depend on the order of the atomics, so
+// results are bit-reproducible.  Components >= 2^27 kJ/mol/nm take FP64
+// global atomics.
+// STEP: drain after every test step (the queue then never holds more than 127
+// pairs and needs 128 slots instead of 1024; the test loop runs over the set
+// bits of the mask and reads the i-positions from LDS).
+template <int W, int JB, int NCOPY, bool STEP, bool FORCES>
 __global__ void __launch_bounds__(64 * W) k_pairs_q(DevPtrs P, int setBegin) {
-    constexpr int NJ = JB * kClusterSize;  // j-atoms per batch
-    constexpr int QCAP = 1024;             // pending pairs < 64 DR + 8 * 64
-    static_assert(64 * DR + 512 <= QCAP, "pair ring too small");
+    constexpr int NH = JB * kClusterSize;  // j-atoms per batch (one half of the j buffer)
+    constexpr int NJ = 2 * NH;
+    constexpr int JS = NJ + 1;             // odd row stride of the j accumulators
+    constexpr int QCAP = STEP ? 128 : 1024;  // pending pairs < 64 + 64 (STEP) or 64 + 8 * 64
     const int c = setBegin + blockIdx.y;
     const SetDesc D = P.desc[c];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -708,13 +730,13 @@ __global__ void __launch_bounds__(64 * W) k_pairs_q(DevPtrs P, int setBegin) {
     const int g = blockIdx.x * W + wave;
     if (g >= P.rt[c].ngroups) return;  // waves synchronise only within themselves
 
-    __shared__ float4 sIPos[W][64];
-    __shared__ double4 sIPrm[W][64];
-    __shared__ float4 sJPos[W][NJ];    // w = 1: real atom, 0: dummy or past the list
-    __shared__ double4 sJPrm[W][NJ];
+    __shared__ float sIX[W][64], sIY[W][64], sIZ[W][64];
+    __shared__ double sIH[W][64], sIE[W][64], sIQ[W][64];  // sigma/2, 2 sqrt(eps), q sqrt(k_e)
+    __shared__ float sJX[W][NJ], sJY[W][NJ], sJZ[W][NJ], sJV[W][NJ];  // V = 1: real atom
+    __shared__ double sJH[W][NJ], sJE[W][NJ], sJQ[W][NJ];
     __shared__ int sJSlot[W][NJ];
     __shared__ unsigned long long sIF[W][3][FORCES ? 64 : 1];
-    __shared__ unsigned long long sJF[W][3][FORCES ? NJ : 1];
+    __shared__ unsigned long long sJF[W][FORCES ? NCOPY * 3 * JS : 1];
     __shared__ unsigned short sQ[W][QCAP];
 
     const int4 gi = P.ginfo[D.groupOff + g];
@@ -729,8 +751,8 @@ __global__ void __launch_bounds__(64 * W) k_pairs_q(DevPtrs P, int setBegin) {
             q = P.sprm64[sBase + lane];
             iv = P.sidx[sBase + lane] >= 0;
         }
-        sIPos[wave][lane] = p;
-        sIPrm[wave][lane] = q;
+        sIX[wave][lane] = p.x; sIY[wave][lane] = p.y; sIZ[wave][lane] = p.z;
+        sIH[wave][lane] = q.x; sIE[wave][lane] = q.y; sIQ[wave][lane] = q.z;
         if (FORCES) sIF[wave][0][lane] = sIF[wave][1][lane] = sIF[wave][2][lane] = 0ull;
     }
     const uint64_t ivb = __ballot(iv);
@@ -738,11 +760,14 @@ __global__ void __launch_bounds__(64 * W) k_pairs_q(DevPtrs P, int setBegin) {
 #pragma unroll
     for (int t = 0; t < kGroupClusters; ++t) ivm |= (uint32_t)((ivb >> (t * kClusterSize + ii)) & 1ull) << t;
     wave_sync();
-    float pix[kGroupClusters], piy[kGroupClusters], piz[kGroupClusters];
+    float pix[STEP ? 1 : kGroupClusters], piy[STEP ? 1 : kGroupClusters], piz[STEP ? 1 : kGroupClusters];
+    if (!STEP) {
 #pragma unroll
-    for (int t = 0; t < kGroupClusters; ++t) {
-        const float4 a = sIPos[wave][t * kClusterSize + ii];
-        pix[t] = a.x; piy[t] = a.y; piz[t] = a.z;
+        for (int t = 0; t < kGroupClusters; ++t) {
+            pix[t] = sIX[wave][t * kClusterSize + ii];
+            piy[t] = sIY[wave][t * kClusterSize + ii];
+            piz[t] = sIZ[wave][t * kClusterSize + ii];
+        }
     }
 
     const float Lx = D.L[0], invLx = D.invL[0], Ly = D.L[1], invLy = D.invL[1], Lz = D.L[2], invLz = D.invL[2];
@@ -752,134 +777,176 @@ __global__ void __launch_bounds__(64 * W) k_pairs_q(DevPtrs P, int setBegin) {
     double eE = 0.0, eQQ = 0.0, eQR2 = 0.0;
     const int ne = __builtin_amdgcn_readfirstlane(P.gcount[D.groupOff + g]);
     const uint32_t* E = P.entries + D.entryOff + (long long)g * D.entryCap;
+    int qh = 0, qn = 0;  // ring head and tail (wave-uniform, never reset)
 
-    for (int b0 = 0; b0 < ne; b0 += JB) {
-        const int nb = min(JB, ne - b0);
-        uint32_t entReg[JB / 8];
-        // load the batch's j-clusters: lane l of round r holds atom (l & 7) of entry r * 8 + (l >> 3)
+    // Evaluate the n (<= 64) pairs at the ring head.
+    auto dense = [&](int n) {
+        wave_sync();
+        const bool act = lane < n;
+        const int code = act ? (int)sQ[wave][(qh + lane) & (QCAP - 1)] : 0;
+        const int i = code & 63, j = code >> 6;
+        float dx = sIX[wave][i] - sJX[wave][j], dy = sIY[wave][i] - sJY[wave][j], dz = sIZ[wave][i] - sJZ[wave][j];
+        if (!shiftOK) { dx = min_image(dx, Lx, invLx); dy = min_image(dy, Ly, invLy); }
+        dz = min_image(dz, Lz, invLz);
+        const double dxd = dx, dyd = dy, dzd = dz;
+        // exact: the squares of 2^-20-grid differences fit in 53 bits
+        const double r2 = fma(dxd, dxd, fma(dyd, dyd, dzd * dzd));
+        const bool in = act && r2 < rc2;
+        const PairE64 pe = pair_term64(in, r2, sIH[wave][i] + sJH[wave][j], sIE[wave][i] * sJE[wave][j],
+                                       sIQ[wave][i] * sJQ[wave][j], krf64);
+        eE += pe.e; eQQ += pe.qq; eQR2 += pe.qq * r2;
+        if (FORCES && in) {
+            const double fx = pe.fs * dxd, fy = pe.fs * dyd, fz = pe.fs * dzd;
+            if (fmax(fabs(fx), fmax(fabs(fy), fabs(fz))) < 134217728.0) {  // 2^27
+                const unsigned long long vx = fixed64(fx), vy = fixed64(fy), vz = fixed64(fz);
+                lds_add_u64(&sIF[wave][0][i], vx);
+                lds_add_u64(&sIF[wave][1][i], vy);
+                lds_add_u64(&sIF[wave][2][i], vz);
+                unsigned long long* jf = &sJF[wave][(lane % NCOPY) * 3 * JS + j];
+                lds_sub_u64(jf, vx);
+                lds_sub_u64(jf + JS, vy);
+                lds_sub_u64(jf + 2 * JS, vz);
+            } else {
+                double* di = P.sd + 3ll * (sBase + i);
+                double* dj = P.sd + 3ll * sJSlot[wave][j];
+                atomicAdd(di, fx); atomicAdd(di + 1, fy); atomicAdd(di + 2, fz);
+                atomicAdd(dj, -fx); atomicAdd(dj + 1, -fy); atomicAdd(dj + 2, -fz);
+                P.rt[c].big = 1;
+            }
+        }
+        qh += n;
+    };
+    // Sum the copies of the j-forces of the batch in half h into the global accumulators.
+    auto flush_half = [&](int h) {
+        if (!FORCES) return;
 #pragma unroll
-        for (int r = 0; r < JB / 8; ++r) {
-            const int k = r * 8 + jj;
-            uint32_t ent = 0u;
-            float4 pj = make_float4(0.f, 0.f, 0.f, 0.f);
-            double4 prm = make_double4(0.0, 0.0, 0.0, 0.0);
-            int js = 0;
-            bool jv = false;
-            if (k < nb) {
-                ent = E[b0 + k];
-                js = D.slotOff + (int)(ent & 0xFFFFFu) * kClusterSize + ii;
-                pj = P.spos[js];
-                prm = P.sprm64[js];
-                jv = P.sidx[js] >= 0;
-                if (shiftOK) {
-                    const uint32_t sxc = (ent >> 28) & 3u, syc = (ent >> 30) & 3u;
-                    pj.x += sxc == 1u ? Lx : (sxc == 2u ? -Lx : 0.f);
-                    pj.y += syc == 1u ? Ly : (syc == 2u ? -Ly : 0.f);
+        for (int r = 0; r < (NH + 63) / 64; ++r) {
+            const int x = h * NH + r * 64 + lane;
+            if (r * 64 + lane < NH && sJV[wave][x] != 0.f) {
+                unsigned long long v[3] = {0ull, 0ull, 0ull};
+#pragma unroll
+                for (int cp = 0; cp < NCOPY; ++cp)
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) v[k] += sJF[wave][(cp * 3 + k) * JS + x];
+                if (v[0] | v[1] | v[2]) {
+                    unsigned long long* f = reinterpret_cast<unsigned long long*>(P.sf) + sJSlot[wave][x];
+                    atomicAdd(f, v[0]);
+                    atomicAdd(f + P.sfn, v[1]);
+                    atomicAdd(f + 2 * P.sfn, v[2]);
                 }
             }
-            pj.w = jv ? 1.f : 0.f;
-            entReg[r] = ent;
-            sJPos[wave][r * 64 + lane] = pj;
-            sJPrm[wave][r * 64 + lane] = prm;
-            sJSlot[wave][r * 64 + lane] = js;
-            if (FORCES) sJF[wave][0][r * 64 + lane] = sJF[wave][1][r * 64 + lane] = sJF[wave][2][r * 64 + lane] = 0ull;
+        }
+    };
+
+    int bstart0 = 0, bstart1 = 0;  // ring position where the batch in half 0 / 1 started
+    int bi = 0;
+    for (int b0 = 0; b0 < ne; b0 += JB, ++bi) {
+        const int h = bi & 1;
+        const int nb = min(JB, ne - b0);
+        if (bi >= 2) {
+            // half h still holds batch bi - 2, whose pairs end where batch bi - 1 began
+            if (qh < (h ? bstart0 : bstart1)) dense(qn - qh);
+            wave_sync();
+            flush_half(h);
         }
         wave_sync();
-        int qh = 0, qn = 0;  // ring head and tail (wave-uniform)
+        uint32_t entReg = 0u;  // lane l holds entry l >> 3 of the batch
+        {
+            const int k = jj;
+            const int x = h * NH + lane;
+            if (lane < NH) {
+                uint32_t ent = 0u;
+                float4 pj = make_float4(0.f, 0.f, 0.f, 0.f);
+                double4 prm = make_double4(0.0, 0.0, 0.0, 0.0);
+                int js = 0;
+                bool jv = false;
+                if (k < nb) {
+                    ent = E[b0 + k];
+                    js = D.slotOff + (int)(ent & 0xFFFFFu) * kClusterSize + ii;
+                    pj = P.spos[js];
+                    prm = P.sprm64[js];
+                    jv = P.sidx[js] >= 0;
+                    if (shiftOK) {
+                        const uint32_t sxc = (ent >> 28) & 3u, syc = (ent >> 30) & 3u;
+                        pj.x += sxc == 1u ? Lx : (sxc == 2u ? -Lx : 0.f);
+                        pj.y += syc == 1u ? Ly : (syc == 2u ? -Ly : 0.f);
+                    }
+                }
+                entReg = ent;
+                sJX[wave][x] = pj.x; sJY[wave][x] = pj.y; sJZ[wave][x] = pj.z; sJV[wave][x] = jv ? 1.f : 0.f;
+                sJH[wave][x] = prm.x; sJE[wave][x] = prm.y; sJQ[wave][x] = prm.z;
+                sJSlot[wave][x] = js;
+                if (FORCES) {
+#pragma unroll
+                    for (int r = 0; r < NCOPY * 3; ++r) sJF[wave][r * JS + x] = 0ull;
+                }
+            }
+        }
+        if (h) bstart1 = qn; else bstart0 = qn;
+        wave_sync();
         for (int k = 0; k < nb; ++k) {
-            const uint32_t ent = __builtin_amdgcn_readlane(JB == 8 ? entReg[0] : ((k >> 3) ? entReg[JB / 8 - 1] : entReg[0]),
-                                                           (k & 7) * 8);
+            const uint32_t ent = __builtin_amdgcn_readlane(entReg, k * 8);
             const int td = (int)(ent & 0xFFFFFu) - g0;  // the i-cluster equal to J, if any
             const uint32_t imask = (ent >> 20) & 0xFFu;
-            const float4 pj = sJPos[wave][k * 8 + jj];
-            uint32_t ok = pj.w != 0.f ? ivm : 0u;
+            const int jx = h * NH + k * 8 + jj;
+            const float pjx = sJX[wave][jx], pjy = sJY[wave][jx], pjz = sJZ[wave][jx];
+            uint32_t ok = sJV[wave][jx] != 0.f ? ivm : 0u;
             if (td >= 0 && td < kGroupClusters && ii >= jj) ok &= ~(1u << td);
-            const unsigned short code0 = (unsigned short)(ii | ((k * 8 + jj) << 6));
-#pragma unroll
-            for (int t = 0; t < kGroupClusters; ++t) {
-                if (!(imask & (1u << t))) continue;
-                float dx = pix[t] - pj.x, dy = piy[t] - pj.y, dz = piz[t] - pj.z;
-                if (!shiftOK) { dx = min_image(dx, Lx, invLx); dy = min_image(dy, Ly, invLy); }
-                dz = min_image(dz, Lz, invLz);
-                const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-                const bool in = r2 < rc2hi && ((ok >> t) & 1u);
-                const uint64_t m = __ballot(in);
-                if (m == 0ull) continue;
-                if (in) {
-                    const int pos = qn + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
-                                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-                    sQ[wave][pos & (QCAP - 1)] = (unsigned short)(code0 + t * 8);
-                }
-                qn += __popcll(m);
-            }
-            const bool last = k == nb - 1;
-            while (qn - qh >= 64 * DR || (last && qn > qh)) {
-              const int n = min(64 * DR, qn - qh);
-              const int m = (n + 63) >> 6;  // chunks of this drain
-              for (int q = 0; q < m; ++q) {
-                wave_sync();
-                const int e = lane * m + q;
-                const bool act = e < n;
-                const int code = act ? (int)sQ[wave][(qh + e) & (QCAP - 1)] : 0;
-                const int i = code & 63, j = code >> 6;
-                const float4 pi = sIPos[wave][i], pq = sJPos[wave][j];
-                float dx = pi.x - pq.x, dy = pi.y - pq.y, dz = pi.z - pq.z;
-                if (!shiftOK) { dx = min_image(dx, Lx, invLx); dy = min_image(dy, Ly, invLy); }
-                dz = min_image(dz, Lz, invLz);
-                const double dxd = dx, dyd = dy, dzd = dz;
-                // exact: the squares of 2^-20-grid differences fit in 53 bits
-                const double r2 = fma(dxd, dxd, fma(dyd, dyd, dzd * dzd));
-                const bool in = act && r2 < rc2;
-                const double4 a = sIPrm[wave][i], b = sJPrm[wave][j];
-                const PairE64 pe = pair_term64(in, r2, a.x + b.x, a.y * b.y, a.z * b.z, krf64);
-                eE += pe.e; eQQ += pe.qq; eQR2 += pe.qq * r2;
-                if (FORCES && in) {
-                    const double fx = pe.fs * dxd, fy = pe.fs * dyd, fz = pe.fs * dzd;
-                    if (fmax(fabs(fx), fmax(fabs(fy), fabs(fz))) < 134217728.0) {  // 2^27
-                        const unsigned long long vx = fixed64(fx), vy = fixed64(fy), vz = fixed64(fz);
-                        lds_add_u64(&sIF[wave][0][i], vx);
-                        lds_add_u64(&sIF[wave][1][i], vy);
-                        lds_add_u64(&sIF[wave][2][i], vz);
-                        lds_sub_u64(&sJF[wave][0][j], vx);
-                        lds_sub_u64(&sJF[wave][1][j], vy);
-                        lds_sub_u64(&sJF[wave][2][j], vz);
-                    } else {
-                        double* di = P.sd + 3ll * (sBase + i);
-                        double* dj = P.sd + 3ll * sJSlot[wave][j];
-                        atomicAdd(di, fx); atomicAdd(di + 1, fy); atomicAdd(di + 2, fz);
-                        atomicAdd(dj, -fx); atomicAdd(dj + 1, -fy); atomicAdd(dj + 2, -fz);
-                        P.rt[c].big = 1;
+            const unsigned short code0 = (unsigned short)(ii | (jx << 6));
+            if (STEP) {
+                uint32_t live = imask;
+                while (live) {
+                    const int t = __builtin_ctz(live);
+                    live &= live - 1u;
+                    const int ia = t * kClusterSize + ii;
+                    float dx = sIX[wave][ia] - pjx, dy = sIY[wave][ia] - pjy, dz = sIZ[wave][ia] - pjz;
+                    if (!shiftOK) { dx = min_image(dx, Lx, invLx); dy = min_image(dy, Ly, invLy); }
+                    dz = min_image(dz, Lz, invLz);
+                    const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                    const bool in = r2 < rc2hi && ((ok >> t) & 1u);
+                    const uint64_t m = __ballot(in);
+                    if (m == 0ull) continue;
+                    if (in) {
+                        const int pos = qn + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                                             __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+                        sQ[wave][pos & (QCAP - 1)] = (unsigned short)(code0 + t * 8);
                     }
+                    qn += __popcll(m);
+                    if (qn - qh >= 64) dense(64);
                 }
-              }
-              qh += n;
+            } else {
+#pragma unroll
+                for (int t = 0; t < kGroupClusters; ++t) {
+                    if (!(imask & (1u << t))) continue;
+                    float dx = pix[t] - pjx, dy = piy[t] - pjy, dz = piz[t] - pjz;
+                    if (!shiftOK) { dx = min_image(dx, Lx, invLx); dy = min_image(dy, Ly, invLy); }
+                    dz = min_image(dz, Lz, invLz);
+                    const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                    const bool in = r2 < rc2hi && ((ok >> t) & 1u);
+                    const uint64_t m = __ballot(in);
+                    if (m == 0ull) continue;
+                    if (in) {
+                        const int pos = qn + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                                             __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+                        sQ[wave][pos & (QCAP - 1)] = (unsigned short)(code0 + t * 8);
+                    }
+                    qn += __popcll(m);
+                }
+                while (qn - qh >= 64) dense(64);
             }
         }
-        wave_sync();
-        if (FORCES) {
-#pragma unroll
-            for (int r = 0; r < JB / 8; ++r) {
-                const int x = r * 64 + lane;
-                if (sJPos[wave][x].w != 0.f) {
-                    const unsigned long long vx = sJF[wave][0][x], vy = sJF[wave][1][x], vz = sJF[wave][2][x];
-                    unsigned long long* f = reinterpret_cast<unsigned long long*>(P.sf) + 3ll * sJSlot[wave][x];
-                    if (vx | vy | vz) {
-                        atomicAdd(f, vx);
-                        atomicAdd(f + 1, vy);
-                        atomicAdd(f + 2, vz);
-                    }
-                }
-            }
-        }
-        wave_sync();  // the next batch overwrites the j arrays
     }
+    if (qn > qh) dense(qn - qh);
+    wave_sync();
+    if (bi >= 1) flush_half((bi - 1) & 1);
+    if (bi >= 2) flush_half(bi & 1);
     if (FORCES && iv) {
         const unsigned long long vx = sIF[wave][0][lane], vy = sIF[wave][1][lane], vz = sIF[wave][2][lane];
         if (vx | vy | vz) {
-            unsigned long long* f = reinterpret_cast<unsigned long long*>(P.sf) + 3ll * (sBase + lane);
+            unsigned long long* f = reinterpret_cast<unsigned long long*>(P.sf) + (sBase + lane);
             atomicAdd(f, vx);
-            atomicAdd(f + 1, vy);
-            atomicAdd(f + 2, vz);
+            atomicAdd(f + P.sfn, vy);
+            atomicAdd(f + 2 * P.sfn, vz);
         }
     }
     double eacc = eE + D.krf64 * eQR2 - D.crf64 * eQQ;
@@ -994,8 +1061,8 @@ __global__ void k_blend(DevPtrs P, int sysBegin, int hostMode, double spHost, do
     const long long a = D0.slotOff + P.slotOf[D0.setAtomOff + i];
     double* F = P.outF + 3ll * D0.atomOff;
     const double inv = 1.0 / kForceScale;
-    const long long* fa = P.sf + 3 * a;
-    double f1x = (double)fa[0] * inv, f1y = (double)fa[1] * inv, f1z = (double)fa[2] * inv;
+    const long long* fa = P.sf + a;
+    double f1x = (double)fa[0] * inv, f1y = (double)fa[P.sfn] * inv, f1z = (double)fa[2 * P.sfn] * inv;
     if (P.rt[2 * s].big) {
         const double* d = P.sd + 3 * a;
         f1x += d[0]; f1y += d[1]; f1z += d[2];
@@ -1007,8 +1074,8 @@ __global__ void k_blend(DevPtrs P, int sysBegin, int hostMode, double spHost, do
         return;
     }
     const long long b = D1.slotOff + P.slotOf[D1.setAtomOff + i];
-    const long long* fb = P.sf + 3 * b;
-    double f2x = (double)fb[0] * inv, f2y = (double)fb[1] * inv, f2z = (double)fb[2] * inv;
+    const long long* fb = P.sf + b;
+    double f2x = (double)fb[0] * inv, f2y = (double)fb[P.sfn] * inv, f2z = (double)fb[2 * P.sfn] * inv;
     if (P.rt[2 * s + 1].big) {
         const double* d = P.sd + 3 * b;
         f2x += d[0]; f2y += d[1]; f2z += d[2];
@@ -1057,16 +1124,21 @@ hipError_t launch_pairlist(const DevPtrs& P, int setBegin, int nsets, int maxGro
 hipError_t launch_pairs(const DevPtrs& P, int setBegin, int nsets, int maxGroups, int waves, bool forces, bool fp64,
                         int variant, hipStream_t st) {
     if (fp64 && variant != 0) {
-#define ATM_PAIRSQ(WV, JB, DR)                                                                                     \
-    if (forces) k_pairs_q<WV, JB, DR, true><<<dim3(cdiv(maxGroups, WV), nsets), 64 * WV, 0, st>>>(P, setBegin);  \
-    else k_pairs_q<WV, JB, DR, false><<<dim3(cdiv(maxGroups, WV), nsets), 64 * WV, 0, st>>>(P, setBegin);
-#define ATM_PAIRSQ_W(JB, DR)                 \
-    switch (waves) {                         \
-        case 1: ATM_PAIRSQ(1, JB, DR) break; \
-        case 2: ATM_PAIRSQ(2, JB, DR) break; \
-        default: ATM_PAIRSQ(4, JB, DR) break; \
+#define ATM_PAIRSQ(WV, JB, NC, SD)                                                                                     \
+    if (forces) k_pairs_q<WV, JB, NC, SD, true><<<dim3(cdiv(maxGroups, WV), nsets), 64 * WV, 0, st>>>(P, setBegin);  \
+    else k_pairs_q<WV, JB, NC, SD, false><<<dim3(cdiv(maxGroups, WV), nsets), 64 * WV, 0, st>>>(P, setBegin);
+#define ATM_PAIRSQ_W(JB, NC, SD)                 \
+    switch (waves) {                             \
+        case 1: ATM_PAIRSQ(1, JB, NC, SD) break; \
+        case 2: ATM_PAIRSQ(2, JB, NC, SD) break; \
+        default: ATM_PAIRSQ(4, JB, NC, SD) break; \
     }
-        if (variant == 1) { ATM_PAIRSQ_W(8, 1) } else if (variant == 2) { ATM_PAIRSQ_W(16, 1) } else { ATM_PAIRSQ_W(16, 4) }
+        switch (variant) {
+            case 1: ATM_PAIRSQ_W(4, 2, true) break;
+            case 2: ATM_PAIRSQ_W(4, 2, false) break;
+            case 3: ATM_PAIRSQ_W(4, 4, true) break;
+            default: ATM_PAIRSQ_W(8, 2, true) break;
+        }
 #undef ATM_PAIRSQ_W
 #undef ATM_PAIRSQ
         return hipGetLastError();

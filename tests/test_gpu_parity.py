@@ -29,6 +29,23 @@ def frms(F, Fref):
     return float(np.linalg.norm(F - Fref) / max(np.linalg.norm(Fref), 1e-300))
 
 
+def report_pair_mismatch(batch, k, state, x, system, pg, po):
+    """Print what a pair-set mismatch looks like before the assertion fails."""
+    a, g = set(map(tuple, po)), set(map(tuple, pg))
+    miss, extra = sorted(a - g), sorted(g - a)
+    count = {}
+    for p in miss + extra:
+        for i in p:
+            count[i] = count.get(i, 0) + 1
+    top = sorted(count.items(), key=lambda kv: -kv[1])[:8]
+    print(f"    pair mismatch sys {k} state {state}: missing {len(miss)} extra {len(extra)}; "
+          f"atoms most involved {top}; their wrapped positions "
+          f"{[tuple(np.round(np.mod(x[i], system.box), 4)) for i, _ in top[:4]]}", flush=True)
+    again = batch.pairs(k, state)
+    print(f"    second read of the same list: {len(again)} pairs, identical {np.array_equal(again, pg)}; "
+          f"stats {batch.stats()}", flush=True)
+
+
 def check_system(batch, k, system, pairs=True, tag="", strict=True, forces=True, ref=None):
     if ref is None:
         ref, Fref = oracle.atm_eval(system, forces=forces)
@@ -61,6 +78,8 @@ def check_system(batch, k, system, pairs=True, tag="", strict=True, forces=True,
         for state, x in ((0, system.positions), (1, system.positions + system.displacements)):
             pg = batch.pairs(k, state)
             po = oracle.pairs(x, system.box, system.cutoff)
+            if pg.shape != po.shape or not np.array_equal(pg, po):
+                report_pair_mismatch(batch, k, state, x, system, pg, po)
             assert pg.shape == po.shape, f"state {state}: {pg.shape[0]} pairs vs oracle {po.shape[0]}"
             assert np.array_equal(pg, po)
     return r, ref
@@ -200,11 +219,12 @@ def _variant_systems():
     return VARIANT_SYSTEMS
 
 
-@pytest.mark.parametrize("kernel,waves", [(0, 1), (0, 2), (0, 4), (0, 8), (1, 1), (1, 2), (1, 4),
-                                          (2, 1), (2, 2), (2, 4), (3, 1), (3, 2), (3, 4)])
+@pytest.mark.parametrize("kernel,waves", [(0, 1), (0, 2), (0, 4), (0, 8), (1, 1), (1, 2), (1, 4), (2, 1), (2, 2),
+                                          (2, 4), (3, 1), (3, 2), (4, 1), (4, 2), (4, 4)])
 def test_pair_kernel_variants(gpu, kernel, waves):
-    """Every pair-kernel variant (8x8 tiles; compacted, 8 or 16 entries per batch)
-    and block shape against the oracle; energy-only evaluation agrees."""
+    """Every pair-kernel variant (8x8 tiles; compacted with 8 or 4 list entries
+    per batch and 1, 2 or 4 copies of the j-force sums) and block shape against
+    the oracle; energy-only evaluation agrees."""
     systems = _variant_systems()
     with ATMBatch(systems) as b:
         b.set_tuning(1.0, waves, 4, kernel)
@@ -223,7 +243,7 @@ def test_compacted_kernel_forces_independent_of_batching(gpu):
     systems = _variant_systems()[:2] + _variant_systems()[3:]   # no clash: every pair force < 2^27
     ref = None
     with ATMBatch(systems) as b:
-        for kernel, waves in ((1, 1), (1, 4), (2, 2), (2, 4), (3, 1), (3, 4)):
+        for kernel, waves in ((1, 1), (1, 4), (2, 2), (2, 4), (3, 1), (3, 2), (4, 1), (4, 2)):
             b.set_tuning(1.0, waves, 4, kernel)
             b.compute()
             f = [b.forces_fixed(k) for k in range(len(systems))]
@@ -234,7 +254,7 @@ def test_compacted_kernel_forces_independent_of_batching(gpu):
                     assert np.array_equal(a, r)
 
 
-@pytest.mark.parametrize("kernel", [0, 2, 3])
+@pytest.mark.parametrize("kernel", [0, 1, 2, 3, 4])
 def test_clash_side_sums_cleared_between_evaluations(gpu, kernel):
     """Pair forces beyond the fixed-point range go to FP64 side sums that are
     cleared only after an evaluation used them.  The displacement alternates
