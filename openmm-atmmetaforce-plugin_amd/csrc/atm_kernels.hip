@@ -608,6 +608,10 @@ __global__ void __launch_bounds__(64 * W) k_pairs(DevPtrs P, int setBegin) {
                 pj.y += syc == 1u ? D.L[1] : (syc == 2u ? -D.L[1] : 0.f);
             }
             const int tdiag = J - g0;  // the i-cluster equal to J, if any
+            // bit t of okm: this lane's (t, ii) x jj pair may count (real atoms;
+            // i < j inside the diagonal cluster); computed once per entry
+            uint32_t okm = (F64 && shiftOK) ? 0xFFu : (jv ? ivm : 0u);
+            if (tdiag >= 0 && tdiag < kGroupClusters && ii >= jj) okm &= ~(1u << tdiag);
             float fjx = 0.f, fjy = 0.f, fjz = 0.f;
 #pragma unroll
             for (int t = 0; t < kGroupClusters; ++t) {
@@ -620,9 +624,7 @@ __global__ void __launch_bounds__(64 * W) k_pairs(DevPtrs P, int setBegin) {
                     if (F64) {
                         // FP32 r^2 with a margin: a superset of the in-cutoff pairs, decides the early-out
                         const float r2f = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-                        bool cand = r2f < rc2hi;
-                        if (!shiftOK) cand = cand && jv && ((ivm >> t) & 1u);
-                        if (t == tdiag) cand = cand && (ii < jj);
+                        const bool cand = r2f < rc2hi && (okm & (1u << t)) != 0u;
                         if (__builtin_amdgcn_ballot_w64(cand) == 0ull) continue;  // no pair of this 8x8 step is inside
                         const double4 id = sPrm64[wave][t * kClusterSize + ii];
                         // exact: the squares of 2^-20-grid differences fit in 53 bits
@@ -632,8 +634,7 @@ __global__ void __launch_bounds__(64 * W) k_pairs(DevPtrs P, int setBegin) {
                         eE += pe.e; eQQ += pe.qq; eQR2 += pe.qq * r2;
                         fs = (float)pe.fs;
                     } else {
-                        bool ok = jv && ((ivm >> t) & 1u);
-                        if (t == tdiag) ok = ok && (ii < jj);
+                        const bool ok = (okm & (1u << t)) != 0u;
                         const float2 qi = sPrm[wave][t * kClusterSize + ii];
                         const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
                         const bool in = ok && in_cutoff(D, dx, dy, dz, r2);
