@@ -222,6 +222,55 @@ int atm_profile(void* context, int32_t flags, int32_t reps, atm_stage_times* out
 int atm_get_stats(void* context, int64_t* pairs_in_cutoff, int64_t* pairs_evaluated,
                   int64_t* list_entries);
 
+/* ------------------------------------------------------------------------
+ * Flat-bottom restraints evaluated beside the ATM force every step (force
+ * group 0 in ATM workflows, python/tests/test_abfe.py:65-79).
+ *
+ * Centroid restraint (replaces the CustomCentroidBondForce built by
+ * ATMMetaForceUtils.addVsiteRestraintForceCMCM, python/ATMMetaForceUtils.py:79-138):
+ *   c_g = sum_i w_i x_i / sum_i w_i over the members of group g (no periodic
+ *   wrapping, as CustomCentroidBondForce by default), d = |c1 - offset - c2|,
+ *   E = kf/2 (d - tol)^2 for d > tol, else 0 (expression at :113-114).
+ * Position restraint (replaces the CustomExternalForce of
+ * ATMMetaForceUtils.addPosRestraints, python/ATMMetaForceUtils.py:707-753):
+ *   d = |x - x0| (minimum image in the orthorhombic box when periodic),
+ *   E = kf/2 (d - tol)^2 for d > tol, else 0 (expressions at :733-736).
+ * Units: nm, kJ/mol, kJ/mol/nm^2.  Forces are summed in 64-bit fixed point
+ * (2^-24 kJ/mol/nm), so results do not depend on the order of the sums.
+ * ------------------------------------------------------------------------ */
+typedef struct atm_centroid_restraint {
+    int32_t group1_first, group1_count; /* members [first, first + count) of the member arrays */
+    int32_t group2_first, group2_count;
+    double kf, tol;
+    double offset[3];
+} atm_centroid_restraint;
+
+typedef struct atm_position_restraint {
+    int32_t atom;     /* index into the positions passed to atm_restraints_eval */
+    int32_t periodic; /* nonzero: minimum-image distance */
+    double ref[3];
+    double kf, tol;
+} atm_position_restraint;
+
+/* A restraint set on GPU `device`; its evaluations take positions of
+ * `num_atoms` atoms (any indexing the caller chooses). */
+int atm_restraints_create(int32_t device, void** out);
+int atm_restraints_destroy(void* restraints);
+
+/* Replace the restraints.  member_atoms/member_weights: the concatenated
+ * groups the centroid restraints refer to (weights > 0; ATMMetaForceUtils
+ * weights by mass). */
+int atm_restraints_set(void* restraints, int32_t num_atoms, int32_t num_members,
+                       const int32_t* member_atoms, const double* member_weights,
+                       int32_t num_centroid, const atm_centroid_restraint* centroid,
+                       int32_t num_position, const atm_position_restraint* position);
+
+/* Energy (kJ/mol) and forces at `positions` (num_atoms x 3, nm) in a box with
+ * edges `box` (nm; used by periodic position restraints).  Forces are added
+ * to `forces` (num_atoms x 3) when it is not NULL. */
+int atm_restraints_eval(void* restraints, const double* positions, const double box[3], double* forces,
+                        double* energy);
+
 #ifdef __cplusplus
 }
 #endif

@@ -11,6 +11,10 @@ openmm.unit is not installed, so no Quantity wrapping is applied.
 used with (System, NonbondedForce, Context, State, Platform, VerletIntegrator,
 XmlSerializer); the only platform is "MI355X".
 
+ATMMetaForceUtils (python/ATMMetaForceUtils.py's setup helpers: centroid and
+position restraints, non-bonded force group, zero-LJ fix) builds
+CentroidRestraintForce / PositionRestraintForce, evaluated on the MI355X.
+
 Beyond the reference: ATMBatch (many systems/replicas per launch through the C
 ABI), ATMEnergyFunction (torch.autograd), and the synthetic workloads.
 """
@@ -20,8 +24,10 @@ from . import workloads  # noqa: E402
 from .batch import ATMBatch  # noqa: E402
 
 try:
-    from ._atmmetaforce import ATMMetaForce, OpenMMException  # noqa: E402,F401
+    from ._atmmetaforce import (ATMMetaForce, CentroidRestraintForce, OpenMMException,  # noqa: E402,F401
+                                PositionRestraintForce)
     from . import openmm  # noqa: E402,F401
+    from .utils import ATMMetaForceUtils  # noqa: E402,F401
 except ImportError as _e:  # the C++ layer is not built: fail when it is used
     _import_error = _e
 else:
@@ -32,9 +38,10 @@ def __getattr__(name):
     if name in ("ATMEnergyFunction", "atm_energy"):  # imports torch on first use
         from . import autograd
         return getattr(autograd, name)
-    if name in ("ATMMetaForce", "OpenMMException", "openmm") and _import_error is not None:
+    if name in ("ATMMetaForce", "OpenMMException", "openmm", "ATMMetaForceUtils", "CentroidRestraintForce",
+                "PositionRestraintForce") and _import_error is not None:
         raise ImportError(f"atmmetaforce C++ module not built ({_import_error}); run __graft_entry__.build()")
     raise AttributeError(name)
 
-__all__ = ["ATMMETAFORCE_VERSION", "ATMMetaForce", "ATMBatch", "ATMEnergyFunction", "OpenMMException", "atm_energy",
-           "workloads"]
+__all__ = ["ATMMETAFORCE_VERSION", "ATMMetaForce", "ATMMetaForceUtils", "ATMBatch", "ATMEnergyFunction",
+           "CentroidRestraintForce", "OpenMMException", "PositionRestraintForce", "atm_energy", "workloads"]

@@ -35,6 +35,7 @@ EXPORTED = (
     "atm_set_box", "atm_set_params", "atm_eval", "atm_synchronize", "atm_get_results", "atm_get_forces",
     "atm_get_forces_fixed", "atm_clear_forces", "atm_get_pairs", "atm_backward", "atm_set_stream",
     "atm_autotune", "atm_get_tuning", "atm_set_tuning", "atm_get_stats", "atm_profile",
+    "atm_restraints_create", "atm_restraints_destroy", "atm_restraints_set", "atm_restraints_eval",
 )
 
 
@@ -70,6 +71,17 @@ class StageTimes(ctypes.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class CentroidRestraint(ctypes.Structure):
+    _fields_ = [("group1_first", ctypes.c_int32), ("group1_count", ctypes.c_int32),
+                ("group2_first", ctypes.c_int32), ("group2_count", ctypes.c_int32),
+                ("kf", ctypes.c_double), ("tol", ctypes.c_double), ("offset", ctypes.c_double * 3)]
+
+
+class PositionRestraint(ctypes.Structure):
+    _fields_ = [("atom", ctypes.c_int32), ("periodic", ctypes.c_int32), ("ref", ctypes.c_double * 3),
+                ("kf", ctypes.c_double), ("tol", ctypes.c_double)]
 
 
 class ATMError(RuntimeError):
@@ -122,6 +134,11 @@ def load(path: str | None = None):
         "atm_get_stats": [vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
                           ctypes.POINTER(ctypes.c_int64)],
         "atm_profile": [vp, i32, i32, ctypes.POINTER(StageTimes)],
+        "atm_restraints_create": [i32, ctypes.POINTER(vp)],
+        "atm_restraints_destroy": [vp],
+        "atm_restraints_set": [vp, i32, i32, ctypes.POINTER(i32), dp, i32, ctypes.POINTER(CentroidRestraint), i32,
+                               ctypes.POINTER(PositionRestraint)],
+        "atm_restraints_eval": [vp, dp, dp, dp, dp],
     }
     for name, args in sigs.items():
         f = getattr(L, name)

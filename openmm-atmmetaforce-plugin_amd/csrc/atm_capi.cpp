@@ -493,6 +493,37 @@ void upload_system(Context& C, int s) {
     HIPCHK(hipMemcpyAsync(C.dPosq.as<float4>() + off, C.hPosq.data() + off, n * sizeof(float4), hipMemcpyHostToDevice, C.stream));
 }
 
+
+// ------------------------------------------------------------------ restraints
+
+// A restraint set: the atoms it touches are gathered into a compact array
+// (positions in, fixed-point forces out), so an evaluation moves only those.
+struct RestraintSet {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int numAtoms = 0;
+    std::vector<int> used;              // compact index -> caller's atom index
+    std::vector<atm_centroid_restraint> cent;
+    std::vector<atm_position_restraint> posr;
+    DevBuf dPos, dF, dMembers, dWeights, dCent, dPosr, dE;
+    std::vector<double> hPos, hE;
+    std::vector<long long> hF;
+    ~RestraintSet() {
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+RestraintSet& restraints_of(void* r) {
+    if (!r) fail(ATM_ERR_INVALID_ARGUMENT, "null restraint set");
+    return *static_cast<RestraintSet*>(r);
+}
+
+void rs_to_device(RestraintSet& R, void* dst, const void* src, size_t bytes) {
+    if (bytes == 0) return;
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, R.stream));
+    HIPCHK(hipStreamSynchronize(R.stream));
+}
+
 }  // namespace
 
 extern "C" {
@@ -1037,6 +1068,136 @@ int atm_backward(void* context, const double* grad_energy, double* d_charges, do
                 d_params[5 * s + 4] = g * 1.0;
             }
         }
+    });
+}
+
+// ------------------------------------------------------------------ restraints
+// Flat-bottom restraints beside the ATM force: python/ATMMetaForceUtils.py:79-138
+// (centroid), :707-753 (position).
+
+int atm_restraints_create(int32_t device, void** out) {
+    return guarded([&] {
+        if (!out) fail(ATM_ERR_INVALID_ARGUMENT, "null output pointer");
+        *out = nullptr;
+        int ndev = 0;
+        HIPCHK(hipGetDeviceCount(&ndev));
+        if (device < 0 || device >= ndev) fail(ATM_ERR_INVALID_ARGUMENT, "no GPU with index " + std::to_string(device));
+        HIPCHK(hipSetDevice(device));
+        std::unique_ptr<RestraintSet> R(new RestraintSet());
+        R->device = device;
+        HIPCHK(hipStreamCreateWithFlags(&R->stream, hipStreamNonBlocking));
+        *out = R.release();
+    });
+}
+
+int atm_restraints_destroy(void* r) {
+    return guarded([&] {
+        if (!r) return;
+        RestraintSet* R = static_cast<RestraintSet*>(r);
+        (void)hipSetDevice(R->device);
+        (void)hipStreamSynchronize(R->stream);
+        delete R;
+    });
+}
+
+int atm_restraints_set(void* r, int32_t num_atoms, int32_t num_members, const int32_t* member_atoms,
+                       const double* member_weights, int32_t num_centroid, const atm_centroid_restraint* centroid,
+                       int32_t num_position, const atm_position_restraint* position) {
+    return guarded([&] {
+        RestraintSet& R = restraints_of(r);
+        if (num_atoms < 0 || num_members < 0 || num_centroid < 0 || num_position < 0)
+            fail(ATM_ERR_INVALID_ARGUMENT, "negative count");
+        if ((num_members && (!member_atoms || !member_weights)) || (num_centroid && !centroid) ||
+            (num_position && !position))
+            fail(ATM_ERR_INVALID_ARGUMENT, "null array");
+        std::vector<int> localOf(num_atoms, -1), used;
+        auto local = [&](int a) {
+            if (a < 0 || a >= num_atoms) fail(ATM_ERR_INVALID_ARGUMENT, "restraint atom index " + std::to_string(a) + " out of range");
+            if (localOf[a] < 0) {
+                localOf[a] = (int)used.size();
+                used.push_back(a);
+            }
+            return localOf[a];
+        };
+        std::vector<int> members(num_members);
+        std::vector<double> weights(member_weights, member_weights + num_members);
+        for (int m = 0; m < num_members; ++m) {
+            if (!(weights[m] > 0.0) || !std::isfinite(weights[m])) fail(ATM_ERR_INVALID_ARGUMENT, "member weights must be positive");
+            members[m] = local(member_atoms[m]);
+        }
+        std::vector<atm_centroid_restraint> cent(centroid, centroid + num_centroid);
+        for (const atm_centroid_restraint& c : cent) {
+            if (c.group1_count <= 0 || c.group2_count <= 0) fail(ATM_ERR_INVALID_ARGUMENT, "centroid groups must not be empty");
+            if (c.group1_first < 0 || c.group1_first + c.group1_count > num_members || c.group2_first < 0 ||
+                c.group2_first + c.group2_count > num_members)
+                fail(ATM_ERR_INVALID_ARGUMENT, "centroid group outside the member arrays");
+            if (!std::isfinite(c.kf) || !std::isfinite(c.tol)) fail(ATM_ERR_INVALID_ARGUMENT, "non-finite restraint parameter");
+        }
+        std::vector<atm_position_restraint> posr(position, position + num_position);
+        for (atm_position_restraint& q : posr) {
+            if (!std::isfinite(q.kf) || !std::isfinite(q.tol)) fail(ATM_ERR_INVALID_ARGUMENT, "non-finite restraint parameter");
+            q.atom = local(q.atom);
+        }
+        HIPCHK(hipSetDevice(R.device));
+        HIPCHK(hipStreamSynchronize(R.stream));
+        R.numAtoms = num_atoms;
+        R.used = used;
+        R.cent = cent;
+        R.posr = posr;
+        const size_t U = std::max<size_t>(used.size(), 1);
+        R.dPos.alloc(sizeof(double) * 3 * U);
+        R.dF.alloc(sizeof(long long) * 3 * U);
+        R.dMembers.alloc(sizeof(int) * std::max(num_members, 1));
+        R.dWeights.alloc(sizeof(double) * std::max(num_members, 1));
+        R.dCent.alloc(sizeof(atm_centroid_restraint) * std::max(num_centroid, 1));
+        R.dPosr.alloc(sizeof(atm_position_restraint) * std::max(num_position, 1));
+        R.dE.alloc(sizeof(double) * std::max(num_centroid + num_position, 1));
+        rs_to_device(R, R.dMembers.p, members.data(), sizeof(int) * num_members);
+        rs_to_device(R, R.dWeights.p, weights.data(), sizeof(double) * num_members);
+        rs_to_device(R, R.dCent.p, cent.data(), sizeof(atm_centroid_restraint) * num_centroid);
+        rs_to_device(R, R.dPosr.p, posr.data(), sizeof(atm_position_restraint) * num_position);
+        R.hPos.assign(3 * U, 0.0);
+        R.hF.assign(3 * U, 0);
+        R.hE.assign(std::max(num_centroid + num_position, 1), 0.0);
+    });
+}
+
+int atm_restraints_eval(void* r, const double* positions, const double box[3], double* forces, double* energy) {
+    return guarded([&] {
+        RestraintSet& R = restraints_of(r);
+        if (!positions || !box || !energy) fail(ATM_ERR_INVALID_ARGUMENT, "null argument");
+        const int nc = (int)R.cent.size(), np = (int)R.posr.size();
+        *energy = 0.0;
+        if (nc + np == 0) return;
+        for (int k = 0; k < 3; ++k)
+            if (!(box[k] > 0.0)) fail(ATM_ERR_INVALID_ARGUMENT, "box edges must be positive");
+        const size_t U = R.used.size();
+        for (size_t u = 0; u < U; ++u)
+            for (int k = 0; k < 3; ++k) R.hPos[3 * u + k] = positions[3ll * R.used[u] + k];
+        HIPCHK(hipSetDevice(R.device));
+        HIPCHK(hipMemcpyAsync(R.dPos.p, R.hPos.data(), sizeof(double) * 3 * U, hipMemcpyHostToDevice, R.stream));
+        HIPCHK(hipMemsetAsync(R.dF.p, 0, sizeof(long long) * 3 * U, R.stream));
+        RestraintPtrs P;
+        P.pos = R.dPos.as<double>();
+        P.fsum = R.dF.as<unsigned long long>();
+        P.members = R.dMembers.as<int>();
+        P.weights = R.dWeights.as<double>();
+        P.cent = R.dCent.as<atm_centroid_restraint>();
+        P.posr = R.dPosr.as<atm_position_restraint>();
+        P.energy = R.dE.as<double>();
+        P.ncent = nc;
+        P.npos = np;
+        for (int k = 0; k < 3; ++k) P.box[k] = box[k];
+        HIPCHK(launch_restraints(P, R.stream));
+        HIPCHK(hipMemcpyAsync(R.hE.data(), R.dE.p, sizeof(double) * (nc + np), hipMemcpyDeviceToHost, R.stream));
+        HIPCHK(hipMemcpyAsync(R.hF.data(), R.dF.p, sizeof(long long) * 3 * U, hipMemcpyDeviceToHost, R.stream));
+        HIPCHK(hipStreamSynchronize(R.stream));
+        double e = 0.0;
+        for (int k = 0; k < nc + np; ++k) e += R.hE[k];  // fixed order
+        *energy = e;
+        if (forces)
+            for (size_t u = 0; u < U; ++u)
+                for (int k = 0; k < 3; ++k) forces[3ll * R.used[u] + k] += (double)R.hF[3 * u + k] / ATM_FORCE_SCALE;
     });
 }
 

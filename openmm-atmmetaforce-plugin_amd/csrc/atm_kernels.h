@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include "atm_internal.h"
+#include "atm_mi355x.h"
 
 namespace atm {
 
@@ -70,5 +71,19 @@ hipError_t launch_fill_i64(long long* p, long long n, hipStream_t st);
 hipError_t launch_fill_zero(void* p, long long n8, hipStream_t st);
 hipError_t launch_pairs_dump(const DevPtrs& P, int c, int maxGroups, int2* out, long long cap,
                              unsigned long long* counters, hipStream_t st);
+
+// Flat-bottom restraints (atm_restraints.hip).
+struct RestraintPtrs {
+    const double* pos;           // atoms x 3
+    unsigned long long* fsum;    // atoms x 3, 64-bit fixed point
+    const int* members;
+    const double* weights;
+    const atm_centroid_restraint* cent;
+    const atm_position_restraint* posr;
+    double* energy;              // one slot per restraint: centroid ones first
+    int ncent, npos;
+    double box[3];
+};
+hipError_t launch_restraints(const RestraintPtrs& R, hipStream_t st);
 
 }  // namespace atm

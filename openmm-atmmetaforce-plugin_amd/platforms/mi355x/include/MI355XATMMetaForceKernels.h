@@ -6,6 +6,7 @@
 #include <string>
 #include <vector>
 #include "ATMMetaForceKernels.h"
+#include "ATMRestraintKernels.h"
 #include "openmm/NonbondedForce.h"
 #include "openmm/Vec3.h"
 #include "openmm/kernels.h"
@@ -71,6 +72,35 @@ public:
 private:
     OpenMM::ContextImpl& context;
     MI355XSystemContext sys;
+};
+
+// Restraint kernels: one libatm_mi355x restraint set each (atm_restraints_*).
+class MI355XCalcCentroidRestraintForceKernel : public CalcCentroidRestraintForceKernel {
+public:
+    MI355XCalcCentroidRestraintForceKernel(std::string name, const OpenMM::Platform& platform, OpenMM::ContextImpl& context)
+        : CalcCentroidRestraintForceKernel(name, platform), context(context) {}
+    ~MI355XCalcCentroidRestraintForceKernel() override;
+    void initialize(const OpenMM::System& system, const CentroidRestraintForce& force) override;
+    double execute(OpenMM::ContextImpl& context, bool includeForces, bool includeEnergy) override;
+    void copyParametersToContext(OpenMM::ContextImpl& context, const CentroidRestraintForce& force) override;
+
+private:
+    OpenMM::ContextImpl& context;
+    void* set = nullptr;
+};
+
+class MI355XCalcPositionRestraintForceKernel : public CalcPositionRestraintForceKernel {
+public:
+    MI355XCalcPositionRestraintForceKernel(std::string name, const OpenMM::Platform& platform, OpenMM::ContextImpl& context)
+        : CalcPositionRestraintForceKernel(name, platform), context(context) {}
+    ~MI355XCalcPositionRestraintForceKernel() override;
+    void initialize(const OpenMM::System& system, const PositionRestraintForce& force) override;
+    double execute(OpenMM::ContextImpl& context, bool includeForces, bool includeEnergy) override;
+    void copyParametersToContext(OpenMM::ContextImpl& context, const PositionRestraintForce& force) override;
+
+private:
+    OpenMM::ContextImpl& context;
+    void* set = nullptr;
 };
 
 }  // namespace ATMMetaForcePlugin

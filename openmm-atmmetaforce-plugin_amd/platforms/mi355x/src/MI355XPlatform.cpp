@@ -62,9 +62,20 @@ class ATMFactory : public KernelFactory {
 public:
     KernelImpl* createKernelImpl(std::string name, const Platform& platform, ContextImpl& context) const override {
         if (name == CalcATMMetaForceKernel::Name()) return new MI355XCalcATMMetaForceKernel(name, platform, context);
+        if (name == CalcCentroidRestraintForceKernel::Name())
+            return new MI355XCalcCentroidRestraintForceKernel(name, platform, context);
+        if (name == CalcPositionRestraintForceKernel::Name())
+            return new MI355XCalcPositionRestraintForceKernel(name, platform, context);
         throw OpenMMException("Tried to create kernel with illegal kernel name '" + name + "'");
     }
 };
+
+// Box edges of an orthorhombic context.
+void boxEdges(ContextImpl& ctx, double L[3]) {
+    Vec3 a, b, c;
+    ctx.getPeriodicBoxVectors(a, b, c);
+    L[0] = a[0]; L[1] = b[1]; L[2] = c[2];
+}
 
 }  // namespace
 
@@ -344,6 +355,101 @@ void MI355XCalcATMMetaForceKernel::copyParametersToContext(ContextImpl& ctx, con
     if (hasVariableForces) sys.setDisplacements(d);
 }
 
+// ------------------------------------------------------------------ restraint kernels
+
+namespace {
+
+void setCentroid(void* set, const System& system, const CentroidRestraintForce& force) {
+    std::vector<int> members;
+    std::vector<double> weights;
+    std::vector<int> first(force.getNumGroups()), count(force.getNumGroups());
+    const int n = system.getNumParticles();
+    for (int g = 0; g < force.getNumGroups(); ++g) {
+        std::vector<int> p;
+        std::vector<double> w;
+        force.getGroupParameters(g, p, w);
+        first[g] = (int)members.size();
+        count[g] = (int)p.size();
+        for (size_t k = 0; k < p.size(); ++k) {
+            if (p[k] < 0 || p[k] >= n) throw OpenMMException("CentroidRestraintForce: particle index out of range");
+            members.push_back(p[k]);
+            // no explicit weights: particle masses, as CustomCentroidBondForce
+            weights.push_back(w.empty() ? system.getParticleMass(p[k]) : w[k]);
+        }
+    }
+    std::vector<atm_centroid_restraint> bonds(force.getNumBonds());
+    for (int b = 0; b < force.getNumBonds(); ++b) {
+        int g1, g2;
+        double kf, tol;
+        Vec3 off;
+        force.getBondParameters(b, g1, g2, kf, tol, off);
+        if (g1 < 0 || g1 >= force.getNumGroups() || g2 < 0 || g2 >= force.getNumGroups())
+            throw OpenMMException("CentroidRestraintForce: group index out of range");
+        bonds[b] = atm_centroid_restraint{first[g1], count[g1], first[g2], count[g2], kf, tol, {off[0], off[1], off[2]}};
+    }
+    check(atm_restraints_set(set, n, (int)members.size(), members.data(), weights.data(), (int)bonds.size(), bonds.data(), 0,
+                             nullptr),
+          "atm_restraints_set");
+}
+
+void setPosition(void* set, const System& system, const PositionRestraintForce& force) {
+    std::vector<atm_position_restraint> q(force.getNumParticles());
+    for (int i = 0; i < force.getNumParticles(); ++i) {
+        int p;
+        Vec3 ref;
+        double kf, tol;
+        force.getParticleParameters(i, p, ref, kf, tol);
+        q[i] = atm_position_restraint{p, force.getPeriodic() ? 1 : 0, {ref[0], ref[1], ref[2]}, kf, tol};
+    }
+    check(atm_restraints_set(set, system.getNumParticles(), 0, nullptr, nullptr, 0, nullptr, (int)q.size(), q.data()),
+          "atm_restraints_set");
+}
+
+double evalRestraints(void* set, ContextImpl& ctx, bool includeForces, bool includeEnergy) {
+    double L[3], e = 0.0;
+    boxEdges(ctx, L);
+    check(atm_restraints_eval(set, reinterpret_cast<const double*>(ctx.positionsRef().data()), L,
+                              includeForces ? reinterpret_cast<double*>(ctx.forcesRef().data()) : nullptr, &e),
+          "atm_restraints_eval");
+    return includeEnergy ? e : 0.0;
+}
+
+}  // namespace
+
+MI355XCalcCentroidRestraintForceKernel::~MI355XCalcCentroidRestraintForceKernel() {
+    if (set) atm_restraints_destroy(set);
+}
+
+void MI355XCalcCentroidRestraintForceKernel::initialize(const System& system, const CentroidRestraintForce& force) {
+    check(atm_restraints_create(MI355XPlatform::data(context).device, &set), "atm_restraints_create");
+    setCentroid(set, system, force);
+}
+
+double MI355XCalcCentroidRestraintForceKernel::execute(ContextImpl& ctx, bool includeForces, bool includeEnergy) {
+    return evalRestraints(set, ctx, includeForces, includeEnergy);
+}
+
+void MI355XCalcCentroidRestraintForceKernel::copyParametersToContext(ContextImpl& ctx, const CentroidRestraintForce& force) {
+    setCentroid(set, ctx.getSystem(), force);
+}
+
+MI355XCalcPositionRestraintForceKernel::~MI355XCalcPositionRestraintForceKernel() {
+    if (set) atm_restraints_destroy(set);
+}
+
+void MI355XCalcPositionRestraintForceKernel::initialize(const System& system, const PositionRestraintForce& force) {
+    check(atm_restraints_create(MI355XPlatform::data(context).device, &set), "atm_restraints_create");
+    setPosition(set, system, force);
+}
+
+double MI355XCalcPositionRestraintForceKernel::execute(ContextImpl& ctx, bool includeForces, bool includeEnergy) {
+    return evalRestraints(set, ctx, includeForces, includeEnergy);
+}
+
+void MI355XCalcPositionRestraintForceKernel::copyParametersToContext(ContextImpl& ctx, const PositionRestraintForce& force) {
+    setPosition(set, ctx.getSystem(), force);
+}
+
 // ------------------------------------------------------------------ plugin entry points
 
 extern "C" __attribute__((visibility("default"))) void registerPlatforms() {
@@ -360,6 +466,8 @@ extern "C" __attribute__((visibility("default"))) void registerKernelFactories()
     try {
         Platform& platform = Platform::getPlatformByName("MI355X");
         platform.registerKernelFactory(CalcATMMetaForceKernel::Name(), new ATMFactory());
+        platform.registerKernelFactory(CalcCentroidRestraintForceKernel::Name(), new ATMFactory());
+        platform.registerKernelFactory(CalcPositionRestraintForceKernel::Name(), new ATMFactory());
     } catch (...) {
     }
 }

@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "ATMMetaForce.h"
+#include "ATMRestraintForces.h"
 #include "ATMMetaForceVersion.h"
 #include "internal/ATMMetaForceImpl.h"
 #include "openmm/Context.h"
@@ -237,6 +238,57 @@ PYBIND11_MODULE(_atmmetaforce, m) {
         .def("setPeriodicBoxVectors", [](Context& c, py::object a, py::object b, py::object d) {
             c.setPeriodicBoxVectors(toVec3(a), toVec3(b), toVec3(d));
         });
+
+    // ---------------------------------------------------------------- restraints
+    using ATMMetaForcePlugin::CentroidRestraintForce;
+    using ATMMetaForcePlugin::PositionRestraintForce;
+    py::class_<CentroidRestraintForce, Force, std::unique_ptr<CentroidRestraintForce, py::nodelete>>(
+        m, "CentroidRestraintForce")
+        .def(py::init([]() { return new CentroidRestraintForce(); }))
+        .def("getNumGroups", &CentroidRestraintForce::getNumGroups)
+        .def("getNumBonds", &CentroidRestraintForce::getNumBonds)
+        .def("addGroup", &CentroidRestraintForce::addGroup, py::arg("particles"),
+             py::arg("weights") = std::vector<double>())
+        .def("getGroupParameters", [](const CentroidRestraintForce& f, int i) {
+            std::vector<int> p;
+            std::vector<double> w;
+            f.getGroupParameters(i, p, w);
+            return py::make_tuple(p, w);
+        })
+        .def("addBond", [](CentroidRestraintForce& f, int g1, int g2, double kf, double tol, const py::object& off) {
+            return f.addBond(g1, g2, kf, tol, off.is_none() ? Vec3() : toVec3(off));
+        }, py::arg("group1"), py::arg("group2"), py::arg("kf"), py::arg("tol"), py::arg("offset") = py::none())
+        .def("getBondParameters", [](const CentroidRestraintForce& f, int i) {
+            int g1, g2;
+            double kf, tol;
+            Vec3 o;
+            f.getBondParameters(i, g1, g2, kf, tol, o);
+            return py::make_tuple(g1, g2, kf, tol, py::make_tuple(o[0], o[1], o[2]));
+        })
+        .def("setBondParameters", [](CentroidRestraintForce& f, int i, int g1, int g2, double kf, double tol,
+                                     const py::object& off) {
+            f.setBondParameters(i, g1, g2, kf, tol, toVec3(off));
+        })
+        .def("updateParametersInContext", &CentroidRestraintForce::updateParametersInContext);
+
+    py::class_<PositionRestraintForce, Force, std::unique_ptr<PositionRestraintForce, py::nodelete>>(
+        m, "PositionRestraintForce")
+        .def(py::init([](bool periodic) { return new PositionRestraintForce(periodic); }), py::arg("periodic") = true)
+        .def("getPeriodic", &PositionRestraintForce::getPeriodic)
+        .def("getNumParticles", &PositionRestraintForce::getNumParticles)
+        .def("addParticle", [](PositionRestraintForce& f, int p, const py::object& ref, double kf, double tol) {
+            return f.addParticle(p, toVec3(ref), kf, tol);
+        }, py::arg("particle"), py::arg("ref"), py::arg("kf"), py::arg("tol"))
+        .def("getParticleParameters", [](const PositionRestraintForce& f, int i) {
+            int p;
+            Vec3 r;
+            double kf, tol;
+            f.getParticleParameters(i, p, r, kf, tol);
+            return py::make_tuple(p, py::make_tuple(r[0], r[1], r[2]), kf, tol);
+        })
+        .def("setParticleParameters", [](PositionRestraintForce& f, int i, int p, const py::object& ref, double kf,
+                                         double tol) { f.setParticleParameters(i, p, toVec3(ref), kf, tol); })
+        .def("updateParametersInContext", &PositionRestraintForce::updateParametersInContext);
 
     // ---------------------------------------------------------------- ATMMetaForce
     py::class_<ATMMetaForce, Force, std::unique_ptr<ATMMetaForce, py::nodelete>>(m, "ATMMetaForce")

@@ -166,3 +166,52 @@ def atm_grad(system, params=None):
     lib().oracle_atm_grad(n, px, pd, pq, ps, pe, pb, system.cutoff, system.rf_dielectric, ctypes.byref(p),
                           ctypes.byref(r), c(dq), c(ds), c(de), c(dp))
     return {"dq": dq, "dsigma": ds, "deps": de, "dparams": dp}
+
+
+# ---------------------------------------------------------------- restraints
+# FP64 numpy restatement of the flat-bottom restraints that
+# python/ATMMetaForceUtils.py builds as OpenMM custom forces (test
+# infrastructure: the checker of the device kernel atm_restraints.hip).
+
+def centroid_restraint(x, group1, group2, weights1, weights2, kf, tol, offset):
+    """Energy and forces of one centroid-centroid restraint,
+    python/ATMMetaForceUtils.py:113-114 (CustomCentroidBondForce expression
+    "(kfcm/2)*step(d12-tolcm)*(d12-tolcm)^2; d12 = |c1 - off - c2|"; member
+    weights per group, aligned with the member lists; positions not wrapped)."""
+    x = np.asarray(x, dtype=np.float64)
+    g1, g2 = np.asarray(group1), np.asarray(group2)
+    w1, w2 = np.asarray(weights1, dtype=np.float64), np.asarray(weights2, dtype=np.float64)
+    c1 = (w1[:, None] * x[g1]).sum(0) / w1.sum()
+    c2 = (w2[:, None] * x[g2]).sum(0) / w2.sum()
+    d = c1 - np.asarray(offset, dtype=np.float64) - c2
+    r = float(np.sqrt(d @ d))
+    F = np.zeros_like(x)
+    if r <= tol:
+        return 0.0, F
+    s = r - tol
+    g = kf * s * d / r                                # dE/dc1
+    np.add.at(F, g1, -g[None, :] * (w1 / w1.sum())[:, None])
+    np.add.at(F, g2, g[None, :] * (w2 / w2.sum())[:, None])
+    return 0.5 * kf * s * s, F
+
+
+def position_restraints(x, atoms, refs, kf, tol, box=None):
+    """Energy and forces of flat-bottom position restraints,
+    python/ATMMetaForceUtils.py:733-736 ("0.5*fc*select(step(dist-tol),
+    (dist-tol)^2, 0)", dist = periodicdistance(x, x0) when `box` is given,
+    else |x - x0|).  kf, tol: scalars or one per restrained atom."""
+    x = np.asarray(x, dtype=np.float64)
+    atoms = np.asarray(atoms)
+    d = x[atoms] - np.asarray(refs, dtype=np.float64)
+    if box is not None:
+        L = np.asarray(box, dtype=np.float64)
+        d = d - L * np.round(d / L)
+    r = np.sqrt((d * d).sum(1))
+    kf = np.broadcast_to(np.asarray(kf, dtype=np.float64), r.shape)
+    tol = np.broadcast_to(np.asarray(tol, dtype=np.float64), r.shape)
+    s = np.where(r > tol, r - tol, 0.0)
+    E = float((0.5 * kf * s * s).sum())
+    F = np.zeros_like(x)
+    k = np.where(r > tol, -kf * s / np.where(r > 0, r, 1.0), 0.0)
+    np.add.at(F, atoms, k[:, None] * d)
+    return E, F

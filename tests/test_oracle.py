@@ -236,3 +236,64 @@ def test_analytic_gradient_against_central_differences(direction):
         pm = dataclasses.replace(p, **{name: getattr(p, name) - h})
         d = (oracle.atm_eval(s, pp, forces=False)[0]["W"] - oracle.atm_eval(s, pm, forces=False)[0]["W"]) / (2 * h)
         assert g["dparams"][k] == pytest.approx(d, rel=1e-6, abs=1e-7), name
+
+
+# ---------------------------------------------------------------- restraints
+
+def _num_grad(f, x, h=1e-6):
+    g = np.zeros_like(x)
+    for i in range(x.shape[0]):
+        for k in range(3):
+            xp, xm = x.copy(), x.copy()
+            xp[i, k] += h
+            xm[i, k] -= h
+            g[i, k] = (f(xp) - f(xm)) / (2 * h)
+    return g
+
+
+def test_centroid_restraint_closed_form_and_gradient():
+    # two single-atom groups: the centroid distance is the atom distance
+    x = np.array([[0.0, 0.0, 0.0], [1.5, 0.0, 0.0]])
+    E, F = oracle.centroid_restraint(x, [1], [0], [1.0], [1.0], kf=100.0, tol=0.5, offset=[0.0, 0.0, 0.0])
+    assert E == pytest.approx(0.5 * 100.0 * 1.0 ** 2)
+    assert F[1] == pytest.approx([-100.0, 0.0, 0.0]) and F[0] == pytest.approx([100.0, 0.0, 0.0])
+    # inside the flat bottom: nothing
+    E0, F0 = oracle.centroid_restraint(x, [1], [0], [1.0], [1.0], kf=100.0, tol=2.0, offset=[0, 0, 0])
+    assert E0 == 0.0 and not F0.any()
+    # the offset is subtracted from c1 - c2
+    E1, _ = oracle.centroid_restraint(x, [1], [0], [1.0], [1.0], kf=100.0, tol=0.0, offset=[1.5, 0.0, 0.0])
+    assert E1 == 0.0
+    # weighted groups: F = -grad E, sum F = 0, weights matter
+    rng = np.random.default_rng(4)
+    y = rng.uniform(0, 3, (9, 3))
+    w = rng.uniform(1, 16, 9)
+    args = dict(kf=250.0, tol=0.3, offset=[0.2, -0.1, 0.4])
+    f = lambda z: oracle.centroid_restraint(z, [0, 1, 2, 3], [4, 5, 6, 7, 8], w[:4], w[4:], **args)[0]
+    E, F = oracle.centroid_restraint(y, [0, 1, 2, 3], [4, 5, 6, 7, 8], w[:4], w[4:], **args)
+    assert E > 0
+    assert np.allclose(F, -_num_grad(f, y), rtol=1e-6, atol=1e-6)
+    assert np.allclose(F.sum(0), 0.0, atol=1e-9)
+    Eu, _ = oracle.centroid_restraint(y, [0, 1, 2, 3], [4, 5, 6, 7, 8], np.ones(4), np.ones(5), **args)
+    # an atom in both groups: each group uses its own weight for it
+    Eo, Fo = oracle.centroid_restraint(y, [0, 1, 4], [4, 5], [1.0, 2.0, 3.0], [5.0, 1.0], **args)
+    fo = lambda z: oracle.centroid_restraint(z, [0, 1, 4], [4, 5], [1.0, 2.0, 3.0], [5.0, 1.0], **args)[0]
+    assert np.allclose(Fo, -_num_grad(fo, y), rtol=1e-6, atol=1e-6)
+    assert Eu != pytest.approx(E)
+
+
+def test_position_restraints_closed_form_periodic_and_gradient():
+    x = np.array([[0.1, 0.2, 0.3], [3.9, 0.0, 0.0]])
+    E, F = oracle.position_restraints(x, [0], [[0.1, 0.2, 1.3]], kf=10.0, tol=0.4)
+    assert E == pytest.approx(0.5 * 10.0 * 0.6 ** 2)
+    assert F[0] == pytest.approx([0.0, 0.0, 10.0 * 0.6]) and not F[1].any()
+    # periodic: the image 0.1 nm away, not the one 3.9 nm away
+    Ep, _ = oracle.position_restraints(x, [1], [[0.0, 0.0, 0.0]], kf=10.0, tol=0.05, box=[4.0, 4.0, 4.0])
+    En, _ = oracle.position_restraints(x, [1], [[0.0, 0.0, 0.0]], kf=10.0, tol=0.05)
+    assert Ep == pytest.approx(0.5 * 10.0 * 0.05 ** 2) and En == pytest.approx(0.5 * 10.0 * 3.85 ** 2)
+    rng = np.random.default_rng(5)
+    y = rng.uniform(0, 4, (6, 3))
+    refs = rng.uniform(0, 4, (4, 3))
+    kf = rng.uniform(10, 100, 4)
+    f = lambda z: oracle.position_restraints(z, [0, 2, 3, 5], refs, kf, 0.2, box=[4.0, 4.5, 5.0])[0]
+    E, F = oracle.position_restraints(y, [0, 2, 3, 5], refs, kf, 0.2, box=[4.0, 4.5, 5.0])
+    assert np.allclose(F, -_num_grad(f, y), rtol=1e-6, atol=1e-6)
