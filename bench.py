@@ -64,6 +64,47 @@ def cpu_baseline(systems, gpu_result0, gpu_forces0, seconds=12.0):
                       f"FP64 cell-list oracle on one host thread ({dt:.1f} s)"}, err
 
 
+def restraint_timing(system, evals=200):
+    """Per-evaluation time of the flat-bottom restraints an ABFE script keeps in
+    force group 0 beside the ATM force (python/tests/test_abfe.py:58-79): one
+    centroid restraint (the 40 ligand atoms against 500 receptor atoms) and 200
+    position restraints, evaluated through the C ABI from host positions of the
+    23,000-atom system (gather, upload, one kernel, download, scatter)."""
+    import ctypes
+    import numpy as np
+    from atmmetaforce import _capi as C
+    L = C.load()
+    n = system.num_atoms
+    x = np.ascontiguousarray(system.positions)
+    box = np.ascontiguousarray(system.box, dtype=np.float64)
+    lig, rcpt = np.arange(40), np.arange(1000, 1500)
+    members = np.ascontiguousarray(np.concatenate([lig, rcpt]).astype(np.int32))
+    weights = np.ascontiguousarray(np.full(len(members), 12.0))
+    cent = (C.CentroidRestraint * 1)(C.CentroidRestraint(0, 40, 40, 500, 104600.0, 0.05, (ctypes.c_double * 3)(0, 0, 0)))
+    pos = (C.PositionRestraint * 200)()
+    for k in range(200):
+        a = 2000 + 7 * k
+        pos[k] = C.PositionRestraint(a, 1, (ctypes.c_double * 3)(*(x[a] + 0.03)), 10460.0, 0.05)
+    dp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    h = ctypes.c_void_p()
+    C.check(L.atm_restraints_create(0, ctypes.byref(h)))
+    try:
+        C.check(L.atm_restraints_set(h, n, len(members), members.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                     dp(weights), 1, cent, 200, pos))
+        F = np.zeros((n, 3))
+        e = ctypes.c_double()
+        for _ in range(10):
+            C.check(L.atm_restraints_eval(h, dp(x), dp(box), dp(F), ctypes.byref(e)))
+        t0 = time.perf_counter()
+        for _ in range(evals):
+            C.check(L.atm_restraints_eval(h, dp(x), dp(box), dp(F), ctypes.byref(e)))
+        ms = (time.perf_counter() - t0) * 1e3 / evals
+    finally:
+        L.atm_restraints_destroy(h)
+    return {"ms_per_eval": round(ms, 4), "bound": "latency (host round trip of 740 atoms; one kernel launch)",
+            "restraints": "1 centroid (40 vs 500 atoms) + 200 position restraints on the 23,000-atom system"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -182,6 +223,9 @@ def main():
         plain_eps = args.plain_steps / plain_dt
         plain.close()
 
+    log("[bench] companion restraints")
+    restraints = restraint_timing(base)
+
     cpu = None
     err = None
     if not args.no_cpu:
@@ -207,6 +251,7 @@ def main():
         "stage_ms": {k: round(v, 4) for k, v in prof.items() if k.endswith("_ms")},
         "blend_kernel": {"bound": "hbm", "achieved": round(blend_gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                          "frac": round(blend_gbs / PEAK_HBM_GBS, 4)},
+        "companion_restraints": restraints,
         "error_vs_fp64_oracle_replica0": err,
         "precision": "coordinates, displacement, cutoff geometry and force vectors FP32; scalar pair function "
                      "(r^2, 1/r, (s/r)^6, energies, force magnitude) FP64; U1/U2/u/W FP64; forces 64-bit "
