@@ -271,6 +271,32 @@ int atm_restraints_set(void* restraints, int32_t num_atoms, int32_t num_members,
 int atm_restraints_eval(void* restraints, const double* positions, const double box[3], double* forces,
                         double* energy);
 
+/* ------------------------------------------------------------------------
+ * Batched Langevin dynamics: every system of a context integrated together on
+ * its ATM force W (its variable-group LJ + reaction-field terms through the
+ * soft-core/softplus combine), one lambda-state per system -- the MD loop of
+ * an ATM run (example/abfe/abfe.py:101-102, 149-160: a LangevinIntegrator
+ * over the force groups {0, ATM group}) for a whole lambda schedule at once.
+ * Splitting as OpenMM's LangevinMiddleIntegrator (BAOAB); FP64 positions and
+ * velocities on the device; reproducible Philox4x32-10 noise keyed by `seed`.
+ * ------------------------------------------------------------------------ */
+
+/* Start: masses (amu) and positions (nm) of all atoms, systems concatenated;
+ * Maxwell-Boltzmann velocities at `temperature` (K); forces of the initial
+ * positions are evaluated.  Massless atoms stay fixed. */
+int atm_md_init(void* context, const double* masses, const double* positions, double temperature, uint64_t seed);
+
+/* Advance every system by `steps` steps of `dt` (ps) with friction (1/ps) at
+ * `temperature` (K).  Afterwards the results (atm_get_results) and forces are
+ * those of the new positions.  A neighbour-list overflow during the steps is
+ * handled inside: the steps since the last check are replayed with doubled
+ * capacity, bit for bit. */
+int atm_md_step(void* context, int32_t steps, double dt, double friction, double temperature);
+
+/* Positions (nm) and velocities (nm/ps) of system `sys` (num_atoms x 3 each;
+ * either may be NULL) and its kinetic energy (kJ/mol; may be NULL). */
+int atm_md_get(void* context, int32_t sys, double* positions, double* velocities, double* kinetic_energy);
+
 #ifdef __cplusplus
 }
 #endif

@@ -36,6 +36,7 @@ EXPORTED = (
     "atm_get_forces_fixed", "atm_clear_forces", "atm_get_pairs", "atm_backward", "atm_set_stream",
     "atm_autotune", "atm_get_tuning", "atm_set_tuning", "atm_get_stats", "atm_profile",
     "atm_restraints_create", "atm_restraints_destroy", "atm_restraints_set", "atm_restraints_eval",
+    "atm_md_init", "atm_md_step", "atm_md_get",
 )
 
 
@@ -139,6 +140,9 @@ def load(path: str | None = None):
         "atm_restraints_set": [vp, i32, i32, ctypes.POINTER(i32), dp, i32, ctypes.POINTER(CentroidRestraint), i32,
                                ctypes.POINTER(PositionRestraint)],
         "atm_restraints_eval": [vp, dp, dp, dp, dp],
+        "atm_md_init": [vp, dp, dp, ctypes.c_double, ctypes.c_uint64],
+        "atm_md_step": [vp, i32, ctypes.c_double, ctypes.c_double, ctypes.c_double],
+        "atm_md_get": [vp, i32, dp, dp, dp],
     }
     for name, args in sigs.items():
         f = getattr(L, name)

@@ -2,12 +2,14 @@
 
 Names follow the `openmm` package so scripts read the same: System,
 NonbondedForce, MonteCarloBarostat (a holder of its two global parameters),
-Context, State, Platform, VerletIntegrator, XmlSerializer.
+Context, State, Platform, VerletIntegrator, LangevinIntegrator (OpenMM's
+leapfrog Langevin), LangevinMiddleIntegrator (BAOAB), XmlSerializer.
 Positions and forces are (N, 3) arrays or lists of 3-tuples in nm and
 kJ/mol/nm; `groups` arguments accept a set of force-group indices or a bit mask.
 """
-from ._atmmetaforce import (Context, Force, Integrator, MonteCarloBarostat, NonbondedForce,  # noqa: F401
-                            OpenMMException, Platform, State, System, VerletIntegrator, XmlSerializer)
+from ._atmmetaforce import (Context, Force, Integrator, LangevinIntegrator, LangevinMiddleIntegrator,  # noqa: F401
+                            MonteCarloBarostat, NonbondedForce, OpenMMException, Platform, State, System,
+                            VerletIntegrator, XmlSerializer)
 
-__all__ = ["Context", "Force", "Integrator", "MonteCarloBarostat", "NonbondedForce", "OpenMMException", "Platform",
-           "State", "System", "VerletIntegrator", "XmlSerializer"]
+__all__ = ["Context", "Force", "Integrator", "LangevinIntegrator", "LangevinMiddleIntegrator", "MonteCarloBarostat",
+           "NonbondedForce", "OpenMMException", "Platform", "State", "System", "VerletIntegrator", "XmlSerializer"]

@@ -108,6 +108,11 @@ struct Context {
     DevBuf dBpos, dBcell, dBrank, dSlotOf, dCellCount, dCellStart, dColStart, dColCount;
     DevBuf dSpos, dSprm, dSprm64, dSidx, dSf, dSd, dBbc, dBbh, dGinfo, dGcount, dGenergy, dEntries, dSetEnergy;
     DevBuf dGq, dGsig, dGeps, dWset, dDq, dDsig, dDeps;
+    // batched Langevin dynamics (atm_md_*)
+    DevBuf dMx, dMv, dMinvm, dMxSnap, dMvSnap;
+    std::vector<double> masses;
+    bool mdReady = false;
+    uint64_t mdSeed = 0, mdStep = 0;
     DevPtrs P{};
 
     // host mirrors
@@ -517,6 +522,9 @@ RestraintSet& restraints_of(void* r) {
     if (!r) fail(ATM_ERR_INVALID_ARGUMENT, "null restraint set");
     return *static_cast<RestraintSet*>(r);
 }
+
+// The FP32 positions the pair kernels read, rewritten from the FP64 MD positions.
+hipError_t launch_md_init_posq(Context& C);
 
 void rs_to_device(RestraintSet& R, void* dst, const void* src, size_t bytes) {
     if (bytes == 0) return;
@@ -1198,6 +1206,137 @@ int atm_restraints_eval(void* r, const double* positions, const double box[3], d
         if (forces)
             for (size_t u = 0; u < U; ++u)
                 for (int k = 0; k < 3; ++k) forces[3ll * R.used[u] + k] += (double)R.hF[3 * u + k] / ATM_FORCE_SCALE;
+    });
+}
+
+// ------------------------------------------------------------------ batched Langevin dynamics
+// The MD loop of an ATM run (example/abfe/abfe.py:101-102, 149-160) for every
+// system of the context at once.
+
+namespace {
+const double kBoltzmann = 0.00831446261815324;  // kJ/mol/K
+
+MDPtrs md_ptrs(Context& C) {
+    MDPtrs M;
+    M.x = C.dMx.as<double>();
+    M.v = C.dMv.as<double>();
+    M.invm = C.dMinvm.as<double>();
+    M.posq = C.dPosq.as<float4>();
+    return M;
+}
+
+hipError_t launch_md_init_posq(Context& C) { return launch_md_posq(C.P, md_ptrs(C), C.totalAtoms, C.stream); }
+
+// One batched evaluation with forces, re-run with doubled list capacity until
+// no list overflows (positions unchanged).
+void md_evaluate(Context& C) {
+    for (int attempt = 0;; ++attempt) {
+        eval_batched(C, ATM_EVAL_FORCES | ATM_EVAL_CLEAR_FORCES);
+        C.evaluated = true;
+        C.resultsOnHost = false;
+        C.resultsValid = false;
+        try {
+            fetch_results(C);
+            return;
+        } catch (const AtmError& e) {
+            if (e.code != ATM_ERR_CAPACITY || attempt >= 4) throw;
+        }
+    }
+}
+}  // namespace
+
+int atm_md_init(void* context, const double* masses, const double* positions, double temperature, uint64_t seed) {
+    return guarded([&] {
+        Context& C = ctx_of(context);
+        require_ready(C);
+        if (!masses || !positions) fail(ATM_ERR_INVALID_ARGUMENT, "null array");
+        if (!(temperature >= 0.0)) fail(ATM_ERR_INVALID_ARGUMENT, "temperature must be >= 0");
+        HIPCHK(hipSetDevice(C.device));
+        const long long n = C.totalAtoms;
+        std::vector<double> invm(n);
+        for (long long i = 0; i < n; ++i) {
+            if (!(masses[i] >= 0.0) || !std::isfinite(masses[i])) fail(ATM_ERR_INVALID_ARGUMENT, "masses must be >= 0");
+            invm[i] = masses[i] > 0.0 ? 1.0 / masses[i] : 0.0;
+        }
+        for (long long i = 0; i < 3 * n; ++i)
+            if (!std::isfinite(positions[i])) fail(ATM_ERR_INVALID_ARGUMENT, "non-finite position");
+        C.masses.assign(masses, masses + n);
+        C.dMx.alloc(sizeof(double) * 3 * n);
+        C.dMv.alloc(sizeof(double) * 3 * n);
+        C.dMxSnap.alloc(sizeof(double) * 3 * n);
+        C.dMvSnap.alloc(sizeof(double) * 3 * n);
+        C.dMinvm.alloc(sizeof(double) * n);
+        to_device(C, C.dMx.p, positions, sizeof(double) * 3 * n);
+        to_device(C, C.dMinvm.p, invm.data(), sizeof(double) * n);
+        for (long long i = 0; i < n; ++i) {
+            C.hPosq[i].x = (float)positions[3 * i];
+            C.hPosq[i].y = (float)positions[3 * i + 1];
+            C.hPosq[i].z = (float)positions[3 * i + 2];
+        }
+        C.mdSeed = seed;
+        C.mdStep = 0;
+        HIPCHK(launch_md_init(C.P, md_ptrs(C), n, kBoltzmann * temperature, seed, C.stream));
+        md_evaluate(C);
+        C.mdReady = true;
+    });
+}
+
+int atm_md_step(void* context, int32_t steps, double dt, double friction, double temperature) {
+    return guarded([&] {
+        Context& C = ctx_of(context);
+        if (!C.mdReady) fail(ATM_ERR_STATE, "atm_md_init has not been called");
+        if (steps < 0 || !(dt > 0.0) || !(friction >= 0.0) || !(temperature >= 0.0))
+            fail(ATM_ERR_INVALID_ARGUMENT, "steps >= 0, dt > 0, friction >= 0 and temperature >= 0 are required");
+        HIPCHK(hipSetDevice(C.device));
+        const double a = std::exp(-friction * dt), kT = kBoltzmann * temperature;
+        const size_t bytes = sizeof(double) * 3 * C.totalAtoms;
+        const int chunk = 50;  // steps between overflow checks (and snapshots for a replay)
+        for (int done = 0; done < steps;) {
+            const int n = std::min(chunk, steps - done);
+            HIPCHK(hipMemcpyAsync(C.dMxSnap.p, C.dMx.p, bytes, hipMemcpyDeviceToDevice, C.stream));
+            HIPCHK(hipMemcpyAsync(C.dMvSnap.p, C.dMv.p, bytes, hipMemcpyDeviceToDevice, C.stream));
+            for (int k = 0; k < n; ++k) {
+                HIPCHK(launch_md_step(C.P, md_ptrs(C), C.nsys, C.maxAtoms, dt, a, kT, C.mdSeed, C.mdStep + k, C.stream));
+                eval_batched(C, ATM_EVAL_FORCES | ATM_EVAL_CLEAR_FORCES);
+            }
+            C.evaluated = true;
+            C.resultsOnHost = false;
+            C.resultsValid = false;
+            try {
+                fetch_results(C);
+            } catch (const AtmError& e) {
+                if (e.code != ATM_ERR_CAPACITY) throw;
+                // a list overflowed somewhere in these steps: capacity has been doubled;
+                // restore the snapshot and replay them (same noise counters, same trajectory)
+                HIPCHK(hipMemcpyAsync(C.dMx.p, C.dMxSnap.p, bytes, hipMemcpyDeviceToDevice, C.stream));
+                HIPCHK(hipMemcpyAsync(C.dMv.p, C.dMvSnap.p, bytes, hipMemcpyDeviceToDevice, C.stream));
+                HIPCHK(launch_md_init_posq(C));
+                md_evaluate(C);
+                continue;
+            }
+            C.mdStep += n;
+            done += n;
+        }
+    });
+}
+
+int atm_md_get(void* context, int32_t sys, double* positions, double* velocities, double* kinetic_energy) {
+    return guarded([&] {
+        Context& C = ctx_of(context);
+        check_sys(C, sys);
+        if (!C.mdReady) fail(ATM_ERR_STATE, "atm_md_init has not been called");
+        HIPCHK(hipSetDevice(C.device));
+        const long long off = C.atomOff[sys], n = C.sys[sys].num_atoms;
+        std::vector<double> v(3 * n);
+        if (positions) to_host(C, positions, C.dMx.as<double>() + 3 * off, sizeof(double) * 3 * n);
+        if (velocities || kinetic_energy) to_host(C, v.data(), C.dMv.as<double>() + 3 * off, sizeof(double) * 3 * n);
+        if (velocities) std::copy(v.begin(), v.end(), velocities);
+        if (kinetic_energy) {
+            double ke = 0.0;
+            for (long long i = 0; i < n; ++i)
+                ke += 0.5 * C.masses[off + i] * (v[3 * i] * v[3 * i] + v[3 * i + 1] * v[3 * i + 1] + v[3 * i + 2] * v[3 * i + 2]);
+            *kinetic_energy = ke;
+        }
     });
 }
 

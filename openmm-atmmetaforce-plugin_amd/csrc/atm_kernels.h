@@ -86,4 +86,16 @@ struct RestraintPtrs {
 };
 hipError_t launch_restraints(const RestraintPtrs& R, hipStream_t st);
 
+// Batched Langevin dynamics (atm_md.hip): FP64 positions/velocities, one per atom of the context.
+struct MDPtrs {
+    double* x;          // total atoms x 3
+    double* v;          // total atoms x 3
+    const double* invm; // total atoms; 0 = not integrated
+    float4* posq;       // the FP32 positions (+ charge) the pair kernels read
+};
+hipError_t launch_md_step(const DevPtrs& P, const MDPtrs& M, int nsys, int maxAtoms, double dt, double a, double kT,
+                          uint64_t seed, uint64_t step, hipStream_t st);
+hipError_t launch_md_init(const DevPtrs& P, const MDPtrs& M, long long total, double kT, uint64_t seed, hipStream_t st);
+hipError_t launch_md_posq(const DevPtrs& P, const MDPtrs& M, long long total, hipStream_t st);
+
 }  // namespace atm

@@ -36,4 +36,38 @@ public:
     void step(int steps) override;
 };
 
+// Langevin dynamics, OpenMM's leapfrog form (the integrator of the ATM example
+// scripts, example/abfe/abfe.py:100-101): per step
+//   v = a v + (1 - a)/friction F/m + sqrt(kT (1 - a^2)/m) R,  x += dt v,
+// a = exp(-friction dt) ((1 - a)/friction -> dt without friction).
+// temperature K, friction 1/ps, step size ps.  Host update around the
+// platform's forces; Gaussian noise from a seeded std::mt19937_64.
+class OPENMM_EXPORT LangevinIntegrator : public Integrator {
+public:
+    LangevinIntegrator(double temperature, double frictionCoeff, double stepSize)
+        : Integrator(stepSize), temperature(temperature), friction(frictionCoeff), seed(0) {}
+    double getTemperature() const { return temperature; }
+    void setTemperature(double t) { temperature = t; }
+    double getFriction() const { return friction; }
+    void setFriction(double f) { friction = f; }
+    int getRandomNumberSeed() const { return seed; }
+    void setRandomNumberSeed(int s) { seed = s; }
+    void step(int steps) override;
+
+protected:
+    double temperature, friction;
+    int seed;
+    unsigned long long drawn = 0;  // deviates drawn so far (the generator restarts from seed + drawn)
+};
+
+// LangevinMiddle / BAOAB splitting: v += dt F/m; x += dt/2 v;
+// v = a v + sqrt(kT (1 - a^2)/m) R; x += dt/2 v (the scheme of the batched
+// device MD, atm_md_step).
+class OPENMM_EXPORT LangevinMiddleIntegrator : public LangevinIntegrator {
+public:
+    LangevinMiddleIntegrator(double temperature, double frictionCoeff, double stepSize)
+        : LangevinIntegrator(temperature, frictionCoeff, stepSize) {}
+    void step(int steps) override;
+};
+
 }  // namespace OpenMM

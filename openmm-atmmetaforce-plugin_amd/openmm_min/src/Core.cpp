@@ -1,6 +1,7 @@
 // Minimal OpenMM-compatible core: Force, System, NonbondedForce, State,
 // Integrator, Context and ContextImpl.
 #include <cmath>
+#include <random>
 #include <cstdlib>
 #include <set>
 #include <sstream>
@@ -305,6 +306,73 @@ void VerletIntegrator::step(int steps) {
             x[i] += v[i] * stepSize;
         }
         c.setTime(c.getTime() + stepSize);
+    }
+}
+
+// ------------------------------------------------------------------ Langevin integrators
+
+namespace {
+const double kBoltz = 0.00831446261815324;  // kJ/mol/K
+
+// N(0, 1) deviates of step `k` of an integrator seeded with `seed`: a generator
+// per step, so a trajectory does not depend on how steps are grouped into calls.
+struct StepNormals {
+    std::mt19937_64 gen;
+    std::normal_distribution<double> dist{0.0, 1.0};
+    StepNormals(int seed, unsigned long long k) : gen((unsigned long long)seed * 0x9E3779B97F4A7C15ull + k) {}
+    Vec3 operator()() {
+        const double a = dist(gen), b = dist(gen), c = dist(gen);
+        return Vec3(a, b, c);
+    }
+};
+}  // namespace
+
+void LangevinIntegrator::step(int steps) {
+    if (!context) throw OpenMMException("This Integrator is not bound to a context!");
+    ContextImpl& c = *context;
+    const System& sys = c.getSystem();
+    const int n = sys.getNumParticles();
+    std::vector<Vec3>& x = c.positionsRef();
+    std::vector<Vec3>& v = c.velocitiesRef();
+    const double dt = stepSize, a = std::exp(-friction * dt), kT = kBoltz * temperature;
+    const double fscale = friction > 0.0 ? (1.0 - a) / friction : dt;
+    for (int s = 0; s < steps; ++s) {
+        StepNormals R(seed, drawn++);
+        c.updateContextState();
+        c.calcForcesAndEnergy(true, false, groups);
+        const std::vector<Vec3>& f = c.forcesRef();
+        for (int i = 0; i < n; ++i) {
+            const double m = sys.getParticleMass(i);
+            if (m == 0.0) continue;
+            v[i] = v[i] * a + f[i] * (fscale / m) + R() * std::sqrt(kT * (1.0 - a * a) / m);
+            x[i] += v[i] * dt;
+        }
+        c.setTime(c.getTime() + dt);
+    }
+}
+
+void LangevinMiddleIntegrator::step(int steps) {
+    if (!context) throw OpenMMException("This Integrator is not bound to a context!");
+    ContextImpl& c = *context;
+    const System& sys = c.getSystem();
+    const int n = sys.getNumParticles();
+    std::vector<Vec3>& x = c.positionsRef();
+    std::vector<Vec3>& v = c.velocitiesRef();
+    const double dt = stepSize, a = std::exp(-friction * dt), kT = kBoltz * temperature;
+    for (int s = 0; s < steps; ++s) {
+        StepNormals R(seed, drawn++);
+        c.updateContextState();
+        c.calcForcesAndEnergy(true, false, groups);
+        const std::vector<Vec3>& f = c.forcesRef();
+        for (int i = 0; i < n; ++i) {
+            const double m = sys.getParticleMass(i);
+            if (m == 0.0) continue;
+            v[i] += f[i] * (dt / m);
+            x[i] += v[i] * (0.5 * dt);
+            v[i] = v[i] * a + R() * std::sqrt(kT * (1.0 - a * a) / m);
+            x[i] += v[i] * (0.5 * dt);
+        }
+        c.setTime(c.getTime() + dt);
     }
 }
 

@@ -186,6 +186,16 @@ PYBIND11_MODULE(_atmmetaforce, m) {
         .def("getIntegrationForceGroups", &Integrator::getIntegrationForceGroups)
         .def("setIntegrationForceGroups", [](Integrator& i, py::object g) { i.setIntegrationForceGroups(groupMask(g)); });
     py::class_<VerletIntegrator, Integrator>(m, "VerletIntegrator").def(py::init<double>());
+    py::class_<LangevinIntegrator, Integrator>(m, "LangevinIntegrator")
+        .def(py::init<double, double, double>(), py::arg("temperature"), py::arg("frictionCoeff"), py::arg("stepSize"))
+        .def("getTemperature", &LangevinIntegrator::getTemperature)
+        .def("setTemperature", &LangevinIntegrator::setTemperature)
+        .def("getFriction", &LangevinIntegrator::getFriction)
+        .def("setFriction", &LangevinIntegrator::setFriction)
+        .def("getRandomNumberSeed", &LangevinIntegrator::getRandomNumberSeed)
+        .def("setRandomNumberSeed", &LangevinIntegrator::setRandomNumberSeed);
+    py::class_<LangevinMiddleIntegrator, LangevinIntegrator>(m, "LangevinMiddleIntegrator")
+        .def(py::init<double, double, double>(), py::arg("temperature"), py::arg("frictionCoeff"), py::arg("stepSize"));
 
     py::class_<State>(m, "State")
         .def("getTime", &State::getTime)

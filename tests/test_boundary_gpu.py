@@ -244,3 +244,32 @@ def test_state_checkpoint_restores_atm_parameters(gpu):
     e1 = c1.getState(getEnergy=True, groups={2}).getPotentialEnergy()
     e3 = c3.getState(getEnergy=True, groups={2}).getPotentialEnergy()
     assert e1 == e3
+
+
+@pytest.mark.parametrize("integrator", ["LangevinIntegrator", "LangevinMiddleIntegrator"])
+def test_langevin_md_loop_like_abfe(gpu, integrator):
+    """The MD loop of example/abfe/abfe.py:100-160 on the MI355X platform:
+    Langevin dynamics over the force groups {0, ATM group}, samples of the
+    potential energy of those groups and of the perturbation energy."""
+    s = W.tiny_system(900, 3.3, 0.9, 9, (1.0, 0.9, 0.8), 13)
+    system, f = build(s)
+    plat = mm.Platform.getPlatformByName("MI355X")
+    finals = []
+    for _ in range(2):
+        integ = getattr(mm, integrator)(300.0, 5.0, 0.001)
+        integ.setIntegrationForceGroups({0, 2})
+        integ.setRandomNumberSeed(17)
+        ctx = mm.Context(system, integ, plat)
+        ctx.setPositions(s.positions)
+        lines = []
+        for _ in range(3):
+            integ.step(5)
+            st = ctx.getState(getEnergy=True, groups={0, 2})
+            pert = f.getPerturbationEnergy(ctx)
+            lines.append("%f %f %f %f" % (300.0, ctx.getParameter(f.Lambda1()), st.getPotentialEnergy(), pert))
+        x = ctx.getState(getPositions=True, getVelocities=True)
+        finals.append((x.getPositions(asNumpy=True), x.getVelocities(asNumpy=True)))
+        assert all(np.isfinite([float(t) for t in l.split()]).all() for l in lines)
+        assert ctx.getState(getEnergy=True).getKineticEnergy() > 0.0
+    assert np.array_equal(finals[0][0], finals[1][0]) and np.array_equal(finals[0][1], finals[1][1])
+    assert not np.array_equal(finals[0][0], s.positions)
