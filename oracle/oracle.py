@@ -168,6 +168,56 @@ def atm_grad(system, params=None):
     return {"dq": dq, "dsigma": ds, "deps": de, "dparams": dp}
 
 
+# ---------------------------------------------------------------- CPU-equivalent
+# Multi-threaded FP32-pair / FP64-sum CPU evaluation (atm_cpu_mt.c): the
+# speed baseline timed beside the device library, not a checker.
+
+_MT_PATH = os.path.join(_HERE, "libatm_cpu_mt.so")
+_mt = None
+
+
+def mt_lib():
+    global _mt
+    if _mt is None:
+        lib()
+        if not os.path.exists(_MT_PATH) or \
+                os.path.getmtime(_MT_PATH) < os.path.getmtime(os.path.join(_HERE, "atm_cpu_mt.c")):
+            build()
+        L = ctypes.CDLL(_MT_PATH)
+        dp = ctypes.POINTER(ctypes.c_double)
+        L.cpu_mt_atm_eval.restype = ctypes.c_int
+        L.cpu_mt_atm_eval.argtypes = [ctypes.c_int, dp, dp, dp, dp, dp, dp, ctypes.c_double, ctypes.c_double,
+                                      ctypes.POINTER(Params), ctypes.c_int, dp, ctypes.POINTER(Result)]
+        L.cpu_mt_max_threads.restype = ctypes.c_int
+        _mt = L
+    return _mt
+
+
+def cpu_mt_atm_eval(system, threads=0, params=None, forces=True):
+    """ATM evaluation of a workloads.System on `threads` host threads (0: all
+    OpenMP threads): (result dict, ATM force (N,3) or None)."""
+    p = params_of(params if params is not None else system.params)
+    x, px = _d(system.positions)
+    d, pd = _d(system.displacements)
+    q, pq = _d(system.charges)
+    s, ps = _d(system.sigmas)
+    e, pe = _d(system.epsilons)
+    b, pb = _d(system.box)
+    n = x.shape[0]
+    F = np.zeros((n, 3)) if forces else None
+    r = Result()
+    rc = mt_lib().cpu_mt_atm_eval(n, px, pd, pq, ps, pe, pb, system.cutoff, system.rf_dielectric, ctypes.byref(p),
+                                  int(threads), F.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) if forces else None,
+                                  ctypes.byref(r))
+    if rc != 0:
+        raise ValueError(f"cpu_mt_atm_eval failed ({rc}): the box needs at least 3 cells of rc per edge")
+    return r.as_dict(), F
+
+
+def cpu_mt_threads() -> int:
+    return mt_lib().cpu_mt_max_threads()
+
+
 # ---------------------------------------------------------------- restraints
 # FP64 numpy restatement of the flat-bottom restraints that
 # python/ATMMetaForceUtils.py builds as OpenMM custom forces (test

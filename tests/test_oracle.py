@@ -297,3 +297,16 @@ def test_position_restraints_closed_form_periodic_and_gradient():
     f = lambda z: oracle.position_restraints(z, [0, 2, 3, 5], refs, kf, 0.2, box=[4.0, 4.5, 5.0])[0]
     E, F = oracle.position_restraints(y, [0, 2, 3, 5], refs, kf, 0.2, box=[4.0, 4.5, 5.0])
     assert np.allclose(F, -_num_grad(f, y), rtol=1e-6, atol=1e-6)
+
+
+@pytest.mark.parametrize("threads", [1, 4])
+def test_multithreaded_cpu_port_agrees_with_oracle(threads):
+    """The multi-threaded CPU baseline (FP32 pair math) reproduces the FP64
+    oracle to FP32 rounding on the 3,000-atom workload."""
+    s = W.config1()
+    ref, Fref = oracle.atm_eval(s)
+    got, F = oracle.cpu_mt_atm_eval(s, threads)
+    for k in ("U1", "U2", "W", "usc"):
+        assert abs(got[k] - ref[k]) <= 1e-6 * abs(ref[k]), (k, got[k], ref[k])
+    assert abs(got["npairs1"] - ref["npairs1"]) <= 4 and abs(got["npairs2"] - ref["npairs2"]) <= 4
+    assert np.linalg.norm(F - Fref) / np.linalg.norm(Fref) < 1e-5
