@@ -43,7 +43,7 @@
 extern "C" {
 #endif
 
-#define ATM_MI355X_ABI_VERSION 2
+#define ATM_MI355X_ABI_VERSION 3
 
 /* status codes */
 #define ATM_OK 0
@@ -296,6 +296,86 @@ int atm_md_step(void* context, int32_t steps, double dt, double friction, double
 /* Positions (nm) and velocities (nm/ps) of system `sys` (num_atoms x 3 each;
  * either may be NULL) and its kinetic energy (kJ/mol; may be NULL). */
 int atm_md_get(void* context, int32_t sys, double* positions, double* velocities, double* kinetic_energy);
+
+/* ------------------------------------------------------------------------
+ * PME variable group.  The ATM evaluation when the variable group is an
+ * OpenMM NonbondedForce with particle-mesh Ewald, as in the reference's
+ * end-to-end test (python/tests/test_abfe.py:42-43: AMBER system, PME, 1 nm
+ * cutoff; the NonbondedForce is the variable group, :86-96).  Replaces the
+ * inner-context evaluations of ATMMetaForceImpl::calcForcesAndEnergy
+ * (openmmapi/src/ATMMetaForceImpl.cpp:110-122) for that force and the
+ * combine of ReferenceATMMetaForceKernels.cpp:56-110.  All arithmetic FP64;
+ * forces summed in 64-bit fixed point.
+ *
+ * Model (OpenMM's documented NonbondedForce PME): alpha = sqrt(-ln(2 tol))/rc
+ * unless given; grid per axis ceil(2 alpha L / (3 tol^(1/5))) raised to the
+ * next 2,3,5,7-smooth size (at least 6) unless given; order-5 B-splines;
+ * direct space erfc(alpha r)/r Coulomb + Lorentz-Berthelot LJ for
+ * minimum-image pairs with r < rc that are not exceptions; each exception
+ * adds k_e qq'/r + 4 eps' [(s'/r)^12 - (s'/r)^6] at the plain (non-periodic)
+ * distance and removes the reciprocal part k_e q_i q_j erf(alpha r)/r; self
+ * energy; optional long-range dispersion correction.  Rectangular boxes with
+ * at least three cutoff lengths per edge.
+ * ------------------------------------------------------------------------ */
+typedef struct atm_pme_desc {
+    int32_t num_atoms;
+    int32_t grid[3];               /* PME grid; 0 = from ewald_tolerance */
+    double box[3];                 /* nm (initial; atm_pme_eval may pass another) */
+    double cutoff;                 /* nm */
+    double ewald_tolerance;        /* OpenMM's default 5e-4 */
+    double alpha;                  /* 1/nm; 0 = from ewald_tolerance */
+    int32_t dispersion_correction; /* nonzero: add the long-range LJ correction */
+    int32_t reserved;
+} atm_pme_desc;
+
+/* Replaces the NonbondedForce clone of ATMMetaForceImpl.cpp:58-63 and the
+ * inner contexts created at :91-101. */
+int atm_pme_create(int32_t device, const atm_pme_desc* desc, void** out);
+int atm_pme_destroy(void* pme);
+
+/* The Ewald alpha (1/nm) and grid the force uses. */
+int atm_pme_get_info(void* pme, double* alpha, int32_t grid[3]);
+
+/* Per-particle charge (e), sigma (nm), epsilon (kJ/mol) -- NonbondedForce
+ * addParticle -- and the ATM displacement d (num_atoms x 3, nm; NULL keeps
+ * the current one), ATMMetaForce::addParticle / setParticleParameters
+ * (ATMMetaForce.h:79-297). */
+int atm_pme_set_particles(void* pme, const double* charges, const double* sigmas, const double* epsilons,
+                          const double* displacements);
+
+/* NonbondedForce exceptions: `num` atom pairs (2*num indices) with their
+ * charge product (e^2), sigma (nm) and epsilon (kJ/mol).  Every listed pair
+ * leaves the direct-space sum; a pair may appear once. */
+int atm_pme_set_exceptions(void* pme, int32_t num, const int32_t* pairs, const double* charge_prod,
+                           const double* sigma, const double* epsilon);
+
+/* The nine ATM parameters (ATMMetaForceImpl.cpp:130-142 names and defaults). */
+int atm_pme_set_params(void* pme, const atm_params* params);
+
+/* One evaluation: U1 at x, U2 at x + d, combine.  positions num_atoms x 3
+ * (nm); box NULL keeps the last one.  forces (num_atoms x 3, may be NULL) are
+ * ADDED the hybrid ATM force; result (may be NULL) as atm_get_results; parts
+ * (may be NULL, 10 doubles): per state (x, then x + d) the direct-space,
+ * exception, reciprocal, self and dispersion energies. */
+int atm_pme_eval(void* pme, const double* positions, const double box[3], double* forces, atm_result* result,
+                 double* parts);
+
+/* ------------------------------------------------------------------------
+ * Bonded terms of group 0 beside the ATM force (the AMBER HarmonicBondForce,
+ * HarmonicAngleForce and PeriodicTorsionForce of test_abfe.py's system).
+ * bond_params (r0 nm, k kJ/mol/nm^2) per bond, energy k/2 (r - r0)^2;
+ * angle_params (theta0 rad, k kJ/mol/rad^2) per angle (atoms i, vertex j, k),
+ * energy k/2 (theta - theta0)^2; torsion_params (periodicity, phase rad,
+ * k kJ/mol) per torsion, energy k (1 + cos(n phi - phase)), IUPAC phi.
+ * ------------------------------------------------------------------------ */
+int atm_bonded_create(int32_t device, void** out);
+int atm_bonded_destroy(void* bonded);
+int atm_bonded_set(void* bonded, int32_t num_atoms, int32_t num_bonds, const int32_t* bond_atoms,
+                   const double* bond_params, int32_t num_angles, const int32_t* angle_atoms,
+                   const double* angle_params, int32_t num_torsions, const int32_t* torsion_atoms,
+                   const double* torsion_params);
+/* energies[3] = bond, angle, torsion energy; forces (may be NULL) are ADDED. */
+int atm_bonded_eval(void* bonded, const double* positions, double* forces, double energies[3]);
 
 #ifdef __cplusplus
 }

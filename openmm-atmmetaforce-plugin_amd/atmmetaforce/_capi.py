@@ -37,6 +37,9 @@ EXPORTED = (
     "atm_autotune", "atm_get_tuning", "atm_set_tuning", "atm_get_stats", "atm_profile",
     "atm_restraints_create", "atm_restraints_destroy", "atm_restraints_set", "atm_restraints_eval",
     "atm_md_init", "atm_md_step", "atm_md_get",
+    "atm_pme_create", "atm_pme_destroy", "atm_pme_get_info", "atm_pme_set_particles", "atm_pme_set_exceptions",
+    "atm_pme_set_params", "atm_pme_eval", "atm_bonded_create", "atm_bonded_destroy", "atm_bonded_set",
+    "atm_bonded_eval",
 )
 
 
@@ -83,6 +86,12 @@ class CentroidRestraint(ctypes.Structure):
 class PositionRestraint(ctypes.Structure):
     _fields_ = [("atom", ctypes.c_int32), ("periodic", ctypes.c_int32), ("ref", ctypes.c_double * 3),
                 ("kf", ctypes.c_double), ("tol", ctypes.c_double)]
+
+
+class PmeDesc(ctypes.Structure):
+    _fields_ = [("num_atoms", ctypes.c_int32), ("grid", ctypes.c_int32 * 3), ("box", ctypes.c_double * 3),
+                ("cutoff", ctypes.c_double), ("ewald_tolerance", ctypes.c_double), ("alpha", ctypes.c_double),
+                ("dispersion_correction", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 class ATMError(RuntimeError):
@@ -143,6 +152,18 @@ def load(path: str | None = None):
         "atm_md_init": [vp, dp, dp, ctypes.c_double, ctypes.c_uint64],
         "atm_md_step": [vp, i32, ctypes.c_double, ctypes.c_double, ctypes.c_double],
         "atm_md_get": [vp, i32, dp, dp, dp],
+        "atm_pme_create": [i32, ctypes.POINTER(PmeDesc), ctypes.POINTER(vp)],
+        "atm_pme_destroy": [vp],
+        "atm_pme_get_info": [vp, dp, ctypes.POINTER(i32)],
+        "atm_pme_set_particles": [vp, dp, dp, dp, dp],
+        "atm_pme_set_exceptions": [vp, i32, ctypes.POINTER(i32), dp, dp, dp],
+        "atm_pme_set_params": [vp, ctypes.POINTER(Params)],
+        "atm_pme_eval": [vp, dp, dp, dp, ctypes.POINTER(Result), dp],
+        "atm_bonded_create": [i32, ctypes.POINTER(vp)],
+        "atm_bonded_destroy": [vp],
+        "atm_bonded_set": [vp, i32, i32, ctypes.POINTER(i32), dp, i32, ctypes.POINTER(i32), dp, i32,
+                           ctypes.POINTER(i32), dp],
+        "atm_bonded_eval": [vp, dp, dp, dp],
     }
     for name, args in sigs.items():
         f = getattr(L, name)

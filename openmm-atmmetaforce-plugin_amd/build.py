@@ -32,7 +32,8 @@ HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-I" + CSRC, "-I" + INCLUDE]
 
-HIP_SOURCES = ["atm_kernels.hip", "atm_backward.hip", "atm_restraints.hip", "atm_md.hip", "atm_capi.cpp"]
+HIP_SOURCES = ["atm_kernels.hip", "atm_backward.hip", "atm_restraints.hip", "atm_md.hip", "atm_pme.hip",
+               "atm_capi.cpp", "atm_pme_capi.cpp"]
 
 
 def _run(cmd, cwd=None):

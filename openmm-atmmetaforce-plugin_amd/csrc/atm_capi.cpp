@@ -17,6 +17,7 @@
 #include "atm_internal.h"
 #include "atm_kernels.h"
 #include "atm_math.h"
+#include "atm_capi_util.h"
 #include "../../include/atm_mi355x.h"
 
 namespace atm {
@@ -28,58 +29,16 @@ hipError_t launch_backward_gather(const DevPtrs& P, int sysBegin, int nsys, int 
 hipError_t launch_set_positions(DevPtrs P, float4* posq, const float* xyz, long long n, hipStream_t st);
 }
 
+namespace atm {
+std::string& last_error() {
+    static thread_local std::string msg;
+    return msg;
+}
+}  // namespace atm
+
 using namespace atm;
 
 namespace {
-
-thread_local std::string g_lastError;
-
-struct AtmError : std::runtime_error {
-    int code;
-    AtmError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
-};
-
-#define HIPCHK(expr)                                                                                   \
-    do {                                                                                               \
-        hipError_t e_ = (expr);                                                                        \
-        if (e_ != hipSuccess)                                                                          \
-            throw AtmError(ATM_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));         \
-    } while (0)
-
-void fail(int code, const std::string& msg) { throw AtmError(code, msg); }
-
-template <typename F>
-int guarded(F&& f) {
-    try {
-        f();
-        return ATM_OK;
-    } catch (const AtmError& e) {
-        g_lastError = e.what();
-        return e.code;
-    } catch (const std::exception& e) {
-        g_lastError = e.what();
-        return ATM_ERR_INVALID_ARGUMENT;
-    }
-}
-
-// Device allocation owned by a context.
-struct DevBuf {
-    void* p = nullptr;
-    size_t bytes = 0;
-    void alloc(size_t n) {
-        release();
-        bytes = std::max<size_t>(n, 256);
-        HIPCHK(hipMalloc(&p, bytes));
-    }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-    }
-    ~DevBuf() { release(); }
-    template <typename T> T* as() const { return static_cast<T*>(p); }
-};
-
 struct Tuning {
     double colScale = 1.0;
     int pairWaves = 1;
@@ -538,7 +497,7 @@ extern "C" {
 
 const char* atm_version(void) { return "atm-mi355x 0.1.0 (ATMMetaForce 0.3.1 semantics)"; }
 int32_t atm_abi_version(void) { return ATM_MI355X_ABI_VERSION; }
-const char* atm_last_error(void) { return g_lastError.c_str(); }
+const char* atm_last_error(void) { return atm::last_error().c_str(); }
 
 int atm_device_count(int32_t* count) {
     return guarded([&] {

@@ -1,0 +1,584 @@
+// gfx950 kernels of the PME variable group: OpenMM's NonbondedForce with
+// particle-mesh Ewald evaluated on x and on x + d, then the ATM combine.
+//
+// Per coordinate set (state s = 0: x, 1: x + d):
+//   k_pme_displace   x + d (state 1)
+//   k_pme_bin        wrap into the box, cell of edge >= rc, rank in the cell
+//   k_pme_scan       cell starts (one block)
+//   k_pme_scatter    atom indices into cell order
+//   k_pme_cellsort   each cell ordered by atom index (deterministic sums), and
+//                    positions / parameters / exclusion keys gathered into
+//                    cell order
+//   k_pme_direct     erfc(alpha r)/r Coulomb + LJ over minimum-image pairs with
+//                    r < rc that are not exceptions; one wave per (64 atoms of
+//                    a cell, one of the 27 neighbour cells); every pair is
+//                    visited from both atoms (no j-force traffic), half the
+//                    energy each
+//   k_pme_spread     order-5 B-spline charge spreading, 64-bit fixed point
+//   k_pme_dft        one axis of the 3-D complex DFT (run 3x forward, 3x back)
+//   k_pme_convolve   influence function, reciprocal energy, convolution
+//   k_pme_gather     reciprocal forces from the potential grid
+//   k_pme_exceptions exception pairs: own Coulomb + LJ, minus erf(alpha r)/r
+// and once per evaluation:
+//   k_pme_finish     fixed-order sums of every energy slot, U1, U2, combine
+//   k_pme_blend      F += (1 - sp) F1 + sp F2 (direction-aware), FP64 out
+//
+// Everything is FP64: positions, pair function, grid, transforms.  Forces are
+// summed in 64-bit fixed point (2^-24 kJ/mol/nm) with integer atomics, energies
+// in per-block slots reduced in a fixed order, so results are bit-reproducible.
+//
+// The model restated is OpenMM's PME NonbondedForce as the reference's
+// end-to-end test builds it (python/tests/test_abfe.py:42-43, through
+// AmberPrmtopFile.createSystem), cloned into the ATM inner contexts at
+// openmmapi/src/ATMMetaForceImpl.cpp:58-63 and combined by
+// platforms/reference/src/ReferenceATMMetaForceKernels.cpp:56-110.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "atm_pme.h"
+#include "atm_math.h"
+
+namespace atm {
+
+namespace {
+
+constexpr double kFix = 16777216.0;          // 2^24 force units per kJ/mol/nm
+constexpr double kGridFix = 68719476736.0;   // 2^36 charge-grid units per e
+constexpr int kOrder = 5;
+
+__device__ __forceinline__ void add_fixed(unsigned long long* p, double f) {
+    atomicAdd(p, (unsigned long long)__double2ll_rn(f * kFix));
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Block sum (fixed order) of one value per thread; valid in thread 0.
+template <int NT>
+__device__ double block_sum(double v, double* red) {
+    v = wave_sum(v);
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) red[w] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0)
+        for (int k = 0; k < NT / 64; ++k) s += red[k];
+    __syncthreads();
+    return s;
+}
+
+__device__ __forceinline__ int wrap_cell(double x, double L, double invL, int m, double* w) {
+    double v = x - L * floor(x * invL);
+    if (v < 0.0) v += L;
+    if (v >= L) v -= L;
+    *w = v;
+    int c = (int)(v * invL * m);
+    return c < 0 ? 0 : (c >= m ? m - 1 : c);
+}
+
+// Order-5 cardinal B-spline weights M5(w + j) and derivatives M5'(w + j),
+// j = 0..4, for w in [0, 1) (two-term recursion M_k(t) = t/(k-1) M_{k-1}(t)
+// + (k-t)/(k-1) M_{k-1}(t-1), from M_2(w) = w, M_2(w+1) = 1 - w).
+__device__ __forceinline__ void bspline5(double w, double th[kOrder], double dth[kOrder]) {
+    double a[kOrder] = {w, 1.0 - w, 0.0, 0.0, 0.0};
+    for (int k = 3; k <= kOrder; ++k) {
+        double nw[kOrder];
+        const double inv = 1.0 / (k - 1);
+        for (int j = 0; j < kOrder; ++j) {
+            const double am1 = j > 0 ? a[j - 1] : 0.0;
+            nw[j] = j < k ? ((w + j) * a[j] + (k - w - j) * am1) * inv : 0.0;
+        }
+        if (k == kOrder)
+            for (int j = 0; j < kOrder; ++j) dth[j] = a[j] - (j > 0 ? a[j - 1] : 0.0);
+        for (int j = 0; j < kOrder; ++j) a[j] = nw[j];
+    }
+    for (int j = 0; j < kOrder; ++j) th[j] = a[j];
+}
+
+}  // namespace
+
+__global__ void k_pme_displace(PmePtrs P, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 3 * n) return;
+    P.x2[i] = P.x[i] + P.displ[i];
+}
+
+__global__ void k_pme_bin(PmeDesc D, PmePtrs P, const double* x) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= D.n) return;
+    double w[3];
+    const int cx = wrap_cell(x[3 * i], D.L[0], D.invL[0], D.m[0], &w[0]);
+    const int cy = wrap_cell(x[3 * i + 1], D.L[1], D.invL[1], D.m[1], &w[1]);
+    const int cz = wrap_cell(x[3 * i + 2], D.L[2], D.invL[2], D.m[2], &w[2]);
+    const int c = (cx * D.m[1] + cy) * D.m[2] + cz;
+    P.cellOf[i] = c;
+    P.rankOf[i] = atomicAdd(&P.cellCount[c], 1);
+    P.wpos[i] = make_double4(w[0], w[1], w[2], 0.0);
+}
+
+__global__ void __launch_bounds__(1024) k_pme_scan(PmeDesc D, PmePtrs P) {
+    __shared__ int part[1024];
+    const int t = threadIdx.x;
+    const int per = (D.ncell + 1023) / 1024;
+    const int b0 = t * per, b1 = min(b0 + per, D.ncell);
+    int s = 0, mx = 0;
+    for (int c = b0; c < b1; ++c) { s += P.cellCount[c]; mx = max(mx, P.cellCount[c]); }
+    part[t] = s;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        const int v = t >= o ? part[t - o] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int run = part[t] - s;
+    for (int c = b0; c < b1; ++c) { P.cellStart[c] = run; run += P.cellCount[c]; }
+    if (t == 1023) P.cellStart[D.ncell] = part[1023];
+    if (mx > D.cellCap) atomicOr(P.overflow, 1);
+}
+
+__global__ void k_pme_scatter(PmeDesc D, PmePtrs P) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= D.n) return;
+    P.cellAtoms[P.cellStart[P.cellOf[i]] + P.rankOf[i]] = i;
+}
+
+// One block per cell: order its atoms by index and gather them into cell order.
+__global__ void __launch_bounds__(256) k_pme_cellsort(PmeDesc D, PmePtrs P) {
+    __shared__ int ids[kPmeCellMax];
+    const int c = blockIdx.x;
+    const int s0 = P.cellStart[c], cnt = min(P.cellStart[c + 1] - s0, kPmeCellMax);
+    for (int k = threadIdx.x; k < cnt; k += blockDim.x) ids[k] = P.cellAtoms[s0 + k];
+    __syncthreads();
+    for (int k = threadIdx.x; k < cnt; k += blockDim.x) {
+        const int a = ids[k];
+        int r = 0;
+        for (int l = 0; l < cnt; ++l) r += ids[l] < a;
+        const int slot = s0 + r;
+        const double4 w = P.wpos[a];
+        const double2 pr = P.prm[a];
+        P.cpos[slot] = make_double4(w.x, w.y, w.z, pr.x);
+        P.cprm[slot] = make_double2(P.prm2[a].x, P.prm2[a].y);
+        const int4 ex = P.exkey[a];
+        P.cex[slot] = make_int4(ex.x, ex.y, ex.z, a);
+    }
+}
+
+// lane = atom i of cell c (64 per block), blockIdx.y = neighbour offset,
+// blockIdx.x = chunk of 64 atoms, blockIdx.z = cell.
+__global__ void __launch_bounds__(64) k_pme_direct(PmeDesc D, PmePtrs P, unsigned long long* F, double* eslot) {
+    const int c = blockIdx.z, nb = blockIdx.y;
+    const int cz = c % D.m[2], cy = (c / D.m[2]) % D.m[1], cx = c / (D.m[1] * D.m[2]);
+    const int ox = nb / 9 - 1, oy = (nb / 3) % 3 - 1, oz = nb % 3 - 1;
+    const int nx = (cx + ox + D.m[0]) % D.m[0], ny = (cy + oy + D.m[1]) % D.m[1], nz = (cz + oz + D.m[2]) % D.m[2];
+    const int cn = (nx * D.m[1] + ny) * D.m[2] + nz;
+    const int s0 = P.cellStart[c], s1 = P.cellStart[c + 1];
+    const int si = s0 + blockIdx.x * 64 + threadIdx.x;
+    const bool active = si < s1;
+    double e = 0.0;
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    const int blockSlot = (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+    if (s0 + blockIdx.x * 64 >= s1) {  // no atom of this chunk: the whole block leaves
+        if (threadIdx.x == 0) eslot[blockSlot] = 0.0;
+        return;
+    }
+    double4 pi = make_double4(0.0, 0.0, 0.0, 0.0);
+    double2 qi = make_double2(0.0, 0.0);
+    int4 xi = make_int4(-1, 0, 0, -1);
+    if (active) { pi = P.cpos[si]; qi = P.cprm[si]; xi = P.cex[si]; }
+    const int j0 = P.cellStart[cn], j1 = P.cellStart[cn + 1];
+    const double a = D.alpha, a2pi = D.twoAlphaSqrtPi;
+    for (int sj = j0; sj < j1; ++sj) {
+        const double4 pj = P.cpos[sj];
+        double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+        dx -= D.L[0] * rint(dx * D.invL[0]);
+        dy -= D.L[1] * rint(dy * D.invL[1]);
+        dz -= D.L[2] * rint(dz * D.invL[2]);
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        bool in = active && r2 < D.rc2 && sj != si;
+        if (__ballot(in) == 0ull) continue;
+        const int4 xj = P.cex[sj];
+        if (in && xj.x == xi.x) {  // same exclusion group: look the pair up
+            const uint32_t bits = P.exbits[xi.z + (xj.y >> 5)];
+            if ((bits >> (xj.y & 31)) & 1u) in = false;
+        }
+        if (in) {
+            const double2 qj = P.cprm[sj];
+            const double rinv = 1.0 / sqrt(r2);
+            const double r = r2 * rinv;
+            const double qq = pi.w * pj.w;
+            const double ec = qq * erfc(a * r) * rinv;
+            const double r2inv = rinv * rinv;
+            const double s = qi.x + qj.x, e4 = qi.y * qj.y;
+            const double s2 = s * s * r2inv, s6 = s2 * s2 * s2;
+            const double elj = e4 * (s6 * s6 - s6);
+            const double fs = (ec + qq * a2pi * exp(-a * a * r2)) * r2inv + e4 * (12.0 * s6 * s6 - 6.0 * s6) * r2inv;
+            e += 0.5 * (ec + elj);
+            fx += fs * dx; fy += fs * dy; fz += fs * dz;
+        }
+    }
+    if (active && (fx != 0.0 || fy != 0.0 || fz != 0.0)) {
+        const int atom = xi.w;
+        add_fixed(F + atom, fx);
+        add_fixed(F + D.n + atom, fy);
+        add_fixed(F + 2 * D.n + atom, fz);
+    }
+    e = wave_sum(e);
+    if (threadIdx.x == 0) eslot[blockSlot] = e;
+}
+
+__global__ void k_pme_spread(PmeDesc D, PmePtrs P, const double* x) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= D.n) return;
+    const double q = P.prm[i].y;
+    if (q == 0.0) return;
+    double th[3][kOrder], dth[kOrder];
+    int g0[3];
+    for (int d = 0; d < 3; ++d) {
+        double f = x[3 * i + d] * D.invL[d];
+        f -= floor(f);
+        const double u = f * D.K[d];
+        int g = (int)floor(u);
+        const double w = u - g;
+        g0[d] = g;
+        bspline5(w, th[d], dth);
+    }
+    for (int a = 0; a < kOrder; ++a) {
+        const int ia = ((g0[0] - a) % D.K[0] + D.K[0]) % D.K[0];
+        for (int b = 0; b < kOrder; ++b) {
+            const int ib = ((g0[1] - b) % D.K[1] + D.K[1]) % D.K[1];
+            const double qab = q * th[0][a] * th[1][b];
+            for (int cc = 0; cc < kOrder; ++cc) {
+                const int ic = ((g0[2] - cc) % D.K[2] + D.K[2]) % D.K[2];
+                atomicAdd(reinterpret_cast<unsigned long long*>(P.qgrid) + ((long long)ia * D.K[1] + ib) * D.K[2] + ic,
+                          (unsigned long long)__double2ll_rn(qab * th[2][cc] * kGridFix));
+            }
+        }
+    }
+}
+
+__global__ void k_pme_q2c(PmeDesc D, PmePtrs P) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= D.ngrid) return;
+    P.g0[k] = make_double2((double)P.qgrid[k] * (1.0 / kGridFix), 0.0);
+}
+
+// One axis of an unnormalised complex DFT: out[..k..] = sum_j in[..j..] e^{sign 2 pi i j k / K}.
+__global__ void k_pme_dft(PmeDesc D, const double2* in, double2* out, const double2* tw, int axis, double sign) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= D.ngrid) return;
+    const int K = D.K[axis];
+    const int stride = axis == 2 ? 1 : (axis == 1 ? D.K[2] : D.K[1] * D.K[2]);
+    const int k = (idx / stride) % K;
+    const int base = idx - k * stride;
+    double re = 0.0, im = 0.0;
+    int t = 0;
+    for (int j = 0; j < K; ++j) {
+        const double2 v = in[base + j * stride];
+        const double2 w = tw[t];
+        const double s = sign * w.y;
+        re += v.x * w.x - v.y * s;
+        im += v.y * w.x + v.x * s;
+        t += k;
+        if (t >= K) t -= K;
+    }
+    out[idx] = make_double2(re, im);
+}
+
+__global__ void __launch_bounds__(256) k_pme_convolve(PmeDesc D, PmePtrs P, double2* g, double* eslot) {
+    __shared__ double red[4];
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    double e = 0.0;
+    if (idx < D.ngrid) {
+        const int c = idx % D.K[2], b = (idx / D.K[2]) % D.K[1], a = idx / (D.K[1] * D.K[2]);
+        const double mx = (a <= D.K[0] / 2 ? a : a - D.K[0]) * D.invL[0];
+        const double my = (b <= D.K[1] / 2 ? b : b - D.K[1]) * D.invL[1];
+        const double mz = (c <= D.K[2] / 2 ? c : c - D.K[2]) * D.invL[2];
+        const double m2 = mx * mx + my * my + mz * mz;
+        double w = 0.0;
+        if (idx != 0) {
+            const double B = P.bmod[a] * P.bmod[D.K[0] + b] * P.bmod[D.K[0] + D.K[1] + c];
+            w = exp(-D.pi2OverAlpha2 * m2) / (m2 * B);
+        }
+        const double2 f = g[idx];
+        e = w * (f.x * f.x + f.y * f.y);
+        g[idx] = make_double2(w * f.x, w * f.y);
+    }
+    const double s = block_sum<256>(e, red);
+    if (threadIdx.x == 0) eslot[blockIdx.x] = s * D.recipEnergyScale;
+}
+
+__global__ void k_pme_gather(PmeDesc D, PmePtrs P, const double* x, const double2* phi, unsigned long long* F) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= D.n) return;
+    const double q = P.prm[i].y;
+    if (q == 0.0) return;
+    double th[3][kOrder], dth[3][kOrder];
+    int g0[3];
+    for (int d = 0; d < 3; ++d) {
+        double f = x[3 * i + d] * D.invL[d];
+        f -= floor(f);
+        const double u = f * D.K[d];
+        const int g = (int)floor(u);
+        g0[d] = g;
+        bspline5(u - g, th[d], dth[d]);
+    }
+    double gx = 0.0, gy = 0.0, gz = 0.0;
+    for (int a = 0; a < kOrder; ++a) {
+        const int ia = ((g0[0] - a) % D.K[0] + D.K[0]) % D.K[0];
+        for (int b = 0; b < kOrder; ++b) {
+            const int ib = ((g0[1] - b) % D.K[1] + D.K[1]) % D.K[1];
+            for (int cc = 0; cc < kOrder; ++cc) {
+                const int ic = ((g0[2] - cc) % D.K[2] + D.K[2]) % D.K[2];
+                const double p = phi[((long long)ia * D.K[1] + ib) * D.K[2] + ic].x;
+                gx += dth[0][a] * th[1][b] * th[2][cc] * p;
+                gy += th[0][a] * dth[1][b] * th[2][cc] * p;
+                gz += th[0][a] * th[1][b] * dth[2][cc] * p;
+            }
+        }
+    }
+    const double s = -q * D.recipPhiScale;
+    add_fixed(F + i, s * gx * D.K[0] * D.invL[0]);
+    add_fixed(F + D.n + i, s * gy * D.K[1] * D.invL[1]);
+    add_fixed(F + 2 * D.n + i, s * gz * D.K[2] * D.invL[2]);
+}
+
+__global__ void __launch_bounds__(256) k_pme_exceptions(PmeDesc D, PmePtrs P, const double* x, unsigned long long* F,
+                                                        double* eslot) {
+    __shared__ double red[4];
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    double e = 0.0;
+    if (k < D.nex) {
+        const int2 pr = P.expairs[k];
+        const double4 ep = P.exprm[k];  // (k_e qq', sigma', 4 eps', k_e q_i q_j)
+        const double dx = x[3 * pr.x] - x[3 * pr.y];
+        const double dy = x[3 * pr.x + 1] - x[3 * pr.y + 1];
+        const double dz = x[3 * pr.x + 2] - x[3 * pr.y + 2];
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        const double rinv = 1.0 / sqrt(r2), r = r2 * rinv, r2inv = rinv * rinv;
+        double fs = 0.0;
+        if (ep.x != 0.0) { e += ep.x * rinv; fs += ep.x * rinv * r2inv; }
+        if (ep.z != 0.0) {
+            const double s2 = ep.y * ep.y * r2inv, s6 = s2 * s2 * s2;
+            e += ep.z * (s6 * s6 - s6);
+            fs += ep.z * (12.0 * s6 * s6 - 6.0 * s6) * r2inv;
+        }
+        if (ep.w != 0.0) {
+            const double ar = D.alpha * r;
+            const double ec = ep.w * erf(ar) * rinv;
+            e -= ec;
+            fs -= (ec - ep.w * D.twoAlphaSqrtPi * exp(-ar * ar)) * r2inv;
+        }
+        if (fs != 0.0) {
+            add_fixed(F + pr.x, fs * dx); add_fixed(F + D.n + pr.x, fs * dy); add_fixed(F + 2 * D.n + pr.x, fs * dz);
+            add_fixed(F + pr.y, -fs * dx); add_fixed(F + D.n + pr.y, -fs * dy); add_fixed(F + 2 * D.n + pr.y, -fs * dz);
+        }
+    }
+    const double s = block_sum<256>(e, red);
+    if (threadIdx.x == 0) eslot[blockIdx.x] = s;
+}
+
+// Fixed-order sums of the energy slots of both states; U = sum + constants;
+// combine.  parts[s][0..4]: direct, exceptions, reciprocal, self, dispersion.
+__global__ void __launch_bounds__(1024) k_pme_finish(PmeDesc D, PmePtrs P, AtmParamsDev prm) {
+    __shared__ double red[16];
+    double U[2];
+    for (int s = 0; s < 2; ++s) {
+        const double* sl = P.eslot + (long long)s * D.nslot;
+        const int ranges[4] = {0, D.nslotDirect, D.nslotDirect + D.nslotEx, D.nslot};
+        double parts[3];
+        for (int r = 0; r < 3; ++r) {
+            double v = 0.0;
+            for (int k = ranges[r] + (int)threadIdx.x; k < ranges[r + 1]; k += 1024) v += sl[k];
+            parts[r] = block_sum<1024>(v, red);
+        }
+        if (threadIdx.x == 0) {
+            P.parts[5 * s + 0] = parts[0];
+            P.parts[5 * s + 1] = parts[1];
+            P.parts[5 * s + 2] = parts[2];
+            P.parts[5 * s + 3] = D.selfEnergy;
+            P.parts[5 * s + 4] = D.dispersionEnergy;
+            U[s] = parts[0] + parts[1] + parts[2] + D.selfEnergy + D.dispersionEnergy;
+        }
+    }
+    if (threadIdx.x == 0) {
+        SysResult r;
+        atm_combine(prm, U[0], U[1], r);
+        *P.result = r;
+    }
+}
+
+__global__ void k_pme_blend(PmeDesc D, PmePtrs P, double* out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= 3 * D.n) return;
+    const SysResult r = *P.result;
+    const double a = D.direction > 0.0 ? r.sp : 1.0 - r.sp;  // weight of state 2 (x + d)
+    const double f1 = (double)(long long)P.F[k] * (1.0 / kFix);
+    const double f2 = (double)(long long)P.F[3ll * D.n + k] * (1.0 / kFix);
+    const int comp = k / D.n, i = k % D.n;
+    out[3 * i + comp] = (1.0 - a) * f1 + a * f2;
+}
+
+__global__ void k_pme_fill(unsigned long long* p, long long n) {
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) p[k] = 0ull;
+}
+
+static inline unsigned blocks_of(long long n, int t) { return (unsigned)((n + t - 1) / t); }
+
+hipError_t launch_pme_fill(unsigned long long* p, long long n, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    k_pme_fill<<<blocks_of(n, 256), 256, 0, st>>>(p, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_pme_state(const PmeDesc& D, const PmePtrs& P, int state, hipStream_t st) {
+    const double* x = state == 0 ? P.x : P.x2;
+    unsigned long long* F = P.F + 3ll * D.n * state;
+    double* es = P.eslot + (long long)D.nslot * state;
+    if (state == 1) {
+        k_pme_displace<<<blocks_of(3ll * D.n, 256), 256, 0, st>>>(P, D.n);
+    }
+    k_pme_fill<<<blocks_of(D.ncell, 256), 256, 0, st>>>(reinterpret_cast<unsigned long long*>(P.cellCount),
+                                                        (D.ncell + 1) / 2);
+    k_pme_fill<<<blocks_of(D.ngrid, 256), 256, 0, st>>>(reinterpret_cast<unsigned long long*>(P.qgrid), D.ngrid);
+    k_pme_fill<<<blocks_of(3ll * D.n, 256), 256, 0, st>>>(F, 3ll * D.n);
+    k_pme_bin<<<blocks_of(D.n, 256), 256, 0, st>>>(D, P, x);
+    k_pme_scan<<<1, 1024, 0, st>>>(D, P);
+    k_pme_scatter<<<blocks_of(D.n, 256), 256, 0, st>>>(D, P);
+    k_pme_cellsort<<<D.ncell, 256, 0, st>>>(D, P);
+    k_pme_direct<<<dim3(D.chunks, 27, D.ncell), 64, 0, st>>>(D, P, F, es);
+    k_pme_exceptions<<<blocks_of(D.nex, 256) > 0 ? blocks_of(D.nex, 256) : 1, 256, 0, st>>>(D, P, x, F,
+                                                                                               es + D.nslotDirect);
+    k_pme_spread<<<blocks_of(D.n, 256), 256, 0, st>>>(D, P, x);
+    k_pme_q2c<<<blocks_of(D.ngrid, 256), 256, 0, st>>>(D, P);
+    const unsigned gb = blocks_of(D.ngrid, 256);
+    k_pme_dft<<<gb, 256, 0, st>>>(D, P.g0, P.g1, P.tw + D.twOff[2], 2, -1.0);
+    k_pme_dft<<<gb, 256, 0, st>>>(D, P.g1, P.g0, P.tw + D.twOff[1], 1, -1.0);
+    k_pme_dft<<<gb, 256, 0, st>>>(D, P.g0, P.g1, P.tw + D.twOff[0], 0, -1.0);
+    k_pme_convolve<<<gb, 256, 0, st>>>(D, P, P.g1, es + D.nslotDirect + D.nslotEx);
+    k_pme_dft<<<gb, 256, 0, st>>>(D, P.g1, P.g0, P.tw + D.twOff[0], 0, 1.0);
+    k_pme_dft<<<gb, 256, 0, st>>>(D, P.g0, P.g1, P.tw + D.twOff[1], 1, 1.0);
+    k_pme_dft<<<gb, 256, 0, st>>>(D, P.g1, P.g0, P.tw + D.twOff[2], 2, 1.0);
+    k_pme_gather<<<blocks_of(D.n, 256), 256, 0, st>>>(D, P, x, P.g0, F);
+    return hipGetLastError();
+}
+
+hipError_t launch_pme_finish(const PmeDesc& D, const PmePtrs& P, const AtmParamsDev& prm, double* forcesOut,
+                             hipStream_t st) {
+    k_pme_finish<<<1, 1024, 0, st>>>(D, P, prm);
+    if (forcesOut) k_pme_blend<<<blocks_of(3ll * D.n, 256), 256, 0, st>>>(D, P, forcesOut);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ bonded terms
+// HarmonicBondForce, HarmonicAngleForce and PeriodicTorsionForce of the
+// system's group 0 (the AMBER terms test_abfe.py keeps beside the ATM force).
+
+__global__ void __launch_bounds__(256) k_bonded(BondedPtrs B, double* eslot) {
+    __shared__ double red[4];
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    double eb = 0.0, ea = 0.0, et = 0.0;
+    const double* x = B.x;
+    auto diff = [&](int i, int j, double d[3]) {
+        for (int c = 0; c < 3; ++c) d[c] = x[3 * i + c] - x[3 * j + c];
+    };
+    auto addf = [&](int i, double s, const double d[3]) {
+        for (int c = 0; c < 3; ++c) add_fixed(B.F + 3ll * i + c, s * d[c]);
+    };
+    if (k < B.nb) {
+        const int2 a = B.bonds[k];
+        const double2 p = B.bprm[k];  // r0, k
+        double d[3];
+        diff(a.x, a.y, d);
+        const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        const double dr = r - p.x;
+        eb = 0.5 * p.y * dr * dr;
+        const double s = -p.y * dr / r;
+        addf(a.x, s, d);
+        addf(a.y, -s, d);
+    } else if (k < B.nb + B.na) {
+        const int4 a = B.angles[k - B.nb];
+        const double2 p = B.aprm[k - B.nb];  // theta0, k
+        double v1[3], v2[3];
+        diff(a.x, a.y, v1);
+        diff(a.z, a.y, v2);
+        const double r1 = sqrt(v1[0] * v1[0] + v1[1] * v1[1] + v1[2] * v1[2]);
+        const double r2 = sqrt(v2[0] * v2[0] + v2[1] * v2[1] + v2[2] * v2[2]);
+        double c = (v1[0] * v2[0] + v1[1] * v2[1] + v1[2] * v2[2]) / (r1 * r2);
+        c = fmin(1.0, fmax(-1.0, c));
+        const double th = acos(c);
+        const double dth = th - p.x;
+        ea = 0.5 * p.y * dth * dth;
+        const double sn = fmax(sqrt(1.0 - c * c), 1e-300);
+        const double g = p.y * dth / sn;  // -dE/dtheta * (-1/sin)
+        double f1[3], f3[3];
+        for (int q = 0; q < 3; ++q) {
+            f1[q] = g * (v2[q] / (r1 * r2) - c * v1[q] / (r1 * r1));
+            f3[q] = g * (v1[q] / (r1 * r2) - c * v2[q] / (r2 * r2));
+        }
+        const double one[3] = {1.0, 1.0, 1.0};
+        for (int q = 0; q < 3; ++q) {
+            add_fixed(B.F + 3ll * a.x + q, f1[q]);
+            add_fixed(B.F + 3ll * a.z + q, f3[q]);
+            add_fixed(B.F + 3ll * a.y + q, -(f1[q] + f3[q]) * one[q]);
+        }
+    } else if (k < B.nb + B.na + B.nt) {
+        const int t = k - B.nb - B.na;
+        const int4 a = B.torsions[t];
+        const double4 p = B.tprm[t];  // periodicity, phase, k
+        double rij[3], rkj[3], rkl[3];
+        diff(a.x, a.y, rij);
+        diff(a.z, a.y, rkj);
+        diff(a.z, a.w, rkl);
+        const double m[3] = {rij[1] * rkj[2] - rij[2] * rkj[1], rij[2] * rkj[0] - rij[0] * rkj[2],
+                             rij[0] * rkj[1] - rij[1] * rkj[0]};
+        const double nv[3] = {rkj[1] * rkl[2] - rkj[2] * rkl[1], rkj[2] * rkl[0] - rkj[0] * rkl[2],
+                              rkj[0] * rkl[1] - rkj[1] * rkl[0]};
+        const double m2 = m[0] * m[0] + m[1] * m[1] + m[2] * m[2];
+        const double n2 = nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2];
+        const double rkj2 = rkj[0] * rkj[0] + rkj[1] * rkj[1] + rkj[2] * rkj[2];
+        const double nrkj = sqrt(rkj2);
+        double cp = (m[0] * nv[0] + m[1] * nv[1] + m[2] * nv[2]) / sqrt(m2 * n2);
+        cp = fmin(1.0, fmax(-1.0, cp));
+        const double sgn = rij[0] * nv[0] + rij[1] * nv[1] + rij[2] * nv[2];
+        const double phi = sgn < 0.0 ? -acos(cp) : acos(cp);
+        et = p.z * (1.0 + cos(p.x * phi - p.y));
+        const double dV = -p.z * p.x * sin(p.x * phi - p.y);
+        double fi[3], fl[3], fj[3], fk[3];
+        const double pp = (rij[0] * rkj[0] + rij[1] * rkj[1] + rij[2] * rkj[2]) / rkj2;
+        const double qq = (rkl[0] * rkj[0] + rkl[1] * rkj[1] + rkl[2] * rkj[2]) / rkj2;
+        for (int q = 0; q < 3; ++q) {
+            fi[q] = -dV * nrkj / m2 * m[q];
+            fl[q] = dV * nrkj / n2 * nv[q];
+        }
+        for (int q = 0; q < 3; ++q) {
+            const double sv = pp * fi[q] - qq * fl[q];
+            fj[q] = fi[q] - sv;
+            fk[q] = fl[q] + sv;
+        }
+        for (int q = 0; q < 3; ++q) {
+            add_fixed(B.F + 3ll * a.x + q, fi[q]);
+            add_fixed(B.F + 3ll * a.y + q, -fj[q]);
+            add_fixed(B.F + 3ll * a.z + q, -fk[q]);
+            add_fixed(B.F + 3ll * a.w + q, fl[q]);
+        }
+    }
+    const double sb = block_sum<256>(eb, red);
+    const double sa = block_sum<256>(ea, red);
+    const double stt = block_sum<256>(et, red);
+    if (threadIdx.x == 0) {
+        eslot[3 * blockIdx.x] = sb;
+        eslot[3 * blockIdx.x + 1] = sa;
+        eslot[3 * blockIdx.x + 2] = stt;
+    }
+}
+
+hipError_t launch_bonded(const BondedPtrs& B, double* eslot, int nblocks, hipStream_t st) {
+    if (nblocks <= 0) return hipSuccess;
+    k_bonded<<<nblocks, 256, 0, st>>>(B, eslot);
+    return hipGetLastError();
+}
+
+}  // namespace atm
