@@ -8,6 +8,8 @@
 namespace atm {
 
 constexpr int kPmeCellMax = 1024;  // atoms per cell the cell sort holds
+constexpr int kPmeJBatch = 192;    // neighbour-cell atoms staged in LDS per batch (direct kernel)
+constexpr int kPmeBinLds = 4096;   // cell grids up to this size are binned through LDS histograms
 
 // Scalars of one evaluation (by value in the kernel arguments).
 struct PmeDesc {

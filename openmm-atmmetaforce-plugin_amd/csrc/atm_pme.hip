@@ -104,17 +104,39 @@ __global__ void k_pme_displace(PmePtrs P, int n) {
     P.x2[i] = P.x[i] + P.displ[i];
 }
 
-__global__ void k_pme_bin(PmeDesc D, PmePtrs P, const double* x) {
+// Cell of every atom and its rank among the cell's atoms (any order: the cell
+// sort fixes it).  Ranks come from a block histogram in LDS and one global
+// atomic per (block, occupied cell), so the few hundred atoms of a cell do not
+// serialise on one counter; grids with more than kPmeBinLds cells use global
+// atomics directly.
+__global__ void __launch_bounds__(256) k_pme_bin(PmeDesc D, PmePtrs P, const double* x) {
+    __shared__ int hist[kPmeBinLds];
+    __shared__ int base[kPmeBinLds];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= D.n) return;
-    double w[3];
-    const int cx = wrap_cell(x[3 * i], D.L[0], D.invL[0], D.m[0], &w[0]);
-    const int cy = wrap_cell(x[3 * i + 1], D.L[1], D.invL[1], D.m[1], &w[1]);
-    const int cz = wrap_cell(x[3 * i + 2], D.L[2], D.invL[2], D.m[2], &w[2]);
-    const int c = (cx * D.m[1] + cy) * D.m[2] + cz;
-    P.cellOf[i] = c;
-    P.rankOf[i] = atomicAdd(&P.cellCount[c], 1);
-    P.wpos[i] = make_double4(w[0], w[1], w[2], 0.0);
+    const bool lds = D.ncell <= kPmeBinLds;
+    if (lds) {
+        for (int c = threadIdx.x; c < D.ncell; c += blockDim.x) hist[c] = 0;
+        __syncthreads();
+    }
+    int c = -1, r = 0;
+    if (i < D.n) {
+        double w[3];
+        const int cx = wrap_cell(x[3 * i], D.L[0], D.invL[0], D.m[0], &w[0]);
+        const int cy = wrap_cell(x[3 * i + 1], D.L[1], D.invL[1], D.m[1], &w[1]);
+        const int cz = wrap_cell(x[3 * i + 2], D.L[2], D.invL[2], D.m[2], &w[2]);
+        c = (cx * D.m[1] + cy) * D.m[2] + cz;
+        P.cellOf[i] = c;
+        P.wpos[i] = make_double4(w[0], w[1], w[2], 0.0);
+        r = lds ? atomicAdd(&hist[c], 1) : atomicAdd(&P.cellCount[c], 1);
+    }
+    if (lds) {
+        __syncthreads();
+        for (int k = threadIdx.x; k < D.ncell; k += blockDim.x)
+            base[k] = hist[k] ? atomicAdd(&P.cellCount[k], hist[k]) : 0;
+        __syncthreads();
+        if (c >= 0) r += base[c];
+    }
+    if (i < D.n) P.rankOf[i] = r;
 }
 
 __global__ void __launch_bounds__(1024) k_pme_scan(PmeDesc D, PmePtrs P) {
@@ -165,67 +187,133 @@ __global__ void __launch_bounds__(256) k_pme_cellsort(PmeDesc D, PmePtrs P) {
     }
 }
 
-// lane = atom i of cell c (64 per block), blockIdx.y = neighbour offset,
-// blockIdx.x = chunk of 64 atoms, blockIdx.z = cell.
+// blockIdx.x = one of the 27 neighbour cells of cell c = blockIdx.y,
+// blockIdx.z = chunk of 64 atoms of c (one per lane).  Every pair is visited
+// from both of its atoms (no j-force traffic), half the energy each.
+//  test:  the wave walks the neighbour cell's atoms; each lane tests its atom
+//         against the common j (FP64 r^2, the image shift known from the cell
+//         offset, exclusion bitmap for atoms of one exclusion group) and the
+//         in-range pairs are appended to a ring in LDS (ballot + mbcnt, so
+//         the order is fixed);
+//  dense: whenever 64 pairs are pending, each lane evaluates one of them
+//         (erfc Coulomb + LJ in FP64), so the expensive pair function runs on
+//         in-cutoff pairs only; i-forces go to per-atom LDS accumulators in
+//         64-bit fixed point (integer atomics: order-independent).
 __global__ void __launch_bounds__(64) k_pme_direct(PmeDesc D, PmePtrs P, unsigned long long* F, double* eslot) {
-    const int c = blockIdx.z, nb = blockIdx.y;
+    __shared__ double4 sI[64];
+    __shared__ double2 sIp[64];
+    __shared__ unsigned long long sF[3][64];
+    __shared__ uint32_t sQ[128];
+    __shared__ double4 sJ[kPmeJBatch];  // the neighbour cell's atoms of the current batch
+    __shared__ double2 sJp[kPmeJBatch];
+    __shared__ int2 sJx[kPmeJBatch];    // their exclusion group and rank
+    const int lane = threadIdx.x;
+    const int nb = blockIdx.x, c = blockIdx.y, chunk = blockIdx.z;
     const int cz = c % D.m[2], cy = (c / D.m[2]) % D.m[1], cx = c / (D.m[1] * D.m[2]);
-    const int ox = nb / 9 - 1, oy = (nb / 3) % 3 - 1, oz = nb % 3 - 1;
-    const int nx = (cx + ox + D.m[0]) % D.m[0], ny = (cy + oy + D.m[1]) % D.m[1], nz = (cz + oz + D.m[2]) % D.m[2];
+    int nx = cx + nb / 9 - 1, ny = cy + (nb / 3) % 3 - 1, nz = cz + nb % 3 - 1;
+    double sx = 0.0, sy = 0.0, sz = 0.0;  // image of the neighbour cell: x_j + s
+    if (nx < 0) { nx += D.m[0]; sx = -D.L[0]; } else if (nx >= D.m[0]) { nx -= D.m[0]; sx = D.L[0]; }
+    if (ny < 0) { ny += D.m[1]; sy = -D.L[1]; } else if (ny >= D.m[1]) { ny -= D.m[1]; sy = D.L[1]; }
+    if (nz < 0) { nz += D.m[2]; sz = -D.L[2]; } else if (nz >= D.m[2]) { nz -= D.m[2]; sz = D.L[2]; }
     const int cn = (nx * D.m[1] + ny) * D.m[2] + nz;
     const int s0 = P.cellStart[c], s1 = P.cellStart[c + 1];
-    const int si = s0 + blockIdx.x * 64 + threadIdx.x;
-    const bool active = si < s1;
-    double e = 0.0;
-    double fx = 0.0, fy = 0.0, fz = 0.0;
-    const int blockSlot = (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
-    if (s0 + blockIdx.x * 64 >= s1) {  // no atom of this chunk: the whole block leaves
-        if (threadIdx.x == 0) eslot[blockSlot] = 0.0;
+    const int blockSlot = (chunk * gridDim.y + c) * gridDim.x + nb;
+    if (s0 + chunk * 64 >= s1) {  // no atom in this chunk: the whole block leaves
+        if (lane == 0) eslot[blockSlot] = 0.0;
         return;
     }
+    const int si = s0 + chunk * 64 + lane;
+    const bool active = si < s1;
     double4 pi = make_double4(0.0, 0.0, 0.0, 0.0);
-    double2 qi = make_double2(0.0, 0.0);
     int4 xi = make_int4(-1, 0, 0, -1);
-    if (active) { pi = P.cpos[si]; qi = P.cprm[si]; xi = P.cex[si]; }
+    uint32_t word0 = 0u;  // first word of this atom's exclusion row (groups of up to 32 atoms)
+    if (active) {
+        pi = P.cpos[si];
+        xi = P.cex[si];
+        sIp[lane] = P.cprm[si];
+        word0 = P.exbits[xi.z];
+    }
+    pi.x -= sx; pi.y -= sy; pi.z -= sz;  // (x_i - s) - x_j = x_i - (x_j + s)
+    sI[lane] = pi;
+    sF[0][lane] = 0ull; sF[1][lane] = 0ull; sF[2][lane] = 0ull;
     const int j0 = P.cellStart[cn], j1 = P.cellStart[cn + 1];
-    const double a = D.alpha, a2pi = D.twoAlphaSqrtPi;
-    for (int sj = j0; sj < j1; ++sj) {
-        const double4 pj = P.cpos[sj];
-        double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-        dx -= D.L[0] * rint(dx * D.invL[0]);
-        dy -= D.L[1] * rint(dy * D.invL[1]);
-        dz -= D.L[2] * rint(dz * D.invL[2]);
-        const double r2 = dx * dx + dy * dy + dz * dz;
-        bool in = active && r2 < D.rc2 && sj != si;
-        if (__ballot(in) == 0ull) continue;
-        const int4 xj = P.cex[sj];
-        if (in && xj.x == xi.x) {  // same exclusion group: look the pair up
-            const uint32_t bits = P.exbits[xi.z + (xj.y >> 5)];
-            if ((bits >> (xj.y & 31)) & 1u) in = false;
-        }
-        if (in) {
-            const double2 qj = P.cprm[sj];
+    const double a = D.alpha, a2pi = D.twoAlphaSqrtPi, rc2 = D.rc2;
+    double e = 0.0;
+    int head = 0, tail = 0;  // ring [head, tail) of pending pairs (wave-uniform)
+    const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+
+    // evaluate `count` queued pairs, one per lane (j data from the staged batch)
+    auto dense = [&](int count) {
+        if (lane < count) {
+            const uint32_t code = sQ[(head + lane) & 127];
+            const int il = code >> 16, k = (int)(code & 0xFFFFu);
+            const double4 p = sI[il];
+            const double2 ip = sIp[il];
+            const double4 pj = sJ[k];
+            const double2 jp = sJp[k];
+            const double dx = p.x - pj.x, dy = p.y - pj.y, dz = p.z - pj.z;
+            const double r2 = dx * dx + dy * dy + dz * dz;
             const double rinv = 1.0 / sqrt(r2);
-            const double r = r2 * rinv;
-            const double qq = pi.w * pj.w;
+            const double r = r2 * rinv, r2inv = rinv * rinv;
+            const double qq = p.w * pj.w;
             const double ec = qq * erfc(a * r) * rinv;
-            const double r2inv = rinv * rinv;
-            const double s = qi.x + qj.x, e4 = qi.y * qj.y;
-            const double s2 = s * s * r2inv, s6 = s2 * s2 * s2;
+            const double sg = ip.x + jp.x, e4 = ip.y * jp.y;
+            const double s2 = sg * sg * r2inv, s6 = s2 * s2 * s2;
             const double elj = e4 * (s6 * s6 - s6);
-            const double fs = (ec + qq * a2pi * exp(-a * a * r2)) * r2inv + e4 * (12.0 * s6 * s6 - 6.0 * s6) * r2inv;
+            const double fs = (ec + qq * a2pi * exp(-a * a * r2) + e4 * (12.0 * s6 * s6 - 6.0 * s6)) * r2inv;
             e += 0.5 * (ec + elj);
-            fx += fs * dx; fy += fs * dy; fz += fs * dz;
+            atomicAdd(&sF[0][il], (unsigned long long)__double2ll_rn(fs * dx * kFix));
+            atomicAdd(&sF[1][il], (unsigned long long)__double2ll_rn(fs * dy * kFix));
+            atomicAdd(&sF[2][il], (unsigned long long)__double2ll_rn(fs * dz * kFix));
+        }
+        head += count;
+    };
+
+    for (int jb = j0; jb < j1; jb += kPmeJBatch) {
+        const int nbat = min(kPmeJBatch, j1 - jb);
+        __syncthreads();  // the previous batch is fully evaluated
+        for (int k = lane; k < nbat; k += 64) {
+            sJ[k] = P.cpos[jb + k];
+            sJp[k] = P.cprm[jb + k];
+            const int4 x = P.cex[jb + k];
+            sJx[k] = make_int2(x.x, x.y);
+        }
+        __syncthreads();
+        for (int k = 0; k < nbat; ++k) {
+            const double4 pj = sJ[k];
+            const double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            bool in = active && r2 < rc2 && jb + k != si;
+            if (__ballot(in) == 0ull) continue;
+            const int2 xj = sJx[k];
+            if (in && xj.x == xi.x) {  // same exclusion group: look the pair up
+                const uint32_t bits = (xj.y >> 5) == 0 ? word0 : P.exbits[xi.z + (xj.y >> 5)];
+                if ((bits >> (xj.y & 31)) & 1u) in = false;
+            }
+            const uint64_t m = __ballot(in);
+            if (in) sQ[(tail + __popcll(m & below)) & 127] = ((uint32_t)lane << 16) | (uint32_t)k;
+            tail += __popcll(m);
+            if (tail - head >= 64) {
+                __syncthreads();
+                dense(64);
+                __syncthreads();
+            }
+        }
+        if (tail > head) {  // drain: the next batch replaces the staged j data
+            __syncthreads();
+            dense(tail - head);
         }
     }
-    if (active && (fx != 0.0 || fy != 0.0 || fz != 0.0)) {
+    __syncthreads();
+    if (active) {
         const int atom = xi.w;
-        add_fixed(F + atom, fx);
-        add_fixed(F + D.n + atom, fy);
-        add_fixed(F + 2 * D.n + atom, fz);
+        for (int k = 0; k < 3; ++k) {
+            const unsigned long long v = sF[k][lane];
+            if (v) atomicAdd(F + (long long)k * D.n + atom, v);
+        }
     }
     e = wave_sum(e);
-    if (threadIdx.x == 0) eslot[blockSlot] = e;
+    if (lane == 0) eslot[blockSlot] = e;
 }
 
 __global__ void k_pme_spread(PmeDesc D, PmePtrs P, const double* x) {
@@ -444,11 +532,13 @@ hipError_t launch_pme_state(const PmeDesc& D, const PmePtrs& P, int state, hipSt
                                                         (D.ncell + 1) / 2);
     k_pme_fill<<<blocks_of(D.ngrid, 256), 256, 0, st>>>(reinterpret_cast<unsigned long long*>(P.qgrid), D.ngrid);
     k_pme_fill<<<blocks_of(3ll * D.n, 256), 256, 0, st>>>(F, 3ll * D.n);
-    k_pme_bin<<<blocks_of(D.n, 256), 256, 0, st>>>(D, P, x);
+    k_pme_bin<<<blocks_of(D.n, 256), 256, 0, st>>>(D, P, x);  // 256 threads: the histogram loops assume it
     k_pme_scan<<<1, 1024, 0, st>>>(D, P);
     k_pme_scatter<<<blocks_of(D.n, 256), 256, 0, st>>>(D, P);
     k_pme_cellsort<<<D.ncell, 256, 0, st>>>(D, P);
-    k_pme_direct<<<dim3(D.chunks, 27, D.ncell), 64, 0, st>>>(D, P, F, es);
+    // neighbour offset fastest, chunk slowest: consecutive workgroups (dealt round-robin to the
+    // XCDs) carry comparable work, and the mostly empty late chunks come last
+    k_pme_direct<<<dim3(27, D.ncell, D.chunks), 64, 0, st>>>(D, P, F, es);
     k_pme_exceptions<<<blocks_of(D.nex, 256) > 0 ? blocks_of(D.nex, 256) : 1, 256, 0, st>>>(D, P, x, F,
                                                                                                es + D.nslotDirect);
     k_pme_spread<<<blocks_of(D.n, 256), 256, 0, st>>>(D, P, x);
