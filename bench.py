@@ -145,6 +145,92 @@ def md_timing(reps, steps=100):
             "note": "each step: Langevin update of all replicas + one batched ATM evaluation (both states)"}
 
 
+def pme_timing(reps=30, cpu=True):
+    """The reference's end-to-end system (python/tests/test_abfe.py): temoa-g1
+    host-guest in water (8,479 atoms, AMBER, PME with a 1 nm cutoff, ligand
+    displaced by 2.2 nm), ATM variable group = the PME NonbondedForce.  One
+    evaluation = U1, U2, combine and hybrid force on the device, positions in
+    and forces out through host memory; the bonded terms and restraints of
+    group 0 beside it.  Checked against the reference's goldens (potential
+    energy -116071.0 and perturbation energy 58.2 kJ/mol, within 0.1)."""
+    import ctypes
+    import numpy as np
+    from atmmetaforce import _capi as C, amber, pme
+    s, arr = amber.system_from_npz(os.path.join(ROOT, "tests", "golden", "temoa-g1.npz"))
+    box, x, ref40 = arr["state/box"], arr["state/positions"], arr["inpcrd/positions40"]
+    kcal = 4.184
+    rcpt, lig = s.residue_atoms(1), s.residue_atoms(2)
+    d = np.zeros((s.num_atoms, 3))
+    d[lig] = 2.2
+    params = dict(lambda1=0.5, lambda2=0.5, alpha=0.0, u0=0.0, w0=0.0, umax=200.0 * kcal, ubcore=100.0 * kcal,
+                  acore=0.0625, direction=0.0625)
+    f, b = pme.from_amber(s, d, box, params=params)
+    try:
+        alpha, grid = f.info()
+        for _ in range(3):
+            r, F, parts = f.evaluate(x, box)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            r, F, parts = f.evaluate(x, box)
+        dt = (time.perf_counter() - t0) / reps
+        Eb, _ = b.evaluate(x)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            b.evaluate(x)
+        dtb = (time.perf_counter() - t0) / reps
+    finally:
+        f.close()
+        b.close()
+    # group-0 restraints of test_abfe.py:58-79 through the C ABI
+    L = C.load()
+    kf = 25.0 * kcal * 100.0
+    posr = np.array([a for a in rcpt if s.atom_names[a].startswith("C") and a < 40])
+    members = np.ascontiguousarray(np.concatenate([lig, rcpt]).astype(np.int32))
+    w = np.ascontiguousarray(np.concatenate([s.masses[lig], s.masses[rcpt]]))
+    cent = (C.CentroidRestraint * 1)()
+    cent[0] = C.CentroidRestraint(0, len(lig), len(lig), len(rcpt), kf, 0.5, (ctypes.c_double * 3)(0.0, 0.0, 0.0))
+    pr = (C.PositionRestraint * len(posr))()
+    for k, a in enumerate(posr):
+        pr[k] = C.PositionRestraint(int(a), 1, (ctypes.c_double * 3)(*ref40[a]), kf, 0.05)
+    h = ctypes.c_void_p()
+    C.check(L.atm_restraints_create(0, ctypes.byref(h)))
+    er = ctypes.c_double()
+    try:
+        dp = lambda a: np.ascontiguousarray(a).ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        xx, bb = np.ascontiguousarray(x), np.ascontiguousarray(box)
+        C.check(L.atm_restraints_set(h, s.num_atoms, len(members),
+                                     members.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), dp(w), 1, cent,
+                                     len(posr), pr))
+        C.check(L.atm_restraints_eval(h, dp(xx), dp(bb), None, ctypes.byref(er)))
+    finally:
+        L.atm_restraints_destroy(h)
+    pot = sum(Eb.values()) + er.value + r["energy"]
+    out = {"workload": "temoa-g1 host-guest in water, 8,479 atoms, AMBER, PME (alpha %.4f/nm, grid %s), rc 1 nm; "
+                       "ATM variable group = PME NonbondedForce, ligand displaced by 2.2 nm" % (alpha, "x".join(map(str, grid))),
+           "ms_per_eval": round(dt * 1e3, 4), "evals_per_s": round(1.0 / dt, 2),
+           "bonded_ms_per_eval": round(dtb * 1e3, 4),
+           "potential_energy": round(pot, 4), "perturbation_energy": round(r["perturbation_energy"], 4),
+           "golden": {"potential_energy": -116071.0, "perturbation_energy": 58.2, "atol": 0.1,
+                      "ok": bool(abs(pot + 116071.0) <= 0.1 and abs(r["perturbation_energy"] - 58.2) <= 0.1)},
+           "note": "host positions in, host forces out (PCIe included); FP64 throughout"}
+    if cpu:
+        sys.path.insert(0, ROOT)
+        from oracle import pme_oracle as po
+        t0 = time.perf_counter()
+        U1, _, _ = po.nonbonded_pme(x, s.charges, s.sigmas, s.epsilons, box, 1.0, s.exception_pairs, s.exception_qq,
+                                    s.exception_sigma, s.exception_eps)
+        x2 = x.copy()
+        x2[lig] += 2.2
+        U2, _, _ = po.nonbonded_pme(x2, s.charges, s.sigmas, s.epsilons, box, 1.0, s.exception_pairs,
+                                    s.exception_qq, s.exception_sigma, s.exception_eps)
+        tc = time.perf_counter() - t0
+        out["cpu_oracle"] = {"s_per_eval": round(tc, 3), "cores": 1, "kind": "port",
+                             "sample": "one evaluation (both states, forces) by the FP64 numpy oracle",
+                             "U1_rel_err": abs(r["state1_energy"] - U1) / abs(U1),
+                             "U2_rel_err": abs(r["state2_energy"] - U2) / abs(U2)}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -267,6 +353,8 @@ def main():
     restraints = restraint_timing(base)
     log("[bench] batched lambda-replica MD")
     md = md_timing(reps)
+    log("[bench] PME variable group (temoa-g1)")
+    pme_v = pme_timing(cpu=not args.no_cpu)
 
     cpu = None
     err = None
@@ -295,6 +383,7 @@ def main():
                          "frac": round(blend_gbs / PEAK_HBM_GBS, 4)},
         "companion_restraints": restraints,
         "replica_md": md,
+        "pme_variable_group": pme_v,
         "error_vs_fp64_oracle_replica0": err,
         "precision": "coordinates, displacement, cutoff geometry and force vectors FP32; scalar pair function "
                      "(r^2, 1/r, (s/r)^6, energies, force magnitude) FP64; U1/U2/u/W FP64; forces 64-bit "

@@ -233,3 +233,14 @@ def system_from_sections(s: dict) -> AmberSystem:
 
 def read_prmtop(path: str) -> AmberSystem:
     return system_from_sections(read_prmtop_sections(path))
+
+
+def system_from_npz(path: str):
+    """(AmberSystem, arrays) from an archive of prmtop sections: entries
+    "prmtop/<FLAG>" hold the sections, every other entry is returned in
+    `arrays` (e.g. the box and positions of a State checkpoint)."""
+    with np.load(path, allow_pickle=False) as z:
+        sec = {k[len("prmtop/"):]: z[k] for k in z.files if k.startswith("prmtop/")}
+        arrays = {k: z[k].copy() for k in z.files if not k.startswith("prmtop/")}
+    sec = {k: ([str(v) for v in a] if a.dtype.kind == "U" else a.tolist()) for k, a in sec.items()}
+    return system_from_sections(sec), arrays

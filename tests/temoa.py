@@ -39,13 +39,8 @@ ATM_PARAMS = dict(lambda1=0.5, lambda2=0.5, alpha=0.0, u0=0.0, w0=0.0, umax=200.
 
 def load():
     """(AmberSystem, box, positions, restraint reference positions of atoms 0..39)."""
-    with np.load(FIXTURE, allow_pickle=False) as z:
-        sec = {k[len("prmtop/"):]: z[k] for k in z.files if k.startswith("prmtop/")}
-        box = z["state/box"].copy()
-        pos = z["state/positions"].copy()
-        ref40 = z["inpcrd/positions40"].copy()
-    sec = {k: ([str(s) for s in v] if v.dtype.kind == "U" else v.tolist()) for k, v in sec.items()}
-    return amber.system_from_sections(sec), box, pos, ref40
+    s, arr = amber.system_from_npz(FIXTURE)
+    return s, arr["state/box"], arr["state/positions"], arr["inpcrd/positions40"]
 
 
 def restraint_setup(s: amber.AmberSystem):
