@@ -172,3 +172,48 @@ def from_amber(system, displacements, box, params=None, cutoff=1.0, ewald_tolera
     bonded = BondedForces(system.num_atoms, system.bonds, system.bond_params, system.angles, system.angle_params,
                           system.torsions, system.torsion_params, device=device)
     return pme, bonded
+
+
+def _main(argv=None):
+    """Time the PME ATM evaluation of an AMBER system:
+    python -m atmmetaforce.pme --npz tests/golden/temoa-g1.npz [--reps N]
+    (or --prmtop F --state F); ligand = residue --ligand, displaced by --displ nm."""
+    import argparse
+    import time
+    from . import amber
+    ap = argparse.ArgumentParser(prog="python -m atmmetaforce.pme")
+    ap.add_argument("--npz")
+    ap.add_argument("--prmtop")
+    ap.add_argument("--state")
+    ap.add_argument("--ligand", type=int, default=2)
+    ap.add_argument("--displ", type=float, default=2.2)
+    ap.add_argument("--reps", type=int, default=20)
+    a = ap.parse_args(argv)
+    if a.npz:
+        system, arr = amber.system_from_npz(a.npz)
+        box, x = arr["state/box"], arr["state/positions"]
+    else:
+        system = amber.read_prmtop(a.prmtop)
+        box, x = amber.read_state_xml(a.state)
+    d = np.zeros((system.num_atoms, 3))
+    d[system.residue_atoms(a.ligand)] = a.displ
+    params = dict(lambda1=0.5, lambda2=0.5, alpha=0.0, u0=0.0, w0=0.0, umax=836.8, ubcore=418.4, acore=0.0625,
+                  direction=1.0)
+    f, b = from_amber(system, d, box, params=params)
+    try:
+        for _ in range(3):
+            f.evaluate(x, box)
+        t0 = time.perf_counter()
+        for _ in range(a.reps):
+            r, F, parts = f.evaluate(x, box)
+        dt = (time.perf_counter() - t0) / a.reps
+        eb, _ = b.evaluate(x)
+    finally:
+        f.close()
+        b.close()
+    print(f"atoms {system.num_atoms}  ms/eval {dt * 1e3:.3f}  W {r['energy']:.4f}  u_sc {r['perturbation_energy']:.4f}  "
+          f"bonded {sum(eb.values()):.4f}", flush=True)
+
+
+if __name__ == "__main__":
+    _main()

@@ -25,6 +25,16 @@ def counters(path):
     return d, rows
 
 
+def trace_table(path, top=20):
+    """Markdown table of a rocprofv3 kernel_stats.csv, slowest total first."""
+    rows = sorted(csv.DictReader(open(path)), key=lambda r: -float(r["TotalDurationNs"]))[:top]
+    out = ["| kernel | calls | avg us | min us | max us | % of GPU time |", "|---|---|---|---|---|---|"]
+    for r in rows:
+        out.append(f"| `{r['Name'].split('(')[0].replace('void ', '')}` | {r['Calls']} | {float(r['AverageNs']) / 1e3:.2f} | "
+                   f"{float(r['MinNs']) / 1e3:.2f} | {float(r['MaxNs']) / 1e3:.2f} | {float(r['Percentage']):.2f} |")
+    return "\n".join(out)
+
+
 def main(root):
     out = []
     stats = list(csv.DictReader(open(os.path.join(root, "trace", "run_kernel_stats.csv"))))
@@ -77,6 +87,10 @@ def main(root):
                "k_blend": {"hbm_bytes_per_launch": round(sum(traffic.get("atm::k_blend", (0, 0, 0))[:2]))}},
               open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "traffic.json"), "w"), indent=1)
 
+
+if __name__ == "__main__" and len(sys.argv) > 2 and sys.argv[1] == "--trace":
+    print(trace_table(sys.argv[2]))
+    sys.exit(0)
 
 if __name__ == "__main__":
     main(sys.argv[1] if len(sys.argv) > 1 else "gpurun_out")
