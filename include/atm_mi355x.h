@@ -110,7 +110,9 @@ typedef struct atm_tuning {
     int32_t pair_kernel;      /* 0: 8x8 cluster-pair tiles; 1-4: compacted in-cutoff pairs
                                  (FP64 pair function only) with (list entries per batch,
                                  copies of the j-force sums, drain after every test step)
-                                 = (4,2,yes), (4,2,no), (4,4,yes), (8,2,yes) */
+                                 = (4,2,yes), (4,2,no), (4,4,yes), (8,2,yes); 5: atom-level
+                                 j lists per i-cluster (j atoms within the list cutoff of
+                                 the cluster's box), 8 i x 8 j lanes per step (FP64 only) */
     int32_t reserved;
 } atm_tuning;
 

@@ -59,6 +59,7 @@ struct Context {
     std::vector<int> entryCapOverride;  // per set, 0 = automatic
     int maxClusters = 0, maxGroups = 0, maxCells = 0, maxSlots = 0;
     long long totalCells = 0, totalCols = 0, totalSlots = 0, totalClusters = 0, totalGroups = 0, totalEntries = 0;
+    long long totalAlist = 0;
     Tuning tuning;
     double tunedMs = 0.0;
 
@@ -66,6 +67,7 @@ struct Context {
     DevBuf dDesc, dRt, dPosq, dDispl, dLjp, dPrm64, dParams, dRes, dOutF;
     DevBuf dBpos, dBcell, dBrank, dSlotOf, dCellCount, dCellStart, dColStart, dColCount;
     DevBuf dSpos, dSprm, dSprm64, dSidx, dSf, dSd, dBbc, dBbh, dGinfo, dGcount, dGenergy, dEntries, dSetEnergy;
+    DevBuf dAlist, dAcount;
     DevBuf dGq, dGsig, dGeps, dWset, dDq, dDsig, dDeps;
     // batched Langevin dynamics (atm_md_*)
     DevBuf dMx, dMv, dMinvm, dMxSnap, dMvSnap;
@@ -189,7 +191,7 @@ SetDesc make_set(const atm_system_desc& d, const Tuning& tu, int s, int state) {
 void layout(Context& C) {
     C.sets.resize(2 * C.nsys);
     C.entryCapOverride.resize(2 * C.nsys, 0);
-    long long cellOff = 0, colOff = 0, slotOff = 0, clOff = 0, gOff = 0, eOff = 0, setAtomOff = 0;
+    long long cellOff = 0, colOff = 0, slotOff = 0, clOff = 0, gOff = 0, eOff = 0, aOff = 0, setAtomOff = 0;
     C.maxClusters = C.maxGroups = C.maxCells = C.maxSlots = 0;
     for (int s = 0; s < C.nsys; ++s) {
         for (int st = 0; st < 2; ++st) {
@@ -204,6 +206,8 @@ void layout(Context& C) {
             D.clusterOff = (int)clOff;
             D.groupOff = (int)gOff;
             D.entryOff = eOff;
+            D.alistCap = (D.entryCap + 7) / 8 * 8;
+            D.alistOff = aOff;
             if (D.clusterCap >= (1 << 20)) fail(ATM_ERR_UNSUPPORTED, "too many clusters in one system");
             setAtomOff += D.natoms;
             cellOff += (long long)D.nx * D.ny * D.nz;
@@ -212,6 +216,7 @@ void layout(Context& C) {
             clOff += D.clusterCap;
             gOff += D.groupCap;
             eOff += (long long)D.groupCap * D.entryCap;
+            aOff += (long long)D.clusterCap * D.alistCap;
             C.maxClusters = std::max(C.maxClusters, D.clusterCap);
             C.maxGroups = std::max(C.maxGroups, D.groupCap);
             C.maxCells = std::max(C.maxCells, D.nx * D.ny * D.nz);
@@ -226,6 +231,7 @@ void layout(Context& C) {
     C.totalClusters = clOff;
     C.totalGroups = gOff;
     C.totalEntries = eOff;
+    C.totalAlist = aOff;
 }
 
 void alloc_layout(Context& C) {
@@ -255,6 +261,13 @@ void alloc_layout(Context& C) {
     C.dGcount.alloc(sizeof(int) * C.totalGroups);
     C.dGenergy.alloc(sizeof(double) * C.totalGroups);
     C.dEntries.alloc(sizeof(uint32_t) * C.totalEntries);
+    if (C.tuning.pairKernel == kPairKernelAtomLists) {  // only the atom-list pair kernel reads them
+        C.dAlist.alloc(sizeof(uint32_t) * C.totalAlist);
+        C.dAcount.alloc(sizeof(int) * C.totalClusters);
+    } else {
+        C.dAlist.release();
+        C.dAcount.release();
+    }
     C.dSetEnergy.alloc(sizeof(double) * S2);
     C.dGq.release(); C.dGsig.release(); C.dGeps.release();
     zero_on_stream(C, C.dRt.p, sizeof(SetRuntime) * S2);
@@ -293,8 +306,15 @@ void alloc_layout(Context& C) {
     P.gcount = C.dGcount.as<int>();
     P.genergy = C.dGenergy.as<double>();
     P.entries = C.dEntries.as<uint32_t>();
+    P.alist = C.dAlist.as<uint32_t>();
+    P.acount = C.dAcount.as<int>();
     P.setEnergy = C.dSetEnergy.as<double>();
     P.gq = P.gsig = P.geps = nullptr;
+}
+
+// Pair kernel variant 5 (FP64 pairs only) reads the atom-level j lists.
+bool uses_atom_lists(const Context& C, int flags) {
+    return C.tuning.pairKernel == kPairKernelAtomLists && (flags & ATM_EVAL_FP32_PAIRS) == 0;
 }
 
 void require_ready(const Context& C) {
@@ -316,6 +336,7 @@ void enqueue_batched(Context& C, int flags, hipEvent_t* ev = nullptr) {
     HIPCHK(launch_build(P, 0, S2, C.maxAtoms, C.maxCells, C.maxSlots, st));
     if (ev) HIPCHK(hipEventRecord(ev[1], st));
     HIPCHK(launch_pairlist(P, 0, S2, C.maxGroups, 4, st));
+    if (uses_atom_lists(C, flags)) HIPCHK(launch_alist(P, 0, S2, C.maxGroups, st));
     if (ev) HIPCHK(hipEventRecord(ev[2], st));
     HIPCHK(launch_pairs(P, 0, S2, C.maxGroups, C.tuning.pairWaves, (flags & ATM_EVAL_FORCES) != 0,
                         (flags & ATM_EVAL_FP32_PAIRS) == 0, C.tuning.pairKernel, st));
@@ -374,6 +395,7 @@ void eval_plain(Context& C, int flags) {
             HIPCHK(launch_zero_counts(P, D.cellOff, D.cellOff + D.nx * D.ny * D.nz, c, c + 1, st));
             HIPCHK(launch_build(P, c, 1, D.natoms, D.nx * D.ny * D.nz, D.slotCap, st));
             HIPCHK(launch_pairlist(P, c, 1, D.groupCap, 4, st));
+            if (uses_atom_lists(C, flags)) HIPCHK(launch_alist(P, c, 1, D.groupCap, st));
             HIPCHK(launch_pairs(P, c, 1, D.groupCap, C.tuning.pairWaves, true, (flags & ATM_EVAL_FP32_PAIRS) == 0,
                                 C.tuning.pairKernel, st));
             HIPCHK(launch_set_energy(P, c, st));
@@ -404,6 +426,7 @@ void eval_state1(Context& C, int flags) {
         HIPCHK(launch_zero_counts(P, D.cellOff, D.cellOff + D.nx * D.ny * D.nz, c, c + 1, st));
         HIPCHK(launch_build(P, c, 1, D.natoms, D.nx * D.ny * D.nz, D.slotCap, st));
         HIPCHK(launch_pairlist(P, c, 1, D.groupCap, 4, st));
+        if (uses_atom_lists(C, flags)) HIPCHK(launch_alist(P, c, 1, D.groupCap, st));
         HIPCHK(launch_pairs(P, c, 1, D.groupCap, C.tuning.pairWaves, (flags & ATM_EVAL_FORCES) != 0,
                             (flags & ATM_EVAL_FP32_PAIRS) == 0, C.tuning.pairKernel, st));
         HIPCHK(launch_set_energy(P, c, st));
@@ -839,12 +862,13 @@ int atm_set_tuning(void* context, const atm_tuning* t) {
             t->pair_waves_per_block != 8)
             fail(ATM_ERR_INVALID_ARGUMENT, "pair_waves_per_block must be 1, 2, 4 or 8");
         if (t->zbins_per_column < 1 || t->zbins_per_column > 16) fail(ATM_ERR_INVALID_ARGUMENT, "zbins_per_column must be in [1, 16]");
-        if (t->pair_kernel < 0 || t->pair_kernel > 4) fail(ATM_ERR_INVALID_ARGUMENT, "pair_kernel must be in [0, 4]");
+        if (t->pair_kernel < 0 || t->pair_kernel > 5) fail(ATM_ERR_INVALID_ARGUMENT, "pair_kernel must be in [0, 5]");
         if (t->pair_kernel != 0 && t->pair_waves_per_block == 8)
-            fail(ATM_ERR_INVALID_ARGUMENT, "the compacted pair kernel takes 1, 2 or 4 waves per block");
+            fail(ATM_ERR_INVALID_ARGUMENT, "pair kernels 1-5 take 1, 2 or 4 waves per block");
         HIPCHK(hipSetDevice(C.device));
         HIPCHK(hipStreamSynchronize(C.stream));
-        const bool relayout = t->column_scale != C.tuning.colScale || t->zbins_per_column != C.tuning.zbinsPerColumn;
+        const bool relayout = t->column_scale != C.tuning.colScale || t->zbins_per_column != C.tuning.zbinsPerColumn ||
+                              (t->pair_kernel == kPairKernelAtomLists) != (C.tuning.pairKernel == kPairKernelAtomLists);
         C.tuning.colScale = t->column_scale;
         C.tuning.zbinsPerColumn = t->zbins_per_column;
         C.tuning.pairWaves = t->pair_waves_per_block;
@@ -868,6 +892,8 @@ int atm_autotune(void* context, int32_t repeats, atm_tuning* out) {
         const int reps = std::max(1, (int)repeats);
         const double scales[] = {0.9, 1.0, 1.15, 1.3, 1.5};
         const int waves[] = {1, 2, 4};
+        // (the atom-list kernel, 5, is left out: it measured slower than the tile kernel on every
+        // column scale tried, and its lists need their own buffers)
         const int kernels[] = {0, 1, 2};
         const int zpcs[] = {2, 4};
         hipEvent_t e0, e1;

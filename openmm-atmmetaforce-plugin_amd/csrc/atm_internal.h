@@ -48,7 +48,8 @@ struct SetDesc {
     int groupOff, groupCap;
     long long entryOff;      // first entry of group 0 of this set
     int entryCap;            // entries per group
-    int pad_;
+    int alistCap;            // atom-level list entries per cluster (multiple of 8)
+    long long alistOff;      // first atom-level list entry of cluster 0 of this set
 };
 
 struct SetRuntime {

@@ -47,6 +47,8 @@ struct DevPtrs {
     int* gcount;
     double* genergy;
     uint32_t* entries;
+    uint32_t* alist;       // atom-level j lists per i-cluster (pair kernel 5)
+    int* acount;           // entries (whole 8-atom blocks) per i-cluster
     double* setEnergy;     // [2 S] (reference-structured path)
     // backward
     double* gq;            // [2 S sets x slots] gradient accumulators in slot order
@@ -58,9 +60,12 @@ hipError_t launch_zero_counts(const DevPtrs& P, int cellBegin, int cellEnd, int 
 hipError_t launch_build(const DevPtrs& P, int setBegin, int nsets, int maxAtoms, int maxCells, int maxSlots,
                         hipStream_t st);
 hipError_t launch_pairlist(const DevPtrs& P, int setBegin, int nsets, int maxGroups, int listWaves, hipStream_t st);
+// atom-level j lists from the cluster list (needed by pair kernel variant kPairKernelAtomLists)
+constexpr int kPairKernelAtomLists = 5;
+hipError_t launch_alist(const DevPtrs& P, int setBegin, int nsets, int maxGroups, hipStream_t st);
 // variant 0: 8x8 tile kernel; 1-4: compacted kernel (FP64 pairs only) with (list entries per batch,
 // copies of the j-force sums, drain after every test step) = (4, 2, yes), (4, 2, no), (4, 4, yes),
-// (8, 2, yes)
+// (8, 2, yes); 5: atom-level j lists (launch_alist first), 8 i x 8 j lanes per step
 hipError_t launch_pairs(const DevPtrs& P, int setBegin, int nsets, int maxGroups, int waves, bool forces, bool fp64,
                         int variant, hipStream_t st);
 hipError_t launch_energy(const DevPtrs& P, int sysBegin, int nsys, hipStream_t st);

@@ -670,6 +670,176 @@ __global__ void __launch_bounds__(64 * W) k_pairs(DevPtrs P, int setBegin) {
     if (lane == 0) P.genergy[D.groupOff + g] = eacc;
 }
 
+// ------------------------------------------------- atom-level j lists
+
+// For every i-cluster t of a group, the j-atoms of the group's list entries
+// (j-clusters in the entry's mask for t) that lie within the list cutoff of
+// t's bounding box, appended in (entry, atom) order to t's list in 8-atom
+// blocks.  Entry = j-slot within the set (23 bits) | x-shift code << 28 |
+// y-shift code << 30; padding entries are kNoAtom.
+constexpr uint32_t kNoAtom = 0xFFFFFFFFu;
+
+__global__ void __launch_bounds__(64) k_alist(DevPtrs P, int setBegin) {
+    // one wave per (group, i-cluster t): lane = 8 e + jj tests atom jj of list
+    // entry e of the current 8-entry window against t's bounding box
+    const int c = setBegin + blockIdx.y;
+    const SetDesc D = P.desc[c];
+    const int g = blockIdx.x >> 3, t = blockIdx.x & 7, lane = threadIdx.x;
+    if (g >= P.rt[c].ngroups) return;
+    const int4 gi = P.ginfo[D.groupOff + g];
+    const int g0 = __builtin_amdgcn_readfirstlane(gi.x), ncl = __builtin_amdgcn_readfirstlane(gi.y);
+    if (t >= ncl) return;
+    const int e = lane >> 3, jj = lane & 7;
+    const float4 bc = P.bbc[D.clusterOff + g0 + t], bh = P.bbh[D.clusterOff + g0 + t];
+    uint32_t* out = P.alist + D.alistOff + (long long)(g0 + t) * D.alistCap;
+    int cnt = 0;
+    const int ne = P.gcount[D.groupOff + g];
+    const uint32_t* E = P.entries + D.entryOff + (long long)g * D.entryCap;
+    const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    for (int e0 = 0; e0 < ne; e0 += 64) {
+        const uint32_t ebuf = (e0 + lane < ne) ? E[e0 + lane] : 0u;
+        const int nb = min(64, ne - e0);
+        for (int w = 0; w < nb; w += 8) {
+            const uint32_t ent = (uint32_t)__shfl((int)ebuf, w + e, 64);
+            const bool has = w + e < nb && ((ent >> (20 + t)) & 1u);
+            const int js = (int)(ent & 0xFFFFFu) * kClusterSize + jj;  // slot within the set
+            bool keep = false;
+            if (has && P.sidx[D.slotOff + js] >= 0) {
+                float4 pj = P.spos[D.slotOff + js];
+                if (D.shiftOK) {
+                    const uint32_t sxc = (ent >> 28) & 3u, syc = (ent >> 30) & 3u;
+                    pj.x += sxc == 1u ? D.L[0] : (sxc == 2u ? -D.L[0] : 0.f);
+                    pj.y += syc == 1u ? D.L[1] : (syc == 2u ? -D.L[1] : 0.f);
+                }
+                float dx = pj.x - bc.x, dy = pj.y - bc.y, dz = pj.z - bc.z;
+                if (!D.shiftOK) { dx = min_image(dx, D.L[0], D.invL[0]); dy = min_image(dy, D.L[1], D.invL[1]); }
+                dz = min_image(dz, D.L[2], D.invL[2]);
+                const float ax = fmaxf(0.f, fabsf(dx) - bh.x), ay = fmaxf(0.f, fabsf(dy) - bh.y);
+                const float az = fmaxf(0.f, fabsf(dz) - bh.z);
+                keep = ax * ax + ay * ay + az * az < D.rcList2;
+            }
+            const uint64_t m = __ballot(keep);
+            const int pos = cnt + __popcll(m & below);
+            if (keep) {
+                if (pos < D.alistCap) out[pos] = (uint32_t)js | (ent & 0xF0000000u);
+                else atomicOr(&P.rt[c].overflow, 1);
+            }
+            cnt += __popcll(m);
+        }
+    }
+    // pad to whole 8-atom blocks
+    const int padded = min((cnt + 7) & ~7, D.alistCap & ~7);
+    if (lane < 8 && cnt + lane < padded) out[cnt + lane] = kNoAtom;
+    if (lane == 0) P.acount[D.clusterOff + g0 + t] = padded;
+}
+
+// One wave64 per group; for each i-cluster t of the group its atom list is
+// walked in 8-atom blocks: lane = 8 jj + ii pairs i-atom (t, ii) with j-atom jj
+// of the block (gathered per lane).  Pair math as the tile kernel's FP64 path;
+// a pair counts when its j-slot is above its i-slot (half shell; inside the
+// diagonal cluster i < j).
+template <int W, bool FORCES>
+__global__ void __launch_bounds__(64 * W) k_pairs_a(DevPtrs P, int setBegin) {
+    const int c = setBegin + blockIdx.y;
+    const SetDesc D = P.desc[c];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int jj = lane >> 3, ii = lane & 7;
+    const int g = blockIdx.x * W + wave;
+    const bool active = g < P.rt[c].ngroups;
+    __shared__ float4 sPos[W][64];
+    __shared__ double4 sPrm64[W][64];
+    int g0 = 0, ncl = 0;
+    uint32_t ivm = 0;
+    if (active) {
+        const int4 gi = P.ginfo[D.groupOff + g];
+        g0 = __builtin_amdgcn_readfirstlane(gi.x);
+        ncl = __builtin_amdgcn_readfirstlane(gi.y);
+        float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
+        double4 qd = make_double4(0.0, 0.0, 0.0, 0.0);
+        bool v = false;
+        if (lane < ncl * kClusterSize) {
+            const int s = D.slotOff + g0 * kClusterSize + lane;
+            p = P.spos[s];
+            qd = P.sprm64[s];
+            v = P.sidx[s] >= 0;
+        }
+        sPos[wave][lane] = p;
+        sPrm64[wave][lane] = qd;
+        const uint64_t vb = __ballot(v);
+#pragma unroll
+        for (int t = 0; t < kGroupClusters; ++t) ivm |= (uint32_t)((vb >> (t * kClusterSize + ii)) & 1ull) << t;
+    }
+    __syncthreads();
+    if (!active) return;
+    const float Lz = D.L[2], invLz = D.invL[2];
+    const float Lx = D.L[0], invLx = D.invL[0], Ly = D.L[1], invLy = D.invL[1];
+    const bool shiftOK = D.shiftOK;
+    const double rc2 = D.rc2, krf64 = D.krf64;
+    const float rc2hi = D.rc2hi;
+    double eE = 0.0, eQQ = 0.0, eQR2 = 0.0;
+    for (int t = 0; t < ncl; ++t) {
+        const bool iv = (ivm >> t) & 1u;
+        const float4 pi = sPos[wave][t * kClusterSize + ii];
+        const double4 id = sPrm64[wave][t * kClusterSize + ii];
+        const int islot = (g0 + t) * kClusterSize + ii;
+        float fix = 0.f, fiy = 0.f, fiz = 0.f;
+        const int na = __builtin_amdgcn_readfirstlane(P.acount[D.clusterOff + g0 + t]);
+        const uint32_t* A = P.alist + D.alistOff + (long long)(g0 + t) * D.alistCap;
+        for (int b0 = 0; b0 < na; b0 += 64) {
+            const uint32_t abuf = (b0 + lane < na) ? A[b0 + lane] : kNoAtom;
+            const int nblk = min(64, na - b0) >> 3;
+            // software pipeline: the j-atom of block b + 1 is loaded while block b computes
+            uint32_t entN = (uint32_t)__shfl((int)abuf, jj, 64);
+            int jsN = entN != kNoAtom ? (int)(entN & 0x7FFFFFu) : 0;
+            float4 pjN = P.spos[D.slotOff + jsN];
+            double4 jdN = P.sprm64[D.slotOff + jsN];
+            for (int b = 0; b < nblk; ++b) {
+                const uint32_t ent = entN;
+                const bool jv = ent != kNoAtom;
+                const int js = jsN;
+                float4 pj = pjN;
+                const double4 jd = jdN;
+                if (b + 1 < nblk) {
+                    entN = (uint32_t)__shfl((int)abuf, (b + 1) * 8 + jj, 64);
+                    jsN = entN != kNoAtom ? (int)(entN & 0x7FFFFFu) : 0;
+                    pjN = P.spos[D.slotOff + jsN];
+                    jdN = P.sprm64[D.slotOff + jsN];
+                }
+                if (shiftOK) {
+                    const uint32_t sxc = (ent >> 28) & 3u, syc = (ent >> 30) & 3u;
+                    pj.x += sxc == 1u ? D.L[0] : (sxc == 2u ? -D.L[0] : 0.f);
+                    pj.y += syc == 1u ? D.L[1] : (syc == 2u ? -D.L[1] : 0.f);
+                }
+                float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+                if (!shiftOK) { dx = min_image(dx, Lx, invLx); dy = min_image(dy, Ly, invLy); }
+                dz = min_image(dz, Lz, invLz);
+                const float r2f = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                const bool cand = r2f < rc2hi && iv && jv && js > islot;
+                if (__builtin_amdgcn_ballot_w64(cand) == 0ull) continue;
+                const double r2 = fma((double)dx, (double)dx, fma((double)dy, (double)dy, (double)dz * (double)dz));
+                const bool in = cand && r2 < rc2;
+                const PairE64 pe = pair_term64(in, r2, id.x + jd.x, id.y * jd.y, id.z * jd.z, krf64);
+                eE += pe.e; eQQ += pe.qq; eQR2 += pe.qq * r2;
+                if (FORCES) {
+                    const float fs = (float)pe.fs;
+                    const float fx = fs * dx, fy = fs * dy, fz = fs * dz;
+                    fix += fx; fiy += fy; fiz += fz;
+                    const float fjx = oct_sum(-fx), fjy = oct_sum(-fy), fjz = oct_sum(-fz);
+                    if (ii == 0 && jv && (fjx != 0.f || fjy != 0.f || fjz != 0.f))
+                        flush_force(P, c, D.slotOff + js, fjx, fjy, fjz);
+                }
+            }
+        }
+        if (FORCES) {
+            const float fx = stride8_sum(fix), fy = stride8_sum(fiy), fz = stride8_sum(fiz);
+            if (jj == 0 && iv) flush_force(P, c, D.slotOff + islot, fx, fy, fz);
+        }
+    }
+    double eacc = eE + D.krf64 * eQR2 - D.crf64 * eQQ;
+    eacc = wave_sum_d(eacc);
+    if (lane == 0) P.genergy[D.groupOff + g] = eacc;
+}
+
 // ------------------------------------------------- compacted pair kernel
 
 // Order LDS accesses of the lanes of one wave (LDS executes a wave's
@@ -1122,8 +1292,25 @@ hipError_t launch_pairlist(const DevPtrs& P, int setBegin, int nsets, int maxGro
     return hipGetLastError();
 }
 
+hipError_t launch_alist(const DevPtrs& P, int setBegin, int nsets, int maxGroups, hipStream_t st) {
+    k_alist<<<dim3(maxGroups * kGroupClusters, nsets), 64, 0, st>>>(P, setBegin);
+    return hipGetLastError();
+}
+
 hipError_t launch_pairs(const DevPtrs& P, int setBegin, int nsets, int maxGroups, int waves, bool forces, bool fp64,
                         int variant, hipStream_t st) {
+    if (fp64 && variant == kPairKernelAtomLists) {
+#define ATM_PAIRSA(WV)                                                                                      \
+    if (forces) k_pairs_a<WV, true><<<dim3(cdiv(maxGroups, WV), nsets), 64 * WV, 0, st>>>(P, setBegin);    \
+    else k_pairs_a<WV, false><<<dim3(cdiv(maxGroups, WV), nsets), 64 * WV, 0, st>>>(P, setBegin);
+        switch (waves) {
+            case 1: ATM_PAIRSA(1) break;
+            case 2: ATM_PAIRSA(2) break;
+            default: ATM_PAIRSA(4) break;
+        }
+#undef ATM_PAIRSA
+        return hipGetLastError();
+    }
     if (fp64 && variant != 0) {
 #define ATM_PAIRSQ(WV, JB, NC, SD)                                                                                     \
     if (forces) k_pairs_q<WV, JB, NC, SD, true><<<dim3(cdiv(maxGroups, WV), nsets), 64 * WV, 0, st>>>(P, setBegin);  \

@@ -220,11 +220,11 @@ def _variant_systems():
 
 
 @pytest.mark.parametrize("kernel,waves", [(0, 1), (0, 2), (0, 4), (0, 8), (1, 1), (1, 2), (1, 4), (2, 1), (2, 2),
-                                          (2, 4), (3, 1), (3, 2), (4, 1), (4, 2), (4, 4)])
+                                          (2, 4), (3, 1), (3, 2), (4, 1), (4, 2), (4, 4), (5, 1), (5, 2), (5, 4)])
 def test_pair_kernel_variants(gpu, kernel, waves):
     """Every pair-kernel variant (8x8 tiles; compacted with 8 or 4 list entries
-    per batch and 1, 2 or 4 copies of the j-force sums) and block shape against
-    the oracle; energy-only evaluation agrees."""
+    per batch and 1, 2 or 4 copies of the j-force sums; atom-level j lists) and
+    block shape against the oracle; energy-only evaluation agrees."""
     systems = _variant_systems()
     with ATMBatch(systems) as b:
         b.set_tuning(1.0, waves, 4, kernel)
@@ -254,7 +254,7 @@ def test_compacted_kernel_forces_independent_of_batching(gpu):
                     assert np.array_equal(a, r)
 
 
-@pytest.mark.parametrize("kernel", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("kernel", [0, 1, 2, 3, 4, 5])
 def test_clash_side_sums_cleared_between_evaluations(gpu, kernel):
     """Pair forces beyond the fixed-point range go to FP64 side sums that are
     cleared only after an evaluation used them.  The displacement alternates
