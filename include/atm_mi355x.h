@@ -341,7 +341,8 @@ int atm_pme_get_info(void* pme, double* alpha, int32_t grid[3]);
 /* Per-particle charge (e), sigma (nm), epsilon (kJ/mol) -- NonbondedForce
  * addParticle -- and the ATM displacement d (num_atoms x 3, nm; NULL keeps
  * the current one), ATMMetaForce::addParticle / setParticleParameters
- * (ATMMetaForce.h:79-297). */
+ * (ATMMetaForce.h:79-297).  charges, sigmas and epsilons all NULL: only the
+ * displacements change (ATMMetaForce::updateParametersInContext). */
 int atm_pme_set_particles(void* pme, const double* charges, const double* sigmas, const double* epsilons,
                           const double* displacements);
 
@@ -355,12 +356,14 @@ int atm_pme_set_exceptions(void* pme, int32_t num, const int32_t* pairs, const d
 int atm_pme_set_params(void* pme, const atm_params* params);
 
 /* One evaluation: U1 at x, U2 at x + d, combine.  positions num_atoms x 3
- * (nm); box NULL keeps the last one.  forces (num_atoms x 3, may be NULL) are
- * ADDED the hybrid ATM force; result (may be NULL) as atm_get_results; parts
- * (may be NULL, 10 doubles): per state (x, then x + d) the direct-space,
+ * (nm); box NULL keeps the last one.  flags: 0, or ATM_EVAL_STATE1_ONLY for
+ * the NonbondedForce alone at x (energy = state1_energy = U1, forces F1).
+ * forces (num_atoms x 3, may be NULL) are ADDED the hybrid ATM force (F1 with
+ * ATM_EVAL_STATE1_ONLY); result (may be NULL) as atm_get_results; parts (may
+ * be NULL, 10 doubles): per state (x, then x + d) the direct-space,
  * exception, reciprocal, self and dispersion energies. */
-int atm_pme_eval(void* pme, const double* positions, const double box[3], double* forces, atm_result* result,
-                 double* parts);
+int atm_pme_eval(void* pme, const double* positions, const double box[3], int32_t flags, double* forces,
+                 atm_result* result, double* parts);
 
 /* ------------------------------------------------------------------------
  * Bonded terms of group 0 beside the ATM force (the AMBER HarmonicBondForce,

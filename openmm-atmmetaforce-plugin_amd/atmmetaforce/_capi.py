@@ -158,7 +158,7 @@ def load(path: str | None = None):
         "atm_pme_set_particles": [vp, dp, dp, dp, dp],
         "atm_pme_set_exceptions": [vp, i32, ctypes.POINTER(i32), dp, dp, dp],
         "atm_pme_set_params": [vp, ctypes.POINTER(Params)],
-        "atm_pme_eval": [vp, dp, dp, dp, ctypes.POINTER(Result), dp],
+        "atm_pme_eval": [vp, dp, dp, i32, dp, ctypes.POINTER(Result), dp],
         "atm_bonded_create": [i32, ctypes.POINTER(vp)],
         "atm_bonded_destroy": [vp],
         "atm_bonded_set": [vp, i32, i32, ctypes.POINTER(i32), dp, i32, ctypes.POINTER(i32), dp, i32,

@@ -244,3 +244,47 @@ def system_from_npz(path: str):
         arrays = {k: z[k].copy() for k in z.files if not k.startswith("prmtop/")}
     sec = {k: ([str(v) for v in a] if a.dtype.kind == "U" else a.tolist()) for k, a in sec.items()}
     return system_from_sections(sec), arrays
+
+
+def create_system(system: AmberSystem, box=None, nonbonded_cutoff: float = 1.0, ewald_tolerance: float = 5e-4,
+                  constraints: bool = True):
+    """An atmmetaforce.openmm.System holding what
+    AmberPrmtopFile.createSystem(nonbondedMethod=PME, nonbondedCutoff, constraints=HBonds)
+    holds: masses, the hydrogen-bond constraints, a PME NonbondedForce with the
+    exceptions, HarmonicBondForce, HarmonicAngleForce and PeriodicTorsionForce
+    (force group 0 each).  box: periodic box edges (nm); default the prmtop's."""
+    from . import openmm as mm
+    out = mm.System()
+    for m in system.masses:
+        out.addParticle(float(m))
+    if constraints:
+        for (i, j), d in zip(system.constraints, system.constraint_lengths):
+            out.addConstraint(int(i), int(j), float(d))
+    b = system.box if box is None else np.asarray(box, dtype=np.float64)
+    if b is None:
+        raise ValueError("no periodic box: pass box=")
+    out.setDefaultPeriodicBoxVectors((float(b[0]), 0.0, 0.0), (0.0, float(b[1]), 0.0), (0.0, 0.0, float(b[2])))
+    nb = mm.NonbondedForce()
+    nb.setNonbondedMethod(mm.NonbondedForce.PME)
+    nb.setCutoffDistance(nonbonded_cutoff)
+    nb.setEwaldErrorTolerance(ewald_tolerance)
+    nb.setUseDispersionCorrection(True)
+    for q, s, e in zip(system.charges, system.sigmas, system.epsilons):
+        nb.addParticle(float(q), float(s), float(e))
+    for (i, j), qq, s, e in zip(system.exception_pairs, system.exception_qq, system.exception_sigma,
+                                system.exception_eps):
+        nb.addException(int(i), int(j), float(qq), float(s), float(e))
+    out.addForce(nb)
+    bf = mm.HarmonicBondForce()
+    for (i, j), (r0, k) in zip(system.bonds, system.bond_params):
+        bf.addBond(int(i), int(j), float(r0), float(k))
+    out.addForce(bf)
+    af = mm.HarmonicAngleForce()
+    for (i, j, k), (t0, kk) in zip(system.angles, system.angle_params):
+        af.addAngle(int(i), int(j), int(k), float(t0), float(kk))
+    out.addForce(af)
+    tf = mm.PeriodicTorsionForce()
+    for (i, j, k, l), (n, ph, kk) in zip(system.torsions, system.torsion_params):
+        tf.addTorsion(int(i), int(j), int(k), int(l), int(n), float(ph), float(kk))
+    out.addForce(tf)
+    return out

@@ -90,8 +90,9 @@ class ATMPMEForce:
         C.check(self._lib.atm_pme_get_info(self._h, ctypes.byref(a), g))
         return a.value, tuple(g)
 
-    def evaluate(self, positions, box=None, forces=True):
-        """(result dict, hybrid force (N,3) or None, energy parts dict)."""
+    def evaluate(self, positions, box=None, forces=True, state1_only=False):
+        """(result dict, hybrid force (N,3) or None, energy parts dict);
+        state1_only: the NonbondedForce alone at x (energy = U1, force F1)."""
         x = np.ascontiguousarray(positions, dtype=np.float64).reshape(self.n, 3)
         b = np.ascontiguousarray(box if box is not None else self.box, dtype=np.float64)
         r = C.Result()
@@ -99,8 +100,8 @@ class ATMPMEForce:
         F = None
         if forces:
             F = np.zeros((self.n, 3))
-        C.check(self._lib.atm_pme_eval(self._h, _dp(x), _dp(b), _dp(F) if forces else None, ctypes.byref(r),
-                                       _dp(parts)))
+        C.check(self._lib.atm_pme_eval(self._h, _dp(x), _dp(b), C.ATM_EVAL_STATE1_ONLY if state1_only else 0,
+                                       _dp(F) if forces else None, ctypes.byref(r), _dp(parts)))
         names = ("direct", "exceptions", "reciprocal", "self", "dispersion")
         p = {f"{k}{s + 1}": parts[5 * s + i] for s in range(2) for i, k in enumerate(names)}
         return r.as_dict(), F, p

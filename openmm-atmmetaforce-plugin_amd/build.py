@@ -67,7 +67,7 @@ CXX = os.environ.get("CXX", "g++")
 CXX_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-parameter", "-fvisibility=hidden"]
 CORE_INC = ["-I" + os.path.join(PKG, "openmm_min", "include")]
 ATM_INC = CORE_INC + ["-I" + os.path.join(PKG, d) for d in
-                      ("openmmapi/include", "serialization/include", "platforms/mi355x/include")]
+                      ("openmmapi/include", "serialization/include", "platforms/mi355x/include")] + ["-I" + INCLUDE]
 
 
 def _sources(*dirs):

@@ -74,8 +74,8 @@ struct BondedPtrs {
 
 hipError_t launch_pme_fill(unsigned long long* p, long long n, hipStream_t st);
 hipError_t launch_pme_state(const PmeDesc& D, const PmePtrs& P, int state, hipStream_t st);
-hipError_t launch_pme_finish(const PmeDesc& D, const PmePtrs& P, const AtmParamsDev& prm, double* forcesOut,
-                             hipStream_t st);
+hipError_t launch_pme_finish(const PmeDesc& D, const PmePtrs& P, const AtmParamsDev& prm, bool single,
+                             double* forcesOut, hipStream_t st);
 hipError_t launch_bonded(const BondedPtrs& B, double* eslot, int nblocks, hipStream_t st);
 
 }  // namespace atm

@@ -469,10 +469,11 @@ __global__ void __launch_bounds__(256) k_pme_exceptions(PmeDesc D, PmePtrs P, co
 
 // Fixed-order sums of the energy slots of both states; U = sum + constants;
 // combine.  parts[s][0..4]: direct, exceptions, reciprocal, self, dispersion.
-__global__ void __launch_bounds__(1024) k_pme_finish(PmeDesc D, PmePtrs P, AtmParamsDev prm) {
+// single: state 0 only (W = U1, hybrid weight 0).
+__global__ void __launch_bounds__(1024) k_pme_finish(PmeDesc D, PmePtrs P, AtmParamsDev prm, int single) {
     __shared__ double red[16];
-    double U[2];
-    for (int s = 0; s < 2; ++s) {
+    double U[2] = {0.0, 0.0};
+    for (int s = 0; s < (single ? 1 : 2); ++s) {
         const double* sl = P.eslot + (long long)s * D.nslot;
         const int ranges[4] = {0, D.nslotDirect, D.nslotDirect + D.nslotEx, D.nslot};
         double parts[3];
@@ -492,18 +493,22 @@ __global__ void __launch_bounds__(1024) k_pme_finish(PmeDesc D, PmePtrs P, AtmPa
     }
     if (threadIdx.x == 0) {
         SysResult r;
-        atm_combine(prm, U[0], U[1], r);
+        if (single) {
+            r.W = U[0]; r.usc = 0.0; r.U1 = U[0]; r.U2 = 0.0; r.u = 0.0; r.fp = 1.0; r.bfp = 0.0; r.sp = 0.0;
+        } else {
+            atm_combine(prm, U[0], U[1], r);
+        }
         *P.result = r;
     }
 }
 
-__global__ void k_pme_blend(PmeDesc D, PmePtrs P, double* out) {
+__global__ void k_pme_blend(PmeDesc D, PmePtrs P, int single, double* out) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= 3 * D.n) return;
     const SysResult r = *P.result;
-    const double a = D.direction > 0.0 ? r.sp : 1.0 - r.sp;  // weight of state 2 (x + d)
+    const double a = single ? 0.0 : (D.direction > 0.0 ? r.sp : 1.0 - r.sp);  // weight of state 2 (x + d)
     const double f1 = (double)(long long)P.F[k] * (1.0 / kFix);
-    const double f2 = (double)(long long)P.F[3ll * D.n + k] * (1.0 / kFix);
+    const double f2 = single ? 0.0 : (double)(long long)P.F[3ll * D.n + k] * (1.0 / kFix);
     const int comp = k / D.n, i = k % D.n;
     out[3 * i + comp] = (1.0 - a) * f1 + a * f2;
 }
@@ -555,10 +560,10 @@ hipError_t launch_pme_state(const PmeDesc& D, const PmePtrs& P, int state, hipSt
     return hipGetLastError();
 }
 
-hipError_t launch_pme_finish(const PmeDesc& D, const PmePtrs& P, const AtmParamsDev& prm, double* forcesOut,
-                             hipStream_t st) {
-    k_pme_finish<<<1, 1024, 0, st>>>(D, P, prm);
-    if (forcesOut) k_pme_blend<<<blocks_of(3ll * D.n, 256), 256, 0, st>>>(D, P, forcesOut);
+hipError_t launch_pme_finish(const PmeDesc& D, const PmePtrs& P, const AtmParamsDev& prm, bool single,
+                             double* forcesOut, hipStream_t st) {
+    k_pme_finish<<<1, 1024, 0, st>>>(D, P, prm, single ? 1 : 0);
+    if (forcesOut) k_pme_blend<<<blocks_of(3ll * D.n, 256), 256, 0, st>>>(D, P, single ? 1 : 0, forcesOut);
     return hipGetLastError();
 }
 

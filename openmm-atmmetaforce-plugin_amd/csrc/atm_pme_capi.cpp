@@ -395,6 +395,13 @@ int atm_pme_set_particles(void* h, const double* charges, const double* sigmas, 
                           const double* displacements) {
     return guarded([&] {
         PmeForce& S = pme_of(h);
+        if (!charges && !sigmas && !epsilons) {  // displacements only
+            if (!displacements) fail(ATM_ERR_INVALID_ARGUMENT, "nothing to set");
+            if (!S.haveParticles) fail(ATM_ERR_INVALID_ARGUMENT, "particles not set");
+            S.displ.assign(displacements, displacements + 3 * (size_t)S.n);
+            S.setupDone = false;
+            return;
+        }
         if (!charges || !sigmas || !epsilons) fail(ATM_ERR_INVALID_ARGUMENT, "null particle array");
         for (int i = 0; i < S.n; ++i) {
             if (!std::isfinite(charges[i]) || !(sigmas[i] >= 0.0) || !(epsilons[i] >= 0.0) ||
@@ -449,11 +456,12 @@ int atm_pme_set_params(void* h, const atm_params* p) {
     });
 }
 
-int atm_pme_eval(void* h, const double* positions, const double box[3], double* forces, atm_result* result,
-                 double* parts) {
+int atm_pme_eval(void* h, const double* positions, const double box[3], int32_t flags, double* forces,
+                 atm_result* result, double* parts) {
     return guarded([&] {
         PmeForce& S = pme_of(h);
         if (!positions) fail(ATM_ERR_INVALID_ARGUMENT, "null positions");
+        if (flags & ~ATM_EVAL_STATE1_ONLY) fail(ATM_ERR_INVALID_ARGUMENT, "unknown flags");
         if (!S.haveParticles) fail(ATM_ERR_INVALID_ARGUMENT, "particles not set");
         for (size_t k = 0; k < 3 * (size_t)S.n; ++k)
             if (!std::isfinite(positions[k])) fail(ATM_ERR_INVALID_ARGUMENT, "non-finite position");
@@ -466,9 +474,10 @@ int atm_pme_eval(void* h, const double* positions, const double box[3], double* 
         for (int attempt = 0; attempt < 2; ++attempt) {
             set_box(S, box ? box : S.box);
             HIPCHK(launch_pme_fill(reinterpret_cast<unsigned long long*>(S.dOverflow.p), 1, st));
+            const bool single = (flags & ATM_EVAL_STATE1_ONLY) != 0;
             HIPCHK(launch_pme_state(S.D, S.P, 0, st));
-            HIPCHK(launch_pme_state(S.D, S.P, 1, st));
-            HIPCHK(launch_pme_finish(S.D, S.P, prm, forces ? S.dOut.as<double>() : nullptr, st));
+            if (!single) HIPCHK(launch_pme_state(S.D, S.P, 1, st));
+            HIPCHK(launch_pme_finish(S.D, S.P, prm, single, forces ? S.dOut.as<double>() : nullptr, st));
             int ovf[2] = {0, 0};
             HIPCHK(hipMemcpyAsync(ovf, S.dOverflow.p, sizeof(int) * 2, hipMemcpyDeviceToHost, st));
             HIPCHK(hipStreamSynchronize(st));

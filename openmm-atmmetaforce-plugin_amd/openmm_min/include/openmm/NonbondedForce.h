@@ -1,9 +1,11 @@
 // Minimal OpenMM-compatible core: the subset of NonbondedForce the ATM
 // evaluation supports on the MI355X platform -- per-particle charge, sigma and
-// epsilon, CutoffPeriodic with reaction-field electrostatics.  Exceptions are
-// stored (so a System round-trips) but the MI355X kernels reject nonzero ones;
-// the dispersion correction is evaluated on the host.
+// epsilon with CutoffPeriodic (reaction-field) electrostatics, or PME with
+// exceptions (scaled 1-4 pairs, exclusions) as AMBER systems use it.  The
+// reaction-field kernels reject exceptions; the dispersion correction is
+// included in either method.
 #pragma once
+#include <unordered_map>
 #include <vector>
 #include "openmm/Force.h"
 #include "openmm/internal/windowsExport.h"
@@ -24,6 +26,11 @@ public:
     void setReactionFieldDielectric(double d);
     bool getUseDispersionCorrection() const { return useDispersionCorrection; }
     void setUseDispersionCorrection(bool use) { useDispersionCorrection = use; }
+    double getEwaldErrorTolerance() const { return ewaldTolerance; }
+    void setEwaldErrorTolerance(double tol);
+    // alpha = 0 (the default): chosen from the Ewald error tolerance, as is the grid
+    void getPMEParameters(double& alpha, int& nx, int& ny, int& nz) const;
+    void setPMEParameters(double alpha, int nx, int ny, int nz);
     int getReciprocalSpaceForceGroup() const { return recipGroup; }
     void setReciprocalSpaceForceGroup(int group);
     int addParticle(double charge, double sigma, double epsilon);
@@ -32,6 +39,8 @@ public:
     int addException(int particle1, int particle2, double chargeProd, double sigma, double epsilon, bool replace = false);
     void getExceptionParameters(int index, int& particle1, int& particle2, double& chargeProd, double& sigma,
                                 double& epsilon) const;
+    void setExceptionParameters(int index, int particle1, int particle2, double chargeProd, double sigma,
+                                double epsilon);
     void updateParametersInContext(Context& context);
     bool usesPeriodicBoundaryConditions() const override {
         return method == CutoffPeriodic || method == Ewald || method == PME || method == LJPME;
@@ -50,11 +59,13 @@ private:
         double qq, sigma, eps;
     };
     NonbondedMethod method;
-    double cutoff, rfDielectric;
+    double cutoff, rfDielectric, ewaldTolerance, pmeAlpha;
+    int pmeGrid[3];
     bool useDispersionCorrection;
     int recipGroup;
     std::vector<Particle> particles;
     std::vector<Exception> exceptions;
+    std::unordered_map<unsigned long long, int> exceptionIndex;  // (min, max) particle pair -> exception
 };
 
 }  // namespace OpenMM

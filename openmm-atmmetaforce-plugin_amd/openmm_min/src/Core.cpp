@@ -1,5 +1,6 @@
 // Minimal OpenMM-compatible core: Force, System, NonbondedForce, State,
 // Integrator, Context and ContextImpl.
+#include <algorithm>
 #include <cmath>
 #include <random>
 #include <cstdlib>
@@ -189,7 +190,29 @@ void MonteCarloBarostat::setFrequency(int freq) {
 ForceImpl* MonteCarloBarostat::createImpl() const { return new MonteCarloBarostatImpl(*this); }
 
 NonbondedForce::NonbondedForce()
-    : method(NoCutoff), cutoff(1.0), rfDielectric(78.3), useDispersionCorrection(true), recipGroup(-1) {}
+    : method(NoCutoff), cutoff(1.0), rfDielectric(78.3), ewaldTolerance(5e-4), pmeAlpha(0.0), pmeGrid{0, 0, 0},
+      useDispersionCorrection(true), recipGroup(-1) {}
+
+void NonbondedForce::setEwaldErrorTolerance(double tol) { ewaldTolerance = tol; }
+
+void NonbondedForce::getPMEParameters(double& alpha, int& nx, int& ny, int& nz) const {
+    alpha = pmeAlpha;
+    nx = pmeGrid[0];
+    ny = pmeGrid[1];
+    nz = pmeGrid[2];
+}
+
+void NonbondedForce::setPMEParameters(double alpha, int nx, int ny, int nz) {
+    pmeAlpha = alpha;
+    pmeGrid[0] = nx;
+    pmeGrid[1] = ny;
+    pmeGrid[2] = nz;
+}
+
+static unsigned long long pairKey(int a, int b) {
+    const unsigned lo = (unsigned)std::min(a, b), hi = (unsigned)std::max(a, b);
+    return ((unsigned long long)lo << 32) | hi;
+}
 
 void NonbondedForce::setNonbondedMethod(NonbondedMethod m) {
     if (m < NoCutoff || m > LJPME) throw OpenMMException("NonbondedForce: Illegal value for nonbonded method");
@@ -222,16 +245,28 @@ void NonbondedForce::setParticleParameters(int index, double charge, double sigm
 }
 
 int NonbondedForce::addException(int p1, int p2, double qq, double sigma, double eps, bool replace) {
-    for (size_t k = 0; k < exceptions.size(); ++k) {
-        Exception& e = exceptions[k];
-        if ((e.p1 == p1 && e.p2 == p2) || (e.p1 == p2 && e.p2 == p1)) {
-            if (!replace) throw OpenMMException("NonbondedForce: There is already an exception for those particles");
-            e = Exception{p1, p2, qq, sigma, eps};
-            return (int)k;
-        }
+    const unsigned long long key = pairKey(p1, p2);
+    auto it = exceptionIndex.find(key);
+    if (it != exceptionIndex.end()) {
+        if (!replace) throw OpenMMException("NonbondedForce: There is already an exception for those particles");
+        exceptions[it->second] = Exception{p1, p2, qq, sigma, eps};
+        return it->second;
     }
     exceptions.push_back(Exception{p1, p2, qq, sigma, eps});
+    exceptionIndex[key] = (int)exceptions.size() - 1;
     return (int)exceptions.size() - 1;
+}
+
+void NonbondedForce::setExceptionParameters(int index, int p1, int p2, double qq, double sigma, double eps) {
+    if (index < 0 || index >= (int)exceptions.size()) throw OpenMMException("Index out of range");
+    Exception& e = exceptions[index];
+    if (pairKey(e.p1, e.p2) != pairKey(p1, p2)) {
+        if (exceptionIndex.count(pairKey(p1, p2)))
+            throw OpenMMException("NonbondedForce: There is already an exception for those particles");
+        exceptionIndex.erase(pairKey(e.p1, e.p2));
+        exceptionIndex[pairKey(p1, p2)] = index;
+    }
+    e = Exception{p1, p2, qq, sigma, eps};
 }
 
 void NonbondedForce::getExceptionParameters(int index, int& p1, int& p2, double& qq, double& sigma, double& eps) const {
@@ -279,6 +314,14 @@ const std::map<std::string, double>& State::getParameters() const {
 
 // ------------------------------------------------------------------ Integrator
 
+// The integrators here do not apply constraints: a System that holds any is
+// refused rather than integrated with unconstrained bonds.
+static void requireNoConstraints(const System& sys) {
+    if (sys.getNumConstraints() > 0)
+        throw OpenMMException("This integrator does not apply constraints; the System has " +
+                              std::to_string(sys.getNumConstraints()) + " (build it without constraints to integrate)");
+}
+
 void Integrator::setIntegrationForceGroups(const std::set<int>& s) {
     int mask = 0;
     for (int g : s) {
@@ -292,6 +335,7 @@ void VerletIntegrator::step(int steps) {
     if (!context) throw OpenMMException("This Integrator is not bound to a context!");
     ContextImpl& c = *context;
     const System& sys = c.getSystem();
+    requireNoConstraints(sys);
     const int n = sys.getNumParticles();
     std::vector<Vec3>& x = c.positionsRef();
     std::vector<Vec3>& v = c.velocitiesRef();
@@ -331,6 +375,7 @@ void LangevinIntegrator::step(int steps) {
     if (!context) throw OpenMMException("This Integrator is not bound to a context!");
     ContextImpl& c = *context;
     const System& sys = c.getSystem();
+    requireNoConstraints(sys);
     const int n = sys.getNumParticles();
     std::vector<Vec3>& x = c.positionsRef();
     std::vector<Vec3>& v = c.velocitiesRef();
@@ -355,6 +400,7 @@ void LangevinMiddleIntegrator::step(int steps) {
     if (!context) throw OpenMMException("This Integrator is not bound to a context!");
     ContextImpl& c = *context;
     const System& sys = c.getSystem();
+    requireNoConstraints(sys);
     const int n = sys.getNumParticles();
     std::vector<Vec3>& x = c.positionsRef();
     std::vector<Vec3>& v = c.velocitiesRef();

@@ -7,6 +7,7 @@
 #include <vector>
 #include "ATMMetaForceKernels.h"
 #include "ATMRestraintKernels.h"
+#include "atm_mi355x.h"
 #include "openmm/NonbondedForce.h"
 #include "openmm/Vec3.h"
 #include "openmm/kernels.h"
@@ -28,9 +29,19 @@ public:
     void* handle() const { return ctx; }
     int numAtoms() const { return n; }
     double dispersionEnergy(const OpenMM::Vec3 box[3]) const;
+    // PME NonbondedForce (atm_pme_*): energies include the dispersion correction
+    bool isPME() const { return pme != nullptr; }
+    // One PME evaluation at the context's positions and box: with params, the ATM
+    // evaluation (U1, U2, combine, hybrid force); without, the force alone at x.
+    // Forces are added to the context's force buffer when includeForces.
+    void evaluatePME(OpenMM::ContextImpl& context, const atm_params* params, bool includeForces, atm_result* r);
 
 private:
+    void createPME(int device, const OpenMM::System& system, const OpenMM::NonbondedForce& nb,
+                   const std::vector<OpenMM::Vec3>* displacements);
+    void setPMEParticles(const OpenMM::NonbondedForce& nb, const std::vector<OpenMM::Vec3>* displacements);
     void* ctx = nullptr;
+    void* pme = nullptr;
     int n = 0;
     double box[3] = {0, 0, 0};
     double cutoff = 0, rfDielectric = 0;
@@ -73,6 +84,31 @@ private:
     OpenMM::ContextImpl& context;
     MI355XSystemContext sys;
 };
+
+// Bonded kernels: one libatm_mi355x bonded set each (atm_bonded_*) holding the
+// terms of one HarmonicBondForce, HarmonicAngleForce or PeriodicTorsionForce.
+template <class F, class Base>
+class MI355XCalcBondedKernel : public Base {
+public:
+    MI355XCalcBondedKernel(std::string name, const OpenMM::Platform& platform, OpenMM::ContextImpl& context)
+        : Base(name, platform), context(context) {}
+    ~MI355XCalcBondedKernel() override;
+    void initialize(const OpenMM::System& system, const F& force) override;
+    double execute(OpenMM::ContextImpl& context, bool includeForces, bool includeEnergy) override;
+    void copyParametersToContext(OpenMM::ContextImpl& context, const F& force) override;
+
+private:
+    void upload(const F& force);
+    OpenMM::ContextImpl& context;
+    void* set = nullptr;
+    int n = 0;
+};
+using MI355XCalcHarmonicBondForceKernel =
+    MI355XCalcBondedKernel<OpenMM::HarmonicBondForce, OpenMM::CalcHarmonicBondForceKernel>;
+using MI355XCalcHarmonicAngleForceKernel =
+    MI355XCalcBondedKernel<OpenMM::HarmonicAngleForce, OpenMM::CalcHarmonicAngleForceKernel>;
+using MI355XCalcPeriodicTorsionForceKernel =
+    MI355XCalcBondedKernel<OpenMM::PeriodicTorsionForce, OpenMM::CalcPeriodicTorsionForceKernel>;
 
 // Restraint kernels: one libatm_mi355x restraint set each (atm_restraints_*).
 class MI355XCalcCentroidRestraintForceKernel : public CalcCentroidRestraintForceKernel {

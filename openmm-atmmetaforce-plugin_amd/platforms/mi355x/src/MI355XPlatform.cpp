@@ -10,8 +10,11 @@
 #include "ATMMetaForceKernels.h"
 #include "MI355XATMMetaForceKernels.h"
 #include "MI355XPlatform.h"
+#include "openmm/HarmonicAngleForce.h"
+#include "openmm/HarmonicBondForce.h"
 #include "openmm/KernelFactory.h"
 #include "openmm/NonbondedForce.h"
+#include "openmm/PeriodicTorsionForce.h"
 #include "openmm/OpenMMException.h"
 #include "openmm/internal/ContextImpl.h"
 
@@ -44,16 +47,32 @@ const NonbondedForce* variableNonbonded(const System& system, int mask, const Fo
 }
 
 void validateNonbonded(const NonbondedForce& nb) {
+    if (nb.getNonbondedMethod() == NonbondedForce::PME) {
+        if (nb.getReciprocalSpaceForceGroup() != -1 && nb.getReciprocalSpaceForceGroup() != nb.getForceGroup())
+            throw OpenMMException("MI355X platform: the PME reciprocal-space part must stay in the NonbondedForce's group");
+        return;
+    }
     if (nb.getNonbondedMethod() != NonbondedForce::CutoffPeriodic)
-        throw OpenMMException("MI355X platform: NonbondedForce must use CutoffPeriodic (reaction-field) electrostatics");
+        throw OpenMMException("MI355X platform: NonbondedForce must use CutoffPeriodic (reaction-field) or PME electrostatics");
     if (nb.getNumExceptions() > 0)
-        throw OpenMMException("MI355X platform: NonbondedForce exceptions are not supported");
+        throw OpenMMException("MI355X platform: NonbondedForce exceptions are supported with PME only");
 }
 
 class NonbondedFactory : public KernelFactory {
 public:
     KernelImpl* createKernelImpl(std::string name, const Platform& platform, ContextImpl& context) const override {
         if (name == CalcNonbondedForceKernel::Name()) return new MI355XCalcNonbondedForceKernel(name, platform, context);
+        throw OpenMMException("Tried to create kernel with illegal kernel name '" + name + "'");
+    }
+};
+
+class BondedFactory : public KernelFactory {
+public:
+    KernelImpl* createKernelImpl(std::string name, const Platform& platform, ContextImpl& context) const override {
+        if (name == CalcHarmonicBondForceKernel::Name()) return new MI355XCalcHarmonicBondForceKernel(name, platform, context);
+        if (name == CalcHarmonicAngleForceKernel::Name()) return new MI355XCalcHarmonicAngleForceKernel(name, platform, context);
+        if (name == CalcPeriodicTorsionForceKernel::Name())
+            return new MI355XCalcPeriodicTorsionForceKernel(name, platform, context);
         throw OpenMMException("Tried to create kernel with illegal kernel name '" + name + "'");
     }
 };
@@ -87,6 +106,9 @@ MI355XPlatform::MI355XPlatform() {
     setPropertyDefaultValue("Precision", "mixed");
     setPropertyDefaultValue("ATMEvaluation", "fused");
     registerKernelFactory(CalcNonbondedForceKernel::Name(), new NonbondedFactory());
+    registerKernelFactory(CalcHarmonicBondForceKernel::Name(), new BondedFactory());
+    registerKernelFactory(CalcHarmonicAngleForceKernel::Name(), new BondedFactory());
+    registerKernelFactory(CalcPeriodicTorsionForceKernel::Name(), new BondedFactory());
 }
 
 const std::string& MI355XPlatform::getPropertyValue(const Context& context, const std::string& property) const {
@@ -134,11 +156,16 @@ MI355XPlatform::PlatformData& MI355XPlatform::data(ContextImpl& context) {
 
 MI355XSystemContext::~MI355XSystemContext() {
     if (ctx) atm_destroy(ctx);
+    if (pme) atm_pme_destroy(pme);
 }
 
 void MI355XSystemContext::create(int device, const System& system, const NonbondedForce& nb,
                                  const std::vector<Vec3>* displacements) {
     validateNonbonded(nb);
+    if (nb.getNonbondedMethod() == NonbondedForce::PME) {
+        createPME(device, system, nb, displacements);
+        return;
+    }
     n = system.getNumParticles();
     if (nb.getNumParticles() != n)
         throw OpenMMException("NonbondedForce must have exactly as many particles as the System it belongs to.");
@@ -160,7 +187,69 @@ void MI355XSystemContext::create(int device, const System& system, const Nonbond
     check(atm_set_positions(ctx, 0, zeros.data()), "atm_set_positions");
 }
 
+// PME: the NonbondedForce's particles, exceptions and PME parameters on an
+// atm_pme object (positions and box are passed at every evaluation).
+void MI355XSystemContext::createPME(int device, const System& system, const NonbondedForce& nb,
+                                    const std::vector<Vec3>* displacements) {
+    n = system.getNumParticles();
+    if (nb.getNumParticles() != n)
+        throw OpenMMException("NonbondedForce must have exactly as many particles as the System it belongs to.");
+    Vec3 a, b, c;
+    system.getDefaultPeriodicBoxVectors(a, b, c);
+    atm_pme_desc d{};
+    d.num_atoms = n;
+    double alpha;
+    int nx, ny, nz;
+    nb.getPMEParameters(alpha, nx, ny, nz);
+    d.grid[0] = alpha > 0.0 ? nx : 0;
+    d.grid[1] = alpha > 0.0 ? ny : 0;
+    d.grid[2] = alpha > 0.0 ? nz : 0;
+    d.box[0] = a[0];
+    d.box[1] = b[1];
+    d.box[2] = c[2];
+    d.cutoff = nb.getCutoffDistance();
+    d.ewald_tolerance = nb.getEwaldErrorTolerance();
+    d.alpha = alpha > 0.0 ? alpha : 0.0;
+    d.dispersion_correction = nb.getUseDispersionCorrection() ? 1 : 0;
+    cutoff = d.cutoff;
+    for (int k = 0; k < 3; ++k) box[k] = d.box[k];
+    check(atm_pme_create(device, &d, &pme), "atm_pme_create");
+    setPMEParticles(nb, displacements);
+}
+
+void MI355XSystemContext::setPMEParticles(const NonbondedForce& nb, const std::vector<Vec3>* displacements) {
+    std::vector<double> q(n), s(n), e(n);
+    for (int i = 0; i < n; ++i) nb.getParticleParameters(i, q[i], s[i], e[i]);
+    const double* dptr = displacements ? reinterpret_cast<const double*>(displacements->data()) : nullptr;
+    check(atm_pme_set_particles(pme, q.data(), s.data(), e.data(), dptr), "atm_pme_set_particles");
+    const int ne = nb.getNumExceptions();
+    std::vector<int32_t> pairs(2 * (size_t)ne);
+    std::vector<double> qq(ne), es(ne), ee(ne);
+    for (int k = 0; k < ne; ++k) {
+        int p1, p2;
+        nb.getExceptionParameters(k, p1, p2, qq[k], es[k], ee[k]);
+        pairs[2 * k] = p1;
+        pairs[2 * k + 1] = p2;
+    }
+    check(atm_pme_set_exceptions(pme, ne, pairs.data(), qq.data(), es.data(), ee.data()), "atm_pme_set_exceptions");
+}
+
+void MI355XSystemContext::evaluatePME(ContextImpl& context, const atm_params* params, bool includeForces, atm_result* r) {
+    Vec3 a, b, c;
+    context.getPeriodicBoxVectors(a, b, c);
+    const double L[3] = {a[0], b[1], c[2]};
+    if (params) check(atm_pme_set_params(pme, params), "atm_pme_set_params");
+    double* f = includeForces ? reinterpret_cast<double*>(context.forcesRef().data()) : nullptr;
+    check(atm_pme_eval(pme, reinterpret_cast<const double*>(context.positionsRef().data()), L,
+                       params ? 0 : ATM_EVAL_STATE1_ONLY, f, r, nullptr),
+          "atm_pme_eval");
+}
+
 void MI355XSystemContext::setParticles(const NonbondedForce& nb, const std::vector<Vec3>* displacements) {
+    if (pme) {
+        setPMEParticles(nb, displacements);
+        return;
+    }
     std::vector<double> q(n), s(n), e(n);
     for (int i = 0; i < n; ++i) nb.getParticleParameters(i, q[i], s[i], e[i]);
     const double* dptr = displacements ? reinterpret_cast<const double*>(displacements->data()) : nullptr;
@@ -189,6 +278,11 @@ void MI355XSystemContext::setParticles(const NonbondedForce& nb, const std::vect
 }
 
 void MI355XSystemContext::setDisplacements(const std::vector<Vec3>& displacements) {
+    if (pme) {  // displacements only: the particle parameters of the last upload stay
+        check(atm_pme_set_particles(pme, nullptr, nullptr, nullptr, reinterpret_cast<const double*>(displacements.data())),
+              "atm_pme_set_particles");
+        return;
+    }
     check(atm_set_displacements(ctx, 0, reinterpret_cast<const double*>(displacements.data())), "atm_set_displacements");
 }
 
@@ -235,6 +329,11 @@ void MI355XCalcNonbondedForceKernel::initialize(const System& system, const Nonb
 }
 
 double MI355XCalcNonbondedForceKernel::execute(ContextImpl& ctx, bool includeForces, bool includeEnergy) {
+    if (sys.isPME()) {
+        atm_result r;
+        sys.evaluatePME(ctx, nullptr, includeForces, &r);
+        return includeEnergy ? r.energy : 0.0;
+    }
     sys.syncState(ctx);
     const MI355XPlatform::PlatformData& pd = MI355XPlatform::data(ctx);
     int flags = ATM_EVAL_STATE1_ONLY | ATM_EVAL_CLEAR_FORCES | (includeForces ? ATM_EVAL_FORCES : 0) |
@@ -310,6 +409,14 @@ double MI355XCalcATMMetaForceKernel::executeFused(ContextImpl& ctx, bool include
         if (p.alpha > 0.0) w += (p.lambda2 - p.lambda1) / p.alpha * (x > 0.0 ? x + std::log1p(std::exp(-x)) : std::log1p(std::exp(x)));
         PerturbationEnergy = 0.0;
         return includeEnergy ? w : 0.0;
+    }
+    if (sys.isPME()) {
+        atm_result r;
+        sys.evaluatePME(ctx, &p, includeForces, &r);  // dispersion correction inside U1 and U2
+        U1 = r.state1_energy;
+        U2 = r.state2_energy;
+        PerturbationEnergy = r.perturbation_energy;
+        return includeEnergy ? r.energy : 0.0;
     }
     sys.syncState(ctx);
     check(atm_set_params(sys.handle(), 0, &p), "atm_set_params");
@@ -462,6 +569,89 @@ extern "C" __attribute__((visibility("default"))) void registerPlatforms() {
 
 // Registers the ATMMetaForce kernel factory with the MI355X platform; errors are
 // ignored, as the reference's factories do (HipATMMetaForceKernelFactory.cpp:18-27).
+// ------------------------------------------------------------------ bonded kernels
+
+namespace {
+
+// Terms of one bonded force as the flat arrays of atm_bonded_set.
+struct BondedTerms {
+    std::vector<int32_t> bonds, angles, torsions;
+    std::vector<double> bprm, aprm, tprm;
+};
+
+void termsOf(const HarmonicBondForce& f, BondedTerms& t) {
+    for (int k = 0; k < f.getNumBonds(); ++k) {
+        int p1, p2;
+        double r0, kf;
+        f.getBondParameters(k, p1, p2, r0, kf);
+        t.bonds.insert(t.bonds.end(), {p1, p2});
+        t.bprm.insert(t.bprm.end(), {r0, kf});
+    }
+}
+
+void termsOf(const HarmonicAngleForce& f, BondedTerms& t) {
+    for (int k = 0; k < f.getNumAngles(); ++k) {
+        int p1, p2, p3;
+        double a0, kf;
+        f.getAngleParameters(k, p1, p2, p3, a0, kf);
+        t.angles.insert(t.angles.end(), {p1, p2, p3});
+        t.aprm.insert(t.aprm.end(), {a0, kf});
+    }
+}
+
+void termsOf(const PeriodicTorsionForce& f, BondedTerms& t) {
+    for (int k = 0; k < f.getNumTorsions(); ++k) {
+        int p1, p2, p3, p4, per;
+        double ph, kf;
+        f.getTorsionParameters(k, p1, p2, p3, p4, per, ph, kf);
+        t.torsions.insert(t.torsions.end(), {p1, p2, p3, p4});
+        t.tprm.insert(t.tprm.end(), {(double)per, ph, kf});
+    }
+}
+
+}  // namespace
+
+template <class F, class Base>
+MI355XCalcBondedKernel<F, Base>::~MI355XCalcBondedKernel() {
+    if (set) atm_bonded_destroy(set);
+}
+
+template <class F, class Base>
+void MI355XCalcBondedKernel<F, Base>::upload(const F& force) {
+    BondedTerms t;
+    termsOf(force, t);
+    check(atm_bonded_set(set, n, (int32_t)t.bonds.size() / 2, t.bonds.data(), t.bprm.data(), (int32_t)t.angles.size() / 3,
+                         t.angles.data(), t.aprm.data(), (int32_t)t.torsions.size() / 4, t.torsions.data(),
+                         t.tprm.data()),
+          "atm_bonded_set");
+}
+
+template <class F, class Base>
+void MI355XCalcBondedKernel<F, Base>::initialize(const System& system, const F& force) {
+    n = system.getNumParticles();
+    check(atm_bonded_create(MI355XPlatform::data(context).device, &set), "atm_bonded_create");
+    upload(force);
+}
+
+template <class F, class Base>
+double MI355XCalcBondedKernel<F, Base>::execute(ContextImpl& ctx, bool includeForces, bool includeEnergy) {
+    double e[3];
+    check(atm_bonded_eval(set, reinterpret_cast<const double*>(ctx.positionsRef().data()),
+                          includeForces ? reinterpret_cast<double*>(ctx.forcesRef().data()) : nullptr, e),
+          "atm_bonded_eval");
+    return includeEnergy ? e[0] + e[1] + e[2] : 0.0;
+}
+
+template <class F, class Base>
+void MI355XCalcBondedKernel<F, Base>::copyParametersToContext(ContextImpl& ctx, const F& force) {
+    (void)ctx;
+    upload(force);
+}
+
+template class MI355XCalcBondedKernel<HarmonicBondForce, CalcHarmonicBondForceKernel>;
+template class MI355XCalcBondedKernel<HarmonicAngleForce, CalcHarmonicAngleForceKernel>;
+template class MI355XCalcBondedKernel<PeriodicTorsionForce, CalcPeriodicTorsionForceKernel>;
+
 extern "C" __attribute__((visibility("default"))) void registerKernelFactories() {
     try {
         Platform& platform = Platform::getPlatformByName("MI355X");

@@ -19,10 +19,13 @@
 #include "ATMMetaForceVersion.h"
 #include "internal/ATMMetaForceImpl.h"
 #include "openmm/Context.h"
+#include "openmm/HarmonicAngleForce.h"
+#include "openmm/HarmonicBondForce.h"
 #include "openmm/Integrator.h"
 #include "openmm/MonteCarloBarostat.h"
 #include "openmm/NonbondedForce.h"
 #include "openmm/OpenMMException.h"
+#include "openmm/PeriodicTorsionForce.h"
 #include "openmm/Platform.h"
 #include "openmm/State.h"
 #include "openmm/System.h"
@@ -149,6 +152,22 @@ PYBIND11_MODULE(_atmmetaforce, m) {
         .def("setUseDispersionCorrection", &NonbondedForce::setUseDispersionCorrection)
         .def("getReciprocalSpaceForceGroup", &NonbondedForce::getReciprocalSpaceForceGroup)
         .def("setReciprocalSpaceForceGroup", &NonbondedForce::setReciprocalSpaceForceGroup)
+        .def("getExceptionParameters", [](const NonbondedForce& f, int i) {
+            int p1, p2;
+            double qq, s, e;
+            f.getExceptionParameters(i, p1, p2, qq, s, e);
+            return py::make_tuple(p1, p2, qq, s, e);
+        })
+        .def("setExceptionParameters", &NonbondedForce::setExceptionParameters)
+        .def("getEwaldErrorTolerance", &NonbondedForce::getEwaldErrorTolerance)
+        .def("setEwaldErrorTolerance", &NonbondedForce::setEwaldErrorTolerance)
+        .def("getPMEParameters", [](const NonbondedForce& f) {
+            double a;
+            int x, y, z;
+            f.getPMEParameters(a, x, y, z);
+            return py::make_tuple(a, x, y, z);
+        })
+        .def("setPMEParameters", &NonbondedForce::setPMEParameters)
         .def("updateParametersInContext", &NonbondedForce::updateParametersInContext);
     nb.attr("NoCutoff") = (int)NonbondedForce::NoCutoff;
     nb.attr("CutoffNonPeriodic") = (int)NonbondedForce::CutoffNonPeriodic;
@@ -156,6 +175,51 @@ PYBIND11_MODULE(_atmmetaforce, m) {
     nb.attr("Ewald") = (int)NonbondedForce::Ewald;
     nb.attr("PME") = (int)NonbondedForce::PME;
     nb.attr("LJPME") = (int)NonbondedForce::LJPME;
+
+    py::class_<HarmonicBondForce, Force, std::unique_ptr<HarmonicBondForce, py::nodelete>>(m, "HarmonicBondForce")
+        .def(py::init([]() { return new HarmonicBondForce(); }))
+        .def("getNumBonds", &HarmonicBondForce::getNumBonds)
+        .def("addBond", &HarmonicBondForce::addBond, py::arg("particle1"), py::arg("particle2"), py::arg("length"),
+             py::arg("k"))
+        .def("getBondParameters", [](const HarmonicBondForce& f, int i) {
+            int a, b;
+            double r0, k;
+            f.getBondParameters(i, a, b, r0, k);
+            return py::make_tuple(a, b, r0, k);
+        })
+        .def("setBondParameters", &HarmonicBondForce::setBondParameters)
+        .def("setUsesPeriodicBoundaryConditions", &HarmonicBondForce::setUsesPeriodicBoundaryConditions)
+        .def("updateParametersInContext", &HarmonicBondForce::updateParametersInContext);
+
+    py::class_<HarmonicAngleForce, Force, std::unique_ptr<HarmonicAngleForce, py::nodelete>>(m, "HarmonicAngleForce")
+        .def(py::init([]() { return new HarmonicAngleForce(); }))
+        .def("getNumAngles", &HarmonicAngleForce::getNumAngles)
+        .def("addAngle", &HarmonicAngleForce::addAngle, py::arg("particle1"), py::arg("particle2"),
+             py::arg("particle3"), py::arg("angle"), py::arg("k"))
+        .def("getAngleParameters", [](const HarmonicAngleForce& f, int i) {
+            int a, b, c;
+            double t0, k;
+            f.getAngleParameters(i, a, b, c, t0, k);
+            return py::make_tuple(a, b, c, t0, k);
+        })
+        .def("setAngleParameters", &HarmonicAngleForce::setAngleParameters)
+        .def("setUsesPeriodicBoundaryConditions", &HarmonicAngleForce::setUsesPeriodicBoundaryConditions)
+        .def("updateParametersInContext", &HarmonicAngleForce::updateParametersInContext);
+
+    py::class_<PeriodicTorsionForce, Force, std::unique_ptr<PeriodicTorsionForce, py::nodelete>>(m, "PeriodicTorsionForce")
+        .def(py::init([]() { return new PeriodicTorsionForce(); }))
+        .def("getNumTorsions", &PeriodicTorsionForce::getNumTorsions)
+        .def("addTorsion", &PeriodicTorsionForce::addTorsion, py::arg("particle1"), py::arg("particle2"),
+             py::arg("particle3"), py::arg("particle4"), py::arg("periodicity"), py::arg("phase"), py::arg("k"))
+        .def("getTorsionParameters", [](const PeriodicTorsionForce& f, int i) {
+            int a, b, c, d, n;
+            double ph, k;
+            f.getTorsionParameters(i, a, b, c, d, n, ph, k);
+            return py::make_tuple(a, b, c, d, n, ph, k);
+        })
+        .def("setTorsionParameters", &PeriodicTorsionForce::setTorsionParameters)
+        .def("setUsesPeriodicBoundaryConditions", &PeriodicTorsionForce::setUsesPeriodicBoundaryConditions)
+        .def("updateParametersInContext", &PeriodicTorsionForce::updateParametersInContext);
 
     py::class_<System>(m, "System")
         .def(py::init<>())

@@ -1,0 +1,94 @@
+"""The reference's end-to-end test, python/tests/test_abfe.py, through this
+package's OpenMM-compatible API on the MI355X platform: the AMBER system with a
+PME NonbondedForce (amber.create_system, the terms of
+AmberPrmtopFile.createSystem(PME, 1 nm, HBonds)), ATMMetaForceUtils restraints,
+the NonbondedForce moved to group 1 and made the ATM variable group, a
+disabled barostat, a Langevin integrator over groups {0, 2}, and the goldens
+(:147-150).  Coordinates and box come from the reference's State checkpoint
+(temoa-g1-equil.xml, through the committed fixture) instead of loadState."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import atmmetaforce as atm
+from atmmetaforce import amber, openmm as mm
+
+import temoa
+
+pytestmark = pytest.mark.gpu
+
+
+def build_abfe_context():
+    s, box, x, ref40 = temoa.load()
+    system = amber.create_system(s, box)
+    utils = atm.ATMMetaForceUtils(system)
+    rcpt, lig, posr = temoa.restraint_setup(s)
+    utils.addVsiteRestraintForceCMCM(lig_cm_particles=list(lig), rcpt_cm_particles=list(rcpt),
+                                     kfcm=temoa.RESTRAINT_K, tolcm=temoa.CM_TOL, offset=[0.0, 0.0, 0.0])
+    refpos = np.zeros((s.num_atoms, 3))
+    refpos[:40] = ref40
+    utils.addPosRestraints(list(posr), refpos, temoa.RESTRAINT_K, temoa.POS_TOL)
+    atm_group, nb_group = 2, 1
+    utils.setNonbondedForceGroup(nb_group)
+    p = temoa.ATM_PARAMS
+    force = atm.ATMMetaForce(p["lambda1"], p["lambda2"], p["alpha"], p["u0"], p["w0"], p["umax"], p["ubcore"],
+                             p["acore"], 1.0, [nb_group])
+    for i in range(s.num_atoms):
+        force.addParticle(i, 0.0, 0.0, 0.0)
+    for i in lig:
+        force.setParticleParameters(int(i), int(i), *temoa.DISPLACEMENT)
+    force.setForceGroup(atm_group)
+    system.addForce(force)
+    barostat = mm.MonteCarloBarostat(1.0, 300.0)
+    barostat.setFrequency(0)
+    system.addForce(barostat)
+    integrator = mm.LangevinIntegrator(300.0, 0.5, 0.001)
+    integrator.setIntegrationForceGroups({0, atm_group})
+    ctx = mm.Context(system, integrator, mm.Platform.getPlatformByName("MI355X"))
+    ctx.setPositions(x)
+    ctx.setPeriodicBoxVectors((box[0], 0, 0), (0, box[1], 0), (0, 0, box[2]))
+    # test_abfe.py:134-142 (its Direction line sets acore: positive, so the forward direction)
+    for name, v in ((force.Lambda1(), p["lambda1"]), (force.Lambda2(), p["lambda2"]), (force.Alpha(), p["alpha"]),
+                    (force.U0(), p["u0"]), (force.W0(), p["w0"]), (force.Umax(), p["umax"]),
+                    (force.Ubcore(), p["ubcore"]), (force.Acore(), p["acore"]), (force.Direction(), p["acore"])):
+        ctx.setParameter(name, v)
+    return ctx, force, integrator, s, x
+
+
+def test_binding_energy_goldens_through_the_api(gpu):
+    print("[abfe-api] temoa-g1 through the OpenMM-compatible API", flush=True)
+    ctx, force, integ, s, x = build_abfe_context()
+    state = ctx.getState(getEnergy=True, groups={0, 2})
+    pot = state.getPotentialEnergy()
+    pert = force.getPerturbationEnergy(ctx)
+    assert abs(pot - temoa.GOLDEN_POTENTIAL) <= temoa.GOLDEN_ATOL, pot
+    assert abs(pert - temoa.GOLDEN_PERTURBATION) <= temoa.GOLDEN_ATOL, pert
+
+
+def test_pme_nonbonded_group_alone_and_forces(gpu):
+    """Group 1 alone is the PME NonbondedForce at x (U1 of the ATM force, minus
+    nothing); the bonded groups' forces match the oracle's."""
+    from oracle import pme_oracle as po
+    ctx, force, integ, s, x = build_abfe_context()
+    st = ctx.getState(getEnergy=True, getForces=True, groups={1})
+    box = temoa.load()[1]
+    U1, F1, _ = po.nonbonded_pme(x, s.charges, s.sigmas, s.epsilons, box, temoa.CUTOFF, s.exception_pairs,
+                                 s.exception_qq, s.exception_sigma, s.exception_eps, tol=temoa.EWALD_TOL)
+    assert abs(st.getPotentialEnergy() - U1) < 1e-9 * abs(U1)
+    assert np.abs(st.getForces(asNumpy=True) - F1).max() < 2e-5
+    eb, Fb = po.bonded(x, s.bonds, s.bond_params, s.angles, s.angle_params, s.torsions, s.torsion_params)
+    st0 = ctx.getState(getEnergy=True, getForces=True, groups={0})
+    # group 0 = bonded terms + restraints (zero at this state: both within their tolerances)
+    assert abs(st0.getPotentialEnergy() - sum(eb.values())) < 1e-8 * sum(eb.values())
+    assert np.abs(st0.getForces(asNumpy=True) - Fb).max() < 1e-5
+
+
+def test_integrators_refuse_constrained_systems(gpu):
+    """createSystem(constraints=HBonds) adds 8,324 hydrogen-bond constraints;
+    the integrators here do not apply constraints and say so instead of
+    integrating unconstrained bonds.  (Energies and forces, as in test_abfe.py,
+    do not involve them.)"""
+    ctx, force, integ, s, x = build_abfe_context()
+    with pytest.raises(Exception, match="does not apply constraints"):
+        integ.step(1)

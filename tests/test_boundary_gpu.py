@@ -157,8 +157,8 @@ def test_dispersion_correction_shifts_energies_not_forces(gpu):
 def test_errors_match_reference_behaviour(gpu):
     with pytest.raises(Exception, match="cannot be one of the variable force groups"):
         context_for(SYS, atm_group=1)
-    with pytest.raises(Exception, match="CutoffPeriodic"):
-        context_for(SYS, method=mm.NonbondedForce.PME)
+    with pytest.raises(Exception, match="CutoffPeriodic .reaction-field. or PME"):
+        context_for(SYS, method=mm.NonbondedForce.Ewald)
     with pytest.raises(Exception, match="every particle"):
         context_for(SYS, add_all=False)
     with pytest.raises(Exception, match="Illegal property name"):
