@@ -164,8 +164,10 @@ __device__ __forceinline__ PairE64 pair_term64(bool in, double r2, double sig, d
     const double r2inv = y * y;
     const double s2 = sig * sig * r2inv;
     const double s6 = s2 * s2 * s2;
-    const double elj = eps4 * fma(s6, s6, -s6);
-    const double flj = eps4 * s6 * fma(12.0, s6, -6.0);
+    // E_LJ = t (s6 - 1) and r dE/dr = -(12 t s6 - 6 t) = -(12 E_LJ + 6 t), t = 4 eps s6
+    const double t = eps4 * s6;
+    const double elj = fma(t, s6, -t);
+    const double flj = fma(12.0, elj, 6.0 * t);
     const double fc = qq * fma(-2.0 * krf, r2, y);
     PairE64 o;
     o.fs = in ? (flj + fc) * r2inv : 0.0;
