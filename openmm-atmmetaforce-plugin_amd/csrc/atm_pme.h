@@ -10,6 +10,8 @@ namespace atm {
 constexpr int kPmeCellMax = 1024;  // atoms per cell the cell sort holds
 constexpr int kPmeJBatch = 192;    // neighbour-cell atoms staged in LDS per batch (direct kernel)
 constexpr int kPmeBinLds = 4096;   // cell grids up to this size are binned through LDS histograms
+constexpr int kPmeDftLdsMax = 600; // grid lines up to this length are transformed from LDS
+constexpr int kPmeTileLdsMax = 49152;  // bytes of LDS for one cell's charge-grid tile
 
 // Scalars of one evaluation (by value in the kernel arguments).
 struct PmeDesc {
@@ -20,13 +22,13 @@ struct PmeDesc {
     int twOff[3];                // offset of each axis' twiddle table
     int nex;                     // exception pairs
     int nslot, nslotDirect, nslotEx;  // energy slots per state (direct | exceptions | reciprocal)
+    int tile[3], tileBytes;      // per-cell charge-grid tile (0 bytes: spread without tiles)
     double L[3], invL[3];
     double rc2, alpha, twoAlphaSqrtPi, pi2OverAlpha2;
     double recipEnergyScale;     // k_e / (2 pi V)
     double recipPhiScale;        // k_e / (pi V)
     double selfEnergy;           // -k_e alpha / sqrt(pi) sum q^2
     double dispersionEnergy;     // long-range LJ correction / V
-    double direction;
 };
 
 struct PmePtrs {
@@ -57,6 +59,7 @@ struct PmePtrs {
     double* eslot;          // [2][nslot]
     double* parts;          // [2][5]
     SysResult* result;
+    const AtmParamsDev* params;  // the nine ATM parameters (uploaded before each evaluation)
     int* overflow;
 };
 
@@ -74,8 +77,7 @@ struct BondedPtrs {
 
 hipError_t launch_pme_fill(unsigned long long* p, long long n, hipStream_t st);
 hipError_t launch_pme_state(const PmeDesc& D, const PmePtrs& P, int state, hipStream_t st);
-hipError_t launch_pme_finish(const PmeDesc& D, const PmePtrs& P, const AtmParamsDev& prm, bool single,
-                             double* forcesOut, hipStream_t st);
+hipError_t launch_pme_finish(const PmeDesc& D, const PmePtrs& P, bool single, double* forcesOut, hipStream_t st);
 hipError_t launch_bonded(const BondedPtrs& B, double* eslot, int nblocks, hipStream_t st);
 
 }  // namespace atm

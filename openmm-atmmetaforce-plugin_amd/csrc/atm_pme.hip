@@ -374,6 +374,102 @@ __global__ void k_pme_dft(PmeDesc D, const double2* in, double2* out, const doub
     out[idx] = make_double2(re, im);
 }
 
+// One wave per grid line along `axis` (four lines per block): the line and the
+// axis' twiddles are staged in LDS, then lane k (k + 64, ...) forms output k.
+// Dynamic LDS: 5 K complex values.
+__global__ void __launch_bounds__(256) k_pme_dft_lines(PmeDesc D, const double2* in, double2* out, const double2* tw,
+                                                      int axis, double sign) {
+    extern __shared__ double2 sm[];
+    const int K = D.K[axis];
+    const int stride = axis == 2 ? 1 : (axis == 1 ? D.K[2] : D.K[1] * D.K[2]);
+    const int nlines = D.ngrid / K;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    double2* stw = sm;
+    double2* line = sm + K + w * K;
+    for (int t = threadIdx.x; t < K; t += 256) stw[t] = tw[t];
+    const int L = blockIdx.x * 4 + w;
+    int base = 0;
+    if (L < nlines) {
+        base = (L / stride) * K * stride + L % stride;
+        for (int j = lane; j < K; j += 64) line[j] = in[base + j * stride];
+    }
+    __syncthreads();
+    if (L >= nlines) return;
+    for (int k = lane; k < K; k += 64) {
+        double re = 0.0, im = 0.0;
+        int t = 0;
+        for (int j = 0; j < K; ++j) {
+            const double2 v = line[j];
+            const double2 c = stw[t];
+            const double sn = sign * c.y;
+            re += v.x * c.x - v.y * sn;
+            im += v.y * c.x + v.x * sn;
+            t += k;
+            if (t >= K) t -= K;
+        }
+        out[base + k * stride] = make_double2(re, im);
+    }
+}
+
+// Charge spreading of one cell's atoms into an LDS tile of the grid (64-bit
+// fixed point, integer LDS atomics), then one global atomic per nonzero tile
+// point.  The tile covers grid indices [o, o + T) per axis with
+// o = floor(c K / m) - 5 and T = D.tile[axis]; a point outside it (rounding at
+// the cell edge) goes straight to the global grid.
+__global__ void __launch_bounds__(256) k_pme_spread_tiles(PmeDesc D, PmePtrs P) {
+    extern __shared__ unsigned long long tile[];
+    const int c = blockIdx.x;
+    const int cz = c % D.m[2], cy = (c / D.m[2]) % D.m[1], cx = c / (D.m[1] * D.m[2]);
+    const int o[3] = {(cx * D.K[0]) / D.m[0] - 5, (cy * D.K[1]) / D.m[1] - 5, (cz * D.K[2]) / D.m[2] - 5};
+    const int T0 = D.tile[0], T1 = D.tile[1], T2 = D.tile[2], TT = T0 * T1 * T2;
+    for (int k = threadIdx.x; k < TT; k += 256) tile[k] = 0ull;
+    __syncthreads();
+    const int s0 = P.cellStart[c], cnt = min(P.cellStart[c + 1] - s0, D.cellCap);
+    // task = (atom, first B-spline index a): 25 points each
+    for (int task = threadIdx.x; task < cnt * kOrder; task += 256) {
+        const int sl = s0 + task / kOrder, a = task % kOrder;
+        const double4 p = P.cpos[sl];
+        const double q = P.prm[P.cex[sl].w].y;
+        if (q == 0.0) continue;
+        double th[3][kOrder], dth[kOrder];
+        int g0[3];
+        const double pv[3] = {p.x, p.y, p.z};
+        for (int d = 0; d < 3; ++d) {
+            const double u = pv[d] * D.invL[d] * D.K[d];
+            const int g = (int)floor(u);
+            g0[d] = g;
+            bspline5(u - g, th[d], dth);
+        }
+        const int la = g0[0] - a - o[0];
+        for (int b = 0; b < kOrder; ++b) {
+            const int lb = g0[1] - b - o[1];
+            const double qab = q * th[0][a] * th[1][b];
+            for (int cc = 0; cc < kOrder; ++cc) {
+                const int lc = g0[2] - cc - o[2];
+                const unsigned long long v = (unsigned long long)__double2ll_rn(qab * th[2][cc] * kGridFix);
+                if (la >= 0 && la < T0 && lb >= 0 && lb < T1 && lc >= 0 && lc < T2) {
+                    atomicAdd(&tile[(la * T1 + lb) * T2 + lc], v);
+                } else {
+                    const int ia = ((g0[0] - a) % D.K[0] + D.K[0]) % D.K[0];
+                    const int ib = ((g0[1] - b) % D.K[1] + D.K[1]) % D.K[1];
+                    const int ic = ((g0[2] - cc) % D.K[2] + D.K[2]) % D.K[2];
+                    atomicAdd(reinterpret_cast<unsigned long long*>(P.qgrid) + ((long long)ia * D.K[1] + ib) * D.K[2] + ic, v);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < TT; k += 256) {
+        const unsigned long long v = tile[k];
+        if (v == 0ull) continue;
+        const int lc = k % T2, lb = (k / T2) % T1, la = k / (T1 * T2);
+        const int ia = ((o[0] + la) % D.K[0] + D.K[0]) % D.K[0];
+        const int ib = ((o[1] + lb) % D.K[1] + D.K[1]) % D.K[1];
+        const int ic = ((o[2] + lc) % D.K[2] + D.K[2]) % D.K[2];
+        atomicAdd(reinterpret_cast<unsigned long long*>(P.qgrid) + ((long long)ia * D.K[1] + ib) * D.K[2] + ic, v);
+    }
+}
+
 __global__ void __launch_bounds__(256) k_pme_convolve(PmeDesc D, PmePtrs P, double2* g, double* eslot) {
     __shared__ double red[4];
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
@@ -470,7 +566,7 @@ __global__ void __launch_bounds__(256) k_pme_exceptions(PmeDesc D, PmePtrs P, co
 // Fixed-order sums of the energy slots of both states; U = sum + constants;
 // combine.  parts[s][0..4]: direct, exceptions, reciprocal, self, dispersion.
 // single: state 0 only (W = U1, hybrid weight 0).
-__global__ void __launch_bounds__(1024) k_pme_finish(PmeDesc D, PmePtrs P, AtmParamsDev prm, int single) {
+__global__ void __launch_bounds__(1024) k_pme_finish(PmeDesc D, PmePtrs P, int single) {
     __shared__ double red[16];
     double U[2] = {0.0, 0.0};
     for (int s = 0; s < (single ? 1 : 2); ++s) {
@@ -496,7 +592,7 @@ __global__ void __launch_bounds__(1024) k_pme_finish(PmeDesc D, PmePtrs P, AtmPa
         if (single) {
             r.W = U[0]; r.usc = 0.0; r.U1 = U[0]; r.U2 = 0.0; r.u = 0.0; r.fp = 1.0; r.bfp = 0.0; r.sp = 0.0;
         } else {
-            atm_combine(prm, U[0], U[1], r);
+            atm_combine(*P.params, U[0], U[1], r);
         }
         *P.result = r;
     }
@@ -506,7 +602,7 @@ __global__ void k_pme_blend(PmeDesc D, PmePtrs P, int single, double* out) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= 3 * D.n) return;
     const SysResult r = *P.result;
-    const double a = single ? 0.0 : (D.direction > 0.0 ? r.sp : 1.0 - r.sp);  // weight of state 2 (x + d)
+    const double a = single ? 0.0 : (P.params->direction > 0.0 ? r.sp : 1.0 - r.sp);  // weight of state 2 (x + d)
     const double f1 = (double)(long long)P.F[k] * (1.0 / kFix);
     const double f2 = single ? 0.0 : (double)(long long)P.F[3ll * D.n + k] * (1.0 / kFix);
     const int comp = k / D.n, i = k % D.n;
@@ -546,23 +642,33 @@ hipError_t launch_pme_state(const PmeDesc& D, const PmePtrs& P, int state, hipSt
     k_pme_direct<<<dim3(27, D.ncell, D.chunks), 64, 0, st>>>(D, P, F, es);
     k_pme_exceptions<<<blocks_of(D.nex, 256) > 0 ? blocks_of(D.nex, 256) : 1, 256, 0, st>>>(D, P, x, F,
                                                                                                es + D.nslotDirect);
-    k_pme_spread<<<blocks_of(D.n, 256), 256, 0, st>>>(D, P, x);
+    if (D.tileBytes > 0)  // the cell's grid tile fits in LDS
+        k_pme_spread_tiles<<<D.ncell, 256, D.tileBytes, st>>>(D, P);
+    else
+        k_pme_spread<<<blocks_of(D.n, 256), 256, 0, st>>>(D, P, x);
     k_pme_q2c<<<blocks_of(D.ngrid, 256), 256, 0, st>>>(D, P);
     const unsigned gb = blocks_of(D.ngrid, 256);
-    k_pme_dft<<<gb, 256, 0, st>>>(D, P.g0, P.g1, P.tw + D.twOff[2], 2, -1.0);
-    k_pme_dft<<<gb, 256, 0, st>>>(D, P.g1, P.g0, P.tw + D.twOff[1], 1, -1.0);
-    k_pme_dft<<<gb, 256, 0, st>>>(D, P.g0, P.g1, P.tw + D.twOff[0], 0, -1.0);
+    auto dft = [&](const double2* in, double2* out, int axis, double sign) {
+        const int K = D.K[axis];
+        if (K <= kPmeDftLdsMax)
+            k_pme_dft_lines<<<blocks_of(D.ngrid / K, 4), 256, 5 * K * sizeof(double2), st>>>(D, in, out, P.tw + D.twOff[axis],
+                                                                                          axis, sign);
+        else
+            k_pme_dft<<<gb, 256, 0, st>>>(D, in, out, P.tw + D.twOff[axis], axis, sign);
+    };
+    dft(P.g0, P.g1, 2, -1.0);
+    dft(P.g1, P.g0, 1, -1.0);
+    dft(P.g0, P.g1, 0, -1.0);
     k_pme_convolve<<<gb, 256, 0, st>>>(D, P, P.g1, es + D.nslotDirect + D.nslotEx);
-    k_pme_dft<<<gb, 256, 0, st>>>(D, P.g1, P.g0, P.tw + D.twOff[0], 0, 1.0);
-    k_pme_dft<<<gb, 256, 0, st>>>(D, P.g0, P.g1, P.tw + D.twOff[1], 1, 1.0);
-    k_pme_dft<<<gb, 256, 0, st>>>(D, P.g1, P.g0, P.tw + D.twOff[2], 2, 1.0);
+    dft(P.g1, P.g0, 0, 1.0);
+    dft(P.g0, P.g1, 1, 1.0);
+    dft(P.g1, P.g0, 2, 1.0);
     k_pme_gather<<<blocks_of(D.n, 256), 256, 0, st>>>(D, P, x, P.g0, F);
     return hipGetLastError();
 }
 
-hipError_t launch_pme_finish(const PmeDesc& D, const PmePtrs& P, const AtmParamsDev& prm, bool single,
-                             double* forcesOut, hipStream_t st) {
-    k_pme_finish<<<1, 1024, 0, st>>>(D, P, prm, single ? 1 : 0);
+hipError_t launch_pme_finish(const PmeDesc& D, const PmePtrs& P, bool single, double* forcesOut, hipStream_t st) {
+    k_pme_finish<<<1, 1024, 0, st>>>(D, P, single ? 1 : 0);
     if (forcesOut) k_pme_blend<<<blocks_of(3ll * D.n, 256), 256, 0, st>>>(D, P, single ? 1 : 0, forcesOut);
     return hipGetLastError();
 }

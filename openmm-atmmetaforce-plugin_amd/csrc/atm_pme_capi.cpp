@@ -83,12 +83,24 @@ struct PmeForce {
 
     DevBuf dX, dX2, dDispl, dCellOf, dRankOf, dCellCount, dCellStart, dCellAtoms, dWpos, dPrm, dPrm2, dExkey;
     DevBuf dCpos, dCprm, dCex, dExbits, dExpairs, dExprm, dQgrid, dG0, dG1, dTw, dBmod, dF, dEslot, dParts;
-    DevBuf dResult, dOverflow, dOut;
+    DevBuf dResult, dOverflow, dOut, dParams;
     PmeDesc D{};
     PmePtrs P{};
     std::vector<double> hOut;
+    // captured evaluation per (state 1 only, forces) and the descriptors it was captured with
+    hipGraphExec_t graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+    PmeDesc graphD[2][2];
+    PmePtrs graphP[2][2];
 
+    void dropGraphs() {
+        for (auto& row : graph)
+            for (hipGraphExec_t& g : row) {
+                if (g) (void)hipGraphExecDestroy(g);
+                g = nullptr;
+            }
+    }
     ~PmeForce() {
+        dropGraphs();
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -217,6 +229,7 @@ void setup(PmeForce& S) {
     S.dF.alloc(sizeof(unsigned long long) * 6 * n);
     S.dParts.alloc(sizeof(double) * 10);
     S.dResult.alloc(sizeof(SysResult));
+    S.dParams.alloc(sizeof(AtmParamsDev));
     S.dOverflow.alloc(sizeof(int) * 2);
     S.dOut.alloc(sizeof(double) * 3 * n);
     S.hOut.assign(3 * (size_t)n, 0.0);
@@ -259,6 +272,13 @@ void set_box(PmeForce& S, const double box[3]) {
     S.D.nslotDirect = S.D.chunks * 27 * S.D.ncell;
     S.D.nslotEx = std::max(1, (S.D.nex + 255) / 256);
     S.D.nslot = S.D.nslotDirect + S.D.nslotEx + (S.D.ngrid + 255) / 256;
+    // charge-grid tile of one cell: grid indices [floor(c K/m) - 5, ... + ceil(K/m) + 7)
+    long long tb = 8;
+    for (int d = 0; d < 3; ++d) {
+        S.D.tile[d] = (S.K[d] + m[d] - 1) / m[d] + 7;
+        tb *= S.D.tile[d];
+    }
+    S.D.tileBytes = tb <= kPmeTileLdsMax ? (int)tb : 0;
     if (S.D.ncell > S.ncellAlloc) {
         S.dCellCount.alloc(sizeof(int) * (S.D.ncell + 2));
         S.dCellStart.alloc(sizeof(int) * (S.D.ncell + 2));
@@ -268,7 +288,6 @@ void set_box(PmeForce& S, const double box[3]) {
         S.dEslot.alloc(sizeof(double) * 2 * (size_t)S.D.nslot);
         S.nslotAlloc = S.D.nslot;
     }
-    S.D.direction = S.params.direction;
     PmePtrs& P = S.P;
     P.x = S.dX.as<double>();
     P.x2 = S.dX2.as<double>();
@@ -297,6 +316,7 @@ void set_box(PmeForce& S, const double box[3]) {
     P.eslot = S.dEslot.as<double>();
     P.parts = S.dParts.as<double>();
     P.result = S.dResult.as<SysResult>();
+    P.params = S.dParams.as<AtmParamsDev>();
     P.overflow = S.dOverflow.as<int>();
 }
 
@@ -471,13 +491,36 @@ int atm_pme_eval(void* h, const double* positions, const double box[3], int32_t 
         HIPCHK(hipMemcpyAsync(S.dX.p, positions, sizeof(double) * 3 * S.n, hipMemcpyHostToDevice, st));
         AtmParamsDev prm{S.params.lambda1, S.params.lambda2, S.params.alpha, S.params.u0, S.params.w0,
                          S.params.umax, S.params.ubcore, S.params.acore, S.params.direction};
+        const bool single = (flags & ATM_EVAL_STATE1_ONLY) != 0;
+        HIPCHK(hipMemcpyAsync(S.dParams.p, &prm, sizeof(prm), hipMemcpyHostToDevice, st));
         for (int attempt = 0; attempt < 2; ++attempt) {
             set_box(S, box ? box : S.box);
-            HIPCHK(launch_pme_fill(reinterpret_cast<unsigned long long*>(S.dOverflow.p), 1, st));
-            const bool single = (flags & ATM_EVAL_STATE1_ONLY) != 0;
-            HIPCHK(launch_pme_state(S.D, S.P, 0, st));
-            if (!single) HIPCHK(launch_pme_state(S.D, S.P, 1, st));
-            HIPCHK(launch_pme_finish(S.D, S.P, prm, single, forces ? S.dOut.as<double>() : nullptr, st));
+            // one captured graph per (state 1 only, forces); captured again when the
+            // descriptor or a buffer changed (box, cell capacity, parameters)
+            hipGraphExec_t& g = S.graph[single][forces != nullptr];
+            PmeDesc& gD = S.graphD[single][forces != nullptr];
+            PmePtrs& gP = S.graphP[single][forces != nullptr];
+            if (g && (std::memcmp(&gD, &S.D, sizeof(PmeDesc)) != 0 || std::memcmp(&gP, &S.P, sizeof(PmePtrs)) != 0)) {
+                (void)hipGraphExecDestroy(g);
+                g = nullptr;
+            }
+            if (!g) {
+                hipGraph_t graph = nullptr;
+                HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+                hipError_t e = launch_pme_fill(reinterpret_cast<unsigned long long*>(S.dOverflow.p), 1, st);
+                if (e == hipSuccess) e = launch_pme_state(S.D, S.P, 0, st);
+                if (e == hipSuccess && !single) e = launch_pme_state(S.D, S.P, 1, st);
+                if (e == hipSuccess) e = launch_pme_finish(S.D, S.P, single, forces ? S.dOut.as<double>() : nullptr, st);
+                const hipError_t ee = hipStreamEndCapture(st, &graph);
+                HIPCHK(e);
+                HIPCHK(ee);
+                const hipError_t ei = hipGraphInstantiate(&g, graph, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(graph);
+                HIPCHK(ei);
+                gD = S.D;
+                gP = S.P;
+            }
+            HIPCHK(hipGraphLaunch(g, st));
             int ovf[2] = {0, 0};
             HIPCHK(hipMemcpyAsync(ovf, S.dOverflow.p, sizeof(int) * 2, hipMemcpyDeviceToHost, st));
             HIPCHK(hipStreamSynchronize(st));

@@ -171,3 +171,21 @@ def test_temoa_goldens_on_gpu(gpu):
     assert abs(r["state2_energy"] - ref["U2"]) <= E_REL * abs(ref["U2"])
     assert abs(r["perturbation_energy"] - ref["usc"]) < 1e-6
     assert np.abs(F - Fref).max() < 1e-4, np.abs(F - Fref).max()
+
+
+def test_pme_parameter_and_box_changes_between_evaluations(gpu):
+    """The evaluation is a captured graph: new ATM parameters and a new box
+    must take effect on the next call (parameters through device memory, box
+    by a new capture)."""
+    x, q, sig, eps, L, pairs, qq, es, ee, d = _small_system(seed=7)
+    pa = dict(PARAMS, lambda1=0.1, lambda2=0.2, direction=1.0)
+    pb = dict(PARAMS, lambda1=0.4, lambda2=0.45, alpha=0.0, direction=-1.0)  # (no softplus: e^x overflows here)
+    L2 = L * 1.01
+    with pme.ATMPMEForce(q, sig, eps, d, L, 1.0, exceptions=(pairs, qq, es, ee), params=pa) as f:
+        for params, box in ((pa, L), (pb, L), (pb, L2), (pa, L)):
+            f.set_params(params)
+            r, F, _ = f.evaluate(x, box=box)
+            ref, Fref, _, _ = _oracle_atm(x, q, sig, eps, box, pairs, qq, es, ee, d, params,
+                                          alpha=f.info()[0], grid=f.info()[1])
+            assert abs(r["energy"] - ref["W"]) <= E_REL * abs(ref["W"]), (params, box, r["energy"], ref["W"])
+            assert np.abs(F - Fref).max() < F_ABS
