@@ -187,18 +187,19 @@ __global__ void __launch_bounds__(256) k_pme_cellsort(PmeDesc D, PmePtrs P) {
     }
 }
 
-// blockIdx.x = one of the 27 neighbour cells of cell c = blockIdx.y,
+// Half shell: blockIdx.x = 0 is cell c = blockIdx.y itself (pairs with the
+// j slot above the i slot), 1..13 its forward neighbour cells (all pairs);
 // blockIdx.z = chunk of 64 atoms of c (one per lane).  Every pair is visited
-// from both of its atoms (no j-force traffic), half the energy each.
-//  test:  the wave walks the neighbour cell's atoms; each lane tests its atom
-//         against the common j (FP64 r^2, the image shift known from the cell
-//         offset, exclusion bitmap for atoms of one exclusion group) and the
-//         in-range pairs are appended to a ring in LDS (ballot + mbcnt, so
-//         the order is fixed);
+// once: +f on i, -f on j, the whole pair energy.
+//  test:  the wave walks the neighbour cell's atoms staged in LDS; each lane
+//         tests its atom against the common j (FP64 r^2, the image shift known
+//         from the cell offset, exclusion bitmap for atoms of one exclusion
+//         group) and the in-range pairs are appended to a ring in LDS
+//         (ballot + mbcnt, so the order is fixed);
 //  dense: whenever 64 pairs are pending, each lane evaluates one of them
-//         (erfc Coulomb + LJ in FP64), so the expensive pair function runs on
-//         in-cutoff pairs only; i-forces go to per-atom LDS accumulators in
-//         64-bit fixed point (integer atomics: order-independent).
+//         (erfc Coulomb + LJ in FP64); forces go to per-atom LDS accumulators
+//         of the chunk (i) and of the staged batch (j) in 64-bit fixed point
+//         (integer atomics: order-independent), flushed to global memory.
 __global__ void __launch_bounds__(64) k_pme_direct(PmeDesc D, PmePtrs P, unsigned long long* F, double* eslot) {
     __shared__ double4 sI[64];
     __shared__ double2 sIp[64];
@@ -206,11 +207,18 @@ __global__ void __launch_bounds__(64) k_pme_direct(PmeDesc D, PmePtrs P, unsigne
     __shared__ uint32_t sQ[128];
     __shared__ double4 sJ[kPmeJBatch];  // the neighbour cell's atoms of the current batch
     __shared__ double2 sJp[kPmeJBatch];
-    __shared__ int2 sJx[kPmeJBatch];    // their exclusion group and rank
+    __shared__ int4 sJx[kPmeJBatch];    // exclusion group, rank, -, atom
+    __shared__ unsigned long long sJF[3][kPmeJBatch];
     const int lane = threadIdx.x;
     const int nb = blockIdx.x, c = blockIdx.y, chunk = blockIdx.z;
     const int cz = c % D.m[2], cy = (c / D.m[2]) % D.m[1], cx = c / (D.m[1] * D.m[2]);
-    int nx = cx + nb / 9 - 1, ny = cy + (nb / 3) % 3 - 1, nz = cz + nb % 3 - 1;
+    // forward offsets: (a, b, e) lexicographically above (0, 0, 0); nb = 0 is the cell itself
+    int ox = 0, oy = 0, oz = 0;
+    if (nb > 0) {
+        const int f = nb + 13;  // 14..26 in the 3x3x3 numbering (a + 1) 9 + (b + 1) 3 + (e + 1)
+        ox = f / 9 - 1; oy = (f / 3) % 3 - 1; oz = f % 3 - 1;
+    }
+    int nx = cx + ox, ny = cy + oy, nz = cz + oz;
     double sx = 0.0, sy = 0.0, sz = 0.0;  // image of the neighbour cell: x_j + s
     if (nx < 0) { nx += D.m[0]; sx = -D.L[0]; } else if (nx >= D.m[0]) { nx -= D.m[0]; sx = D.L[0]; }
     if (ny < 0) { ny += D.m[1]; sy = -D.L[1]; } else if (ny >= D.m[1]) { ny -= D.m[1]; sy = D.L[1]; }
@@ -237,6 +245,8 @@ __global__ void __launch_bounds__(64) k_pme_direct(PmeDesc D, PmePtrs P, unsigne
     sI[lane] = pi;
     sF[0][lane] = 0ull; sF[1][lane] = 0ull; sF[2][lane] = 0ull;
     const int j0 = P.cellStart[cn], j1 = P.cellStart[cn + 1];
+    // in the cell itself only j slots above the chunk's first slot can pair with it
+    const int jstart = nb == 0 ? s0 + chunk * 64 + 1 : j0;
     const double a = D.alpha, a2pi = D.twoAlphaSqrtPi, rc2 = D.rc2;
     double e = 0.0;
     int head = 0, tail = 0;  // ring [head, tail) of pending pairs (wave-uniform)
@@ -259,33 +269,40 @@ __global__ void __launch_bounds__(64) k_pme_direct(PmeDesc D, PmePtrs P, unsigne
             const double ec = qq * erfc(a * r) * rinv;
             const double sg = ip.x + jp.x, e4 = ip.y * jp.y;
             const double s2 = sg * sg * r2inv, s6 = s2 * s2 * s2;
-            const double elj = e4 * (s6 * s6 - s6);
-            const double fs = (ec + qq * a2pi * exp(-a * a * r2) + e4 * (12.0 * s6 * s6 - 6.0 * s6)) * r2inv;
-            e += 0.5 * (ec + elj);
-            atomicAdd(&sF[0][il], (unsigned long long)__double2ll_rn(fs * dx * kFix));
-            atomicAdd(&sF[1][il], (unsigned long long)__double2ll_rn(fs * dy * kFix));
-            atomicAdd(&sF[2][il], (unsigned long long)__double2ll_rn(fs * dz * kFix));
+            const double t = e4 * s6;
+            const double elj = fma(t, s6, -t);
+            const double fs = (ec + qq * a2pi * exp(-a * a * r2) + fma(12.0, elj, 6.0 * t)) * r2inv;
+            e += ec + elj;
+            const long long fx = __double2ll_rn(fs * dx * kFix), fy = __double2ll_rn(fs * dy * kFix);
+            const long long fz = __double2ll_rn(fs * dz * kFix);
+            atomicAdd(&sF[0][il], (unsigned long long)fx);
+            atomicAdd(&sF[1][il], (unsigned long long)fy);
+            atomicAdd(&sF[2][il], (unsigned long long)fz);
+            atomicAdd(&sJF[0][k], (unsigned long long)(-fx));
+            atomicAdd(&sJF[1][k], (unsigned long long)(-fy));
+            atomicAdd(&sJF[2][k], (unsigned long long)(-fz));
         }
         head += count;
     };
 
-    for (int jb = j0; jb < j1; jb += kPmeJBatch) {
+    for (int jb = jstart; jb < j1; jb += kPmeJBatch) {
         const int nbat = min(kPmeJBatch, j1 - jb);
-        __syncthreads();  // the previous batch is fully evaluated
+        __syncthreads();  // the previous batch is fully evaluated and flushed
         for (int k = lane; k < nbat; k += 64) {
             sJ[k] = P.cpos[jb + k];
             sJp[k] = P.cprm[jb + k];
-            const int4 x = P.cex[jb + k];
-            sJx[k] = make_int2(x.x, x.y);
+            sJx[k] = P.cex[jb + k];
+            sJF[0][k] = 0ull; sJF[1][k] = 0ull; sJF[2][k] = 0ull;
         }
         __syncthreads();
         for (int k = 0; k < nbat; ++k) {
+            const int sj = jb + k;
             const double4 pj = sJ[k];
             const double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
             const double r2 = dx * dx + dy * dy + dz * dz;
-            bool in = active && r2 < rc2 && jb + k != si;
+            bool in = active && r2 < rc2 && (nb != 0 || sj > si);
             if (__ballot(in) == 0ull) continue;
-            const int2 xj = sJx[k];
+            const int4 xj = sJx[k];
             if (in && xj.x == xi.x) {  // same exclusion group: look the pair up
                 const uint32_t bits = (xj.y >> 5) == 0 ? word0 : P.exbits[xi.z + (xj.y >> 5)];
                 if ((bits >> (xj.y & 31)) & 1u) in = false;
@@ -302,6 +319,14 @@ __global__ void __launch_bounds__(64) k_pme_direct(PmeDesc D, PmePtrs P, unsigne
         if (tail > head) {  // drain: the next batch replaces the staged j data
             __syncthreads();
             dense(tail - head);
+        }
+        __syncthreads();
+        for (int k = lane; k < nbat; k += 64) {  // j forces of the batch
+            const int atom = sJx[k].w;
+            for (int q = 0; q < 3; ++q) {
+                const unsigned long long v = sJF[q][k];
+                if (v) atomicAdd(F + (long long)q * D.n + atom, v);
+            }
         }
     }
     __syncthreads();
@@ -639,7 +664,7 @@ hipError_t launch_pme_state(const PmeDesc& D, const PmePtrs& P, int state, hipSt
     k_pme_cellsort<<<D.ncell, 256, 0, st>>>(D, P);
     // neighbour offset fastest, chunk slowest: consecutive workgroups (dealt round-robin to the
     // XCDs) carry comparable work, and the mostly empty late chunks come last
-    k_pme_direct<<<dim3(27, D.ncell, D.chunks), 64, 0, st>>>(D, P, F, es);
+    k_pme_direct<<<dim3(kPmeHalfShell, D.ncell, D.chunks), 64, 0, st>>>(D, P, F, es);
     k_pme_exceptions<<<blocks_of(D.nex, 256) > 0 ? blocks_of(D.nex, 256) : 1, 256, 0, st>>>(D, P, x, F,
                                                                                                es + D.nslotDirect);
     if (D.tileBytes > 0)  // the cell's grid tile fits in LDS

@@ -269,7 +269,7 @@ void set_box(PmeForce& S, const double box[3]) {
     cap = std::min(kPmeCellMax, (cap + 63) / 64 * 64);
     S.D.cellCap = cap;
     S.D.chunks = cap / 64;
-    S.D.nslotDirect = S.D.chunks * 27 * S.D.ncell;
+    S.D.nslotDirect = S.D.chunks * kPmeHalfShell * S.D.ncell;
     S.D.nslotEx = std::max(1, (S.D.nex + 255) / 256);
     S.D.nslot = S.D.nslotDirect + S.D.nslotEx + (S.D.ngrid + 255) / 256;
     // charge-grid tile of one cell: grid indices [floor(c K/m) - 5, ... + ceil(K/m) + 7)
