@@ -10,10 +10,14 @@ class ContextImpl;
 
 class OPENMM_EXPORT Integrator {
 public:
-    explicit Integrator(double stepSize) : stepSize(stepSize), groups(0xFFFFFFFF), context(nullptr) {}
+    explicit Integrator(double stepSize)
+        : stepSize(stepSize), constraintTolerance(1e-5), groups(0xFFFFFFFF), context(nullptr) {}
     virtual ~Integrator() {}
     double getStepSize() const { return stepSize; }
     void setStepSize(double s) { stepSize = s; }
+    // Relative tolerance of the distance constraints (OpenMM's default 1e-5).
+    double getConstraintTolerance() const { return constraintTolerance; }
+    void setConstraintTolerance(double tol) { constraintTolerance = tol; }
     // Force groups whose forces drive the integration (all by default).
     int getIntegrationForceGroups() const { return groups; }
     void setIntegrationForceGroups(int mask) { groups = mask; }
@@ -23,13 +27,14 @@ public:
 protected:
     friend class ContextImpl;
     virtual void initialize(ContextImpl& ctx) { context = &ctx; }
-    double stepSize;
+    double stepSize, constraintTolerance;
     int groups;
     ContextImpl* context;
 };
 
 // Velocity Verlet (leapfrog form) evaluated on the host around the platform's
-// force evaluation.  Constraints are not applied.
+// force evaluation; distance constraints by SHAKE on the new positions, the
+// velocities then (x' - x) / dt.
 class OPENMM_EXPORT VerletIntegrator : public Integrator {
 public:
     explicit VerletIntegrator(double stepSize) : Integrator(stepSize) {}
@@ -41,7 +46,9 @@ public:
 //   v = a v + (1 - a)/friction F/m + sqrt(kT (1 - a^2)/m) R,  x += dt v,
 // a = exp(-friction dt) ((1 - a)/friction -> dt without friction).
 // temperature K, friction 1/ps, step size ps.  Host update around the
-// platform's forces; Gaussian noise from a seeded std::mt19937_64.
+// platform's forces; Gaussian noise from a seeded std::mt19937_64.  Distance
+// constraints (createSystem's HBonds) by SHAKE on the new positions, the
+// velocities then (x' - x) / dt.
 class OPENMM_EXPORT LangevinIntegrator : public Integrator {
 public:
     LangevinIntegrator(double temperature, double frictionCoeff, double stepSize)
@@ -62,7 +69,8 @@ protected:
 
 // LangevinMiddle / BAOAB splitting: v += dt F/m; x += dt/2 v;
 // v = a v + sqrt(kT (1 - a^2)/m) R; x += dt/2 v (the scheme of the batched
-// device MD, atm_md_step).
+// device MD, atm_md_step).  Constraints are not applied (a System with any is
+// refused).
 class OPENMM_EXPORT LangevinMiddleIntegrator : public LangevinIntegrator {
 public:
     LangevinMiddleIntegrator(double temperature, double frictionCoeff, double stepSize)

@@ -314,12 +314,50 @@ const std::map<std::string, double>& State::getParameters() const {
 
 // ------------------------------------------------------------------ Integrator
 
-// The integrators here do not apply constraints: a System that holds any is
-// refused rather than integrated with unconstrained bonds.
+// LangevinMiddleIntegrator does not apply constraints: a System that holds any
+// is refused rather than integrated with unconstrained bonds.
 static void requireNoConstraints(const System& sys) {
     if (sys.getNumConstraints() > 0)
         throw OpenMMException("This integrator does not apply constraints; the System has " +
-                              std::to_string(sys.getNumConstraints()) + " (build it without constraints to integrate)");
+                              std::to_string(sys.getNumConstraints()) + " (use LangevinIntegrator or VerletIntegrator)");
+}
+
+// SHAKE (Ryckaert, Ciccotti, Berendsen, J. Comput. Phys. 23, 327 (1977)): move
+// the new positions xn along the old bond vectors until every constraint
+// |xn_i - xn_j| = d holds to `tol` (relative).  Massless particles do not
+// move; a constraint between two of them is left alone.
+static void shake(const System& sys, const std::vector<Vec3>& xo, std::vector<Vec3>& xn, double tol) {
+    const int nc = sys.getNumConstraints();
+    if (nc == 0) return;
+    std::vector<int> p1(nc), p2(nc);
+    std::vector<double> d2(nc), w1(nc), w2(nc);
+    std::vector<Vec3> ref(nc);
+    for (int k = 0; k < nc; ++k) {
+        double d;
+        sys.getConstraintParameters(k, p1[k], p2[k], d);
+        d2[k] = d * d;
+        const double m1 = sys.getParticleMass(p1[k]), m2 = sys.getParticleMass(p2[k]);
+        w1[k] = m1 > 0.0 ? 1.0 / m1 : 0.0;
+        w2[k] = m2 > 0.0 ? 1.0 / m2 : 0.0;
+        ref[k] = xo[p1[k]] - xo[p2[k]];
+    }
+    for (int it = 0; it < 1000; ++it) {
+        bool done = true;
+        for (int k = 0; k < nc; ++k) {
+            if (w1[k] + w2[k] == 0.0) continue;
+            const Vec3 sv = xn[p1[k]] - xn[p2[k]];
+            const double diff = d2[k] - sv.dot(sv);
+            if (std::fabs(diff) <= 2.0 * tol * d2[k]) continue;
+            done = false;
+            double rs = ref[k].dot(sv);
+            if (rs < 1e-6 * d2[k]) rs = 1e-6 * d2[k];  // bond turned past 90 degrees: keep the step bounded
+            const double g = diff / (2.0 * rs * (w1[k] + w2[k]));
+            xn[p1[k]] += ref[k] * (g * w1[k]);
+            xn[p2[k]] -= ref[k] * (g * w2[k]);
+        }
+        if (done) return;
+    }
+    throw OpenMMException("SHAKE: constraints did not converge in 1000 iterations");
 }
 
 void Integrator::setIntegrationForceGroups(const std::set<int>& s) {
@@ -335,20 +373,28 @@ void VerletIntegrator::step(int steps) {
     if (!context) throw OpenMMException("This Integrator is not bound to a context!");
     ContextImpl& c = *context;
     const System& sys = c.getSystem();
-    requireNoConstraints(sys);
     const int n = sys.getNumParticles();
+    const bool constrained = sys.getNumConstraints() > 0;
     std::vector<Vec3>& x = c.positionsRef();
     std::vector<Vec3>& v = c.velocitiesRef();
+    std::vector<Vec3> xn;
     for (int s = 0; s < steps; ++s) {
         c.updateContextState();
         c.calcForcesAndEnergy(true, false, groups);
         const std::vector<Vec3>& f = c.forcesRef();
+        xn = x;
         for (int i = 0; i < n; ++i) {
             const double m = sys.getParticleMass(i);
             if (m == 0.0) continue;
             v[i] += f[i] * (stepSize / m);
-            x[i] += v[i] * stepSize;
+            xn[i] += v[i] * stepSize;
         }
+        if (constrained) {
+            shake(sys, x, xn, constraintTolerance);
+            for (int i = 0; i < n; ++i)
+                if (sys.getParticleMass(i) != 0.0) v[i] = (xn[i] - x[i]) * (1.0 / stepSize);
+        }
+        x.swap(xn);
         c.setTime(c.getTime() + stepSize);
     }
 }
@@ -375,23 +421,31 @@ void LangevinIntegrator::step(int steps) {
     if (!context) throw OpenMMException("This Integrator is not bound to a context!");
     ContextImpl& c = *context;
     const System& sys = c.getSystem();
-    requireNoConstraints(sys);
     const int n = sys.getNumParticles();
+    const bool constrained = sys.getNumConstraints() > 0;
     std::vector<Vec3>& x = c.positionsRef();
     std::vector<Vec3>& v = c.velocitiesRef();
     const double dt = stepSize, a = std::exp(-friction * dt), kT = kBoltz * temperature;
     const double fscale = friction > 0.0 ? (1.0 - a) / friction : dt;
+    std::vector<Vec3> xn;
     for (int s = 0; s < steps; ++s) {
         StepNormals R(seed, drawn++);
         c.updateContextState();
         c.calcForcesAndEnergy(true, false, groups);
         const std::vector<Vec3>& f = c.forcesRef();
+        xn = x;
         for (int i = 0; i < n; ++i) {
             const double m = sys.getParticleMass(i);
             if (m == 0.0) continue;
             v[i] = v[i] * a + f[i] * (fscale / m) + R() * std::sqrt(kT * (1.0 - a * a) / m);
-            x[i] += v[i] * dt;
+            xn[i] += v[i] * dt;
         }
+        if (constrained) {
+            shake(sys, x, xn, constraintTolerance);
+            for (int i = 0; i < n; ++i)
+                if (sys.getParticleMass(i) != 0.0) v[i] = (xn[i] - x[i]) * (1.0 / dt);
+        }
+        x.swap(xn);
         c.setTime(c.getTime() + dt);
     }
 }

@@ -228,6 +228,12 @@ PYBIND11_MODULE(_atmmetaforce, m) {
         .def("getParticleMass", &System::getParticleMass)
         .def("getNumConstraints", &System::getNumConstraints)
         .def("addConstraint", &System::addConstraint)
+        .def("getConstraintParameters", [](const System& s, int i) {
+            int a, b;
+            double d;
+            s.getConstraintParameters(i, a, b, d);
+            return py::make_tuple(a, b, d);
+        })
         .def("getNumForces", &System::getNumForces)
         // the System takes ownership of the force, as in OpenMM
         .def("addForce", [](System& s, Force* f) { return s.addForce(f); }, py::keep_alive<1, 2>())
@@ -246,6 +252,8 @@ PYBIND11_MODULE(_atmmetaforce, m) {
     py::class_<Integrator>(m, "Integrator")
         .def("step", &Integrator::step)
         .def("getStepSize", &Integrator::getStepSize)
+        .def("getConstraintTolerance", &Integrator::getConstraintTolerance)
+        .def("setConstraintTolerance", &Integrator::setConstraintTolerance)
         .def("setStepSize", &Integrator::setStepSize)
         .def("getIntegrationForceGroups", &Integrator::getIntegrationForceGroups)
         .def("setIntegrationForceGroups", [](Integrator& i, py::object g) { i.setIntegrationForceGroups(groupMask(g)); });

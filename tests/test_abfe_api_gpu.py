@@ -4,7 +4,7 @@ PME NonbondedForce (amber.create_system, the terms of
 AmberPrmtopFile.createSystem(PME, 1 nm, HBonds)), ATMMetaForceUtils restraints,
 the NonbondedForce moved to group 1 and made the ATM variable group, a
 disabled barostat, a Langevin integrator over groups {0, 2}, and the goldens
-(:147-150).  Coordinates and box come from the reference's State checkpoint
+(:147-150); then abfe.py's MD loop with the hydrogen-bond constraints.  Coordinates and box come from the reference's State checkpoint
 (temoa-g1-equil.xml, through the committed fixture) instead of loadState."""
 from __future__ import annotations
 
@@ -84,11 +84,28 @@ def test_pme_nonbonded_group_alone_and_forces(gpu):
     assert np.abs(st0.getForces(asNumpy=True) - Fb).max() < 1e-5
 
 
-def test_integrators_refuse_constrained_systems(gpu):
-    """createSystem(constraints=HBonds) adds 8,324 hydrogen-bond constraints;
-    the integrators here do not apply constraints and say so instead of
-    integrating unconstrained bonds.  (Energies and forces, as in test_abfe.py,
-    do not involve them.)"""
-    ctx, force, integ, s, x = build_abfe_context()
-    with pytest.raises(Exception, match="does not apply constraints"):
-        integ.step(1)
+def test_abfe_md_with_hbond_constraints(gpu):
+    """abfe.py's MD loop on the temoa-g1 system: LangevinIntegrator (300 K,
+    0.5/ps, 1 fs) over groups {0, ATM} with createSystem's 8,324 hydrogen-bond
+    constraints applied by SHAKE; a second run with the same seed retraces
+    the first."""
+    finals = []
+    for _ in range(2):
+        ctx, force, integ, s, x = build_abfe_context()
+        integ.setRandomNumberSeed(5)
+        for _ in range(4):
+            integ.step(25)
+            st = ctx.getState(getEnergy=True, groups={0, 2})
+            assert np.isfinite(st.getPotentialEnergy())
+            # the binding energy fluctuates by tens of kJ/mol in MD; a broken force would blow it up
+            assert abs(force.getPerturbationEnergy(ctx) - temoa.GOLDEN_PERTURBATION) < 250.0
+        ke = ctx.getState(getEnergy=True).getKineticEnergy()
+        dof = 3 * s.num_atoms - len(s.constraints)
+        temperature = 2.0 * ke / (dof * 0.00831446261815324)
+        assert 10.0 < temperature < 600.0, temperature   # started at 0 K, thermostat at 300 K
+        xs = ctx.getState(getPositions=True).getPositions(asNumpy=True)
+        d = np.linalg.norm(xs[s.constraints[:, 0]] - xs[s.constraints[:, 1]], axis=1)
+        assert np.abs(d / s.constraint_lengths - 1.0).max() < 2e-5
+        assert not np.array_equal(xs, x)
+        finals.append(xs)
+    assert np.array_equal(finals[0], finals[1])
