@@ -189,3 +189,27 @@ def test_pme_parameter_and_box_changes_between_evaluations(gpu):
                                           alpha=f.info()[0], grid=f.info()[1])
             assert abs(r["energy"] - ref["W"]) <= E_REL * abs(ref["W"]), (params, box, r["energy"], ref["W"])
             assert np.abs(F - Fref).max() < F_ABS
+
+
+def test_pme_dense_cell_takes_the_capacity_retry(gpu):
+    """Most atoms packed into one cell exceed the first per-cell capacity
+    (3x the mean + 64); the evaluation detects it, widens the capacity and
+    returns the right result."""
+    rng = np.random.default_rng(21)
+    n, L = 400, np.array([3.3, 3.3, 3.3])
+    x = rng.uniform(0.0, L, (n, 3))
+    g = int(math.ceil(250 ** (1 / 3)))
+    lat = np.array([(i, j, k) for i in range(g) for j in range(g) for k in range(g)][:250], dtype=float)
+    x[:250] = 0.1 + lat * (0.9 / g)          # 250 atoms in the first 1.1 nm cell, >= 0.14 nm apart
+    q = rng.normal(0, 0.3, n)
+    q -= q.mean()
+    sig = np.full(n, 0.12)
+    eps = np.full(n, 0.2)
+    pairs = np.zeros((0, 2), dtype=np.int64)
+    d = np.zeros((n, 3))
+    d[300:310] = 0.5
+    with pme.ATMPMEForce(q, sig, eps, d, L, 1.0, params=PARAMS) as f:
+        r, F, _ = f.evaluate(x)
+    ref, Fref, _, _ = _oracle_atm(x, q, sig, eps, L, pairs, [], [], [], d, PARAMS)
+    assert abs(r["state1_energy"] - ref["U1"]) <= E_REL * abs(ref["U1"])
+    assert np.abs(F - Fref).max() < F_ABS
