@@ -11,11 +11,13 @@
 //                    cell order
 //   k_pme_direct     erfc(alpha r)/r Coulomb + LJ over minimum-image pairs with
 //                    r < rc that are not exceptions; one wave per (64 atoms of
-//                    a cell, one of the 27 neighbour cells); every pair is
-//                    visited from both atoms (no j-force traffic), half the
-//                    energy each
-//   k_pme_spread     order-5 B-spline charge spreading, 64-bit fixed point
-//   k_pme_dft        one axis of the 3-D complex DFT (run 3x forward, 3x back)
+//                    a cell, the cell or one of its 13 forward neighbours):
+//                    each pair once
+//   k_pme_spread_tiles  order-5 B-spline charge spreading per cell into an LDS
+//                    tile of the grid, 64-bit fixed point (k_pme_spread: the
+//                    same straight into the grid, when a tile does not fit)
+//   k_pme_dft_lines  one axis of the 3-D complex DFT, lines staged in LDS (run
+//                    3x forward, 3x back; k_pme_dft for lines too long for LDS)
 //   k_pme_convolve   influence function, reciprocal energy, convolution
 //   k_pme_gather     reciprocal forces from the potential grid
 //   k_pme_exceptions exception pairs: own Coulomb + LJ, minus erf(alpha r)/r
