@@ -201,3 +201,103 @@ class ATMReplicaExchange:
 
     def __exit__(self, *a):
         self.close()
+
+
+class ContextReplicaExchange:
+    """Replica exchange over lambda-states for replicas that are separate
+    Contexts of the OpenMM-compatible API (one per replica), as the
+    reference's asynchronous driver runs them (README.md:179-201): any
+    variable group works, e.g. the PME NonbondedForce of an AMBER system.
+
+    replicas: [(context, integrator, atm_force)], each force the ATMMetaForce of
+    that context's System (all in force group `atm_group`, default the first
+    force's group); states: ATMParams (or dicts) of the schedule, replica k
+    starting in state k; `lambdas` label the sample lines.  Exchanges use the
+    rule of ATMReplicaExchange (Metropolis on the softplus bias of u_sc,
+    neighbouring states, random order)."""
+
+    _NAMES = ("Lambda1", "Lambda2", "Alpha", "U0", "W0", "Umax", "Ubcore", "Acore", "Direction")
+
+    def __init__(self, replicas, states, temperature: float = 300.0, atm_group: int | None = None,
+                 lambdas=None, seed: int = 1):
+        if len(replicas) != len(states) or len(states) < 1:
+            raise ValueError("one replica per state")
+        self.replicas = [(c, i) for c, i, _ in replicas]
+        self.forces = [f for _, _, f in replicas]
+        force = self.forces[0]
+        self.states = [p if isinstance(p, ATMParams) else ATMParams(**p) for p in states]
+        if len({p.direction > 0 for p in self.states}) != 1:
+            raise ValueError("all states must have the same direction")
+        self.lambdas = list(lambdas) if lambdas is not None else [0.5 * (p.lambda1 + p.lambda2) for p in self.states]
+        self.temperature = float(temperature)
+        self.beta = 1.0 / (KB * self.temperature)
+        self.atm_group = force.getForceGroup() if atm_group is None else atm_group
+        self.state_of = list(range(len(self.states)))
+        self.rng = np.random.default_rng(seed)
+        n = max(len(self.states) - 1, 1)
+        self.attempts = np.zeros(n, dtype=np.int64)
+        self.accepts = np.zeros(n, dtype=np.int64)
+        self.cycle = 0
+        for a, (ctx, _) in enumerate(self.replicas):
+            self._apply(ctx, self.states[a])
+
+    def _apply(self, ctx, p: ATMParams):
+        for name, v in zip(self._NAMES, p.as_tuple()):
+            ctx.setParameter(getattr(self.forces[0], name)(), float(v))
+
+    def _perturbation_energies(self):
+        u = []
+        for (ctx, _), f in zip(self.replicas, self.forces):
+            ctx.getState(getEnergy=True, groups={self.atm_group})   # evaluates at the current positions
+            u.append(f.getPerturbationEnergy(ctx))
+        return u
+
+    def step(self, steps: int):
+        for _, integ in self.replicas:
+            integ.step(int(steps))
+
+    def exchange(self, sweeps: int = 1) -> int:
+        R = len(self.states)
+        if R < 2:
+            return 0
+        u = self._perturbation_energies()
+        replica_in = {s: a for a, s in enumerate(self.state_of)}
+        swapped = 0
+        for _ in range(sweeps):
+            for i in self.rng.permutation(R - 1):
+                a, b = replica_in[i], replica_in[i + 1]
+                self.attempts[i] += 1
+                lr = exchange_log_ratio(self.beta, self.states[i], self.states[i + 1], u[a], u[b])
+                if lr >= 0.0 or self.rng.random() < math.exp(lr):
+                    self.accepts[i] += 1
+                    replica_in[i], replica_in[i + 1] = b, a
+                    swapped += 1
+        for s, a in replica_in.items():
+            if self.state_of[a] != s:
+                self.state_of[a] = s
+                self._apply(self.replicas[a][0], self.states[s])
+        return swapped
+
+    def run(self, cycles: int, steps_per_cycle: int, exchange: bool = True, sweeps: int = 1, out=None):
+        """`cycles` x (MD steps, one sample line per replica, exchanges)."""
+        samples = []
+        for _ in range(cycles):
+            self.step(steps_per_cycle)
+            self.cycle += 1
+            for a, (ctx, _) in enumerate(self.replicas):
+                st = ctx.getState(getEnergy=True, groups={0, self.atm_group})
+                s = self.state_of[a]
+                smp = Sample(self.cycle, a, s, self.temperature, self.states[s], self.lambdas[s],
+                             st.getPotentialEnergy(), self.forces[a].getPerturbationEnergy(ctx),
+                             st.getKineticEnergy())
+                samples.append(smp)
+                if out is not None:
+                    print(smp.out_line(), file=out)
+            if exchange:
+                self.exchange(sweeps)
+        if out is not None:
+            out.flush()
+        return samples
+
+    def acceptance(self) -> np.ndarray:
+        return np.where(self.attempts > 0, self.accepts / np.maximum(self.attempts, 1), 0.0)

@@ -109,3 +109,25 @@ def test_abfe_md_with_hbond_constraints(gpu):
         assert not np.array_equal(xs, x)
         finals.append(xs)
     assert np.array_equal(finals[0], finals[1])
+
+
+def test_replica_exchange_over_pme_contexts(gpu):
+    """Three lambda-states of temoa-g1 as three contexts (the reference's
+    asynchronous-driver layout), MD cycles with exchanges: every context
+    ends up holding its assigned state's parameters and the sample lines
+    follow abfe.py's .out format."""
+    from atmmetaforce import replica
+    reps = [build_abfe_context() for _ in range(3)]
+    p = temoa.ATM_PARAMS
+    states = [dict(p, lambda1=l, lambda2=l, direction=1.0) for l in (0.0, 0.25, 0.5)]
+    re = replica.ContextReplicaExchange([(r[0], r[2], r[1]) for r in reps], states, temperature=300.0, seed=4)
+    lines = re.run(cycles=2, steps_per_cycle=5)
+    assert sorted(re.state_of) == [0, 1, 2]
+    for a, (ctx, f, integ, s, x) in enumerate(reps):
+        st = states[re.state_of[a]]
+        assert ctx.getParameter(f.Lambda1()) == st["lambda1"] and ctx.getParameter(f.Lambda2()) == st["lambda2"]
+    assert len(lines) == 6
+    for smp in lines:
+        cols = smp.out_line().split()
+        assert len(cols) == 9 and np.isfinite([float(c) for c in cols]).all()
+    assert re.attempts.sum() == 4

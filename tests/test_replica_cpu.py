@@ -68,3 +68,60 @@ def test_metropolis_exchanges_sample_the_assignment_distribution():
     for perm, wt in w.items():
         assert counts.get(perm, 0) / total == pytest.approx(wt / Z, abs=0.01)
     assert 0 < re.acceptance().min() <= re.acceptance().max() <= 1
+
+
+def test_context_replica_exchange_applies_states_and_samples():
+    """ContextReplicaExchange on stand-in contexts (the API it drives:
+    setParameter, getState, the force's parameter names and
+    getPerturbationEnergy): each exchange leaves every context holding the
+    parameters of its assigned state, and the assignments sample the
+    Boltzmann weights of the softplus biases."""
+    import itertools
+    T = 300.0
+    beta = 1 / (replica.KB * T)
+    states = [W.ATMParams(0.0, 0.0, 0.1, 50.0, 0.0), W.ATMParams(0.2, 0.3, 0.1, 50.0, 0.0),
+              W.ATMParams(0.4, 0.5, 0.1, 50.0, 0.0)]
+    u = [2.0, 5.0, 9.0]
+    names = replica.ContextReplicaExchange._NAMES
+
+    class Force:
+        def getForceGroup(self):
+            return 2
+
+        def getPerturbationEnergy(self, ctx):
+            return ctx.u
+
+    for n in names:
+        setattr(Force, n, staticmethod(lambda n=n: "ATM" + n))
+
+    class Ctx:
+        def __init__(self, u):
+            self.u, self.p = u, {}
+
+        def setParameter(self, k, v):
+            self.p[k] = v
+
+        def getState(self, getEnergy=False, groups=None):
+            return types.SimpleNamespace(getPotentialEnergy=lambda: 0.0, getKineticEnergy=lambda: 0.0)
+
+    class Integ:
+        def step(self, n):
+            pass
+
+    ctxs = [Ctx(x) for x in u]
+    re = replica.ContextReplicaExchange([(c, Integ(), Force()) for c in ctxs], states, temperature=T, seed=3)
+    counts = {}
+    for _ in range(40000):
+        re.exchange(1)
+        for a, c in enumerate(ctxs):
+            p = states[re.state_of[a]].as_tuple()
+            assert all(c.p["ATM" + n] == v for n, v in zip(names, p))
+        key = tuple(re.state_of)
+        counts[key] = counts.get(key, 0) + 1
+    w = {perm: math.exp(-beta * sum(replica.softplus_bias(states[s], u[a]) for a, s in enumerate(perm)))
+         for perm in itertools.permutations(range(3))}
+    Z = sum(w.values())
+    for perm, wt in w.items():
+        assert counts.get(perm, 0) / 40000 == pytest.approx(wt / Z, abs=0.012)
+    lines = re.run(1, 10, exchange=False)
+    assert len(lines) == 3 and all(len(s.out_line().split()) == 9 for s in lines)
