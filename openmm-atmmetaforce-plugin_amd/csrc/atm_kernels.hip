@@ -542,8 +542,10 @@ __device__ __forceinline__ void flush_force(const DevPtrs& P, int c, int slot, f
 // way; forces are flushed to 64-bit fixed point, energies summed in FP64.
 // List entries are fetched 64 at a time (one per lane) and broadcast with
 // readlane, so the cluster mask and the diagonal test are scalar branches.
+// At most 80 VGPRs (6 waves per SIMD): the FP64 forces variant fits without
+// spills and hides more of its load latency than at its unconstrained 82.
 template <int W, bool FORCES, bool F64>
-__global__ void __launch_bounds__(64 * W) k_pairs(DevPtrs P, int setBegin) {
+__global__ void __launch_bounds__(64 * W, 6) k_pairs(DevPtrs P, int setBegin) {
     const int c = setBegin + blockIdx.y;
     const SetDesc D = P.desc[c];  // by value: lives in SGPRs for the whole kernel
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -624,10 +626,13 @@ __global__ void __launch_bounds__(64 * W) k_pairs(DevPtrs P, int setBegin) {
                     dz = min_image(dz, Lz, invLz);
                     float fs;
                     if (F64) {
-                        // FP32 r^2 with a margin: a superset of the in-cutoff pairs, decides the early-out
+                        // FP32 r^2 with a margin: a superset of the in-cutoff pairs, decides the early-out.
+                        // The lane masks stay in scalar registers (ballot / inverse ballot).
                         const float r2f = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-                        const bool cand = r2f < rc2hi && (okm & (1u << t)) != 0u;
-                        if (__builtin_amdgcn_ballot_w64(cand) == 0ull) continue;  // no pair of this 8x8 step is inside
+                        const uint64_t okLanes = __builtin_amdgcn_ballot_w64((okm & (1u << t)) != 0u);
+                        const uint64_t candLanes = __builtin_amdgcn_ballot_w64(r2f < rc2hi) & okLanes;
+                        if (candLanes == 0ull) continue;  // no pair of this 8x8 step is inside
+                        const bool cand = __builtin_amdgcn_inverse_ballot_w64(candLanes);
                         const double4 id = sPrm64[wave][t * kClusterSize + ii];
                         // exact: the squares of 2^-20-grid differences fit in 53 bits
                         const double r2 = fma((double)dx, (double)dx, fma((double)dy, (double)dy, (double)dz * (double)dz));
