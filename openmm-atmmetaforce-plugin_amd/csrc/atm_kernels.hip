@@ -544,9 +544,8 @@ __device__ __forceinline__ void flush_force(const DevPtrs& P, int c, int slot, f
 // readlane, so the cluster mask and the diagonal test are scalar branches.
 // At most 72 VGPRs (7 waves per SIMD): the FP64 forces variant then spills a
 // few per-batch values to scratch and still runs faster on a 22 x 23k batch
-// (0.660 ms per launch, against 0.677 ms at 6 waves without spills and about
-// 0.70 ms at its unconstrained 82 VGPRs); at 8 waves it spills inside the
-// entry loop and takes 0.80 ms.
+// (0.660 ms per launch, against 0.677 ms at 6 waves without spills); at 8
+// waves it spills inside the entry loop and takes 0.80 ms.
 template <int W, bool FORCES, bool F64>
 __global__ void __launch_bounds__(64 * W, 7) k_pairs(DevPtrs P, int setBegin) {
     const int c = setBegin + blockIdx.y;
