@@ -176,6 +176,33 @@ __device__ __forceinline__ PairE64 pair_term64(bool in, double r2, double sig, d
     return o;
 }
 
+// The same FP64 pair function returning the whole pair energy,
+// E_LJ + q'q' (1/r + k_rf r^2 - c_rf), so one FP64 sum per lane suffices.
+struct PairEF64 {
+    double fs, e;
+};
+
+__device__ __forceinline__ PairEF64 pair_ef64(bool in, double r2, double sig, double eps4, double qq,
+                                             double krf, double crf) {
+    double y = __builtin_amdgcn_rsq(r2);
+    {
+        const double t = r2 * y;
+        const double e = fma(-t, y, 1.0);
+        y = fma(0.5 * y, e, y);
+    }
+    const double r2inv = y * y;
+    const double s2 = sig * sig * r2inv;
+    const double s6 = s2 * s2 * s2;
+    const double t = eps4 * s6;
+    const double elj = fma(t, s6, -t);
+    const double flj = fma(12.0, elj, 6.0 * t);
+    const double fc = qq * fma(-2.0 * krf, r2, y);
+    PairEF64 o;
+    o.fs = in ? (flj + fc) * r2inv : 0.0;
+    o.e = in ? fma(qq, fma(krf, r2, y) - crf, elj) : 0.0;
+    return o;
+}
+
 __device__ __forceinline__ uint64_t lanemask_lt() {
     const int lane = threadIdx.x & 63;
     return lane == 0 ? 0ull : (~0ull >> (64 - lane));
@@ -542,12 +569,13 @@ __device__ __forceinline__ void flush_force(const DevPtrs& P, int c, int slot, f
 // way; forces are flushed to 64-bit fixed point, energies summed in FP64.
 // List entries are fetched 64 at a time (one per lane) and broadcast with
 // readlane, so the cluster mask and the diagonal test are scalar branches.
-// At most 72 VGPRs (7 waves per SIMD): the FP64 forces variant then spills a
-// few per-batch values to scratch and still runs faster on a 22 x 23k batch
-// (0.660 ms per launch, against 0.677 ms at 6 waves without spills); at 8
-// waves it spills inside the entry loop and takes 0.80 ms.
+// At most 80 VGPRs (6 waves per SIMD): the FP64 forces variant with its two
+// j-atom buffers then spills a few per-batch values outside the entry loop;
+// on a 22 x 23k batch it takes 0.625 ms per launch, against 0.654 ms at its
+// unconstrained 88 VGPRs (5 waves) and 0.712 ms at 72 VGPRs (7 waves, spills
+// inside the entry loop).
 template <int W, bool FORCES, bool F64>
-__global__ void __launch_bounds__(64 * W, 7) k_pairs(DevPtrs P, int setBegin) {
+__global__ void __launch_bounds__(64 * W, 6) k_pairs(DevPtrs P, int setBegin) {
     const int c = setBegin + blockIdx.y;
     const SetDesc D = P.desc[c];  // by value: lives in SGPRs for the whole kernel
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -585,83 +613,111 @@ __global__ void __launch_bounds__(64 * W, 7) k_pairs(DevPtrs P, int setBegin) {
     const float Lz = D.L[2], invLz = D.invL[2];
     const float Lx = D.L[0], invLx = D.invL[0], Ly = D.L[1], invLy = D.invL[1];
     const bool shiftOK = D.shiftOK;
-    const double rc2 = D.rc2, krf64 = D.krf64;
+    const double rc2 = D.rc2, krf64 = D.krf64, crf64 = D.crf64;
     const float krf = D.krf, rc2hi = D.rc2hi;
     float fi[kGroupClusters][3];
 #pragma unroll
     for (int t = 0; t < kGroupClusters; ++t) fi[t][0] = fi[t][1] = fi[t][2] = 0.f;
-    double eE = 0.0, eQQ = 0.0, eQR2 = 0.0;  // sum(E_LJ + qq/r), sum(qq), sum(qq r^2)
+    // FP64: eE sums whole pair energies; FP32: eE = sum(E_LJ + qq/r), eQQ = sum(qq), eQR2 = sum(qq r^2)
+    double eE = 0.0, eQQ = 0.0, eQR2 = 0.0;
     const int ne = __builtin_amdgcn_readfirstlane(P.gcount[D.groupOff + g]);
     const uint32_t* E = P.entries + D.entryOff + (long long)g * D.entryCap;
+    // This lane's j-atom of one list entry.  Two of these are in flight: entry
+    // k + 1's loads are issued before entry k is evaluated, into the other
+    // buffer, so the wait for them falls after entry k's work (and not behind
+    // entry k's force atomics, which a single rotating buffer would wait for).
+    struct JBuf {
+        float4 p;
+        double4 d;
+        float2 q;
+        int idx;
+    };
+    auto load_jbuf = [&](uint32_t ent) {
+        JBuf b;
+        const int js = D.slotOff + (int)(ent & 0xFFFFFu) * kClusterSize + jj;
+        b.p = P.spos[js];
+        b.d = make_double4(0.0, 0.0, 0.0, 0.0);
+        b.q = make_float2(0.f, 0.f);
+        if (F64) b.d = P.sprm64[js]; else b.q = P.sprm[js];
+        // with explicit x/y image shifts, dummy slots sit far outside the box
+        // (k_gather_bbox), so the FP64 path needs no validity test there
+        b.idx = (F64 && shiftOK) ? 0 : P.sidx[js];
+        return b;
+    };
+    auto do_entry = [&](uint32_t ent, const JBuf& b) __attribute__((always_inline)) {
+        const int J = ent & 0xFFFFF;
+        const uint32_t imask = (ent >> 20) & 0xFFu;
+        const int js = D.slotOff + J * kClusterSize + jj;
+        float4 pj = b.p;
+        const bool jv = b.idx >= 0;
+        const float2 qj = b.q;
+        const double4 jd = b.d;
+        if (shiftOK) {
+            const uint32_t sxc = (ent >> 28) & 3u, syc = (ent >> 30) & 3u;
+            pj.x += sxc == 1u ? D.L[0] : (sxc == 2u ? -D.L[0] : 0.f);
+            pj.y += syc == 1u ? D.L[1] : (syc == 2u ? -D.L[1] : 0.f);
+        }
+        const int tdiag = J - g0;  // the i-cluster equal to J, if any
+        // bit t of okm: this lane's (t, ii) x jj pair may count (real atoms;
+        // i < j inside the diagonal cluster); computed once per entry
+        uint32_t okm = (F64 && shiftOK) ? 0xFFu : (jv ? ivm : 0u);
+        if (tdiag >= 0 && tdiag < kGroupClusters && ii >= jj) okm &= ~(1u << tdiag);
+        float fjx = 0.f, fjy = 0.f, fjz = 0.f;
+#pragma unroll
+        for (int t = 0; t < kGroupClusters; ++t) {
+            if (imask & (1u << t)) {
+                const float4 pi = sPos[wave][t * kClusterSize + ii];
+                float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+                if (!shiftOK) { dx = min_image(dx, Lx, invLx); dy = min_image(dy, Ly, invLy); }
+                dz = min_image(dz, Lz, invLz);
+                float fs;
+                if (F64) {
+                    // FP32 r^2 with a margin: a superset of the in-cutoff pairs, decides the early-out.
+                    // The lane masks stay in scalar registers (ballot / inverse ballot).
+                    const float r2f = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                    const uint64_t okLanes = __builtin_amdgcn_ballot_w64((okm & (1u << t)) != 0u);
+                    const uint64_t candLanes = __builtin_amdgcn_ballot_w64(r2f < rc2hi) & okLanes;
+                    if (candLanes == 0ull) continue;  // no pair of this 8x8 step is inside
+                    const bool cand = __builtin_amdgcn_inverse_ballot_w64(candLanes);
+                    const double4 id = sPrm64[wave][t * kClusterSize + ii];
+                    // exact: the squares of 2^-20-grid differences fit in 53 bits
+                    const double r2 = fma((double)dx, (double)dx, fma((double)dy, (double)dy, (double)dz * (double)dz));
+                    const bool in = cand && r2 < rc2;
+                    const PairEF64 pe = pair_ef64(in, r2, id.x + jd.x, id.y * jd.y, id.z * jd.z, krf64, crf64);
+                    eE += pe.e;
+                    fs = (float)pe.fs;
+                } else {
+                    const bool ok = (okm & (1u << t)) != 0u;
+                    const float2 qi = sPrm[wave][t * kClusterSize + ii];
+                    const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                    const bool in = ok && in_cutoff(D, dx, dy, dz, r2);
+                    if (__ballot(in) == 0ull) continue;
+                    const PairE pe = pair_term(in, r2, pi.w * pj.w, qi.x + qj.x, qi.y * qj.y, krf);
+                    eE += (double)(pe.elj + pe.qr); eQQ += (double)pe.qq; eQR2 += (double)pe.qr2;
+                    fs = pe.fs;
+                }
+                if (FORCES) {
+                    const float fx = fs * dx, fy = fs * dy, fz = fs * dz;
+                    fi[t][0] += fx; fi[t][1] += fy; fi[t][2] += fz;
+                    fjx -= fx; fjy -= fy; fjz -= fz;
+                }
+            }
+        }
+        if (FORCES) {
+            fjx = oct_sum(fjx); fjy = oct_sum(fjy); fjz = oct_sum(fjz);
+            if (ii == 0 && jv && (fjx != 0.f || fjy != 0.f || fjz != 0.f)) flush_force(P, c, js, fjx, fjy, fjz);
+        }
+    };
     for (int e0 = 0; e0 < ne; e0 += 64) {
         const uint32_t ebuf = (e0 + lane < ne) ? E[e0 + lane] : 0u;
         const int nb = min(64, ne - e0);
-        for (int k = 0; k < nb; ++k) {
-            const uint32_t ent = __builtin_amdgcn_readlane(ebuf, k);
-            const int J = ent & 0xFFFFF;
-            const uint32_t imask = (ent >> 20) & 0xFFu;
-            const int js = D.slotOff + J * kClusterSize + jj;
-            float4 pj = P.spos[js];
-            // with explicit x/y image shifts, dummy slots sit far outside the box
-            // (k_gather_bbox), so the FP64 path needs no validity test there
-            const bool jv = (F64 && shiftOK) ? true : P.sidx[js] >= 0;
-            float2 qj = make_float2(0.f, 0.f);
-            double4 jd = make_double4(0.0, 0.0, 0.0, 0.0);
-            if (F64) jd = P.sprm64[js]; else qj = P.sprm[js];
-            if (shiftOK) {
-                const uint32_t sxc = (ent >> 28) & 3u, syc = (ent >> 30) & 3u;
-                pj.x += sxc == 1u ? D.L[0] : (sxc == 2u ? -D.L[0] : 0.f);
-                pj.y += syc == 1u ? D.L[1] : (syc == 2u ? -D.L[1] : 0.f);
-            }
-            const int tdiag = J - g0;  // the i-cluster equal to J, if any
-            // bit t of okm: this lane's (t, ii) x jj pair may count (real atoms;
-            // i < j inside the diagonal cluster); computed once per entry
-            uint32_t okm = (F64 && shiftOK) ? 0xFFu : (jv ? ivm : 0u);
-            if (tdiag >= 0 && tdiag < kGroupClusters && ii >= jj) okm &= ~(1u << tdiag);
-            float fjx = 0.f, fjy = 0.f, fjz = 0.f;
-#pragma unroll
-            for (int t = 0; t < kGroupClusters; ++t) {
-                if (imask & (1u << t)) {
-                    const float4 pi = sPos[wave][t * kClusterSize + ii];
-                    float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-                    if (!shiftOK) { dx = min_image(dx, Lx, invLx); dy = min_image(dy, Ly, invLy); }
-                    dz = min_image(dz, Lz, invLz);
-                    float fs;
-                    if (F64) {
-                        // FP32 r^2 with a margin: a superset of the in-cutoff pairs, decides the early-out.
-                        // The lane masks stay in scalar registers (ballot / inverse ballot).
-                        const float r2f = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-                        const uint64_t okLanes = __builtin_amdgcn_ballot_w64((okm & (1u << t)) != 0u);
-                        const uint64_t candLanes = __builtin_amdgcn_ballot_w64(r2f < rc2hi) & okLanes;
-                        if (candLanes == 0ull) continue;  // no pair of this 8x8 step is inside
-                        const bool cand = __builtin_amdgcn_inverse_ballot_w64(candLanes);
-                        const double4 id = sPrm64[wave][t * kClusterSize + ii];
-                        // exact: the squares of 2^-20-grid differences fit in 53 bits
-                        const double r2 = fma((double)dx, (double)dx, fma((double)dy, (double)dy, (double)dz * (double)dz));
-                        const bool in = cand && r2 < rc2;
-                        const PairE64 pe = pair_term64(in, r2, id.x + jd.x, id.y * jd.y, id.z * jd.z, krf64);
-                        eE += pe.e; eQQ += pe.qq; eQR2 += pe.qq * r2;
-                        fs = (float)pe.fs;
-                    } else {
-                        const bool ok = (okm & (1u << t)) != 0u;
-                        const float2 qi = sPrm[wave][t * kClusterSize + ii];
-                        const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-                        const bool in = ok && in_cutoff(D, dx, dy, dz, r2);
-                        if (__ballot(in) == 0ull) continue;
-                        const PairE pe = pair_term(in, r2, pi.w * pj.w, qi.x + qj.x, qi.y * qj.y, krf);
-                        eE += (double)(pe.elj + pe.qr); eQQ += (double)pe.qq; eQR2 += (double)pe.qr2;
-                        fs = pe.fs;
-                    }
-                    if (FORCES) {
-                        const float fx = fs * dx, fy = fs * dy, fz = fs * dz;
-                        fi[t][0] += fx; fi[t][1] += fy; fi[t][2] += fz;
-                        fjx -= fx; fjy -= fy; fjz -= fz;
-                    }
-                }
-            }
-            if (FORCES) {
-                fjx = oct_sum(fjx); fjy = oct_sum(fjy); fjz = oct_sum(fjz);
-                if (ii == 0 && jv && (fjx != 0.f || fjy != 0.f || fjz != 0.f)) flush_force(P, c, js, fjx, fjy, fjz);
+        JBuf bufA = load_jbuf(__builtin_amdgcn_readlane(ebuf, 0)), bufB;
+        for (int k = 0; k < nb; k += 2) {
+            if (k + 1 < nb) bufB = load_jbuf(__builtin_amdgcn_readlane(ebuf, k + 1));
+            do_entry(__builtin_amdgcn_readlane(ebuf, k), bufA);
+            if (k + 1 < nb) {
+                if (k + 2 < nb) bufA = load_jbuf(__builtin_amdgcn_readlane(ebuf, k + 2));
+                do_entry(__builtin_amdgcn_readlane(ebuf, k + 1), bufB);
             }
         }
     }
