@@ -562,6 +562,20 @@ __device__ __forceinline__ void flush_force(const DevPtrs& P, int c, int slot, f
     }
 }
 
+// Flush one force component of a slot (component k of slot `slot`): 64-bit
+// fixed point, or the FP64 side accumulator above 2^27 (clashing pairs).
+__device__ __forceinline__ void flush_component(const DevPtrs& P, int c, int slot, int k, float f) {
+    if (fabsf(f) < 134217728.0f) {  // 2^27: |f scale| < 2^51, the magic-number conversion is exact
+        atomic_add_fixed(P.sf + (long long)k * P.sfn + slot,
+                         __double_as_longlong((double)f * kForceScale + 6755399441055744.0) - 0x4338000000000000LL);
+    } else if (fabsf(f) < kFixedMax) {
+        atomic_add_fixed(P.sf + (long long)k * P.sfn + slot, to_fixed(f));
+    } else {
+        atomicAdd(P.sd + 3ll * slot + k, (double)f);
+        P.rt[c].big = 1;
+    }
+}
+
 // One wave64 per group of up to 8 i-clusters; lane = 8 jj + ii pairs i-atom ii
 // of each interacting i-cluster with j-atom jj of the list entry's j-cluster.
 // F64: scalar pair function in FP64 (exact r^2, default); otherwise FP32.
@@ -586,7 +600,8 @@ __global__ void __launch_bounds__(64 * W, 6) k_pairs(DevPtrs P, int setBegin) {
     __shared__ float2 sPrm[W][F64 ? 1 : 64];
     __shared__ double4 sPrm64[W][F64 ? 64 : 1];
     int g0 = 0, ncl = 0;
-    uint32_t ivm = 0;  // bit t: i-atom (t, ii) is a real atom
+    uint32_t ivm = 0;       // bit t: i-atom (t, ii) is a real atom
+    uint64_t ivLanes = 0;   // bit 8 t + ii: the same, over the group's 64 slots
     if (active) {
         const int4 gi = P.ginfo[D.groupOff + g];
         g0 = __builtin_amdgcn_readfirstlane(gi.x);
@@ -604,11 +619,18 @@ __global__ void __launch_bounds__(64 * W, 6) k_pairs(DevPtrs P, int setBegin) {
         sPos[wave][lane] = p;
         if (F64) sPrm64[wave][lane] = qd; else sPrm[wave][lane] = q;
         const uint64_t vb = __ballot(v);
+        ivLanes = vb;
 #pragma unroll
         for (int t = 0; t < kGroupClusters; ++t) ivm |= (uint32_t)((vb >> (t * kClusterSize + ii)) & 1ull) << t;
     }
     __syncthreads();
     if (!active) return;
+    // wave-uniform from here on: keep the group's values in scalar registers
+    g0 = __builtin_amdgcn_readfirstlane(g0);
+    ncl = __builtin_amdgcn_readfirstlane(ncl);
+    ivLanes = (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)ivLanes) |
+              (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(ivLanes >> 32)) << 32;
+    const uint64_t upperLanes = __builtin_amdgcn_ballot_w64(ii < jj);  // i < j inside a diagonal cluster
 
     const float Lz = D.L[2], invLz = D.invL[2];
     const float Lx = D.L[0], invLx = D.invL[0], Ly = D.L[1], invLy = D.invL[1];
@@ -662,6 +684,9 @@ __global__ void __launch_bounds__(64 * W, 6) k_pairs(DevPtrs P, int setBegin) {
         // i < j inside the diagonal cluster); computed once per entry
         uint32_t okm = (F64 && shiftOK) ? 0xFFu : (jv ? ivm : 0u);
         if (tdiag >= 0 && tdiag < kGroupClusters && ii >= jj) okm &= ~(1u << tdiag);
+        // the same as wave masks in scalar registers (FP64 path): lanes with a real
+        // j-atom (all of them with explicit shifts: dummies sit far away)
+        const uint64_t jvLanes = (F64 && shiftOK) ? ~0ull : __builtin_amdgcn_ballot_w64(jv);
         float fjx = 0.f, fjy = 0.f, fjz = 0.f;
 #pragma unroll
         for (int t = 0; t < kGroupClusters; ++t) {
@@ -675,7 +700,10 @@ __global__ void __launch_bounds__(64 * W, 6) k_pairs(DevPtrs P, int setBegin) {
                     // FP32 r^2 with a margin: a superset of the in-cutoff pairs, decides the early-out.
                     // The lane masks stay in scalar registers (ballot / inverse ballot).
                     const float r2f = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-                    const uint64_t okLanes = __builtin_amdgcn_ballot_w64((okm & (1u << t)) != 0u);
+                    // lanes whose i-atom (t, ii) is real: byte t of ivLanes repeated over the 8 jj
+                    uint64_t iv = (ivLanes >> (t * kClusterSize)) & 0xFFull;
+                    iv |= iv << 8; iv |= iv << 16; iv |= iv << 32;
+                    const uint64_t okLanes = (t == tdiag ? upperLanes : ~0ull) & jvLanes & iv;
                     const uint64_t candLanes = __builtin_amdgcn_ballot_w64(r2f < rc2hi) & okLanes;
                     if (candLanes == 0ull) continue;  // no pair of this 8x8 step is inside
                     const bool cand = __builtin_amdgcn_inverse_ballot_w64(candLanes);
@@ -704,8 +732,10 @@ __global__ void __launch_bounds__(64 * W, 6) k_pairs(DevPtrs P, int setBegin) {
             }
         }
         if (FORCES) {
+            // every lane of a jj octet holds the sums; lanes ii = 0, 1, 2 flush x, y, z
             fjx = oct_sum(fjx); fjy = oct_sum(fjy); fjz = oct_sum(fjz);
-            if (ii == 0 && jv && (fjx != 0.f || fjy != 0.f || fjz != 0.f)) flush_force(P, c, js, fjx, fjy, fjz);
+            const float fk = ii == 0 ? fjx : (ii == 1 ? fjy : fjz);
+            if (ii < 3 && jv && fk != 0.f) flush_component(P, c, js, ii, fk);
         }
     };
     for (int e0 = 0; e0 < ne; e0 += 64) {
