@@ -562,6 +562,32 @@ __device__ __forceinline__ void flush_force(const DevPtrs& P, int c, int slot, f
     }
 }
 
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Add one force component (component k of slot `slot`) with an atomic that
+// every lane issues on every path, so the number of memory operations between
+// a load and its use does not depend on the data; values beyond the magic-number
+// range (|f| >= 2^27, clashing pairs) add zero there and take a second, rare
+// path: the general conversion, or the FP64 side accumulator above 2^31.
+__device__ __forceinline__ void flush_component_u(const DevPtrs& P, int c, int slot, int k, float f) {
+    const bool small = fabsf(f) < 134217728.0f;
+    long long* dst = P.sf + (long long)k * P.sfn + slot;
+    atomic_add_fixed(dst, small ? __double_as_longlong((double)f * kForceScale + 6755399441055744.0) - 0x4338000000000000LL
+                                : 0ll);
+    if (!small) {
+        if (fabsf(f) < kFixedMax) {
+            atomic_add_fixed(dst, to_fixed(f));
+        } else {
+            atomicAdd(P.sd + 3ll * slot + k, (double)f);
+            P.rt[c].big = 1;
+        }
+    }
+}
+
 // Flush one force component of a slot (component k of slot `slot`): 64-bit
 // fixed point, or the FP64 side accumulator above 2^27 (clashing pairs).
 __device__ __forceinline__ void flush_component(const DevPtrs& P, int c, int slot, int k, float f) {
@@ -666,6 +692,24 @@ __global__ void __launch_bounds__(64 * W, 6) k_pairs(DevPtrs P, int setBegin) {
         b.idx = (F64 && shiftOK) ? 0 : P.sidx[js];
         return b;
     };
+    // j-force sums of the last kJFlush entries, flushed together: 3 dense rounds
+    // of one atomic per lane instead of one sparse atomic per entry
+    constexpr int kJFlush = 8;
+    __shared__ float sJF[W][kJFlush][kClusterSize * 3];  // [entry][jj * 3 + component]
+    __shared__ int sJS[W][kJFlush];                       // the entry's first j-slot
+    int ecount = 0;
+    auto flush_j = [&](int cnt) __attribute__((always_inline)) {
+        wave_sync();  // the octets' writes are visible to every lane of the wave
+        int ln = lane;
+        asm volatile("" : "+v"(ln));  // index math stays here, not hoisted into registers held across the loop
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            const int v = r * 64 + ln;  // entry v / 24, then jj * 3 + component
+            const int pe = v / (kClusterSize * 3), rem = v - pe * (kClusterSize * 3);
+            if (pe < cnt) flush_component_u(P, c, sJS[wave][pe] + rem / 3, rem % 3, sJF[wave][pe][rem]);
+        }
+        wave_sync();  // read before the next entries overwrite them
+    };
     auto do_entry = [&](uint32_t ent, const JBuf& b) __attribute__((always_inline)) {
         const int J = ent & 0xFFFFF;
         const uint32_t imask = (ent >> 20) & 0xFFu;
@@ -732,25 +776,34 @@ __global__ void __launch_bounds__(64 * W, 6) k_pairs(DevPtrs P, int setBegin) {
             }
         }
         if (FORCES) {
-            // every lane of a jj octet holds the sums; lanes ii = 0, 1, 2 flush x, y, z
+            // every lane of a jj octet holds the sums; lanes ii = 0, 1, 2 keep x, y, z
+            // (zero for dummy j-slots, whose accumulators nothing reads)
             fjx = oct_sum(fjx); fjy = oct_sum(fjy); fjz = oct_sum(fjz);
             const float fk = ii == 0 ? fjx : (ii == 1 ? fjy : fjz);
-            if (ii < 3 && jv && fk != 0.f) flush_component(P, c, js, ii, fk);
+            const int pe = ecount & (kJFlush - 1);
+            if (ii < 3) sJF[wave][pe][jj * 3 + ii] = fk;
+            if (lane == 0) sJS[wave][pe] = js - jj;
+            if (pe == kJFlush - 1) flush_j(kJFlush);
         }
+        ++ecount;
     };
+    // Entries are taken in pairs without conditions, so that the loads in flight
+    // between a buffer's load and its use are the same on every path.  Past the
+    // list end the batch holds entry 0 (cluster 0, empty i-mask): no pair step
+    // runs and its j sums are zero; the load after a batch's last pair wraps to
+    // lane 0 and is not used.
     for (int e0 = 0; e0 < ne; e0 += 64) {
         const uint32_t ebuf = (e0 + lane < ne) ? E[e0 + lane] : 0u;
         const int nb = min(64, ne - e0);
         JBuf bufA = load_jbuf(__builtin_amdgcn_readlane(ebuf, 0)), bufB;
         for (int k = 0; k < nb; k += 2) {
-            if (k + 1 < nb) bufB = load_jbuf(__builtin_amdgcn_readlane(ebuf, k + 1));
+            bufB = load_jbuf(__builtin_amdgcn_readlane(ebuf, k + 1));
             do_entry(__builtin_amdgcn_readlane(ebuf, k), bufA);
-            if (k + 1 < nb) {
-                if (k + 2 < nb) bufA = load_jbuf(__builtin_amdgcn_readlane(ebuf, k + 2));
-                do_entry(__builtin_amdgcn_readlane(ebuf, k + 1), bufB);
-            }
+            bufA = load_jbuf(__builtin_amdgcn_readlane(ebuf, (k + 2) & 63));
+            do_entry(__builtin_amdgcn_readlane(ebuf, k + 1), bufB);
         }
     }
+    if (FORCES && (ecount & (kJFlush - 1)) != 0) flush_j(ecount & (kJFlush - 1));
     if (FORCES) {
 #pragma unroll
         for (int t = 0; t < kGroupClusters; ++t) {
@@ -939,11 +992,6 @@ __global__ void __launch_bounds__(64 * W) k_pairs_a(DevPtrs P, int setBegin) {
 
 // Order LDS accesses of the lanes of one wave (LDS executes a wave's
 // operations in issue order; this keeps the compiler from moving them).
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // FP64 -> 64-bit fixed point (round to nearest) for |f| < 2^27 kJ/mol/nm.
 __device__ __forceinline__ unsigned long long fixed64(double f) {
