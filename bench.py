@@ -245,6 +245,8 @@ def main():
                     help="column_scale,pair_waves_per_block,zbins_per_column[,pair_kernel] (skips the autotuner)")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-legs", action="store_true",
+                    help="skip the restraint, replica-MD and PME legs (profiles of the batched evaluation alone)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="first run a device-to-device copy of this many MiB five times (a kernel of known "
                          "traffic for scaling the FETCH_SIZE/WRITE_SIZE counters)")
@@ -275,7 +277,7 @@ def main():
     tuning = batch.tuning()
     if args.tuning:
         f = args.tuning.split(",")
-        batch.set_tuning(float(f[0]), int(f[1]), int(f[2]), int(f[3]) if len(f) > 3 else 2)
+        batch.set_tuning(float(f[0]), int(f[1]), int(f[2]), int(f[3]) if len(f) > 3 else 0)
         batch.compute()
         tuning = batch.tuning()
     elif not args.no_autotune:
@@ -349,12 +351,14 @@ def main():
         plain_eps = args.plain_steps / plain_dt
         plain.close()
 
-    log("[bench] companion restraints")
-    restraints = restraint_timing(base)
-    log("[bench] batched lambda-replica MD")
-    md = md_timing(reps)
-    log("[bench] PME variable group (temoa-g1)")
-    pme_v = pme_timing(cpu=not args.no_cpu)
+    restraints = md = pme_v = None
+    if not args.no_legs:
+        log("[bench] companion restraints")
+        restraints = restraint_timing(base)
+        log("[bench] batched lambda-replica MD")
+        md = md_timing(reps)
+        log("[bench] PME variable group (temoa-g1)")
+        pme_v = pme_timing(cpu=not args.no_cpu)
 
     cpu = None
     err = None

@@ -210,7 +210,7 @@ class ATMBatch:
         C.check(self._lib.atm_get_tuning(self._h, ctypes.byref(t)))
         return _tuning_dict(t)
 
-    def set_tuning(self, column_scale=1.0, pair_waves_per_block=1, zbins_per_column=4, pair_kernel=1):
+    def set_tuning(self, column_scale=1.3, pair_waves_per_block=2, zbins_per_column=4, pair_kernel=0):
         """pair_kernel 0: 8x8 cluster-pair tiles; 1-4: compacted in-cutoff pairs with (list entries
         per batch, copies of the j-force sums, drain after every test step) = (4, 2, yes), (4, 2, no),
         (4, 4, yes), (8, 2, yes); 5: atom-level j lists per i-cluster, 8 i x 8 j lanes per step."""

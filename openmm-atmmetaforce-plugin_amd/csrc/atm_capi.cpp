@@ -39,11 +39,13 @@ std::string& last_error() {
 using namespace atm;
 
 namespace {
+// Defaults: the autotuner's choice on a 22 x 23k-atom batch (cluster-pair tile
+// kernel, 2 waves per block, columns 1.3 x the 8-atom cube edge).
 struct Tuning {
-    double colScale = 1.0;
-    int pairWaves = 1;
+    double colScale = 1.3;
+    int pairWaves = 2;
     int zbinsPerColumn = 4;
-    int pairKernel = 1;
+    int pairKernel = 0;
 };
 
 struct Context {
@@ -896,51 +898,73 @@ int atm_autotune(void* context, int32_t repeats, atm_tuning* out) {
         // column scale tried, and its lists need their own buffers)
         const int kernels[] = {0, 1, 2};
         const int zpcs[] = {2, 4};
+        const Tuning orig = C.tuning;
         hipEvent_t e0, e1;
         HIPCHK(hipEventCreate(&e0));
         HIPCHK(hipEventCreate(&e1));
-        Tuning best = C.tuning;
-        double bestMs = 1e300;
         const int flags = ATM_EVAL_FORCES | ATM_EVAL_CLEAR_FORCES;
+        // Time one configuration (layout already set for its column scale and
+        // z-bins): a first evaluation that grows list capacity if needed (and
+        // captures the graph), then `n` timed ones; +inf if capacity never fits.
+        auto time_config = [&](int n) {
+            C.dropGraphs();
+            bool ok = true;
+            for (int attempt = 0; attempt < 3; ++attempt) {
+                eval_batched(C, flags);
+                C.evaluated = true;
+                C.resultsOnHost = false;
+                C.resultsValid = false;
+                try {
+                    fetch_results(C);
+                    ok = true;
+                    break;
+                } catch (const AtmError& e) {
+                    if (e.code != ATM_ERR_CAPACITY) throw;
+                    ok = false;
+                }
+            }
+            if (!ok) return 1e300;
+            HIPCHK(hipEventRecord(e0, C.stream));
+            for (int r = 0; r < n; ++r) eval_batched(C, flags);
+            HIPCHK(hipEventRecord(e1, C.stream));
+            HIPCHK(hipEventSynchronize(e1));
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+            return (double)ms / n;
+        };
+        auto set_layout = [&](double sc, int zp) {
+            C.tuning.colScale = sc;
+            C.tuning.zbinsPerColumn = zp;
+            C.entryCapOverride.assign(2 * C.nsys, 0);
+            layout(C);
+            alloc_layout(C);
+        };
+        // sweep, then re-time the four fastest with 4x the repeats (a single short
+        // timing of near-equal configurations picks by noise)
+        std::vector<std::pair<double, Tuning>> timed;
         for (double sc : scales) {
             for (int zp : zpcs) {
-                C.tuning.colScale = sc;
-                C.tuning.zbinsPerColumn = zp;
-                C.entryCapOverride.assign(2 * C.nsys, 0);
-                layout(C);
-                alloc_layout(C);
+                set_layout(sc, zp);
                 for (int kv = 0; kv < 3 * 3; ++kv) {
                     C.tuning.pairKernel = kernels[kv / 3];
                     C.tuning.pairWaves = waves[kv % 3];
-                    C.dropGraphs();
-                    bool ok = true;
-                    for (int attempt = 0; attempt < 3; ++attempt) {  // grow list capacity if needed
-                        eval_batched(C, flags);
-                        C.evaluated = true;
-                        C.resultsOnHost = false;
-                        C.resultsValid = false;
-                        try {
-                            fetch_results(C);
-                            ok = true;
-                            break;
-                        } catch (const AtmError& e) {
-                            if (e.code != ATM_ERR_CAPACITY) throw;
-                            ok = false;
-                        }
-                    }
-                    if (!ok) continue;
-                    HIPCHK(hipEventRecord(e0, C.stream));
-                    for (int r = 0; r < reps; ++r) eval_batched(C, flags);
-                    HIPCHK(hipEventRecord(e1, C.stream));
-                    HIPCHK(hipEventSynchronize(e1));
-                    float ms = 0.f;
-                    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-                    const double per = ms / reps;
-                    if (per < bestMs) {
-                        bestMs = per;
-                        best = C.tuning;
-                    }
+                    const double per = time_config(reps);
+                    if (per < 1e300) timed.emplace_back(per, C.tuning);
                 }
+            }
+        }
+        std::stable_sort(timed.begin(), timed.end(),
+                         [](const std::pair<double, Tuning>& a, const std::pair<double, Tuning>& b) { return a.first < b.first; });
+        Tuning best = orig;  // kept if no configuration fits the list capacity
+        double bestMs = 1e300;
+        for (size_t i = 0; i < timed.size() && i < 4; ++i) {
+            const Tuning t = timed[i].second;
+            set_layout(t.colScale, t.zbinsPerColumn);
+            C.tuning = t;
+            const double per = time_config(4 * reps);
+            if (per < bestMs) {
+                bestMs = per;
+                best = t;
             }
         }
         (void)hipEventDestroy(e0);

@@ -271,3 +271,19 @@ def test_clash_side_sums_cleared_between_evaluations(gpu, kernel):
             b.set_displacements(0, s.displacements)
             b.compute()
             check_system(b, 0, s, pairs=False, tag=f"side sums kernel {kernel}")
+
+
+def test_autotune_then_results_match_oracle(gpu):
+    """atm_autotune sweeps column scale, z-bins, pair kernel and block shape,
+    re-times the fastest four and keeps the best; the batch it leaves behind
+    evaluates like any fixed configuration."""
+    systems = _variant_systems()
+    with ATMBatch(systems) as b:
+        t = b.autotune(repeats=2)
+        print(f"autotune: {t}", flush=True)
+        assert t["pair_kernel"] in (0, 1, 2) and t["pair_waves_per_block"] in (1, 2, 4)
+        assert t["zbins_per_column"] in (2, 4) and 0.5 <= t["column_scale"] <= 2.0 and t["best_ms"] > 0.0
+        assert b.tuning()["pair_kernel"] == t["pair_kernel"]
+        b.compute()
+        for k, s in enumerate(systems):
+            check_system(b, k, s, pairs=(k < 2), tag="autotuned")
