@@ -253,10 +253,10 @@ __device__ __forceinline__ int wave_incl_scan(int v) {
 // One block of 1024 threads (16 waves) per set.
 //  A: one wave per column (16 columns in flight), wave prefix sums over its
 //     z-bins -> cell starts relative to their column, and the column count;
-//  B: each thread scans a contiguous chunk of columns (padded slot counts and
-//     group counts; it sums the cell counts of its columns itself, so no
-//     value passes between waves through global memory), a block scan joins
-//     the chunks; column starts, dummy slots and the group table are written.
+//  B: after a block barrier, each thread scans a contiguous chunk of columns
+//     (padded slot counts and group counts from A's column counts), a block
+//     scan joins the chunks; column starts, dummy slots and the group table
+//     are written.
 // A cell's first slot is colStart[column] + cellStart[cell].
 __global__ void __launch_bounds__(1024) k_scan(DevPtrs P, int setBegin) {
     constexpr int NT = 1024, NW = NT / 64;
@@ -277,14 +277,16 @@ __global__ void __launch_bounds__(1024) k_scan(DevPtrs P, int setBegin) {
         }
         if (lane == 0) P.colCount[D.colOff + col] = run;
     }
+    // the column counts written above by other waves of this block: the barrier
+    // orders them (workgroup scope covers global memory of the same block)
+    __syncthreads();
+    const int* colCount = P.colCount + D.colOff;
     const int chunk = (ncols + NT - 1) / NT;
     const int b = min(t * chunk, ncols), e = min(b + chunk, ncols);
     __shared__ int sS[NW], sG[NW];
     int accS = 0, accG = 0;
     for (int col = b; col < e; ++col) {
-        const int* cnt = P.cellCount + D.cellOff + (size_t)col * nz;
-        int n = 0;
-        for (int z = 0; z < nz; ++z) n += cnt[z];
+        const int n = colCount[col];
         const int padded = (n + 7) & ~7;
         accS += padded;
         accG += ((padded >> 3) + kGroupClusters - 1) / kGroupClusters;
@@ -301,9 +303,7 @@ __global__ void __launch_bounds__(1024) k_scan(DevPtrs P, int setBegin) {
     }
     int offS = baseS + inS - accS, offG = baseG + inG - accG;
     for (int col = b; col < e; ++col) {
-        const int* cc = P.cellCount + D.cellOff + (size_t)col * nz;
-        int cnt = 0;
-        for (int z = 0; z < nz; ++z) cnt += cc[z];
+        const int cnt = colCount[col];
         const int padded = (cnt + 7) & ~7;
         const int ncl = padded >> 3;
         const int ng = (ncl + kGroupClusters - 1) / kGroupClusters;
@@ -337,6 +337,9 @@ __global__ void k_scatter_idx(DevPtrs P, int setBegin) {
 // Order each fine cell's atom indices ascending (the atomic ranks of k_bin are
 // not deterministic).  Fixed cluster composition makes the FP32 partial sums,
 // and so every result of an evaluation, bit-reproducible.
+// Cells of up to 16 atoms (nearly all) are sorted in registers by an odd-even
+// transposition network (independent loads, no dependent global round trips);
+// larger cells by insertion sort in place.
 __global__ void k_cellsort_idx(DevPtrs P, int setBegin) {
     const int c = setBegin + blockIdx.y;
     const SetDesc& D = P.desc[c];
@@ -345,6 +348,25 @@ __global__ void k_cellsort_idx(DevPtrs P, int setBegin) {
     const int cnt = P.cellCount[D.cellOff + cell];
     if (cnt < 2) return;
     int* idx = P.sidx + D.slotOff + P.colStart[D.colOff + cell / D.nz] + P.cellStart[D.cellOff + cell];
+    constexpr int kNet = 16;
+    if (cnt <= kNet) {
+        int v[kNet];
+#pragma unroll
+        for (int a = 0; a < kNet; ++a) v[a] = a < cnt ? idx[a] : 0x7FFFFFFF;
+#pragma unroll
+        for (int r = 0; r < kNet; ++r) {
+#pragma unroll
+            for (int a = r & 1; a + 1 < kNet; a += 2) {
+                const int lo = min(v[a], v[a + 1]), hi = max(v[a], v[a + 1]);
+                v[a] = lo;
+                v[a + 1] = hi;
+            }
+        }
+#pragma unroll
+        for (int a = 0; a < kNet; ++a)
+            if (a < cnt) idx[a] = v[a];
+        return;
+    }
     for (int a = 1; a < cnt; ++a) {
         const int key = idx[a];
         int b = a - 1;
@@ -576,8 +598,8 @@ __device__ __forceinline__ void wave_sync() {
 __device__ __forceinline__ void flush_component_u(const DevPtrs& P, int c, int slot, int k, float f) {
     const bool small = fabsf(f) < 134217728.0f;
     long long* dst = P.sf + (long long)k * P.sfn + slot;
-    atomic_add_fixed(dst, small ? __double_as_longlong((double)f * kForceScale + 6755399441055744.0) - 0x4338000000000000LL
-                                : 0ll);
+    const long long v = small ? __double_as_longlong((double)f * kForceScale + 6755399441055744.0) - 0x4338000000000000LL : 0ll;
+    if (v != 0) atomic_add_fixed(dst, v);
     if (!small) {
         if (fabsf(f) < kFixedMax) {
             atomic_add_fixed(dst, to_fixed(f));
