@@ -79,6 +79,138 @@ def cpu_baseline(systems, gpu_result0, gpu_forces0, seconds=12.0):
     return cpu, err
 
 
+def config_report(steps=50, cpu_seconds=5.0):
+    """The other configurations, as parity and throughput figures beside the
+    headline (bench.py --configs prints this instead of the headline line):
+      config1: the 3,000-atom CPU-runnable case -- FP64 oracle on one thread,
+               the OpenMP port on every host thread, the GPU plain path;
+      config2: the 23,000-atom system on the GPU plain path against the oracle;
+      config4: 64 systems of 5k-60k atoms in one batched launch, each checked
+               against the FP64 oracle;
+      config5: forward + backward (dW/d charges, sigma, epsilon, 5 scalars) of
+               the 22-replica 23k-atom batch through ATMEnergyFunction."""
+    import numpy as np
+    import torch
+    from atmmetaforce import ATMBatch, ATMEnergyFunction, workloads as W
+    sys.path.insert(0, ROOT)
+    from oracle import oracle
+
+    def timed(fn, budget, least=3):
+        t0 = time.perf_counter()
+        n, out = 0, None
+        while n < least or time.perf_counter() - t0 < budget:
+            out = fn()
+            n += 1
+        return n / (time.perf_counter() - t0), out
+
+    def errors(r, F, ref, Fref):
+        return {"U1_rel": abs(r["state1_energy"] - ref["U1"]) / abs(ref["U1"]),
+                "U2_rel": abs(r["state2_energy"] - ref["U2"]) / abs(ref["U2"]),
+                "W_rel": abs(r["energy"] - ref["W"]) / max(abs(ref["W"]), 1e-300),
+                "force_rel_rms": float(np.linalg.norm(F - Fref) / np.linalg.norm(Fref))}
+
+    out = {"metric": "configuration report", "n_gpus": 1}
+    log("[bench] config1: 3,000 atoms")
+    s1 = W.config1()
+    threads = oracle.cpu_mt_threads()
+    ref_eps, (ref, Fref) = timed(lambda: oracle.atm_eval(s1), cpu_seconds)
+    oracle.cpu_mt_atm_eval(s1, threads)
+    mt_eps, (rmt, Fmt) = timed(lambda: oracle.cpu_mt_atm_eval(s1, threads), cpu_seconds)
+    err_mt = errors({"state1_energy": rmt["U1"], "state2_energy": rmt["U2"], "energy": rmt["W"]}, Fmt, ref, Fref)
+    with ATMBatch([s1]) as b:
+        for _ in range(3):
+            b.compute(plain=True)
+        b.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps * 4):
+            b.evaluate(forces=True, clear=True, plain=True)
+        b.synchronize()
+        gpu_eps = steps * 4 / (time.perf_counter() - t0)
+        b.compute(plain=True)
+        err1 = errors(b.result(0), b.forces(0), ref, Fref)
+    out["config1"] = {"atoms": s1.num_atoms, "fp64_oracle_evals_per_s": round(ref_eps, 2), "oracle_cores": 1,
+                      "cpu_port_evals_per_s": round(mt_eps, 2), "cpu_port_cores": threads,
+                      "cpu_port_error_vs_fp64_oracle": err_mt,
+                      "gpu_plain_evals_per_s": round(gpu_eps, 1), "error_vs_fp64_oracle": err1,
+                      "pairs_in_cutoff_both_states": int(ref["npairs1"] + ref["npairs2"])}
+
+    log("[bench] config2: 23,000 atoms, plain path")
+    s2 = W.config2()
+    ref2, Fref2 = oracle.atm_eval(s2)
+    with ATMBatch([s2]) as b:
+        for _ in range(3):
+            b.compute(plain=True)
+        b.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            b.evaluate(forces=True, clear=True, plain=True)
+        b.synchronize()
+        plain_eps = steps / (time.perf_counter() - t0)
+        b.compute(plain=True)
+        err2 = errors(b.result(0), b.forces(0), ref2, Fref2)
+    out["config2"] = {"atoms": s2.num_atoms, "gpu_plain_evals_per_s": round(plain_eps, 1),
+                      "error_vs_fp64_oracle": err2, "pairs_in_cutoff_both_states": int(ref2["npairs1"] + ref2["npairs2"])}
+
+    log("[bench] config4: 64 uneven systems")
+    systems = W.uneven_batch(64)
+    with ATMBatch(systems) as b:
+        b.compute()
+        for _ in range(5):
+            b.evaluate(forces=True, clear=True)
+        b.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            b.evaluate(forces=True, clear=True)
+        b.synchronize()
+        dt = time.perf_counter() - t0
+        pairs = b.stats()["pairs_in_cutoff"]
+        b.compute()
+        worst = {"U1_rel": 0.0, "U2_rel": 0.0, "W_rel": 0.0, "force_rel_rms": 0.0}
+        t_or = time.perf_counter()
+        for k, s in enumerate(systems):
+            rk, Fk = oracle.atm_eval(s)
+            e = errors(b.result(k), b.forces(k), rk, Fk)
+            for key in worst:
+                worst[key] = max(worst[key], e[key])
+        log(f"[bench] config4 oracle pass {time.perf_counter() - t_or:.1f} s")
+    out["config4"] = {"systems": len(systems), "atoms": sum(s.num_atoms for s in systems),
+                      "atoms_min_max": [min(s.num_atoms for s in systems), max(s.num_atoms for s in systems)],
+                      "system_evals_per_s": round(len(systems) * steps / dt, 1), "ms_per_batch": round(dt * 1e3 / steps, 4),
+                      "pair_interactions_per_s": round(pairs * steps / dt, 1),
+                      "max_error_vs_fp64_oracle_over_systems": worst}
+
+    log("[bench] config5: forward + backward, 22 replicas")
+    reps = W.replica_batch(22)
+
+    def tensors():
+        q = torch.tensor(np.concatenate([s.charges for s in reps]), requires_grad=True)
+        sg = torch.tensor(np.concatenate([s.sigmas for s in reps]), requires_grad=True)
+        ep = torch.tensor(np.concatenate([s.epsilons for s in reps]), requires_grad=True)
+        sc = [torch.tensor([getattr(s.params, n) for s in reps], dtype=torch.float64, requires_grad=True)
+              for n in ("lambda1", "lambda2", "alpha", "u0", "w0")]
+        return q, sg, ep, sc
+
+    with ATMBatch(reps) as b:
+        q, sg, ep, sc = tensors()
+
+        def fwd_bwd():
+            Wt = ATMEnergyFunction.apply(b, q, sg, ep, *sc)
+            Wt.sum().backward()
+            return Wt
+
+        fwd_bwd()
+        n = 10
+        t0 = time.perf_counter()
+        for _ in range(n):
+            fwd_bwd()
+        fb_ms = (time.perf_counter() - t0) * 1e3 / n
+    out["config5"] = {"replicas": 22, "ms_per_forward_backward": round(fb_ms, 3),
+                      "note": "host tensors in and gradients out (parameter upload and gradient download included); "
+                              "gradients checked against the FP64 oracle and central differences in "
+                              "tests/test_autograd_gpu.py"}
+    return out
+
+
 def restraint_timing(system, evals=200):
     """Per-evaluation time of the flat-bottom restraints an ABFE script keeps in
     force group 0 beside the ATM force (python/tests/test_abfe.py:58-79): one
@@ -245,6 +377,9 @@ def main():
                     help="column_scale,pair_waves_per_block,zbins_per_column[,pair_kernel] (skips the autotuner)")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--configs", action="store_true",
+                    help="print the report of the other configurations (3k-atom CPU case, uneven batch, backward) "
+                         "instead of the headline line")
     ap.add_argument("--no-legs", action="store_true",
                     help="skip the restraint, replica-MD and PME legs (profiles of the batched evaluation alone)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
@@ -259,6 +394,10 @@ def main():
     import numpy as np
     import torch
     from atmmetaforce import ATMBatch, workloads as W
+
+    if args.configs:
+        print(json.dumps(config_report()), flush=True)
+        return
 
     if args.calibrate_copy_mib > 0:
         n = args.calibrate_copy_mib * 1024 * 1024 // 4
