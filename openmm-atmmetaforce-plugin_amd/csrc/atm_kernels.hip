@@ -471,9 +471,24 @@ __global__ void __launch_bounds__(64 * W) k_pairlist(DevPtrs P, int setBegin) {
     int nEnt = 0;
     const int ncx = D.fullX ? D.nx : 2 * D.Rx + 1, ncy = D.fullY ? D.ny : 2 * D.Ry + 1;
     const int ncand = ncx * ncy;
+    // z-window of the group: a j-atom within the list cutoff of any i-atom has
+    // z in [zlo - rcl, zhi + rcl] (periodic), so only clusters holding an atom
+    // in the z-bins of that window (one bin of slack on each side for rounding)
+    // can pair; they form one or two (wrapped) slot ranges per column.
+    float zlo = icz[0] - ihz[0], zhi = icz[0] + ihz[0];
+#pragma unroll
+    for (int q = 1; q < kGroupClusters; ++q) {
+        if (q < ncl) { zlo = fminf(zlo, icz[q] - ihz[q]); zhi = fmaxf(zhi, icz[q] + ihz[q]); }
+    }
+    const float rcl = sqrtf(D.rcList2);
+    const int blo = (int)floorf((zlo - rcl) * D.invZe) - 1, bhi = (int)floorf((zhi + rcl) * D.invZe) + 1;
+    const bool zall = bhi - blo + 1 >= D.nz;  // the window covers the column
+    const int bl = zall ? 0 : ((blo % D.nz) + D.nz) % D.nz;
+    const int bh = zall ? D.nz - 1 : ((bhi % D.nz) + D.nz) % D.nz;
+    const bool zwrap = !zall && bl > bh;       // two ranges: [bl, nz) and [0, bh]
     for (int cb = 0; cb < ncand; cb += 64) {
-        // this lane's candidate column: first cluster, cluster count, shift codes
-        int cnt_l = 0, c0_l = 0, codes_l = 0;
+        // this lane's candidate column: up to two cluster ranges (a, b), shift codes
+        int cnt_l = 0, c0_l = 0, codes_l = 0, cntb_l = 0, c0b_l = 0;
         const int ci = cb + lane;
         if (ci < ncand) {
             const int a = ci / ncy, b = ci % ncy;
@@ -494,10 +509,38 @@ __global__ void __launch_bounds__(64 * W) k_pairlist(DevPtrs P, int setBegin) {
             if (gapx * gapx + gapy * gapy < D.rcList2) {
                 const int colJ = jx * D.ny + jy;
                 const int cs = P.colStart[D.colOff + colJ], cnt = P.colCount[D.colOff + colJ];
-                const int cl0 = max(cs >> 3, g0), cl1 = (cs + ((cnt + 7) & ~7)) >> 3;
-                if (cl1 > cl0) { cnt_l = cl1 - cl0; c0_l = cl0; codes_l = sxc | (syc << 2); }
+                const int* cst = P.cellStart + D.cellOff + (size_t)colJ * D.nz;
+                const int* ccn = P.cellCount + D.cellOff + (size_t)colJ * D.nz;
+                // slots [s0, s1) of bins [b0, b1] -> clusters [s0 >> 3, ceil(s1 / 8)), at least g0
+                auto range = [&](int b0, int b1, int& c0, int& n) {
+                    const int s0 = cs + cst[b0], s1 = cs + cst[b1] + ccn[b1];
+                    const int lo = max(s0 >> 3, g0), hi = (s1 + 7) >> 3;
+                    c0 = lo;
+                    n = max(0, hi - lo);
+                };
+                if (cnt > 0) {
+                    if (zall) {
+                        const int cl0 = max(cs >> 3, g0), cl1 = (cs + ((cnt + 7) & ~7)) >> 3;
+                        c0_l = cl0;
+                        cnt_l = max(0, cl1 - cl0);
+                    } else if (!zwrap) {
+                        range(bl, bh, c0_l, cnt_l);
+                    } else {
+                        range(bl, D.nz - 1, c0_l, cnt_l);
+                        range(0, bh, c0b_l, cntb_l);
+                        // a cluster straddling the empty bins between the two windows
+                        // would fall in both ranges: the top range starts after the bottom one
+                        const int endb = c0b_l + cntb_l;
+                        if (cntb_l > 0 && c0_l < endb) {
+                            cnt_l = max(0, c0_l + cnt_l - endb);
+                            c0_l = endb;
+                        }
+                    }
+                    codes_l = sxc | (syc << 2);
+                }
             }
         }
+        cnt_l += cntb_l;  // flattened: range a, then range b
         const int incl = wave_incl_scan(cnt_l);
         const int excl = incl - cnt_l;
         const int total = __shfl(incl, 63, 64);
@@ -510,7 +553,12 @@ __global__ void __launch_bounds__(64 * W) k_pairlist(DevPtrs P, int setBegin) {
                 const int ex = __shfl(excl, probe, 64);
                 if (lo + step <= 63 && ex <= f) lo = probe;
             }
-            const int J = __shfl(c0_l, lo, 64) + (f - __shfl(excl, lo, 64));
+            // every shuffle with all lanes active: a lane that a branch masks off
+            // does not provide its value to the others
+            const int off = f - __shfl(excl, lo, 64);
+            const int na = __shfl(cnt_l - cntb_l, lo, 64);
+            const int c0a = __shfl(c0_l, lo, 64), c0b = __shfl(c0b_l, lo, 64);
+            const int J = off < na ? c0a + off : c0b + (off - na);
             const int codes = __shfl(codes_l, lo, 64);
             uint32_t mask = 0;
             if (f < total) {
