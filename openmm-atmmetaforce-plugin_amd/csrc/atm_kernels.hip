@@ -732,12 +732,11 @@ __global__ void __launch_bounds__(64 * W, 6) k_pairs(DevPtrs P, int setBegin) {
     const float Lx = D.L[0], invLx = D.invL[0], Ly = D.L[1], invLy = D.invL[1];
     const bool shiftOK = D.shiftOK;
     const double rc2 = D.rc2, krf64 = D.krf64, crf64 = D.crf64;
-    const float krf = D.krf, rc2hi = D.rc2hi;
+    const float krf = D.krf, crf = D.crf, rc2hi = D.rc2hi;
     float fi[kGroupClusters][3];
 #pragma unroll
     for (int t = 0; t < kGroupClusters; ++t) fi[t][0] = fi[t][1] = fi[t][2] = 0.f;
-    // FP64: eE sums whole pair energies; FP32: eE = sum(E_LJ + qq/r), eQQ = sum(qq), eQR2 = sum(qq r^2)
-    double eE = 0.0, eQQ = 0.0, eQR2 = 0.0;
+    double eE = 0.0;  // this lane's sum of whole pair energies
     const int ne = __builtin_amdgcn_readfirstlane(P.gcount[D.groupOff + g]);
     const uint32_t* E = P.entries + D.entryOff + (long long)g * D.entryCap;
     // This lane's j-atom of one list entry.  Two of these are in flight: entry
@@ -829,13 +828,17 @@ __global__ void __launch_bounds__(64 * W, 6) k_pairs(DevPtrs P, int setBegin) {
                     eE += pe.e;
                     fs = (float)pe.fs;
                 } else {
-                    const bool ok = (okm & (1u << t)) != 0u;
+                    uint64_t iv = (ivLanes >> (t * kClusterSize)) & 0xFFull;
+                    iv |= iv << 8; iv |= iv << 16; iv |= iv << 32;
+                    const uint64_t okLanes = (t == tdiag ? upperLanes : ~0ull) & jvLanes & iv;
                     const float2 qi = sPrm[wave][t * kClusterSize + ii];
                     const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-                    const bool in = ok && in_cutoff(D, dx, dy, dz, r2);
-                    if (__ballot(in) == 0ull) continue;
+                    const uint64_t inLanes = __builtin_amdgcn_ballot_w64(in_cutoff(D, dx, dy, dz, r2)) & okLanes;
+                    if (inLanes == 0ull) continue;
+                    const bool in = __builtin_amdgcn_inverse_ballot_w64(inLanes);
                     const PairE pe = pair_term(in, r2, pi.w * pj.w, qi.x + qj.x, qi.y * qj.y, krf);
-                    eE += (double)(pe.elj + pe.qr); eQQ += (double)pe.qq; eQR2 += (double)pe.qr2;
+                    // the whole pair energy in FP32 (its terms carry FP32 errors anyway), summed in FP64
+                    eE += (double)(pe.elj + pe.qr + fmaf(krf, pe.qr2, -crf * pe.qq));
                     fs = pe.fs;
                 }
                 if (FORCES) {
@@ -883,7 +886,7 @@ __global__ void __launch_bounds__(64 * W, 6) k_pairs(DevPtrs P, int setBegin) {
             }
         }
     }
-    double eacc = eE + D.krf64 * eQR2 - D.crf64 * eQQ;
+    double eacc = eE;
     eacc = wave_sum_d(eacc);
     if (lane == 0) P.genergy[D.groupOff + g] = eacc;
 }

@@ -287,3 +287,25 @@ def test_autotune_then_results_match_oracle(gpu):
         b.compute()
         for k, s in enumerate(systems):
             check_system(b, k, s, pairs=(k < 2), tag="autotuned")
+
+
+def test_fp32_pair_mode_batch(gpu):
+    """FP32 scalar pair function (ATM_EVAL_FP32_PAIRS) over the variant systems
+    (small boxes without image shifts, a clashing displaced ligand, uneven
+    sizes): U1, U2 within the FP32 error model (1e-6 of max(|U|, sum |e_ij|)),
+    W within the bound propagated from them, forces within F_TOL."""
+    systems = _variant_systems()
+    with ATMBatch(systems) as b:
+        b.compute(fp32_pairs=True)
+        for k, s in enumerate(systems):
+            ref, Fref = oracle.atm_eval(s)
+            r = b.result(k)
+            b1 = E_TOL * max(abs(ref["U1"]), ref["abs1"])
+            b2 = E_TOL * max(abs(ref["U2"]), ref["abs2"])
+            w1, w2 = (1 - ref["sp"], ref["sp"]) if s.params.direction > 0 else (ref["sp"], 1 - ref["sp"])
+            print(f"[fp32 sys {k}] U1 {r['state1_energy']:.10g} vs {ref['U1']:.10g} (bound {b1:.3g}); "
+                  f"U2 {r['state2_energy']:.10g} vs {ref['U2']:.10g} (bound {b2:.3g})", flush=True)
+            assert abs(r["state1_energy"] - ref["U1"]) <= b1
+            assert abs(r["state2_energy"] - ref["U2"]) <= b2
+            assert abs(r["energy"] - ref["W"]) <= E_TOL * abs(ref["W"]) + abs(w1) * b1 + abs(w2) * b2
+            assert frms(b.forces(k), Fref) < F_TOL
