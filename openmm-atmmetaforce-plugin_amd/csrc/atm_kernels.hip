@@ -801,10 +801,14 @@ __global__ void __launch_bounds__(64 * W, 6) k_pairs(DevPtrs P, int setBegin) {
         // j-atom (all of them with explicit shifts: dummies sit far away)
         const uint64_t jvLanes = (F64 && shiftOK) ? ~0ull : __builtin_amdgcn_ballot_w64(jv);
         float fjx = 0.f, fjy = 0.f, fjz = 0.f;
+        // i-atom positions read one step ahead (whether or not that step runs),
+        // so the LDS latency falls behind the current step's work
+        float4 piN = sPos[wave][ii];
 #pragma unroll
         for (int t = 0; t < kGroupClusters; ++t) {
+            const float4 pi = piN;
+            if (t + 1 < kGroupClusters) piN = sPos[wave][(t + 1) * kClusterSize + ii];
             if (imask & (1u << t)) {
-                const float4 pi = sPos[wave][t * kClusterSize + ii];
                 float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
                 if (!shiftOK) { dx = min_image(dx, Lx, invLx); dy = min_image(dy, Ly, invLy); }
                 dz = min_image(dz, Lz, invLz);
