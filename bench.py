@@ -461,16 +461,21 @@ def main():
     F0 = batch.forces(0)
 
     pair_tflops = FLOP_PER_PAIR * pairs / (prof["pairs_ms"] * 1e-3) / 1e12
-    traffic = None
+    traffic = issue = None
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
-    if os.path.exists(tpath):  # HBM bytes per launch from the committed rocprofv3 --pmc summaries
+    if os.path.exists(tpath):  # HBM bytes and VALU issue rate per launch from the committed rocprofv3 --pmc summaries
         with open(tpath) as fh:
-            traffic = json.load(fh).get("k_pairs", {}).get("hbm_bytes_per_launch")
+            kp = json.load(fh).get("k_pairs", {})
+        traffic, issue = kp.get("hbm_bytes_per_launch"), kp.get("valu_issue_frac")
     roofline = {"bound": "valu", "achieved": round(pair_tflops, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(pair_tflops / PEAK_FP32_TFLOPS, 4), "traffic": traffic,
                 "kernel": "k_pairs (LJ + reaction-field pair kernel, all 44 coordinate sets per launch)",
                 "per_launch_flop": FLOP_PER_PAIR * pairs, "pairs_per_launch": pairs,
-                "avg_launch_ms": round(prof["pairs_ms"], 4)}
+                "avg_launch_ms": round(prof["pairs_ms"], 4),
+                "valu_issue_frac": issue,
+                "note": "achieved counts 57 FLOP per in-cutoff pair only; the kernel issues VALU instructions "
+                        "at valu_issue_frac of the chip's rate (SQ_INSTS_VALU over GRBM_GUI_ACTIVE, "
+                        "profiles/k_pairs_22rep_23k.md): 8x8 cluster tiles hold 21 % in-cutoff pairs"}
     total_atoms = sum(s.num_atoms for s in reps)
     blend_gbs = BLEND_BYTES_PER_ATOM * total_atoms / (prof["blend_ms"] * 1e-3) / 1e9
     batch.close()

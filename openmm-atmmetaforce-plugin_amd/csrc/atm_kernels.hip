@@ -8,11 +8,12 @@
 //   k_cellsort_idx order each fine cell by atom index (deterministic results)
 //   k_gather_bbox  gather positions/parameters into cluster order, clear the
 //                  set's force accumulators, per-cluster bounding boxes
-//   k_pairlist     per group of 8 clusters: j-clusters whose boxes are within
-//                  the cutoff of at least one i-cluster (half shell, i <= j)
+//   k_pairlist     per group of 8 clusters: j-clusters (from the group's
+//                  z-window) whose boxes are within the cutoff of at least one
+//                  i-cluster (half shell, i <= j)
 //   k_pairs        LJ + reaction-field pair kernel, one wave64 per group,
-//                  lanes = 8 j-atoms x 8 i-atoms, FP32 pair math, FP64 energy,
-//                  64-bit fixed-point forces
+//                  lanes = 8 j-atoms x 8 i-atoms, FP32 geometry, FP64 pair
+//                  function (or FP32), FP64 energy, 64-bit fixed-point forces
 // and per system:
 //   k_energy       U1, U2 (fixed-order FP64 sums), soft-core, softplus -> W, sp
 //   k_blend        F += (1-sp) F1 + sp F2 (direction-aware), gathered back to
@@ -658,20 +659,6 @@ __device__ __forceinline__ void flush_component_u(const DevPtrs& P, int c, int s
     }
 }
 
-// Flush one force component of a slot (component k of slot `slot`): 64-bit
-// fixed point, or the FP64 side accumulator above 2^27 (clashing pairs).
-__device__ __forceinline__ void flush_component(const DevPtrs& P, int c, int slot, int k, float f) {
-    if (fabsf(f) < 134217728.0f) {  // 2^27: |f scale| < 2^51, the magic-number conversion is exact
-        atomic_add_fixed(P.sf + (long long)k * P.sfn + slot,
-                         __double_as_longlong((double)f * kForceScale + 6755399441055744.0) - 0x4338000000000000LL);
-    } else if (fabsf(f) < kFixedMax) {
-        atomic_add_fixed(P.sf + (long long)k * P.sfn + slot, to_fixed(f));
-    } else {
-        atomicAdd(P.sd + 3ll * slot + k, (double)f);
-        P.rt[c].big = 1;
-    }
-}
-
 // One wave64 per group of up to 8 i-clusters; lane = 8 jj + ii pairs i-atom ii
 // of each interacting i-cluster with j-atom jj of the list entry's j-cluster.
 // F64: scalar pair function in FP64 (exact r^2, default); otherwise FP32.
@@ -792,13 +779,9 @@ __global__ void __launch_bounds__(64 * W, 6) k_pairs(DevPtrs P, int setBegin) {
             pj.x += sxc == 1u ? D.L[0] : (sxc == 2u ? -D.L[0] : 0.f);
             pj.y += syc == 1u ? D.L[1] : (syc == 2u ? -D.L[1] : 0.f);
         }
-        const int tdiag = J - g0;  // the i-cluster equal to J, if any
-        // bit t of okm: this lane's (t, ii) x jj pair may count (real atoms;
-        // i < j inside the diagonal cluster); computed once per entry
-        uint32_t okm = (F64 && shiftOK) ? 0xFFu : (jv ? ivm : 0u);
-        if (tdiag >= 0 && tdiag < kGroupClusters && ii >= jj) okm &= ~(1u << tdiag);
-        // the same as wave masks in scalar registers (FP64 path): lanes with a real
-        // j-atom (all of them with explicit shifts: dummies sit far away)
+        const int tdiag = J - g0;  // the i-cluster equal to J, if any (its lanes count only i < j)
+        // lanes with a real j-atom, as a wave mask (FP64 path with explicit shifts:
+        // all of them, dummies sit far away)
         const uint64_t jvLanes = (F64 && shiftOK) ? ~0ull : __builtin_amdgcn_ballot_w64(jv);
         float fjx = 0.f, fjy = 0.f, fjz = 0.f;
         // i-atom positions read one step ahead (whether or not that step runs),

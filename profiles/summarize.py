@@ -78,11 +78,18 @@ def main(root):
     print("\n".join(out))
     # the pair kernel of the timed loop: the k_pairs instantiation with the most launches
     kf = max((k for k in traffic if k.startswith("atm::k_pairs") and "dump" not in k), key=lambda k: traffic[k][2])
+    # VALU issue rate: GRBM_GUI_ACTIVE sums the 8 XCDs' busy cycles; 256 CUs x 4
+    # SIMDs issue at most one wave64 VALU instruction per 4 cycles each
+    mean = lambda c: sum(s[(pk, c)]) / len(s[(pk, c)])
+    valu, grbm = mean("SQ_INSTS_VALU"), mean("GRBM_GUI_ACTIVE")
+    issue = valu / (grbm / 8.0 * 256.0)
     json.dump({"k_pairs": {"kernel": kf.replace("atm::", ""),
                            "hbm_bytes_per_launch": round(traffic[kf][0] + traffic[kf][1]),
                            "hbm_read_bytes_per_launch": round(traffic[kf][0]),
                            "hbm_write_bytes_per_launch": round(traffic[kf][1]),
                            "fetch_scale": fs, "write_scale": ws,
+                           "valu_insts_per_launch": round(valu), "busy_cycles_per_xcd": round(grbm / 8.0),
+                           "valu_issue_frac": round(issue, 3),
                            "workload": "22 replicas x 23,000 atoms, both states (44 sets) per launch"},
                "k_blend": {"hbm_bytes_per_launch": round(sum(traffic.get("atm::k_blend", (0, 0, 0))[:2]))}},
               open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "traffic.json"), "w"), indent=1)
