@@ -212,7 +212,9 @@ __global__ void __launch_bounds__(64) k_pme_direct(PmeDesc D, PmePtrs P, unsigne
     __shared__ int4 sJx[kPmeJBatch];    // exclusion group, rank, -, atom
     __shared__ unsigned long long sJF[3][kPmeJBatch];
     const int lane = threadIdx.x;
-    const int nb = blockIdx.x, c = blockIdx.y, chunk = blockIdx.z;
+    // x: neighbour offset (fastest) and part of the neighbour cell's atoms; y: cell; z: 64-atom chunk
+    const int nb = blockIdx.x % kPmeHalfShell, part = blockIdx.x / kPmeHalfShell;
+    const int c = blockIdx.y, chunk = blockIdx.z;
     const int cz = c % D.m[2], cy = (c / D.m[2]) % D.m[1], cx = c / (D.m[1] * D.m[2]);
     // forward offsets: (a, b, e) lexicographically above (0, 0, 0); nb = 0 is the cell itself
     int ox = 0, oy = 0, oz = 0;
@@ -227,7 +229,7 @@ __global__ void __launch_bounds__(64) k_pme_direct(PmeDesc D, PmePtrs P, unsigne
     if (nz < 0) { nz += D.m[2]; sz = -D.L[2]; } else if (nz >= D.m[2]) { nz -= D.m[2]; sz = D.L[2]; }
     const int cn = (nx * D.m[1] + ny) * D.m[2] + nz;
     const int s0 = P.cellStart[c], s1 = P.cellStart[c + 1];
-    const int blockSlot = (chunk * gridDim.y + c) * gridDim.x + nb;
+    const int blockSlot = (chunk * gridDim.y + c) * gridDim.x + blockIdx.x;
     if (s0 + chunk * 64 >= s1) {  // no atom in this chunk: the whole block leaves
         if (lane == 0) eslot[blockSlot] = 0.0;
         return;
@@ -247,8 +249,11 @@ __global__ void __launch_bounds__(64) k_pme_direct(PmeDesc D, PmePtrs P, unsigne
     sI[lane] = pi;
     sF[0][lane] = 0ull; sF[1][lane] = 0ull; sF[2][lane] = 0ull;
     const int j0 = P.cellStart[cn], j1 = P.cellStart[cn + 1];
-    // in the cell itself only j slots above the chunk's first slot can pair with it
-    const int jstart = nb == 0 ? s0 + chunk * 64 + 1 : j0;
+    // in the cell itself only j slots above the chunk's first slot can pair with it;
+    // this wave takes part `part` of kPmeJSplit contiguous parts of the j range
+    const int jall = nb == 0 ? s0 + chunk * 64 + 1 : j0;
+    const int jlen = max(0, j1 - jall);
+    const int jstart = jall + (jlen * part) / kPmeJSplit, jend = jall + (jlen * (part + 1)) / kPmeJSplit;
     const double a = D.alpha, a2pi = D.twoAlphaSqrtPi, rc2 = D.rc2;
     double e = 0.0;
     int head = 0, tail = 0;  // ring [head, tail) of pending pairs (wave-uniform)
@@ -287,8 +292,8 @@ __global__ void __launch_bounds__(64) k_pme_direct(PmeDesc D, PmePtrs P, unsigne
         head += count;
     };
 
-    for (int jb = jstart; jb < j1; jb += kPmeJBatch) {
-        const int nbat = min(kPmeJBatch, j1 - jb);
+    for (int jb = jstart; jb < jend; jb += kPmeJBatch) {
+        const int nbat = min(kPmeJBatch, jend - jb);
         __syncthreads();  // the previous batch is fully evaluated and flushed
         for (int k = lane; k < nbat; k += 64) {
             sJ[k] = P.cpos[jb + k];
@@ -601,9 +606,17 @@ __global__ void __launch_bounds__(1024) k_pme_finish(PmeDesc D, PmePtrs P, int s
         const int ranges[4] = {0, D.nslotDirect, D.nslotDirect + D.nslotEx, D.nslot};
         double parts[3];
         for (int r = 0; r < 3; ++r) {
-            double v = 0.0;
-            for (int k = ranges[r] + (int)threadIdx.x; k < ranges[r + 1]; k += 1024) v += sl[k];
-            parts[r] = block_sum<1024>(v, red);
+            // four independent partial sums per thread (fixed order): their loads overlap
+            double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
+            int k = ranges[r] + (int)threadIdx.x;
+            for (; k + 3 * 1024 < ranges[r + 1]; k += 4 * 1024) {
+                v0 += sl[k];
+                v1 += sl[k + 1024];
+                v2 += sl[k + 2 * 1024];
+                v3 += sl[k + 3 * 1024];
+            }
+            for (; k < ranges[r + 1]; k += 1024) v0 += sl[k];
+            parts[r] = block_sum<1024>((v0 + v1) + (v2 + v3), red);
         }
         if (threadIdx.x == 0) {
             P.parts[5 * s + 0] = parts[0];
@@ -666,7 +679,7 @@ hipError_t launch_pme_state(const PmeDesc& D, const PmePtrs& P, int state, hipSt
     k_pme_cellsort<<<D.ncell, 256, 0, st>>>(D, P);
     // neighbour offset fastest, chunk slowest: consecutive workgroups (dealt round-robin to the
     // XCDs) carry comparable work, and the mostly empty late chunks come last
-    k_pme_direct<<<dim3(kPmeHalfShell, D.ncell, D.chunks), 64, 0, st>>>(D, P, F, es);
+    k_pme_direct<<<dim3(kPmeHalfShell * kPmeJSplit, D.ncell, D.chunks), 64, 0, st>>>(D, P, F, es);
     k_pme_exceptions<<<blocks_of(D.nex, 256) > 0 ? blocks_of(D.nex, 256) : 1, 256, 0, st>>>(D, P, x, F,
                                                                                                es + D.nslotDirect);
     if (D.tileBytes > 0)  // the cell's grid tile fits in LDS

@@ -265,11 +265,11 @@ void set_box(PmeForce& S, const double box[3]) {
     S.D.recipPhiScale = ATM_ONE_4PI_EPS0 / (kPi * V);
     S.D.dispersionEnergy = S.dispCoef / V;
     const double mean = (double)S.n / S.D.ncell;
-    int cap = S.cellCap > 0 ? S.cellCap : (int)std::ceil(3.0 * mean + 64.0);
+    int cap = S.cellCap > 0 ? S.cellCap : (int)std::ceil(1.5 * mean + 64.0);  // grown on overflow
     cap = std::min(kPmeCellMax, (cap + 63) / 64 * 64);
     S.D.cellCap = cap;
     S.D.chunks = cap / 64;
-    S.D.nslotDirect = S.D.chunks * kPmeHalfShell * S.D.ncell;
+    S.D.nslotDirect = S.D.chunks * kPmeHalfShell * kPmeJSplit * S.D.ncell;
     S.D.nslotEx = std::max(1, (S.D.nex + 255) / 256);
     S.D.nslot = S.D.nslotDirect + S.D.nslotEx + (S.D.ngrid + 255) / 256;
     // charge-grid tile of one cell: grid indices [floor(c K/m) - 5, ... + ceil(K/m) + 7)
