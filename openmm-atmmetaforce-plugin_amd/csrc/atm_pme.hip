@@ -654,6 +654,18 @@ __global__ void k_pme_fill(unsigned long long* p, long long n) {
     if (k < n) p[k] = 0ull;
 }
 
+// Three buffers cleared by one launch (each launch of a graph costs a few us):
+// thread k clears word k of the concatenation a | b | c.
+__global__ void k_pme_fill3(unsigned long long* a, long long na, unsigned long long* b, long long nb,
+                            unsigned long long* c, long long nc) {
+    long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < na) { a[k] = 0ull; return; }
+    k -= na;
+    if (k < nb) { b[k] = 0ull; return; }
+    k -= nb;
+    if (k < nc) c[k] = 0ull;
+}
+
 static inline unsigned blocks_of(long long n, int t) { return (unsigned)((n + t - 1) / t); }
 
 hipError_t launch_pme_fill(unsigned long long* p, long long n, hipStream_t st) {
@@ -669,10 +681,11 @@ hipError_t launch_pme_state(const PmeDesc& D, const PmePtrs& P, int state, hipSt
     if (state == 1) {
         k_pme_displace<<<blocks_of(3ll * D.n, 256), 256, 0, st>>>(P, D.n);
     }
-    k_pme_fill<<<blocks_of(D.ncell, 256), 256, 0, st>>>(reinterpret_cast<unsigned long long*>(P.cellCount),
-                                                        (D.ncell + 1) / 2);
-    k_pme_fill<<<blocks_of(D.ngrid, 256), 256, 0, st>>>(reinterpret_cast<unsigned long long*>(P.qgrid), D.ngrid);
-    k_pme_fill<<<blocks_of(3ll * D.n, 256), 256, 0, st>>>(F, 3ll * D.n);
+    {
+        const long long nc = (D.ncell + 1) / 2, ng = D.ngrid, nf = 3ll * D.n;
+        k_pme_fill3<<<blocks_of(nc + ng + nf, 256), 256, 0, st>>>(reinterpret_cast<unsigned long long*>(P.cellCount), nc,
+                                                                  reinterpret_cast<unsigned long long*>(P.qgrid), ng, F, nf);
+    }
     k_pme_bin<<<blocks_of(D.n, 256), 256, 0, st>>>(D, P, x);  // 256 threads: the histogram loops assume it
     k_pme_scan<<<1, 1024, 0, st>>>(D, P);
     k_pme_scatter<<<blocks_of(D.n, 256), 256, 0, st>>>(D, P);
