@@ -2,8 +2,9 @@
 // particle-mesh Ewald evaluated on x and on x + d, then the ATM combine.
 //
 // Per coordinate set (state s = 0: x, 1: x + d):
-//   k_pme_displace   x + d (state 1)
-//   k_pme_bin        wrap into the box, cell of edge >= rc, rank in the cell
+//   k_pme_fill3      clear the cell counts, charge grid and force sums
+//   k_pme_bin        (state 1: form x + d) wrap into the box, cell of edge
+//                    >= rc, rank in the cell
 //   k_pme_scan       cell starts (one block)
 //   k_pme_scatter    atom indices into cell order
 //   k_pme_cellsort   each cell ordered by atom index (deterministic sums), and
@@ -11,13 +12,14 @@
 //                    cell order
 //   k_pme_direct     erfc(alpha r)/r Coulomb + LJ over minimum-image pairs with
 //                    r < rc that are not exceptions; one wave per (64 atoms of
-//                    a cell, the cell or one of its 13 forward neighbours):
-//                    each pair once
+//                    a cell, the cell or one of its 13 forward neighbours, a
+//                    quarter of that cell's atoms): each pair once
 //   k_pme_spread_tiles  order-5 B-spline charge spreading per cell into an LDS
 //                    tile of the grid, 64-bit fixed point (k_pme_spread: the
 //                    same straight into the grid, when a tile does not fit)
 //   k_pme_dft_lines  one axis of the 3-D complex DFT, lines staged in LDS (run
-//                    3x forward, 3x back; k_pme_dft for lines too long for LDS)
+//                    3x forward, the first reading the fixed-point charge grid,
+//                    3x back; k_pme_q2c + k_pme_dft for lines too long for LDS)
 //   k_pme_convolve   influence function, reciprocal energy, convolution
 //   k_pme_gather     reciprocal forces from the potential grid
 //   k_pme_exceptions exception pairs: own Coulomb + LJ, minus erf(alpha r)/r
@@ -100,18 +102,14 @@ __device__ __forceinline__ void bspline5(double w, double th[kOrder], double dth
 
 }  // namespace
 
-__global__ void k_pme_displace(PmePtrs P, int n) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= 3 * n) return;
-    P.x2[i] = P.x[i] + P.displ[i];
-}
-
 // Cell of every atom and its rank among the cell's atoms (any order: the cell
 // sort fixes it).  Ranks come from a block histogram in LDS and one global
 // atomic per (block, occupied cell), so the few hundred atoms of a cell do not
 // serialise on one counter; grids with more than kPmeBinLds cells use global
 // atomics directly.
-__global__ void __launch_bounds__(256) k_pme_bin(PmeDesc D, PmePtrs P, const double* x) {
+// d: when not null, the positions binned are x + d (the displaced state), also
+// written to P.x2 for the kernels after this one.
+__global__ void __launch_bounds__(256) k_pme_bin(PmeDesc D, PmePtrs P, const double* x, const double* d) {
     __shared__ int hist[kPmeBinLds];
     __shared__ int base[kPmeBinLds];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -122,10 +120,16 @@ __global__ void __launch_bounds__(256) k_pme_bin(PmeDesc D, PmePtrs P, const dou
     }
     int c = -1, r = 0;
     if (i < D.n) {
-        double w[3];
-        const int cx = wrap_cell(x[3 * i], D.L[0], D.invL[0], D.m[0], &w[0]);
-        const int cy = wrap_cell(x[3 * i + 1], D.L[1], D.invL[1], D.m[1], &w[1]);
-        const int cz = wrap_cell(x[3 * i + 2], D.L[2], D.invL[2], D.m[2], &w[2]);
+        double w[3], xi[3] = {x[3 * i], x[3 * i + 1], x[3 * i + 2]};
+        if (d) {
+            for (int k = 0; k < 3; ++k) {
+                xi[k] += d[3 * i + k];
+                P.x2[3 * i + k] = xi[k];
+            }
+        }
+        const int cx = wrap_cell(xi[0], D.L[0], D.invL[0], D.m[0], &w[0]);
+        const int cy = wrap_cell(xi[1], D.L[1], D.invL[1], D.m[1], &w[1]);
+        const int cz = wrap_cell(xi[2], D.L[2], D.invL[2], D.m[2], &w[2]);
         c = (cx * D.m[1] + cy) * D.m[2] + cz;
         P.cellOf[i] = c;
         P.wpos[i] = make_double4(w[0], w[1], w[2], 0.0);
@@ -409,8 +413,10 @@ __global__ void k_pme_dft(PmeDesc D, const double2* in, double2* out, const doub
 // One wave per grid line along `axis` (four lines per block): the line and the
 // axis' twiddles are staged in LDS, then lane k (k + 64, ...) forms output k.
 // Dynamic LDS: 5 K complex values.
+// qfix: when not null, the input is the fixed-point charge grid (real), read in
+// place of `in` (the first forward pass needs no separate conversion kernel).
 __global__ void __launch_bounds__(256) k_pme_dft_lines(PmeDesc D, const double2* in, double2* out, const double2* tw,
-                                                      int axis, double sign) {
+                                                      int axis, double sign, const long long* qfix) {
     extern __shared__ double2 sm[];
     const int K = D.K[axis];
     const int stride = axis == 2 ? 1 : (axis == 1 ? D.K[2] : D.K[1] * D.K[2]);
@@ -423,7 +429,10 @@ __global__ void __launch_bounds__(256) k_pme_dft_lines(PmeDesc D, const double2*
     int base = 0;
     if (L < nlines) {
         base = (L / stride) * K * stride + L % stride;
-        for (int j = lane; j < K; j += 64) line[j] = in[base + j * stride];
+        if (qfix)
+            for (int j = lane; j < K; j += 64) line[j] = make_double2((double)qfix[base + j * stride] * (1.0 / kGridFix), 0.0);
+        else
+            for (int j = lane; j < K; j += 64) line[j] = in[base + j * stride];
     }
     __syncthreads();
     if (L >= nlines) return;
@@ -678,15 +687,13 @@ hipError_t launch_pme_state(const PmeDesc& D, const PmePtrs& P, int state, hipSt
     const double* x = state == 0 ? P.x : P.x2;
     unsigned long long* F = P.F + 3ll * D.n * state;
     double* es = P.eslot + (long long)D.nslot * state;
-    if (state == 1) {
-        k_pme_displace<<<blocks_of(3ll * D.n, 256), 256, 0, st>>>(P, D.n);
-    }
     {
         const long long nc = (D.ncell + 1) / 2, ng = D.ngrid, nf = 3ll * D.n;
         k_pme_fill3<<<blocks_of(nc + ng + nf, 256), 256, 0, st>>>(reinterpret_cast<unsigned long long*>(P.cellCount), nc,
                                                                   reinterpret_cast<unsigned long long*>(P.qgrid), ng, F, nf);
     }
-    k_pme_bin<<<blocks_of(D.n, 256), 256, 0, st>>>(D, P, x);  // 256 threads: the histogram loops assume it
+    // 256 threads: the histogram loops assume it; state 1 forms x + d here
+    k_pme_bin<<<blocks_of(D.n, 256), 256, 0, st>>>(D, P, state == 1 ? P.x : x, state == 1 ? P.displ : nullptr);
     k_pme_scan<<<1, 1024, 0, st>>>(D, P);
     k_pme_scatter<<<blocks_of(D.n, 256), 256, 0, st>>>(D, P);
     k_pme_cellsort<<<D.ncell, 256, 0, st>>>(D, P);
@@ -699,17 +706,19 @@ hipError_t launch_pme_state(const PmeDesc& D, const PmePtrs& P, int state, hipSt
         k_pme_spread_tiles<<<D.ncell, 256, D.tileBytes, st>>>(D, P);
     else
         k_pme_spread<<<blocks_of(D.n, 256), 256, 0, st>>>(D, P, x);
-    k_pme_q2c<<<blocks_of(D.ngrid, 256), 256, 0, st>>>(D, P);
     const unsigned gb = blocks_of(D.ngrid, 256);
-    auto dft = [&](const double2* in, double2* out, int axis, double sign) {
+    auto dft = [&](const double2* in, double2* out, int axis, double sign, const long long* qfix = nullptr) {
         const int K = D.K[axis];
-        if (K <= kPmeDftLdsMax)
+        if (K <= kPmeDftLdsMax) {
             k_pme_dft_lines<<<blocks_of(D.ngrid / K, 4), 256, 5 * K * sizeof(double2), st>>>(D, in, out, P.tw + D.twOff[axis],
-                                                                                          axis, sign);
-        else
+                                                                                          axis, sign, qfix);
+        } else {
+            if (qfix) k_pme_q2c<<<gb, 256, 0, st>>>(D, P);  // into g0 = in
             k_pme_dft<<<gb, 256, 0, st>>>(D, in, out, P.tw + D.twOff[axis], axis, sign);
+        }
     };
-    dft(P.g0, P.g1, 2, -1.0);
+    // first pass straight from the fixed-point charge grid
+    dft(P.g0, P.g1, 2, -1.0, reinterpret_cast<const long long*>(P.qgrid));
     dft(P.g1, P.g0, 1, -1.0);
     dft(P.g0, P.g1, 0, -1.0);
     k_pme_convolve<<<gb, 256, 0, st>>>(D, P, P.g1, es + D.nslotDirect + D.nslotEx);
