@@ -53,6 +53,12 @@ __device__ __forceinline__ void add_fixed(unsigned long long* p, double f) {
     atomicAdd(p, (unsigned long long)__double2ll_rn(f * kFix));
 }
 
+// Grid index v in [-K, 2K) wrapped into [0, K) (no integer division).
+__device__ __forceinline__ int wrap_index(int v, int K) {
+    v = v < 0 ? v + K : v;
+    return v >= K ? v - K : v;
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;
@@ -491,22 +497,23 @@ __global__ void __launch_bounds__(256) k_pme_spread_tiles(PmeDesc D, PmePtrs P) 
                 if (la >= 0 && la < T0 && lb >= 0 && lb < T1 && lc >= 0 && lc < T2) {
                     atomicAdd(&tile[(la * T1 + lb) * T2 + lc], v);
                 } else {
-                    const int ia = ((g0[0] - a) % D.K[0] + D.K[0]) % D.K[0];
-                    const int ib = ((g0[1] - b) % D.K[1] + D.K[1]) % D.K[1];
-                    const int ic = ((g0[2] - cc) % D.K[2] + D.K[2]) % D.K[2];
+                    const int ia = wrap_index(g0[0] - a, D.K[0]);
+                    const int ib = wrap_index(g0[1] - b, D.K[1]);
+                    const int ic = wrap_index(g0[2] - cc, D.K[2]);
                     atomicAdd(reinterpret_cast<unsigned long long*>(P.qgrid) + ((long long)ia * D.K[1] + ib) * D.K[2] + ic, v);
                 }
             }
         }
     }
     __syncthreads();
+    // tile index o + l lies in [-5, K + 3): one wrap per axis
     for (int k = threadIdx.x; k < TT; k += 256) {
         const unsigned long long v = tile[k];
         if (v == 0ull) continue;
         const int lc = k % T2, lb = (k / T2) % T1, la = k / (T1 * T2);
-        const int ia = ((o[0] + la) % D.K[0] + D.K[0]) % D.K[0];
-        const int ib = ((o[1] + lb) % D.K[1] + D.K[1]) % D.K[1];
-        const int ic = ((o[2] + lc) % D.K[2] + D.K[2]) % D.K[2];
+        const int ia = wrap_index(o[0] + la, D.K[0]);
+        const int ib = wrap_index(o[1] + lb, D.K[1]);
+        const int ic = wrap_index(o[2] + lc, D.K[2]);
         atomicAdd(reinterpret_cast<unsigned long long*>(P.qgrid) + ((long long)ia * D.K[1] + ib) * D.K[2] + ic, v);
     }
 }
@@ -534,39 +541,60 @@ __global__ void __launch_bounds__(256) k_pme_convolve(PmeDesc D, PmePtrs P, doub
     if (threadIdx.x == 0) eslot[blockIdx.x] = s * D.recipEnergyScale;
 }
 
-__global__ void k_pme_gather(PmeDesc D, PmePtrs P, const double* x, const double2* phi, unsigned long long* F) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= D.n) return;
-    const double q = P.prm[i].y;
-    if (q == 0.0) return;
-    double th[3][kOrder], dth[3][kOrder];
-    int g0[3];
-    for (int d = 0; d < 3; ++d) {
-        double f = x[3 * i + d] * D.invL[d];
-        f -= floor(f);
-        const double u = f * D.K[d];
-        const int g = (int)floor(u);
-        g0[d] = g;
-        bspline5(u - g, th[d], dth[d]);
-    }
+// Reciprocal forces, 8 lanes per atom: lane a < 5 of the octet sums the 25 grid
+// points of B-spline index a, then the octet adds its partial gradients
+// (xor shuffles 1, 2, 4; lanes 5-7 add zeros).  Eight atoms per wave.
+__global__ void __launch_bounds__(256) k_pme_gather(PmeDesc D, PmePtrs P, const double* x, const double2* phi,
+                                                   unsigned long long* F) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = t >> 3, a = t & 7;
+    const bool on = i < D.n;
+    const double q = on ? P.prm[i].y : 0.0;
     double gx = 0.0, gy = 0.0, gz = 0.0;
-    for (int a = 0; a < kOrder; ++a) {
-        const int ia = ((g0[0] - a) % D.K[0] + D.K[0]) % D.K[0];
+    if (on && q != 0.0 && a < kOrder) {
+        double th[3][kOrder], dth[3][kOrder];
+        int g0[3];
+        for (int d = 0; d < 3; ++d) {
+            double f = x[3 * i + d] * D.invL[d];
+            f -= floor(f);
+            const double u = f * D.K[d];
+            const int g = (int)floor(u);
+            g0[d] = g;
+            bspline5(u - g, th[d], dth[d]);
+        }
+        int ic[kOrder];  // g0 in [0, K]: g0 - k in [-4, K]
+#pragma unroll
+        for (int cc = 0; cc < kOrder; ++cc) ic[cc] = wrap_index(g0[2] - cc, D.K[2]);
+        // th[0][a], dth[0][a] with a run-time a: select from the unrolled values
+        double t0 = th[0][0], dt0 = dth[0][0];
+#pragma unroll
+        for (int k = 1; k < kOrder; ++k)
+            if (a == k) { t0 = th[0][k]; dt0 = dth[0][k]; }
+        const int ia = wrap_index(g0[0] - a, D.K[0]);
+#pragma unroll
         for (int b = 0; b < kOrder; ++b) {
-            const int ib = ((g0[1] - b) % D.K[1] + D.K[1]) % D.K[1];
+            const int ib = wrap_index(g0[1] - b, D.K[1]);
+            const long long row = ((long long)ia * D.K[1] + ib) * D.K[2];
+#pragma unroll
             for (int cc = 0; cc < kOrder; ++cc) {
-                const int ic = ((g0[2] - cc) % D.K[2] + D.K[2]) % D.K[2];
-                const double p = phi[((long long)ia * D.K[1] + ib) * D.K[2] + ic].x;
-                gx += dth[0][a] * th[1][b] * th[2][cc] * p;
-                gy += th[0][a] * dth[1][b] * th[2][cc] * p;
-                gz += th[0][a] * th[1][b] * dth[2][cc] * p;
+                const double p = phi[row + ic[cc]].x;
+                gx += dt0 * th[1][b] * th[2][cc] * p;
+                gy += t0 * dth[1][b] * th[2][cc] * p;
+                gz += t0 * th[1][b] * dth[2][cc] * p;
             }
         }
     }
-    const double s = -q * D.recipPhiScale;
-    add_fixed(F + i, s * gx * D.K[0] * D.invL[0]);
-    add_fixed(F + D.n + i, s * gy * D.K[1] * D.invL[1]);
-    add_fixed(F + 2 * D.n + i, s * gz * D.K[2] * D.invL[2]);
+#pragma unroll
+    for (int o = 1; o < 8; o <<= 1) {
+        gx += __shfl_xor(gx, o, 64);
+        gy += __shfl_xor(gy, o, 64);
+        gz += __shfl_xor(gz, o, 64);
+    }
+    if (on && q != 0.0 && a < 3) {
+        const double s = -q * D.recipPhiScale;
+        const double g = a == 0 ? gx * D.K[0] * D.invL[0] : (a == 1 ? gy * D.K[1] * D.invL[1] : gz * D.K[2] * D.invL[2]);
+        add_fixed(F + (long long)a * D.n + i, s * g);
+    }
 }
 
 __global__ void __launch_bounds__(256) k_pme_exceptions(PmeDesc D, PmePtrs P, const double* x, unsigned long long* F,
@@ -725,7 +753,7 @@ hipError_t launch_pme_state(const PmeDesc& D, const PmePtrs& P, int state, hipSt
     dft(P.g1, P.g0, 0, 1.0);
     dft(P.g0, P.g1, 1, 1.0);
     dft(P.g1, P.g0, 2, 1.0);
-    k_pme_gather<<<blocks_of(D.n, 256), 256, 0, st>>>(D, P, x, P.g0, F);
+    k_pme_gather<<<blocks_of(8ll * D.n, 256), 256, 0, st>>>(D, P, x, P.g0, F);
     return hipGetLastError();
 }
 
