@@ -10,6 +10,7 @@ namespace atm {
 constexpr int kPmeCellMax = 1024;  // atoms per cell the cell sort holds
 constexpr int kPmeJBatch = 64;     // neighbour-cell atoms staged in LDS per batch (direct kernel)
 constexpr int kPmeJSplit = 4;      // parts of a neighbour cell's atoms taken by separate waves
+constexpr int kPmeSpreadSplit = 4; // parts of a cell's atoms spread by separate blocks (own LDS tiles)
 constexpr int kPmeHalfShell = 14;  // the cell itself and its 13 forward neighbour cells
 constexpr int kPmeBinLds = 4096;   // cell grids up to this size are binned through LDS histograms
 constexpr int kPmeDftLdsMax = 600; // grid lines up to this length are transformed from LDS

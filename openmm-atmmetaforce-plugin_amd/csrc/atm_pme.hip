@@ -465,13 +465,15 @@ __global__ void __launch_bounds__(256) k_pme_dft_lines(PmeDesc D, const double2*
 // the cell edge) goes straight to the global grid.
 __global__ void __launch_bounds__(256) k_pme_spread_tiles(PmeDesc D, PmePtrs P) {
     extern __shared__ unsigned long long tile[];
-    const int c = blockIdx.x;
+    // block (c, part): part `part` of kPmeSpreadSplit contiguous parts of cell c's atoms
+    const int c = blockIdx.x / kPmeSpreadSplit, part = blockIdx.x % kPmeSpreadSplit;
     const int cz = c % D.m[2], cy = (c / D.m[2]) % D.m[1], cx = c / (D.m[1] * D.m[2]);
     const int o[3] = {(cx * D.K[0]) / D.m[0] - 5, (cy * D.K[1]) / D.m[1] - 5, (cz * D.K[2]) / D.m[2] - 5};
     const int T0 = D.tile[0], T1 = D.tile[1], T2 = D.tile[2], TT = T0 * T1 * T2;
     for (int k = threadIdx.x; k < TT; k += 256) tile[k] = 0ull;
     __syncthreads();
-    const int s0 = P.cellStart[c], cnt = min(P.cellStart[c + 1] - s0, D.cellCap);
+    const int cs = P.cellStart[c], call = min(P.cellStart[c + 1] - cs, D.cellCap);
+    const int s0 = cs + (call * part) / kPmeSpreadSplit, cnt = cs + (call * (part + 1)) / kPmeSpreadSplit - s0;
     // task = (atom, first B-spline index a): 25 points each
     for (int task = threadIdx.x; task < cnt * kOrder; task += 256) {
         const int sl = s0 + task / kOrder, a = task % kOrder;
@@ -731,7 +733,7 @@ hipError_t launch_pme_state(const PmeDesc& D, const PmePtrs& P, int state, hipSt
     k_pme_exceptions<<<blocks_of(D.nex, 256) > 0 ? blocks_of(D.nex, 256) : 1, 256, 0, st>>>(D, P, x, F,
                                                                                                es + D.nslotDirect);
     if (D.tileBytes > 0)  // the cell's grid tile fits in LDS
-        k_pme_spread_tiles<<<D.ncell, 256, D.tileBytes, st>>>(D, P);
+        k_pme_spread_tiles<<<D.ncell * kPmeSpreadSplit, 256, D.tileBytes, st>>>(D, P);
     else
         k_pme_spread<<<blocks_of(D.n, 256), 256, 0, st>>>(D, P, x);
     const unsigned gb = blocks_of(D.ngrid, 256);
