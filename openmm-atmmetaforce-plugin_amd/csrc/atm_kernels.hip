@@ -669,7 +669,7 @@ __device__ __forceinline__ void flush_component_u(const DevPtrs& P, int c, int s
 // At most 80 VGPRs (6 waves per SIMD): the FP64 forces variant with its two
 // j-atom buffers then spills a few per-batch values outside the entry loop;
 // on a 22 x 23k batch that measured 5 % faster than its unconstrained 88
-// VGPRs (5 waves) and 12 % faster than 72 VGPRs (7 waves, which spill inside
+// VGPRs (5 waves) and 14 % faster than 72 VGPRs (7 waves, which spill inside
 // the entry loop).
 template <int W, bool FORCES, bool F64>
 __global__ void __launch_bounds__(64 * W, 6) k_pairs(DevPtrs P, int setBegin) {
