@@ -204,7 +204,9 @@ int atm_backward(void* context, const double* grad_energy, double* d_charges,
 int atm_set_stream(void* context, void* hip_stream);
 
 /* Time the candidate launch configurations on the current inputs and keep the
- * fastest (cell shape, z-binning, pair-kernel variant and waves per block). */
+ * fastest (cell shape, z-binning, pair-kernel variant and waves per block; the
+ * four fastest of the sweep are timed again with 4x the repeats).  A context
+ * starts at column scale 1.3, 2 waves per block, 4 z-bins, pair kernel 0. */
 int atm_autotune(void* context, int32_t repeats, atm_tuning* out);
 int atm_get_tuning(void* context, atm_tuning* out);
 int atm_set_tuning(void* context, const atm_tuning* tuning);
