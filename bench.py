@@ -163,7 +163,7 @@ def config_report(steps=50, cpu_seconds=5.0):
             b.evaluate(forces=True, clear=True)
         b.synchronize()
         dt = time.perf_counter() - t0
-        pairs = b.stats()["pairs_in_cutoff"]
+        pairs = int(b.list_stats()["pairs"].sum())
         b.compute()
         worst = {"U1_rel": 0.0, "U2_rel": 0.0, "W_rel": 0.0, "force_rel_rms": 0.0}
         t_or = time.perf_counter()
@@ -423,8 +423,8 @@ def main():
         log("[bench] autotuning")
         tuning = batch.autotune(repeats=5)
         batch.compute()
-    stats = batch.stats()
-    pairs = stats["pairs_in_cutoff"]
+    # in-cutoff pairs counted by the pair kernel itself (lane masks of its exact r < rc tests)
+    pairs = int(batch.list_stats()["pairs"].sum())
 
     log(f"[bench] warmup {args.warmup}")
     for _ in range(args.warmup):

@@ -27,8 +27,11 @@
  *  - A context is not thread-safe: calls on one context must not overlap,
  *    matching the single-threaded use of an OpenMM Context by the reference.
  *  - Pair forces accumulate in 64-bit fixed point (ATM_FORCE_SCALE units per
- *    kJ/mol/nm; partial forces beyond 2^31 kJ/mol/nm in an FP64 side
- *    accumulator), so their sums do not depend on summation order.  The
+ *    kJ/mol/nm), so their sums do not depend on summation order.  Partial
+ *    forces at or above a per-set limit (at most 2^27 kJ/mol/nm, lowered so
+ *    that the number of terms one atom's sum can receive times the limit stays
+ *    below the int64 range: the sums cannot wrap) go to an FP64 side
+ *    accumulator instead.  The
  *    blended ATM force of a system is accumulated in FP64: it is the sum over
  *    evaluations since its last atm_clear_forces() (ATM_EVAL_CLEAR_FORCES
  *    clears it first), mirroring the reference's "+=" into the context force
@@ -43,7 +46,7 @@
 extern "C" {
 #endif
 
-#define ATM_MI355X_ABI_VERSION 3
+#define ATM_MI355X_ABI_VERSION 4
 
 /* status codes */
 #define ATM_OK 0
@@ -113,7 +116,9 @@ typedef struct atm_tuning {
                                  = (4,2,yes), (4,2,no), (4,4,yes), (8,2,yes); 5: atom-level
                                  j lists per i-cluster (j atoms within the list cutoff of
                                  the cluster's box), 8 i x 8 j lanes per step (FP64 only) */
-    int32_t reserved;
+    int32_t list_capacity;    /* neighbour-list entries per group to start from; 0 = sized
+                                 from the density.  A list that overflows grows it (the
+                                 evaluation reports ATM_ERR_CAPACITY and is re-run) */
 } atm_tuning;
 
 /* Library version string and ABI number. */
@@ -221,10 +226,21 @@ typedef struct atm_stage_times {
 int atm_profile(void* context, int32_t flags, int32_t reps, atm_stage_times* out);
 
 /* Counters of the last evaluation, for measurement: number of in-cutoff pairs
- * (both states, all systems), candidate pairs evaluated by the pair kernel,
- * and neighbour-list entries. */
+ * (both states, all systems) as counted by the pair kernel that ran, candidate
+ * pairs in the cluster-pair tiles of the lists (a traversal of the lists), and
+ * neighbour-list entries.  Any pointer may be NULL. */
 int atm_get_stats(void* context, int64_t* pairs_in_cutoff, int64_t* pairs_evaluated,
                   int64_t* list_entries);
+
+/* Per coordinate set of the last evaluation (arrays of 2 * num_systems; set
+ * 2 s + state is system s at x (state 0) or x + d (state 1)): the in-cutoff
+ * pairs the pair kernel evaluated (each i < j once; counted from the lane masks
+ * of its exact r < rc decisions), the longest neighbour list of one group
+ * (before truncation to the capacity) and the list capacity per group.  Any
+ * pointer may be NULL.  The parity counterpart of the pair count is the size
+ * of the FP64 oracle's pair set, which OpenMM's NonbondedForce evaluates at
+ * ATMMetaForceImpl.cpp:113,116. */
+int atm_get_list_stats(void* context, int64_t* pairs, int32_t* max_entries, int32_t* capacity);
 
 /* ------------------------------------------------------------------------
  * Flat-bottom restraints evaluated beside the ATM force every step (force

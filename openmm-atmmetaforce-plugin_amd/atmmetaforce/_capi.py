@@ -34,7 +34,7 @@ EXPORTED = (
     "atm_set_particles", "atm_set_displacements", "atm_set_positions", "atm_set_positions_device",
     "atm_set_box", "atm_set_params", "atm_eval", "atm_synchronize", "atm_get_results", "atm_get_forces",
     "atm_get_forces_fixed", "atm_clear_forces", "atm_get_pairs", "atm_backward", "atm_set_stream",
-    "atm_autotune", "atm_get_tuning", "atm_set_tuning", "atm_get_stats", "atm_profile",
+    "atm_autotune", "atm_get_tuning", "atm_set_tuning", "atm_get_stats", "atm_get_list_stats", "atm_profile",
     "atm_restraints_create", "atm_restraints_destroy", "atm_restraints_set", "atm_restraints_eval",
     "atm_md_init", "atm_md_step", "atm_md_get",
     "atm_pme_create", "atm_pme_destroy", "atm_pme_get_info", "atm_pme_set_particles", "atm_pme_set_exceptions",
@@ -65,7 +65,7 @@ class Result(ctypes.Structure):
 class Tuning(ctypes.Structure):
     _fields_ = [("column_scale", ctypes.c_double), ("pair_waves_per_block", ctypes.c_int32),
                 ("zbins_per_column", ctypes.c_int32), ("best_ms", ctypes.c_double),
-                ("pair_kernel", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("pair_kernel", ctypes.c_int32), ("list_capacity", ctypes.c_int32)]
 
 
 class StageTimes(ctypes.Structure):
@@ -143,6 +143,7 @@ def load(path: str | None = None):
         "atm_set_tuning": [vp, ctypes.POINTER(Tuning)],
         "atm_get_stats": [vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
                           ctypes.POINTER(ctypes.c_int64)],
+        "atm_get_list_stats": [vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(i32), ctypes.POINTER(i32)],
         "atm_profile": [vp, i32, i32, ctypes.POINTER(StageTimes)],
         "atm_restraints_create": [i32, ctypes.POINTER(vp)],
         "atm_restraints_destroy": [vp],

@@ -16,7 +16,9 @@ closed-form derivatives of the softplus in the five scalar parameters.
 Pair terms of the backward are FP32, their sums FP64.
 
 If the batch was evaluated again between a forward and its backward, the
-backward re-runs that forward's evaluation first.
+backward re-runs that forward's evaluation first: at the forward's parameters
+and at its positions and boxes (restored from the forward's snapshot; a
+backward whose positions were since changed on the device raises).
 """
 from __future__ import annotations
 
@@ -47,6 +49,7 @@ class ATMEnergyFunction(torch.autograd.Function):
         W = np.array([batch.result(k)["energy"] for k in range(S)])
         ctx.batch = batch
         ctx.state = state
+        ctx.positions = batch.position_snapshot()
         ctx.eval_id = batch._eval_id
         ctx.meta = [(t.dtype, t.device) for t in (charges, sigmas, epsilons, *scal)]
         return torch.as_tensor(W, dtype=torch.float64, device=charges.device)
@@ -55,6 +58,7 @@ class ATMEnergyFunction(torch.autograd.Function):
     def backward(ctx, gW):
         batch = ctx.batch
         if batch._eval_id != ctx.eval_id:
+            batch.restore_positions(ctx.positions)
             _upload(batch, *ctx.state)
         g = batch.backward(_np(gW))
         outs = [g["dq"], g["dsigma"], g["deps"]] + [g["dparams"][:, k] for k in range(5)]

@@ -54,6 +54,11 @@ class ATMBatch:
         self._h = h
         self._eval_id = 0
         self.num_atoms = [int(s.positions.shape[0]) for s in self.systems]
+        # what the device holds, for ATMEnergyFunction's re-evaluation: positions as last set
+        # from the host (None once set on the device), boxes, and a version bumped on changes
+        self._pos_version = 0
+        self._host_positions = [None] * n
+        self._boxes = [np.array(s.box, dtype=np.float64).reshape(3) for s in self.systems]
         for k, s in enumerate(self.systems):
             self.set_particles(k, s.charges, s.sigmas, s.epsilons, s.displacements)
             self.set_positions(k, s.positions)
@@ -102,13 +107,41 @@ class ATMBatch:
     def set_positions(self, k, positions):
         x = np.ascontiguousarray(positions, dtype=np.float64).reshape(self.num_atoms[k], 3)
         C.check(self._lib.atm_set_positions(self._h, k, _dptr(x)))
+        self._host_positions[k] = x.copy()
+        self._pos_version += 1
 
     def set_positions_device(self, ptr: int):
         C.check(self._lib.atm_set_positions_device(self._h, ctypes.c_void_p(ptr)))
+        self.positions_changed_on_device()
+
+    def positions_changed_on_device(self):
+        """Record that device code (a device position upload, MD steps) moved the atoms."""
+        self._host_positions = [None] * len(self.systems)
+        self._pos_version += 1
 
     def set_box(self, k, box):
         b = np.ascontiguousarray(box, dtype=np.float64).reshape(3)
         C.check(self._lib.atm_set_box(self._h, k, _dptr(b)))
+        self._boxes[k] = b.copy()
+        self._pos_version += 1
+
+    def position_snapshot(self):
+        """(version, host positions or None per system, boxes) of the current inputs."""
+        return (self._pos_version, list(self._host_positions), [b.copy() for b in self._boxes])
+
+    def restore_positions(self, snap):
+        """Upload the positions and boxes of a snapshot again (no-op if nothing changed)."""
+        version, xs, boxes = snap
+        if version == self._pos_version:
+            return
+        if any(x is None for x in xs):
+            raise RuntimeError("the positions of this evaluation were set on the device and have changed "
+                               "since; they cannot be restored")
+        for k, (x, b) in enumerate(zip(xs, boxes)):
+            if not np.array_equal(b, self._boxes[k]):
+                self.set_box(k, b)
+            self.set_positions(k, x)
+        self._pos_version = version
 
     def set_params(self, k, params):
         p = _params(params)
@@ -183,6 +216,18 @@ class ATMBatch:
         C.check(self._lib.atm_get_stats(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
         return {"pairs_in_cutoff": a.value, "pairs_evaluated": b.value, "list_entries": c.value}
 
+    def list_stats(self) -> dict:
+        """Per coordinate set (index 2 k + state) of the last evaluation: in-cutoff pairs counted
+        by the pair kernel, the longest group list and the list capacity per group."""
+        n = 2 * len(self.systems)
+        pairs = np.zeros(n, dtype=np.int64)
+        mx = np.zeros(n, dtype=np.int32)
+        cap = np.zeros(n, dtype=np.int32)
+        C.check(self._lib.atm_get_list_stats(self._h, pairs.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                             mx.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                             cap.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
+        return {"pairs": pairs, "max_entries": mx, "capacity": cap}
+
     def backward(self, grad_energy) -> dict:
         g = np.ascontiguousarray(grad_energy, dtype=np.float64).reshape(len(self.systems))
         tot = sum(self.num_atoms)
@@ -210,14 +255,17 @@ class ATMBatch:
         C.check(self._lib.atm_get_tuning(self._h, ctypes.byref(t)))
         return _tuning_dict(t)
 
-    def set_tuning(self, column_scale=1.3, pair_waves_per_block=2, zbins_per_column=4, pair_kernel=0):
+    def set_tuning(self, column_scale=1.3, pair_waves_per_block=2, zbins_per_column=4, pair_kernel=0,
+                   list_capacity=0):
         """pair_kernel 0: 8x8 cluster-pair tiles; 1-4: compacted in-cutoff pairs with (list entries
         per batch, copies of the j-force sums, drain after every test step) = (4, 2, yes), (4, 2, no),
-        (4, 4, yes), (8, 2, yes); 5: atom-level j lists per i-cluster, 8 i x 8 j lanes per step."""
-        t = C.Tuning(column_scale, pair_waves_per_block, zbins_per_column, 0.0, pair_kernel, 0)
+        (4, 4, yes), (8, 2, yes); 5: atom-level j lists per i-cluster, 8 i x 8 j lanes per step.
+        list_capacity: list entries per group to start from (0: sized from the density)."""
+        t = C.Tuning(column_scale, pair_waves_per_block, zbins_per_column, 0.0, pair_kernel, list_capacity)
         C.check(self._lib.atm_set_tuning(self._h, ctypes.byref(t)))
 
 
 def _tuning_dict(t) -> dict:
     return {"column_scale": t.column_scale, "pair_waves_per_block": t.pair_waves_per_block,
-            "zbins_per_column": t.zbins_per_column, "pair_kernel": t.pair_kernel, "best_ms": t.best_ms}
+            "zbins_per_column": t.zbins_per_column, "pair_kernel": t.pair_kernel, "best_ms": t.best_ms,
+            "list_capacity": t.list_capacity}

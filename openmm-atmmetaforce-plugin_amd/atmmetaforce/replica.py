@@ -22,7 +22,8 @@ Delta = B_i(u_b) + B_j(u_a) - B_i(u_a) - B_j(u_b), which leaves the joint
 distribution prod_a exp(-beta W_{s(a)}(x_a)) invariant (Metropolis).  Swaps
 are proposed between neighbouring states of the schedule, in a random order,
 several sweeps per cycle.  All states must share the direction (e0 must not
-depend on the state).
+depend on the state) and the soft-core parameters umax, ubcore and acore (u is
+the soft-core perturbation energy of the replica's own state).
 
 Units: nm, ps, K, kJ/mol (the sample lines use the reference's kcal/mol
 columns).
@@ -62,6 +63,17 @@ def exchange_log_ratio(beta: float, pi: ATMParams, pj: ATMParams, ua: float, ub:
     return -beta * delta
 
 
+def _check_states(states):
+    """The exchange rule evaluates each state's softplus bias at the other replica's
+    soft-core perturbation energy u_sc (computed with that replica's own umax,
+    ubcore, acore): it is exact only when every state shares those three
+    parameters and the direction, so schedules that vary them are rejected."""
+    if len({p.direction > 0 for p in states}) != 1:
+        raise ValueError("all states must have the same direction")
+    if len({(float(p.umax), float(p.ubcore), float(p.acore)) for p in states}) != 1:
+        raise ValueError("all states must share the soft-core parameters umax, ubcore and acore")
+
+
 @dataclasses.dataclass
 class Sample:
     cycle: int
@@ -99,8 +111,7 @@ class ATMReplicaExchange:
                  lambdas: Sequence[float] | None = None, device: int = 0):
         if len(states) < 1:
             raise ValueError("need at least one state")
-        if len({p.direction > 0 for p in states}) != 1:
-            raise ValueError("all states must have the same direction")
+        _check_states(states)
         self.states = list(states)
         self.lambdas = list(lambdas) if lambdas is not None else [0.5 * (p.lambda1 + p.lambda2) for p in states]
         self.temperature, self.friction, self.dt = float(temperature), float(friction), float(dt)
@@ -122,11 +133,13 @@ class ATMReplicaExchange:
         C.check(lib.atm_md_init(self.batch.handle, m.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
                                 x0.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), self.temperature,
                                 ctypes.c_uint64(int(seed) & 0xFFFFFFFFFFFFFFFF)))
+        self.batch.positions_changed_on_device()
         self.cycle = 0
 
     # ------------------------------------------------------------ dynamics
     def step(self, steps: int):
         C.check(self._lib.atm_md_step(self.batch.handle, int(steps), self.dt, self.friction, self.temperature))
+        self.batch.positions_changed_on_device()
 
     def state(self, replica: int):
         """(positions, velocities, kinetic energy) of a replica."""
@@ -226,8 +239,7 @@ class ContextReplicaExchange:
         self.forces = [f for _, _, f in replicas]
         force = self.forces[0]
         self.states = [p if isinstance(p, ATMParams) else ATMParams(**p) for p in states]
-        if len({p.direction > 0 for p in self.states}) != 1:
-            raise ValueError("all states must have the same direction")
+        _check_states(self.states)
         self.lambdas = list(lambdas) if lambdas is not None else [0.5 * (p.lambda1 + p.lambda2) for p in self.states]
         self.temperature = float(temperature)
         self.beta = 1.0 / (KB * self.temperature)

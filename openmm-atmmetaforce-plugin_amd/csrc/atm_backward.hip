@@ -8,9 +8,10 @@
 //   q' = q sqrt(k_e):        dE/dq'_i  = q'_j (1/r + k_rf r^2 - c_rf)
 //   h  = sigma/2:            dE/dh_i   = e'_i e'_j (12 (s/r)^12 - 6 (s/r)^6) / s
 //   e' = 2 sqrt(eps):        dE/de'_i  = e'_j ((s/r)^12 - (s/r)^6)
-// (s = h_i + h_j).  The gather kernel sums both states and applies the chain
-// rule to physical units: dq = sqrt(k_e) dE/dq', dsigma = dE/dh / 2,
-// deps = dE/de' * 2 / e'.
+// (s = h_i + h_j; dE/dh = 0 where s = 0).  The gather kernel sums both states
+// and applies the chain rule to physical units: dq = sqrt(k_e) dE/dq',
+// dsigma = dE/dh / 2, deps = dE/de' * 2 / e' (0 for an atom with eps = 0, where
+// the derivative does not exist).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include "atm_internal.h"
@@ -102,7 +103,8 @@ __global__ void __launch_bounds__(64) k_backward(DevPtrs P, int setBegin, const 
                 const float lj = fmaf(s6, s6, -s6);                // s12 - s6
                 const float dlj = s6 * fmaf(12.0f, s6, -6.0f);     // 12 s12 - 6 s6
                 const float eps4 = qi.y * qj.y;
-                const float dh = in ? eps4 * dlj / sig : 0.f;
+                // dE/dh = 4 eps (12 s^11/r^12 - 6 s^5/r^6): 0 at s = 0, where the quotient is 0/0
+                const float dh = (in && sig > 0.f) ? eps4 * dlj / sig : 0.f;
                 const float dqi = in ? pj.w * phi : 0.f, dqj = in ? pi.w * phi : 0.f;
                 const float dei = in ? qj.y * lj : 0.f, dej = in ? qi.y * lj : 0.f;
                 gi_q[t] += dqi; gi_h[t] += dh; gi_e[t] += dei;
@@ -145,10 +147,12 @@ __global__ void k_backward_gather(DevPtrs P, int sysBegin, double* dq, double* d
     const int b = D1.slotOff + P.slotOf[D1.setAtomOff + i];
     const int k = D0.atomOff + i;
     const double sqk = 11.787089759563214;  // sqrt(138.935456)
-    const double ep = (double)P.ljp[k].y;   // 2 sqrt(eps)
+    const double ep = P.prm64[k].y;          // 2 sqrt(eps)
     dq[k] = sqk * (P.gq[a] + P.gq[b]);
     dsig[k] = 0.5 * (P.gsig[a] + P.gsig[b]);
-    deps[k] = (P.geps[a] + P.geps[b]) * 2.0 / ep;
+    // dE/deps = dE/de' de'/deps with de'/deps = 1/sqrt(eps) = 2/e': no derivative at eps = 0,
+    // where 0 is returned (the convention of the FP64 oracle)
+    deps[k] = ep > 0.0 ? (P.geps[a] + P.geps[b]) * 2.0 / ep : 0.0;
 }
 
 __global__ void k_set_positions(float4* posq, const float* xyz, long long n) {

@@ -46,6 +46,7 @@ struct Tuning {
     int pairWaves = 2;
     int zbinsPerColumn = 4;
     int pairKernel = 0;
+    int listCapacity = 0;  // list entries per group to start from (0: sized from the density)
 };
 
 struct Context {
@@ -186,8 +187,19 @@ SetDesc make_set(const atm_system_desc& d, const Tuning& tu, int s, int state) {
     long long cap = (long long)(ncand * (1.5 * m + 8.0));
     cap = std::max<long long>(64, (cap + 31) / 32 * 32);
     cap = std::min<long long>(cap, (long long)D.clusterCap + 32);
-    D.entryCap = (int)cap;
+    D.entryCap = tu.listCapacity > 0 ? tu.listCapacity : (int)cap;
     return D;
+}
+
+// Largest partial force (kJ/mol/nm) that enters a set's 64-bit fixed-point sums.
+// One slot receives at most: one flush per group that lists its cluster (tile
+// kernel), up to 64 pair terms per group's j-batch sum (compacted kernel), one
+// flush per i-cluster that lists it (atom-list kernel) and the pair terms of its
+// own group; each term is below fixMax * 2^24, so choosing
+// fixMax <= 2^63 / (2^24 maxTerms) keeps every sum inside the int64 range.
+float fixed_point_limit(const SetDesc& D) {
+    const double maxTerms = 64.0 * D.groupCap + (double)D.clusterCap + 8.0 * D.entryCap + 2.0;
+    return (float)std::min<double>(kFixedMax, std::ldexp(1.0, 39) / maxTerms);
 }
 
 void layout(Context& C) {
@@ -200,6 +212,7 @@ void layout(Context& C) {
             const int c = 2 * s + st;
             SetDesc D = make_set(C.sys[s], C.tuning, s, st);
             if (C.entryCapOverride[c] > D.entryCap) D.entryCap = C.entryCapOverride[c];
+            D.fixMax = fixed_point_limit(D);
             D.atomOff = C.atomOff[s];
             D.setAtomOff = (int)setAtomOff;
             D.cellOff = (int)cellOff;
@@ -451,11 +464,16 @@ void fetch_results(Context& C) {
     bool overflow = false;
     for (int c = 0; c < 2 * C.nsys; ++c) {
         if (rt[c].overflow) {
+            // the flag is sticky over the evaluations since it was last cleared (a chunk of
+            // MD steps); the new capacity covers the longest list of the last evaluation
             overflow = true;
-            C.entryCapOverride[c] = C.sets[c].entryCap * 2;
+            const long long need = (long long)rt[c].maxEntries + rt[c].maxEntries / 8;
+            C.entryCapOverride[c] = (int)std::min<long long>(std::max<long long>(2ll * C.sets[c].entryCap, (need + 31) / 32 * 32),
+                                                             (long long)C.sets[c].clusterCap + 32);
         }
     }
     if (overflow) {
+        // the relayout reallocates and zeroes the runtime records, clearing the flags
         layout(C);
         alloc_layout(C);
         C.evaluated = false;
@@ -614,6 +632,9 @@ int atm_set_particles(void* context, int32_t sys, const double* charges, const d
                                                 (float)displacements[3 * i + 2], 0.f);
         }
         upload_system(C, sys);
+        // during dynamics the device positions are the integrator's: rewrite them over the
+        // host copy just uploaded (which holds only the charges current)
+        if (C.mdReady) HIPCHK(launch_md_init_posq(C));
         HIPCHK(hipMemcpyAsync(C.dLjp.as<float2>() + off, C.hLjp.data() + off, n * sizeof(float2), hipMemcpyHostToDevice, C.stream));
         HIPCHK(hipMemcpyAsync(C.dPrm64.as<double4>() + off, C.hPrm64.data() + off, n * sizeof(double4), hipMemcpyHostToDevice,
                               C.stream));
@@ -810,25 +831,44 @@ int atm_get_stats(void* context, int64_t* pairs_in_cutoff, int64_t* pairs_evalua
         Context& C = ctx_of(context);
         HIPCHK(hipSetDevice(C.device));
         fetch_results(C);
-        DevBuf ctr;
-        ctr.alloc(sizeof(unsigned long long) * 4);
+        std::vector<SetRuntime> rt(2 * C.nsys);
+        to_host(C, rt.data(), C.dRt.p, sizeof(SetRuntime) * rt.size());
         long long nin = 0, nev = 0, nent = 0;
+        DevBuf ctr;
+        if (pairs_evaluated) ctr.alloc(sizeof(unsigned long long) * 4);
         for (int c = 0; c < 2 * C.nsys; ++c) {
-            zero_on_stream(C, ctr.p, sizeof(unsigned long long) * 4);
-            HIPCHK(launch_pairs_dump(C.P, c, C.sets[c].groupCap, nullptr, 0, ctr.as<unsigned long long>(), C.stream));
-            unsigned long long h[4];
-            to_host(C, h, ctr.p, sizeof(h));
-            nin += (long long)h[0];
-            nev += (long long)h[1];
-            std::vector<int> gc(C.sets[c].groupCap);
-            to_host(C, gc.data(), C.dGcount.as<int>() + C.sets[c].groupOff, sizeof(int) * gc.size());
-            SetRuntime rt;
-            to_host(C, &rt, C.dRt.as<SetRuntime>() + c, sizeof(rt));
-            for (int g = 0; g < rt.ngroups; ++g) nent += gc[g];
+            nin += (long long)rt[c].npairs;  // counted by the pair kernel of the evaluation
+            if (pairs_evaluated) {
+                zero_on_stream(C, ctr.p, sizeof(unsigned long long) * 4);
+                HIPCHK(launch_pairs_dump(C.P, c, C.sets[c].groupCap, nullptr, 0, ctr.as<unsigned long long>(), C.stream));
+                unsigned long long h[4];
+                to_host(C, h, ctr.p, sizeof(h));
+                nev += (long long)h[1];
+            }
+            if (list_entries) {
+                std::vector<int> gc(C.sets[c].groupCap);
+                to_host(C, gc.data(), C.dGcount.as<int>() + C.sets[c].groupOff, sizeof(int) * gc.size());
+                for (int g = 0; g < rt[c].ngroups; ++g) nent += gc[g];
+            }
         }
         if (pairs_in_cutoff) *pairs_in_cutoff = nin;
         if (pairs_evaluated) *pairs_evaluated = nev;
         if (list_entries) *list_entries = nent;
+    });
+}
+
+int atm_get_list_stats(void* context, int64_t* pairs, int32_t* max_entries, int32_t* capacity) {
+    return guarded([&] {
+        Context& C = ctx_of(context);
+        HIPCHK(hipSetDevice(C.device));
+        fetch_results(C);
+        std::vector<SetRuntime> rt(2 * C.nsys);
+        to_host(C, rt.data(), C.dRt.p, sizeof(SetRuntime) * rt.size());
+        for (int c = 0; c < 2 * C.nsys; ++c) {
+            if (pairs) pairs[c] = (int64_t)rt[c].npairs;
+            if (max_entries) max_entries[c] = rt[c].maxEntries;
+            if (capacity) capacity[c] = C.sets[c].entryCap;
+        }
     });
 }
 
@@ -851,7 +891,7 @@ int atm_get_tuning(void* context, atm_tuning* out) {
         out->zbins_per_column = C.tuning.zbinsPerColumn;
         out->best_ms = C.tunedMs;
         out->pair_kernel = C.tuning.pairKernel;
-        out->reserved = 0;
+        out->list_capacity = C.tuning.listCapacity;
     });
 }
 
@@ -867,14 +907,18 @@ int atm_set_tuning(void* context, const atm_tuning* t) {
         if (t->pair_kernel < 0 || t->pair_kernel > 5) fail(ATM_ERR_INVALID_ARGUMENT, "pair_kernel must be in [0, 5]");
         if (t->pair_kernel != 0 && t->pair_waves_per_block == 8)
             fail(ATM_ERR_INVALID_ARGUMENT, "pair kernels 1-5 take 1, 2 or 4 waves per block");
+        if (t->list_capacity < 0) fail(ATM_ERR_INVALID_ARGUMENT, "list_capacity must be >= 0");
         HIPCHK(hipSetDevice(C.device));
         HIPCHK(hipStreamSynchronize(C.stream));
         const bool relayout = t->column_scale != C.tuning.colScale || t->zbins_per_column != C.tuning.zbinsPerColumn ||
-                              (t->pair_kernel == kPairKernelAtomLists) != (C.tuning.pairKernel == kPairKernelAtomLists);
+                              (t->pair_kernel == kPairKernelAtomLists) != (C.tuning.pairKernel == kPairKernelAtomLists) ||
+                              t->list_capacity != C.tuning.listCapacity;
         C.tuning.colScale = t->column_scale;
         C.tuning.zbinsPerColumn = t->zbins_per_column;
         C.tuning.pairWaves = t->pair_waves_per_block;
         C.tuning.pairKernel = t->pair_kernel;
+        if (t->list_capacity != C.tuning.listCapacity) C.entryCapOverride.assign(2 * C.nsys, 0);
+        C.tuning.listCapacity = t->list_capacity;
         if (relayout) {
             layout(C);
             alloc_layout(C);
@@ -982,7 +1026,7 @@ int atm_autotune(void* context, int32_t repeats, atm_tuning* out) {
             out->zbins_per_column = best.zbinsPerColumn;
             out->best_ms = bestMs;
             out->pair_kernel = best.pairKernel;
-            out->reserved = 0;
+            out->list_capacity = best.listCapacity;
         }
     });
 }

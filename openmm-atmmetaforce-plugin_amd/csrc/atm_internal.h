@@ -21,9 +21,11 @@ namespace atm {
 constexpr int kClusterSize = 8;      // atoms per cluster
 constexpr int kGroupClusters = 8;    // clusters per group (one wave64 = 8 j x 8 i lanes)
 constexpr double kForceScale = 16777216.0;  // 2^24 fixed-point units per kJ/mol/nm
-// Partial forces at or above this magnitude (kJ/mol/nm) go to the FP64 side
-// accumulator instead of the fixed-point one, so the 64-bit sums cannot wrap.
-constexpr float kFixedMax = 2147483648.0f;  // 2^31
+// Upper bound of the fixed-point partial-force threshold (kJ/mol/nm).  Each set
+// uses fixMax <= this: partial forces at or above it go to the FP64 side
+// accumulator, and fixMax is chosen so that the at most maxTerms integer terms
+// one slot can receive cannot sum past 2^63 (no wrap, by construction).
+constexpr float kFixedMax = 134217728.0f;  // 2^27
 
 struct SetDesc {
     // geometry (FP32 copies of the FP64 host values)
@@ -50,12 +52,16 @@ struct SetDesc {
     int entryCap;            // entries per group
     int alistCap;            // atom-level list entries per cluster (multiple of 8)
     long long alistOff;      // first atom-level list entry of cluster 0 of this set
+    float fixMax;            // partial forces >= fixMax (kJ/mol/nm) go to the FP64 side sums
+    int pad_;
 };
 
 struct SetRuntime {
-    int nslots, nclusters, ngroups, overflow;
-    int big;   // a partial force of this evaluation went to the FP64 side sums
-    int pad_[3];
+    int nslots, nclusters, ngroups;
+    int overflow;  // a group's list exceeded entryCap; sticky until the host relays out
+    int big;       // a partial force of this evaluation went to the FP64 side sums
+    int maxEntries;  // largest untruncated list length of one group (this evaluation)
+    unsigned long long npairs;  // in-cutoff pairs the pair kernel evaluated (this evaluation)
 };
 
 // Per-system state of the last evaluation on the device (mirrors atm_result).

@@ -1,7 +1,7 @@
 // gfx950 kernels of the ATM evaluation.
 //
 // Per evaluation, for every coordinate set c = 2*system + state of the batch:
-//   k_zero_counts  clear fine-cell counters and overflow flags
+//   k_zero_counts  clear fine-cell counters and the per-set pair counters
 //   k_bin          x (+ d for state 1), wrap into the box, fine-cell id and rank
 //   k_scan         per-set column/cell scans, dummy padding, group table
 //   k_scatter_idx  counting-sort scatter of atom indices into cluster order
@@ -62,15 +62,16 @@ __device__ __forceinline__ void atomic_add_fixed(long long* p, long long v) {
     atomicAdd(reinterpret_cast<unsigned long long*>(p), static_cast<unsigned long long>(v));
 }
 
-// Add a partial force to a slot: 64-bit fixed point below kFixedMax, FP64 side
-// accumulator above (clashing pairs of the displaced state).
-__device__ __forceinline__ void add_force(const DevPtrs& P, int c, int slot, float fx, float fy, float fz) {
+// Add a partial force to a slot: 64-bit fixed point below the set's fixMax
+// (<= 2^27 kJ/mol/nm, so the magic-number conversion is exact), FP64 side
+// accumulator at or above it (clashing pairs of the displaced state).
+__device__ __forceinline__ void add_force(const DevPtrs& P, int c, float fixMax, int slot, float fx, float fy, float fz) {
     long long* f = P.sf + slot;
     double* d = P.sd + 3ll * slot;
     bool big = false;
-    if (fabsf(fx) < kFixedMax) atomic_add_fixed(f, to_fixed(fx)); else { atomicAdd(d, (double)fx); big = true; }
-    if (fabsf(fy) < kFixedMax) atomic_add_fixed(f + P.sfn, to_fixed(fy)); else { atomicAdd(d + 1, (double)fy); big = true; }
-    if (fabsf(fz) < kFixedMax) atomic_add_fixed(f + 2 * P.sfn, to_fixed(fz)); else { atomicAdd(d + 2, (double)fz); big = true; }
+    if (fabsf(fx) < fixMax) atomic_add_fixed(f, to_fixed(fx)); else { atomicAdd(d, (double)fx); big = true; }
+    if (fabsf(fy) < fixMax) atomic_add_fixed(f + P.sfn, to_fixed(fy)); else { atomicAdd(d + 1, (double)fy); big = true; }
+    if (fabsf(fz) < fixMax) atomic_add_fixed(f + 2 * P.sfn, to_fixed(fz)); else { atomicAdd(d + 2, (double)fz); big = true; }
     if (big) P.rt[c].big = 1;
 }
 
@@ -214,7 +215,11 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
 __global__ void k_zero_counts(DevPtrs P, int cellBegin, int cellEnd, int setBegin, int setEnd) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     for (int k = cellBegin + i; k < cellEnd; k += gridDim.x * blockDim.x) P.cellCount[k] = 0;
-    if (i < setEnd - setBegin) P.rt[setBegin + i].overflow = 0;
+    // the overflow flag stays set until the host has seen it (it relays out and clears it)
+    if (i < setEnd - setBegin) {
+        P.rt[setBegin + i].npairs = 0ull;
+        P.rt[setBegin + i].maxEntries = 0;
+    }
 }
 
 __global__ void k_bin(DevPtrs P, int setBegin) {
@@ -591,6 +596,7 @@ __global__ void __launch_bounds__(64 * W) k_pairlist(DevPtrs P, int setBegin) {
     }
     if (lane == 0) {
         P.gcount[D.groupOff + g] = min(nEnt, D.entryCap);
+        atomicMax(&P.rt[c].maxEntries, nEnt);
         if (nEnt > D.entryCap) atomicOr(&P.rt[c].overflow, 1);
     }
 }
@@ -621,15 +627,15 @@ __device__ __forceinline__ JAtom load_j(const DevPtrs& P, const SetDesc& D, uint
 }
 
 // Flush one partial force vector to a slot.  The common case (all three
-// components below kFixedMax) takes the branch-free fixed-point path.
-__device__ __forceinline__ void flush_force(const DevPtrs& P, int c, int slot, float fx, float fy, float fz) {
-    if (fmaxf(fabsf(fx), fmaxf(fabsf(fy), fabsf(fz))) < 134217728.0f) {  // 2^27: |f scale| < 2^51
+// components below fixMax) takes the branch-free fixed-point path.
+__device__ __forceinline__ void flush_force(const DevPtrs& P, int c, float fixMax, int slot, float fx, float fy, float fz) {
+    if (fmaxf(fabsf(fx), fmaxf(fabsf(fy), fabsf(fz))) < fixMax) {  // <= 2^27: |f scale| < 2^51
         long long* f = P.sf + slot;
         atomic_add_fixed(f, __double_as_longlong((double)fx * kForceScale + 6755399441055744.0) - 0x4338000000000000LL);
         atomic_add_fixed(f + P.sfn, __double_as_longlong((double)fy * kForceScale + 6755399441055744.0) - 0x4338000000000000LL);
         atomic_add_fixed(f + 2 * P.sfn, __double_as_longlong((double)fz * kForceScale + 6755399441055744.0) - 0x4338000000000000LL);
     } else {
-        add_force(P, c, slot, fx, fy, fz);
+        add_force(P, c, fixMax, slot, fx, fy, fz);
     }
 }
 
@@ -641,21 +647,17 @@ __device__ __forceinline__ void wave_sync() {
 
 // Add one force component (component k of slot `slot`) with an atomic that
 // every lane issues on every path, so the number of memory operations between
-// a load and its use does not depend on the data; values beyond the magic-number
-// range (|f| >= 2^27, clashing pairs) add zero there and take a second, rare
-// path: the general conversion, or the FP64 side accumulator above 2^31.
-__device__ __forceinline__ void flush_component_u(const DevPtrs& P, int c, int slot, int k, float f) {
-    const bool small = fabsf(f) < 134217728.0f;
+// a load and its use does not depend on the data; values at or beyond the set's
+// fixMax (clashing pairs) add zero there and take a second, rare path: the FP64
+// side accumulator.
+__device__ __forceinline__ void flush_component_u(const DevPtrs& P, int c, float fixMax, int slot, int k, float f) {
+    const bool small = fabsf(f) < fixMax;
     long long* dst = P.sf + (long long)k * P.sfn + slot;
     const long long v = small ? __double_as_longlong((double)f * kForceScale + 6755399441055744.0) - 0x4338000000000000LL : 0ll;
     if (v != 0) atomic_add_fixed(dst, v);
     if (!small) {
-        if (fabsf(f) < kFixedMax) {
-            atomic_add_fixed(dst, to_fixed(f));
-        } else {
-            atomicAdd(P.sd + 3ll * slot + k, (double)f);
-            P.rt[c].big = 1;
-        }
+        atomicAdd(P.sd + 3ll * slot + k, (double)f);
+        P.rt[c].big = 1;
     }
 }
 
@@ -719,11 +721,12 @@ __global__ void __launch_bounds__(64 * W, 6) k_pairs(DevPtrs P, int setBegin) {
     const float Lx = D.L[0], invLx = D.invL[0], Ly = D.L[1], invLy = D.invL[1];
     const bool shiftOK = D.shiftOK;
     const double rc2 = D.rc2, krf64 = D.krf64, crf64 = D.crf64;
-    const float krf = D.krf, crf = D.crf, rc2hi = D.rc2hi;
+    const float krf = D.krf, crf = D.crf, rc2hi = D.rc2hi, fixMax = D.fixMax;
     float fi[kGroupClusters][3];
 #pragma unroll
     for (int t = 0; t < kGroupClusters; ++t) fi[t][0] = fi[t][1] = fi[t][2] = 0.f;
     double eE = 0.0;  // this lane's sum of whole pair energies
+    uint32_t npair = 0;  // in-cutoff pairs evaluated by this wave (wave-uniform)
     const int ne = __builtin_amdgcn_readfirstlane(P.gcount[D.groupOff + g]);
     const uint32_t* E = P.entries + D.entryOff + (long long)g * D.entryCap;
     // This lane's j-atom of one list entry.  Two of these are in flight: entry
@@ -762,7 +765,7 @@ __global__ void __launch_bounds__(64 * W, 6) k_pairs(DevPtrs P, int setBegin) {
         for (int r = 0; r < 3; ++r) {
             const int v = r * 64 + ln;  // entry v / 24, then jj * 3 + component
             const int pe = v / (kClusterSize * 3), rem = v - pe * (kClusterSize * 3);
-            if (pe < cnt) flush_component_u(P, c, sJS[wave][pe] + rem / 3, rem % 3, sJF[wave][pe][rem]);
+            if (pe < cnt) flush_component_u(P, c, fixMax, sJS[wave][pe] + rem / 3, rem % 3, sJF[wave][pe][rem]);
         }
         wave_sync();  // read before the next entries overwrite them
     };
@@ -811,6 +814,7 @@ __global__ void __launch_bounds__(64 * W, 6) k_pairs(DevPtrs P, int setBegin) {
                     // exact: the squares of 2^-20-grid differences fit in 53 bits
                     const double r2 = fma((double)dx, (double)dx, fma((double)dy, (double)dy, (double)dz * (double)dz));
                     const bool in = cand && r2 < rc2;
+                    npair += (uint32_t)__popcll(__builtin_amdgcn_ballot_w64(in));
                     const PairEF64 pe = pair_ef64(in, r2, id.x + jd.x, id.y * jd.y, id.z * jd.z, krf64, crf64);
                     eE += pe.e;
                     fs = (float)pe.fs;
@@ -822,6 +826,7 @@ __global__ void __launch_bounds__(64 * W, 6) k_pairs(DevPtrs P, int setBegin) {
                     const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
                     const uint64_t inLanes = __builtin_amdgcn_ballot_w64(in_cutoff(D, dx, dy, dz, r2)) & okLanes;
                     if (inLanes == 0ull) continue;
+                    npair += (uint32_t)__popcll(inLanes);
                     const bool in = __builtin_amdgcn_inverse_ballot_w64(inLanes);
                     const PairE pe = pair_term(in, r2, pi.w * pj.w, qi.x + qj.x, qi.y * qj.y, krf);
                     // the whole pair energy in FP32 (its terms carry FP32 errors anyway), summed in FP64
@@ -869,13 +874,16 @@ __global__ void __launch_bounds__(64 * W, 6) k_pairs(DevPtrs P, int setBegin) {
         for (int t = 0; t < kGroupClusters; ++t) {
             if (t < ncl) {
                 const float fx = stride8_sum(fi[t][0]), fy = stride8_sum(fi[t][1]), fz = stride8_sum(fi[t][2]);
-                if (jj == 0 && ((ivm >> t) & 1u)) flush_force(P, c, D.slotOff + (g0 + t) * kClusterSize + ii, fx, fy, fz);
+                if (jj == 0 && ((ivm >> t) & 1u)) flush_force(P, c, fixMax, D.slotOff + (g0 + t) * kClusterSize + ii, fx, fy, fz);
             }
         }
     }
     double eacc = eE;
     eacc = wave_sum_d(eacc);
-    if (lane == 0) P.genergy[D.groupOff + g] = eacc;
+    if (lane == 0) {
+        P.genergy[D.groupOff + g] = eacc;
+        atomicAdd(&P.rt[c].npairs, (unsigned long long)npair);
+    }
 }
 
 // ------------------------------------------------- atom-level j lists
@@ -985,6 +993,7 @@ __global__ void __launch_bounds__(64 * W) k_pairs_a(DevPtrs P, int setBegin) {
     const double rc2 = D.rc2, krf64 = D.krf64;
     const float rc2hi = D.rc2hi;
     double eE = 0.0, eQQ = 0.0, eQR2 = 0.0;
+    uint32_t npair = 0;
     for (int t = 0; t < ncl; ++t) {
         const bool iv = (ivm >> t) & 1u;
         const float4 pi = sPos[wave][t * kClusterSize + ii];
@@ -1026,6 +1035,7 @@ __global__ void __launch_bounds__(64 * W) k_pairs_a(DevPtrs P, int setBegin) {
                 if (__builtin_amdgcn_ballot_w64(cand) == 0ull) continue;
                 const double r2 = fma((double)dx, (double)dx, fma((double)dy, (double)dy, (double)dz * (double)dz));
                 const bool in = cand && r2 < rc2;
+                npair += (uint32_t)__popcll(__builtin_amdgcn_ballot_w64(in));
                 const PairE64 pe = pair_term64(in, r2, id.x + jd.x, id.y * jd.y, id.z * jd.z, krf64);
                 eE += pe.e; eQQ += pe.qq; eQR2 += pe.qq * r2;
                 if (FORCES) {
@@ -1034,18 +1044,21 @@ __global__ void __launch_bounds__(64 * W) k_pairs_a(DevPtrs P, int setBegin) {
                     fix += fx; fiy += fy; fiz += fz;
                     const float fjx = oct_sum(-fx), fjy = oct_sum(-fy), fjz = oct_sum(-fz);
                     if (ii == 0 && jv && (fjx != 0.f || fjy != 0.f || fjz != 0.f))
-                        flush_force(P, c, D.slotOff + js, fjx, fjy, fjz);
+                        flush_force(P, c, D.fixMax, D.slotOff + js, fjx, fjy, fjz);
                 }
             }
         }
         if (FORCES) {
             const float fx = stride8_sum(fix), fy = stride8_sum(fiy), fz = stride8_sum(fiz);
-            if (jj == 0 && iv) flush_force(P, c, D.slotOff + islot, fx, fy, fz);
+            if (jj == 0 && iv) flush_force(P, c, D.fixMax, D.slotOff + islot, fx, fy, fz);
         }
     }
     double eacc = eE + D.krf64 * eQR2 - D.crf64 * eQQ;
     eacc = wave_sum_d(eacc);
-    if (lane == 0) P.genergy[D.groupOff + g] = eacc;
+    if (lane == 0) {
+        P.genergy[D.groupOff + g] = eacc;
+        atomicAdd(&P.rt[c].npairs, (unsigned long long)npair);
+    }
 }
 
 // ------------------------------------------------- compacted pair kernel
@@ -1148,7 +1161,9 @@ __global__ void __launch_bounds__(64 * W) k_pairs_q(DevPtrs P, int setBegin) {
     const bool shiftOK = D.shiftOK;
     const float rc2hi = D.rc2hi;
     const double rc2 = D.rc2, krf64 = D.krf64;
+    const double fixMax = D.fixMax;
     double eE = 0.0, eQQ = 0.0, eQR2 = 0.0;
+    uint32_t npair = 0;
     const int ne = __builtin_amdgcn_readfirstlane(P.gcount[D.groupOff + g]);
     const uint32_t* E = P.entries + D.entryOff + (long long)g * D.entryCap;
     int qh = 0, qn = 0;  // ring head and tail (wave-uniform, never reset)
@@ -1166,12 +1181,13 @@ __global__ void __launch_bounds__(64 * W) k_pairs_q(DevPtrs P, int setBegin) {
         // exact: the squares of 2^-20-grid differences fit in 53 bits
         const double r2 = fma(dxd, dxd, fma(dyd, dyd, dzd * dzd));
         const bool in = act && r2 < rc2;
+        npair += (uint32_t)__popcll(__builtin_amdgcn_ballot_w64(in));
         const PairE64 pe = pair_term64(in, r2, sIH[wave][i] + sJH[wave][j], sIE[wave][i] * sJE[wave][j],
                                        sIQ[wave][i] * sJQ[wave][j], krf64);
         eE += pe.e; eQQ += pe.qq; eQR2 += pe.qq * r2;
         if (FORCES && in) {
             const double fx = pe.fs * dxd, fy = pe.fs * dyd, fz = pe.fs * dzd;
-            if (fmax(fabs(fx), fmax(fabs(fy), fabs(fz))) < 134217728.0) {  // 2^27
+            if (fmax(fabs(fx), fmax(fabs(fy), fabs(fz))) < fixMax) {  // <= 2^27
                 const unsigned long long vx = fixed64(fx), vy = fixed64(fy), vz = fixed64(fz);
                 lds_add_u64(&sIF[wave][0][i], vx);
                 lds_add_u64(&sIF[wave][1][i], vy);
@@ -1325,7 +1341,10 @@ __global__ void __launch_bounds__(64 * W) k_pairs_q(DevPtrs P, int setBegin) {
     }
     double eacc = eE + D.krf64 * eQR2 - D.crf64 * eQQ;
     eacc = wave_sum_d(eacc);
-    if (lane == 0) P.genergy[D.groupOff + g] = eacc;
+    if (lane == 0) {
+        P.genergy[D.groupOff + g] = eacc;
+        atomicAdd(&P.rt[c].npairs, (unsigned long long)npair);
+    }
 }
 
 // Parity/statistics: the same traversal, emitting in-cutoff pairs as original

@@ -331,11 +331,14 @@ static void gr_cb(void* vc, int i, int j, double dx, double dy, double dz, doubl
     double phi = ORACLE_KE * (1.0 / r + c->krf * r2 - c->crf);
     c->dq[i] += c->w * phi * c->q[j];
     c->dq[j] += c->w * phi * c->q[i];
-    /* dE/dsig_ij = 4 eps (12 sr12 - 6 sr6)/sig; dsig_ij/dsig_i = 1/2 */
-    double dsg = 0.5 * 4.0 * eij * (12.0 * sr6 * sr6 - 6.0 * sr6) / sig;
+    /* dE/dsig_ij = 4 eps (12 sr12 - 6 sr6)/sig = 4 eps (12 sig^11/r^12 - 6 sig^5/r^6), which
+     * is 0 at sig_ij = 0 (the quotient form is 0/0 there); dsig_ij/dsig_i = 1/2 */
+    double dsg = sig > 0.0 ? 0.5 * 4.0 * eij * (12.0 * sr6 * sr6 - 6.0 * sr6) / sig : 0.0;
     c->dsig[i] += c->w * dsg;
     c->dsig[j] += c->w * dsg;
-    /* dE/deps_i = E_LJ / (2 eps_i) = 2 sqrt(eps_j/eps_i) (sr12 - sr6) */
+    /* dE/deps_i = E_LJ / (2 eps_i) = 2 sqrt(eps_j/eps_i) (sr12 - sr6).  At eps_i = 0 the
+     * derivative does not exist (sqrt(eps_i eps_j) has an infinite slope there); this
+     * restatement returns 0 for such an atom, a convention the device path follows. */
     double lj = sr6 * sr6 - sr6;
     if (c->eps[i] > 0) c->deps[i] += c->w * 2.0 * sqrt(c->eps[j] / c->eps[i]) * lj;
     if (c->eps[j] > 0) c->deps[j] += c->w * 2.0 * sqrt(c->eps[i] / c->eps[j]) * lj;

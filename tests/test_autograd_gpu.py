@@ -95,3 +95,46 @@ def test_config5_replica_batch_gradients(gpu):
                 print(f"atom {idx} d{which}: backward {got:.8g} central {fd:.8g}", flush=True)
                 assert got == pytest.approx(fd, rel=2e-4, abs=1e-4)
         del base_q
+
+
+def test_zero_epsilon_and_sigma_sites(gpu):
+    """Force fields have sites with eps = 0 (TIP3P hydrogens) and sigma = 0.
+    dW/dsigma at sigma_i + sigma_j = 0 is 0 (the limit of 4 eps (12 s^11/r^12 -
+    6 s^5/r^6)); dW/deps of an eps = 0 atom does not exist and is returned as 0,
+    the oracle's convention.  Every gradient is finite and matches the oracle."""
+    s = W.tiny_system(1500, 3.0, 0.9, 12, (0.9, 0.8, 0.7), 61,
+                      W.ATMParams(0.2, 0.35, 0.1, 110.0, 0.5, 200.0, 100.0, 0.0625, 1.0))
+    eps = s.epsilons.copy()
+    sig = s.sigmas.copy()
+    eps[::5] = 0.0
+    sig[::7] = 0.0        # every 7th atom: pairs of two such atoms have sigma_ij = 0
+    eps[::7] = 0.3
+    import dataclasses
+    s = dataclasses.replace(s, epsilons=eps, sigmas=sig)
+    with ATMBatch([s]) as b:
+        q, sg, ep, sc = tensors([s])
+        Wt = ATMEnergyFunction.apply(b, q, sg, ep, *sc)
+        Wt.sum().backward()
+        for g in (q.grad, sg.grad, ep.grad):
+            assert torch.isfinite(g).all()
+        assert (ep.grad.numpy()[eps == 0.0] == 0.0).all()
+        check_against_oracle([s], (q.grad.numpy(), sg.grad.numpy(), ep.grad.numpy(),
+                                   [t.grad.numpy() for t in sc]), rtol=2e-5)
+
+
+def test_backward_after_re_evaluation_at_other_positions(gpu):
+    """forward at x1, then the batch is evaluated at x2, then the backward of
+    the first forward: the gradient is the one at x1 (the backward restores
+    the forward's positions before it re-runs the evaluation)."""
+    s1 = W.tiny_system(1200, 3.0, 0.9, 10, (0.9, 0.8, 0.7), 63,
+                       W.ATMParams(0.2, 0.35, 0.1, 110.0, 0.5, 200.0, 100.0, 0.0625, 1.0))
+    x2 = W.quantize(s1.positions + np.random.default_rng(5).normal(0.0, 0.05, s1.positions.shape))
+    with ATMBatch([s1]) as b:
+        q, sg, ep, sc = tensors([s1])
+        Wt = ATMEnergyFunction.apply(b, q, sg, ep, *sc)
+        b.set_positions(0, x2)
+        b.compute()
+        assert b.result(0)["energy"] != pytest.approx(Wt[0].item(), rel=1e-6)
+        Wt.sum().backward()
+        check_against_oracle([s1], (q.grad.numpy(), sg.grad.numpy(), ep.grad.numpy(),
+                                    [t.grad.numpy() for t in sc]), rtol=2e-5)

@@ -4,10 +4,20 @@ Bar for the benchmark configurations: in-cutoff pair sets identical to the
 oracle's; U1, U2 and W within 1e-6 relative; per-atom forces within 1e-5
 relative RMS (||F - F_ref|| / ||F_ref||).
 
-Small contrived systems whose energy is a small difference of large pair terms
-are held to the FP32 error model instead: |dU| <= 1e-6 max(|U|, sum_ij |e_ij|)
-(FP32 pair arithmetic: a few ulp per pair, up to ~16 ulp where (s/r)^12
-amplifies), and W to the bound propagated from U1 and U2.
+Pair sets: every evaluation's pair kernel counts the in-cutoff pairs it
+evaluates (from the lane masks of its exact r < rc decisions); the count of
+every coordinate set must equal the size of the oracle's pair set.  The pair
+lists themselves are compared element by element through the list traversal
+of atm_get_pairs.
+
+Energies of the default path (FP64 scalar pair function from the exact FP64
+r^2) are held to |dU| <= 1e-10 sum_ij |e_ij| on every system: far above the
+FP64 rounding of the pair terms (the 1/r estimate after one Newton step is
+good to ~6e-14, i.e. < 1e-12 relative per pair term) and far below one pair
+term at the cutoff (a missing or extra pair at r = rc changes U by the LJ tail,
+~1e-3 kJ/mol).  W is held to the bound propagated from U1 and U2.  The FP32
+pair mode is held to the FP32 error model instead: |dU| <= 1e-6 max(|U|,
+sum_ij |e_ij|).
 """
 import numpy as np
 import pytest
@@ -19,6 +29,7 @@ pytestmark = pytest.mark.gpu
 
 E_TOL = 1e-6
 F_TOL = 1e-5
+FP64_TOL = 1e-10   # of sum_ij |e_ij|, default (FP64 scalar) pair path
 
 
 def rel(a, b):
@@ -46,7 +57,11 @@ def report_pair_mismatch(batch, k, state, x, system, pg, po):
           f"stats {batch.stats()}", flush=True)
 
 
-def check_system(batch, k, system, pairs=True, tag="", strict=True, forces=True, ref=None):
+def check_system(batch, k, system, pairs=True, tag="", strict=True, forces=True, ref=None, fp32=False,
+                 counts=None):
+    """strict: also the 1e-6 relative bar on U1, U2, W (benchmark configurations).
+    fp32: the FP32 pair mode's error model instead of the FP64 bound.
+    counts: the batch's list_stats()["pairs"] (read once per batch by the caller)."""
     if ref is None:
         ref, Fref = oracle.atm_eval(system, forces=forces)
     else:
@@ -61,14 +76,23 @@ def check_system(batch, k, system, pairs=True, tag="", strict=True, forces=True,
         assert dU1 <= E_TOL * abs(ref["U1"])
         assert dU2 <= E_TOL * abs(ref["U2"])
         assert dW <= E_TOL * abs(ref["W"])
-    else:
+    if fp32:
         b1 = E_TOL * max(abs(ref["U1"]), ref["abs1"])
         b2 = E_TOL * max(abs(ref["U2"]), ref["abs2"])
-        w1, w2 = (1 - ref["sp"], ref["sp"]) if system.params.direction > 0 else (ref["sp"], 1 - ref["sp"])
-        assert dU1 <= b1
-        assert dU2 <= b2
-        assert dW <= E_TOL * abs(ref["W"]) + abs(w1) * b1 + abs(w2) * b2
+    else:
+        b1 = FP64_TOL * ref["abs1"]
+        b2 = FP64_TOL * ref["abs2"]
+    w1, w2 = (1 - ref["sp"], ref["sp"]) if system.params.direction > 0 else (ref["sp"], 1 - ref["sp"])
+    print(f"    |dU1| {dU1:.3g} (bound {b1:.3g}); |dU2| {dU2:.3g} (bound {b2:.3g})", flush=True)
+    assert dU1 <= b1
+    assert dU2 <= b2
+    assert dW <= 1e-12 * abs(ref["W"]) + abs(w1) * b1 + abs(w2) * b2
     assert abs(r["perturbation_energy"] - ref["usc"]) <= E_TOL * max(abs(ref["usc"]), 1.0)
+    if counts is None:
+        counts = batch.list_stats()["pairs"]
+    assert counts[2 * k] == ref["npairs1"], f"state 0: pair kernel evaluated {counts[2 * k]} vs oracle {ref['npairs1']}"
+    assert counts[2 * k + 1] == ref["npairs2"], \
+        f"state 1: pair kernel evaluated {counts[2 * k + 1]} vs oracle {ref['npairs2']}"
     if forces:
         F = batch.forces(k)
         err = frms(F, Fref)
@@ -117,8 +141,9 @@ def test_config3_replica_batch(gpu):
     reps = W.replica_batch(22)
     with ATMBatch(reps) as b:
         b.compute()
+        counts = b.list_stats()["pairs"]
         for k, s in enumerate(reps):
-            check_system(b, k, s, pairs=(k in (0, 21)), tag="cfg3")
+            check_system(b, k, s, pairs=(k in (0, 21)), tag="cfg3", counts=counts)
 
 
 def test_config4_uneven_batch(gpu):
@@ -126,8 +151,9 @@ def test_config4_uneven_batch(gpu):
     systems = W.uneven_batch(64)
     with ATMBatch(systems) as b:
         b.compute()
+        counts = b.list_stats()["pairs"]
         for k, s in enumerate(systems):
-            check_system(b, k, s, pairs=(k % 16 == 0), forces=(k % 4 == 0), tag="cfg4")
+            check_system(b, k, s, pairs=(k % 16 == 0), tag="cfg4", counts=counts)
 
 
 def test_small_uneven_batch(gpu):
@@ -229,8 +255,9 @@ def test_pair_kernel_variants(gpu, kernel, waves):
     with ATMBatch(systems) as b:
         b.set_tuning(1.0, waves, 4, kernel)
         b.compute()
+        counts = b.list_stats()["pairs"]
         for k, s in enumerate(systems):
-            check_system(b, k, s, pairs=(k < 2), tag=f"kernel {kernel} waves {waves}")
+            check_system(b, k, s, pairs=(k < 2), tag=f"kernel {kernel} waves {waves}", counts=counts)
         e = [b.result(k)["energy"] for k in range(len(systems))]
         b.compute(forces=False)
         for k in range(len(systems)):
@@ -297,8 +324,10 @@ def test_fp32_pair_mode_batch(gpu):
     systems = _variant_systems()
     with ATMBatch(systems) as b:
         b.compute(fp32_pairs=True)
+        counts = b.list_stats()["pairs"]
         for k, s in enumerate(systems):
             ref, Fref = oracle.atm_eval(s)
+            assert counts[2 * k] == ref["npairs1"] and counts[2 * k + 1] == ref["npairs2"]
             r = b.result(k)
             b1 = E_TOL * max(abs(ref["U1"]), ref["abs1"])
             b2 = E_TOL * max(abs(ref["U2"]), ref["abs2"])

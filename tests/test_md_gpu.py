@@ -102,3 +102,53 @@ def test_replica_exchange_run(gpu):
     assert len(lines) == 6 * 4 and len(samples) == 24
     assert all(len(l.split()) == 9 for l in lines)
     assert float(lines[0].split()[0]) == 300.0
+
+
+def _md_pair(s, m, steps, capacity_from_init=False):
+    """Positions after `steps` Langevin steps; optionally with the list capacity cut to
+    the longest group list of the initial positions (so a later step overflows)."""
+    with replica.ATMReplicaExchange(s, [s.params], m, temperature=600.0, friction=2.0, dt=0.002, seed=17) as re:
+        cap0 = None
+        if capacity_from_init:
+            cap0 = int(re.batch.list_stats()["max_entries"].max())
+            t = re.batch.tuning()
+            re.batch.set_tuning(t["column_scale"], t["pair_waves_per_block"], t["zbins_per_column"],
+                                t["pair_kernel"], list_capacity=cap0)
+        re.step(steps)
+        x = re.state(0)[0]
+        cap1 = int(re.batch.list_stats()["capacity"].max())
+        print(f"list capacity {cap0} -> {cap1}", flush=True)
+    return x, cap0, cap1
+
+
+def test_list_overflow_inside_a_chunk_is_replayed(gpu):
+    """atm_md_step checks list overflow once per chunk of steps.  The overflow
+    flag is sticky over the chunk, so an overflow on any step (here: the list
+    capacity is cut to the longest list at the initial positions, and the
+    atoms then move at 600 K) is seen, the capacity grows and the steps since
+    the chunk's snapshot are replayed: the trajectory equals one run with
+    ample capacity bit for bit."""
+    s = _system(1500, 3.4, 49, lig=10)
+    m = np.full(s.num_atoms, 4.0)
+    x_ref, _, _ = _md_pair(s, m, 40)
+    x, cap0, cap1 = _md_pair(s, m, 40, capacity_from_init=True)
+    assert cap1 > cap0  # an overflow happened and was replayed
+    assert np.array_equal(x, x_ref)
+
+
+def test_set_particles_during_md_keeps_the_md_positions(gpu):
+    """New particle parameters during dynamics leave the integrator's positions
+    on the device: the next evaluation is at the atm_md_get positions."""
+    s = _system(600, 3.2, 53)
+    m = np.full(s.num_atoms, 12.0)
+    with replica.ATMReplicaExchange(s, [s.params], m, temperature=300.0, friction=5.0, dt=0.002, seed=19) as re:
+        re.step(60)
+        x, _, _ = re.state(0)
+        q2 = s.charges * 0.5
+        re.batch.set_particles(0, q2, s.sigmas, s.epsilons, s.displacements)
+        re.batch.compute()
+        got = re.batch.result(0)
+    ref, _ = oracle.atm_eval(W.System(**{**s.__dict__, "positions": x, "charges": q2}), forces=False)
+    # device positions are the FP32 roundings of the FP64 MD positions
+    assert got["state1_energy"] == pytest.approx(ref["U1"], rel=1e-5, abs=1e-2)
+    assert got["energy"] == pytest.approx(ref["W"], rel=1e-5, abs=1e-2)

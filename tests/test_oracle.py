@@ -238,6 +238,35 @@ def test_analytic_gradient_against_central_differences(direction):
         assert g["dparams"][k] == pytest.approx(d, rel=1e-6, abs=1e-7), name
 
 
+def test_gradient_at_zero_sigma_and_zero_epsilon():
+    """sigma_i = sigma_j = 0: E_LJ = 4 eps ((s/r)^12 - (s/r)^6) is even in s, so
+    dW/dsigma is 0 there (central differences through +-h agree); the quotient
+    form of the derivative would be 0/0.  eps_i = 0: the derivative of
+    sqrt(eps_i eps_j) does not exist at 0 and the restatement returns 0 for that
+    atom, while the other atoms' gradients stay the central differences."""
+    import dataclasses
+    p = W.ATMParams(0.15, 0.45, 0.08, 30.0, 1.0, 200.0, 100.0, 0.0625, 1.0)
+    s = W.tiny_system(250, 3.2, 0.9, 8, (1.1, 0.7, 0.45), 5, p)
+    sig = s.sigmas.copy()
+    eps = s.epsilons.copy()
+    sig[[2, 9]] = 0.0          # atoms 2 and 9: a sigma_ij = 0 pair if within the cutoff
+    eps[[4, 30]] = 0.0
+    s = dataclasses.replace(s, sigmas=sig, epsilons=eps)
+    x = s.positions
+    d = x[2] - x[9]
+    d -= s.box * np.round(d / s.box)
+    assert np.linalg.norm(d) < s.cutoff
+    g = oracle.atm_grad(s)
+    assert np.isfinite(g["dsigma"]).all() and np.isfinite(g["deps"]).all()
+    assert g["deps"][4] == 0.0 and g["deps"][30] == 0.0
+    h = 1e-7
+    for field, key, i in (("sigmas", "dsigma", 2), ("sigmas", "dsigma", 9), ("sigmas", "dsigma", 3),
+                          ("epsilons", "deps", 5), ("charges", "dq", 4)):
+        wp = oracle.atm_eval(_perturbed(s, field, i, h), forces=False)[0]["W"]
+        wm = oracle.atm_eval(_perturbed(s, field, i, -h), forces=False)[0]["W"]
+        assert g[key][i] == pytest.approx((wp - wm) / (2 * h), rel=1e-5, abs=1e-6), (field, i)
+
+
 # ---------------------------------------------------------------- restraints
 
 def _num_grad(f, x, h=1e-6):

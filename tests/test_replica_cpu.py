@@ -125,3 +125,19 @@ def test_context_replica_exchange_applies_states_and_samples():
         assert counts.get(perm, 0) / 40000 == pytest.approx(wt / Z, abs=0.012)
     lines = re.run(1, 10, exchange=False)
     assert len(lines) == 3 and all(len(s.out_line().split()) == 9 for s in lines)
+
+
+def test_states_must_share_softcore_parameters():
+    """The exchange ratio evaluates each state's bias at the other replica's u_sc,
+    which that replica computed with its own soft-core map: a schedule whose
+    umax, ubcore or acore differ between states is rejected."""
+    from atmmetaforce.replica import _check_states
+    base = [W.ATMParams(0.0, 0.1), W.ATMParams(0.1, 0.2)]
+    _check_states(base)
+    for field, value in (("umax", 250.0), ("ubcore", 80.0), ("acore", 0.125)):
+        import dataclasses
+        bad = [base[0], dataclasses.replace(base[1], **{field: value})]
+        with pytest.raises(ValueError, match="soft-core"):
+            _check_states(bad)
+    with pytest.raises(ValueError, match="direction"):
+        _check_states([base[0], W.ATMParams(0.1, 0.2, direction=-1.0)])
