@@ -18,6 +18,7 @@
 #include "atm_kernels.h"
 #include "atm_math.h"
 #include "atm_capi_util.h"
+#include "atm_layout.h"
 #include "../../include/atm_mi355x.h"
 
 namespace atm {
@@ -39,16 +40,6 @@ std::string& last_error() {
 using namespace atm;
 
 namespace {
-// Defaults: the autotuner's choice on a 22 x 23k-atom batch (cluster-pair tile
-// kernel, 2 waves per block, columns 1.3 x the 8-atom cube edge).
-struct Tuning {
-    double colScale = 1.3;
-    int pairWaves = 2;
-    int zbinsPerColumn = 4;
-    int pairKernel = 0;
-    int listCapacity = 0;  // list entries per group to start from (0: sized from the density)
-};
-
 struct Context {
     int device = 0;
     hipStream_t own = nullptr;
@@ -60,6 +51,7 @@ struct Context {
     int maxAtoms = 0;
     std::vector<SetDesc> sets;
     std::vector<int> entryCapOverride;  // per set, 0 = automatic
+    std::vector<int> alistCapOverride;  // per set, 0 = from the entry capacity
     int maxClusters = 0, maxGroups = 0, maxCells = 0, maxSlots = 0;
     long long totalCells = 0, totalCols = 0, totalSlots = 0, totalClusters = 0, totalGroups = 0, totalEntries = 0;
     long long totalAlist = 0;
@@ -125,6 +117,10 @@ void to_host(Context& C, void* dst, const void* src, size_t bytes) {
     HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, C.stream));
     HIPCHK(hipStreamSynchronize(C.stream));
 }
+
+}  // namespace
+
+namespace atm {
 
 void validate_desc(const atm_system_desc& d, int s) {
     const std::string pre = "system " + std::to_string(s) + ": ";
@@ -202,9 +198,14 @@ float fixed_point_limit(const SetDesc& D) {
     return (float)std::min<double>(kFixedMax, std::ldexp(1.0, 39) / maxTerms);
 }
 
+}  // namespace atm
+
+namespace {
+
 void layout(Context& C) {
     C.sets.resize(2 * C.nsys);
     C.entryCapOverride.resize(2 * C.nsys, 0);
+    C.alistCapOverride.resize(2 * C.nsys, 0);
     long long cellOff = 0, colOff = 0, slotOff = 0, clOff = 0, gOff = 0, eOff = 0, aOff = 0, setAtomOff = 0;
     C.maxClusters = C.maxGroups = C.maxCells = C.maxSlots = 0;
     for (int s = 0; s < C.nsys; ++s) {
@@ -221,7 +222,7 @@ void layout(Context& C) {
             D.clusterOff = (int)clOff;
             D.groupOff = (int)gOff;
             D.entryOff = eOff;
-            D.alistCap = (D.entryCap + 7) / 8 * 8;
+            D.alistCap = std::max((D.entryCap + 7) / 8 * 8, C.alistCapOverride[c]);
             D.alistOff = aOff;
             if (D.clusterCap >= (1 << 20)) fail(ATM_ERR_UNSUPPORTED, "too many clusters in one system");
             setAtomOff += D.natoms;
@@ -463,13 +464,20 @@ void fetch_results(Context& C) {
     to_host(C, rt.data(), C.dRt.p, sizeof(SetRuntime) * rt.size());
     bool overflow = false;
     for (int c = 0; c < 2 * C.nsys; ++c) {
-        if (rt[c].overflow) {
-            // the flag is sticky over the evaluations since it was last cleared (a chunk of
-            // MD steps); the new capacity covers the longest list of the last evaluation
+        // the flags are sticky over the evaluations since they were last cleared (a chunk
+        // of MD steps); the new capacities cover the longest lists of the last evaluation
+        if (rt[c].overflow & 1) {
             overflow = true;
             const long long need = (long long)rt[c].maxEntries + rt[c].maxEntries / 8;
             C.entryCapOverride[c] = (int)std::min<long long>(std::max<long long>(2ll * C.sets[c].entryCap, (need + 31) / 32 * 32),
                                                              (long long)C.sets[c].clusterCap + 32);
+        }
+        if (rt[c].overflow & 2) {
+            // an atom list holds at most the 8 atoms of each of the set's clusters
+            overflow = true;
+            const long long need = (long long)rt[c].maxAtoms + rt[c].maxAtoms / 8;
+            C.alistCapOverride[c] = (int)std::min<long long>(std::max<long long>(2ll * C.sets[c].alistCap, (need + 7) / 8 * 8),
+                                                             8ll * (C.sets[c].clusterCap + 1));
         }
     }
     if (overflow) {
@@ -917,7 +925,10 @@ int atm_set_tuning(void* context, const atm_tuning* t) {
         C.tuning.zbinsPerColumn = t->zbins_per_column;
         C.tuning.pairWaves = t->pair_waves_per_block;
         C.tuning.pairKernel = t->pair_kernel;
-        if (t->list_capacity != C.tuning.listCapacity) C.entryCapOverride.assign(2 * C.nsys, 0);
+        if (t->list_capacity != C.tuning.listCapacity) {
+            C.entryCapOverride.assign(2 * C.nsys, 0);
+            C.alistCapOverride.assign(2 * C.nsys, 0);
+        }
         C.tuning.listCapacity = t->list_capacity;
         if (relayout) {
             layout(C);
@@ -980,6 +991,7 @@ int atm_autotune(void* context, int32_t repeats, atm_tuning* out) {
             C.tuning.colScale = sc;
             C.tuning.zbinsPerColumn = zp;
             C.entryCapOverride.assign(2 * C.nsys, 0);
+            C.alistCapOverride.assign(2 * C.nsys, 0);
             layout(C);
             alloc_layout(C);
         };
@@ -1016,6 +1028,7 @@ int atm_autotune(void* context, int32_t repeats, atm_tuning* out) {
         C.tuning = best;
         C.tunedMs = bestMs;
         C.entryCapOverride.assign(2 * C.nsys, 0);
+        C.alistCapOverride.assign(2 * C.nsys, 0);
         layout(C);
         alloc_layout(C);
         C.evaluated = false;

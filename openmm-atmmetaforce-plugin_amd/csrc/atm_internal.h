@@ -58,10 +58,13 @@ struct SetDesc {
 
 struct SetRuntime {
     int nslots, nclusters, ngroups;
-    int overflow;  // a group's list exceeded entryCap; sticky until the host relays out
+    int overflow;  // bit 0: a group's list exceeded entryCap, bit 1: an atom list exceeded
+                   // alistCap; sticky until the host relays out
     int big;       // a partial force of this evaluation went to the FP64 side sums
     int maxEntries;  // largest untruncated list length of one group (this evaluation)
     unsigned long long npairs;  // in-cutoff pairs the pair kernel evaluated (this evaluation)
+    int maxAtoms;  // largest untruncated atom-level list of one cluster (this evaluation)
+    int pad_;
 };
 
 // Per-system state of the last evaluation on the device (mirrors atm_result).

@@ -219,6 +219,7 @@ __global__ void k_zero_counts(DevPtrs P, int cellBegin, int cellEnd, int setBegi
     if (i < setEnd - setBegin) {
         P.rt[setBegin + i].npairs = 0ull;
         P.rt[setBegin + i].maxEntries = 0;
+        P.rt[setBegin + i].maxAtoms = 0;
     }
 }
 
@@ -938,7 +939,7 @@ __global__ void __launch_bounds__(64) k_alist(DevPtrs P, int setBegin) {
             const int pos = cnt + __popcll(m & below);
             if (keep) {
                 if (pos < D.alistCap) out[pos] = (uint32_t)js | (ent & 0xF0000000u);
-                else atomicOr(&P.rt[c].overflow, 1);
+                else atomicOr(&P.rt[c].overflow, 2);
             }
             cnt += __popcll(m);
         }
@@ -946,7 +947,10 @@ __global__ void __launch_bounds__(64) k_alist(DevPtrs P, int setBegin) {
     // pad to whole 8-atom blocks
     const int padded = min((cnt + 7) & ~7, D.alistCap & ~7);
     if (lane < 8 && cnt + lane < padded) out[cnt + lane] = kNoAtom;
-    if (lane == 0) P.acount[D.clusterOff + g0 + t] = padded;
+    if (lane == 0) {
+        P.acount[D.clusterOff + g0 + t] = padded;
+        atomicMax(&P.rt[c].maxAtoms, cnt);
+    }
 }
 
 // One wave64 per group; for each i-cluster t of the group its atom list is
