@@ -183,10 +183,11 @@ def config_report(steps=50, cpu_seconds=5.0):
     reps = W.replica_batch(22)
 
     def tensors():
-        q = torch.tensor(np.concatenate([s.charges for s in reps]), requires_grad=True)
-        sg = torch.tensor(np.concatenate([s.sigmas for s in reps]), requires_grad=True)
-        ep = torch.tensor(np.concatenate([s.epsilons for s in reps]), requires_grad=True)
-        sc = [torch.tensor([getattr(s.params, n) for s in reps], dtype=torch.float64, requires_grad=True)
+        dev = torch.device("cuda:0")
+        q = torch.tensor(np.concatenate([s.charges for s in reps]), device=dev, requires_grad=True)
+        sg = torch.tensor(np.concatenate([s.sigmas for s in reps]), device=dev, requires_grad=True)
+        ep = torch.tensor(np.concatenate([s.epsilons for s in reps]), device=dev, requires_grad=True)
+        sc = [torch.tensor([getattr(s.params, n) for s in reps], dtype=torch.float64, device=dev, requires_grad=True)
               for n in ("lambda1", "lambda2", "alpha", "u0", "w0")]
         return q, sg, ep, sc
 
@@ -205,10 +206,57 @@ def config_report(steps=50, cpu_seconds=5.0):
             fwd_bwd()
         fb_ms = (time.perf_counter() - t0) * 1e3 / n
     out["config5"] = {"replicas": 22, "ms_per_forward_backward": round(fb_ms, 3),
-                      "note": "host tensors in and gradients out (parameter upload and gradient download included); "
-                              "gradients checked against the FP64 oracle and central differences in "
-                              "tests/test_autograd_gpu.py"}
+                      "note": "device tensors in and device gradients out; gradients checked against the FP64 "
+                              "oracle and central differences in tests/test_autograd_gpu.py"}
     return out
+
+
+def backward_timing(batch, reps, eval_ms, n=20):
+    """dW/d(charges, sigmas, epsilons, lambda1, lambda2, alpha, u0, w0) of the
+    whole replica batch through ATMEnergyFunction with the per-atom tensors on
+    the GPU (parameters in and gradients out as device arrays): the forward
+    (parameter upload, energy-only evaluation of every replica, W read back)
+    and the backward (pair-term derivatives recomputed on the forward's lists,
+    per-atom gather, softplus derivatives)."""
+    import numpy as np
+    import torch
+    from atmmetaforce import ATMEnergyFunction
+    dev = torch.device("cuda:0")
+    q = torch.tensor(np.concatenate([s.charges for s in reps]), device=dev, requires_grad=True)
+    sg = torch.tensor(np.concatenate([s.sigmas for s in reps]), device=dev, requires_grad=True)
+    ep = torch.tensor(np.concatenate([s.epsilons for s in reps]), device=dev, requires_grad=True)
+    sc = [torch.tensor([getattr(s.params, k) for s in reps], dtype=torch.float64, device=dev, requires_grad=True)
+          for k in ("lambda1", "lambda2", "alpha", "u0", "w0")]
+
+    def fwd():
+        return ATMEnergyFunction.apply(batch, q, sg, ep, *sc)
+
+    def fwd_bwd():
+        Wt = fwd()
+        Wt.sum().backward()
+        for t in (q, sg, ep, *sc):
+            t.grad = None
+        return Wt
+
+    for _ in range(3):
+        fwd_bwd()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(n):
+        fwd()
+    torch.cuda.synchronize()
+    f_ms = (time.perf_counter() - t0) * 1e3 / n
+    t0 = time.perf_counter()
+    for _ in range(n):
+        fwd_bwd()
+    torch.cuda.synchronize()
+    fb_ms = (time.perf_counter() - t0) * 1e3 / n
+    return {"replicas": len(reps), "ms_per_forward": round(f_ms, 4), "ms_per_forward_backward": round(fb_ms, 4),
+            "ms_per_backward": round(fb_ms - f_ms, 4),
+            "forward_backward_over_batched_eval": round(fb_ms / eval_ms, 3),
+            "note": "device tensors in, device gradients out; the forward is an energy-only evaluation; "
+                    "gradients are checked against the FP64 oracle and central differences in "
+                    "tests/test_autograd_gpu.py"}
 
 
 def restraint_timing(system, evals=200):
@@ -380,6 +428,7 @@ def main():
     ap.add_argument("--configs", action="store_true",
                     help="print the report of the other configurations (3k-atom CPU case, uneven batch, backward) "
                          "instead of the headline line")
+    ap.add_argument("--no-backward", action="store_true", help="skip the forward + backward timing")
     ap.add_argument("--no-legs", action="store_true",
                     help="skip the restraint, replica-MD and PME legs (profiles of the batched evaluation alone)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
@@ -478,6 +527,10 @@ def main():
                         "profiles/k_pairs_22rep_23k.md): 8x8 cluster tiles hold 21 % in-cutoff pairs"}
     total_atoms = sum(s.num_atoms for s in reps)
     blend_gbs = BLEND_BYTES_PER_ATOM * total_atoms / (prof["blend_ms"] * 1e-3) / 1e9
+    backward = None
+    if not args.no_backward:
+        log("[bench] forward + backward through ATMEnergyFunction (device tensors)")
+        backward = backward_timing(batch, reps, ms_per_step)
     batch.close()
 
     plain_eps = None
@@ -529,6 +582,7 @@ def main():
         "stage_ms": {k: round(v, 4) for k, v in prof.items() if k.endswith("_ms")},
         "blend_kernel": {"bound": "hbm", "achieved": round(blend_gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                          "frac": round(blend_gbs / PEAK_HBM_GBS, 4)},
+        "backward": backward,
         "companion_restraints": restraints,
         "replica_md": md,
         "pme_variable_group": pme_v,

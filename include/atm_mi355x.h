@@ -115,7 +115,10 @@ typedef struct atm_tuning {
                                  copies of the j-force sums, drain after every test step)
                                  = (4,2,yes), (4,2,no), (4,4,yes), (8,2,yes); 5: atom-level
                                  j lists per i-cluster (j atoms within the list cutoff of
-                                 the cluster's box), 8 i x 8 j lanes per step (FP64 only) */
+                                 the cluster's box), 8 i x 8 j lanes per step (FP64 only);
+                                 6: one wave per i-cluster filtering its clusters' j atoms
+                                 against its box into an LDS queue, 8 i x 8 j lanes per
+                                 queued step (FP64 only; FP32 pairs run variant 0) */
     int32_t list_capacity;    /* neighbour-list entries per group to start from; 0 = sized
                                  from the density.  A list that overflows grows it (the
                                  evaluation reports ATM_ERR_CAPACITY and is re-run) */
@@ -158,6 +161,17 @@ int atm_set_positions(void* context, int32_t sys, const double* positions);
 /* Positions of every system at once from a device array (float32, 3 per atom,
  * systems concatenated in order); the copy is enqueued on the context stream. */
 int atm_set_positions_device(void* context, const float* positions_dev);
+
+/* Charges, sigmas and epsilons of every system at once from device arrays
+ * (doubles, systems concatenated in order, num_atoms of the whole context
+ * each), converted on the device as atm_set_particles converts host arrays;
+ * the displacements are kept.  The arrays are read on the context stream
+ * (the caller orders its writes before the call, e.g. by synchronizing or
+ * with atm_set_stream).  Rejects the whole update (nothing changed) if any
+ * entry is non-finite or a sigma/epsilon is negative.  The device path of
+ * copyParametersToContext for per-atom parameters (ATMMetaForceKernels.h:46),
+ * used by the autograd entry point. */
+int atm_set_particles_device(void* context, const double* charges, const double* sigmas, const double* epsilons);
 
 /* New box edges for system `sys` (copyState copies box vectors, :130-133). */
 int atm_set_box(void* context, int32_t sys, const double* box3);
@@ -203,6 +217,14 @@ int atm_get_pairs(void* context, int32_t sys, int32_t state, int32_t* pairs,
  * evaluated positions; any output pointer may be NULL. */
 int atm_backward(void* context, const double* grad_energy, double* d_charges,
                  double* d_sigmas, double* d_epsilons, double* d_params);
+
+/* atm_backward with the per-atom gradients written to device arrays
+ * (d_charges, d_sigmas, d_epsilons: num_atoms of the whole context each, any
+ * may be NULL) on the context stream, with no host copy; grad_energy and
+ * d_params are host arrays as in atm_backward.  Returns after the gradients
+ * are written. */
+int atm_backward_device(void* context, const double* grad_energy, double* d_charges, double* d_sigmas,
+                        double* d_epsilons, double* d_params);
 
 /* Use a caller-provided HIP stream (hipStream_t) instead of the context's own
  * (NULL restores the context stream). */

@@ -20,6 +20,15 @@ ABI), ATMEnergyFunction (torch.autograd), and the synthetic workloads.
 """
 ATMMETAFORCE_VERSION = "0.3.1"
 
+# An installed torch brings its own HIP runtime, with the same library names
+# as the /opt/rocm one this package's native libraries link.  The first one
+# loaded serves the whole process, and torch sees no GPU when it is not its
+# own; so torch (when present) is loaded before the native libraries are.
+try:
+    import torch as _torch  # noqa: F401
+except ImportError:
+    pass
+
 from . import workloads  # noqa: E402
 from .batch import ATMBatch  # noqa: E402
 

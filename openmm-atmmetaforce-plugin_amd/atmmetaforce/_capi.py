@@ -31,9 +31,11 @@ ONE_4PI_EPS0 = 138.935456
 # Every entry point declared in include/atm_mi355x.h.
 EXPORTED = (
     "atm_version", "atm_abi_version", "atm_last_error", "atm_device_count", "atm_create", "atm_destroy",
-    "atm_set_particles", "atm_set_displacements", "atm_set_positions", "atm_set_positions_device",
+    "atm_set_particles", "atm_set_particles_device", "atm_set_displacements", "atm_set_positions",
+    "atm_set_positions_device",
     "atm_set_box", "atm_set_params", "atm_eval", "atm_synchronize", "atm_get_results", "atm_get_forces",
-    "atm_get_forces_fixed", "atm_clear_forces", "atm_get_pairs", "atm_backward", "atm_set_stream",
+    "atm_get_forces_fixed", "atm_clear_forces", "atm_get_pairs", "atm_backward", "atm_backward_device",
+    "atm_set_stream",
     "atm_autotune", "atm_get_tuning", "atm_set_tuning", "atm_get_stats", "atm_get_list_stats", "atm_profile",
     "atm_restraints_create", "atm_restraints_destroy", "atm_restraints_set", "atm_restraints_eval",
     "atm_md_init", "atm_md_step", "atm_md_get",
@@ -124,6 +126,7 @@ def load(path: str | None = None):
         "atm_create": [ctypes.POINTER(SystemDesc), i32, i32, ctypes.POINTER(vp)],
         "atm_destroy": [vp],
         "atm_set_particles": [vp, i32, dp, dp, dp, dp],
+        "atm_set_particles_device": [vp, vp, vp, vp],
         "atm_set_displacements": [vp, i32, dp],
         "atm_set_positions": [vp, i32, dp],
         "atm_set_positions_device": [vp, vp],
@@ -137,6 +140,7 @@ def load(path: str | None = None):
         "atm_clear_forces": [vp],
         "atm_get_pairs": [vp, i32, i32, ctypes.POINTER(i32), ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)],
         "atm_backward": [vp, dp, dp, dp, dp, dp],
+        "atm_backward_device": [vp, dp, vp, vp, vp, dp],
         "atm_set_stream": [vp, vp],
         "atm_autotune": [vp, i32, ctypes.POINTER(Tuning)],
         "atm_get_tuning": [vp, ctypes.POINTER(Tuning)],

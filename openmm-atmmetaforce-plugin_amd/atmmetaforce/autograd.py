@@ -7,13 +7,16 @@ alpha, u0, w0) -> W (num_systems,) float64.
       (systems concatenated in order), any float dtype, any device;
   lambda1, lambda2, alpha, u0, w0: per-system tensors (num_systems,).
 
-The forward uploads the parameters and runs one evaluation of every system on
-the GPU.  The backward runs the hand-written backward kernel (atm_backward in
-include/atm_mi355x.h), which walks the forward's cluster-pair lists again and
-recomputes every in-cutoff pair term at x and at x + d, weighted by
+The forward sets the parameters and runs one evaluation of every system on
+the GPU.  Per-atom tensors on a GPU go to the library as device arrays
+(atm_set_particles_device) and their gradients come back as device arrays
+(atm_backward_device): no per-atom data crosses to the host.  The backward
+runs the hand-written backward kernel, which walks the forward's lists again
+and recomputes every in-cutoff pair term at x and at x + d, weighted by
 dW/dU1 and dW/dU2 -- nothing per pair is stored between the passes -- and the
-closed-form derivatives of the softplus in the five scalar parameters.
-Pair terms of the backward are FP32, their sums FP64.
+closed-form derivatives of the softplus in the five scalar parameters.  Pair
+terms of the backward are FP64 from the exact r^2, their per-lane sums FP32,
+their per-atom sums FP64.
 
 If the batch was evaluated again between a forward and its backward, the
 backward re-runs that forward's evaluation first: at the forward's parameters
@@ -32,12 +35,15 @@ def _np(t) -> np.ndarray:
     return t.detach().to("cpu", torch.float64).numpy()
 
 
+def _dev64(t) -> torch.Tensor:
+    return t.detach().to(torch.float64).contiguous()
+
+
 class ATMEnergyFunction(torch.autograd.Function):
     @staticmethod
     def forward(ctx, batch: ATMBatch, charges, sigmas, epsilons, lambda1, lambda2, alpha, u0, w0):
         S = len(batch.systems)
-        sizes = batch.num_atoms
-        total = sum(sizes)
+        total = sum(batch.num_atoms)
         for name, t in (("charges", charges), ("sigmas", sigmas), ("epsilons", epsilons)):
             if t.shape != (total,):
                 raise ValueError(f"{name} must have shape ({total},), got {tuple(t.shape)}")
@@ -45,7 +51,13 @@ class ATMEnergyFunction(torch.autograd.Function):
         for t in scal:
             if t.shape != (S,):
                 raise ValueError(f"per-system parameters must have shape ({S},)")
-        state = _upload(batch, _np(charges), _np(sigmas), _np(epsilons), [_np(t) for t in scal])
+        on_gpu = charges.is_cuda and sigmas.is_cuda and epsilons.is_cuda
+        if on_gpu:
+            atoms = (_dev64(charges), _dev64(sigmas), _dev64(epsilons))
+        else:
+            atoms = (_np(charges), _np(sigmas), _np(epsilons))
+        state = (on_gpu, atoms, [_np(t) for t in scal])
+        _upload(batch, state)
         W = np.array([batch.result(k)["energy"] for k in range(S)])
         ctx.batch = batch
         ctx.state = state
@@ -59,24 +71,42 @@ class ATMEnergyFunction(torch.autograd.Function):
         batch = ctx.batch
         if batch._eval_id != ctx.eval_id:
             batch.restore_positions(ctx.positions)
-            _upload(batch, *ctx.state)
-        g = batch.backward(_np(gW))
-        outs = [g["dq"], g["dsigma"], g["deps"]] + [g["dparams"][:, k] for k in range(5)]
-        grads = [torch.as_tensor(o).to(device=dev, dtype=dt) for o, (dt, dev) in zip(outs, ctx.meta)]
+            _upload(batch, ctx.state)
+        g = _np(gW)
+        on_gpu = ctx.state[0]
         need = ctx.needs_input_grad[1:]
-        return (None,) + tuple(gr if n else None for gr, n in zip(grads, need))
+        if on_gpu:
+            total = sum(batch.num_atoms)
+            dev = ctx.meta[0][1]
+            outs_atoms = [torch.empty(total, dtype=torch.float64, device=dev) if need[k] else None for k in range(3)]
+            dparams = batch.backward_device(g, *[o.data_ptr() if o is not None else 0 for o in outs_atoms])
+        else:
+            r = batch.backward(g)
+            outs_atoms = [torch.as_tensor(r["dq"]), torch.as_tensor(r["dsigma"]), torch.as_tensor(r["deps"])]
+            dparams = r["dparams"]
+        outs = outs_atoms + [torch.as_tensor(dparams[:, k]) for k in range(5)]
+        grads = [o.to(device=dev, dtype=dt) if (o is not None and n) else None
+                 for o, (dt, dev), n in zip(outs, ctx.meta, need)]
+        return (None,) + tuple(grads)
 
 
-def _upload(batch: ATMBatch, q, s, e, scal):
-    off = 0
-    for k, (sysk, n) in enumerate(zip(batch.systems, batch.num_atoms)):
-        batch.set_particles(k, q[off:off + n], s[off:off + n], e[off:off + n], sysk.displacements)
+def _upload(batch: ATMBatch, state):
+    on_gpu, atoms, scal = state
+    if on_gpu:
+        q, s, e = atoms
+        torch.cuda.current_stream(q.device).synchronize()  # the library reads on its own stream
+        batch.set_particles_device(q.data_ptr(), s.data_ptr(), e.data_ptr())
+    else:
+        q, s, e = atoms
+        off = 0
+        for k, (sysk, n) in enumerate(zip(batch.systems, batch.num_atoms)):
+            batch.set_particles(k, q[off:off + n], s[off:off + n], e[off:off + n], sysk.displacements)
+            off += n
+    for k, sysk in enumerate(batch.systems):
         p = sysk.params
         batch.set_params(k, (scal[0][k], scal[1][k], scal[2][k], scal[3][k], scal[4][k], p.umax, p.ubcore, p.acore,
                              p.direction))
-        off += n
     batch.compute(forces=False)
-    return (q, s, e, scal)
 
 
 def atm_energy(batch: ATMBatch, charges, sigmas, epsilons, lambda1, lambda2, alpha, u0, w0):

@@ -228,6 +228,20 @@ class ATMBatch:
                                              cap.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
         return {"pairs": pairs, "max_entries": mx, "capacity": cap}
 
+    def set_particles_device(self, charges_ptr: int, sigmas_ptr: int, epsilons_ptr: int):
+        """Charges, sigmas, epsilons of every system from device float64 arrays (pointers)."""
+        C.check(self._lib.atm_set_particles_device(self._h, ctypes.c_void_p(charges_ptr), ctypes.c_void_p(sigmas_ptr),
+                                                   ctypes.c_void_p(epsilons_ptr)))
+
+    def backward_device(self, grad_energy, dq_ptr: int, dsigma_ptr: int, deps_ptr: int) -> np.ndarray:
+        """atm_backward with the per-atom gradients written to device float64 arrays
+        (pointers, 0 = skip); returns dW/d(lambda1, lambda2, alpha, u0, w0) per system."""
+        g = np.ascontiguousarray(grad_energy, dtype=np.float64).reshape(len(self.systems))
+        dp = np.zeros(5 * len(self.systems))
+        vp = lambda p: ctypes.c_void_p(p) if p else None
+        C.check(self._lib.atm_backward_device(self._h, _dptr(g), vp(dq_ptr), vp(dsigma_ptr), vp(deps_ptr), _dptr(dp)))
+        return dp.reshape(-1, 5)
+
     def backward(self, grad_energy) -> dict:
         g = np.ascontiguousarray(grad_energy, dtype=np.float64).reshape(len(self.systems))
         tot = sum(self.num_atoms)

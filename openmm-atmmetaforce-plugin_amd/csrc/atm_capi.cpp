@@ -28,6 +28,10 @@ hipError_t launch_backward(const DevPtrs& P, long long totalSlots, int setBegin,
 hipError_t launch_backward_gather(const DevPtrs& P, int sysBegin, int nsys, int maxAtoms, double* dq, double* dsig,
                                   double* deps, hipStream_t st);
 hipError_t launch_set_positions(DevPtrs P, float4* posq, const float* xyz, long long n, hipStream_t st);
+hipError_t launch_check_particles(const double* q, const double* sig, const double* eps, long long n, int* bad,
+                                  hipStream_t st);
+hipError_t launch_set_particles(const DevPtrs& P, const double* q, const double* sig, const double* eps, long long n,
+                                double sqk, hipStream_t st);
 }
 
 namespace atm {
@@ -61,9 +65,10 @@ struct Context {
     // device buffers
     DevBuf dDesc, dRt, dPosq, dDispl, dLjp, dPrm64, dParams, dRes, dOutF;
     DevBuf dBpos, dBcell, dBrank, dSlotOf, dCellCount, dCellStart, dColStart, dColCount;
-    DevBuf dSpos, dSprm, dSprm64, dSidx, dSf, dSd, dBbc, dBbh, dGinfo, dGcount, dGenergy, dEntries, dSetEnergy;
+    DevBuf dSpos, dSprm, dSprm64, dSidx, dSslot, dSf, dSd, dBbc, dBbh, dGinfo, dGcount, dGenergy, dEntries, dSetEnergy;
     DevBuf dAlist, dAcount;
     DevBuf dGq, dGsig, dGeps, dWset, dDq, dDsig, dDeps;
+    DevBuf dFlag, dPosStage;  // validation counter; FP32 staging of host positions
     // batched Langevin dynamics (atm_md_*)
     DevBuf dMx, dMv, dMinvm, dMxSnap, dMvSnap;
     std::vector<double> masses;
@@ -81,6 +86,8 @@ struct Context {
     bool resultsValid = false;
     bool resultsOnHost = false;  // reference-structured path computed results on the host
     bool evaluated = false;
+    bool paramsDirty = false;    // host params not yet on the device
+    bool chargesOnDevice = false;  // the device charges were set from device arrays (host mirror stale)
 
     std::map<int, hipGraphExec_t> graphs;
 
@@ -268,6 +275,7 @@ void alloc_layout(Context& C) {
     C.dSprm.alloc(sizeof(float2) * C.totalSlots);
     C.dSprm64.alloc(sizeof(double4) * C.totalSlots);
     C.dSidx.alloc(sizeof(int) * C.totalSlots);
+    C.dSslot.alloc(sizeof(int2) * C.totalSlots);
     C.dSf.alloc(sizeof(long long) * 3 * C.totalSlots);
     C.dSd.alloc(sizeof(double) * 3 * C.totalSlots);
     zero_on_stream(C, C.dSd.p, sizeof(double) * 3 * C.totalSlots);  // side sums start clear
@@ -275,7 +283,7 @@ void alloc_layout(Context& C) {
     C.dBbh.alloc(sizeof(float4) * C.totalClusters);
     C.dGinfo.alloc(sizeof(int4) * C.totalGroups);
     C.dGcount.alloc(sizeof(int) * C.totalGroups);
-    C.dGenergy.alloc(sizeof(double) * C.totalGroups);
+    C.dGenergy.alloc(sizeof(double) * 8 * C.totalGroups);
     C.dEntries.alloc(sizeof(uint32_t) * C.totalEntries);
     if (C.tuning.pairKernel == kPairKernelAtomLists) {  // only the atom-list pair kernel reads them
         C.dAlist.alloc(sizeof(uint32_t) * C.totalAlist);
@@ -313,6 +321,7 @@ void alloc_layout(Context& C) {
     P.sprm = C.dSprm.as<float2>();
     P.sprm64 = C.dSprm64.as<double4>();
     P.sidx = C.dSidx.as<int>();
+    P.sslot = C.dSslot.as<int2>();
     P.sf = C.dSf.as<long long>();
     P.sfn = C.totalSlots;
     P.sd = C.dSd.as<double>();
@@ -339,6 +348,16 @@ void require_ready(const Context& C) {
             fail(ATM_ERR_STATE, "system " + std::to_string(s) + ": particles were never set (atm_set_particles)");
 }
 
+// Upload the ATM parameters of every system if any changed since the last upload.
+static_assert(sizeof(atm_params) == sizeof(AtmParamsDev), "atm_params mirrors AtmParamsDev");
+
+void flush_params(Context& C) {
+    if (!C.paramsDirty) return;
+    HIPCHK(hipMemcpyAsync(C.dParams.p, C.params.data(), sizeof(AtmParamsDev) * C.nsys, hipMemcpyHostToDevice, C.stream));
+    HIPCHK(hipStreamSynchronize(C.stream));
+    C.paramsDirty = false;
+}
+
 // Batched evaluation of every set and system, kernel by kernel.  With `ev`
 // (profiling), an event is recorded after each stage: ev[0] before the first
 // kernel, then after build, pair list, pair kernel, energy and blend.
@@ -347,8 +366,8 @@ void enqueue_batched(Context& C, int flags, hipEvent_t* ev = nullptr) {
     const DevPtrs& P = C.P;
     const int S2 = 2 * C.nsys;
     if (ev) HIPCHK(hipEventRecord(ev[0], st));
-    if (flags & ATM_EVAL_CLEAR_FORCES) HIPCHK(launch_fill_zero(P.outF, 3 * C.totalAtoms, st));
-    HIPCHK(launch_zero_counts(P, 0, (int)C.totalCells, 0, S2, st));
+    HIPCHK(launch_zero_counts(P, 0, (int)C.totalCells, 0, S2, P.outF,
+                              (flags & ATM_EVAL_CLEAR_FORCES) ? 3 * C.totalAtoms : 0, st));
     HIPCHK(launch_build(P, 0, S2, C.maxAtoms, C.maxCells, C.maxSlots, st));
     if (ev) HIPCHK(hipEventRecord(ev[1], st));
     HIPCHK(launch_pairlist(P, 0, S2, C.maxGroups, 4, st));
@@ -357,13 +376,14 @@ void enqueue_batched(Context& C, int flags, hipEvent_t* ev = nullptr) {
     HIPCHK(launch_pairs(P, 0, S2, C.maxGroups, C.tuning.pairWaves, (flags & ATM_EVAL_FORCES) != 0,
                         (flags & ATM_EVAL_FP32_PAIRS) == 0, C.tuning.pairKernel, st));
     if (ev) HIPCHK(hipEventRecord(ev[3], st));
-    HIPCHK(launch_energy(P, 0, C.nsys, st));
+    HIPCHK(launch_energy(P, 0, C.nsys, pair_energy_stride(C.tuning.pairKernel, (flags & ATM_EVAL_FP32_PAIRS) == 0), st));
     if (ev) HIPCHK(hipEventRecord(ev[4], st));
     if (flags & ATM_EVAL_FORCES) HIPCHK(launch_blend(P, 0, C.nsys, C.maxAtoms, 0, 0.0, 0.0, st));
     if (ev) HIPCHK(hipEventRecord(ev[5], st));
 }
 
 void eval_batched(Context& C, int flags) {
+    flush_params(C);
     const int key = flags & (ATM_EVAL_FORCES | ATM_EVAL_CLEAR_FORCES | ATM_EVAL_FP32_PAIRS);
     if (flags & ATM_EVAL_NO_GRAPH) {
         enqueue_batched(C, key);
@@ -397,6 +417,7 @@ void eval_batched(Context& C, int flags) {
 // and softplus on the host (ReferenceATMMetaForceKernels.cpp:83-98) and a
 // hybrid-force launch with a host scalar (CommonATMMetaForceKernels.cpp:199-201).
 void eval_plain(Context& C, int flags) {
+    flush_params(C);
     const hipStream_t st = C.stream;
     const DevPtrs& P = C.P;
     const bool forces = (flags & ATM_EVAL_FORCES) != 0;
@@ -408,13 +429,13 @@ void eval_plain(Context& C, int flags) {
         for (int stt = 0; stt < 2; ++stt) {
             const int c = 2 * s + stt;
             const SetDesc& D = C.sets[c];
-            HIPCHK(launch_zero_counts(P, D.cellOff, D.cellOff + D.nx * D.ny * D.nz, c, c + 1, st));
+            HIPCHK(launch_zero_counts(P, D.cellOff, D.cellOff + D.nx * D.ny * D.nz, c, c + 1, nullptr, 0, st));
             HIPCHK(launch_build(P, c, 1, D.natoms, D.nx * D.ny * D.nz, D.slotCap, st));
             HIPCHK(launch_pairlist(P, c, 1, D.groupCap, 4, st));
             if (uses_atom_lists(C, flags)) HIPCHK(launch_alist(P, c, 1, D.groupCap, st));
             HIPCHK(launch_pairs(P, c, 1, D.groupCap, C.tuning.pairWaves, true, (flags & ATM_EVAL_FP32_PAIRS) == 0,
                                 C.tuning.pairKernel, st));
-            HIPCHK(launch_set_energy(P, c, st));
+            HIPCHK(launch_set_energy(P, c, pair_energy_stride(C.tuning.pairKernel, (flags & ATM_EVAL_FP32_PAIRS) == 0), st));
             HIPCHK(hipMemcpyAsync(&U[stt], P.setEnergy + c, sizeof(double), hipMemcpyDeviceToHost, st));
             HIPCHK(hipStreamSynchronize(st));
         }
@@ -433,19 +454,20 @@ void eval_plain(Context& C, int flags) {
 // system, the state-1 set's launch sequence, its energy read back to the host,
 // F += F1.  Serves NonbondedForce evaluations outside the ATM force.
 void eval_state1(Context& C, int flags) {
+    flush_params(C);
     const hipStream_t st = C.stream;
     const DevPtrs& P = C.P;
     for (int s = 0; s < C.nsys; ++s) {
         const SetDesc& D = C.sets[2 * s];
         const int c = 2 * s;
         if (flags & ATM_EVAL_CLEAR_FORCES) HIPCHK(launch_fill_zero(P.outF + 3ll * D.atomOff, 3ll * D.natoms, st));
-        HIPCHK(launch_zero_counts(P, D.cellOff, D.cellOff + D.nx * D.ny * D.nz, c, c + 1, st));
+        HIPCHK(launch_zero_counts(P, D.cellOff, D.cellOff + D.nx * D.ny * D.nz, c, c + 1, nullptr, 0, st));
         HIPCHK(launch_build(P, c, 1, D.natoms, D.nx * D.ny * D.nz, D.slotCap, st));
         HIPCHK(launch_pairlist(P, c, 1, D.groupCap, 4, st));
         if (uses_atom_lists(C, flags)) HIPCHK(launch_alist(P, c, 1, D.groupCap, st));
         HIPCHK(launch_pairs(P, c, 1, D.groupCap, C.tuning.pairWaves, (flags & ATM_EVAL_FORCES) != 0,
                             (flags & ATM_EVAL_FP32_PAIRS) == 0, C.tuning.pairKernel, st));
-        HIPCHK(launch_set_energy(P, c, st));
+        HIPCHK(launch_set_energy(P, c, pair_energy_stride(C.tuning.pairKernel, (flags & ATM_EVAL_FP32_PAIRS) == 0), st));
         double U = 0.0;
         HIPCHK(hipMemcpyAsync(&U, P.setEnergy + c, sizeof(double), hipMemcpyDeviceToHost, st));
         if (flags & ATM_EVAL_FORCES) HIPCHK(launch_blend(P, s, 1, D.natoms, 2, 0.0, 1.0, st));
@@ -679,7 +701,19 @@ int atm_set_positions(void* context, int32_t sys, const double* x) {
             float4& p = C.hPosq[off + i];
             p.x = (float)x[3 * i]; p.y = (float)x[3 * i + 1]; p.z = (float)x[3 * i + 2];
         }
-        upload_system(C, sys);
+        if (C.chargesOnDevice) {
+            // the device holds charges the host mirror does not: replace x, y, z only
+            std::vector<float> xyz(3 * (size_t)n);
+            for (int i = 0; i < n; ++i) {
+                xyz[3 * i] = C.hPosq[off + i].x; xyz[3 * i + 1] = C.hPosq[off + i].y; xyz[3 * i + 2] = C.hPosq[off + i].z;
+            }
+            if (!C.dPosStage.p) C.dPosStage.alloc(sizeof(float) * 3 * C.totalAtoms);
+            HIPCHK(hipMemcpyAsync(C.dPosStage.as<float>() + 3ll * off, xyz.data(), sizeof(float) * 3 * n,
+                                  hipMemcpyHostToDevice, C.stream));
+            HIPCHK(launch_set_positions(C.P, C.dPosq.as<float4>() + off, C.dPosStage.as<float>() + 3ll * off, n, C.stream));
+        } else {
+            upload_system(C, sys);
+        }
         HIPCHK(hipStreamSynchronize(C.stream));
     });
 }
@@ -690,6 +724,25 @@ int atm_set_positions_device(void* context, const float* positions_dev) {
         if (!positions_dev) fail(ATM_ERR_INVALID_ARGUMENT, "null device pointer");
         HIPCHK(hipSetDevice(C.device));
         HIPCHK(launch_set_positions(C.P, C.dPosq.as<float4>(), positions_dev, C.totalAtoms, C.stream));
+    });
+}
+
+int atm_set_particles_device(void* context, const double* charges, const double* sigmas, const double* epsilons) {
+    return guarded([&] {
+        Context& C = ctx_of(context);
+        if (!charges || !sigmas || !epsilons) fail(ATM_ERR_INVALID_ARGUMENT, "null device array");
+        HIPCHK(hipSetDevice(C.device));
+        if (!C.dFlag.p) C.dFlag.alloc(sizeof(int));
+        zero_on_stream(C, C.dFlag.p, sizeof(int));
+        HIPCHK(launch_check_particles(charges, sigmas, epsilons, C.totalAtoms, C.dFlag.as<int>(), C.stream));
+        int bad = 0;
+        to_host(C, &bad, C.dFlag.p, sizeof(int));
+        if (bad) fail(ATM_ERR_INVALID_ARGUMENT, std::to_string(bad) + " atoms with a non-finite parameter or a negative "
+                                                "sigma or epsilon (nothing was changed)");
+        HIPCHK(launch_set_particles(C.P, charges, sigmas, epsilons, C.totalAtoms, std::sqrt(ATM_ONE_4PI_EPS0), C.stream));
+        HIPCHK(hipStreamSynchronize(C.stream));
+        C.chargesOnDevice = true;
+        std::fill(C.haveParticles.begin(), C.haveParticles.end(), 1);
     });
 }
 
@@ -718,9 +771,7 @@ int atm_set_params(void* context, int32_t sys, const atm_params* p) {
         if (!p) fail(ATM_ERR_INVALID_ARGUMENT, "null parameters");
         HIPCHK(hipSetDevice(C.device));
         C.params[sys] = *p;
-        HIPCHK(hipMemcpyAsync(C.dParams.as<AtmParamsDev>() + sys, &C.params[sys], sizeof(AtmParamsDev), hipMemcpyHostToDevice,
-                              C.stream));
-        HIPCHK(hipStreamSynchronize(C.stream));
+        C.paramsDirty = true;  // uploaded (all systems at once) before the next evaluation
     });
 }
 
@@ -912,9 +963,10 @@ int atm_set_tuning(void* context, const atm_tuning* t) {
             t->pair_waves_per_block != 8)
             fail(ATM_ERR_INVALID_ARGUMENT, "pair_waves_per_block must be 1, 2, 4 or 8");
         if (t->zbins_per_column < 1 || t->zbins_per_column > 16) fail(ATM_ERR_INVALID_ARGUMENT, "zbins_per_column must be in [1, 16]");
-        if (t->pair_kernel < 0 || t->pair_kernel > 5) fail(ATM_ERR_INVALID_ARGUMENT, "pair_kernel must be in [0, 5]");
+        if (t->pair_kernel < 0 || t->pair_kernel >= kPairKernelCount)
+            fail(ATM_ERR_INVALID_ARGUMENT, "pair_kernel must be in [0, " + std::to_string(kPairKernelCount - 1) + "]");
         if (t->pair_kernel != 0 && t->pair_waves_per_block == 8)
-            fail(ATM_ERR_INVALID_ARGUMENT, "pair kernels 1-5 take 1, 2 or 4 waves per block");
+            fail(ATM_ERR_INVALID_ARGUMENT, "pair kernels 1-6 take 1, 2 or 4 waves per block");
         if (t->list_capacity < 0) fail(ATM_ERR_INVALID_ARGUMENT, "list_capacity must be >= 0");
         HIPCHK(hipSetDevice(C.device));
         HIPCHK(hipStreamSynchronize(C.stream));
@@ -947,11 +999,11 @@ int atm_autotune(void* context, int32_t repeats, atm_tuning* out) {
         require_ready(C);
         HIPCHK(hipSetDevice(C.device));
         const int reps = std::max(1, (int)repeats);
-        const double scales[] = {0.9, 1.0, 1.15, 1.3, 1.5};
+        const double scales[] = {0.8, 0.9, 1.0, 1.15, 1.3};
         const int waves[] = {1, 2, 4};
         // (the atom-list kernel, 5, is left out: it measured slower than the tile kernel on every
         // column scale tried, and its lists need their own buffers)
-        const int kernels[] = {0, 1, 2};
+        const int kernels[] = {0, 1, kPairKernelFiltered};
         const int zpcs[] = {2, 4};
         const Tuning orig = C.tuning;
         hipEvent_t e0, e1;
@@ -1051,6 +1103,7 @@ int atm_profile(void* context, int32_t flags, int32_t reps, atm_stage_times* out
         if (reps < 1) fail(ATM_ERR_INVALID_ARGUMENT, "reps must be positive");
         require_ready(C);
         HIPCHK(hipSetDevice(C.device));
+        flush_params(C);
         hipEvent_t ev[6];
         for (auto& e : ev) HIPCHK(hipEventCreate(&e));
         double acc[5] = {0, 0, 0, 0, 0};
@@ -1078,71 +1131,92 @@ int atm_profile(void* context, int32_t flags, int32_t reps, atm_stage_times* out
     });
 }
 
-int atm_backward(void* context, const double* grad_energy, double* d_charges, double* d_sigmas, double* d_epsilons,
-                 double* d_params) {
-    return guarded([&] {
-        Context& C = ctx_of(context);
-        if (!grad_energy) fail(ATM_ERR_INVALID_ARGUMENT, "null grad_energy");
-        HIPCHK(hipSetDevice(C.device));
-        fetch_results(C);
-        const int S = C.nsys;
-        // per-set weights: dW/dU_state = gW * ((1-sp) for e0's state, sp for the other)
-        std::vector<double> wset(2 * S);
-        for (int s = 0; s < S; ++s) {
-            const atm_result& r = C.results[s];
-            const bool fwd = C.params[s].direction > 0.0;
-            const double w_e0 = 1.0 - r.hybrid_weight, w_o = r.hybrid_weight;
-            wset[2 * s] = grad_energy[s] * (fwd ? w_e0 : w_o);
-            wset[2 * s + 1] = grad_energy[s] * (fwd ? w_o : w_e0);
+namespace {
+
+// atm_backward / atm_backward_device: per-atom gradients into host arrays or
+// (deviceOut) straight into the caller's device arrays.
+void backward_impl(Context& C, const double* grad_energy, double* d_charges, double* d_sigmas, double* d_epsilons,
+                   double* d_params, bool deviceOut) {
+    if (!grad_energy) fail(ATM_ERR_INVALID_ARGUMENT, "null grad_energy");
+    HIPCHK(hipSetDevice(C.device));
+    fetch_results(C);
+    const int S = C.nsys;
+    // per-set weights: dW/dU_state = gW * ((1-sp) for e0's state, sp for the other)
+    std::vector<double> wset(2 * S);
+    for (int s = 0; s < S; ++s) {
+        const atm_result& r = C.results[s];
+        const bool fwd = C.params[s].direction > 0.0;
+        const double w_e0 = 1.0 - r.hybrid_weight, w_o = r.hybrid_weight;
+        wset[2 * s] = grad_energy[s] * (fwd ? w_e0 : w_o);
+        wset[2 * s + 1] = grad_energy[s] * (fwd ? w_o : w_e0);
+    }
+    if (d_charges || d_sigmas || d_epsilons) {
+        if (!C.dGq.p) {
+            C.dGq.alloc(sizeof(double) * C.totalSlots);
+            C.dGsig.alloc(sizeof(double) * C.totalSlots);
+            C.dGeps.alloc(sizeof(double) * C.totalSlots);
+            C.P.gq = C.dGq.as<double>();
+            C.P.gsig = C.dGsig.as<double>();
+            C.P.geps = C.dGeps.as<double>();
+            C.dropGraphs();
         }
-        if (d_charges || d_sigmas || d_epsilons) {
-            if (!C.dGq.p) {
-                C.dGq.alloc(sizeof(double) * C.totalSlots);
-                C.dGsig.alloc(sizeof(double) * C.totalSlots);
-                C.dGeps.alloc(sizeof(double) * C.totalSlots);
-                C.P.gq = C.dGq.as<double>();
-                C.P.gsig = C.dGsig.as<double>();
-                C.P.geps = C.dGeps.as<double>();
-                C.dropGraphs();
+        if (C.dWset.bytes < sizeof(double) * 2 * S) C.dWset.alloc(sizeof(double) * 2 * S);
+        HIPCHK(hipMemcpyAsync(C.dWset.p, wset.data(), sizeof(double) * 2 * S, hipMemcpyHostToDevice, C.stream));
+        HIPCHK(launch_backward(C.P, C.totalSlots, 0, 2 * S, C.maxGroups, C.dWset.as<double>(), C.stream));
+        const size_t nb = sizeof(double) * C.totalAtoms;
+        double *oq = d_charges, *os = d_sigmas, *oe = d_epsilons;
+        if (!deviceOut) {
+            // host outputs: through context buffers (allocated once)
+            if (C.dDq.bytes < nb) {
+                C.dDq.alloc(nb);
+                C.dDsig.alloc(nb);
+                C.dDeps.alloc(nb);
             }
-            C.dWset.alloc(sizeof(double) * 2 * S);
-            C.dDq.alloc(sizeof(double) * C.totalAtoms);
-            C.dDsig.alloc(sizeof(double) * C.totalAtoms);
-            C.dDeps.alloc(sizeof(double) * C.totalAtoms);
-            HIPCHK(hipMemcpyAsync(C.dWset.p, wset.data(), sizeof(double) * 2 * S, hipMemcpyHostToDevice, C.stream));
-            HIPCHK(launch_backward(C.P, C.totalSlots, 0, 2 * S, C.maxGroups, C.dWset.as<double>(), C.stream));
-            HIPCHK(launch_backward_gather(C.P, 0, S, C.maxAtoms, C.dDq.as<double>(), C.dDsig.as<double>(),
-                                          C.dDeps.as<double>(), C.stream));
-            HIPCHK(hipStreamSynchronize(C.stream));
-            const size_t nb = sizeof(double) * C.totalAtoms;
+            oq = C.dDq.as<double>(); os = C.dDsig.as<double>(); oe = C.dDeps.as<double>();
+        }
+        HIPCHK(launch_backward_gather(C.P, 0, S, C.maxAtoms, oq, os, oe, C.stream));
+        HIPCHK(hipStreamSynchronize(C.stream));
+        if (!deviceOut) {
             if (d_charges) to_host(C, d_charges, C.dDq.p, nb);
             if (d_sigmas) to_host(C, d_sigmas, C.dDsig.p, nb);
             if (d_epsilons) to_host(C, d_epsilons, C.dDeps.p, nb);
         }
-        if (d_params) {
-            for (int s = 0; s < S; ++s) {
-                const atm_params& p = C.params[s];
-                const atm_result& r = C.results[s];
-                const double g = grad_energy[s];
-                const double usc = r.perturbation_energy;
-                const double x = -p.alpha * (usc - p.u0);
-                const double lnee = x > 0.0 ? x + std::log1p(std::exp(-x)) : std::log1p(std::exp(x));
-                const double inv_ee = x > 0.0 ? std::exp(-x) / (1.0 + std::exp(-x)) : 1.0 / (1.0 + std::exp(x));
-                double dl1 = 0.0, dl2 = usc, da = 0.0, du0 = 0.0;
-                if (p.alpha > 0.0) {
-                    dl1 = -lnee / p.alpha;
-                    dl2 += lnee / p.alpha;
-                    du0 = (p.lambda2 - p.lambda1) * (1.0 - inv_ee);
-                    da = (p.lambda2 - p.lambda1) * (-lnee / (p.alpha * p.alpha) - (usc - p.u0) * (1.0 - inv_ee) / p.alpha);
-                }
-                d_params[5 * s + 0] = g * dl1;
-                d_params[5 * s + 1] = g * dl2;
-                d_params[5 * s + 2] = g * da;
-                d_params[5 * s + 3] = g * du0;
-                d_params[5 * s + 4] = g * 1.0;
+    }
+    if (d_params) {
+        for (int s = 0; s < S; ++s) {
+            const atm_params& p = C.params[s];
+            const atm_result& r = C.results[s];
+            const double g = grad_energy[s];
+            const double usc = r.perturbation_energy;
+            const double x = -p.alpha * (usc - p.u0);
+            const double lnee = x > 0.0 ? x + std::log1p(std::exp(-x)) : std::log1p(std::exp(x));
+            const double inv_ee = x > 0.0 ? std::exp(-x) / (1.0 + std::exp(-x)) : 1.0 / (1.0 + std::exp(x));
+            double dl1 = 0.0, dl2 = usc, da = 0.0, du0 = 0.0;
+            if (p.alpha > 0.0) {
+                dl1 = -lnee / p.alpha;
+                dl2 += lnee / p.alpha;
+                du0 = (p.lambda2 - p.lambda1) * (1.0 - inv_ee);
+                da = (p.lambda2 - p.lambda1) * (-lnee / (p.alpha * p.alpha) - (usc - p.u0) * (1.0 - inv_ee) / p.alpha);
             }
+            d_params[5 * s + 0] = g * dl1;
+            d_params[5 * s + 1] = g * dl2;
+            d_params[5 * s + 2] = g * da;
+            d_params[5 * s + 3] = g * du0;
+            d_params[5 * s + 4] = g * 1.0;
         }
-    });
+    }
+}
+
+}  // namespace
+
+int atm_backward(void* context, const double* grad_energy, double* d_charges, double* d_sigmas, double* d_epsilons,
+                 double* d_params) {
+    return guarded([&] { backward_impl(ctx_of(context), grad_energy, d_charges, d_sigmas, d_epsilons, d_params, false); });
+}
+
+int atm_backward_device(void* context, const double* grad_energy, double* d_charges, double* d_sigmas,
+                        double* d_epsilons, double* d_params) {
+    return guarded([&] { backward_impl(ctx_of(context), grad_energy, d_charges, d_sigmas, d_epsilons, d_params, true); });
 }
 
 // ------------------------------------------------------------------ restraints

@@ -33,7 +33,8 @@ struct DevPtrs {
     float4* spos;
     float2* sprm;
     double4* sprm64;
-    int* sidx;
+    int* sidx;             // atom of each slot in its final order (-1: padding)
+    int2* sslot;           // (atom, cell) in the counting scatter's provisional order ((-1, -1): padding)
     long long* sf;         // 64-bit fixed-point pair-force sums: x of slot s at sf[s], y at
     long long sfn;         // sf[sfn + s], z at sf[2 sfn + s] (one array per component keeps
                            // the atomics of consecutive slots on shared cache lines)
@@ -45,7 +46,7 @@ struct DevPtrs {
     // groups
     int4* ginfo;
     int* gcount;
-    double* genergy;
+    double* genergy;       // pair-kernel energy slots: one per group, or one per i-cluster wave (8 per group)
     uint32_t* entries;
     uint32_t* alist;       // atom-level j lists per i-cluster (pair kernel 5)
     int* acount;           // entries (whole 8-atom blocks) per i-cluster
@@ -56,7 +57,10 @@ struct DevPtrs {
     double* geps;
 };
 
-hipError_t launch_zero_counts(const DevPtrs& P, int cellBegin, int cellEnd, int setBegin, int setEnd, hipStream_t st);
+// Clears cell counters [cellBegin, cellEnd), the pair counters of sets [setBegin, setEnd)
+// and, when nf > 0, the doubles f[0, nf).
+hipError_t launch_zero_counts(const DevPtrs& P, int cellBegin, int cellEnd, int setBegin, int setEnd, double* f,
+                              long long nf, hipStream_t st);
 hipError_t launch_build(const DevPtrs& P, int setBegin, int nsets, int maxAtoms, int maxCells, int maxSlots,
                         hipStream_t st);
 hipError_t launch_pairlist(const DevPtrs& P, int setBegin, int nsets, int maxGroups, int listWaves, hipStream_t st);
@@ -66,10 +70,19 @@ hipError_t launch_alist(const DevPtrs& P, int setBegin, int nsets, int maxGroups
 // variant 0: 8x8 tile kernel; 1-4: compacted kernel (FP64 pairs only) with (list entries per batch,
 // copies of the j-force sums, drain after every test step) = (4, 2, yes), (4, 2, no), (4, 4, yes),
 // (8, 2, yes); 5: atom-level j lists (launch_alist first), 8 i x 8 j lanes per step
+// 6: filtered atom queues (one wave per i-cluster, FP64 pairs only; FP32 pairs run variant 0)
+constexpr int kPairKernelFiltered = 6;
+constexpr int kPairKernelCount = 7;
 hipError_t launch_pairs(const DevPtrs& P, int setBegin, int nsets, int maxGroups, int waves, bool forces, bool fp64,
                         int variant, hipStream_t st);
-hipError_t launch_energy(const DevPtrs& P, int sysBegin, int nsys, hipStream_t st);
-hipError_t launch_set_energy(const DevPtrs& P, int c, hipStream_t st);
+// Energy slots the pair kernel fills per group in genergy (1, or 8 for the filtered kernel).
+inline int pair_energy_stride(int variant, bool fp64) { return (fp64 && variant == kPairKernelFiltered) ? 8 : 1; }
+hipError_t launch_energy(const DevPtrs& P, int sysBegin, int nsys, int estride, hipStream_t st);
+hipError_t launch_set_energy(const DevPtrs& P, int c, int estride, hipStream_t st);
+// Backward pass of the pair terms (k_pairs_f's filtered queues): adds wset[c] dE/d(q', h, e')
+// of every in-cutoff pair of sets [setBegin, setBegin + nsets) to gq, gsig, geps (slot order).
+hipError_t launch_backward_filtered(const DevPtrs& P, int setBegin, int nsets, int maxGroups, const double* wset,
+                                    hipStream_t st);
 hipError_t launch_blend(const DevPtrs& P, int sysBegin, int nsys, int maxAtoms, int hostMode, double sp, double dir,
                         hipStream_t st);
 hipError_t launch_fill_i64(long long* p, long long n, hipStream_t st);

@@ -1,19 +1,22 @@
 // gfx950 kernels of the ATM evaluation.
 //
 // Per evaluation, for every coordinate set c = 2*system + state of the batch:
-//   k_zero_counts  clear fine-cell counters and the per-set pair counters
+//   k_zero_counts  clear fine-cell counters, the per-set pair counters and
+//                  (batched path) the accumulated ATM forces
 //   k_bin          x (+ d for state 1), wrap into the box, fine-cell id and rank
 //   k_scan         per-set column/cell scans, dummy padding, group table
-//   k_scatter_idx  counting-sort scatter of atom indices into cluster order
-//   k_cellsort_idx order each fine cell by atom index (deterministic results)
-//   k_gather_bbox  gather positions/parameters into cluster order, clear the
+//   k_scatter_idx  counting-sort scatter of (atom, cell) into provisional slots
+//   k_gather_bbox  order each fine cell by atom index (deterministic results),
+//                  gather positions/parameters into cluster order, clear the
 //                  set's force accumulators, per-cluster bounding boxes
 //   k_pairlist     per group of 8 clusters: j-clusters (from the group's
 //                  z-window) whose boxes are within the cutoff of at least one
 //                  i-cluster (half shell, i <= j)
 //   k_pairs        LJ + reaction-field pair kernel, one wave64 per group,
 //                  lanes = 8 j-atoms x 8 i-atoms, FP32 geometry, FP64 pair
-//                  function (or FP32), FP64 energy, 64-bit fixed-point forces
+//                  function (or FP32), FP64 energy, 64-bit fixed-point forces;
+//                  k_pairs_f: one wave per i-cluster over its atom-filtered j
+//                  atoms
 // and per system:
 //   k_energy       U1, U2 (fixed-order FP64 sums), soft-core, softplus -> W, sp
 //   k_blend        F += (1-sp) F1 + sp F2 (direction-aware), gathered back to
@@ -212,9 +215,11 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
 
 // ---------------------------------------------------------------- binning
 
-__global__ void k_zero_counts(DevPtrs P, int cellBegin, int cellEnd, int setBegin, int setEnd) {
+__global__ void k_zero_counts(DevPtrs P, int cellBegin, int cellEnd, int setBegin, int setEnd, double* f,
+                              long long nf) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     for (int k = cellBegin + i; k < cellEnd; k += gridDim.x * blockDim.x) P.cellCount[k] = 0;
+    for (long long k = i; k < nf; k += (long long)gridDim.x * blockDim.x) f[k] = 0.0;
     // the overflow flag stays set until the host has seen it (it relays out and clears it)
     if (i < setEnd - setBegin) {
         P.rt[setBegin + i].npairs = 0ull;
@@ -315,7 +320,7 @@ __global__ void __launch_bounds__(1024) k_scan(DevPtrs P, int setBegin) {
         const int ncl = padded >> 3;
         const int ng = (ncl + kGroupClusters - 1) / kGroupClusters;
         P.colStart[D.colOff + col] = offS;
-        for (int s2 = offS + cnt; s2 < offS + padded; ++s2) P.sidx[D.slotOff + s2] = -1;
+        for (int s2 = offS + cnt; s2 < offS + padded; ++s2) P.sslot[D.slotOff + s2] = make_int2(-1, -1);
         for (int gg = 0; gg < ng; ++gg)
             P.ginfo[D.groupOff + offG + gg] =
                 make_int4((offS >> 3) + gg * kGroupClusters, min(kGroupClusters, ncl - gg * kGroupClusters), col, 0);
@@ -329,7 +334,8 @@ __global__ void __launch_bounds__(1024) k_scan(DevPtrs P, int setBegin) {
     }
 }
 
-// Counting-sort scatter of atom indices only (4 bytes per atom).
+// Counting-sort scatter: each atom's (index, cell) into its provisional slot
+// (cell start + the atomic rank of k_bin, whose order is not deterministic).
 __global__ void k_scatter_idx(DevPtrs P, int setBegin) {
     const int c = setBegin + blockIdx.y;
     const SetDesc& D = P.desc[c];
@@ -338,51 +344,62 @@ __global__ void k_scatter_idx(DevPtrs P, int setBegin) {
     const int k = D.setAtomOff + i;
     const int cell = P.bcell[k];
     const int slot = P.colStart[D.colOff + cell / D.nz] + P.cellStart[D.cellOff + cell] + P.brank[k];
-    P.sidx[D.slotOff + slot] = i;
+    P.sslot[D.slotOff + slot] = make_int2(i, cell);
 }
 
-// Order each fine cell's atom indices ascending (the atomic ranks of k_bin are
-// not deterministic).  Fixed cluster composition makes the FP32 partial sums,
-// and so every result of an evaluation, bit-reproducible.
-// Cells of up to 16 atoms (nearly all) are sorted in registers by an odd-even
-// transposition network (independent loads, no dependent global round trips);
-// larger cells by insertion sort in place.
-__global__ void k_cellsort_idx(DevPtrs P, int setBegin) {
-    const int c = setBegin + blockIdx.y;
-    const SetDesc& D = P.desc[c];
-    const int cell = blockIdx.x * blockDim.x + threadIdx.x;
-    if (cell >= D.nx * D.ny * D.nz) return;
-    const int cnt = P.cellCount[D.cellOff + cell];
-    if (cnt < 2) return;
-    int* idx = P.sidx + D.slotOff + P.colStart[D.colOff + cell / D.nz] + P.cellStart[D.cellOff + cell];
-    constexpr int kNet = 16;
-    if (cnt <= kNet) {
-        int v[kNet];
+// The atom that belongs at rank r of its fine cell: the cell's atoms ordered
+// by index (fixed cluster composition makes the FP32 partial sums, and so
+// every result of an evaluation, bit-reproducible).  mem: the cell's
+// provisional (atom, cell) slots.  Cells of up to 8 atoms (nearly all at the
+// default binning) are sorted in registers by an optimal 19-comparator
+// network, up to 16 by odd-even transposition, larger ones by rank counting.
+__device__ __forceinline__ int cell_rank_atom(const int2* mem, int cnt, int r) {
+    if (cnt <= 8) {
+        int v[8];
 #pragma unroll
-        for (int a = 0; a < kNet; ++a) v[a] = a < cnt ? idx[a] : 0x7FFFFFFF;
+        for (int a = 0; a < 8; ++a) v[a] = a < cnt ? mem[a].x : 0x7FFFFFFF;
+        auto cx = [&](int a, int b) {
+            const int lo = min(v[a], v[b]), hi = max(v[a], v[b]);
+            v[a] = lo;
+            v[b] = hi;
+        };
+        cx(0, 2); cx(1, 3); cx(4, 6); cx(5, 7);
+        cx(0, 4); cx(1, 5); cx(2, 6); cx(3, 7);
+        cx(0, 1); cx(2, 3); cx(4, 5); cx(6, 7);
+        cx(2, 4); cx(3, 5);
+        cx(1, 4); cx(3, 6);
+        cx(1, 2); cx(3, 4); cx(5, 6);
+        int out = v[0];
 #pragma unroll
-        for (int r = 0; r < kNet; ++r) {
+        for (int a = 1; a < 8; ++a) out = r == a ? v[a] : out;
+        return out;
+    }
+    if (cnt <= 16) {
+        int v[16];
 #pragma unroll
-            for (int a = r & 1; a + 1 < kNet; a += 2) {
+        for (int a = 0; a < 16; ++a) v[a] = a < cnt ? mem[a].x : 0x7FFFFFFF;
+#pragma unroll
+        for (int rr = 0; rr < 16; ++rr) {
+#pragma unroll
+            for (int a = rr & 1; a + 1 < 16; a += 2) {
                 const int lo = min(v[a], v[a + 1]), hi = max(v[a], v[a + 1]);
                 v[a] = lo;
                 v[a + 1] = hi;
             }
         }
+        int out = v[0];
 #pragma unroll
-        for (int a = 0; a < kNet; ++a)
-            if (a < cnt) idx[a] = v[a];
-        return;
+        for (int a = 1; a < 16; ++a) out = r == a ? v[a] : out;
+        return out;
     }
-    for (int a = 1; a < cnt; ++a) {
-        const int key = idx[a];
-        int b = a - 1;
-        while (b >= 0 && idx[b] > key) {
-            idx[b + 1] = idx[b];
-            --b;
-        }
-        idx[b + 1] = key;
+    int out = -1;
+    for (int a = 0; a < cnt; ++a) {
+        const int va = mem[a].x;
+        int less = 0;
+        for (int b = 0; b < cnt; ++b) less += mem[b].x < va ? 1 : 0;
+        if (less == r) out = va;
     }
+    return out;
 }
 
 __device__ __forceinline__ float oct_min(float v) {
@@ -415,7 +432,14 @@ __global__ void __launch_bounds__(256) k_gather_bbox(DevPtrs P, int setBegin) {
     }
     if (s >= P.rt[c].nslots) return;
     const int k = D.slotOff + s;
-    const int i = P.sidx[k];
+    // this slot's atom: rank s - (cell start) of its cell in index order
+    const int2 me = P.sslot[k];
+    int i = -1;
+    if (me.y >= 0) {
+        const int base = P.colStart[D.colOff + me.y / D.nz] + P.cellStart[D.cellOff + me.y];
+        i = cell_rank_atom(P.sslot + D.slotOff + base, P.cellCount[D.cellOff + me.y], s - base);
+    }
+    P.sidx[k] = i;
     // dummy slots: zero parameters, and positions far outside the box and at
     // least 4 nm from each other (x = 1e6 + 4 (s mod 2^20), y = 1e6 + 4 (s >> 20),
     // exact in FP32), so no pair with a dummy passes a cutoff test unless x/y
@@ -645,6 +669,15 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
+
+// Fixed-order sum of a set's energy slots: one per group, or (estride 8) one
+// per i-cluster wave, 8 per group.
+__device__ __forceinline__ double group_energy(const DevPtrs& P, const SetDesc& D, int g, int estride) {
+    if (estride == 1) return P.genergy[D.groupOff + g];
+    const double* e = P.genergy + 8ll * (D.groupOff + g);
+    return ((e[0] + e[1]) + (e[2] + e[3])) + ((e[4] + e[5]) + (e[6] + e[7]));
+}
+
 
 // Add one force component (component k of slot `slot`) with an atomic that
 // every lane issues on every path, so the number of memory operations between
@@ -1393,11 +1426,256 @@ __global__ void __launch_bounds__(64) k_pairs_dump(DevPtrs P, int c, int2* out, 
     atomicAdd(counters + 2, nin);  // per lane: in-cutoff pairs
 }
 
+// ------------------------------------------------- filtered atom queues
+//
+// One wave64 per i-cluster t of a group (grid: 8 waves per group).  The wave
+// keeps the group's list entries whose mask has bit t, tests the 8 atoms of 8
+// kept entries at a time (lane = 8 k + jj: atom jj of kept entry k) against
+// t's bounding box grown by the list cutoff, and appends the atoms that pass
+// to a queue in LDS: position relative to a grid point r0 at the box centre
+// (the image of each atom nearest r0, exact on the 2^-20 grid), j-slot and
+// FP64 parameters.  Every 8 queued atoms make one 8 x 8 pair step
+// (lane = 8 jj + ii: queued atom jj with i-atom ii) with the exact FP64 r^2,
+// the exact r < rc and the FP64 pair function; a pair counts when its j-slot
+// is above its i-slot (each entry's j-cluster is at or above t).  The atom
+// test keeps ~45 % of a step's lane pairs inside the cutoff at 23k atoms
+// against ~21 % for whole 8 x 8 cluster tiles.
+//
+// Nearest image to r0 = pairwise minimum image for every pair inside the
+// cutoff when rc + (box half-extent) < L/2 in each dimension: |x_j - r0| <
+// rc + h <= L/2 for the in-cutoff image.  Clusters for which that fails (small
+// boxes, sparse columns) accept every atom in that dimension and apply the
+// pairwise minimum image in their steps.
+constexpr int kQueueCap = 72;  // 7 left over from the last step + one test round of 64
+
+//
+// MODE kFiltEnergy / kFiltForces: the forward (energy; energy and forces).
+// MODE kFiltBackward: the backward pass on the same queues -- per in-cutoff
+// pair, the derivatives of its energy with respect to the stored per-atom
+// parameters q' = q sqrt(k_e), h = sigma/2, e' = 2 sqrt(eps) of both atoms,
+// in FP64 from the exact r^2:
+//   dE/dq'_i = q'_j phi,  phi = 1/r + k_rf r^2 - c_rf
+//   dE/dh    = e'_i e'_j (12 (s/r)^12 - 6 (s/r)^6) / s = e'_i e'_j (12 s6 - 6) s2^2 u
+//              with s = h_i + h_j, u = s / r^2, s2 = s u, s6 = s2^3 (0 at s = 0)
+//   dE/de'_i = e'_j (s6^2 - s6)
+// summed per lane in FP32, weighted by wset[c] (dW/dU of the set) and added
+// to FP64 per-slot accumulators (gq, gsig, geps).
+constexpr int kFiltEnergy = 0, kFiltForces = 1, kFiltBackward = 2;
+
+template <int W, int MODE>
+__global__ void __launch_bounds__(64 * W) k_pairs_f(DevPtrs P, int setBegin, const double* wset) {
+    constexpr bool FORCES = MODE == kFiltForces;
+    constexpr bool BWD = MODE == kFiltBackward;
+    const int c = setBegin + blockIdx.y;
+    const SetDesc D = P.desc[c];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int gw = blockIdx.x * W + wave;
+    const int g = gw >> 3, t = gw & 7;
+    if (g >= P.rt[c].ngroups) return;
+    double* eOut = P.genergy + 8ll * (D.groupOff + g) + t;
+    const int4 gi = P.ginfo[D.groupOff + g];
+    const int g0 = __builtin_amdgcn_readfirstlane(gi.x), ncl = __builtin_amdgcn_readfirstlane(gi.y);
+    if (t >= ncl) {
+        if (!BWD && lane == 0) *eOut = 0.0;
+        return;
+    }
+    __shared__ uint32_t sSel[W][64];              // kept entries of the current batch (j-cluster)
+    __shared__ float4 sQp[W][kQueueCap];          // queued atom: position - r0, j-slot (bits of .w)
+    __shared__ double2 sQa[W][kQueueCap];         // (sigma/2, 2 sqrt(eps))
+    __shared__ double sQq[W][kQueueCap];          // q sqrt(k_e)
+    constexpr int kJFlush = 8;
+    __shared__ float sJF[W][kJFlush][kClusterSize * 3];
+    __shared__ int sJS[W][kJFlush][kClusterSize];
+
+    const int jj = lane >> 3, ii = lane & 7;
+    const int I = g0 + t;                         // this wave's i-cluster (in the set)
+    const float4 bc = P.bbc[D.clusterOff + I], bh = P.bbh[D.clusterOff + I];
+    // reference point: the box centre rounded to the 2^-20 grid (exact for |x| < 16)
+    const float r0x = rintf(bc.x * 1048576.0f) * (1.0f / 1048576.0f);
+    const float r0y = rintf(bc.y * 1048576.0f) * (1.0f / 1048576.0f);
+    const float r0z = rintf(bc.z * 1048576.0f) * (1.0f / 1048576.0f);
+    const float rcl = sqrtf(D.rcList2);
+    const float Lx = D.L[0], Ly = D.L[1], Lz = D.L[2];
+    const float iLx = D.invL[0], iLy = D.invL[1], iLz = D.invL[2];
+    // dimensions where the nearest image to r0 may differ from the pairwise one
+    const bool imgX = rcl + bh.x + 1e-5f >= 0.5f * Lx, imgY = rcl + bh.y + 1e-5f >= 0.5f * Ly;
+    const bool imgZ = rcl + bh.z + 1e-5f >= 0.5f * Lz;
+    const bool pairImg = imgX || imgY || imgZ;
+    const float hx = imgX ? 1e30f : bh.x, hy = imgY ? 1e30f : bh.y, hz = imgZ ? 1e30f : bh.z;
+
+    // this lane's i-atom (ii), relative to r0; a dummy i-slot is moved far away
+    const int islot = D.slotOff + I * kClusterSize + ii;
+    float4 pi = P.spos[islot];
+    const double4 ip = P.sprm64[islot];
+    const bool iReal = P.sidx[islot] >= 0;
+    const float rix = iReal ? pi.x - r0x : 1.0e6f, riy = iReal ? pi.y - r0y : 1.0e6f, riz = iReal ? pi.z - r0z : 1.0e6f;
+    const double rc2 = D.rc2, krf64 = D.krf64, crf64 = D.crf64;
+    const float fixMax = D.fixMax;
+    const double wgt = BWD ? wset[c] : 0.0;
+
+    float fix = 0.f, fiy = 0.f, fiz = 0.f;
+    double eE = 0.0;
+    uint32_t npair = 0;
+    int qn = 0;      // queued atoms (wave-uniform)
+    int nflush = 0;  // pair steps staged for the j-force flush
+
+    // staged per-atom sums of the last kJFlush steps: forces (fixed point) or
+    // the three parameter derivatives (FP64, weighted)
+    auto flush_j = [&](int cnt) __attribute__((always_inline)) {
+        wave_sync();
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            const int v = r * 64 + lane;
+            const int pe = v / (kClusterSize * 3), rem = v - pe * (kClusterSize * 3);
+            if (pe < cnt) {
+                const int slot = sJS[wave][pe][rem / 3], k = rem % 3;
+                const float f = sJF[wave][pe][rem];
+                if (BWD) {
+                    double* acc = k == 0 ? P.gq : (k == 1 ? P.gsig : P.geps);
+                    if (f != 0.f) atomicAdd(acc + slot, wgt * (double)f);
+                } else {
+                    flush_component_u(P, c, fixMax, slot, k, f);
+                }
+            }
+        }
+        wave_sync();
+    };
+
+    // one 8 x 8 step on queued atoms [b, b + nj)
+    auto step = [&](int b, int nj) __attribute__((always_inline)) {
+        const float4 rj = sQp[wave][b + jj];
+        const double2 aj = sQa[wave][b + jj];
+        const double qj = sQq[wave][b + jj];
+        const int jslot = __float_as_int(rj.w);
+        float dx = rix - rj.x, dy = riy - rj.y, dz = riz - rj.z;
+        if (pairImg) {
+            dx = min_image(dx, Lx, iLx);
+            dy = min_image(dy, Ly, iLy);
+            dz = min_image(dz, Lz, iLz);
+        }
+        // exact: the squares of 2^-20-grid differences fit in 53 bits
+        const double r2 = fma((double)dx, (double)dx, fma((double)dy, (double)dy, (double)dz * (double)dz));
+        const bool in = iReal && jj < nj && jslot > islot && r2 < rc2;
+        npair += (uint32_t)__popcll(__builtin_amdgcn_ballot_w64(in));
+        float fjx = 0.f, fjy = 0.f, fjz = 0.f;
+        if (in) {
+            if (BWD) {
+                double y = __builtin_amdgcn_rsq(r2);
+                {
+                    const double tt = r2 * y;
+                    const double ee = fma(-tt, y, 1.0);
+                    y = fma(0.5 * y, ee, y);
+                }
+                const double u = (ip.x + aj.x) * (y * y);
+                const double s2 = (ip.x + aj.x) * u;
+                const double s4 = s2 * s2, s6 = s4 * s2;
+                const double lj = fma(s6, s6, -s6);
+                const double phi = fma(krf64, r2, y) - crf64;
+                const float dh = (float)(ip.y * aj.y * fma(12.0, s6, -6.0) * s4 * u);
+                // i-side sums in fix/fiy/fiz, j-side in fjx/fjy/fjz: (q', h, e')
+                fix += (float)(qj * phi); fjx = (float)(ip.z * phi);
+                fiy += dh; fjy = dh;
+                fiz += (float)(aj.y * lj); fjz = (float)(ip.y * lj);
+            } else {
+                const PairEF64 pe = pair_ef64(true, r2, ip.x + aj.x, ip.y * aj.y, ip.z * qj, krf64, crf64);
+                eE += pe.e;
+                if (FORCES) {
+                    const float fs = (float)pe.fs;
+                    const float fx = fs * dx, fy = fs * dy, fz = fs * dz;
+                    fix += fx; fiy += fy; fiz += fz;
+                    fjx = -fx; fjy = -fy; fjz = -fz;
+                }
+            }
+        }
+        if (FORCES || BWD) {
+            fjx = oct_sum(fjx); fjy = oct_sum(fjy); fjz = oct_sum(fjz);
+            const float fk = ii == 0 ? fjx : (ii == 1 ? fjy : fjz);
+            if (ii < 3) sJF[wave][nflush][jj * 3 + ii] = jj < nj ? fk : 0.f;
+            if (ii == 0) sJS[wave][nflush][jj] = jj < nj ? jslot : islot - ii;
+            if (++nflush == kJFlush) {
+                flush_j(kJFlush);
+                nflush = 0;
+            }
+        }
+    };
+
+    const int ne = __builtin_amdgcn_readfirstlane(P.gcount[D.groupOff + g]);
+    const uint32_t* E = P.entries + D.entryOff + (long long)g * D.entryCap;
+    const uint64_t below = lanemask_lt();
+    const int k = lane >> 3;
+    for (int e0 = 0; e0 < ne; e0 += 64) {
+        const uint32_t ent = (e0 + lane < ne) ? E[e0 + lane] : 0u;
+        const uint64_t keepE = __builtin_amdgcn_ballot_w64((ent >> (20 + t)) & 1u);
+        if (keepE & (1ull << lane)) sSel[wave][__popcll(keepE & below)] = ent & 0xFFFFFu;
+        const int nsel = __popcll(keepE);
+        wave_sync();
+        for (int r0 = 0; r0 < nsel; r0 += 8) {
+            // test: atom (lane & 7) of kept entry r0 + k
+            const bool ek = r0 + k < nsel;
+            const int J = (int)sSel[wave][ek ? r0 + k : 0];
+            const int js = D.slotOff + J * kClusterSize + (lane & 7);
+            const float4 pj = P.spos[js];
+            const double4 pa = P.sprm64[js];
+            float ax = pj.x - r0x, ay = pj.y - r0y, az = pj.z - r0z;
+            ax = fmaf(-Lx, rintf(ax * iLx), ax);
+            ay = fmaf(-Ly, rintf(ay * iLy), ay);
+            az = fmaf(-Lz, rintf(az * iLz), az);
+            const float gx = fmaxf(fabsf(ax) - hx, 0.f), gy = fmaxf(fabsf(ay) - hy, 0.f), gz = fmaxf(fabsf(az) - hz, 0.f);
+            // dummy slots sit at x >= 1e6 (k_gather_bbox)
+            const bool keep = ek && pj.x < 1.0e5f && fmaf(gx, gx, fmaf(gy, gy, gz * gz)) < D.rcList2;
+            const uint64_t km = __builtin_amdgcn_ballot_w64(keep);
+            if (keep) {
+                const int q = qn + __popcll(km & below);
+                sQp[wave][q] = make_float4(ax, ay, az, __int_as_float(js));
+                sQa[wave][q] = make_double2(pa.x, pa.y);
+                sQq[wave][q] = pa.z;
+            }
+            qn += __popcll(km);
+            wave_sync();
+            int b = 0;
+            for (; b + 8 <= qn; b += 8) step(b, 8);
+            // move the < 8 atoms left over to the front (sources >= 8 > destinations)
+            const int left = qn - b;
+            if (b > 0 && left > 0) {
+                float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
+                double2 a = make_double2(0.0, 0.0);
+                double qq = 0.0;
+                if (lane < left) { p = sQp[wave][b + lane]; a = sQa[wave][b + lane]; qq = sQq[wave][b + lane]; }
+                wave_sync();
+                if (lane < left) { sQp[wave][lane] = p; sQa[wave][lane] = a; sQq[wave][lane] = qq; }
+            }
+            qn = left;
+            wave_sync();  // the steps' queue reads come before the next test round's writes
+        }
+        wave_sync();  // sSel is rewritten by the next batch
+    }
+    if (qn > 0) step(0, qn);
+    if (FORCES || BWD) {
+        if (nflush > 0) flush_j(nflush);
+        fix = stride8_sum(fix); fiy = stride8_sum(fiy); fiz = stride8_sum(fiz);
+        if (jj == 0 && iReal) {
+            if (BWD) {
+                if (fix != 0.f) atomicAdd(P.gq + islot, wgt * (double)fix);
+                if (fiy != 0.f) atomicAdd(P.gsig + islot, wgt * (double)fiy);
+                if (fiz != 0.f) atomicAdd(P.geps + islot, wgt * (double)fiz);
+            } else {
+                flush_force(P, c, fixMax, islot, fix, fiy, fiz);
+            }
+        }
+    }
+    if (BWD) return;
+    eE = wave_sum_d(eE);
+    if (lane == 0) {
+        *eOut = eE;
+        atomicAdd(&P.rt[c].npairs, (unsigned long long)npair);
+    }
+}
+
 // ---------------------------------------------------------------- ATM combine
 
-// One block of 256 threads per system: fixed-order FP64 sums of the group
-// energies of both states, then the scalar ATM combine.
-__global__ void __launch_bounds__(256) k_energy(DevPtrs P, int sysBegin) {
+// One block of 256 threads per system: fixed-order FP64 sums of the energy
+// slots of both states (estride: pair_energy_stride), then the scalar ATM combine.
+__global__ void __launch_bounds__(256) k_energy(DevPtrs P, int sysBegin, int estride) {
     const int s = sysBegin + blockIdx.x;
     __shared__ double red[256];
     double U[2];
@@ -1406,7 +1684,7 @@ __global__ void __launch_bounds__(256) k_energy(DevPtrs P, int sysBegin) {
         const SetDesc& D = P.desc[c];
         const int ng = P.rt[c].ngroups;
         double acc = 0.0;
-        for (int g = threadIdx.x; g < ng; g += 256) acc += P.genergy[D.groupOff + g];
+        for (int g = threadIdx.x; g < ng; g += 256) acc += group_energy(P, D, g, estride);
         red[threadIdx.x] = acc;
         __syncthreads();
         for (int o = 128; o > 0; o >>= 1) {
@@ -1424,12 +1702,12 @@ __global__ void __launch_bounds__(256) k_energy(DevPtrs P, int sysBegin) {
 }
 
 // Fixed-order energy of one set into P.setEnergy[c] (reference-structured path).
-__global__ void __launch_bounds__(256) k_set_energy(DevPtrs P, int c) {
+__global__ void __launch_bounds__(256) k_set_energy(DevPtrs P, int c, int estride) {
     __shared__ double red[256];
     const SetDesc& D = P.desc[c];
     const int ng = P.rt[c].ngroups;
     double acc = 0.0;
-    for (int g = threadIdx.x; g < ng; g += 256) acc += P.genergy[D.groupOff + g];
+    for (int g = threadIdx.x; g < ng; g += 256) acc += group_energy(P, D, g, estride);
     red[threadIdx.x] = acc;
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {
@@ -1490,11 +1768,12 @@ __global__ void k_fill_i64(long long* p, long long n) {
 
 static inline unsigned cdiv(long long a, long long b) { return (unsigned)((a + b - 1) / b); }
 
-hipError_t launch_zero_counts(const DevPtrs& P, int cellBegin, int cellEnd, int setBegin, int setEnd, hipStream_t st) {
-    long long n = cellEnd - cellBegin;
+hipError_t launch_zero_counts(const DevPtrs& P, int cellBegin, int cellEnd, int setBegin, int setEnd, double* f,
+                              long long nf, hipStream_t st) {
+    long long n = std::max<long long>(cellEnd - cellBegin, nf);
     unsigned blocks = (unsigned)std::min<long long>(std::max<long long>(cdiv(n, 256), cdiv(setEnd - setBegin, 256)), 4096);
     if (blocks == 0) blocks = 1;
-    k_zero_counts<<<blocks, 256, 0, st>>>(P, cellBegin, cellEnd, setBegin, setEnd);
+    k_zero_counts<<<blocks, 256, 0, st>>>(P, cellBegin, cellEnd, setBegin, setEnd, f, nf);
     return hipGetLastError();
 }
 
@@ -1504,7 +1783,6 @@ hipError_t launch_build(const DevPtrs& P, int setBegin, int nsets, int maxAtoms,
     k_bin<<<ga, 256, 0, st>>>(P, setBegin);
     k_scan<<<nsets, 1024, 0, st>>>(P, setBegin);
     k_scatter_idx<<<ga, 256, 0, st>>>(P, setBegin);
-    k_cellsort_idx<<<dim3(cdiv(maxCells, 256), nsets), 256, 0, st>>>(P, setBegin);
     k_gather_bbox<<<dim3(cdiv(maxSlots, 256), nsets), 256, 0, st>>>(P, setBegin);
     return hipGetLastError();
 }
@@ -1525,6 +1803,18 @@ hipError_t launch_alist(const DevPtrs& P, int setBegin, int nsets, int maxGroups
 
 hipError_t launch_pairs(const DevPtrs& P, int setBegin, int nsets, int maxGroups, int waves, bool forces, bool fp64,
                         int variant, hipStream_t st) {
+    if (fp64 && variant == kPairKernelFiltered) {
+#define ATM_PAIRSF(WV)                                                                                          \
+    if (forces) k_pairs_f<WV, kFiltForces><<<dim3(cdiv(8ll * maxGroups, WV), nsets), 64 * WV, 0, st>>>(P, setBegin, nullptr); \
+    else k_pairs_f<WV, kFiltEnergy><<<dim3(cdiv(8ll * maxGroups, WV), nsets), 64 * WV, 0, st>>>(P, setBegin, nullptr);
+        switch (waves) {
+            case 1: ATM_PAIRSF(1) break;
+            case 2: ATM_PAIRSF(2) break;
+            default: ATM_PAIRSF(4) break;
+        }
+#undef ATM_PAIRSF
+        return hipGetLastError();
+    }
     if (fp64 && variant == kPairKernelAtomLists) {
 #define ATM_PAIRSA(WV)                                                                                      \
     if (forces) k_pairs_a<WV, true><<<dim3(cdiv(maxGroups, WV), nsets), 64 * WV, 0, st>>>(P, setBegin);    \
@@ -1573,13 +1863,19 @@ hipError_t launch_pairs(const DevPtrs& P, int setBegin, int nsets, int maxGroups
     return hipGetLastError();
 }
 
-hipError_t launch_energy(const DevPtrs& P, int sysBegin, int nsys, hipStream_t st) {
-    k_energy<<<nsys, 256, 0, st>>>(P, sysBegin);
+hipError_t launch_energy(const DevPtrs& P, int sysBegin, int nsys, int estride, hipStream_t st) {
+    k_energy<<<nsys, 256, 0, st>>>(P, sysBegin, estride);
     return hipGetLastError();
 }
 
-hipError_t launch_set_energy(const DevPtrs& P, int c, hipStream_t st) {
-    k_set_energy<<<1, 256, 0, st>>>(P, c);
+hipError_t launch_set_energy(const DevPtrs& P, int c, int estride, hipStream_t st) {
+    k_set_energy<<<1, 256, 0, st>>>(P, c, estride);
+    return hipGetLastError();
+}
+
+hipError_t launch_backward_filtered(const DevPtrs& P, int setBegin, int nsets, int maxGroups, const double* wset,
+                                    hipStream_t st) {
+    k_pairs_f<2, kFiltBackward><<<dim3(cdiv(8ll * maxGroups, 2), nsets), 128, 0, st>>>(P, setBegin, wset);
     return hipGetLastError();
 }
 

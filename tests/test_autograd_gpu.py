@@ -37,7 +37,8 @@ def check_against_oracle(systems, grads, rtol):
         off += n
 
 
-def test_small_batch_gradients(gpu):
+@pytest.mark.parametrize("device", ["cpu", "cuda"])
+def test_small_batch_gradients(gpu, device):
     systems = [W.tiny_system(1200, 3.0, 0.9, 10, (0.9, 0.8, 0.7), 40 + k,
                              W.ATMParams(0.1 + 0.1 * k, 0.3 + 0.05 * k, 0.1, 110.0, 0.5, 200.0, 100.0, 0.0625,
                                          1.0 if k % 2 == 0 else -1.0))
@@ -45,25 +46,27 @@ def test_small_batch_gradients(gpu):
     # reverse-direction members use a sparse box so the reference's softplus stays finite
     systems[1] = W.tiny_system(300, 3.4, 0.9, 8, (1.1, 0.7, 0.45), 41, systems[1].params)
     with ATMBatch(systems) as b:
-        q, sg, ep, sc = tensors(systems)
+        q, sg, ep, sc = tensors(systems, device)
         Wt = ATMEnergyFunction.apply(b, q, sg, ep, *sc)
-        gw = torch.tensor([1.0, -0.5, 2.0], dtype=torch.float64)
+        assert Wt.device.type == device
+        gw = torch.tensor([1.0, -0.5, 2.0], dtype=torch.float64, device=device)
         (Wt * gw).sum().backward()
         for k, s in enumerate(systems):
             assert Wt[k].item() == pytest.approx(oracle.atm_eval(s, forces=False)[0]["W"], rel=1e-9)
-        grads = (q.grad.numpy() / np.repeat(gw.numpy(), [s.num_atoms for s in systems]),
-                 sg.grad.numpy() / np.repeat(gw.numpy(), [s.num_atoms for s in systems]),
-                 ep.grad.numpy() / np.repeat(gw.numpy(), [s.num_atoms for s in systems]),
-                 [t.grad.numpy() / gw.numpy() for t in sc])
+        gwn = gw.cpu().numpy()
+        rep = np.repeat(gwn, [s.num_atoms for s in systems])
+        grads = (q.grad.cpu().numpy() / rep, sg.grad.cpu().numpy() / rep, ep.grad.cpu().numpy() / rep,
+                 [t.grad.cpu().numpy() / gwn for t in sc])
         check_against_oracle(systems, grads, rtol=2e-5)
 
 
 def test_config5_replica_batch_gradients(gpu):
-    """22 replicas of the 23k-atom system: oracle gradient on three replicas,
-    central differences of the GPU energy on chosen atoms and all scalars."""
+    """22 replicas of the 23k-atom system, parameters as device tensors: oracle
+    gradient on three replicas, central differences of the GPU energy on chosen
+    atoms and all scalars."""
     reps = W.replica_batch(22)
     with ATMBatch(reps) as b:
-        q, sg, ep, sc = tensors(reps)
+        q, sg, ep, sc = tensors(reps, "cuda")
         Wt = ATMEnergyFunction.apply(b, q, sg, ep, *sc)
         Wt.sum().backward()
         sel = [0, 7, 21]
@@ -73,7 +76,7 @@ def test_config5_replica_batch_gradients(gpu):
             ref = oracle.atm_grad(s)
             for name, g, r in (("dq", q.grad, ref["dq"]), ("dsigma", sg.grad, ref["dsigma"]),
                                ("deps", ep.grad, ref["deps"])):
-                got = g[off[k]:off[k + 1]].numpy()
+                got = g[off[k]:off[k + 1]].cpu().numpy()
                 err = np.abs(got - r).max() / np.abs(r).max()
                 print(f"replica {k} {name}: {err:.2e}", flush=True)
                 assert err < 2e-5
@@ -97,7 +100,8 @@ def test_config5_replica_batch_gradients(gpu):
         del base_q
 
 
-def test_zero_epsilon_and_sigma_sites(gpu):
+@pytest.mark.parametrize("device", ["cpu", "cuda"])
+def test_zero_epsilon_and_sigma_sites(gpu, device):
     """Force fields have sites with eps = 0 (TIP3P hydrogens) and sigma = 0.
     dW/dsigma at sigma_i + sigma_j = 0 is 0 (the limit of 4 eps (12 s^11/r^12 -
     6 s^5/r^6)); dW/deps of an eps = 0 atom does not exist and is returned as 0,
@@ -112,17 +116,18 @@ def test_zero_epsilon_and_sigma_sites(gpu):
     import dataclasses
     s = dataclasses.replace(s, epsilons=eps, sigmas=sig)
     with ATMBatch([s]) as b:
-        q, sg, ep, sc = tensors([s])
+        q, sg, ep, sc = tensors([s], device)
         Wt = ATMEnergyFunction.apply(b, q, sg, ep, *sc)
         Wt.sum().backward()
         for g in (q.grad, sg.grad, ep.grad):
             assert torch.isfinite(g).all()
-        assert (ep.grad.numpy()[eps == 0.0] == 0.0).all()
-        check_against_oracle([s], (q.grad.numpy(), sg.grad.numpy(), ep.grad.numpy(),
-                                   [t.grad.numpy() for t in sc]), rtol=2e-5)
+        assert (ep.grad.cpu().numpy()[eps == 0.0] == 0.0).all()
+        check_against_oracle([s], (q.grad.cpu().numpy(), sg.grad.cpu().numpy(), ep.grad.cpu().numpy(),
+                                   [t.grad.cpu().numpy() for t in sc]), rtol=2e-5)
 
 
-def test_backward_after_re_evaluation_at_other_positions(gpu):
+@pytest.mark.parametrize("device", ["cpu", "cuda"])
+def test_backward_after_re_evaluation_at_other_positions(gpu, device):
     """forward at x1, then the batch is evaluated at x2, then the backward of
     the first forward: the gradient is the one at x1 (the backward restores
     the forward's positions before it re-runs the evaluation)."""
@@ -130,11 +135,11 @@ def test_backward_after_re_evaluation_at_other_positions(gpu):
                        W.ATMParams(0.2, 0.35, 0.1, 110.0, 0.5, 200.0, 100.0, 0.0625, 1.0))
     x2 = W.quantize(s1.positions + np.random.default_rng(5).normal(0.0, 0.05, s1.positions.shape))
     with ATMBatch([s1]) as b:
-        q, sg, ep, sc = tensors([s1])
+        q, sg, ep, sc = tensors([s1], device)
         Wt = ATMEnergyFunction.apply(b, q, sg, ep, *sc)
         b.set_positions(0, x2)
         b.compute()
         assert b.result(0)["energy"] != pytest.approx(Wt[0].item(), rel=1e-6)
         Wt.sum().backward()
-        check_against_oracle([s1], (q.grad.numpy(), sg.grad.numpy(), ep.grad.numpy(),
-                                    [t.grad.numpy() for t in sc]), rtol=2e-5)
+        check_against_oracle([s1], (q.grad.cpu().numpy(), sg.grad.cpu().numpy(), ep.grad.cpu().numpy(),
+                                    [t.grad.cpu().numpy() for t in sc]), rtol=2e-5)

@@ -245,15 +245,19 @@ def _variant_systems():
     return VARIANT_SYSTEMS
 
 
-@pytest.mark.parametrize("kernel,waves", [(0, 1), (0, 2), (0, 4), (0, 8), (1, 1), (1, 2), (1, 4), (2, 1), (2, 2),
-                                          (2, 4), (3, 1), (3, 2), (4, 1), (4, 2), (4, 4), (5, 1), (5, 2), (5, 4)])
-def test_pair_kernel_variants(gpu, kernel, waves):
+@pytest.mark.parametrize("kernel,waves,scale", [
+    (0, 1, 1.0), (0, 2, 1.0), (0, 4, 1.0), (0, 8, 1.0), (1, 1, 1.0), (1, 2, 1.0), (1, 4, 1.0), (2, 1, 1.0),
+    (2, 2, 1.0), (2, 4, 1.0), (3, 1, 1.0), (3, 2, 1.0), (4, 1, 1.0), (4, 2, 1.0), (4, 4, 1.0), (5, 1, 1.0),
+    (5, 2, 1.0), (5, 4, 1.0), (6, 1, 1.0), (6, 2, 1.0), (6, 4, 1.0), (6, 2, 0.8), (6, 2, 1.3), (6, 4, 1.5)])
+def test_pair_kernel_variants(gpu, kernel, waves, scale):
     """Every pair-kernel variant (8x8 tiles; compacted with 8 or 4 list entries
-    per batch and 1, 2 or 4 copies of the j-force sums; atom-level j lists) and
-    block shape against the oracle; energy-only evaluation agrees."""
+    per batch and 1, 2 or 4 copies of the j-force sums; atom-level j lists;
+    filtered atom queues) and block shape against the oracle; energy-only
+    evaluation agrees.  The 64-atom system (1.7 nm box, rc 0.8 nm) takes the
+    pairwise minimum-image path of the filtered kernel."""
     systems = _variant_systems()
     with ATMBatch(systems) as b:
-        b.set_tuning(1.0, waves, 4, kernel)
+        b.set_tuning(scale, waves, 4, kernel)
         b.compute()
         counts = b.list_stats()["pairs"]
         for k, s in enumerate(systems):
@@ -281,7 +285,7 @@ def test_compacted_kernel_forces_independent_of_batching(gpu):
                     assert np.array_equal(a, r)
 
 
-@pytest.mark.parametrize("kernel", [0, 1, 2, 3, 4, 5])
+@pytest.mark.parametrize("kernel", [0, 1, 2, 3, 4, 5, 6])
 def test_clash_side_sums_cleared_between_evaluations(gpu, kernel):
     """Pair forces beyond the fixed-point range go to FP64 side sums that are
     cleared only after an evaluation used them.  The displacement alternates
@@ -308,7 +312,7 @@ def test_autotune_then_results_match_oracle(gpu):
     with ATMBatch(systems) as b:
         t = b.autotune(repeats=2)
         print(f"autotune: {t}", flush=True)
-        assert t["pair_kernel"] in (0, 1, 2) and t["pair_waves_per_block"] in (1, 2, 4)
+        assert t["pair_kernel"] in (0, 1, 6) and t["pair_waves_per_block"] in (1, 2, 4)
         assert t["zbins_per_column"] in (2, 4) and 0.5 <= t["column_scale"] <= 2.0 and t["best_ms"] > 0.0
         assert b.tuning()["pair_kernel"] == t["pair_kernel"]
         b.compute()
