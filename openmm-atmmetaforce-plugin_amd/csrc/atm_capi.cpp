@@ -65,7 +65,7 @@ struct Context {
     // device buffers
     DevBuf dDesc, dRt, dPosq, dDispl, dLjp, dPrm64, dParams, dRes, dOutF;
     DevBuf dBpos, dBcell, dBrank, dSlotOf, dCellCount, dCellStart, dColStart, dColCount;
-    DevBuf dSpos, dSprm, dSprm64, dSidx, dSslot, dSf, dSd, dBbc, dBbh, dGinfo, dGcount, dGenergy, dEntries, dSetEnergy;
+    DevBuf dSpos, dSprm, dSprm64, dSidx, dSslot, dSf, dSd, dSdMask, dBbc, dBbh, dGinfo, dGcount, dGenergy, dEntries, dSetEnergy;
     DevBuf dAlist, dAcount;
     DevBuf dGq, dGsig, dGeps, dWset, dDq, dDsig, dDeps;
     DevBuf dFlag, dPosStage;  // validation counter; FP32 staging of host positions
@@ -182,7 +182,7 @@ SetDesc make_set(const atm_system_desc& d, const Tuning& tu, int s, int state) {
     D.sys = s; D.state = state;
     D.natoms = d.num_atoms;
     const int ncols = nx * ny;
-    D.slotCap = d.num_atoms + 7 * ncols;
+    D.slotCap = (d.num_atoms + 7 * ncols + 63) & ~63;  // set slot ranges start on multiples of 64
     D.clusterCap = D.slotCap / kClusterSize;
     D.groupCap = ncols + D.clusterCap / kGroupClusters + 1;
     const long long ncand = (long long)(fullX ? nx : 2 * Rx + 1) * (fullY ? ny : 2 * Ry + 1);
@@ -278,6 +278,7 @@ void alloc_layout(Context& C) {
     C.dSslot.alloc(sizeof(int2) * C.totalSlots);
     C.dSf.alloc(sizeof(long long) * 3 * C.totalSlots);
     C.dSd.alloc(sizeof(double) * 3 * C.totalSlots);
+    C.dSdMask.alloc(sizeof(unsigned long long) * (C.totalSlots / 64 + 1));
     zero_on_stream(C, C.dSd.p, sizeof(double) * 3 * C.totalSlots);  // side sums start clear
     C.dBbc.alloc(sizeof(float4) * C.totalClusters);
     C.dBbh.alloc(sizeof(float4) * C.totalClusters);
@@ -295,6 +296,7 @@ void alloc_layout(Context& C) {
     C.dSetEnergy.alloc(sizeof(double) * S2);
     C.dGq.release(); C.dGsig.release(); C.dGeps.release();
     zero_on_stream(C, C.dRt.p, sizeof(SetRuntime) * S2);
+    zero_on_stream(C, C.dSdMask.p, C.dSdMask.bytes);
     zero_on_stream(C, C.dCellCount.p, sizeof(int) * C.totalCells);
     zero_on_stream(C, C.dGcount.p, sizeof(int) * C.totalGroups);
     to_device(C, C.dDesc.p, C.sets.data(), sizeof(SetDesc) * S2);
@@ -325,6 +327,7 @@ void alloc_layout(Context& C) {
     P.sf = C.dSf.as<long long>();
     P.sfn = C.totalSlots;
     P.sd = C.dSd.as<double>();
+    P.sdMask = C.dSdMask.as<unsigned long long>();
     P.bbc = C.dBbc.as<float4>();
     P.bbh = C.dBbh.as<float4>();
     P.ginfo = C.dGinfo.as<int4>();
@@ -366,8 +369,9 @@ void enqueue_batched(Context& C, int flags, hipEvent_t* ev = nullptr) {
     const DevPtrs& P = C.P;
     const int S2 = 2 * C.nsys;
     if (ev) HIPCHK(hipEventRecord(ev[0], st));
-    HIPCHK(launch_zero_counts(P, 0, (int)C.totalCells, 0, S2, P.outF,
-                              (flags & ATM_EVAL_CLEAR_FORCES) ? 3 * C.totalAtoms : 0, st));
+    // with forces, the blend writes the cleared force itself
+    const bool clearHere = (flags & ATM_EVAL_CLEAR_FORCES) && !(flags & ATM_EVAL_FORCES);
+    HIPCHK(launch_zero_counts(P, 0, (int)C.totalCells, 0, S2, P.outF, clearHere ? 3 * C.totalAtoms : 0, st));
     HIPCHK(launch_build(P, 0, S2, C.maxAtoms, C.maxCells, C.maxSlots, st));
     if (ev) HIPCHK(hipEventRecord(ev[1], st));
     HIPCHK(launch_pairlist(P, 0, S2, C.maxGroups, 4, st));
@@ -378,7 +382,8 @@ void enqueue_batched(Context& C, int flags, hipEvent_t* ev = nullptr) {
     if (ev) HIPCHK(hipEventRecord(ev[3], st));
     HIPCHK(launch_energy(P, 0, C.nsys, pair_energy_stride(C.tuning.pairKernel, (flags & ATM_EVAL_FP32_PAIRS) == 0), st));
     if (ev) HIPCHK(hipEventRecord(ev[4], st));
-    if (flags & ATM_EVAL_FORCES) HIPCHK(launch_blend(P, 0, C.nsys, C.maxAtoms, 0, 0.0, 0.0, st));
+    if (flags & ATM_EVAL_FORCES)
+        HIPCHK(launch_blend(P, 0, C.nsys, C.maxAtoms, 0, 0.0, 0.0, (flags & ATM_EVAL_CLEAR_FORCES) != 0, st));
     if (ev) HIPCHK(hipEventRecord(ev[5], st));
 }
 
@@ -446,7 +451,7 @@ void eval_plain(Context& C, int flags) {
         atm_result& o = C.results[s];
         o.energy = r.W; o.perturbation_energy = r.usc; o.state1_energy = r.U1; o.state2_energy = r.U2;
         o.raw_perturbation = r.u; o.softcore_derivative = r.fp; o.softplus_derivative = r.bfp; o.hybrid_weight = r.sp;
-        if (forces) HIPCHK(launch_blend(P, s, 1, D0.natoms, 1, r.sp, p.direction, st));
+        if (forces) HIPCHK(launch_blend(P, s, 1, D0.natoms, 1, r.sp, p.direction, false, st));
     }
 }
 
@@ -470,7 +475,7 @@ void eval_state1(Context& C, int flags) {
         HIPCHK(launch_set_energy(P, c, pair_energy_stride(C.tuning.pairKernel, (flags & ATM_EVAL_FP32_PAIRS) == 0), st));
         double U = 0.0;
         HIPCHK(hipMemcpyAsync(&U, P.setEnergy + c, sizeof(double), hipMemcpyDeviceToHost, st));
-        if (flags & ATM_EVAL_FORCES) HIPCHK(launch_blend(P, s, 1, D.natoms, 2, 0.0, 1.0, st));
+        if (flags & ATM_EVAL_FORCES) HIPCHK(launch_blend(P, s, 1, D.natoms, 2, 0.0, 1.0, false, st));
         HIPCHK(hipStreamSynchronize(st));
         atm_result& o = C.results[s];
         o = atm_result{};

@@ -40,6 +40,8 @@ struct DevPtrs {
                            // the atomics of consecutive slots on shared cache lines)
     double* sd;            // FP64 side sums, (x, y, z) per slot, of partial forces beyond the
                            // fixed-point range; read and cleared only when rt[c].big is set
+    unsigned long long* sdMask;  // bit per slot: its side sums are nonzero (set slot ranges
+                                 // start on multiples of 64)
     // clusters
     float4* bbc;
     float4* bbh;
@@ -83,8 +85,9 @@ hipError_t launch_set_energy(const DevPtrs& P, int c, int estride, hipStream_t s
 // of every in-cutoff pair of sets [setBegin, setBegin + nsets) to gq, gsig, geps (slot order).
 hipError_t launch_backward_filtered(const DevPtrs& P, int setBegin, int nsets, int maxGroups, const double* wset,
                                     hipStream_t st);
+// clear: write the blended force instead of adding it to outF.
 hipError_t launch_blend(const DevPtrs& P, int sysBegin, int nsys, int maxAtoms, int hostMode, double sp, double dir,
-                        hipStream_t st);
+                        bool clear, hipStream_t st);
 hipError_t launch_fill_i64(long long* p, long long n, hipStream_t st);
 hipError_t launch_fill_zero(void* p, long long n8, hipStream_t st);
 hipError_t launch_pairs_dump(const DevPtrs& P, int c, int maxGroups, int2* out, long long cap,

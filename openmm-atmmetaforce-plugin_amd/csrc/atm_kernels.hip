@@ -65,6 +65,12 @@ __device__ __forceinline__ void atomic_add_fixed(long long* p, long long v) {
     atomicAdd(reinterpret_cast<unsigned long long*>(p), static_cast<unsigned long long>(v));
 }
 
+// Mark a slot's FP64 side sums as nonzero (the blend reads only marked slots).
+__device__ __forceinline__ void mark_side(const DevPtrs& P, int c, long long slot) {
+    atomicOr(P.sdMask + (slot >> 6), 1ull << (slot & 63));
+    P.rt[c].big = 1;
+}
+
 // Add a partial force to a slot: 64-bit fixed point below the set's fixMax
 // (<= 2^27 kJ/mol/nm, so the magic-number conversion is exact), FP64 side
 // accumulator at or above it (clashing pairs of the displaced state).
@@ -75,7 +81,7 @@ __device__ __forceinline__ void add_force(const DevPtrs& P, int c, float fixMax,
     if (fabsf(fx) < fixMax) atomic_add_fixed(f, to_fixed(fx)); else { atomicAdd(d, (double)fx); big = true; }
     if (fabsf(fy) < fixMax) atomic_add_fixed(f + P.sfn, to_fixed(fy)); else { atomicAdd(d + 1, (double)fy); big = true; }
     if (fabsf(fz) < fixMax) atomic_add_fixed(f + 2 * P.sfn, to_fixed(fz)); else { atomicAdd(d + 2, (double)fz); big = true; }
-    if (big) P.rt[c].big = 1;
+    if (big) mark_side(P, c, slot);
 }
 
 // DPP-fused sum over each aligned group of 8 lanes (xor 1, xor 2, half-row mirror).
@@ -424,11 +430,17 @@ __global__ void __launch_bounds__(256) k_gather_bbox(DevPtrs P, int setBegin) {
     const int c = setBegin + blockIdx.y;
     const SetDesc& D = P.desc[c];
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    // FP64 side sums: cleared over the whole slot range after an evaluation
-    // that used them (k_pairlist then resets the flag)
+    // FP64 side sums: after an evaluation that used them (k_pairlist then resets
+    // the flag), the marked slots are cleared and so is their mask word (the 64
+    // slots of a word are one wave's: set slot ranges start on multiples of 64)
     if (P.rt[c].big && s < D.slotCap) {
-        double* d = P.sd + 3ll * (D.slotOff + s);
-        d[0] = 0.0; d[1] = 0.0; d[2] = 0.0;
+        const long long k = D.slotOff + s;
+        const unsigned long long m = P.sdMask[k >> 6];
+        if ((m >> (k & 63)) & 1ull) {
+            double* d = P.sd + 3 * k;
+            d[0] = 0.0; d[1] = 0.0; d[2] = 0.0;
+        }
+        if ((k & 63) == 0 && m != 0ull) P.sdMask[k >> 6] = 0ull;
     }
     if (s >= P.rt[c].nslots) return;
     const int k = D.slotOff + s;
@@ -691,7 +703,7 @@ __device__ __forceinline__ void flush_component_u(const DevPtrs& P, int c, float
     if (v != 0) atomic_add_fixed(dst, v);
     if (!small) {
         atomicAdd(P.sd + 3ll * slot + k, (double)f);
-        P.rt[c].big = 1;
+        mark_side(P, c, slot);
     }
 }
 
@@ -1238,7 +1250,8 @@ __global__ void __launch_bounds__(64 * W) k_pairs_q(DevPtrs P, int setBegin) {
                 double* dj = P.sd + 3ll * sJSlot[wave][j];
                 atomicAdd(di, fx); atomicAdd(di + 1, fy); atomicAdd(di + 2, fz);
                 atomicAdd(dj, -fx); atomicAdd(dj + 1, -fy); atomicAdd(dj + 2, -fz);
-                P.rt[c].big = 1;
+                mark_side(P, c, sBase + i);
+                mark_side(P, c, sJSlot[wave][j]);
             }
         }
         qh += n;
@@ -1462,8 +1475,11 @@ constexpr int kQueueCap = 72;  // 7 left over from the last step + one test roun
 // to FP64 per-slot accumulators (gq, gsig, geps).
 constexpr int kFiltEnergy = 0, kFiltForces = 1, kFiltBackward = 2;
 
+#ifndef ATM_FILT_MINW
+#define ATM_FILT_MINW 1
+#endif
 template <int W, int MODE>
-__global__ void __launch_bounds__(64 * W) k_pairs_f(DevPtrs P, int setBegin, const double* wset) {
+__global__ void __launch_bounds__(64 * W, ATM_FILT_MINW) k_pairs_f(DevPtrs P, int setBegin, const double* wset) {
     constexpr bool FORCES = MODE == kFiltForces;
     constexpr bool BWD = MODE == kFiltBackward;
     const int c = setBegin + blockIdx.y;
@@ -1609,13 +1625,24 @@ __global__ void __launch_bounds__(64 * W) k_pairs_f(DevPtrs P, int setBegin, con
         if (keepE & (1ull << lane)) sSel[wave][__popcll(keepE & below)] = ent & 0xFFFFFu;
         const int nsel = __popcll(keepE);
         wave_sync();
+        // the test round's j atoms: atom (lane & 7) of kept entry r0 + k; the next
+        // round's loads are issued before this round's pair steps run
+        auto load_round = [&](int r0, int& js, float4& pj, double4& pa) __attribute__((always_inline)) {
+            const int J = (int)sSel[wave][r0 + k < nsel ? r0 + k : 0];
+            js = D.slotOff + J * kClusterSize + (lane & 7);
+            pj = P.spos[js];
+            pa = P.sprm64[js];
+        };
+        int jsN;
+        float4 pjN;
+        double4 paN;
+        if (nsel > 0) load_round(0, jsN, pjN, paN);
         for (int r0 = 0; r0 < nsel; r0 += 8) {
-            // test: atom (lane & 7) of kept entry r0 + k
             const bool ek = r0 + k < nsel;
-            const int J = (int)sSel[wave][ek ? r0 + k : 0];
-            const int js = D.slotOff + J * kClusterSize + (lane & 7);
-            const float4 pj = P.spos[js];
-            const double4 pa = P.sprm64[js];
+            const int js = jsN;
+            const float4 pj = pjN;
+            const double4 pa = paN;
+            if (r0 + 8 < nsel) load_round(r0 + 8, jsN, pjN, paN);
             float ax = pj.x - r0x, ay = pj.y - r0y, az = pj.z - r0z;
             ax = fmaf(-Lx, rintf(ax * iLx), ax);
             ay = fmaf(-Ly, rintf(ay * iLy), ay);
@@ -1718,11 +1745,13 @@ __global__ void __launch_bounds__(256) k_set_energy(DevPtrs P, int c, int estrid
 }
 
 // F_sys += w1 F1 + w2 F2 with (w1, w2) = (1-sp, sp) for direction > 0 and
-// (sp, 1-sp) otherwise (ReferenceATMMetaForceKernels.cpp:100-110).  hostMode 2
+// (sp, 1-sp) otherwise (ReferenceATMMetaForceKernels.cpp:100-110); with
+// `clear`, F_sys = w1 F1 + w2 F2 (no read of the old force).  hostMode 2
 // adds F1 alone (state-1-only evaluation).  In host mode 1
 // sp and direction come from kernel arguments (the reference's HybridForce
 // receives sp as a scalar argument, CommonATMMetaForceKernels.cpp:199-201).
-__global__ void k_blend(DevPtrs P, int sysBegin, int hostMode, double spHost, double dirHost) {
+// The FP64 side sums of a slot are read only where its mask bit is set.
+__global__ void k_blend(DevPtrs P, int sysBegin, int hostMode, double spHost, double dirHost, int clear) {
     const int s = sysBegin + blockIdx.y;
     const SetDesc& D0 = P.desc[2 * s];
     const SetDesc& D1 = P.desc[2 * s + 1];
@@ -1738,26 +1767,27 @@ __global__ void k_blend(DevPtrs P, int sysBegin, int hostMode, double spHost, do
     const double inv = 1.0 / kForceScale;
     const long long* fa = P.sf + a;
     double f1x = (double)fa[0] * inv, f1y = (double)fa[P.sfn] * inv, f1z = (double)fa[2 * P.sfn] * inv;
-    if (P.rt[2 * s].big) {
+    if (P.rt[2 * s].big && ((P.sdMask[a >> 6] >> (a & 63)) & 1ull)) {
         const double* d = P.sd + 3 * a;
         f1x += d[0]; f1y += d[1]; f1z += d[2];
     }
+    const double ox = clear ? 0.0 : F[i], oy = clear ? 0.0 : F[N + i], oz = clear ? 0.0 : F[2 * N + i];
     if (hostMode == 2) {  // state 1 only: F += F1 (the state-2 set was not evaluated)
-        F[i] += f1x;
-        F[N + i] += f1y;
-        F[2 * N + i] += f1z;
+        F[i] = ox + f1x;
+        F[N + i] = oy + f1y;
+        F[2 * N + i] = oz + f1z;
         return;
     }
     const long long b = D1.slotOff + P.slotOf[D1.setAtomOff + i];
     const long long* fb = P.sf + b;
     double f2x = (double)fb[0] * inv, f2y = (double)fb[P.sfn] * inv, f2z = (double)fb[2 * P.sfn] * inv;
-    if (P.rt[2 * s + 1].big) {
+    if (P.rt[2 * s + 1].big && ((P.sdMask[b >> 6] >> (b & 63)) & 1ull)) {
         const double* d = P.sd + 3 * b;
         f2x += d[0]; f2y += d[1]; f2z += d[2];
     }
-    F[i] += w1 * f1x + w2 * f2x;
-    F[N + i] += w1 * f1y + w2 * f2y;
-    F[2 * N + i] += w1 * f1z + w2 * f2z;
+    F[i] = ox + (w1 * f1x + w2 * f2x);
+    F[N + i] = oy + (w1 * f1y + w2 * f2y);
+    F[2 * N + i] = oz + (w1 * f1z + w2 * f2z);
 }
 
 __global__ void k_fill_i64(long long* p, long long n) {
@@ -1880,8 +1910,8 @@ hipError_t launch_backward_filtered(const DevPtrs& P, int setBegin, int nsets, i
 }
 
 hipError_t launch_blend(const DevPtrs& P, int sysBegin, int nsys, int maxAtoms, int hostMode, double sp, double dir,
-                        hipStream_t st) {
-    k_blend<<<dim3(cdiv(maxAtoms, 256), nsys), 256, 0, st>>>(P, sysBegin, hostMode, sp, dir);
+                        bool clear, hipStream_t st) {
+    k_blend<<<dim3(cdiv(maxAtoms, 256), nsys), 256, 0, st>>>(P, sysBegin, hostMode, sp, dir, clear ? 1 : 0);
     return hipGetLastError();
 }
 
