@@ -265,6 +265,45 @@ int atm_get_stats(void* context, int64_t* pairs_in_cutoff, int64_t* pairs_evalua
 int atm_get_list_stats(void* context, int64_t* pairs, int32_t* max_entries, int32_t* capacity);
 
 /* ------------------------------------------------------------------------
+ * No-allocation evaluation.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).
+ *
+ * One ATM evaluation of one system (both coordinate sets, the combine and,
+ * with ATM_EVAL_FORCES, the hybrid force) from caller-owned device arrays,
+ * enqueued on `hip_stream` (hipStream_t; NULL = the null stream) on the
+ * current device.  It calls no allocating routine: every intermediate buffer
+ * lives in a static device array of ATM_NOALLOC_SCRATCH_MIB MiB that each GPU
+ * the library runs on holds until the process ends.  The kernels and launch
+ * configuration are atm_eval's defaults, so the results equal those of a
+ * one-system context (atm_create + atm_eval) on the same inputs.
+ *   positions      num_atoms x 3 doubles (nm), device
+ *   charges, sigmas, epsilons   num_atoms doubles each, device
+ *   displacements  num_atoms x 3 doubles (nm), device
+ *   forces         num_atoms x 3 doubles, device: the hybrid ATM force is
+ *                  added (written with ATM_EVAL_CLEAR_FORCES); may be NULL
+ *                  without ATM_EVAL_FORCES
+ *   result         one atm_result, device
+ *   status         one int32, device: ATM_OK, ATM_ERR_INVALID_ARGUMENT (a
+ *                  non-finite input or a negative sigma/epsilon) or
+ *                  ATM_ERR_CAPACITY (a neighbour list outgrew its capacity,
+ *                  sized from the density with 1.5x headroom); on either error
+ *                  the result fields are NaN and the forces are untouched.
+ * flags: ATM_EVAL_FORCES, ATM_EVAL_CLEAR_FORCES, ATM_EVAL_FP32_PAIRS.
+ * Limits: num_atoms <= ATM_NOALLOC_MAX_ATOMS and the system's buffers within
+ * the scratch (ATM_ERR_UNSUPPORTED otherwise, checked on the host before any
+ * launch).  Every call on one GPU shares that GPU's scratch: calls that
+ * overlap on one GPU (other streams, other threads) race; calls on one
+ * stream are ordered and each starts from cleared state.
+ * ------------------------------------------------------------------------ */
+#define ATM_NOALLOC_MAX_ATOMS 32768
+#define ATM_NOALLOC_SCRATCH_MIB 32
+int atm_eval_noalloc(const atm_system_desc* system, const atm_params* params, const double* positions,
+                     const double* charges, const double* sigmas, const double* epsilons,
+                     const double* displacements, int32_t flags, double* forces, atm_result* result,
+                     int32_t* status, void* hip_stream);
+
+/* ------------------------------------------------------------------------
  * Flat-bottom restraints evaluated beside the ATM force every step (force
  * group 0 in ATM workflows, python/tests/test_abfe.py:65-79).
  *

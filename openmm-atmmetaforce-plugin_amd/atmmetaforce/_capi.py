@@ -37,6 +37,7 @@ EXPORTED = (
     "atm_get_forces_fixed", "atm_clear_forces", "atm_get_pairs", "atm_backward", "atm_backward_device",
     "atm_set_stream",
     "atm_autotune", "atm_get_tuning", "atm_set_tuning", "atm_get_stats", "atm_get_list_stats", "atm_profile",
+    "atm_eval_noalloc",
     "atm_restraints_create", "atm_restraints_destroy", "atm_restraints_set", "atm_restraints_eval",
     "atm_md_init", "atm_md_step", "atm_md_get",
     "atm_pme_create", "atm_pme_destroy", "atm_pme_get_info", "atm_pme_set_particles", "atm_pme_set_exceptions",
@@ -149,6 +150,8 @@ def load(path: str | None = None):
                           ctypes.POINTER(ctypes.c_int64)],
         "atm_get_list_stats": [vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(i32), ctypes.POINTER(i32)],
         "atm_profile": [vp, i32, i32, ctypes.POINTER(StageTimes)],
+        "atm_eval_noalloc": [ctypes.POINTER(SystemDesc), ctypes.POINTER(Params), vp, vp, vp, vp, vp, i32, vp, vp, vp,
+                             vp],
         "atm_restraints_create": [i32, ctypes.POINTER(vp)],
         "atm_restraints_destroy": [vp],
         "atm_restraints_set": [vp, i32, i32, ctypes.POINTER(i32), dp, i32, ctypes.POINTER(CentroidRestraint), i32,

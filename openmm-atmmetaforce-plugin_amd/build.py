@@ -33,6 +33,7 @@ HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
              "-I" + CSRC, "-I" + INCLUDE]
 
 HIP_SOURCES = ["atm_kernels.hip", "atm_backward.hip", "atm_restraints.hip", "atm_md.hip", "atm_pme.hip",
+               "atm_noalloc.cpp",
                "atm_capi.cpp", "atm_pme_capi.cpp"]
 
 
