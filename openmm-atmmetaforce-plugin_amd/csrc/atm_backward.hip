@@ -4,8 +4,8 @@
 // dW/dU_state given per set (w[c] = gW (1 - sp) for the state that supplies e0,
 // gW sp for the other).  The pair terms' derivatives with respect to the
 // stored parameters (q' = q sqrt(k_e), h = sigma/2, e' = 2 sqrt(eps)) are
-// recomputed on the forward's lists by the filtered-queue pair kernel in its
-// backward mode (launch_backward_filtered, atm_kernels.hip) and summed per
+// recomputed on the forward's lists by the cluster-tile pair kernel in its
+// backward mode (launch_backward_pairs, atm_kernels.hip) and summed per
 // slot; nothing per pair is stored between the passes.  The gather kernel here
 // sums both states per atom and applies the chain rule to physical units:
 // dq = sqrt(k_e) dE/dq', dsigma = dE/dh / 2, deps = dE/de' * 2 / e' (0 for an
@@ -92,7 +92,7 @@ hipError_t launch_backward(const DevPtrs& P, long long totalSlots, int setBegin,
     k_fill_f64<<<blocks, 256, 0, st>>>(P.gq, totalSlots);
     k_fill_f64<<<blocks, 256, 0, st>>>(P.gsig, totalSlots);
     k_fill_f64<<<blocks, 256, 0, st>>>(P.geps, totalSlots);
-    return launch_backward_filtered(P, setBegin, nsets, maxGroups, wset, st);
+    return launch_backward_pairs(P, setBegin, nsets, maxGroups, wset, st);
 }
 
 hipError_t launch_backward_gather(const DevPtrs& P, int sysBegin, int nsys, int maxAtoms, double* dq, double* dsig,

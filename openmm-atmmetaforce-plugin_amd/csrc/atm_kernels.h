@@ -81,10 +81,10 @@ hipError_t launch_pairs(const DevPtrs& P, int setBegin, int nsets, int maxGroups
 inline int pair_energy_stride(int variant, bool fp64) { return (fp64 && variant == kPairKernelFiltered) ? 8 : 1; }
 hipError_t launch_energy(const DevPtrs& P, int sysBegin, int nsys, int estride, hipStream_t st);
 hipError_t launch_set_energy(const DevPtrs& P, int c, int estride, hipStream_t st);
-// Backward pass of the pair terms (k_pairs_f's filtered queues): adds wset[c] dE/d(q', h, e')
+// Backward pass of the pair terms (the tile kernel's backward mode): adds wset[c] dE/d(q', h, e')
 // of every in-cutoff pair of sets [setBegin, setBegin + nsets) to gq, gsig, geps (slot order).
-hipError_t launch_backward_filtered(const DevPtrs& P, int setBegin, int nsets, int maxGroups, const double* wset,
-                                    hipStream_t st);
+hipError_t launch_backward_pairs(const DevPtrs& P, int setBegin, int nsets, int maxGroups, const double* wset,
+                                 hipStream_t st);
 // clear: write the blended force instead of adding it to outF.
 hipError_t launch_blend(const DevPtrs& P, int sysBegin, int nsys, int maxAtoms, int hostMode, double sp, double dir,
                         bool clear, hipStream_t st);

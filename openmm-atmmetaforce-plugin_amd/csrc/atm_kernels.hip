@@ -719,8 +719,21 @@ __device__ __forceinline__ void flush_component_u(const DevPtrs& P, int c, float
 // on a 22 x 23k batch that measured 5 % faster than its unconstrained 88
 // VGPRs (5 waves) and 14 % faster than 72 VGPRs (7 waves, which spill inside
 // the entry loop).
-template <int W, bool FORCES, bool F64>
-__global__ void __launch_bounds__(64 * W, 6) k_pairs(DevPtrs P, int setBegin) {
+//
+// BWD (FP64 scalars, no forces): the backward pass on the same tiles -- per
+// in-cutoff pair, the derivatives of its energy with respect to the stored
+// per-atom parameters q' = q sqrt(k_e), h = sigma/2, e' = 2 sqrt(eps) of both
+// atoms, from the exact r^2:
+//   dE/dq'_i = q'_j phi,  phi = 1/r + k_rf r^2 - c_rf
+//   dE/dh    = e'_i e'_j (12 (s/r)^12 - 6 (s/r)^6) / s = e'_i e'_j (12 s6 - 6) s2^2 u
+//              with s = h_i + h_j, u = s / r^2, s2 = s u, s6 = s2^3 (0 at s = 0)
+//   dE/de'_i = e'_j (s6^2 - s6)
+// summed per lane in FP32 where the forward sums forces, weighted by wset[c]
+// (dW/dU of the set) and added to FP64 per-slot accumulators (gq, gsig, geps).
+template <int W, bool FORCES, bool F64, bool BWD>
+__global__ void __launch_bounds__(64 * W, 6) k_pairs(DevPtrs P, int setBegin, const double* wset) {
+    static_assert(!BWD || (F64 && !FORCES), "the backward mode is FP64 and computes no forces");
+    constexpr bool JSUMS = FORCES || BWD;  // per-atom j sums flushed every kJFlush entries
     const int c = setBegin + blockIdx.y;
     const SetDesc D = P.desc[c];  // by value: lives in SGPRs for the whole kernel
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -768,6 +781,7 @@ __global__ void __launch_bounds__(64 * W, 6) k_pairs(DevPtrs P, int setBegin) {
     const bool shiftOK = D.shiftOK;
     const double rc2 = D.rc2, krf64 = D.krf64, crf64 = D.crf64;
     const float krf = D.krf, crf = D.crf, rc2hi = D.rc2hi, fixMax = D.fixMax;
+    const double wgt = BWD ? wset[c] : 0.0;
     float fi[kGroupClusters][3];
 #pragma unroll
     for (int t = 0; t < kGroupClusters; ++t) fi[t][0] = fi[t][1] = fi[t][2] = 0.f;
@@ -811,7 +825,16 @@ __global__ void __launch_bounds__(64 * W, 6) k_pairs(DevPtrs P, int setBegin) {
         for (int r = 0; r < 3; ++r) {
             const int v = r * 64 + ln;  // entry v / 24, then jj * 3 + component
             const int pe = v / (kClusterSize * 3), rem = v - pe * (kClusterSize * 3);
-            if (pe < cnt) flush_component_u(P, c, fixMax, sJS[wave][pe] + rem / 3, rem % 3, sJF[wave][pe][rem]);
+            if (pe < cnt) {
+                const int slot = sJS[wave][pe] + rem / 3, k = rem % 3;
+                const float f = sJF[wave][pe][rem];
+                if (BWD) {
+                    double* acc = k == 0 ? P.gq : (k == 1 ? P.gsig : P.geps);
+                    if (f != 0.f) atomicAdd(acc + slot, wgt * (double)f);
+                } else {
+                    flush_component_u(P, c, fixMax, slot, k, f);
+                }
+            }
         }
         wave_sync();  // read before the next entries overwrite them
     };
@@ -855,15 +878,37 @@ __global__ void __launch_bounds__(64 * W, 6) k_pairs(DevPtrs P, int setBegin) {
                     const uint64_t okLanes = (t == tdiag ? upperLanes : ~0ull) & jvLanes & iv;
                     const uint64_t candLanes = __builtin_amdgcn_ballot_w64(r2f < rc2hi) & okLanes;
                     if (candLanes == 0ull) continue;  // no pair of this 8x8 step is inside
-                    const bool cand = __builtin_amdgcn_inverse_ballot_w64(candLanes);
                     const double4 id = sPrm64[wave][t * kClusterSize + ii];
                     // exact: the squares of 2^-20-grid differences fit in 53 bits
                     const double r2 = fma((double)dx, (double)dx, fma((double)dy, (double)dy, (double)dz * (double)dz));
-                    const bool in = cand && r2 < rc2;
-                    npair += (uint32_t)__popcll(__builtin_amdgcn_ballot_w64(in));
-                    const PairEF64 pe = pair_ef64(in, r2, id.x + jd.x, id.y * jd.y, id.z * jd.z, krf64, crf64);
-                    eE += pe.e;
-                    fs = (float)pe.fs;
+                    const uint64_t inLanes = __builtin_amdgcn_ballot_w64(r2 < rc2) & candLanes;
+                    npair += (uint32_t)__popcll(inLanes);
+                    // lanes outside the cutoff are masked off (exec), not zeroed with selects
+                    fs = 0.f;
+                    if (__builtin_amdgcn_inverse_ballot_w64(inLanes)) {
+                        if (BWD) {
+                            double y = __builtin_amdgcn_rsq(r2);
+                            {
+                                const double tt = r2 * y;
+                                const double ee = fma(-tt, y, 1.0);
+                                y = fma(0.5 * y, ee, y);
+                            }
+                            const double u = (id.x + jd.x) * (y * y);
+                            const double s2 = (id.x + jd.x) * u;
+                            const double s4 = s2 * s2, s6 = s4 * s2;
+                            const double lj = fma(s6, s6, -s6);
+                            const double phi = fma(krf64, r2, y) - crf64;
+                            const float dh = (float)(id.y * jd.y * fma(12.0, s6, -6.0) * s4 * u);
+                            // (q', h, e') sums: i side in fi[t], j side in fjx, fjy, fjz
+                            fi[t][0] += (float)(jd.z * phi); fjx += (float)(id.z * phi);
+                            fi[t][1] += dh; fjy += dh;
+                            fi[t][2] += (float)(jd.y * lj); fjz += (float)(id.y * lj);
+                        } else {
+                            const PairEF64 pe = pair_ef64(true, r2, id.x + jd.x, id.y * jd.y, id.z * jd.z, krf64, crf64);
+                            eE += pe.e;
+                            fs = (float)pe.fs;
+                        }
+                    }
                 } else {
                     uint64_t iv = (ivLanes >> (t * kClusterSize)) & 0xFFull;
                     iv |= iv << 8; iv |= iv << 16; iv |= iv << 32;
@@ -880,13 +925,12 @@ __global__ void __launch_bounds__(64 * W, 6) k_pairs(DevPtrs P, int setBegin) {
                     fs = pe.fs;
                 }
                 if (FORCES) {
-                    const float fx = fs * dx, fy = fs * dy, fz = fs * dz;
-                    fi[t][0] += fx; fi[t][1] += fy; fi[t][2] += fz;
-                    fjx -= fx; fjy -= fy; fjz -= fz;
+                    fi[t][0] = fmaf(fs, dx, fi[t][0]); fi[t][1] = fmaf(fs, dy, fi[t][1]); fi[t][2] = fmaf(fs, dz, fi[t][2]);
+                    fjx = fmaf(-fs, dx, fjx); fjy = fmaf(-fs, dy, fjy); fjz = fmaf(-fs, dz, fjz);
                 }
             }
         }
-        if (FORCES) {
+        if (JSUMS) {
             // every lane of a jj octet holds the sums; lanes ii = 0, 1, 2 keep x, y, z
             // (zero for dummy j-slots, whose accumulators nothing reads)
             fjx = oct_sum(fjx); fjy = oct_sum(fjy); fjz = oct_sum(fjz);
@@ -914,16 +958,26 @@ __global__ void __launch_bounds__(64 * W, 6) k_pairs(DevPtrs P, int setBegin) {
             do_entry(__builtin_amdgcn_readlane(ebuf, k + 1), bufB);
         }
     }
-    if (FORCES && (ecount & (kJFlush - 1)) != 0) flush_j(ecount & (kJFlush - 1));
-    if (FORCES) {
+    if (JSUMS && (ecount & (kJFlush - 1)) != 0) flush_j(ecount & (kJFlush - 1));
+    if (JSUMS) {
 #pragma unroll
         for (int t = 0; t < kGroupClusters; ++t) {
             if (t < ncl) {
                 const float fx = stride8_sum(fi[t][0]), fy = stride8_sum(fi[t][1]), fz = stride8_sum(fi[t][2]);
-                if (jj == 0 && ((ivm >> t) & 1u)) flush_force(P, c, fixMax, D.slotOff + (g0 + t) * kClusterSize + ii, fx, fy, fz);
+                const int is = D.slotOff + (g0 + t) * kClusterSize + ii;
+                if (jj == 0 && ((ivm >> t) & 1u)) {
+                    if (BWD) {
+                        if (fx != 0.f) atomicAdd(P.gq + is, wgt * (double)fx);
+                        if (fy != 0.f) atomicAdd(P.gsig + is, wgt * (double)fy);
+                        if (fz != 0.f) atomicAdd(P.geps + is, wgt * (double)fz);
+                    } else {
+                        flush_force(P, c, fixMax, is, fx, fy, fz);
+                    }
+                }
             }
         }
     }
+    if (BWD) return;
     double eacc = eE;
     eacc = wave_sum_d(eacc);
     if (lane == 0) {
@@ -1461,27 +1515,14 @@ __global__ void __launch_bounds__(64) k_pairs_dump(DevPtrs P, int c, int2* out, 
 // pairwise minimum image in their steps.
 constexpr int kQueueCap = 72;  // 7 left over from the last step + one test round of 64
 
-//
-// MODE kFiltEnergy / kFiltForces: the forward (energy; energy and forces).
-// MODE kFiltBackward: the backward pass on the same queues -- per in-cutoff
-// pair, the derivatives of its energy with respect to the stored per-atom
-// parameters q' = q sqrt(k_e), h = sigma/2, e' = 2 sqrt(eps) of both atoms,
-// in FP64 from the exact r^2:
-//   dE/dq'_i = q'_j phi,  phi = 1/r + k_rf r^2 - c_rf
-//   dE/dh    = e'_i e'_j (12 (s/r)^12 - 6 (s/r)^6) / s = e'_i e'_j (12 s6 - 6) s2^2 u
-//              with s = h_i + h_j, u = s / r^2, s2 = s u, s6 = s2^3 (0 at s = 0)
-//   dE/de'_i = e'_j (s6^2 - s6)
-// summed per lane in FP32, weighted by wset[c] (dW/dU of the set) and added
-// to FP64 per-slot accumulators (gq, gsig, geps).
-constexpr int kFiltEnergy = 0, kFiltForces = 1, kFiltBackward = 2;
+constexpr int kFiltEnergy = 0, kFiltForces = 1;
 
 #ifndef ATM_FILT_MINW
 #define ATM_FILT_MINW 1
 #endif
 template <int W, int MODE>
-__global__ void __launch_bounds__(64 * W, ATM_FILT_MINW) k_pairs_f(DevPtrs P, int setBegin, const double* wset) {
+__global__ void __launch_bounds__(64 * W, ATM_FILT_MINW) k_pairs_f(DevPtrs P, int setBegin) {
     constexpr bool FORCES = MODE == kFiltForces;
-    constexpr bool BWD = MODE == kFiltBackward;
     const int c = setBegin + blockIdx.y;
     const SetDesc D = P.desc[c];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -1492,7 +1533,7 @@ __global__ void __launch_bounds__(64 * W, ATM_FILT_MINW) k_pairs_f(DevPtrs P, in
     const int4 gi = P.ginfo[D.groupOff + g];
     const int g0 = __builtin_amdgcn_readfirstlane(gi.x), ncl = __builtin_amdgcn_readfirstlane(gi.y);
     if (t >= ncl) {
-        if (!BWD && lane == 0) *eOut = 0.0;
+        if (lane == 0) *eOut = 0.0;
         return;
     }
     __shared__ uint32_t sSel[W][64];              // kept entries of the current batch (j-cluster)
@@ -1516,7 +1557,7 @@ __global__ void __launch_bounds__(64 * W, ATM_FILT_MINW) k_pairs_f(DevPtrs P, in
     // dimensions where the nearest image to r0 may differ from the pairwise one
     const bool imgX = rcl + bh.x + 1e-5f >= 0.5f * Lx, imgY = rcl + bh.y + 1e-5f >= 0.5f * Ly;
     const bool imgZ = rcl + bh.z + 1e-5f >= 0.5f * Lz;
-    const bool pairImg = imgX || imgY || imgZ;
+    const bool pairImg = __builtin_amdgcn_readfirstlane(imgX || imgY || imgZ ? 1 : 0) != 0;  // wave-uniform
     const float hx = imgX ? 1e30f : bh.x, hy = imgY ? 1e30f : bh.y, hz = imgZ ? 1e30f : bh.z;
 
     // this lane's i-atom (ii), relative to r0; a dummy i-slot is moved far away
@@ -1527,7 +1568,6 @@ __global__ void __launch_bounds__(64 * W, ATM_FILT_MINW) k_pairs_f(DevPtrs P, in
     const float rix = iReal ? pi.x - r0x : 1.0e6f, riy = iReal ? pi.y - r0y : 1.0e6f, riz = iReal ? pi.z - r0z : 1.0e6f;
     const double rc2 = D.rc2, krf64 = D.krf64, crf64 = D.crf64;
     const float fixMax = D.fixMax;
-    const double wgt = BWD ? wset[c] : 0.0;
 
     float fix = 0.f, fiy = 0.f, fiz = 0.f;
     double eE = 0.0;
@@ -1535,24 +1575,14 @@ __global__ void __launch_bounds__(64 * W, ATM_FILT_MINW) k_pairs_f(DevPtrs P, in
     int qn = 0;      // queued atoms (wave-uniform)
     int nflush = 0;  // pair steps staged for the j-force flush
 
-    // staged per-atom sums of the last kJFlush steps: forces (fixed point) or
-    // the three parameter derivatives (FP64, weighted)
+    // j-force sums of the last kJFlush steps, flushed together
     auto flush_j = [&](int cnt) __attribute__((always_inline)) {
         wave_sync();
 #pragma unroll
         for (int r = 0; r < 3; ++r) {
             const int v = r * 64 + lane;
             const int pe = v / (kClusterSize * 3), rem = v - pe * (kClusterSize * 3);
-            if (pe < cnt) {
-                const int slot = sJS[wave][pe][rem / 3], k = rem % 3;
-                const float f = sJF[wave][pe][rem];
-                if (BWD) {
-                    double* acc = k == 0 ? P.gq : (k == 1 ? P.gsig : P.geps);
-                    if (f != 0.f) atomicAdd(acc + slot, wgt * (double)f);
-                } else {
-                    flush_component_u(P, c, fixMax, slot, k, f);
-                }
-            }
+            if (pe < cnt) flush_component_u(P, c, fixMax, sJS[wave][pe][rem / 3], rem % 3, sJF[wave][pe][rem]);
         }
         wave_sync();
     };
@@ -1575,35 +1605,15 @@ __global__ void __launch_bounds__(64 * W, ATM_FILT_MINW) k_pairs_f(DevPtrs P, in
         npair += (uint32_t)__popcll(__builtin_amdgcn_ballot_w64(in));
         float fjx = 0.f, fjy = 0.f, fjz = 0.f;
         if (in) {
-            if (BWD) {
-                double y = __builtin_amdgcn_rsq(r2);
-                {
-                    const double tt = r2 * y;
-                    const double ee = fma(-tt, y, 1.0);
-                    y = fma(0.5 * y, ee, y);
-                }
-                const double u = (ip.x + aj.x) * (y * y);
-                const double s2 = (ip.x + aj.x) * u;
-                const double s4 = s2 * s2, s6 = s4 * s2;
-                const double lj = fma(s6, s6, -s6);
-                const double phi = fma(krf64, r2, y) - crf64;
-                const float dh = (float)(ip.y * aj.y * fma(12.0, s6, -6.0) * s4 * u);
-                // i-side sums in fix/fiy/fiz, j-side in fjx/fjy/fjz: (q', h, e')
-                fix += (float)(qj * phi); fjx = (float)(ip.z * phi);
-                fiy += dh; fjy = dh;
-                fiz += (float)(aj.y * lj); fjz = (float)(ip.y * lj);
-            } else {
-                const PairEF64 pe = pair_ef64(true, r2, ip.x + aj.x, ip.y * aj.y, ip.z * qj, krf64, crf64);
-                eE += pe.e;
-                if (FORCES) {
-                    const float fs = (float)pe.fs;
-                    const float fx = fs * dx, fy = fs * dy, fz = fs * dz;
-                    fix += fx; fiy += fy; fiz += fz;
-                    fjx = -fx; fjy = -fy; fjz = -fz;
-                }
+            const PairEF64 pe = pair_ef64(true, r2, ip.x + aj.x, ip.y * aj.y, ip.z * qj, krf64, crf64);
+            eE += pe.e;
+            if (FORCES) {
+                const float fs = (float)pe.fs;
+                fix = fmaf(fs, dx, fix); fiy = fmaf(fs, dy, fiy); fiz = fmaf(fs, dz, fiz);
+                fjx = -fs * dx; fjy = -fs * dy; fjz = -fs * dz;
             }
         }
-        if (FORCES || BWD) {
+        if (FORCES) {
             fjx = oct_sum(fjx); fjy = oct_sum(fjy); fjz = oct_sum(fjz);
             const float fk = ii == 0 ? fjx : (ii == 1 ? fjy : fjz);
             if (ii < 3) sJF[wave][nflush][jj * 3 + ii] = jj < nj ? fk : 0.f;
@@ -1677,20 +1687,11 @@ __global__ void __launch_bounds__(64 * W, ATM_FILT_MINW) k_pairs_f(DevPtrs P, in
         wave_sync();  // sSel is rewritten by the next batch
     }
     if (qn > 0) step(0, qn);
-    if (FORCES || BWD) {
+    if (FORCES) {
         if (nflush > 0) flush_j(nflush);
         fix = stride8_sum(fix); fiy = stride8_sum(fiy); fiz = stride8_sum(fiz);
-        if (jj == 0 && iReal) {
-            if (BWD) {
-                if (fix != 0.f) atomicAdd(P.gq + islot, wgt * (double)fix);
-                if (fiy != 0.f) atomicAdd(P.gsig + islot, wgt * (double)fiy);
-                if (fiz != 0.f) atomicAdd(P.geps + islot, wgt * (double)fiz);
-            } else {
-                flush_force(P, c, fixMax, islot, fix, fiy, fiz);
-            }
-        }
+        if (jj == 0 && iReal) flush_force(P, c, fixMax, islot, fix, fiy, fiz);
     }
-    if (BWD) return;
     eE = wave_sum_d(eE);
     if (lane == 0) {
         *eOut = eE;
@@ -1835,8 +1836,8 @@ hipError_t launch_pairs(const DevPtrs& P, int setBegin, int nsets, int maxGroups
                         int variant, hipStream_t st) {
     if (fp64 && variant == kPairKernelFiltered) {
 #define ATM_PAIRSF(WV)                                                                                          \
-    if (forces) k_pairs_f<WV, kFiltForces><<<dim3(cdiv(8ll * maxGroups, WV), nsets), 64 * WV, 0, st>>>(P, setBegin, nullptr); \
-    else k_pairs_f<WV, kFiltEnergy><<<dim3(cdiv(8ll * maxGroups, WV), nsets), 64 * WV, 0, st>>>(P, setBegin, nullptr);
+    if (forces) k_pairs_f<WV, kFiltForces><<<dim3(cdiv(8ll * maxGroups, WV), nsets), 64 * WV, 0, st>>>(P, setBegin); \
+    else k_pairs_f<WV, kFiltEnergy><<<dim3(cdiv(8ll * maxGroups, WV), nsets), 64 * WV, 0, st>>>(P, setBegin);
         switch (waves) {
             case 1: ATM_PAIRSF(1) break;
             case 2: ATM_PAIRSF(2) break;
@@ -1877,9 +1878,9 @@ hipError_t launch_pairs(const DevPtrs& P, int setBegin, int nsets, int maxGroups
 #undef ATM_PAIRSQ
         return hipGetLastError();
     }
-#define ATM_PAIRS(WV, F64)                                                                                 \
-    if (forces) k_pairs<WV, true, F64><<<dim3(cdiv(maxGroups, WV), nsets), 64 * WV, 0, st>>>(P, setBegin);  \
-    else k_pairs<WV, false, F64><<<dim3(cdiv(maxGroups, WV), nsets), 64 * WV, 0, st>>>(P, setBegin);
+#define ATM_PAIRS(WV, F64)                                                                                         \
+    if (forces) k_pairs<WV, true, F64, false><<<dim3(cdiv(maxGroups, WV), nsets), 64 * WV, 0, st>>>(P, setBegin, nullptr); \
+    else k_pairs<WV, false, F64, false><<<dim3(cdiv(maxGroups, WV), nsets), 64 * WV, 0, st>>>(P, setBegin, nullptr);
 #define ATM_PAIRS_W(F64)               \
     switch (waves) {                   \
         case 1: ATM_PAIRS(1, F64) break; \
@@ -1903,9 +1904,9 @@ hipError_t launch_set_energy(const DevPtrs& P, int c, int estride, hipStream_t s
     return hipGetLastError();
 }
 
-hipError_t launch_backward_filtered(const DevPtrs& P, int setBegin, int nsets, int maxGroups, const double* wset,
-                                    hipStream_t st) {
-    k_pairs_f<2, kFiltBackward><<<dim3(cdiv(8ll * maxGroups, 2), nsets), 128, 0, st>>>(P, setBegin, wset);
+hipError_t launch_backward_pairs(const DevPtrs& P, int setBegin, int nsets, int maxGroups, const double* wset,
+                                 hipStream_t st) {
+    k_pairs<1, false, true, true><<<dim3(maxGroups, nsets), 64, 0, st>>>(P, setBegin, wset);
     return hipGetLastError();
 }
 
