@@ -149,6 +149,21 @@ int atm_set_particles(void* context, int32_t sys, const double* charges,
                       const double* sigmas, const double* epsilons,
                       const double* displacements);
 
+/* Exceptions of system `sys`'s NonbondedForce: `count` atom pairs (pairs[2k],
+ * pairs[2k+1]) with charge product (e^2), sigma (nm) and epsilon (kJ/mol), as
+ * NonbondedForce::addException takes them; charge product 0 and epsilon 0 is an
+ * exclusion.  Every listed pair leaves the cutoff sum; the others add
+ * k_e qq/r + 4 eps ((sigma/r)^12 - (sigma/r)^6) at the plain distance (no
+ * cutoff, no periodic image: NonbondedForce's default for exceptions) in both
+ * coordinate sets.  count 0 removes them.  Rejects the call (nothing changed)
+ * for an index out of range, i == j, a pair listed twice, a non-finite value or
+ * a negative sigma/epsilon.  Relays out the context (a setup call: buffers and
+ * graphs are rebuilt).  The reference's inner contexts evaluate clones of the
+ * system's forces, exceptions included (ATMMetaForceImpl.cpp:52-62); this is
+ * the exception half of what those clones carry to the nonbonded kernel. */
+int atm_set_exceptions(void* context, int32_t sys, int32_t count, const int32_t* pairs,
+                       const double* charge_prod, const double* sigma, const double* epsilon);
+
 /* Displacements only (num_atoms*3), the part copyParametersToContext updates in
  * the reference (CommonATMMetaForceKernels.cpp:229-251). */
 int atm_set_displacements(void* context, int32_t sys, const double* displacements);

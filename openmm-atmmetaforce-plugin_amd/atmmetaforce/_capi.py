@@ -31,7 +31,7 @@ ONE_4PI_EPS0 = 138.935456
 # Every entry point declared in include/atm_mi355x.h.
 EXPORTED = (
     "atm_version", "atm_abi_version", "atm_last_error", "atm_device_count", "atm_create", "atm_destroy",
-    "atm_set_particles", "atm_set_particles_device", "atm_set_displacements", "atm_set_positions",
+    "atm_set_particles", "atm_set_particles_device", "atm_set_exceptions", "atm_set_displacements", "atm_set_positions",
     "atm_set_positions_device",
     "atm_set_box", "atm_set_params", "atm_eval", "atm_synchronize", "atm_get_results", "atm_get_forces",
     "atm_get_forces_fixed", "atm_clear_forces", "atm_get_pairs", "atm_backward", "atm_backward_device",
@@ -129,6 +129,7 @@ def load(path: str | None = None):
         "atm_set_particles": [vp, i32, dp, dp, dp, dp],
         "atm_set_particles_device": [vp, vp, vp, vp],
         "atm_set_displacements": [vp, i32, dp],
+        "atm_set_exceptions": [vp, i32, i32, ctypes.POINTER(i32), dp, dp, dp],
         "atm_set_positions": [vp, i32, dp],
         "atm_set_positions_device": [vp, vp],
         "atm_set_box": [vp, i32, dp],

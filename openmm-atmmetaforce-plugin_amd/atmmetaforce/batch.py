@@ -63,6 +63,9 @@ class ATMBatch:
             self.set_particles(k, s.charges, s.sigmas, s.epsilons, s.displacements)
             self.set_positions(k, s.positions)
             self.set_params(k, s.params)
+            ex = getattr(s, "exceptions", None)
+            if ex is not None and ex.count > 0:
+                self.set_exceptions(k, ex.pairs, ex.charge_prod, ex.sigma, ex.epsilon)
 
     # ------------------------------------------------------------ lifecycle
     def close(self):
@@ -99,6 +102,17 @@ class ATMBatch:
             d = np.ascontiguousarray(displacements, dtype=np.float64).reshape(n, 3)
         C.check(self._lib.atm_set_particles(self._h, k, _dptr(q), _dptr(s), _dptr(e),
                                             _dptr(d) if d is not None else None))
+
+    def set_exceptions(self, k, pairs, charge_prod, sigma, epsilon):
+        """NonbondedForce exceptions of system k (atm_set_exceptions): pairs (m, 2),
+        charge products (e^2), sigmas (nm), epsilons (kJ/mol); m = 0 removes them."""
+        pr = np.ascontiguousarray(pairs, dtype=np.int32).reshape(-1, 2)
+        m = pr.shape[0]
+        qq = np.ascontiguousarray(charge_prod, dtype=np.float64).reshape(m)
+        sg = np.ascontiguousarray(sigma, dtype=np.float64).reshape(m)
+        ep = np.ascontiguousarray(epsilon, dtype=np.float64).reshape(m)
+        C.check(self._lib.atm_set_exceptions(self._h, k, m, pr.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                             _dptr(qq), _dptr(sg), _dptr(ep)))
 
     def set_displacements(self, k, displacements):
         d = np.ascontiguousarray(displacements, dtype=np.float64).reshape(self.num_atoms[k], 3)

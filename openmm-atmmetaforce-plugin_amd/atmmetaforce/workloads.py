@@ -58,6 +58,20 @@ class ATMParams:
 
 
 @dataclasses.dataclass
+class Exceptions:
+    """NonbondedForce exceptions: atom pairs (m, 2) with their charge product (e^2),
+    sigma (nm) and epsilon (kJ/mol); charge product 0 and epsilon 0 is an exclusion."""
+    pairs: np.ndarray
+    charge_prod: np.ndarray
+    sigma: np.ndarray
+    epsilon: np.ndarray
+
+    @property
+    def count(self) -> int:
+        return int(self.pairs.shape[0])
+
+
+@dataclasses.dataclass
 class System:
     """One ATM system: N atoms in an orthorhombic periodic box."""
     positions: np.ndarray      # (N, 3) float64, nm
@@ -71,6 +85,7 @@ class System:
     params: ATMParams = dataclasses.field(default_factory=ATMParams)
     seed: int = 0
     num_ligand: int = 0
+    exceptions: Exceptions | None = None
 
     @property
     def num_atoms(self) -> int:
@@ -170,3 +185,51 @@ def tiny_system(num_atoms: int, box: float, cutoff: float, num_ligand: int,
                 displacement, seed: int, params: ATMParams | None = None) -> System:
     """Small systems for exhaustive tests (same generator, any size)."""
     return lattice_system(num_atoms, box, cutoff, num_ligand, displacement, seed, params)
+
+
+def with_molecules(system: System, size: int = 3, scale14: float = 0.5, coulomb14: float = 0.8333,
+                   chain_every: int = 4, crosslink_every: int = 0) -> System:
+    """The same atoms grouped into molecules of `size` consecutive atoms (on the
+    lattice they are z-neighbours ~0.3 nm apart), with the exceptions a force field
+    would create from their bonds: every pair of a molecule is excluded, and every
+    `chain_every`-th molecule is instead a chain whose 1-4 pair (first, last atom)
+    keeps a scaled interaction (charge product coulomb14 q_i q_j, sigma the mixed
+    sigma, epsilon scale14 sqrt(eps_i eps_j)), like AMBER's 1-4 scaling.
+
+    crosslink_every > 0 adds, from every crosslink_every-th atom a, a pair with
+    its lattice x-neighbour a + n^2 (n = lattice sites per edge; spatially close,
+    far apart in index, like a disulfide bond), alternately an exclusion and a
+    scaled pair."""
+    n = system.num_atoms
+    pairs, qq, sg, ep = [], [], [], []
+
+    def scaled(i, j):
+        qq.append(coulomb14 * system.charges[i] * system.charges[j])
+        sg.append(0.5 * (system.sigmas[i] + system.sigmas[j]))
+        ep.append(scale14 * math.sqrt(system.epsilons[i] * system.epsilons[j]))
+
+    for m, a in enumerate(range(0, n - size + 1, size)):
+        chain = size >= 4 and chain_every > 0 and m % chain_every == 0
+        for i in range(a, a + size):
+            for j in range(i + 1, a + size):
+                pairs.append((i, j))
+                if chain and i == a and j == a + size - 1:
+                    scaled(i, j)
+                else:
+                    qq.append(0.0); sg.append(0.0); ep.append(0.0)
+    if crosslink_every > 0:
+        side = int(math.ceil(n ** (1.0 / 3.0) - 1e-9))
+        while side ** 3 < n:
+            side += 1
+        stride = side * side
+        seen = set(pairs)
+        for k, a in enumerate(range(0, n - stride, crosslink_every)):
+            if (a, a + stride) in seen:
+                continue
+            pairs.append((a, a + stride))
+            if k % 2 == 0:
+                qq.append(0.0); sg.append(0.0); ep.append(0.0)
+            else:
+                scaled(a, a + stride)
+    ex = Exceptions(np.asarray(pairs, dtype=np.int32).reshape(-1, 2), np.asarray(qq), np.asarray(sg), np.asarray(ep))
+    return dataclasses.replace(system, exceptions=ex)

@@ -69,6 +69,18 @@ struct Context {
     DevBuf dAlist, dAcount;
     DevBuf dGq, dGsig, dGeps, dWset, dDq, dDsig, dDeps;
     DevBuf dFlag, dPosStage;  // validation counter; FP32 staging of host positions
+    // exceptions (atm_set_exceptions): per system, near masks, far partner lists and
+    // the terms with a nonzero charge product or epsilon; batch arrays on the device
+    struct SysExceptions {
+        std::vector<unsigned long long> mask;  // empty: none
+        std::vector<std::vector<int>> far;
+        std::vector<int2> pairs;
+        std::vector<double4> prm;              // (k_e q'q, sigma, 4 epsilon, 0)
+        int maxPerAtom = 0;
+    };
+    std::vector<SysExceptions> exc;
+    bool hasExc = false;
+    DevBuf dXmask, dXfarStart, dXfar, dXpair, dXprm, dXenergy;
     // batched Langevin dynamics (atm_md_*)
     DevBuf dMx, dMv, dMinvm, dMxSnap, dMvSnap;
     std::vector<double> masses;
@@ -197,11 +209,11 @@ SetDesc make_set(const atm_system_desc& d, const Tuning& tu, int s, int state) {
 // Largest partial force (kJ/mol/nm) that enters a set's 64-bit fixed-point sums.
 // One slot receives at most: one flush per group that lists its cluster (tile
 // kernel), up to 64 pair terms per group's j-batch sum (compacted kernel), one
-// flush per i-cluster that lists it (atom-list kernel) and the pair terms of its
-// own group; each term is below fixMax * 2^24, so choosing
+// flush per i-cluster that lists it (atom-list kernel), the pair terms of its
+// own group and extraTerms more (exception terms of one atom); each term is below fixMax * 2^24, so choosing
 // fixMax <= 2^63 / (2^24 maxTerms) keeps every sum inside the int64 range.
-float fixed_point_limit(const SetDesc& D) {
-    const double maxTerms = 64.0 * D.groupCap + (double)D.clusterCap + 8.0 * D.entryCap + 2.0;
+float fixed_point_limit(const SetDesc& D, int extraTerms) {
+    const double maxTerms = 64.0 * D.groupCap + (double)D.clusterCap + 8.0 * D.entryCap + 2.0 + extraTerms;
     return (float)std::min<double>(kFixedMax, std::ldexp(1.0, 39) / maxTerms);
 }
 
@@ -209,18 +221,84 @@ float fixed_point_limit(const SetDesc& D) {
 
 namespace {
 
+// Batch arrays of the systems' exceptions (near masks of every atom, far partner
+// lists as CSR, exception terms in system order; SetDesc::excOff follows the same
+// order); all null when no system has any.
+void upload_exceptions(Context& C) {
+    DevPtrs& P = C.P;
+    P.xmask = nullptr; P.xfarStart = nullptr; P.xfar = nullptr;
+    P.xpair = nullptr; P.xprm = nullptr; P.xenergy = nullptr;
+    C.hasExc = false;
+    for (const auto& X : C.exc) C.hasExc |= !X.mask.empty();
+    if (!C.hasExc) {
+        C.dXmask.release(); C.dXfarStart.release(); C.dXfar.release();
+        C.dXpair.release(); C.dXprm.release(); C.dXenergy.release();
+        return;
+    }
+    std::vector<unsigned long long> mask(C.totalAtoms, 0ull);
+    std::vector<int> farStart(C.totalAtoms + 1, 0), far;
+    std::vector<int2> pairs;
+    std::vector<double4> prm;
+    for (int s = 0; s < C.nsys; ++s) {
+        const auto& X = C.exc[s];
+        const long long off = C.atomOff[s];
+        for (int a = 0; a < C.sys[s].num_atoms; ++a) {
+            farStart[off + a] = (int)far.size();
+            if (X.mask.empty()) continue;
+            mask[off + a] = X.mask[a];
+            far.insert(far.end(), X.far[a].begin(), X.far[a].end());
+        }
+        pairs.insert(pairs.end(), X.pairs.begin(), X.pairs.end());
+        prm.insert(prm.end(), X.prm.begin(), X.prm.end());
+    }
+    farStart[C.totalAtoms] = (int)far.size();
+    if (far.empty()) far.push_back(-1);
+    if (pairs.empty()) { pairs.push_back(make_int2(0, 0)); prm.push_back(make_double4(0, 0, 0, 0)); }
+    C.dXmask.alloc(sizeof(unsigned long long) * mask.size());
+    C.dXfarStart.alloc(sizeof(int) * farStart.size());
+    C.dXfar.alloc(sizeof(int) * far.size());
+    C.dXpair.alloc(sizeof(int2) * pairs.size());
+    C.dXprm.alloc(sizeof(double4) * prm.size());
+    C.dXenergy.alloc(sizeof(double) * 2 * C.nsys);
+    // (DevBuf rounds small allocations up: copy the vectors' own sizes)
+    to_device(C, C.dXmask.p, mask.data(), sizeof(unsigned long long) * mask.size());
+    to_device(C, C.dXfarStart.p, farStart.data(), sizeof(int) * farStart.size());
+    to_device(C, C.dXfar.p, far.data(), sizeof(int) * far.size());
+    to_device(C, C.dXpair.p, pairs.data(), sizeof(int2) * pairs.size());
+    to_device(C, C.dXprm.p, prm.data(), sizeof(double4) * prm.size());
+    zero_on_stream(C, C.dXenergy.p, C.dXenergy.bytes);
+    P.xmask = C.dXmask.as<unsigned long long>();
+    P.xfarStart = C.dXfarStart.as<int>();
+    P.xfar = C.dXfar.as<int>();
+    P.xpair = C.dXpair.as<int2>();
+    P.xprm = C.dXprm.as<double4>();
+    P.xenergy = C.dXenergy.as<double>();
+}
+
+// Some set has exception terms to evaluate.
+bool has_exception_terms(const Context& C) {
+    for (const SetDesc& D : C.sets)
+        if (D.excCount > 0) return true;
+    return false;
+}
+
 void layout(Context& C) {
     C.sets.resize(2 * C.nsys);
     C.entryCapOverride.resize(2 * C.nsys, 0);
     C.alistCapOverride.resize(2 * C.nsys, 0);
     long long cellOff = 0, colOff = 0, slotOff = 0, clOff = 0, gOff = 0, eOff = 0, aOff = 0, setAtomOff = 0;
+    long long xOff = 0;
+    C.exc.resize(C.nsys);
     C.maxClusters = C.maxGroups = C.maxCells = C.maxSlots = 0;
     for (int s = 0; s < C.nsys; ++s) {
+        const Context::SysExceptions& X = C.exc[s];
         for (int st = 0; st < 2; ++st) {
             const int c = 2 * s + st;
             SetDesc D = make_set(C.sys[s], C.tuning, s, st);
             if (C.entryCapOverride[c] > D.entryCap) D.entryCap = C.entryCapOverride[c];
-            D.fixMax = fixed_point_limit(D);
+            D.fixMax = fixed_point_limit(D, X.maxPerAtom);
+            D.excOff = (int)xOff;
+            D.excCount = (int)X.pairs.size();
             D.atomOff = C.atomOff[s];
             D.setAtomOff = (int)setAtomOff;
             D.cellOff = (int)cellOff;
@@ -246,7 +324,9 @@ void layout(Context& C) {
             C.maxSlots = std::max(C.maxSlots, D.slotCap);
             C.sets[c] = D;
         }
+        xOff += (long long)X.pairs.size();
     }
+    if (xOff >= (1ll << 31)) fail(ATM_ERR_UNSUPPORTED, "too many exception terms in one batch");
     if (slotOff >= (1ll << 31) || cellOff >= (1ll << 31)) fail(ATM_ERR_UNSUPPORTED, "batch too large for 32-bit slot indices");
     C.totalCells = cellOff;
     C.totalCols = colOff;
@@ -338,11 +418,16 @@ void alloc_layout(Context& C) {
     P.acount = C.dAcount.as<int>();
     P.setEnergy = C.dSetEnergy.as<double>();
     P.gq = P.gsig = P.geps = nullptr;
+    upload_exceptions(C);
 }
 
 // Pair kernel variant 5 (FP64 pairs only) reads the atom-level j lists.
 bool uses_atom_lists(const Context& C, int flags) {
-    return C.tuning.pairKernel == kPairKernelAtomLists && (flags & ATM_EVAL_FP32_PAIRS) == 0;
+    return effective_pair_variant(C.P, C.tuning.pairKernel) == kPairKernelAtomLists && (flags & ATM_EVAL_FP32_PAIRS) == 0;
+}
+
+int energy_stride(const Context& C, int flags) {
+    return pair_energy_stride(effective_pair_variant(C.P, C.tuning.pairKernel), (flags & ATM_EVAL_FP32_PAIRS) == 0);
 }
 
 void require_ready(const Context& C) {
@@ -379,8 +464,9 @@ void enqueue_batched(Context& C, int flags, hipEvent_t* ev = nullptr) {
     if (ev) HIPCHK(hipEventRecord(ev[2], st));
     HIPCHK(launch_pairs(P, 0, S2, C.maxGroups, C.tuning.pairWaves, (flags & ATM_EVAL_FORCES) != 0,
                         (flags & ATM_EVAL_FP32_PAIRS) == 0, C.tuning.pairKernel, st));
+    if (has_exception_terms(C)) HIPCHK(launch_exceptions(P, 0, S2, (flags & ATM_EVAL_FORCES) != 0, st));
     if (ev) HIPCHK(hipEventRecord(ev[3], st));
-    HIPCHK(launch_energy(P, 0, C.nsys, pair_energy_stride(C.tuning.pairKernel, (flags & ATM_EVAL_FP32_PAIRS) == 0), st));
+    HIPCHK(launch_energy(P, 0, C.nsys, energy_stride(C, flags), st));
     if (ev) HIPCHK(hipEventRecord(ev[4], st));
     if (flags & ATM_EVAL_FORCES)
         HIPCHK(launch_blend(P, 0, C.nsys, C.maxAtoms, 0, 0.0, 0.0, (flags & ATM_EVAL_CLEAR_FORCES) != 0, st));
@@ -440,7 +526,8 @@ void eval_plain(Context& C, int flags) {
             if (uses_atom_lists(C, flags)) HIPCHK(launch_alist(P, c, 1, D.groupCap, st));
             HIPCHK(launch_pairs(P, c, 1, D.groupCap, C.tuning.pairWaves, true, (flags & ATM_EVAL_FP32_PAIRS) == 0,
                                 C.tuning.pairKernel, st));
-            HIPCHK(launch_set_energy(P, c, pair_energy_stride(C.tuning.pairKernel, (flags & ATM_EVAL_FP32_PAIRS) == 0), st));
+            if (D.excCount > 0) HIPCHK(launch_exceptions(P, c, 1, true, st));
+            HIPCHK(launch_set_energy(P, c, energy_stride(C, flags), st));
             HIPCHK(hipMemcpyAsync(&U[stt], P.setEnergy + c, sizeof(double), hipMemcpyDeviceToHost, st));
             HIPCHK(hipStreamSynchronize(st));
         }
@@ -472,7 +559,8 @@ void eval_state1(Context& C, int flags) {
         if (uses_atom_lists(C, flags)) HIPCHK(launch_alist(P, c, 1, D.groupCap, st));
         HIPCHK(launch_pairs(P, c, 1, D.groupCap, C.tuning.pairWaves, (flags & ATM_EVAL_FORCES) != 0,
                             (flags & ATM_EVAL_FP32_PAIRS) == 0, C.tuning.pairKernel, st));
-        HIPCHK(launch_set_energy(P, c, pair_energy_stride(C.tuning.pairKernel, (flags & ATM_EVAL_FP32_PAIRS) == 0), st));
+        if (D.excCount > 0) HIPCHK(launch_exceptions(P, c, 1, (flags & ATM_EVAL_FORCES) != 0, st));
+        HIPCHK(launch_set_energy(P, c, energy_stride(C, flags), st));
         double U = 0.0;
         HIPCHK(hipMemcpyAsync(&U, P.setEnergy + c, sizeof(double), hipMemcpyDeviceToHost, st));
         if (flags & ATM_EVAL_FORCES) HIPCHK(launch_blend(P, s, 1, D.natoms, 2, 0.0, 1.0, false, st));
@@ -484,8 +572,11 @@ void eval_state1(Context& C, int flags) {
     }
 }
 
+// The first call after an evaluation checks the list-overflow flags and copies
+// the results; later calls return the copies (every evaluation clears resultsValid).
 void fetch_results(Context& C) {
     if (!C.evaluated) fail(ATM_ERR_STATE, "no evaluation has been run");
+    if (C.resultsValid) return;
     HIPCHK(hipStreamSynchronize(C.stream));
     std::vector<SetRuntime> rt(2 * C.nsys);
     to_host(C, rt.data(), C.dRt.p, sizeof(SetRuntime) * rt.size());
@@ -676,6 +767,60 @@ int atm_set_particles(void* context, int32_t sys, const double* charges, const d
         HIPCHK(hipMemcpyAsync(C.dDispl.as<float4>() + off, C.hDispl.data() + off, n * sizeof(float4), hipMemcpyHostToDevice, C.stream));
         HIPCHK(hipStreamSynchronize(C.stream));
         C.haveParticles[sys] = 1;
+    });
+}
+
+int atm_set_exceptions(void* context, int32_t sys, int32_t count, const int32_t* pairs, const double* charge_prod,
+                       const double* sigma, const double* epsilon) {
+    return guarded([&] {
+        Context& C = ctx_of(context);
+        check_sys(C, sys);
+        if (count < 0) fail(ATM_ERR_INVALID_ARGUMENT, "negative exception count");
+        if (count > 0 && (!pairs || !charge_prod || !sigma || !epsilon)) fail(ATM_ERR_INVALID_ARGUMENT, "null exception array");
+        HIPCHK(hipSetDevice(C.device));
+        const int n = C.sys[sys].num_atoms;
+        Context::SysExceptions X;
+        if (count > 0) {
+            X.mask.assign(n, 0ull);
+            X.far.assign(n, std::vector<int>());
+            std::vector<int> per(n, 0);
+            std::vector<unsigned long long> keys((size_t)count);
+            for (int k = 0; k < count; ++k) {
+                const int i = pairs[2 * k], j = pairs[2 * k + 1];
+                const std::string pre = "exception " + std::to_string(k) + ": ";
+                if (i < 0 || j < 0 || i >= n || j >= n || i == j) fail(ATM_ERR_INVALID_ARGUMENT, pre + "invalid atom pair");
+                if (!std::isfinite(charge_prod[k]) || !(sigma[k] >= 0.0) || !(epsilon[k] >= 0.0) ||
+                    !std::isfinite(sigma[k]) || !std::isfinite(epsilon[k]))
+                    fail(ATM_ERR_INVALID_ARGUMENT, pre + "charge product must be finite, sigma and epsilon finite and non-negative");
+                keys[k] = (unsigned long long)std::min(i, j) << 32 | (unsigned)std::max(i, j);
+                const int d = j - i;
+                if (d >= -31 && d <= 31) {
+                    X.mask[i] |= 1ull << (32 + d);
+                    X.mask[j] |= 1ull << (32 - d);
+                } else {
+                    X.mask[i] |= 1ull << 32;
+                    X.mask[j] |= 1ull << 32;
+                    X.far[i].push_back(j);
+                    X.far[j].push_back(i);
+                }
+                if (charge_prod[k] != 0.0 || epsilon[k] != 0.0) {
+                    X.pairs.push_back(make_int2(i, j));
+                    X.prm.push_back(make_double4(ATM_ONE_4PI_EPS0 * charge_prod[k], sigma[k], 4.0 * epsilon[k], 0.0));
+                    ++per[i];
+                    ++per[j];
+                }
+            }
+            std::sort(keys.begin(), keys.end());
+            if (std::adjacent_find(keys.begin(), keys.end()) != keys.end())
+                fail(ATM_ERR_INVALID_ARGUMENT, "an atom pair is listed twice among the exceptions");
+            X.maxPerAtom = *std::max_element(per.begin(), per.end());
+        }
+        C.exc.resize(C.nsys);
+        C.exc[sys] = std::move(X);
+        layout(C);
+        alloc_layout(C);
+        C.evaluated = false;
+        C.resultsValid = false;
     });
 }
 
@@ -1038,6 +1183,7 @@ int atm_autotune(void* context, int32_t repeats, atm_tuning* out) {
             if (!ok) return 1e300;
             HIPCHK(hipEventRecord(e0, C.stream));
             for (int r = 0; r < n; ++r) eval_batched(C, flags);
+            C.resultsValid = false;
             HIPCHK(hipEventRecord(e1, C.stream));
             HIPCHK(hipEventSynchronize(e1));
             float ms = 0.f;

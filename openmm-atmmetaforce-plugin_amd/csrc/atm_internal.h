@@ -13,6 +13,13 @@
 //     of 8 with dummy slots (sidx = -1), so 8-slot clusters never straddle
 //     columns.  Up to 8 consecutive clusters of one column form a group; a group
 //     is the unit of work of the pair kernel (one wave64).
+//   * Exceptions (NonbondedForce exclusions and scaled pairs) of system s: a
+//     64-bit near mask per atom a, bit (b - a + 32) set for each excepted
+//     partner b with |b - a| <= 31 and bit 32 (b = a, never a pair) flagging
+//     partners further away, which are listed per atom (CSR over the batch's
+//     atoms); the pair kernels drop every excepted pair.  Exception terms with
+//     a nonzero charge product or epsilon are evaluated by a kernel of their own
+//     (one block per set) at the plain, non-periodic distance.
 #pragma once
 #include <stdint.h>
 
@@ -53,6 +60,8 @@ struct SetDesc {
     int alistCap;            // atom-level list entries per cluster (multiple of 8)
     long long alistOff;      // first atom-level list entry of cluster 0 of this set
     float fixMax;            // partial forces >= fixMax (kJ/mol/nm) go to the FP64 side sums
+    int excCount;            // exception terms of this set's system (nonzero charge product or epsilon)
+    int excOff;              // its first exception term in the batch arrays
     int pad_;
 };
 

@@ -53,6 +53,13 @@ struct DevPtrs {
     uint32_t* alist;       // atom-level j lists per i-cluster (pair kernel 5)
     int* acount;           // entries (whole 8-atom blocks) per i-cluster
     double* setEnergy;     // [2 S] (reference-structured path)
+    // exceptions (null when the batch has none)
+    const unsigned long long* xmask;  // per atom (batch order): near excepted partners, bit 32: far ones
+    const int* xfarStart;  // [total atoms + 1]: far partners of atom a at xfar[xfarStart[a], xfarStart[a + 1])
+    const int* xfar;       // system-local atom indices
+    const int2* xpair;     // exception terms: system-local atom pair
+    const double4* xprm;   // (k_e q'q, sigma, 4 epsilon, 0)
+    double* xenergy;       // [2 S] exception energy per set
     // backward
     double* gq;            // [2 S sets x slots] gradient accumulators in slot order
     double* gsig;
@@ -75,10 +82,17 @@ hipError_t launch_alist(const DevPtrs& P, int setBegin, int nsets, int maxGroups
 // 6: filtered atom queues (one wave per i-cluster, FP64 pairs only; FP32 pairs run variant 0)
 constexpr int kPairKernelFiltered = 6;
 constexpr int kPairKernelCount = 7;
+// A batch with exceptions runs the tile kernel whatever the variant asked for:
+// only it applies the exclusion masks.
+inline int effective_pair_variant(const DevPtrs& P, int variant) { return P.xmask ? 0 : variant; }
 hipError_t launch_pairs(const DevPtrs& P, int setBegin, int nsets, int maxGroups, int waves, bool forces, bool fp64,
                         int variant, hipStream_t st);
 // Energy slots the pair kernel fills per group in genergy (1, or 8 for the filtered kernel).
 inline int pair_energy_stride(int variant, bool fp64) { return (fp64 && variant == kPairKernelFiltered) ? 8 : 1; }
+// Exception terms of sets [setBegin, setBegin + nsets): energy into xenergy[c], with
+// `forces` their forces added to the set's slot sums (call after the pair kernel
+// and before launch_energy / launch_set_energy, which add xenergy when excCount > 0).
+hipError_t launch_exceptions(const DevPtrs& P, int setBegin, int nsets, bool forces, hipStream_t st);
 hipError_t launch_energy(const DevPtrs& P, int sysBegin, int nsys, int estride, hipStream_t st);
 hipError_t launch_set_energy(const DevPtrs& P, int c, int estride, hipStream_t st);
 // Backward pass of the pair terms (the tile kernel's backward mode): adds wset[c] dE/d(q', h, e')

@@ -676,6 +676,22 @@ __device__ __forceinline__ void flush_force(const DevPtrs& P, int c, float fixMa
     }
 }
 
+// b is one of the far excepted partners of atom a (system-local indices).
+__device__ __forceinline__ bool far_excluded(const DevPtrs& P, const SetDesc& D, int a, int b) {
+    const int* st = P.xfarStart + D.atomOff + a;
+    const int k1 = st[1];
+    for (int k = st[0]; k < k1; ++k)
+        if (P.xfar[k] == b) return true;
+    return false;
+}
+
+// The pair (a, b), a != b, of D's system is an exception; ma = a's near mask.
+__device__ __forceinline__ bool excluded_pair(const DevPtrs& P, const SetDesc& D, int a, int b, unsigned long long ma) {
+    const uint32_t dd = (uint32_t)(b - a + 32);
+    if (dd - 1u < 63u) return (ma >> dd) & 1ull;
+    return ((ma >> 32) & 1ull) && far_excluded(P, D, a, b);
+}
+
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -730,7 +746,11 @@ __device__ __forceinline__ void flush_component_u(const DevPtrs& P, int c, float
 //   dE/de'_i = e'_j (s6^2 - s6)
 // summed per lane in FP32 where the forward sums forces, weighted by wset[c]
 // (dW/dU of the set) and added to FP64 per-slot accumulators (gq, gsig, geps).
-template <int W, bool FORCES, bool F64, bool BWD>
+//
+// EXC: the batch has exceptions; every in-cutoff pair is tested against the
+// i-atom's near mask (and, for far partners, its list) and excepted pairs are
+// dropped before the pair function.
+template <int W, bool FORCES, bool F64, bool BWD, bool EXC>
 __global__ void __launch_bounds__(64 * W, 6) k_pairs(DevPtrs P, int setBegin, const double* wset) {
     static_assert(!BWD || (F64 && !FORCES), "the backward mode is FP64 and computes no forces");
     constexpr bool JSUMS = FORCES || BWD;  // per-atom j sums flushed every kJFlush entries
@@ -743,6 +763,8 @@ __global__ void __launch_bounds__(64 * W, 6) k_pairs(DevPtrs P, int setBegin, co
     __shared__ float4 sPos[W][64];
     __shared__ float2 sPrm[W][F64 ? 1 : 64];
     __shared__ double4 sPrm64[W][F64 ? 64 : 1];
+    __shared__ int sAi[W][EXC ? 64 : 1];                   // i-atom indices (system-local, -1: dummy)
+    __shared__ unsigned long long sXm[W][EXC ? 64 : 1];    // their near exception masks
     int g0 = 0, ncl = 0;
     uint32_t ivm = 0;       // bit t: i-atom (t, ii) is a real atom
     uint64_t ivLanes = 0;   // bit 8 t + ii: the same, over the group's 64 slots
@@ -754,11 +776,19 @@ __global__ void __launch_bounds__(64 * W, 6) k_pairs(DevPtrs P, int setBegin, co
         float2 q = make_float2(0.f, 0.f);
         double4 qd = make_double4(0.0, 0.0, 0.0, 0.0);
         bool v = false;
+        int ai = -1;
+        unsigned long long xm = 0ull;
         if (lane < ncl * kClusterSize) {
             const int s = D.slotOff + g0 * kClusterSize + lane;
             p = P.spos[s];
             if (F64) qd = P.sprm64[s]; else q = P.sprm[s];
-            v = P.sidx[s] >= 0;
+            ai = P.sidx[s];
+            v = ai >= 0;
+            if (EXC && v) xm = P.xmask[D.atomOff + ai];
+        }
+        if (EXC) {
+            sAi[wave][lane] = ai;
+            sXm[wave][lane] = xm;
         }
         sPos[wave][lane] = p;
         if (F64) sPrm64[wave][lane] = qd; else sPrm[wave][lane] = q;
@@ -808,8 +838,19 @@ __global__ void __launch_bounds__(64 * W, 6) k_pairs(DevPtrs P, int setBegin, co
         if (F64) b.d = P.sprm64[js]; else b.q = P.sprm[js];
         // with explicit x/y image shifts, dummy slots sit far outside the box
         // (k_gather_bbox), so the FP64 path needs no validity test there
-        b.idx = (F64 && shiftOK) ? 0 : P.sidx[js];
+        b.idx = (F64 && shiftOK && !EXC) ? 0 : P.sidx[js];
         return b;
+    };
+    // lanes of `in` whose pair (i-atom (t, ii), j-atom aj) is an exception
+    auto excepted = [&](int t, int aj, uint64_t in) __attribute__((always_inline)) {
+        const int ai = sAi[wave][t * kClusterSize + ii];
+        const unsigned long long mi = sXm[wave][t * kClusterSize + ii];
+        const uint32_t dd = (uint32_t)(aj - ai + 32);
+        const bool near = dd - 1u < 63u;
+        bool ex = near && ((mi >> (dd & 63u)) & 1ull);
+        const uint64_t farLanes = __builtin_amdgcn_ballot_w64(!near && ((mi >> 32) & 1ull)) & in;
+        if (farLanes != 0ull && __builtin_amdgcn_inverse_ballot_w64(farLanes)) ex = far_excluded(P, D, ai, aj);
+        return __builtin_amdgcn_ballot_w64(ex) & in;
     };
     // j-force sums of the last kJFlush entries, flushed together: 3 dense rounds
     // of one atomic per lane instead of one sparse atomic per entry
@@ -881,7 +922,8 @@ __global__ void __launch_bounds__(64 * W, 6) k_pairs(DevPtrs P, int setBegin, co
                     const double4 id = sPrm64[wave][t * kClusterSize + ii];
                     // exact: the squares of 2^-20-grid differences fit in 53 bits
                     const double r2 = fma((double)dx, (double)dx, fma((double)dy, (double)dy, (double)dz * (double)dz));
-                    const uint64_t inLanes = __builtin_amdgcn_ballot_w64(r2 < rc2) & candLanes;
+                    uint64_t inLanes = __builtin_amdgcn_ballot_w64(r2 < rc2) & candLanes;
+                    if (EXC) inLanes &= ~excepted(t, b.idx, inLanes);
                     npair += (uint32_t)__popcll(inLanes);
                     // lanes outside the cutoff are masked off (exec), not zeroed with selects
                     fs = 0.f;
@@ -915,7 +957,8 @@ __global__ void __launch_bounds__(64 * W, 6) k_pairs(DevPtrs P, int setBegin, co
                     const uint64_t okLanes = (t == tdiag ? upperLanes : ~0ull) & jvLanes & iv;
                     const float2 qi = sPrm[wave][t * kClusterSize + ii];
                     const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-                    const uint64_t inLanes = __builtin_amdgcn_ballot_w64(in_cutoff(D, dx, dy, dz, r2)) & okLanes;
+                    uint64_t inLanes = __builtin_amdgcn_ballot_w64(in_cutoff(D, dx, dy, dz, r2)) & okLanes;
+                    if (EXC && inLanes != 0ull) inLanes &= ~excepted(t, b.idx, inLanes);
                     if (inLanes == 0ull) continue;
                     npair += (uint32_t)__popcll(inLanes);
                     const bool in = __builtin_amdgcn_inverse_ballot_w64(inLanes);
@@ -1482,6 +1525,7 @@ __global__ void __launch_bounds__(64) k_pairs_dump(DevPtrs P, int c, int2* out, 
             const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
             bool in = in_cutoff(D, dx, dy, dz, r2) && ja.valid && iidx >= 0;
             if (g0 + t == J) in = in && (ii < jj);
+            if (in && P.xmask) in = !excluded_pair(P, D, iidx, jidx, P.xmask[D.atomOff + iidx]);
             if (in) {
                 nin += 1;
                 unsigned long long k = atomicAdd(counters, 1ull);
@@ -1699,6 +1743,70 @@ __global__ void __launch_bounds__(64 * W, ATM_FILT_MINW) k_pairs_f(DevPtrs P, in
     }
 }
 
+// ---------------------------------------------------------------- exceptions
+
+// Add a partial force in FP64 to a slot: fixed point below fixMax, FP64 side sums beyond.
+__device__ __forceinline__ void add_force64(const DevPtrs& P, int c, double fixMax, long long slot, double fx,
+                                            double fy, double fz) {
+    long long* f = P.sf + slot;
+    double* d = P.sd + 3 * slot;
+    bool big = false;
+    if (fabs(fx) < fixMax) atomic_add_fixed(f, __double2ll_rn(fx * kForceScale)); else { atomicAdd(d, fx); big = true; }
+    if (fabs(fy) < fixMax) atomic_add_fixed(f + P.sfn, __double2ll_rn(fy * kForceScale)); else { atomicAdd(d + 1, fy); big = true; }
+    if (fabs(fz) < fixMax) atomic_add_fixed(f + 2 * P.sfn, __double2ll_rn(fz * kForceScale)); else { atomicAdd(d + 2, fz); big = true; }
+    if (big) mark_side(P, c, slot);
+}
+
+// One block per set: the exception terms k_e q'q / r + 4 eps' ((s'/r)^12 - (s'/r)^6)
+// of its system at the plain distance of the set's coordinates (x, or x + d for
+// state 1; no cutoff, no periodic image), as NonbondedForce computes its
+// exceptions by default (exceptions do not use periodic boundary conditions).
+// FP64 throughout; energy summed per thread in a fixed order, then a fixed tree.
+__global__ void __launch_bounds__(256) k_exceptions(DevPtrs P, int setBegin, int forces) {
+    const int c = setBegin + blockIdx.x;
+    const SetDesc& D = P.desc[c];
+    __shared__ double red[256];
+    const int n = D.excCount;
+    const float4* X = P.posq + D.atomOff;
+    const float4* Xd = P.displ + D.atomOff;
+    const bool shifted = D.state == 1;
+    const double fixMax = D.fixMax;
+    double e = 0.0;
+    for (int k = threadIdx.x; k < n; k += 256) {
+        const int2 pr = P.xpair[D.excOff + k];
+        const double4 q = P.xprm[D.excOff + k];
+        const float4 a = X[pr.x], b = X[pr.y];
+        double ax = a.x, ay = a.y, az = a.z, bx = b.x, by = b.y, bz = b.z;
+        if (shifted) {
+            const float4 da = Xd[pr.x], db = Xd[pr.y];
+            ax += (double)da.x; ay += (double)da.y; az += (double)da.z;
+            bx += (double)db.x; by += (double)db.y; bz += (double)db.z;
+        }
+        const double dx = ax - bx, dy = ay - by, dz = az - bz;
+        const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+        const double rinv = 1.0 / sqrt(r2);
+        const double r2inv = rinv * rinv;
+        const double s2 = q.y * q.y * r2inv, s6 = s2 * s2 * s2;
+        const double elj = q.z * fma(s6, s6, -s6);
+        const double ec = q.x * rinv;
+        e += elj + ec;
+        if (forces) {
+            const double fs = fma(q.z, fma(12.0 * s6, s6, -6.0 * s6), ec) * r2inv;
+            const long long si = D.slotOff + P.slotOf[D.setAtomOff + pr.x];
+            const long long sj = D.slotOff + P.slotOf[D.setAtomOff + pr.y];
+            add_force64(P, c, fixMax, si, fs * dx, fs * dy, fs * dz);
+            add_force64(P, c, fixMax, sj, -fs * dx, -fs * dy, -fs * dz);
+        }
+    }
+    red[threadIdx.x] = e;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) P.xenergy[c] = red[0];
+}
+
 // ---------------------------------------------------------------- ATM combine
 
 // One block of 256 threads per system: fixed-order FP64 sums of the energy
@@ -1719,7 +1827,7 @@ __global__ void __launch_bounds__(256) k_energy(DevPtrs P, int sysBegin, int est
             if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
             __syncthreads();
         }
-        U[st] = red[0];
+        U[st] = red[0] + (D.excCount > 0 ? P.xenergy[c] : 0.0);
         __syncthreads();
     }
     if (threadIdx.x == 0) {
@@ -1742,7 +1850,7 @@ __global__ void __launch_bounds__(256) k_set_energy(DevPtrs P, int c, int estrid
         if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
         __syncthreads();
     }
-    if (threadIdx.x == 0) P.setEnergy[c] = red[0];
+    if (threadIdx.x == 0) P.setEnergy[c] = red[0] + (D.excCount > 0 ? P.xenergy[c] : 0.0);
 }
 
 // F_sys += w1 F1 + w2 F2 with (w1, w2) = (1-sp, sp) for direction > 0 and
@@ -1834,6 +1942,7 @@ hipError_t launch_alist(const DevPtrs& P, int setBegin, int nsets, int maxGroups
 
 hipError_t launch_pairs(const DevPtrs& P, int setBegin, int nsets, int maxGroups, int waves, bool forces, bool fp64,
                         int variant, hipStream_t st) {
+    variant = effective_pair_variant(P, variant);
     if (fp64 && variant == kPairKernelFiltered) {
 #define ATM_PAIRSF(WV)                                                                                          \
     if (forces) k_pairs_f<WV, kFiltForces><<<dim3(cdiv(8ll * maxGroups, WV), nsets), 64 * WV, 0, st>>>(P, setBegin); \
@@ -1878,9 +1987,11 @@ hipError_t launch_pairs(const DevPtrs& P, int setBegin, int nsets, int maxGroups
 #undef ATM_PAIRSQ
         return hipGetLastError();
     }
-#define ATM_PAIRS(WV, F64)                                                                                         \
-    if (forces) k_pairs<WV, true, F64, false><<<dim3(cdiv(maxGroups, WV), nsets), 64 * WV, 0, st>>>(P, setBegin, nullptr); \
-    else k_pairs<WV, false, F64, false><<<dim3(cdiv(maxGroups, WV), nsets), 64 * WV, 0, st>>>(P, setBegin, nullptr);
+#define ATM_PAIRS_X(WV, F64, X)                                                                                          \
+    if (forces) k_pairs<WV, true, F64, false, X><<<dim3(cdiv(maxGroups, WV), nsets), 64 * WV, 0, st>>>(P, setBegin, nullptr); \
+    else k_pairs<WV, false, F64, false, X><<<dim3(cdiv(maxGroups, WV), nsets), 64 * WV, 0, st>>>(P, setBegin, nullptr);
+#define ATM_PAIRS(WV, F64)                                \
+    if (P.xmask) { ATM_PAIRS_X(WV, F64, true) } else { ATM_PAIRS_X(WV, F64, false) }
 #define ATM_PAIRS_W(F64)               \
     switch (waves) {                   \
         case 1: ATM_PAIRS(1, F64) break; \
@@ -1891,6 +2002,7 @@ hipError_t launch_pairs(const DevPtrs& P, int setBegin, int nsets, int maxGroups
     if (fp64) { ATM_PAIRS_W(true) } else { ATM_PAIRS_W(false) }
 #undef ATM_PAIRS_W
 #undef ATM_PAIRS
+#undef ATM_PAIRS_X
     return hipGetLastError();
 }
 
@@ -1906,7 +2018,13 @@ hipError_t launch_set_energy(const DevPtrs& P, int c, int estride, hipStream_t s
 
 hipError_t launch_backward_pairs(const DevPtrs& P, int setBegin, int nsets, int maxGroups, const double* wset,
                                  hipStream_t st) {
-    k_pairs<1, false, true, true><<<dim3(maxGroups, nsets), 64, 0, st>>>(P, setBegin, wset);
+    if (P.xmask) k_pairs<1, false, true, true, true><<<dim3(maxGroups, nsets), 64, 0, st>>>(P, setBegin, wset);
+    else k_pairs<1, false, true, true, false><<<dim3(maxGroups, nsets), 64, 0, st>>>(P, setBegin, wset);
+    return hipGetLastError();
+}
+
+hipError_t launch_exceptions(const DevPtrs& P, int setBegin, int nsets, bool forces, hipStream_t st) {
+    k_exceptions<<<nsets, 256, 0, st>>>(P, setBegin, forces ? 1 : 0);
     return hipGetLastError();
 }
 

@@ -23,7 +23,8 @@ void validate_desc(const atm_system_desc& d, int s);
 // into the batch arrays are left zero (the caller places the set).
 SetDesc make_set(const atm_system_desc& d, const Tuning& tu, int s, int state);
 
-// Largest partial force (kJ/mol/nm) that enters a set's 64-bit fixed-point sums.
-float fixed_point_limit(const SetDesc& D);
+// Largest partial force (kJ/mol/nm) that enters a set's 64-bit fixed-point sums
+// when each slot may also receive extraTerms exception terms.
+float fixed_point_limit(const SetDesc& D, int extraTerms = 0);
 
 }  // namespace atm

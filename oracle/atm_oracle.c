@@ -20,8 +20,12 @@
  *                          E_LJ = 4 eps_ij [(s_ij/r)^12 - (s_ij/r)^6], s_ij = (s_i+s_j)/2, eps_ij = sqrt(eps_i eps_j)
  *                          E_C  = k_e q_i q_j (1/r + k_rf r^2 - c_rf),
  *                          k_rf = (e_rf-1)/((2 e_rf+1) rc^3), c_rf = 3 e_rf/((2 e_rf+1) rc)
- *                        for pairs i<j with minimum-image r^2 < rc^2 (strict), k_e = 138.935456.
- *                        No exclusions, no dispersion correction.
+ *                        for pairs i<j with minimum-image r^2 < rc^2 (strict), k_e = 138.935456,
+ *                        that are not NonbondedForce exceptions; no dispersion correction.
+ *                        Each exception (i, j, q', s', e') adds k_e q'/r + 4 e' [(s'/r)^12 - (s'/r)^6]
+ *                        at the plain (non-periodic) distance, with no cutoff (OpenMM's documented
+ *                        exception terms; q' = e' = 0 is an exclusion).  The build's convention for
+ *                        the reaction-field method, the same as its PME path's.
  *   oracle_atm_grad      analytic dW/d(q, sigma, epsilon) and dW/d(lambda1, lambda2, alpha, u0, w0)
  *                        (no reference counterpart: the reference has no gradients).
  *
@@ -150,9 +154,37 @@ static void grid_free(grid* g) {
 
 typedef void (*pair_fn)(void* ctx, int i, int j, double dx, double dy, double dz, double r2);
 
-/* Visit every pair i<j with minimum-image r^2 < rc^2 exactly once, in a fixed order.
- * brute != 0 forces the O(N^2) loop. */
-static long long visit_pairs(int n, const double* x, const double* box, double rc, int brute, pair_fn fn, void* ctx) {
+/* Exception pairs as sorted keys i << 32 | j (i < j). */
+typedef struct {
+    unsigned long long* key;
+    long long n;
+} excl_set;
+
+static int cmp_u64(const void* a, const void* b) {
+    unsigned long long x = *(const unsigned long long*)a, y = *(const unsigned long long*)b;
+    return x < y ? -1 : (x > y ? 1 : 0);
+}
+
+static void excl_build(excl_set* e, int nexc, const int* pairs) {
+    e->n = nexc;
+    e->key = nexc > 0 ? (unsigned long long*)malloc(sizeof(unsigned long long) * nexc) : NULL;
+    for (int k = 0; k < nexc; ++k) {
+        unsigned long long a = (unsigned)pairs[2 * k], b = (unsigned)pairs[2 * k + 1];
+        e->key[k] = a < b ? (a << 32 | b) : (b << 32 | a);
+    }
+    if (nexc > 1) qsort(e->key, nexc, sizeof(unsigned long long), cmp_u64);
+}
+
+static int excl_has(const excl_set* e, int i, int j) {
+    if (!e || e->n == 0) return 0;
+    unsigned long long k = (unsigned long long)(unsigned)i << 32 | (unsigned)j;
+    return bsearch(&k, e->key, e->n, sizeof(unsigned long long), cmp_u64) != NULL;
+}
+
+/* Visit every pair i<j with minimum-image r^2 < rc^2 that is not in `ex` exactly once, in a
+ * fixed order.  brute != 0 forces the O(N^2) loop. */
+static long long visit_pairs(int n, const double* x, const double* box, double rc, int brute, const excl_set* ex,
+                             pair_fn fn, void* ctx) {
     const double rc2 = rc * rc;
     long long cnt = 0;
     grid g;
@@ -179,7 +211,7 @@ static long long visit_pairs(int n, const double* x, const double* box, double r
                             double dy = mimg(x[3 * i + 1] - x[3 * j + 1], box[1]);
                             double dz = mimg(x[3 * i + 2] - x[3 * j + 2], box[2]);
                             double r2 = dx * dx + dy * dy + dz * dz;
-                            if (r2 < rc2) {
+                            if (r2 < rc2 && !excl_has(ex, i, j)) {
                                 ++cnt;
                                 if (fn) fn(ctx, i, j, dx, dy, dz, r2);
                             }
@@ -195,7 +227,7 @@ static long long visit_pairs(int n, const double* x, const double* box, double r
             double dy = mimg(x[3 * i + 1] - x[3 * j + 1], box[1]);
             double dz = mimg(x[3 * i + 2] - x[3 * j + 2], box[2]);
             double r2 = dx * dx + dy * dy + dz * dz;
-            if (r2 < rc2) {
+            if (r2 < rc2 && !excl_has(ex, i, j)) {
                 ++cnt;
                 if (fn) fn(ctx, i, j, dx, dy, dz, r2);
             }
@@ -218,10 +250,20 @@ static void pairs_cb(void* vc, int i, int j, double dx, double dy, double dz, do
     c->k++;
 }
 
-/* In-cutoff pairs (i<j) in visiting order; returns the count. */
-long long oracle_pairs(int n, const double* x, const double* box, double rc, int brute, int* pairs, long long cap) {
+/* In-cutoff pairs (i<j) that are not exceptions (nexc pairs, 2 indices each), in visiting
+ * order; returns the count. */
+long long oracle_pairs_ex(int n, const double* x, const double* box, double rc, int brute, int nexc,
+                          const int* exc, int* pairs, long long cap) {
     pairs_ctx c = {pairs, pairs ? cap : 0, 0};
-    return visit_pairs(n, x, box, rc, brute, pairs_cb, &c);
+    excl_set e;
+    excl_build(&e, nexc, exc);
+    long long r = visit_pairs(n, x, box, rc, brute, &e, pairs_cb, &c);
+    free(e.key);
+    return r;
+}
+
+long long oracle_pairs(int n, const double* x, const double* box, double rc, int brute, int* pairs, long long cap) {
+    return oracle_pairs_ex(n, x, box, rc, brute, 0, NULL, pairs, cap);
 }
 
 typedef struct {
@@ -258,12 +300,40 @@ static void nb_cb(void* vc, int i, int j, double dx, double dy, double dz, doubl
     nsum_add(&c->Eabs, fabs(elj) + fabs(ec));
 }
 
+/* The exception terms: k_e q'/r + 4 e' [(s'/r)^12 - (s'/r)^6] per listed pair at the plain
+ * distance (no cutoff, no periodic image); forces added to F (may be NULL); energy added to
+ * E, sum of |terms| to Eabs. */
+static void exception_terms(const double* x, int nexc, const int* exc, const double* qq, const double* es,
+                            const double* ee, double* F, nsum* E, nsum* Eabs) {
+    for (int k = 0; k < nexc; ++k) {
+        const int i = exc[2 * k], j = exc[2 * k + 1];
+        if (qq[k] == 0.0 && ee[k] == 0.0) continue;  /* an exclusion */
+        double dx = x[3 * i] - x[3 * j], dy = x[3 * i + 1] - x[3 * j + 1], dz = x[3 * i + 2] - x[3 * j + 2];
+        double r2 = dx * dx + dy * dy + dz * dz;
+        double inv_r = 1.0 / sqrt(r2);
+        double sr2 = es[k] * es[k] / r2;
+        double sr6 = sr2 * sr2 * sr2;
+        double elj = 4.0 * ee[k] * (sr6 * sr6 - sr6);
+        double ec = ORACLE_KE * qq[k] * inv_r;
+        double f_over_r = 4.0 * ee[k] * (12.0 * sr6 * sr6 - 6.0 * sr6) / r2 + ec / r2;
+        if (F) {
+            F[3 * i] += f_over_r * dx; F[3 * i + 1] += f_over_r * dy; F[3 * i + 2] += f_over_r * dz;
+            F[3 * j] -= f_over_r * dx; F[3 * j + 1] -= f_over_r * dy; F[3 * j + 2] -= f_over_r * dz;
+        }
+        nsum_add(E, elj + ec);
+        nsum_add(Eabs, fabs(elj) + fabs(ec));
+    }
+}
+
 /* Energy (returned) and forces (F, n*3, overwritten; may be NULL) of the LJ +
- * reaction-field cutoff interaction.  parts (may be NULL) receives
- * {E_LJ, E_C, sum |E_LJ,ij| + |E_C,ij|}. */
-double oracle_nonbonded(int n, const double* x, const double* q, const double* sig, const double* eps,
-                        const double* box, double rc, double eps_rf, int brute, double* F, long long* npairs,
-                        double* parts) {
+ * reaction-field cutoff interaction with nexc exceptions (exc: 2 indices each; qq: k_e-free
+ * charge products, es: sigma, ee: epsilon).  parts (may be NULL, 4 doubles) receives
+ * E_LJ and E_C of the cutoff pairs, the sum of |terms| over the cutoff pairs and the
+ * exceptions, and the exception energy. */
+double oracle_nonbonded_ex(int n, const double* x, const double* q, const double* sig, const double* eps,
+                           const double* box, double rc, double eps_rf, int brute, int nexc, const int* exc,
+                           const double* qq, const double* es, const double* ee, double* F, long long* npairs,
+                           double* parts) {
     nb_ctx c;
     memset(&c, 0, sizeof(c));
     c.q = q;
@@ -273,30 +343,49 @@ double oracle_nonbonded(int n, const double* x, const double* q, const double* s
     c.crf = 3.0 * eps_rf / ((2.0 * eps_rf + 1.0) * rc);
     c.F = F;
     if (F) memset(F, 0, sizeof(double) * 3 * n);
-    long long np = visit_pairs(n, x, box, rc, brute, nb_cb, &c);
+    excl_set e;
+    excl_build(&e, nexc, exc);
+    long long np = visit_pairs(n, x, box, rc, brute, &e, nb_cb, &c);
+    free(e.key);
+    nsum Ex = {0.0, 0.0};
+    exception_terms(x, nexc, exc, qq, es, ee, F, &Ex, &c.Eabs);
     if (npairs) *npairs = np;
     if (parts) {
         parts[0] = c.Elj.s + c.Elj.c;
         parts[1] = c.Ec.s + c.Ec.c;
         parts[2] = c.Eabs.s + c.Eabs.c;
+        parts[3] = Ex.s + Ex.c;
     }
+    nsum_add(&c.E, Ex.s);
+    nsum_add(&c.E, Ex.c);
     return c.E.s + c.E.c;
+}
+
+double oracle_nonbonded(int n, const double* x, const double* q, const double* sig, const double* eps,
+                        const double* box, double rc, double eps_rf, int brute, double* F, long long* npairs,
+                        double* parts) {
+    double p4[4];
+    double E = oracle_nonbonded_ex(n, x, q, sig, eps, box, rc, eps_rf, brute, 0, NULL, NULL, NULL, NULL, F, npairs,
+                                   p4);
+    if (parts) { parts[0] = p4[0]; parts[1] = p4[1]; parts[2] = p4[2]; }
+    return E;
 }
 
 /* One ATM evaluation: U1,F1 at x; U2,F2 at x + d (entry i displaces atom i);
  * combine; F (n*3, overwritten with the ATM force; may be NULL). */
-int oracle_atm_eval(int n, const double* x, const double* d, const double* q, const double* sig, const double* eps,
-                    const double* box, double rc, double eps_rf, const oracle_params* p, int brute, double* F,
-                    oracle_result* r) {
+int oracle_atm_eval_ex(int n, const double* x, const double* d, const double* q, const double* sig,
+                       const double* eps, const double* box, double rc, double eps_rf, int nexc, const int* exc,
+                       const double* qq, const double* es, const double* ee, const oracle_params* p, int brute,
+                       double* F, oracle_result* r) {
     double* x2 = (double*)malloc(sizeof(double) * 3 * n);
     double* F1 = F ? (double*)malloc(sizeof(double) * 3 * n) : NULL;
     double* F2 = F ? (double*)malloc(sizeof(double) * 3 * n) : NULL;
     if (!x2 || (F && (!F1 || !F2))) return -1;
     for (int i = 0; i < 3 * n; ++i) x2[i] = x[i] + d[i];
     long long n1 = 0, n2 = 0;
-    double parts1[3], parts2[3];
-    double U1 = oracle_nonbonded(n, x, q, sig, eps, box, rc, eps_rf, brute, F1, &n1, parts1);
-    double U2 = oracle_nonbonded(n, x2, q, sig, eps, box, rc, eps_rf, brute, F2, &n2, parts2);
+    double parts1[4], parts2[4];
+    double U1 = oracle_nonbonded_ex(n, x, q, sig, eps, box, rc, eps_rf, brute, nexc, exc, qq, es, ee, F1, &n1, parts1);
+    double U2 = oracle_nonbonded_ex(n, x2, q, sig, eps, box, rc, eps_rf, brute, nexc, exc, qq, es, ee, F2, &n2, parts2);
     oracle_combine(p, U1, U2, r);
     r->npairs1 = n1;
     r->npairs2 = n2;
@@ -310,6 +399,12 @@ int oracle_atm_eval(int n, const double* x, const double* d, const double* q, co
     free(F1);
     free(F2);
     return 0;
+}
+
+int oracle_atm_eval(int n, const double* x, const double* d, const double* q, const double* sig, const double* eps,
+                    const double* box, double rc, double eps_rf, const oracle_params* p, int brute, double* F,
+                    oracle_result* r) {
+    return oracle_atm_eval_ex(n, x, d, q, sig, eps, box, rc, eps_rf, 0, NULL, NULL, NULL, NULL, p, brute, F, r);
 }
 
 /* ------------------------------------------------------------------ gradients */
@@ -347,9 +442,10 @@ static void gr_cb(void* vc, int i, int j, double dx, double dy, double dz, doubl
 /* dW/d(q, sigma, eps) (n each, overwritten) and dW/d(lambda1, lambda2, alpha, u0, w0)
  * (dparams, 5) at the evaluation described by (x, d, ..., p), using the result r of
  * oracle_atm_eval.  dW/dU1 and dW/dU2 use sp, the weight the reference's force uses. */
-void oracle_atm_grad(int n, const double* x, const double* d, const double* q, const double* sig, const double* eps,
-                     const double* box, double rc, double eps_rf, const oracle_params* p, const oracle_result* r,
-                     double* dq, double* dsig, double* deps, double* dparams) {
+void oracle_atm_grad_ex(int n, const double* x, const double* d, const double* q, const double* sig,
+                        const double* eps, const double* box, double rc, double eps_rf, int nexc, const int* exc,
+                        const oracle_params* p, const oracle_result* r, double* dq, double* dsig, double* deps,
+                        double* dparams) {
     double* x2 = (double*)malloc(sizeof(double) * 3 * n);
     for (int i = 0; i < 3 * n; ++i) x2[i] = x[i] + d[i];
     memset(dq, 0, sizeof(double) * n);
@@ -361,10 +457,14 @@ void oracle_atm_grad(int n, const double* x, const double* d, const double* q, c
     c.crf = 3.0 * eps_rf / ((2.0 * eps_rf + 1.0) * rc);
     c.dq = dq; c.dsig = dsig; c.deps = deps;
     int fwd = p->direction > 0;
+    /* exception terms carry their own parameters: no per-atom derivative */
+    excl_set e;
+    excl_build(&e, nexc, exc);
     c.w = fwd ? 1.0 - r->sp : r->sp; /* dW/dU1 */
-    visit_pairs(n, x, box, rc, 0, gr_cb, &c);
+    visit_pairs(n, x, box, rc, 0, &e, gr_cb, &c);
     c.w = fwd ? r->sp : 1.0 - r->sp; /* dW/dU2 */
-    visit_pairs(n, x2, box, rc, 0, gr_cb, &c);
+    visit_pairs(n, x2, box, rc, 0, &e, gr_cb, &c);
+    free(e.key);
     free(x2);
     if (dparams) {
         double usc = r->usc;
@@ -383,4 +483,10 @@ void oracle_atm_grad(int n, const double* x, const double* d, const double* q, c
         dparams[3] = du0;
         dparams[4] = 1.0;
     }
+}
+
+void oracle_atm_grad(int n, const double* x, const double* d, const double* q, const double* sig, const double* eps,
+                     const double* box, double rc, double eps_rf, const oracle_params* p, const oracle_result* r,
+                     double* dq, double* dsig, double* deps, double* dparams) {
+    oracle_atm_grad_ex(n, x, d, q, sig, eps, box, rc, eps_rf, 0, NULL, p, r, dq, dsig, deps, dparams);
 }

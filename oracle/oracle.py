@@ -58,6 +58,20 @@ def lib():
                                       ctypes.POINTER(Params), ctypes.c_int, dp, ctypes.POINTER(Result)]
         L.oracle_atm_grad.argtypes = [ctypes.c_int, dp, dp, dp, dp, dp, dp, ctypes.c_double, ctypes.c_double,
                                       ctypes.POINTER(Params), ctypes.POINTER(Result), dp, dp, dp, dp]
+        L.oracle_pairs_ex.restype = ctypes.c_longlong
+        L.oracle_pairs_ex.argtypes = [ctypes.c_int, dp, dp, ctypes.c_double, ctypes.c_int, ctypes.c_int, ip, ip,
+                                      ctypes.c_longlong]
+        L.oracle_nonbonded_ex.restype = ctypes.c_double
+        L.oracle_nonbonded_ex.argtypes = [ctypes.c_int, dp, dp, dp, dp, dp, ctypes.c_double, ctypes.c_double,
+                                          ctypes.c_int, ctypes.c_int, ip, dp, dp, dp, dp,
+                                          ctypes.POINTER(ctypes.c_longlong), dp]
+        L.oracle_atm_eval_ex.restype = ctypes.c_int
+        L.oracle_atm_eval_ex.argtypes = [ctypes.c_int, dp, dp, dp, dp, dp, dp, ctypes.c_double, ctypes.c_double,
+                                         ctypes.c_int, ip, dp, dp, dp, ctypes.POINTER(Params), ctypes.c_int, dp,
+                                         ctypes.POINTER(Result)]
+        L.oracle_atm_grad_ex.argtypes = [ctypes.c_int, dp, dp, dp, dp, dp, dp, ctypes.c_double, ctypes.c_double,
+                                         ctypes.c_int, ip, ctypes.POINTER(Params), ctypes.POINTER(Result), dp, dp, dp,
+                                         dp]
         _lib = L
     return _lib
 
@@ -99,21 +113,34 @@ def blend(sp, direction, F1, F2, F=None):
     return out
 
 
-def pairs(x, box, rc, brute=False) -> np.ndarray:
-    """Sorted (i, j) in-cutoff pairs, i < j, shape (P, 2)."""
+def _exc(ex):
+    """(count, pairs, qq, sigma, epsilon) ctypes arguments of a workloads.Exceptions (or None)."""
+    ip = ctypes.POINTER(ctypes.c_int)
+    if ex is None or ex.count == 0:
+        return 0, None, None, None, None, ()
+    pr = np.ascontiguousarray(ex.pairs, dtype=np.int32)
+    qq, pq = _d(ex.charge_prod)
+    sg, ps = _d(ex.sigma)
+    ep, pe = _d(ex.epsilon)
+    return ex.count, pr.ctypes.data_as(ip), pq, ps, pe, (pr, qq, sg, ep)
+
+
+def pairs(x, box, rc, brute=False, exceptions=None) -> np.ndarray:
+    """Sorted (i, j) in-cutoff pairs, i < j, that are not exceptions, shape (P, 2)."""
     x, px = _d(x)
     box, pb = _d(box)
     n = x.shape[0]
-    count = lib().oracle_pairs(n, px, pb, rc, int(brute), None, 0)
+    ne, pex, _, _, _, keep = _exc(exceptions)
+    count = lib().oracle_pairs_ex(n, px, pb, rc, int(brute), ne, pex, None, 0)
     out = np.zeros((max(count, 1), 2), dtype=np.int32)
-    lib().oracle_pairs(n, px, pb, rc, int(brute), out.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), count)
+    lib().oracle_pairs_ex(n, px, pb, rc, int(brute), ne, pex, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), count)
     out = out[:count]
     order = np.lexsort((out[:, 1], out[:, 0]))
     return out[order]
 
 
-def nonbonded(x, q, sig, eps, box, rc, eps_rf, brute=False, forces=True):
-    """(energy, forces (N,3) or None, npairs, (E_LJ, E_C, sum |e_ij|))."""
+def nonbonded(x, q, sig, eps, box, rc, eps_rf, brute=False, forces=True, exceptions=None):
+    """(energy, forces (N,3) or None, npairs, (E_LJ, E_C, sum |e_ij|, E_exceptions))."""
     x, px = _d(x)
     q, pq = _d(q)
     sig, ps = _d(sig)
@@ -122,11 +149,12 @@ def nonbonded(x, q, sig, eps, box, rc, eps_rf, brute=False, forces=True):
     n = x.shape[0]
     F = np.zeros((n, 3)) if forces else None
     np_ = ctypes.c_longlong()
-    parts = np.zeros(3)
-    E = lib().oracle_nonbonded(n, px, pq, ps, pe, pb, rc, eps_rf, int(brute),
-                               F.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) if forces else None,
-                               ctypes.byref(np_), parts.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
-    return E, F, np_.value, (parts[0], parts[1], parts[2])
+    parts = np.zeros(4)
+    ne, pex, pqq, psg, pep, keep = _exc(exceptions)
+    E = lib().oracle_nonbonded_ex(n, px, pq, ps, pe, pb, rc, eps_rf, int(brute), ne, pex, pqq, psg, pep,
+                                  F.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) if forces else None,
+                                  ctypes.byref(np_), parts.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    return E, F, np_.value, (parts[0], parts[1], parts[2], parts[3])
 
 
 def atm_eval(system, params=None, brute=False, forces=True):
@@ -141,9 +169,11 @@ def atm_eval(system, params=None, brute=False, forces=True):
     n = x.shape[0]
     F = np.zeros((n, 3)) if forces else None
     r = Result()
-    rc = lib().oracle_atm_eval(n, px, pd, pq, ps, pe, pb, system.cutoff, system.rf_dielectric, ctypes.byref(p),
-                               int(brute), F.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) if forces else None,
-                               ctypes.byref(r))
+    ne, pex, pqq, psg, pep, keep = _exc(getattr(system, "exceptions", None))
+    rc = lib().oracle_atm_eval_ex(n, px, pd, pq, ps, pe, pb, system.cutoff, system.rf_dielectric, ne, pex, pqq, psg,
+                                  pep, ctypes.byref(p), int(brute),
+                                  F.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) if forces else None,
+                                  ctypes.byref(r))
     if rc != 0:
         raise MemoryError("oracle_atm_eval failed")
     return r.as_dict(), F
@@ -163,8 +193,9 @@ def atm_grad(system, params=None):
     n = x.shape[0]
     dq, ds, de, dp = np.zeros(n), np.zeros(n), np.zeros(n), np.zeros(5)
     c = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
-    lib().oracle_atm_grad(n, px, pd, pq, ps, pe, pb, system.cutoff, system.rf_dielectric, ctypes.byref(p),
-                          ctypes.byref(r), c(dq), c(ds), c(de), c(dp))
+    ne, pex, _, _, _, keep = _exc(getattr(system, "exceptions", None))
+    lib().oracle_atm_grad_ex(n, px, pd, pq, ps, pe, pb, system.cutoff, system.rf_dielectric, ne, pex, ctypes.byref(p),
+                             ctypes.byref(r), c(dq), c(ds), c(de), c(dp))
     return {"dq": dq, "dsigma": ds, "deps": de, "dparams": dp}
 
 

@@ -339,3 +339,135 @@ def test_multithreaded_cpu_port_agrees_with_oracle(threads):
         assert abs(got[k] - ref[k]) <= 1e-6 * abs(ref[k]), (k, got[k], ref[k])
     assert abs(got["npairs1"] - ref["npairs1"]) <= 4 and abs(got["npairs2"] - ref["npairs2"]) <= 4
     assert np.linalg.norm(F - Fref) / np.linalg.norm(Fref) < 1e-5
+
+
+# ---------------------------------------------------------------- exceptions
+
+def _ex(pairs, qq=None, sig=None, eps=None):
+    pairs = np.asarray(pairs, dtype=np.int32).reshape(-1, 2)
+    m = pairs.shape[0]
+    z = np.zeros(m)
+    return W.Exceptions(pairs, z if qq is None else np.asarray(qq, float), z if sig is None else np.asarray(sig, float),
+                        z if eps is None else np.asarray(eps, float))
+
+
+def _mol(n=300, box=2.8, seed=21, **kw):
+    s = W.tiny_system(n, box, 0.9, 6, (0.7, 0.5, 0.3), seed)
+    return W.with_molecules(s, **kw)
+
+
+def test_exception_pairs_leave_the_pair_set():
+    """Pair sets with exceptions: brute force = cell list = the plain pair set
+    minus the listed pairs (set difference in numpy)."""
+    s = _mol(size=4, crosslink_every=7)
+    ex = s.exceptions
+    keys = {(min(i, j), max(i, j)) for i, j in ex.pairs}
+    full = oracle.pairs(s.positions, s.box, s.cutoff)
+    want = np.array([p for p in map(tuple, full) if p not in keys], dtype=np.int32)
+    assert len(want) < len(full)  # some listed pairs are inside the cutoff
+    got = oracle.pairs(s.positions, s.box, s.cutoff, exceptions=ex)
+    got_b = oracle.pairs(s.positions, s.box, s.cutoff, brute=True, exceptions=ex)
+    assert np.array_equal(got, want) and np.array_equal(got_b, want)
+    # the crosslinks reach beyond a 31-atom index window
+    assert (np.abs(ex.pairs[:, 1] - ex.pairs[:, 0]) > 31).any()
+
+
+def test_exclusions_remove_exactly_their_pair_terms():
+    """E and F with exclusions only = E and F without exceptions minus, for every
+    excluded pair inside the cutoff, the energy and forces of that pair alone (a
+    two-atom evaluation at the same separation)."""
+    s = _mol(size=3, crosslink_every=9)
+    ex = s.exceptions
+    excl = ex.charge_prod == 0
+    exo = _ex(ex.pairs[excl])
+    args = (s.positions, s.charges, s.sigmas, s.epsilons, s.box, s.cutoff, s.rf_dielectric)
+    E0, F0, n0, _ = oracle.nonbonded(*args)
+    E1, F1, n1, _ = oracle.nonbonded(*args, exceptions=exo)
+    dE, dF, inside = 0.0, np.zeros_like(F0), 0
+    for i, j in exo.pairs:
+        d = s.positions[j] - s.positions[i]
+        d -= s.box * np.round(d / s.box)
+        if np.dot(d, d) >= s.cutoff ** 2:
+            continue
+        inside += 1
+        x2 = np.array([[1.0, 1.0, 1.0], 1.0 + d])
+        e, f, _, _ = oracle.nonbonded(x2, s.charges[[i, j]], s.sigmas[[i, j]], s.epsilons[[i, j]], s.box, s.cutoff,
+                                      s.rf_dielectric)
+        dE += e
+        dF[i] += f[0]
+        dF[j] += f[1]
+    assert inside > 50
+    assert n0 - n1 == inside
+    assert E1 == pytest.approx(E0 - dE, rel=1e-11, abs=1e-9)
+    assert np.allclose(F1, F0 - dF, rtol=1e-9, atol=1e-7)
+
+
+def test_exception_term_special_values():
+    """One exception pair, in and beyond the cutoff: at r = sigma' the LJ part
+    vanishes (E = k_e q'/r); at r = 2^(1/6) sigma' with q' = 0 it is the minimum
+    -eps' with zero force; the reaction-field pair term is gone either way."""
+    ke = 138.935456
+    for r in (0.35, 1.5):                          # inside and beyond rc = 1
+        x = np.array([[1.0, 1.0, 1.0], [1.0 + r, 1.0, 1.0]])
+        q, sg, ep = [0.8, -0.6], [0.3, 0.3], [0.5, 0.5]
+        E, F, n, parts = oracle.nonbonded(x, q, sg, ep, BIG, 1.0, 78.5, exceptions=_ex([[0, 1]], [0.25], [r], [0.7]))
+        assert n == 0
+        assert E == pytest.approx(ke * 0.25 / r, rel=1e-13)
+        assert parts[3] == pytest.approx(E, rel=1e-15)
+        rmin = 2 ** (1 / 6) * 0.31
+        x = np.array([[1.0, 1.0, 1.0], [1.0 + rmin, 1.0, 1.0]])
+        E, F, n, _ = oracle.nonbonded(x, q, sg, ep, BIG, 1.0, 78.5, exceptions=_ex([[1, 0]], [0.0], [0.31], [0.7]))
+        assert E == pytest.approx(-0.7, rel=1e-12)
+        assert np.abs(F).max() < 1e-9
+    # an exclusion inside the cutoff leaves nothing at all
+    x = np.array([[1.0, 1.0, 1.0], [1.3, 1.0, 1.0]])
+    E, F, n, _ = oracle.nonbonded(x, q, sg, ep, BIG, 1.0, 78.5, exceptions=_ex([[0, 1]]))
+    assert n == 0 and E == 0.0 and not F.any()
+
+
+def test_exception_term_uses_the_plain_distance():
+    """Atoms 0.4 nm apart through the periodic boundary but L - 0.4 apart in the
+    given coordinates: the exception term uses L - 0.4 (no periodic image)."""
+    ke = 138.935456
+    L = 3.0
+    x = np.array([[0.2, 1.0, 1.0], [L - 0.2, 1.0, 1.0]])
+    E, _, n, _ = oracle.nonbonded(x, [0.0, 0.0], [0.3, 0.3], [0.0, 0.0], np.array([L, L, L]), 1.0, 78.5,
+                                  exceptions=_ex([[0, 1]], [0.5], [0.0], [0.0]))
+    assert n == 0
+    assert E == pytest.approx(ke * 0.5 / (L - 0.4), rel=1e-13)
+
+
+def test_forces_with_exceptions_are_minus_gradient():
+    s = _mol(size=4, crosslink_every=5, seed=23)
+    ex = s.exceptions
+    x = s.positions.copy()
+    args = (s.charges, s.sigmas, s.epsilons, s.box, s.cutoff, s.rf_dielectric)
+    E0, F, _, parts = oracle.nonbonded(x, *args, exceptions=ex)
+    assert parts[3] != 0.0
+    rng = np.random.default_rng(5)
+    h = 1e-6
+    scaled_atoms = np.unique(ex.pairs[ex.charge_prod != 0])
+    for a in list(rng.choice(len(x), 3, replace=False)) + list(scaled_atoms[:3]):
+        for k in range(3):
+            xp, xm = x.copy(), x.copy()
+            xp[a, k] += h
+            xm[a, k] -= h
+            Ep = oracle.nonbonded(xp, *args, forces=False, exceptions=ex)[0]
+            Em = oracle.nonbonded(xm, *args, forces=False, exceptions=ex)[0]
+            assert -(Ep - Em) / (2 * h) == pytest.approx(F[a, k], rel=1e-5, abs=1e-4)
+    assert np.abs(F.sum(0)).max() < 1e-8 * np.abs(F).sum()
+
+
+def test_atm_gradient_with_exceptions_against_central_differences():
+    import dataclasses
+    p = W.ATMParams(0.15, 0.45, 0.08, 30.0, 1.0, 200.0, 100.0, 0.0625, 1.0)
+    s = W.with_molecules(W.tiny_system(250, 3.2, 0.9, 8, (1.1, 0.7, 0.45), 5, p), size=4, crosslink_every=6)
+    r, _ = oracle.atm_eval(s, forces=False)
+    r0, _ = oracle.atm_eval(dataclasses.replace(s, exceptions=None), forces=False)
+    assert r["U1"] != r0["U1"] and r["npairs1"] < r0["npairs1"]
+    g = oracle.atm_grad(s)
+    for field, key, h in (("charges", "dq", 1e-6), ("sigmas", "dsigma", 1e-7), ("epsilons", "deps", 1e-7)):
+        for i in (0, 3, 77, 8):
+            wp = oracle.atm_eval(_perturbed(s, field, i, h), forces=False)[0]["W"]
+            wm = oracle.atm_eval(_perturbed(s, field, i, -h), forces=False)[0]["W"]
+            assert g[key][i] == pytest.approx((wp - wm) / (2 * h), rel=1e-5, abs=1e-6), (field, i)
