@@ -465,7 +465,8 @@ def main():
     tuning = batch.tuning()
     if args.tuning:
         f = args.tuning.split(",")
-        batch.set_tuning(float(f[0]), int(f[1]), int(f[2]), int(f[3]) if len(f) > 3 else 0)
+        batch.set_tuning(float(f[0]), int(f[1]), int(f[2]), int(f[3]) if len(f) > 3 else 0,
+                         pair_split=int(f[4]) if len(f) > 4 else 1)
         batch.compute()
         tuning = batch.tuning()
     elif not args.no_autotune:

@@ -46,7 +46,7 @@
 extern "C" {
 #endif
 
-#define ATM_MI355X_ABI_VERSION 4
+#define ATM_MI355X_ABI_VERSION 5
 
 /* status codes */
 #define ATM_OK 0
@@ -122,6 +122,10 @@ typedef struct atm_tuning {
     int32_t list_capacity;    /* neighbour-list entries per group to start from; 0 = sized
                                  from the density.  A list that overflows grows it (the
                                  evaluation reports ATM_ERR_CAPACITY and is re-run) */
+    int32_t pair_split;       /* waves sharing one group's list in the tile kernel (variant 0
+                                 and FP32 pairs): 1, 2, 4 or 8; more, shorter waves shorten
+                                 the end of the launch where few waves are left */
+    int32_t reserved_;        /* 0 */
 } atm_tuning;
 
 /* Library version string and ABI number. */

@@ -1,7 +1,7 @@
 """Minimal OpenMM-compatible classes (the subset the ATM force is used with).
 
 Names follow the `openmm` package so scripts read the same: System,
-NonbondedForce (CutoffPeriodic reaction field, or PME with exceptions),
+NonbondedForce (CutoffPeriodic reaction field or PME, with exceptions),
 HarmonicBondForce, HarmonicAngleForce, PeriodicTorsionForce,
 MonteCarloBarostat (a holder of its two global parameters), Context, State,
 Platform, VerletIntegrator, LangevinIntegrator (OpenMM's leapfrog Langevin),

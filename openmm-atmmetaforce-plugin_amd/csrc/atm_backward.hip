@@ -87,12 +87,12 @@ hipError_t launch_set_particles(const DevPtrs& P, const double* q, const double*
 }
 
 hipError_t launch_backward(const DevPtrs& P, long long totalSlots, int setBegin, int nsets, int maxGroups,
-                           const double* wset, hipStream_t st) {
+                           const double* wset, int split, hipStream_t st) {
     const unsigned blocks = (unsigned)std::min<long long>(cdivb(totalSlots, 256), 4096);
     k_fill_f64<<<blocks, 256, 0, st>>>(P.gq, totalSlots);
     k_fill_f64<<<blocks, 256, 0, st>>>(P.gsig, totalSlots);
     k_fill_f64<<<blocks, 256, 0, st>>>(P.geps, totalSlots);
-    return launch_backward_pairs(P, setBegin, nsets, maxGroups, wset, st);
+    return launch_backward_pairs(P, setBegin, nsets, maxGroups, wset, split, st);
 }
 
 hipError_t launch_backward_gather(const DevPtrs& P, int sysBegin, int nsys, int maxAtoms, double* dq, double* dsig,

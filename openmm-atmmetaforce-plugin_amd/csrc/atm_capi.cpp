@@ -24,7 +24,7 @@
 namespace atm {
 // defined in atm_backward.hip
 hipError_t launch_backward(const DevPtrs& P, long long totalSlots, int setBegin, int nsets, int maxGroups,
-                           const double* wset, hipStream_t st);
+                           const double* wset, int split, hipStream_t st);
 hipError_t launch_backward_gather(const DevPtrs& P, int sysBegin, int nsys, int maxAtoms, double* dq, double* dsig,
                                   double* deps, hipStream_t st);
 hipError_t launch_set_positions(DevPtrs P, float4* posq, const float* xyz, long long n, hipStream_t st);
@@ -427,7 +427,8 @@ bool uses_atom_lists(const Context& C, int flags) {
 }
 
 int energy_stride(const Context& C, int flags) {
-    return pair_energy_stride(effective_pair_variant(C.P, C.tuning.pairKernel), (flags & ATM_EVAL_FP32_PAIRS) == 0);
+    return pair_energy_stride(effective_pair_variant(C.P, C.tuning.pairKernel), (flags & ATM_EVAL_FP32_PAIRS) == 0,
+                              C.tuning.pairSplit);
 }
 
 void require_ready(const Context& C) {
@@ -463,7 +464,7 @@ void enqueue_batched(Context& C, int flags, hipEvent_t* ev = nullptr) {
     if (uses_atom_lists(C, flags)) HIPCHK(launch_alist(P, 0, S2, C.maxGroups, st));
     if (ev) HIPCHK(hipEventRecord(ev[2], st));
     HIPCHK(launch_pairs(P, 0, S2, C.maxGroups, C.tuning.pairWaves, (flags & ATM_EVAL_FORCES) != 0,
-                        (flags & ATM_EVAL_FP32_PAIRS) == 0, C.tuning.pairKernel, st));
+                        (flags & ATM_EVAL_FP32_PAIRS) == 0, C.tuning.pairKernel, C.tuning.pairSplit, st));
     if (has_exception_terms(C)) HIPCHK(launch_exceptions(P, 0, S2, (flags & ATM_EVAL_FORCES) != 0, st));
     if (ev) HIPCHK(hipEventRecord(ev[3], st));
     HIPCHK(launch_energy(P, 0, C.nsys, energy_stride(C, flags), st));
@@ -525,7 +526,7 @@ void eval_plain(Context& C, int flags) {
             HIPCHK(launch_pairlist(P, c, 1, D.groupCap, 4, st));
             if (uses_atom_lists(C, flags)) HIPCHK(launch_alist(P, c, 1, D.groupCap, st));
             HIPCHK(launch_pairs(P, c, 1, D.groupCap, C.tuning.pairWaves, true, (flags & ATM_EVAL_FP32_PAIRS) == 0,
-                                C.tuning.pairKernel, st));
+                                C.tuning.pairKernel, C.tuning.pairSplit, st));
             if (D.excCount > 0) HIPCHK(launch_exceptions(P, c, 1, true, st));
             HIPCHK(launch_set_energy(P, c, energy_stride(C, flags), st));
             HIPCHK(hipMemcpyAsync(&U[stt], P.setEnergy + c, sizeof(double), hipMemcpyDeviceToHost, st));
@@ -558,7 +559,7 @@ void eval_state1(Context& C, int flags) {
         HIPCHK(launch_pairlist(P, c, 1, D.groupCap, 4, st));
         if (uses_atom_lists(C, flags)) HIPCHK(launch_alist(P, c, 1, D.groupCap, st));
         HIPCHK(launch_pairs(P, c, 1, D.groupCap, C.tuning.pairWaves, (flags & ATM_EVAL_FORCES) != 0,
-                            (flags & ATM_EVAL_FP32_PAIRS) == 0, C.tuning.pairKernel, st));
+                            (flags & ATM_EVAL_FP32_PAIRS) == 0, C.tuning.pairKernel, C.tuning.pairSplit, st));
         if (D.excCount > 0) HIPCHK(launch_exceptions(P, c, 1, (flags & ATM_EVAL_FORCES) != 0, st));
         HIPCHK(launch_set_energy(P, c, energy_stride(C, flags), st));
         double U = 0.0;
@@ -572,12 +573,13 @@ void eval_state1(Context& C, int flags) {
     }
 }
 
-// The first call after an evaluation checks the list-overflow flags and copies
-// the results; later calls return the copies (every evaluation clears resultsValid).
+// Waits for the context stream.  The first call after an evaluation also checks
+// the list-overflow flags and copies the results; later calls use the copies
+// (every evaluation clears resultsValid).
 void fetch_results(Context& C) {
     if (!C.evaluated) fail(ATM_ERR_STATE, "no evaluation has been run");
-    if (C.resultsValid) return;
     HIPCHK(hipStreamSynchronize(C.stream));
+    if (C.resultsValid) return;
     std::vector<SetRuntime> rt(2 * C.nsys);
     to_host(C, rt.data(), C.dRt.p, sizeof(SetRuntime) * rt.size());
     bool overflow = false;
@@ -1101,6 +1103,8 @@ int atm_get_tuning(void* context, atm_tuning* out) {
         out->best_ms = C.tunedMs;
         out->pair_kernel = C.tuning.pairKernel;
         out->list_capacity = C.tuning.listCapacity;
+        out->pair_split = C.tuning.pairSplit;
+        out->reserved_ = 0;
     });
 }
 
@@ -1118,6 +1122,8 @@ int atm_set_tuning(void* context, const atm_tuning* t) {
         if (t->pair_kernel != 0 && t->pair_waves_per_block == 8)
             fail(ATM_ERR_INVALID_ARGUMENT, "pair kernels 1-6 take 1, 2 or 4 waves per block");
         if (t->list_capacity < 0) fail(ATM_ERR_INVALID_ARGUMENT, "list_capacity must be >= 0");
+        if (t->pair_split != 1 && t->pair_split != 2 && t->pair_split != 4 && t->pair_split != 8)
+            fail(ATM_ERR_INVALID_ARGUMENT, "pair_split must be 1, 2, 4 or 8");
         HIPCHK(hipSetDevice(C.device));
         HIPCHK(hipStreamSynchronize(C.stream));
         const bool relayout = t->column_scale != C.tuning.colScale || t->zbins_per_column != C.tuning.zbinsPerColumn ||
@@ -1127,6 +1133,7 @@ int atm_set_tuning(void* context, const atm_tuning* t) {
         C.tuning.zbinsPerColumn = t->zbins_per_column;
         C.tuning.pairWaves = t->pair_waves_per_block;
         C.tuning.pairKernel = t->pair_kernel;
+        C.tuning.pairSplit = t->pair_split;
         if (t->list_capacity != C.tuning.listCapacity) {
             C.entryCapOverride.assign(2 * C.nsys, 0);
             C.alistCapOverride.assign(2 * C.nsys, 0);
@@ -1153,7 +1160,10 @@ int atm_autotune(void* context, int32_t repeats, atm_tuning* out) {
         const int waves[] = {1, 2, 4};
         // (the atom-list kernel, 5, is left out: it measured slower than the tile kernel on every
         // column scale tried, and its lists need their own buffers)
-        const int kernels[] = {0, 1, kPairKernelFiltered};
+        // (kernel, group split): the tile kernel with its lists split 1, 2 or 4 ways,
+        // the compacted kernel 1 and the filtered kernel
+        const int kernels[][2] = {{0, 1}, {0, 2}, {0, 4}, {1, 1}, {kPairKernelFiltered, 1}};
+        const int nk = (int)(sizeof(kernels) / sizeof(kernels[0]));
         const int zpcs[] = {2, 4};
         const Tuning orig = C.tuning;
         hipEvent_t e0, e1;
@@ -1204,8 +1214,9 @@ int atm_autotune(void* context, int32_t repeats, atm_tuning* out) {
         for (double sc : scales) {
             for (int zp : zpcs) {
                 set_layout(sc, zp);
-                for (int kv = 0; kv < 3 * 3; ++kv) {
-                    C.tuning.pairKernel = kernels[kv / 3];
+                for (int kv = 0; kv < nk * 3; ++kv) {
+                    C.tuning.pairKernel = kernels[kv / 3][0];
+                    C.tuning.pairSplit = kernels[kv / 3][1];
                     C.tuning.pairWaves = waves[kv % 3];
                     const double per = time_config(reps);
                     if (per < 1e300) timed.emplace_back(per, C.tuning);
@@ -1243,6 +1254,8 @@ int atm_autotune(void* context, int32_t repeats, atm_tuning* out) {
             out->best_ms = bestMs;
             out->pair_kernel = best.pairKernel;
             out->list_capacity = best.listCapacity;
+            out->pair_split = best.pairSplit;
+            out->reserved_ = 0;
         }
     });
 }
@@ -1313,7 +1326,8 @@ void backward_impl(Context& C, const double* grad_energy, double* d_charges, dou
         }
         if (C.dWset.bytes < sizeof(double) * 2 * S) C.dWset.alloc(sizeof(double) * 2 * S);
         HIPCHK(hipMemcpyAsync(C.dWset.p, wset.data(), sizeof(double) * 2 * S, hipMemcpyHostToDevice, C.stream));
-        HIPCHK(launch_backward(C.P, C.totalSlots, 0, 2 * S, C.maxGroups, C.dWset.as<double>(), C.stream));
+        HIPCHK(launch_backward(C.P, C.totalSlots, 0, 2 * S, C.maxGroups, C.dWset.as<double>(), C.tuning.pairSplit,
+                               C.stream));
         const size_t nb = sizeof(double) * C.totalAtoms;
         double *oq = d_charges, *os = d_sigmas, *oe = d_epsilons;
         if (!deviceOut) {

@@ -85,10 +85,15 @@ constexpr int kPairKernelCount = 7;
 // A batch with exceptions runs the tile kernel whatever the variant asked for:
 // only it applies the exclusion masks.
 inline int effective_pair_variant(const DevPtrs& P, int variant) { return P.xmask ? 0 : variant; }
+// split (1-8): the tile kernel divides each group's list between that many waves.
 hipError_t launch_pairs(const DevPtrs& P, int setBegin, int nsets, int maxGroups, int waves, bool forces, bool fp64,
-                        int variant, hipStream_t st);
-// Energy slots the pair kernel fills per group in genergy (1, or 8 for the filtered kernel).
-inline int pair_energy_stride(int variant, bool fp64) { return (fp64 && variant == kPairKernelFiltered) ? 8 : 1; }
+                        int variant, int split, hipStream_t st);
+// Energy slots the pair kernel fills per group in genergy: 8 for the filtered
+// kernel, `split` for the tile kernel, 1 for the others.
+inline int pair_energy_stride(int variant, bool fp64, int split) {
+    if (fp64 && variant == kPairKernelFiltered) return 8;
+    return (variant == 0 || !fp64) ? split : 1;
+}
 // Exception terms of sets [setBegin, setBegin + nsets): energy into xenergy[c], with
 // `forces` their forces added to the set's slot sums (call after the pair kernel
 // and before launch_energy / launch_set_energy, which add xenergy when excCount > 0).
@@ -98,7 +103,7 @@ hipError_t launch_set_energy(const DevPtrs& P, int c, int estride, hipStream_t s
 // Backward pass of the pair terms (the tile kernel's backward mode): adds wset[c] dE/d(q', h, e')
 // of every in-cutoff pair of sets [setBegin, setBegin + nsets) to gq, gsig, geps (slot order).
 hipError_t launch_backward_pairs(const DevPtrs& P, int setBegin, int nsets, int maxGroups, const double* wset,
-                                 hipStream_t st);
+                                 int split, hipStream_t st);
 // clear: write the blended force instead of adding it to outF.
 hipError_t launch_blend(const DevPtrs& P, int sysBegin, int nsys, int maxAtoms, int hostMode, double sp, double dir,
                         bool clear, hipStream_t st);

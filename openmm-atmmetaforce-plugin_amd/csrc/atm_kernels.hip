@@ -92,6 +92,21 @@ __device__ __forceinline__ float oct_sum(float v) {
     return v;
 }
 
+// x, y and z summed over each aligned group of 8 lanes, scattered: lanes 0 and 7
+// of the group return the x sum, 1 and 6 the y sum, 2 and 5 the z sum (3 and 4:
+// partial sums).  Pairs and quads of all three, then one half-row mirror of the
+// value each lane keeps; selX / selY: this lane keeps x / y.
+__device__ __forceinline__ float oct_sum3(float x, float y, float z, bool selX, bool selY) {
+    x += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), 0xB1, 0xF, 0xF, true));
+    y += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, y), 0xB1, 0xF, 0xF, true));
+    z += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, z), 0xB1, 0xF, 0xF, true));
+    x += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), 0x4E, 0xF, 0xF, true));
+    y += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, y), 0x4E, 0xF, 0xF, true));
+    z += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, z), 0x4E, 0xF, 0xF, true));
+    const float v = selX ? x : (selY ? y : z);
+    return v + __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x141, 0xF, 0xF, true));
+}
+
 // Sum over the 8 lanes {l & 7 + 8 k}: the same i-atom across the 8 j-columns.
 __device__ __forceinline__ float stride8_sum(float v) {
     v += __shfl_xor(v, 8, 64);
@@ -698,12 +713,16 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Fixed-order sum of a set's energy slots: one per group, or (estride 8) one
-// per i-cluster wave, 8 per group.
+// Fixed-order sum of a set's energy slots: one per group, or estride per group
+// (8: one per i-cluster wave of the filtered kernel; 2 or 4: one per part of a
+// group whose list the tile kernel splits between waves).
 __device__ __forceinline__ double group_energy(const DevPtrs& P, const SetDesc& D, int g, int estride) {
     if (estride == 1) return P.genergy[D.groupOff + g];
-    const double* e = P.genergy + 8ll * (D.groupOff + g);
-    return ((e[0] + e[1]) + (e[2] + e[3])) + ((e[4] + e[5]) + (e[6] + e[7]));
+    const double* e = P.genergy + (long long)estride * (D.groupOff + g);
+    if (estride == 8) return ((e[0] + e[1]) + (e[2] + e[3])) + ((e[4] + e[5]) + (e[6] + e[7]));
+    double s = e[0];
+    for (int k = 1; k < estride; ++k) s += e[k];
+    return s;
 }
 
 
@@ -716,10 +735,12 @@ __device__ __forceinline__ void flush_component_u(const DevPtrs& P, int c, float
     const bool small = fabsf(f) < fixMax;
     long long* dst = P.sf + (long long)k * P.sfn + slot;
     const long long v = small ? __double_as_longlong((double)f * kForceScale + 6755399441055744.0) - 0x4338000000000000LL : 0ll;
-    if (v != 0) atomic_add_fixed(dst, v);
+    atomic_add_fixed(dst, v);  // zeros too: a data-dependent skip would make the count unknown
     if (!small) {
-        atomicAdd(P.sd + 3ll * slot + k, (double)f);
-        mark_side(P, c, slot);
+        int sl = slot;
+        asm volatile("" : "+v"(sl));  // keeps the rare path's address math from being hoisted
+        atomicAdd(P.sd + 3ll * sl + k, (double)f);
+        mark_side(P, c, sl);
     }
 }
 
@@ -750,15 +771,21 @@ __device__ __forceinline__ void flush_component_u(const DevPtrs& P, int c, float
 // EXC: the batch has exceptions; every in-cutoff pair is tested against the
 // i-atom's near mask (and, for far partners, its list) and excepted pairs are
 // dropped before the pair function.
+//
+// split: each group's list is divided into `split` consecutive parts, one per
+// wave (blockIdx.x = group block * split + part): shorter waves, so the launch
+// ends with less of a tail when the group lists are long.  Each part adds its
+// own i-force sums and writes its own energy slot (split slots per group).
 template <int W, bool FORCES, bool F64, bool BWD, bool EXC>
-__global__ void __launch_bounds__(64 * W, 6) k_pairs(DevPtrs P, int setBegin, const double* wset) {
+__global__ void __launch_bounds__(64 * W, 6) k_pairs(DevPtrs P, int setBegin, const double* wset, int split) {
     static_assert(!BWD || (F64 && !FORCES), "the backward mode is FP64 and computes no forces");
     constexpr bool JSUMS = FORCES || BWD;  // per-atom j sums flushed every kJFlush entries
     const int c = setBegin + blockIdx.y;
     const SetDesc D = P.desc[c];  // by value: lives in SGPRs for the whole kernel
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int jj = lane >> 3, ii = lane & 7;
-    const int g = blockIdx.x * W + wave;
+    const int part = blockIdx.x % split;
+    const int g = (blockIdx.x / split) * W + wave;
     const bool active = g < P.rt[c].ngroups;
     __shared__ float4 sPos[W][64];
     __shared__ float2 sPrm[W][F64 ? 1 : 64];
@@ -817,8 +844,9 @@ __global__ void __launch_bounds__(64 * W, 6) k_pairs(DevPtrs P, int setBegin, co
     for (int t = 0; t < kGroupClusters; ++t) fi[t][0] = fi[t][1] = fi[t][2] = 0.f;
     double eE = 0.0;  // this lane's sum of whole pair energies
     uint32_t npair = 0;  // in-cutoff pairs evaluated by this wave (wave-uniform)
-    const int ne = __builtin_amdgcn_readfirstlane(P.gcount[D.groupOff + g]);
-    const uint32_t* E = P.entries + D.entryOff + (long long)g * D.entryCap;
+    const int nall = __builtin_amdgcn_readfirstlane(P.gcount[D.groupOff + g]);
+    const int eb = (int)(((long long)part * nall) / split), ne = (int)(((long long)(part + 1) * nall) / split) - eb;
+    const uint32_t* E = P.entries + D.entryOff + (long long)g * D.entryCap + eb;
     // This lane's j-atom of one list entry.  Two of these are in flight: entry
     // k + 1's loads are issued before entry k is evaluated, into the other
     // buffer, so the wait for them falls after entry k's work (and not behind
@@ -852,26 +880,30 @@ __global__ void __launch_bounds__(64 * W, 6) k_pairs(DevPtrs P, int setBegin, co
         if (farLanes != 0ull && __builtin_amdgcn_inverse_ballot_w64(farLanes)) ex = far_excluded(P, D, ai, aj);
         return __builtin_amdgcn_ballot_w64(ex) & in;
     };
-    // j-force sums of the last kJFlush entries, flushed together: 3 dense rounds
-    // of one atomic per lane instead of one sparse atomic per entry
+    // j-force sums of the last kJFlush entries, flushed together: one dense round
+    // per component, lane 8 e + a adding atom a of entry e (8 consecutive slots,
+    // one cache line, per 8 lanes)
     constexpr int kJFlush = 8;
-    __shared__ float sJF[W][kJFlush][kClusterSize * 3];  // [entry][jj * 3 + component]
+    static_assert(kJFlush * kClusterSize == 64, "one flush round covers the wave");
+    __shared__ float sJF[W][3][kJFlush * kClusterSize];  // [component][entry * 8 + jj]
     __shared__ int sJS[W][kJFlush];                       // the entry's first j-slot
+    // the component each lane keeps after oct_sum3 (x: ii = 0, 7; y: 1, 6; z: 2, 5)
+    const bool selX = ii == 0 || ii == 7, selY = ii == 1 || ii == 6;
     int ecount = 0;
+    // Inside the entry loop every lane issues its three atomics whatever the data
+    // (a full flush, cnt = kJFlush, zeros added as zeros): the loads of the next
+    // entries are then followed by a known number of memory operations, and the
+    // wait for them does not also wait for these atomics.
     auto flush_j = [&](int cnt) __attribute__((always_inline)) {
         wave_sync();  // the octets' writes are visible to every lane of the wave
-        int ln = lane;
-        asm volatile("" : "+v"(ln));  // index math stays here, not hoisted into registers held across the loop
+        if (cnt == kJFlush || (lane >> 3) < cnt) {
+            const int slot = sJS[wave][lane >> 3] + (lane & 7);
 #pragma unroll
-        for (int r = 0; r < 3; ++r) {
-            const int v = r * 64 + ln;  // entry v / 24, then jj * 3 + component
-            const int pe = v / (kClusterSize * 3), rem = v - pe * (kClusterSize * 3);
-            if (pe < cnt) {
-                const int slot = sJS[wave][pe] + rem / 3, k = rem % 3;
-                const float f = sJF[wave][pe][rem];
+            for (int k = 0; k < 3; ++k) {
+                const float f = sJF[wave][k][lane];
                 if (BWD) {
                     double* acc = k == 0 ? P.gq : (k == 1 ? P.gsig : P.geps);
-                    if (f != 0.f) atomicAdd(acc + slot, wgt * (double)f);
+                    atomicAdd(acc + slot, wgt * (double)f);
                 } else {
                     flush_component_u(P, c, fixMax, slot, k, f);
                 }
@@ -974,12 +1006,11 @@ __global__ void __launch_bounds__(64 * W, 6) k_pairs(DevPtrs P, int setBegin, co
             }
         }
         if (JSUMS) {
-            // every lane of a jj octet holds the sums; lanes ii = 0, 1, 2 keep x, y, z
-            // (zero for dummy j-slots, whose accumulators nothing reads)
-            fjx = oct_sum(fjx); fjy = oct_sum(fjy); fjz = oct_sum(fjz);
-            const float fk = ii == 0 ? fjx : (ii == 1 ? fjy : fjz);
+            // lanes ii = 0, 1, 2 of each jj octet keep the x, y, z sums (zero for dummy
+            // j-slots, whose accumulators nothing reads)
+            const float fk = oct_sum3(fjx, fjy, fjz, selX, selY);
             const int pe = ecount & (kJFlush - 1);
-            if (ii < 3) sJF[wave][pe][jj * 3 + ii] = fk;
+            if (ii < 3) sJF[wave][ii][pe * kClusterSize + jj] = fk;
             if (lane == 0) sJS[wave][pe] = js - jj;
             if (pe == kJFlush - 1) flush_j(kJFlush);
         }
@@ -1024,7 +1055,7 @@ __global__ void __launch_bounds__(64 * W, 6) k_pairs(DevPtrs P, int setBegin, co
     double eacc = eE;
     eacc = wave_sum_d(eacc);
     if (lane == 0) {
-        P.genergy[D.groupOff + g] = eacc;
+        P.genergy[(long long)(D.groupOff + g) * split + part] = eacc;
         atomicAdd(&P.rt[c].npairs, (unsigned long long)npair);
     }
 }
@@ -1941,7 +1972,7 @@ hipError_t launch_alist(const DevPtrs& P, int setBegin, int nsets, int maxGroups
 }
 
 hipError_t launch_pairs(const DevPtrs& P, int setBegin, int nsets, int maxGroups, int waves, bool forces, bool fp64,
-                        int variant, hipStream_t st) {
+                        int variant, int split, hipStream_t st) {
     variant = effective_pair_variant(P, variant);
     if (fp64 && variant == kPairKernelFiltered) {
 #define ATM_PAIRSF(WV)                                                                                          \
@@ -1987,9 +2018,12 @@ hipError_t launch_pairs(const DevPtrs& P, int setBegin, int nsets, int maxGroups
 #undef ATM_PAIRSQ
         return hipGetLastError();
     }
-#define ATM_PAIRS_X(WV, F64, X)                                                                                          \
-    if (forces) k_pairs<WV, true, F64, false, X><<<dim3(cdiv(maxGroups, WV), nsets), 64 * WV, 0, st>>>(P, setBegin, nullptr); \
-    else k_pairs<WV, false, F64, false, X><<<dim3(cdiv(maxGroups, WV), nsets), 64 * WV, 0, st>>>(P, setBegin, nullptr);
+    if (split < 1 || split > 8) return hipErrorInvalidValue;
+#define ATM_PAIRS_X(WV, F64, X)                                                                                   \
+    if (forces) k_pairs<WV, true, F64, false, X><<<dim3(cdiv(maxGroups, WV) * split, nsets), 64 * WV, 0, st>>>(   \
+        P, setBegin, nullptr, split);                                                                             \
+    else k_pairs<WV, false, F64, false, X><<<dim3(cdiv(maxGroups, WV) * split, nsets), 64 * WV, 0, st>>>(        \
+        P, setBegin, nullptr, split);
 #define ATM_PAIRS(WV, F64)                                \
     if (P.xmask) { ATM_PAIRS_X(WV, F64, true) } else { ATM_PAIRS_X(WV, F64, false) }
 #define ATM_PAIRS_W(F64)               \
@@ -2017,9 +2051,10 @@ hipError_t launch_set_energy(const DevPtrs& P, int c, int estride, hipStream_t s
 }
 
 hipError_t launch_backward_pairs(const DevPtrs& P, int setBegin, int nsets, int maxGroups, const double* wset,
-                                 hipStream_t st) {
-    if (P.xmask) k_pairs<1, false, true, true, true><<<dim3(maxGroups, nsets), 64, 0, st>>>(P, setBegin, wset);
-    else k_pairs<1, false, true, true, false><<<dim3(maxGroups, nsets), 64, 0, st>>>(P, setBegin, wset);
+                                 int split, hipStream_t st) {
+    if (split < 1 || split > 8) return hipErrorInvalidValue;
+    if (P.xmask) k_pairs<1, false, true, true, true><<<dim3(maxGroups * split, nsets), 64, 0, st>>>(P, setBegin, wset, split);
+    else k_pairs<1, false, true, true, false><<<dim3(maxGroups * split, nsets), 64, 0, st>>>(P, setBegin, wset, split);
     return hipGetLastError();
 }
 

@@ -14,6 +14,7 @@ struct Tuning {
     int zbinsPerColumn = 4;    // fine z-bins per column edge
     int pairKernel = 0;        // pair-kernel variant (launch_pairs)
     int listCapacity = 0;      // list entries per group to start from (0: sized from the density)
+    int pairSplit = 1;         // waves sharing one group's list in the tile kernel (1, 2, 4 or 8)
 };
 
 // Throws AtmError(ATM_ERR_INVALID_ARGUMENT) for an unusable system description.

@@ -1,9 +1,9 @@
 // Minimal OpenMM-compatible core: the subset of NonbondedForce the ATM
 // evaluation supports on the MI355X platform -- per-particle charge, sigma and
-// epsilon with CutoffPeriodic (reaction-field) electrostatics, or PME with
-// exceptions (scaled 1-4 pairs, exclusions) as AMBER systems use it.  The
-// reaction-field kernels reject exceptions; the dispersion correction is
-// included in either method.
+// epsilon with CutoffPeriodic (reaction-field) or PME electrostatics, and
+// exceptions (scaled 1-4 pairs, exclusions) as AMBER systems use them, at the
+// plain (non-periodic) distance; the dispersion correction is included in
+// either method.
 #pragma once
 #include <unordered_map>
 #include <vector>

@@ -46,6 +46,7 @@ private:
     double box[3] = {0, 0, 0};
     double cutoff = 0, rfDielectric = 0;
     double dispersionCoefficient = 0;  // E_disp = coefficient / volume
+    bool haveExceptions = false;       // exceptions were uploaded (atm_set_exceptions)
 };
 
 class MI355XCalcATMMetaForceKernel : public CalcATMMetaForceKernel {

@@ -54,8 +54,22 @@ void validateNonbonded(const NonbondedForce& nb) {
     }
     if (nb.getNonbondedMethod() != NonbondedForce::CutoffPeriodic)
         throw OpenMMException("MI355X platform: NonbondedForce must use CutoffPeriodic (reaction-field) or PME electrostatics");
-    if (nb.getNumExceptions() > 0)
-        throw OpenMMException("MI355X platform: NonbondedForce exceptions are supported with PME only");
+}
+
+// The NonbondedForce's exceptions as the C ABI takes them.
+void exceptionArrays(const NonbondedForce& nb, std::vector<int32_t>& pairs, std::vector<double>& qq,
+                     std::vector<double>& es, std::vector<double>& ee) {
+    const int ne = nb.getNumExceptions();
+    pairs.resize(2 * (size_t)ne);
+    qq.resize(ne);
+    es.resize(ne);
+    ee.resize(ne);
+    for (int k = 0; k < ne; ++k) {
+        int p1, p2;
+        nb.getExceptionParameters(k, p1, p2, qq[k], es[k], ee[k]);
+        pairs[2 * k] = p1;
+        pairs[2 * k + 1] = p2;
+    }
 }
 
 class NonbondedFactory : public KernelFactory {
@@ -222,15 +236,10 @@ void MI355XSystemContext::setPMEParticles(const NonbondedForce& nb, const std::v
     for (int i = 0; i < n; ++i) nb.getParticleParameters(i, q[i], s[i], e[i]);
     const double* dptr = displacements ? reinterpret_cast<const double*>(displacements->data()) : nullptr;
     check(atm_pme_set_particles(pme, q.data(), s.data(), e.data(), dptr), "atm_pme_set_particles");
-    const int ne = nb.getNumExceptions();
-    std::vector<int32_t> pairs(2 * (size_t)ne);
-    std::vector<double> qq(ne), es(ne), ee(ne);
-    for (int k = 0; k < ne; ++k) {
-        int p1, p2;
-        nb.getExceptionParameters(k, p1, p2, qq[k], es[k], ee[k]);
-        pairs[2 * k] = p1;
-        pairs[2 * k + 1] = p2;
-    }
+    std::vector<int32_t> pairs;
+    std::vector<double> qq, es, ee;
+    exceptionArrays(nb, pairs, qq, es, ee);
+    const int ne = (int)qq.size();
     check(atm_pme_set_exceptions(pme, ne, pairs.data(), qq.data(), es.data(), ee.data()), "atm_pme_set_exceptions");
 }
 
@@ -254,6 +263,14 @@ void MI355XSystemContext::setParticles(const NonbondedForce& nb, const std::vect
     for (int i = 0; i < n; ++i) nb.getParticleParameters(i, q[i], s[i], e[i]);
     const double* dptr = displacements ? reinterpret_cast<const double*>(displacements->data()) : nullptr;
     check(atm_set_particles(ctx, 0, q.data(), s.data(), e.data(), dptr), "atm_set_particles");
+    if (nb.getNumExceptions() > 0 || haveExceptions) {  // (re)lays out the context: only when there are any
+        std::vector<int32_t> pairs;
+        std::vector<double> qq, es, ee;
+        exceptionArrays(nb, pairs, qq, es, ee);
+        check(atm_set_exceptions(ctx, 0, (int)qq.size(), pairs.data(), qq.data(), es.data(), ee.data()),
+              "atm_set_exceptions");
+        haveExceptions = !qq.empty();
+    }
     // Long-range dispersion correction: E = 8 pi N^2 / V * (<eps s^12>/(9 rc^9) - <eps s^6>/(3 rc^3)),
     // averaged over unordered pairs i <= j of particle classes with equal (sigma, epsilon).
     dispersionCoefficient = 0.0;

@@ -33,6 +33,10 @@ def build(s, atm_group=2, nb_group=1, dispersion=False, method=None, add_all=Tru
     nb.setUseDispersionCorrection(dispersion)
     for q, sg, e in zip(s.charges, s.sigmas, s.epsilons):
         nb.addParticle(float(q), float(sg), float(e))
+    if s.exceptions is not None:
+        ex = s.exceptions
+        for (i, j), qq, sg, e in zip(ex.pairs, ex.charge_prod, ex.sigma, ex.epsilon):
+            nb.addException(int(i), int(j), float(qq), float(sg), float(e))
     nb.setForceGroup(nb_group)
     system.addForce(nb)
     p = s.params
@@ -65,6 +69,24 @@ def frms(F, R):
 
 SYS = W.tiny_system(3000, 3.1, 0.9, 30, (1.03, 0.77, 0.62), 1,
                     W.ATMParams(0.25, 0.45, 0.1, 110.0, 2.0, 200.0, 100.0, 0.0625, 1.0))
+
+
+@pytest.mark.parametrize("props", [{}, {"ATMEvaluation": "plain"}])
+def test_reaction_field_exceptions(gpu, props):
+    """NonbondedForce.addException on the reaction-field path: exclusions and
+    scaled pairs of molecules of 4 atoms and of index-distant crosslinks."""
+    s = W.with_molecules(SYS, size=4, crosslink_every=9)
+    ref, Fref = oracle.atm_eval(s)
+    system, f, integ, ctx = context_for(s, props)
+    st = ctx.getState(getEnergy=True, getForces=True, groups={2})
+    print(props, st.getPotentialEnergy(), ref["W"], flush=True)
+    assert rel(st.getPotentialEnergy(), ref["W"]) < E_TOL
+    assert frms(st.getForces(asNumpy=True), Fref) < F_TOL
+    assert rel(f.getPerturbationEnergy(ctx), ref["usc"]) < E_TOL
+    st1 = ctx.getState(getEnergy=True, groups={1})  # the NonbondedForce alone at x
+    e1 = oracle.nonbonded(s.positions, s.charges, s.sigmas, s.epsilons, s.box, s.cutoff, s.rf_dielectric,
+                          forces=False, exceptions=s.exceptions)[0]
+    assert rel(st1.getPotentialEnergy(), e1) < E_TOL
 
 
 @pytest.mark.parametrize("props", [{}, {"ATMEvaluation": "plain"}, {"Precision": "double"}])

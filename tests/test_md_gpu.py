@@ -113,7 +113,7 @@ def _md_pair(s, m, steps, capacity_from_init=False):
             cap0 = int(re.batch.list_stats()["max_entries"].max())
             t = re.batch.tuning()
             re.batch.set_tuning(t["column_scale"], t["pair_waves_per_block"], t["zbins_per_column"],
-                                t["pair_kernel"], list_capacity=cap0)
+                                t["pair_kernel"], list_capacity=cap0, pair_split=t["pair_split"])
         re.step(steps)
         x = re.state(0)[0]
         cap1 = int(re.batch.list_stats()["capacity"].max())
