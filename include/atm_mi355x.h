@@ -77,6 +77,13 @@ extern "C" {
 #define ATM_EVAL_STATE1_ONLY   0x20 /* plain cutoff LJ + reaction-field evaluation at x:
                                        energy = state1_energy = U1, forces += F1; no ATM
                                        combine (a NonbondedForce outside the ATM force) */
+#define ATM_EVAL_PARAM_GRADS   0x40 /* energies (no forces) of the batched FP64 path, and in
+                                       the same pass over the pairs dU1/dp and dU2/dp for
+                                       the per-atom charges, sigmas and epsilons p: a
+                                       following atm_backward / atm_backward_device only
+                                       weights and gathers them (no second pair pass).
+                                       Not with ATM_EVAL_FORCES, ATM_EVAL_FP32_PAIRS,
+                                       ATM_EVAL_PLAIN or ATM_EVAL_STATE1_ONLY */
 
 /* One system of the batch.  Box edges must satisfy cutoff <= box/2. */
 typedef struct atm_system_desc {
@@ -233,7 +240,9 @@ int atm_get_pairs(void* context, int32_t sys, int32_t state, int32_t* pairs,
  * epsilons (each sum_s num_atoms doubles, systems concatenated) and the
  * per-system softplus parameters (num_systems x 5 doubles in the order
  * lambda1, lambda2, alpha, u0, w0).  Recomputes the pair terms at the last
- * evaluated positions; any output pointer may be NULL. */
+ * evaluated positions, unless that evaluation ran with ATM_EVAL_PARAM_GRADS
+ * (then it only weights the stored per-state derivatives); any output pointer
+ * may be NULL. */
 int atm_backward(void* context, const double* grad_energy, double* d_charges,
                  double* d_sigmas, double* d_epsilons, double* d_params);
 

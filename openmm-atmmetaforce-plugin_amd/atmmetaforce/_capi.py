@@ -24,6 +24,7 @@ ATM_EVAL_PLAIN = 0x4
 ATM_EVAL_NO_GRAPH = 0x8
 ATM_EVAL_FP32_PAIRS = 0x10
 ATM_EVAL_STATE1_ONLY = 0x20
+ATM_EVAL_PARAM_GRADS = 0x40
 
 FORCE_SCALE = 16777216.0
 ONE_4PI_EPS0 = 138.935456

@@ -10,13 +10,15 @@ alpha, u0, w0) -> W (num_systems,) float64.
 The forward sets the parameters and runs one evaluation of every system on
 the GPU.  Per-atom tensors on a GPU go to the library as device arrays
 (atm_set_particles_device) and their gradients come back as device arrays
-(atm_backward_device): no per-atom data crosses to the host.  The backward
-runs the hand-written backward kernel, which walks the forward's lists again
-and recomputes every in-cutoff pair term at x and at x + d, weighted by
-dW/dU1 and dW/dU2 -- nothing per pair is stored between the passes -- and the
-closed-form derivatives of the softplus in the five scalar parameters.  Pair
-terms of the backward are FP64 from the exact r^2, their per-lane sums FP32,
-their per-atom sums FP64.
+(atm_backward_device): no per-atom data crosses to the host.  When a per-atom
+tensor requires a gradient, the forward evaluates with ATM_EVAL_PARAM_GRADS:
+the pair kernel's backward mode computes the energies and, in the same pass
+over the pairs, dU1/dp and dU2/dp per atom (p = charge, sigma, epsilon); the
+backward then only weights them by dW/dU1 and dW/dU2 and adds the closed-form
+derivatives of the softplus in the five scalar parameters.  Otherwise the
+forward is an energy-only evaluation and the backward makes one weighted pass
+over the forward's lists.  Pair terms are FP64 from the exact r^2, their
+per-lane sums FP32, their per-atom sums FP64.
 
 If the batch was evaluated again between a forward and its backward, the
 backward re-runs that forward's evaluation first: at the forward's parameters
@@ -56,7 +58,8 @@ class ATMEnergyFunction(torch.autograd.Function):
             atoms = (_dev64(charges), _dev64(sigmas), _dev64(epsilons))
         else:
             atoms = (_np(charges), _np(sigmas), _np(epsilons))
-        state = (on_gpu, atoms, [_np(t) for t in scal])
+        grads = any(t.requires_grad for t in (charges, sigmas, epsilons))
+        state = (on_gpu, atoms, [_np(t) for t in scal], grads)
         _upload(batch, state)
         W = np.array([batch.result(k)["energy"] for k in range(S)])
         ctx.batch = batch
@@ -84,14 +87,17 @@ class ATMEnergyFunction(torch.autograd.Function):
             r = batch.backward(g)
             outs_atoms = [torch.as_tensor(r["dq"]), torch.as_tensor(r["dsigma"]), torch.as_tensor(r["deps"])]
             dparams = r["dparams"]
-        outs = outs_atoms + [torch.as_tensor(dparams[:, k]) for k in range(5)]
+        # the five per-system parameter gradients cross to their device in one copy
+        pdev = ctx.meta[3][1]
+        dp = torch.as_tensor(dparams).to(pdev) if any(need[3:]) else None
+        outs = outs_atoms + [dp[:, k] if dp is not None else None for k in range(5)]
         grads = [o.to(device=dev, dtype=dt) if (o is not None and n) else None
                  for o, (dt, dev), n in zip(outs, ctx.meta, need)]
         return (None,) + tuple(grads)
 
 
 def _upload(batch: ATMBatch, state):
-    on_gpu, atoms, scal = state
+    on_gpu, atoms, scal, grads = state
     if on_gpu:
         q, s, e = atoms
         torch.cuda.current_stream(q.device).synchronize()  # the library reads on its own stream
@@ -106,7 +112,7 @@ def _upload(batch: ATMBatch, state):
         p = sysk.params
         batch.set_params(k, (scal[0][k], scal[1][k], scal[2][k], scal[3][k], scal[4][k], p.umax, p.ubcore, p.acore,
                              p.direction))
-    batch.compute(forces=False)
+    batch.compute(forces=False, param_grads=grads)
 
 
 def atm_energy(batch: ATMBatch, charges, sigmas, epsilons, lambda1, lambda2, alpha, u0, w0):

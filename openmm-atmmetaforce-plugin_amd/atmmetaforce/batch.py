@@ -166,9 +166,12 @@ class ATMBatch:
 
     # ------------------------------------------------------------ evaluation
     def evaluate(self, forces: bool = True, clear: bool = True, plain: bool = False, graph: bool = True,
-                 fp32_pairs: bool = False):
-        """Enqueue one evaluation of every system (asynchronous)."""
+                 fp32_pairs: bool = False, param_grads: bool = False):
+        """Enqueue one evaluation of every system (asynchronous).  param_grads: energies only,
+        with the per-atom parameter derivatives for a following backward in the same pass."""
         flags = C.ATM_EVAL_FP32_PAIRS if fp32_pairs else 0
+        if param_grads:
+            flags |= C.ATM_EVAL_PARAM_GRADS
         if forces:
             flags |= C.ATM_EVAL_FORCES
         if clear:
@@ -184,10 +187,11 @@ class ATMBatch:
         C.check(self._lib.atm_synchronize(self._h))
 
     def compute(self, forces: bool = True, plain: bool = False, graph: bool = True, retries: int = 3,
-                fp32_pairs: bool = False):
+                fp32_pairs: bool = False, param_grads: bool = False):
         """Evaluate (clearing forces) and wait; re-runs if a neighbour list overflowed."""
         for attempt in range(retries + 1):
-            self.evaluate(forces=forces, clear=True, plain=plain, graph=graph, fp32_pairs=fp32_pairs)
+            self.evaluate(forces=forces, clear=True, plain=plain, graph=graph, fp32_pairs=fp32_pairs,
+                          param_grads=param_grads)
             r = C.Result()
             rc = self._lib.atm_get_results(self._h, 0, ctypes.byref(r))
             if rc == C.ATM_ERR_CAPACITY and attempt < retries:

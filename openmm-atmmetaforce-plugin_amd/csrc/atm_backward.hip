@@ -22,7 +22,10 @@ __global__ void k_fill_f64(double* p, long long n) {
         p[k] = 0.0;
 }
 
-__global__ void k_backward_gather(DevPtrs P, int sysBegin, double* dq, double* dsig, double* deps) {
+// wset null: the accumulators already hold weighted sums; otherwise they hold
+// dU/dp of each set and wset[2 s], wset[2 s + 1] weight the two states here.
+__global__ void k_backward_gather(DevPtrs P, int sysBegin, double* dq, double* dsig, double* deps,
+                                  const double* wset) {
     const int s = sysBegin + blockIdx.y;
     const SetDesc& D0 = P.desc[2 * s];
     const SetDesc& D1 = P.desc[2 * s + 1];
@@ -31,13 +34,14 @@ __global__ void k_backward_gather(DevPtrs P, int sysBegin, double* dq, double* d
     const int a = D0.slotOff + P.slotOf[D0.setAtomOff + i];
     const int b = D1.slotOff + P.slotOf[D1.setAtomOff + i];
     const int k = D0.atomOff + i;
+    const double wa = wset ? wset[2 * s] : 1.0, wb = wset ? wset[2 * s + 1] : 1.0;
     const double sqk = 11.787089759563214;  // sqrt(138.935456)
     const double ep = P.prm64[k].y;          // 2 sqrt(eps)
-    dq[k] = sqk * (P.gq[a] + P.gq[b]);
-    dsig[k] = 0.5 * (P.gsig[a] + P.gsig[b]);
+    if (dq) dq[k] = sqk * fma(wa, P.gq[a], wb * P.gq[b]);
+    if (dsig) dsig[k] = 0.5 * fma(wa, P.gsig[a], wb * P.gsig[b]);
     // dE/deps = dE/de' de'/deps with de'/deps = 1/sqrt(eps) = 2/e': no derivative at eps = 0,
     // where 0 is returned (the convention of the FP64 oracle)
-    deps[k] = ep > 0.0 ? (P.geps[a] + P.geps[b]) * 2.0 / ep : 0.0;
+    if (deps) deps[k] = ep > 0.0 ? fma(wa, P.geps[a], wb * P.geps[b]) * 2.0 / ep : 0.0;
 }
 
 __global__ void k_set_positions(float4* posq, const float* xyz, long long n) {
@@ -96,8 +100,8 @@ hipError_t launch_backward(const DevPtrs& P, long long totalSlots, int setBegin,
 }
 
 hipError_t launch_backward_gather(const DevPtrs& P, int sysBegin, int nsys, int maxAtoms, double* dq, double* dsig,
-                                  double* deps, hipStream_t st) {
-    k_backward_gather<<<dim3(cdivb(maxAtoms, 256), nsys), 256, 0, st>>>(P, sysBegin, dq, dsig, deps);
+                                  double* deps, const double* wset, hipStream_t st) {
+    k_backward_gather<<<dim3(cdivb(maxAtoms, 256), nsys), 256, 0, st>>>(P, sysBegin, dq, dsig, deps, wset);
     return hipGetLastError();
 }
 

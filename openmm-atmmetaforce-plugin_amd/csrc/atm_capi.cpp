@@ -26,7 +26,7 @@ namespace atm {
 hipError_t launch_backward(const DevPtrs& P, long long totalSlots, int setBegin, int nsets, int maxGroups,
                            const double* wset, int split, hipStream_t st);
 hipError_t launch_backward_gather(const DevPtrs& P, int sysBegin, int nsys, int maxAtoms, double* dq, double* dsig,
-                                  double* deps, hipStream_t st);
+                                  double* deps, const double* wset, hipStream_t st);
 hipError_t launch_set_positions(DevPtrs P, float4* posq, const float* xyz, long long n, hipStream_t st);
 hipError_t launch_check_particles(const double* q, const double* sig, const double* eps, long long n, int* bad,
                                   hipStream_t st);
@@ -100,6 +100,7 @@ struct Context {
     bool evaluated = false;
     bool paramsDirty = false;    // host params not yet on the device
     bool chargesOnDevice = false;  // the device charges were set from device arrays (host mirror stale)
+    bool gradsReady = false;     // the last evaluation left dU/dp per set in gq, gsig, geps (ATM_EVAL_PARAM_GRADS)
 
     std::map<int, hipGraphExec_t> graphs;
 
@@ -375,6 +376,7 @@ void alloc_layout(Context& C) {
     }
     C.dSetEnergy.alloc(sizeof(double) * S2);
     C.dGq.release(); C.dGsig.release(); C.dGeps.release();
+    C.gradsReady = false;
     zero_on_stream(C, C.dRt.p, sizeof(SetRuntime) * S2);
     zero_on_stream(C, C.dSdMask.p, C.dSdMask.bytes);
     zero_on_stream(C, C.dCellCount.p, sizeof(int) * C.totalCells);
@@ -447,6 +449,19 @@ void flush_params(Context& C) {
     C.paramsDirty = false;
 }
 
+// Per-slot gradient accumulators of the backward pass (allocated on first use;
+// graphs captured before hold no pointer to them and are kept).
+void ensure_grad_buffers(Context& C) {
+    if (C.dGq.p) return;
+    C.dGq.alloc(sizeof(double) * C.totalSlots);
+    C.dGsig.alloc(sizeof(double) * C.totalSlots);
+    C.dGeps.alloc(sizeof(double) * C.totalSlots);
+    C.P.gq = C.dGq.as<double>();
+    C.P.gsig = C.dGsig.as<double>();
+    C.P.geps = C.dGeps.as<double>();
+    C.dropGraphs();  // DevPtrs is captured by value: graphs must see the new pointers
+}
+
 // Batched evaluation of every set and system, kernel by kernel.  With `ev`
 // (profiling), an event is recorded after each stage: ev[0] before the first
 // kernel, then after build, pair list, pair kernel, energy and blend.
@@ -461,13 +476,20 @@ void enqueue_batched(Context& C, int flags, hipEvent_t* ev = nullptr) {
     HIPCHK(launch_build(P, 0, S2, C.maxAtoms, C.maxCells, C.maxSlots, st));
     if (ev) HIPCHK(hipEventRecord(ev[1], st));
     HIPCHK(launch_pairlist(P, 0, S2, C.maxGroups, 4, st));
-    if (uses_atom_lists(C, flags)) HIPCHK(launch_alist(P, 0, S2, C.maxGroups, st));
+    const bool grads = (flags & ATM_EVAL_PARAM_GRADS) != 0;
+    if (uses_atom_lists(C, flags) && !grads) HIPCHK(launch_alist(P, 0, S2, C.maxGroups, st));
     if (ev) HIPCHK(hipEventRecord(ev[2], st));
-    HIPCHK(launch_pairs(P, 0, S2, C.maxGroups, C.tuning.pairWaves, (flags & ATM_EVAL_FORCES) != 0,
-                        (flags & ATM_EVAL_FP32_PAIRS) == 0, C.tuning.pairKernel, C.tuning.pairSplit, st));
+    if (grads) {
+        // the backward mode of the tile kernel with unit weights: energies, pair counts
+        // and dU/d(q', h, e') of every set in one pass over the pairs
+        HIPCHK(launch_backward(P, C.totalSlots, 0, S2, C.maxGroups, nullptr, C.tuning.pairSplit, st));
+    } else {
+        HIPCHK(launch_pairs(P, 0, S2, C.maxGroups, C.tuning.pairWaves, (flags & ATM_EVAL_FORCES) != 0,
+                            (flags & ATM_EVAL_FP32_PAIRS) == 0, C.tuning.pairKernel, C.tuning.pairSplit, st));
+    }
     if (has_exception_terms(C)) HIPCHK(launch_exceptions(P, 0, S2, (flags & ATM_EVAL_FORCES) != 0, st));
     if (ev) HIPCHK(hipEventRecord(ev[3], st));
-    HIPCHK(launch_energy(P, 0, C.nsys, energy_stride(C, flags), st));
+    HIPCHK(launch_energy(P, 0, C.nsys, grads ? C.tuning.pairSplit : energy_stride(C, flags), st));
     if (ev) HIPCHK(hipEventRecord(ev[4], st));
     if (flags & ATM_EVAL_FORCES)
         HIPCHK(launch_blend(P, 0, C.nsys, C.maxAtoms, 0, 0.0, 0.0, (flags & ATM_EVAL_CLEAR_FORCES) != 0, st));
@@ -476,7 +498,8 @@ void enqueue_batched(Context& C, int flags, hipEvent_t* ev = nullptr) {
 
 void eval_batched(Context& C, int flags) {
     flush_params(C);
-    const int key = flags & (ATM_EVAL_FORCES | ATM_EVAL_CLEAR_FORCES | ATM_EVAL_FP32_PAIRS);
+    if (flags & ATM_EVAL_PARAM_GRADS) ensure_grad_buffers(C);
+    const int key = flags & (ATM_EVAL_FORCES | ATM_EVAL_CLEAR_FORCES | ATM_EVAL_FP32_PAIRS | ATM_EVAL_PARAM_GRADS);
     if (flags & ATM_EVAL_NO_GRAPH) {
         enqueue_batched(C, key);
         return;
@@ -931,8 +954,13 @@ int atm_eval(void* context, int32_t flags) {
     return guarded([&] {
         Context& C = ctx_of(context);
         require_ready(C);
+        if ((flags & ATM_EVAL_PARAM_GRADS) &&
+            (flags & (ATM_EVAL_FORCES | ATM_EVAL_FP32_PAIRS | ATM_EVAL_PLAIN | ATM_EVAL_STATE1_ONLY)))
+            fail(ATM_ERR_INVALID_ARGUMENT,
+                 "ATM_EVAL_PARAM_GRADS is an energy-only evaluation of the batched FP64 path");
         HIPCHK(hipSetDevice(C.device));
         C.resultsValid = false;
+        C.gradsReady = false;
         if (flags & ATM_EVAL_STATE1_ONLY) {
             eval_state1(C, flags);
             C.resultsOnHost = true;
@@ -942,6 +970,7 @@ int atm_eval(void* context, int32_t flags) {
         } else {
             eval_batched(C, flags);
             C.resultsOnHost = false;
+            C.gradsReady = (flags & ATM_EVAL_PARAM_GRADS) != 0;
         }
         C.evaluated = true;
     });
@@ -1179,6 +1208,7 @@ int atm_autotune(void* context, int32_t repeats, atm_tuning* out) {
             for (int attempt = 0; attempt < 3; ++attempt) {
                 eval_batched(C, flags);
                 C.evaluated = true;
+                C.gradsReady = false;
                 C.resultsOnHost = false;
                 C.resultsValid = false;
                 try {
@@ -1283,6 +1313,7 @@ int atm_profile(void* context, int32_t flags, int32_t reps, atm_stage_times* out
         }
         for (auto& e : ev) (void)hipEventDestroy(e);
         C.evaluated = true;
+        C.gradsReady = false;
         C.resultsOnHost = false;
         C.resultsValid = false;
         out->build_ms = acc[0] / reps;
@@ -1315,19 +1346,15 @@ void backward_impl(Context& C, const double* grad_energy, double* d_charges, dou
         wset[2 * s + 1] = grad_energy[s] * (fwd ? w_o : w_e0);
     }
     if (d_charges || d_sigmas || d_epsilons) {
-        if (!C.dGq.p) {
-            C.dGq.alloc(sizeof(double) * C.totalSlots);
-            C.dGsig.alloc(sizeof(double) * C.totalSlots);
-            C.dGeps.alloc(sizeof(double) * C.totalSlots);
-            C.P.gq = C.dGq.as<double>();
-            C.P.gsig = C.dGsig.as<double>();
-            C.P.geps = C.dGeps.as<double>();
-            C.dropGraphs();
-        }
+        // after an evaluation with ATM_EVAL_PARAM_GRADS the accumulators hold dU/dp of
+        // each set: only the weighted gather remains; otherwise one weighted pass
+        const bool fused = C.gradsReady;
+        ensure_grad_buffers(C);
         if (C.dWset.bytes < sizeof(double) * 2 * S) C.dWset.alloc(sizeof(double) * 2 * S);
         HIPCHK(hipMemcpyAsync(C.dWset.p, wset.data(), sizeof(double) * 2 * S, hipMemcpyHostToDevice, C.stream));
-        HIPCHK(launch_backward(C.P, C.totalSlots, 0, 2 * S, C.maxGroups, C.dWset.as<double>(), C.tuning.pairSplit,
-                               C.stream));
+        if (!fused)
+            HIPCHK(launch_backward(C.P, C.totalSlots, 0, 2 * S, C.maxGroups, C.dWset.as<double>(), C.tuning.pairSplit,
+                                   C.stream));
         const size_t nb = sizeof(double) * C.totalAtoms;
         double *oq = d_charges, *os = d_sigmas, *oe = d_epsilons;
         if (!deviceOut) {
@@ -1339,7 +1366,8 @@ void backward_impl(Context& C, const double* grad_energy, double* d_charges, dou
             }
             oq = C.dDq.as<double>(); os = C.dDsig.as<double>(); oe = C.dDeps.as<double>();
         }
-        HIPCHK(launch_backward_gather(C.P, 0, S, C.maxAtoms, oq, os, oe, C.stream));
+        HIPCHK(launch_backward_gather(C.P, 0, S, C.maxAtoms, oq, os, oe, fused ? C.dWset.as<double>() : nullptr,
+                                      C.stream));
         HIPCHK(hipStreamSynchronize(C.stream));
         if (!deviceOut) {
             if (d_charges) to_host(C, d_charges, C.dDq.p, nb);
@@ -1538,6 +1566,7 @@ void md_evaluate(Context& C) {
     for (int attempt = 0;; ++attempt) {
         eval_batched(C, ATM_EVAL_FORCES | ATM_EVAL_CLEAR_FORCES);
         C.evaluated = true;
+        C.gradsReady = false;
         C.resultsOnHost = false;
         C.resultsValid = false;
         try {
@@ -1605,6 +1634,7 @@ int atm_md_step(void* context, int32_t steps, double dt, double friction, double
                 eval_batched(C, ATM_EVAL_FORCES | ATM_EVAL_CLEAR_FORCES);
             }
             C.evaluated = true;
+            C.gradsReady = false;
             C.resultsOnHost = false;
             C.resultsValid = false;
             try {

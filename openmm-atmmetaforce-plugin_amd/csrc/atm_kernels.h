@@ -102,6 +102,8 @@ hipError_t launch_energy(const DevPtrs& P, int sysBegin, int nsys, int estride, 
 hipError_t launch_set_energy(const DevPtrs& P, int c, int estride, hipStream_t st);
 // Backward pass of the pair terms (the tile kernel's backward mode): adds wset[c] dE/d(q', h, e')
 // of every in-cutoff pair of sets [setBegin, setBegin + nsets) to gq, gsig, geps (slot order).
+// wset null: unit weights, and the pass also records the energies and pair counts (an evaluation
+// with ATM_EVAL_PARAM_GRADS).
 hipError_t launch_backward_pairs(const DevPtrs& P, int setBegin, int nsets, int maxGroups, const double* wset,
                                  int split, hipStream_t st);
 // clear: write the blended force instead of adding it to outF.

@@ -767,6 +767,8 @@ __device__ __forceinline__ void flush_component_u(const DevPtrs& P, int c, float
 //   dE/de'_i = e'_j (s6^2 - s6)
 // summed per lane in FP32 where the forward sums forces, weighted by wset[c]
 // (dW/dU of the set) and added to FP64 per-slot accumulators (gq, gsig, geps).
+// With wset null (an evaluation that also differentiates) the weights are 1,
+// and the pair energies and counts are recorded as by the forward modes.
 //
 // EXC: the batch has exceptions; every in-cutoff pair is tested against the
 // i-atom's near mask (and, for far partners, its list) and excepted pairs are
@@ -838,7 +840,7 @@ __global__ void __launch_bounds__(64 * W, 6) k_pairs(DevPtrs P, int setBegin, co
     const bool shiftOK = D.shiftOK;
     const double rc2 = D.rc2, krf64 = D.krf64, crf64 = D.crf64;
     const float krf = D.krf, crf = D.crf, rc2hi = D.rc2hi, fixMax = D.fixMax;
-    const double wgt = BWD ? wset[c] : 0.0;
+    const double wgt = BWD ? (wset ? wset[c] : 1.0) : 0.0;  // no weights: dU/dp of each set
     float fi[kGroupClusters][3];
 #pragma unroll
     for (int t = 0; t < kGroupClusters; ++t) fi[t][0] = fi[t][1] = fi[t][2] = 0.f;
@@ -973,6 +975,7 @@ __global__ void __launch_bounds__(64 * W, 6) k_pairs(DevPtrs P, int setBegin, co
                             const double lj = fma(s6, s6, -s6);
                             const double phi = fma(krf64, r2, y) - crf64;
                             const float dh = (float)(id.y * jd.y * fma(12.0, s6, -6.0) * s4 * u);
+                            eE += fma(id.y * jd.y, lj, id.z * jd.z * phi);  // the pair energy
                             // (q', h, e') sums: i side in fi[t], j side in fjx, fjy, fjz
                             fi[t][0] += (float)(jd.z * phi); fjx += (float)(id.z * phi);
                             fi[t][1] += dh; fjy += dh;
@@ -1051,7 +1054,7 @@ __global__ void __launch_bounds__(64 * W, 6) k_pairs(DevPtrs P, int setBegin, co
             }
         }
     }
-    if (BWD) return;
+    if (BWD && wset) return;  // a backward pass after the evaluation: energies and counts are the evaluation's
     double eacc = eE;
     eacc = wave_sum_d(eacc);
     if (lane == 0) {

@@ -143,3 +143,53 @@ def test_backward_after_re_evaluation_at_other_positions(gpu, device):
         Wt.sum().backward()
         check_against_oracle([s1], (q.grad.cpu().numpy(), sg.grad.cpu().numpy(), ep.grad.cpu().numpy(),
                                     [t.grad.cpu().numpy() for t in sc]), rtol=2e-5)
+
+
+def test_fused_parameter_gradients_match_the_separate_pass(gpu):
+    """ATM_EVAL_PARAM_GRADS: the energies and pair counts of the evaluation equal
+    an energy-only evaluation's, and atm_backward after it (weights and gather
+    only) equals atm_backward after an energy-only evaluation (a second, weighted
+    pass over the pairs), to the FP64 rounding of the per-slot sums."""
+    systems = [W.tiny_system(1500, 3.0, 0.9, 12, (0.9, 0.8, 0.7), 60 + k,
+                             W.ATMParams(0.2 + 0.1 * k, 0.4, 0.1, 110.0, 0.5, 200.0, 100.0, 0.0625, 1.0))
+               for k in range(3)]
+    systems.append(W.with_molecules(W.tiny_system(1200, 3.0, 0.9, 10, (0.9, 0.8, 0.7), 63, systems[0].params),
+                                    size=3, crosslink_every=7))
+    g = np.array([1.0, -0.7, 2.5, 0.3])
+    with ATMBatch(systems) as b:
+        b.compute(forces=False)
+        e0 = [b.result(k) for k in range(len(systems))]
+        n0 = b.list_stats()["pairs"].copy()
+        r0 = b.backward(g)
+        b.compute(forces=False, param_grads=True)
+        n1 = b.list_stats()["pairs"]
+        assert np.array_equal(n0, n1)
+        for k in range(len(systems)):
+            r = b.result(k)
+            for key in ("energy", "state1_energy", "state2_energy"):
+                assert r[key] == pytest.approx(e0[k][key], rel=1e-12)
+        r1 = b.backward(g)
+        for key in ("dq", "dsigma", "deps"):
+            err = np.abs(r1[key] - r0[key]).max() / np.abs(r0[key]).max()
+            print(f"{key}: fused vs separate {err:.2e}", flush=True)
+            assert err < 1e-11
+        assert np.allclose(r1["dparams"], r0["dparams"], rtol=1e-12, atol=1e-14)
+        with pytest.raises(Exception):
+            b.evaluate(forces=True, param_grads=True)
+
+
+def test_partial_gradient_outputs_on_device(gpu):
+    """Only the charges require a gradient: the device backward writes that array
+    alone (the others are not allocated) and it matches the oracle."""
+    s = W.tiny_system(1200, 3.0, 0.9, 10, (0.9, 0.8, 0.7), 71,
+                      W.ATMParams(0.2, 0.4, 0.1, 110.0, 0.5, 200.0, 100.0, 0.0625, 1.0))
+    with ATMBatch([s]) as b:
+        q, sg, ep, sc = tensors([s], "cuda")
+        sg.requires_grad_(False)
+        ep.requires_grad_(False)
+        Wt = ATMEnergyFunction.apply(b, q, sg, ep, *sc)
+        Wt.sum().backward()
+        ref = oracle.atm_grad(s)
+        err = np.abs(q.grad.cpu().numpy() - ref["dq"]).max() / np.abs(ref["dq"]).max()
+        assert err < 2e-5
+        assert sg.grad is None and ep.grad is None
