@@ -273,6 +273,84 @@ def backward_timing(batch, reps, eval_ms, n=20):
                     "tests/test_autograd_gpu.py"}
 
 
+def summarize_profiles(root: str, tuning_label: str, out_json: str) -> str:
+    """Markdown tables of the rocprofv3 runs of bench.py under `root` (kernel
+    trace in trace/, FETCH_SIZE in pmc_fetch/, WRITE_SIZE in pmc_write/, SQ
+    counters in pmc_valu/ and pmc_lds/) and the per-launch figures bench.py
+    reads for roofline.traffic, written to out_json.  The HBM scale factors
+    are the 256 MiB device copies of --calibrate-copy-mib 256: known bytes over
+    what each counter reads for them."""
+    import collections
+    import csv
+    import glob
+
+    def rows(sub, name):
+        f = glob.glob(os.path.join(root, sub, "**", name), recursive=True)
+        return list(csv.DictReader(open(f[0]))) if f else []
+
+    def kname(r):
+        return r.get("Kernel_Name", r.get("Name", "")).split("(")[0].replace("void ", "")
+
+    def per_kernel(rs):
+        d = collections.defaultdict(list)
+        for r in rs:
+            d[(kname(r), r["Counter_Name"])].append(float(r["Counter_Value"]))
+        return d
+
+    out = [f"Tuning of every run: {tuning_label}", "",
+           "| kernel | calls | avg us | min us | max us | % of GPU time |", "|---|---|---|---|---|---|"]
+    for r in sorted(rows("trace", "*kernel_stats.csv"), key=lambda r: -float(r["TotalDurationNs"]))[:16]:
+        out.append(f"| `{kname(r)}` | {r['Calls']} | {float(r['AverageNs']) / 1e3:.2f} | {float(r['MinNs']) / 1e3:.2f} | "
+                   f"{float(r['MaxNs']) / 1e3:.2f} | {float(r['Percentage']):.2f} |")
+    fr, wr = rows("pmc_fetch", "*counter_collection.csv"), rows("pmc_write", "*counter_collection.csv")
+    copy_kib = 256 * 1024
+    big = lambda rs: [float(r["Counter_Value"]) for r in rs
+                      if kname(r).startswith("__amd_rocclr_copyBuffer") and float(r["Counter_Value"]) > copy_kib / 4]
+    cf, cw = big(fr), big(wr)
+    fs, ws = copy_kib / (sum(cf) / len(cf)), copy_kib / (sum(cw) / len(cw))
+    f, w = per_kernel(fr), per_kernel(wr)
+    out += ["", f"Calibration: {len(cf)} copies of 256 MiB read FETCH_SIZE {sum(cf) / len(cf):.1f} KiB (scale {fs:.4f}), "
+                f"WRITE_SIZE {sum(cw) / len(cw):.1f} KiB (scale {ws:.4f}).", "",
+            "| kernel | launches | FETCH_SIZE KiB/launch | WRITE_SIZE KiB/launch | HBM read MB (scaled) | "
+            "HBM write MB (scaled) |", "|---|---|---|---|---|---|"]
+    traffic = {}
+    for (k, c), v in sorted(f.items()):
+        wv = w.get((k, "WRITE_SIZE"))
+        if c != "FETCH_SIZE" or not k.startswith("atm::") or not wv:
+            continue
+        rd, wrb = sum(v) / len(v) * 1024 * fs, sum(wv) / len(wv) * 1024 * ws
+        traffic[k] = (rd, wrb, len(v))
+        out.append(f"| `{k.replace('atm::', '')}` | {len(v)} | {sum(v) / len(v):.1f} | {sum(wv) / len(wv):.1f} | "
+                   f"{rd / 1e6:.2f} | {wrb / 1e6:.2f} |")
+    # the pair kernel of the timed loop: the forward k_pairs instantiation with the most launches
+    kf = max((k for k in traffic if k.startswith("atm::k_pairs<")), key=lambda k: traffic[k][2])
+    sq = per_kernel(rows("pmc_valu", "*counter_collection.csv"))
+    lds = per_kernel(rows("pmc_lds", "*counter_collection.csv"))
+    mean = lambda d, c: sum(d[(kf, c)]) / len(d[(kf, c)]) if d.get((kf, c)) else None
+    out += ["", f"| counter (`{kf.replace('atm::', '')}`, mean per launch) | value |", "|---|---|"]
+    for d in (sq, lds):
+        for (k, c), v in sorted(d.items()):
+            if k == kf:
+                out.append(f"| {c} | {sum(v) / len(v):,.0f} |")
+    valu, grbm = mean(sq, "SQ_INSTS_VALU"), mean(sq, "GRBM_GUI_ACTIVE")
+    # GRBM_GUI_ACTIVE sums the 8 XCDs' busy cycles; 256 CUs x 4 SIMDs issue at most
+    # one wave64 VALU instruction per 4 cycles each
+    issue = valu / (grbm / 8.0 * 256.0) if valu and grbm else None
+    rec = {"k_pairs": {"kernel": kf.replace("atm::", ""), "tuning": tuning_label,
+                       "hbm_bytes_per_launch": round(traffic[kf][0] + traffic[kf][1]),
+                       "hbm_read_bytes_per_launch": round(traffic[kf][0]),
+                       "hbm_write_bytes_per_launch": round(traffic[kf][1]),
+                       "fetch_scale": fs, "write_scale": ws,
+                       "valu_insts_per_launch": round(valu) if valu else None,
+                       "busy_cycles_per_xcd": round(grbm / 8.0) if grbm else None,
+                       "valu_issue_frac": round(issue, 3) if issue else None,
+                       "workload": "22 replicas x 23,000 atoms, both states (44 sets) per launch"},
+           "k_blend": {"hbm_bytes_per_launch": round(sum(traffic.get("atm::k_blend", (0, 0, 0))[:2]))}}
+    with open(out_json, "w") as fh:
+        json.dump(rec, fh, indent=1)
+    return "\n".join(out)
+
+
 def restraint_timing(system, evals=200):
     """Per-evaluation time of the flat-bottom restraints an ABFE script keeps in
     force group 0 beside the ATM force (python/tests/test_abfe.py:58-79): one
@@ -448,7 +526,15 @@ def main():
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="first run a device-to-device copy of this many MiB five times (a kernel of known "
                          "traffic for scaling the FETCH_SIZE/WRITE_SIZE counters)")
+    ap.add_argument("--summarize-profiles", default=None, metavar="DIR",
+                    help="print markdown tables of the rocprofv3 runs under DIR and write DIR/traffic.json "
+                         "(copied to profiles/traffic.json, which the roofline reads); no GPU work")
+    ap.add_argument("--tuning-label", default="", help="the tuning the profiled runs used (with --summarize-profiles)")
     args = ap.parse_args()
+    if args.summarize_profiles:
+        print(summarize_profiles(args.summarize_profiles, args.tuning_label,
+                                 os.path.join(args.summarize_profiles, "traffic.json")))
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(replicas and systems would shard across GPUs with no exchange; see DESIGN.md)", file=sys.stderr)
@@ -526,17 +612,21 @@ def main():
 
     pair_tflops = FLOP_PER_PAIR * pairs / (prof["pairs_ms"] * 1e-3) / 1e12
     traffic = issue = None
+    tsource = None
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tpath):  # HBM bytes and VALU issue rate per launch from the committed rocprofv3 --pmc summaries
         with open(tpath) as fh:
             kp = json.load(fh).get("k_pairs", {})
         traffic, issue = kp.get("hbm_bytes_per_launch"), kp.get("valu_issue_frac")
+        tsource = {"file": "profiles/traffic.json", "measured_with": "rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE, "
+                   "SQ_INSTS_VALU, GRBM_GUI_ACTIVE (separate runs of bench.py, see profiles/k_pairs_22rep_23k.md)",
+                   "kernel": kp.get("kernel"), "tuning": kp.get("tuning")}
     roofline = {"bound": "valu", "achieved": round(pair_tflops, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(pair_tflops / PEAK_FP32_TFLOPS, 4), "traffic": traffic,
                 "kernel": "k_pairs (LJ + reaction-field pair kernel, all 44 coordinate sets per launch)",
                 "per_launch_flop": FLOP_PER_PAIR * pairs, "pairs_per_launch": pairs,
                 "avg_launch_ms": round(prof["pairs_ms"], 4),
-                "valu_issue_frac": issue,
+                "valu_issue_frac": issue, "traffic_and_issue_source": tsource,
                 "note": "achieved counts 57 FLOP per in-cutoff pair only; the kernel issues VALU instructions "
                         "at valu_issue_frac of the chip's rate (SQ_INSTS_VALU over GRBM_GUI_ACTIVE, "
                         "profiles/k_pairs_22rep_23k.md): 8x8 cluster tiles hold 21 % in-cutoff pairs"}

@@ -726,16 +726,15 @@ __device__ __forceinline__ double group_energy(const DevPtrs& P, const SetDesc& 
 }
 
 
-// Add one force component (component k of slot `slot`) with an atomic that
-// every lane issues on every path, so the number of memory operations between
-// a load and its use does not depend on the data; values at or beyond the set's
-// fixMax (clashing pairs) add zero there and take a second, rare path: the FP64
-// side accumulator.
+// Add one force component (component k of slot `slot`) in 64-bit fixed point;
+// zero sums (j-atoms without an in-cutoff partner, dummy slots) issue no atomic.
+// Values at or beyond the set's fixMax (clashing pairs) take a second, rare
+// path: the FP64 side accumulator.
 __device__ __forceinline__ void flush_component_u(const DevPtrs& P, int c, float fixMax, int slot, int k, float f) {
     const bool small = fabsf(f) < fixMax;
     long long* dst = P.sf + (long long)k * P.sfn + slot;
     const long long v = small ? __double_as_longlong((double)f * kForceScale + 6755399441055744.0) - 0x4338000000000000LL : 0ll;
-    atomic_add_fixed(dst, v);  // zeros too: a data-dependent skip would make the count unknown
+    if (v != 0) atomic_add_fixed(dst, v);
     if (!small) {
         int sl = slot;
         asm volatile("" : "+v"(sl));  // keeps the rare path's address math from being hoisted
@@ -892,10 +891,7 @@ __global__ void __launch_bounds__(64 * W, 6) k_pairs(DevPtrs P, int setBegin, co
     // the component each lane keeps after oct_sum3 (x: ii = 0, 7; y: 1, 6; z: 2, 5)
     const bool selX = ii == 0 || ii == 7, selY = ii == 1 || ii == 6;
     int ecount = 0;
-    // Inside the entry loop every lane issues its three atomics whatever the data
-    // (a full flush, cnt = kJFlush, zeros added as zeros): the loads of the next
-    // entries are then followed by a known number of memory operations, and the
-    // wait for them does not also wait for these atomics.
+    // cnt: entries held (kJFlush inside the entry loop, fewer at the end)
     auto flush_j = [&](int cnt) __attribute__((always_inline)) {
         wave_sync();  // the octets' writes are visible to every lane of the wave
         if (cnt == kJFlush || (lane >> 3) < cnt) {
@@ -905,7 +901,7 @@ __global__ void __launch_bounds__(64 * W, 6) k_pairs(DevPtrs P, int setBegin, co
                 const float f = sJF[wave][k][lane];
                 if (BWD) {
                     double* acc = k == 0 ? P.gq : (k == 1 ? P.gsig : P.geps);
-                    atomicAdd(acc + slot, wgt * (double)f);
+                    if (f != 0.f) atomicAdd(acc + slot, wgt * (double)f);
                 } else {
                     flush_component_u(P, c, fixMax, slot, k, f);
                 }
