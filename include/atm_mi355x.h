@@ -160,6 +160,16 @@ int atm_set_particles(void* context, int32_t sys, const double* charges,
                       const double* sigmas, const double* epsilons,
                       const double* displacements);
 
+/* Energies of the other forces of the variable groups (e.g. bonded terms
+ * evaluated beside this library) at x (state1) and at x + d (state2) of
+ * system `sys`: added to U1 and U2 before the soft-core/softplus combine of
+ * every following evaluation, until changed (0, 0 at creation).  Their forces
+ * are the caller's to blend with the returned hybrid weight: F += w1 F1' +
+ * w2 F2' with (w1, w2) = (1 - sp, sp) for direction > 0, (sp, 1 - sp)
+ * otherwise.  The reference's inner contexts evaluate every force of the
+ * variable groups (ATMMetaForceImpl.cpp:52-62, :113, :116). */
+int atm_set_energy_offsets(void* context, int32_t sys, double state1, double state2);
+
 /* Exceptions of system `sys`'s NonbondedForce: `count` atom pairs (pairs[2k],
  * pairs[2k+1]) with charge product (e^2), sigma (nm) and epsilon (kJ/mol), as
  * NonbondedForce::addException takes them; charge product 0 and epsilon 0 is an
@@ -471,6 +481,10 @@ int atm_pme_set_params(void* pme, const atm_params* params);
  * exception, reciprocal, self and dispersion energies. */
 int atm_pme_eval(void* pme, const double* positions, const double box[3], int32_t flags, double* forces,
                  atm_result* result, double* parts);
+
+/* atm_set_energy_offsets for a PME NonbondedForce: added to U1 and U2 before the
+ * combine of every following ATM evaluation (not to a state-1-only one's U1). */
+int atm_pme_set_energy_offsets(void* pme, double state1, double state2);
 
 /* ------------------------------------------------------------------------
  * Bonded terms of group 0 beside the ATM force (the AMBER HarmonicBondForce,

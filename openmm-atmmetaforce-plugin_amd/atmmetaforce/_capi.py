@@ -32,7 +32,7 @@ ONE_4PI_EPS0 = 138.935456
 # Every entry point declared in include/atm_mi355x.h.
 EXPORTED = (
     "atm_version", "atm_abi_version", "atm_last_error", "atm_device_count", "atm_create", "atm_destroy",
-    "atm_set_particles", "atm_set_particles_device", "atm_set_exceptions", "atm_set_displacements", "atm_set_positions",
+    "atm_set_particles", "atm_set_particles_device", "atm_set_exceptions", "atm_set_energy_offsets", "atm_set_displacements", "atm_set_positions",
     "atm_set_positions_device",
     "atm_set_box", "atm_set_params", "atm_eval", "atm_synchronize", "atm_get_results", "atm_get_forces",
     "atm_get_forces_fixed", "atm_clear_forces", "atm_get_pairs", "atm_backward", "atm_backward_device",
@@ -41,7 +41,7 @@ EXPORTED = (
     "atm_eval_noalloc",
     "atm_restraints_create", "atm_restraints_destroy", "atm_restraints_set", "atm_restraints_eval",
     "atm_md_init", "atm_md_step", "atm_md_get",
-    "atm_pme_create", "atm_pme_destroy", "atm_pme_get_info", "atm_pme_set_particles", "atm_pme_set_exceptions",
+    "atm_pme_create", "atm_pme_destroy", "atm_pme_get_info", "atm_pme_set_particles", "atm_pme_set_exceptions", "atm_pme_set_energy_offsets",
     "atm_pme_set_params", "atm_pme_eval", "atm_bonded_create", "atm_bonded_destroy", "atm_bonded_set",
     "atm_bonded_eval",
 )
@@ -132,6 +132,7 @@ def load(path: str | None = None):
         "atm_set_particles_device": [vp, vp, vp, vp],
         "atm_set_displacements": [vp, i32, dp],
         "atm_set_exceptions": [vp, i32, i32, ctypes.POINTER(i32), dp, dp, dp],
+        "atm_set_energy_offsets": [vp, i32, ctypes.c_double, ctypes.c_double],
         "atm_set_positions": [vp, i32, dp],
         "atm_set_positions_device": [vp, vp],
         "atm_set_box": [vp, i32, dp],
@@ -168,6 +169,7 @@ def load(path: str | None = None):
         "atm_pme_get_info": [vp, dp, ctypes.POINTER(i32)],
         "atm_pme_set_particles": [vp, dp, dp, dp, dp],
         "atm_pme_set_exceptions": [vp, i32, ctypes.POINTER(i32), dp, dp, dp],
+        "atm_pme_set_energy_offsets": [vp, ctypes.c_double, ctypes.c_double],
         "atm_pme_set_params": [vp, ctypes.POINTER(Params)],
         "atm_pme_eval": [vp, dp, dp, i32, dp, ctypes.POINTER(Result), dp],
         "atm_bonded_create": [i32, ctypes.POINTER(vp)],

@@ -114,6 +114,11 @@ class ATMBatch:
         C.check(self._lib.atm_set_exceptions(self._h, k, m, pr.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
                                              _dptr(qq), _dptr(sg), _dptr(ep)))
 
+    def set_energy_offsets(self, k, state1, state2):
+        """Energies (kJ/mol) of the other variable-group forces of system k at x and at
+        x + d (atm_set_energy_offsets): added to U1 and U2 before the soft-core combine."""
+        C.check(self._lib.atm_set_energy_offsets(self._h, k, float(state1), float(state2)))
+
     def set_displacements(self, k, displacements):
         d = np.ascontiguousarray(displacements, dtype=np.float64).reshape(self.num_atoms[k], 3)
         C.check(self._lib.atm_set_displacements(self._h, k, _dptr(d)))

@@ -80,6 +80,11 @@ class ATMPMEForce:
         ep = np.ascontiguousarray(epsilon, dtype=np.float64)
         C.check(self._lib.atm_pme_set_exceptions(self._h, len(p), _ip(p), _dp(qq), _dp(sg), _dp(ep)))
 
+    def set_energy_offsets(self, state1, state2):
+        """Energies (kJ/mol) of the other variable-group forces at x and at x + d
+        (atm_pme_set_energy_offsets): added to U1 and U2 before the soft-core combine."""
+        C.check(self._lib.atm_pme_set_energy_offsets(self._h, float(state1), float(state2)))
+
     def set_params(self, params):
         p = _params(params)
         C.check(self._lib.atm_pme_set_params(self._h, ctypes.byref(p)))

@@ -101,6 +101,9 @@ struct Context {
     bool paramsDirty = false;    // host params not yet on the device
     bool chargesOnDevice = false;  // the device charges were set from device arrays (host mirror stale)
     bool gradsReady = false;     // the last evaluation left dU/dp per set in gq, gsig, geps (ATM_EVAL_PARAM_GRADS)
+    std::vector<double> eoff;    // [2 S] energies of other variable-group forces per set (atm_set_energy_offsets)
+    DevBuf dEoff;
+    bool eoffDirty = false;
 
     std::map<int, hipGraphExec_t> graphs;
 
@@ -419,6 +422,7 @@ void alloc_layout(Context& C) {
     P.alist = C.dAlist.as<uint32_t>();
     P.acount = C.dAcount.as<int>();
     P.setEnergy = C.dSetEnergy.as<double>();
+    P.eoff = C.dEoff.as<double>();
     P.gq = P.gsig = P.geps = nullptr;
     upload_exceptions(C);
 }
@@ -443,6 +447,11 @@ void require_ready(const Context& C) {
 static_assert(sizeof(atm_params) == sizeof(AtmParamsDev), "atm_params mirrors AtmParamsDev");
 
 void flush_params(Context& C) {
+    if (C.eoffDirty) {
+        HIPCHK(hipMemcpyAsync(C.dEoff.p, C.eoff.data(), sizeof(double) * C.eoff.size(), hipMemcpyHostToDevice, C.stream));
+        HIPCHK(hipStreamSynchronize(C.stream));
+        C.eoffDirty = false;
+    }
     if (!C.paramsDirty) return;
     HIPCHK(hipMemcpyAsync(C.dParams.p, C.params.data(), sizeof(AtmParamsDev) * C.nsys, hipMemcpyHostToDevice, C.stream));
     HIPCHK(hipStreamSynchronize(C.stream));
@@ -551,7 +560,7 @@ void eval_plain(Context& C, int flags) {
             HIPCHK(launch_pairs(P, c, 1, D.groupCap, C.tuning.pairWaves, true, (flags & ATM_EVAL_FP32_PAIRS) == 0,
                                 C.tuning.pairKernel, C.tuning.pairSplit, st));
             if (D.excCount > 0) HIPCHK(launch_exceptions(P, c, 1, true, st));
-            HIPCHK(launch_set_energy(P, c, energy_stride(C, flags), st));
+            HIPCHK(launch_set_energy(P, c, energy_stride(C, flags), true, st));
             HIPCHK(hipMemcpyAsync(&U[stt], P.setEnergy + c, sizeof(double), hipMemcpyDeviceToHost, st));
             HIPCHK(hipStreamSynchronize(st));
         }
@@ -584,7 +593,7 @@ void eval_state1(Context& C, int flags) {
         HIPCHK(launch_pairs(P, c, 1, D.groupCap, C.tuning.pairWaves, (flags & ATM_EVAL_FORCES) != 0,
                             (flags & ATM_EVAL_FP32_PAIRS) == 0, C.tuning.pairKernel, C.tuning.pairSplit, st));
         if (D.excCount > 0) HIPCHK(launch_exceptions(P, c, 1, (flags & ATM_EVAL_FORCES) != 0, st));
-        HIPCHK(launch_set_energy(P, c, energy_stride(C, flags), st));
+        HIPCHK(launch_set_energy(P, c, energy_stride(C, flags), false, st));
         double U = 0.0;
         HIPCHK(hipMemcpyAsync(&U, P.setEnergy + c, sizeof(double), hipMemcpyDeviceToHost, st));
         if (flags & ATM_EVAL_FORCES) HIPCHK(launch_blend(P, s, 1, D.natoms, 2, 0.0, 1.0, false, st));
@@ -747,6 +756,9 @@ int atm_create(const atm_system_desc* systems, int32_t num_systems, int32_t devi
         zero_on_stream(*C, C->dPrm64.p, sizeof(double4) * off);
         zero_on_stream(*C, C->dOutF.p, sizeof(double) * 3 * off);
         to_device(*C, C->dParams.p, C->params.data(), sizeof(AtmParamsDev) * num_systems);
+        C->eoff.assign(2ll * num_systems, 0.0);
+        C->dEoff.alloc(sizeof(double) * 2 * num_systems);
+        to_device(*C, C->dEoff.p, C->eoff.data(), sizeof(double) * 2 * num_systems);
         layout(*C);
         alloc_layout(*C);
         *out_context = C.release();
@@ -792,6 +804,17 @@ int atm_set_particles(void* context, int32_t sys, const double* charges, const d
         HIPCHK(hipMemcpyAsync(C.dDispl.as<float4>() + off, C.hDispl.data() + off, n * sizeof(float4), hipMemcpyHostToDevice, C.stream));
         HIPCHK(hipStreamSynchronize(C.stream));
         C.haveParticles[sys] = 1;
+    });
+}
+
+int atm_set_energy_offsets(void* context, int32_t sys, double state1, double state2) {
+    return guarded([&] {
+        Context& C = ctx_of(context);
+        check_sys(C, sys);
+        if (!std::isfinite(state1) || !std::isfinite(state2)) fail(ATM_ERR_INVALID_ARGUMENT, "energy offsets must be finite");
+        C.eoff[2 * sys] = state1;
+        C.eoff[2 * sys + 1] = state2;
+        C.eoffDirty = true;
     });
 }
 

@@ -60,6 +60,7 @@ struct DevPtrs {
     const int2* xpair;     // exception terms: system-local atom pair
     const double4* xprm;   // (k_e q'q, sigma, 4 epsilon, 0)
     double* xenergy;       // [2 S] exception energy per set
+    const double* eoff;    // [2 S] energies of other variable-group forces added to U per set (may be null)
     // backward
     double* gq;            // [2 S sets x slots] gradient accumulators in slot order
     double* gsig;
@@ -99,7 +100,8 @@ inline int pair_energy_stride(int variant, bool fp64, int split) {
 // and before launch_energy / launch_set_energy, which add xenergy when excCount > 0).
 hipError_t launch_exceptions(const DevPtrs& P, int setBegin, int nsets, bool forces, hipStream_t st);
 hipError_t launch_energy(const DevPtrs& P, int sysBegin, int nsys, int estride, hipStream_t st);
-hipError_t launch_set_energy(const DevPtrs& P, int c, int estride, hipStream_t st);
+// withOffsets: add eoff[c] (an ATM evaluation; not a plain state-1 one)
+hipError_t launch_set_energy(const DevPtrs& P, int c, int estride, bool withOffsets, hipStream_t st);
 // Backward pass of the pair terms (the tile kernel's backward mode): adds wset[c] dE/d(q', h, e')
 // of every in-cutoff pair of sets [setBegin, setBegin + nsets) to gq, gsig, geps (slot order).
 // wset null: unit weights, and the pass also records the energies and pair counts (an evaluation

@@ -229,6 +229,13 @@ __device__ __forceinline__ PairEF64 pair_ef64(bool in, double r2, double sig, do
     return o;
 }
 
+// LDS writes of a wave visible to all its lanes (wavefront-scope fences around a wave barrier).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 __device__ __forceinline__ uint64_t lanemask_lt() {
     const int lane = threadIdx.x & 63;
     return lane == 0 ? 0ull : (~0ull >> (64 - lane));
@@ -516,15 +523,13 @@ __global__ void __launch_bounds__(64 * W) k_pairlist(DevPtrs P, int setBegin) {
     const int g0 = __builtin_amdgcn_readfirstlane(gi.x), ncl = __builtin_amdgcn_readfirstlane(gi.y);
     const int col = __builtin_amdgcn_readfirstlane(gi.z);
     const int gx = col / D.ny, gy = col % D.ny;
+    // the group's cluster boxes in LDS (read per test: registers for all 8 would
+    // hold the wave to 5 per SIMD, and the kernel waits on memory)
+    __shared__ float4 sIc[W][kGroupClusters], sIh[W][kGroupClusters];
     float4 myc = make_float4(0.f, 0.f, 0.f, 0.f), myh = myc;
     if (lane < ncl) { myc = P.bbc[D.clusterOff + g0 + lane]; myh = P.bbh[D.clusterOff + g0 + lane]; }
-    float icx[kGroupClusters], icy[kGroupClusters], icz[kGroupClusters];
-    float ihx[kGroupClusters], ihy[kGroupClusters], ihz[kGroupClusters];
-#pragma unroll
-    for (int q = 0; q < kGroupClusters; ++q) {
-        icx[q] = __shfl(myc.x, q, 64); icy[q] = __shfl(myc.y, q, 64); icz[q] = __shfl(myc.z, q, 64);
-        ihx[q] = __shfl(myh.x, q, 64); ihy[q] = __shfl(myh.y, q, 64); ihz[q] = __shfl(myh.z, q, 64);
-    }
+    if (lane < kGroupClusters) { sIc[wave][lane] = myc; sIh[wave][lane] = myh; }
+    wave_sync();
     uint32_t* out = P.entries + D.entryOff + (long long)g * D.entryCap;
     int nEnt = 0;
     const int ncx = D.fullX ? D.nx : 2 * D.Rx + 1, ncy = D.fullY ? D.ny : 2 * D.Ry + 1;
@@ -533,10 +538,11 @@ __global__ void __launch_bounds__(64 * W) k_pairlist(DevPtrs P, int setBegin) {
     // z in [zlo - rcl, zhi + rcl] (periodic), so only clusters holding an atom
     // in the z-bins of that window (one bin of slack on each side for rounding)
     // can pair; they form one or two (wrapped) slot ranges per column.
-    float zlo = icz[0] - ihz[0], zhi = icz[0] + ihz[0];
-#pragma unroll
-    for (int q = 1; q < kGroupClusters; ++q) {
-        if (q < ncl) { zlo = fminf(zlo, icz[q] - ihz[q]); zhi = fmaxf(zhi, icz[q] + ihz[q]); }
+    float zlo = sIc[wave][0].z - sIh[wave][0].z, zhi = sIc[wave][0].z + sIh[wave][0].z;
+    for (int q = 1; q < ncl; ++q) {
+        const float cz = sIc[wave][q].z, hz = sIh[wave][q].z;
+        zlo = fminf(zlo, cz - hz);
+        zhi = fmaxf(zhi, cz + hz);
     }
     const float rcl = sqrtf(D.rcList2);
     const int blo = (int)floorf((zlo - rcl) * D.invZe) - 1, bhi = (int)floorf((zhi + rcl) * D.invZe) + 1;
@@ -627,12 +633,13 @@ __global__ void __launch_bounds__(64 * W) k_pairlist(DevPtrs P, int setBegin) {
 #pragma unroll
                 for (int q = 0; q < kGroupClusters; ++q) {
                     if (q < ncl && g0 + q <= J) {
-                        float dx = jc.x + sx - icx[q], dy = jc.y + sy - icy[q], dz = jc.z - icz[q];
+                        const float4 ic = sIc[wave][q], ih = sIh[wave][q];
+                        float dx = jc.x + sx - ic.x, dy = jc.y + sy - ic.y, dz = jc.z - ic.z;
                         if (!D.shiftOK) { dx = min_image(dx, D.L[0], D.invL[0]); dy = min_image(dy, D.L[1], D.invL[1]); }
                         dz = min_image(dz, D.L[2], D.invL[2]);
-                        const float ax = fmaxf(0.f, fabsf(dx) - ihx[q] - jh.x);
-                        const float ay = fmaxf(0.f, fabsf(dy) - ihy[q] - jh.y);
-                        const float az = fmaxf(0.f, fabsf(dz) - ihz[q] - jh.z);
+                        const float ax = fmaxf(0.f, fabsf(dx) - ih.x - jh.x);
+                        const float ay = fmaxf(0.f, fabsf(dy) - ih.y - jh.y);
+                        const float az = fmaxf(0.f, fabsf(dz) - ih.z - jh.z);
                         if (ax * ax + ay * ay + az * az < D.rcList2) mask |= 1u << q;
                     }
                 }
@@ -707,11 +714,6 @@ __device__ __forceinline__ bool excluded_pair(const DevPtrs& P, const SetDesc& D
     return ((ma >> 32) & 1ull) && far_excluded(P, D, a, b);
 }
 
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // Fixed-order sum of a set's energy slots: one per group, or estride per group
 // (8: one per i-cluster wave of the filtered kernel; 2 or 4: one per part of a
@@ -1857,7 +1859,7 @@ __global__ void __launch_bounds__(256) k_energy(DevPtrs P, int sysBegin, int est
             if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
             __syncthreads();
         }
-        U[st] = red[0] + (D.excCount > 0 ? P.xenergy[c] : 0.0);
+        U[st] = red[0] + (D.excCount > 0 ? P.xenergy[c] : 0.0) + (P.eoff ? P.eoff[c] : 0.0);
         __syncthreads();
     }
     if (threadIdx.x == 0) {
@@ -1868,7 +1870,7 @@ __global__ void __launch_bounds__(256) k_energy(DevPtrs P, int sysBegin, int est
 }
 
 // Fixed-order energy of one set into P.setEnergy[c] (reference-structured path).
-__global__ void __launch_bounds__(256) k_set_energy(DevPtrs P, int c, int estride) {
+__global__ void __launch_bounds__(256) k_set_energy(DevPtrs P, int c, int estride, int withOffsets) {
     __shared__ double red[256];
     const SetDesc& D = P.desc[c];
     const int ng = P.rt[c].ngroups;
@@ -1880,7 +1882,8 @@ __global__ void __launch_bounds__(256) k_set_energy(DevPtrs P, int c, int estrid
         if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
         __syncthreads();
     }
-    if (threadIdx.x == 0) P.setEnergy[c] = red[0] + (D.excCount > 0 ? P.xenergy[c] : 0.0);
+    if (threadIdx.x == 0)
+        P.setEnergy[c] = red[0] + (D.excCount > 0 ? P.xenergy[c] : 0.0) + (P.eoff && withOffsets ? P.eoff[c] : 0.0);
 }
 
 // F_sys += w1 F1 + w2 F2 with (w1, w2) = (1-sp, sp) for direction > 0 and
@@ -2044,8 +2047,8 @@ hipError_t launch_energy(const DevPtrs& P, int sysBegin, int nsys, int estride, 
     return hipGetLastError();
 }
 
-hipError_t launch_set_energy(const DevPtrs& P, int c, int estride, hipStream_t st) {
-    k_set_energy<<<1, 256, 0, st>>>(P, c, estride);
+hipError_t launch_set_energy(const DevPtrs& P, int c, int estride, bool withOffsets, hipStream_t st) {
+    k_set_energy<<<1, 256, 0, st>>>(P, c, estride, withOffsets ? 1 : 0);
     return hipGetLastError();
 }
 

@@ -63,6 +63,7 @@ struct PmePtrs {
     double* parts;          // [2][5]
     SysResult* result;
     const AtmParamsDev* params;  // the nine ATM parameters (uploaded before each evaluation)
+    const double* eoff;          // [2] energies of other variable-group forces added to U1, U2 (uploaded with params)
     int* overflow;
 };
 

@@ -663,7 +663,7 @@ __global__ void __launch_bounds__(1024) k_pme_finish(PmeDesc D, PmePtrs P, int s
             P.parts[5 * s + 2] = parts[2];
             P.parts[5 * s + 3] = D.selfEnergy;
             P.parts[5 * s + 4] = D.dispersionEnergy;
-            U[s] = parts[0] + parts[1] + parts[2] + D.selfEnergy + D.dispersionEnergy;
+            U[s] = parts[0] + parts[1] + parts[2] + D.selfEnergy + D.dispersionEnergy + (P.eoff && !single ? P.eoff[s] : 0.0);
         }
     }
     if (threadIdx.x == 0) {

@@ -76,6 +76,7 @@ struct PmeForce {
     std::vector<int> exI, exJ;
     std::vector<double> exQQ, exSig, exEps;
     atm_params params{};
+    double eoff[2] = {0.0, 0.0};  // atm_pme_set_energy_offsets
     bool haveParticles = false, haveParams = false, setupDone = false;
     double dispCoef = 0.0, selfEnergy = 0.0;
     int cellCap = 0;
@@ -229,7 +230,7 @@ void setup(PmeForce& S) {
     S.dF.alloc(sizeof(unsigned long long) * 6 * n);
     S.dParts.alloc(sizeof(double) * 10);
     S.dResult.alloc(sizeof(SysResult));
-    S.dParams.alloc(sizeof(AtmParamsDev));
+    S.dParams.alloc(sizeof(AtmParamsDev) + 2 * sizeof(double));  // parameters, then the two energy offsets
     S.dOverflow.alloc(sizeof(int) * 2);
     S.dOut.alloc(sizeof(double) * 3 * n);
     S.hOut.assign(3 * (size_t)n, 0.0);
@@ -317,6 +318,7 @@ void set_box(PmeForce& S, const double box[3]) {
     P.parts = S.dParts.as<double>();
     P.result = S.dResult.as<SysResult>();
     P.params = S.dParams.as<AtmParamsDev>();
+    P.eoff = reinterpret_cast<const double*>(S.dParams.as<char>() + sizeof(AtmParamsDev));
     P.overflow = S.dOverflow.as<int>();
 }
 
@@ -476,6 +478,15 @@ int atm_pme_set_params(void* h, const atm_params* p) {
     });
 }
 
+int atm_pme_set_energy_offsets(void* h, double state1, double state2) {
+    return guarded([&] {
+        PmeForce& S = pme_of(h);
+        if (!std::isfinite(state1) || !std::isfinite(state2)) fail(ATM_ERR_INVALID_ARGUMENT, "energy offsets must be finite");
+        S.eoff[0] = state1;
+        S.eoff[1] = state2;
+    });
+}
+
 int atm_pme_eval(void* h, const double* positions, const double box[3], int32_t flags, double* forces,
                  atm_result* result, double* parts) {
     return guarded([&] {
@@ -489,10 +500,15 @@ int atm_pme_eval(void* h, const double* positions, const double box[3], int32_t 
         if (!S.setupDone) setup(S);
         const hipStream_t st = S.stream;
         HIPCHK(hipMemcpyAsync(S.dX.p, positions, sizeof(double) * 3 * S.n, hipMemcpyHostToDevice, st));
-        AtmParamsDev prm{S.params.lambda1, S.params.lambda2, S.params.alpha, S.params.u0, S.params.w0,
-                         S.params.umax, S.params.ubcore, S.params.acore, S.params.direction};
+        struct {
+            AtmParamsDev prm;
+            double eoff[2];
+        } up{{S.params.lambda1, S.params.lambda2, S.params.alpha, S.params.u0, S.params.w0, S.params.umax,
+              S.params.ubcore, S.params.acore, S.params.direction},
+             {S.eoff[0], S.eoff[1]}};
+        static_assert(sizeof(up) == sizeof(AtmParamsDev) + 2 * sizeof(double), "parameters and offsets packed");
         const bool single = (flags & ATM_EVAL_STATE1_ONLY) != 0;
-        HIPCHK(hipMemcpyAsync(S.dParams.p, &prm, sizeof(prm), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(S.dParams.p, &up, sizeof(up), hipMemcpyHostToDevice, st));
         for (int attempt = 0; attempt < 2; ++attempt) {
             set_box(S, box ? box : S.box);
             // one captured graph per (state 1 only, forces); captured again when the

@@ -22,6 +22,8 @@ public:
                 const std::vector<OpenMM::Vec3>* displacements);
     void setParticles(const OpenMM::NonbondedForce& nb, const std::vector<OpenMM::Vec3>* displacements);
     void setDisplacements(const std::vector<OpenMM::Vec3>& displacements);
+    // energies of the other variable-group forces at x and at x + d (added to U1, U2)
+    void setEnergyOffsets(double state1, double state2);
     // Upload box (re-laying out the cell grid when it changed) and positions.
     void syncState(OpenMM::ContextImpl& context);
     // Evaluate with the given C-ABI flags; re-runs once if a list overflowed.
@@ -52,6 +54,7 @@ private:
 class MI355XCalcATMMetaForceKernel : public CalcATMMetaForceKernel {
 public:
     MI355XCalcATMMetaForceKernel(std::string name, const OpenMM::Platform& platform, OpenMM::ContextImpl& context);
+    ~MI355XCalcATMMetaForceKernel();
     void initialize(const OpenMM::System& system, const ATMMetaForce& force) override;
     double execute(OpenMM::ContextImpl& context, OpenMM::ContextImpl& innerContext1, OpenMM::ContextImpl& innerContext2,
                    double State1Energy, double State2Energy, bool includeForces, bool includeEnergy) override;
@@ -70,6 +73,8 @@ private:
     MI355XSystemContext sys;
     bool hasVariableForces = false;
     std::vector<int> particles;
+    std::vector<OpenMM::Vec3> displacements;
+    void* variableBonded = nullptr;  // bonded terms of the variable groups (atm_bonded_*), or null
     double PerturbationEnergy = 0.0, U1 = 0.0, U2 = 0.0;
 };
 

@@ -29,16 +29,25 @@ void check(int rc, const char* what) {
     if (rc != ATM_OK) throw OpenMMException(std::string("MI355X: ") + what + ": " + atm_last_error());
 }
 
-// First NonbondedForce among the forces whose group is in `mask`; every other
-// force in those groups is rejected (the device path evaluates only this term).
-const NonbondedForce* variableNonbonded(const System& system, int mask, const Force* skip) {
+// The NonbondedForce among the forces whose group is in `mask`, with the
+// bonded forces of those groups (HarmonicBondForce, HarmonicAngleForce,
+// PeriodicTorsionForce) appended to `bonded`; any other force in those groups
+// is rejected, as is a second NonbondedForce.
+const NonbondedForce* variableNonbonded(const System& system, int mask, const Force* skip,
+                                        std::vector<const Force*>* bonded) {
     const NonbondedForce* nb = nullptr;
     for (int i = 0; i < system.getNumForces(); ++i) {
         const Force& f = system.getForce(i);
         if (&f == skip || !(mask & (1 << f.getForceGroup()))) continue;
+        if (dynamic_cast<const HarmonicBondForce*>(&f) || dynamic_cast<const HarmonicAngleForce*>(&f) ||
+            dynamic_cast<const PeriodicTorsionForce*>(&f)) {
+            bonded->push_back(&f);
+            continue;
+        }
         const NonbondedForce* n = dynamic_cast<const NonbondedForce*>(&f);
         if (!n)
-            throw OpenMMException("MI355X ATMMetaForce: only a NonbondedForce may be in the variable force groups (found '" +
+            throw OpenMMException("MI355X ATMMetaForce: the variable force groups may hold one NonbondedForce and "
+                                  "harmonic bond, harmonic angle and periodic torsion forces (found '" +
                                   f.getName() + "')");
         if (nb) throw OpenMMException("MI355X ATMMetaForce: more than one NonbondedForce in the variable force groups");
         nb = n;
@@ -294,6 +303,11 @@ void MI355XSystemContext::setParticles(const NonbondedForce& nb, const std::vect
     }
 }
 
+void MI355XSystemContext::setEnergyOffsets(double state1, double state2) {
+    if (pme) check(atm_pme_set_energy_offsets(pme, state1, state2), "atm_pme_set_energy_offsets");
+    else check(atm_set_energy_offsets(ctx, 0, state1, state2), "atm_set_energy_offsets");
+}
+
 void MI355XSystemContext::setDisplacements(const std::vector<Vec3>& displacements) {
     if (pme) {  // displacements only: the particle parameters of the last upload stay
         check(atm_pme_set_particles(pme, nullptr, nullptr, nullptr, reinterpret_cast<const double*>(displacements.data())),
@@ -372,6 +386,46 @@ void MI355XCalcNonbondedForceKernel::copyParametersToContext(ContextImpl& ctx, c
     sys.setParticles(force, nullptr);
 }
 
+namespace {
+
+// Terms of one bonded force as the flat arrays of atm_bonded_set.
+struct BondedTerms {
+    std::vector<int32_t> bonds, angles, torsions;
+    std::vector<double> bprm, aprm, tprm;
+};
+
+void termsOf(const HarmonicBondForce& f, BondedTerms& t) {
+    for (int k = 0; k < f.getNumBonds(); ++k) {
+        int p1, p2;
+        double r0, kf;
+        f.getBondParameters(k, p1, p2, r0, kf);
+        t.bonds.insert(t.bonds.end(), {p1, p2});
+        t.bprm.insert(t.bprm.end(), {r0, kf});
+    }
+}
+
+void termsOf(const HarmonicAngleForce& f, BondedTerms& t) {
+    for (int k = 0; k < f.getNumAngles(); ++k) {
+        int p1, p2, p3;
+        double a0, kf;
+        f.getAngleParameters(k, p1, p2, p3, a0, kf);
+        t.angles.insert(t.angles.end(), {p1, p2, p3});
+        t.aprm.insert(t.aprm.end(), {a0, kf});
+    }
+}
+
+void termsOf(const PeriodicTorsionForce& f, BondedTerms& t) {
+    for (int k = 0; k < f.getNumTorsions(); ++k) {
+        int p1, p2, p3, p4, per;
+        double ph, kf;
+        f.getTorsionParameters(k, p1, p2, p3, p4, per, ph, kf);
+        t.torsions.insert(t.torsions.end(), {p1, p2, p3, p4});
+        t.tprm.insert(t.tprm.end(), {(double)per, ph, kf});
+    }
+}
+
+}  // namespace
+
 // ------------------------------------------------------------------ ATMMetaForce kernel
 
 MI355XCalcATMMetaForceKernel::MI355XCalcATMMetaForceKernel(std::string name, const Platform& platform,
@@ -398,13 +452,35 @@ static std::vector<Vec3> displacementTable(const System& system, const ATMMetaFo
     return d;
 }
 
+MI355XCalcATMMetaForceKernel::~MI355XCalcATMMetaForceKernel() {
+    if (variableBonded) atm_bonded_destroy(variableBonded);
+}
+
 void MI355XCalcATMMetaForceKernel::initialize(const System& system, const ATMMetaForce& force) {
     int mask = 0;
     for (int g : force.getVariableForceGroups()) mask |= 1 << g;
-    const NonbondedForce* nb = variableNonbonded(system, mask, &force);
-    std::vector<Vec3> d = displacementTable(system, force, &particles);
+    std::vector<const Force*> bonded;
+    const NonbondedForce* nb = variableNonbonded(system, mask, &force, &bonded);
+    displacements = displacementTable(system, force, &particles);
     hasVariableForces = nb != nullptr;
-    if (hasVariableForces) sys.create(MI355XPlatform::data(context).device, system, *nb, &d);
+    if (!bonded.empty()) {
+        if (!nb)
+            throw OpenMMException("MI355X ATMMetaForce: bonded forces in the variable force groups need the "
+                                  "NonbondedForce in those groups too");
+        // every bonded term of the variable groups in one set, evaluated at x and at x + d
+        BondedTerms t;
+        for (const Force* f : bonded) {
+            if (auto b = dynamic_cast<const HarmonicBondForce*>(f)) termsOf(*b, t);
+            else if (auto a = dynamic_cast<const HarmonicAngleForce*>(f)) termsOf(*a, t);
+            else termsOf(*dynamic_cast<const PeriodicTorsionForce*>(f), t);
+        }
+        check(atm_bonded_create(MI355XPlatform::data(context).device, &variableBonded), "atm_bonded_create");
+        check(atm_bonded_set(variableBonded, system.getNumParticles(), (int32_t)t.bonds.size() / 2, t.bonds.data(),
+                             t.bprm.data(), (int32_t)t.angles.size() / 3, t.angles.data(), t.aprm.data(),
+                             (int32_t)t.torsions.size() / 4, t.torsions.data(), t.tprm.data()),
+              "atm_bonded_set");
+    }
+    if (hasVariableForces) sys.create(MI355XPlatform::data(context).device, system, *nb, &displacements);
 }
 
 double MI355XCalcATMMetaForceKernel::executeFused(ContextImpl& ctx, bool includeForces, bool includeEnergy) {
@@ -427,9 +503,36 @@ double MI355XCalcATMMetaForceKernel::executeFused(ContextImpl& ctx, bool include
         PerturbationEnergy = 0.0;
         return includeEnergy ? w : 0.0;
     }
+    // bonded terms of the variable groups at x and at x + d: their energies enter U1 and
+    // U2 before the combine, their forces are blended with the evaluation's weights
+    std::vector<double> fb1, fb2;
+    if (variableBonded) {
+        const std::vector<Vec3>& x = ctx.positionsRef();
+        std::vector<Vec3> xd(x.size());
+        for (size_t i = 0; i < x.size(); ++i) xd[i] = x[i] + displacements[i];
+        if (includeForces) {
+            fb1.assign(3 * x.size(), 0.0);
+            fb2.assign(3 * x.size(), 0.0);
+        }
+        double e1[3], e2[3];
+        check(atm_bonded_eval(variableBonded, reinterpret_cast<const double*>(x.data()),
+                              includeForces ? fb1.data() : nullptr, e1), "atm_bonded_eval");
+        check(atm_bonded_eval(variableBonded, reinterpret_cast<const double*>(xd.data()),
+                              includeForces ? fb2.data() : nullptr, e2), "atm_bonded_eval");
+        sys.setEnergyOffsets(e1[0] + e1[1] + e1[2], e2[0] + e2[1] + e2[2]);
+    }
+    auto blendBonded = [&](const atm_result& r) {
+        if (fb1.empty()) return;
+        const double w2 = p.direction > 0.0 ? r.hybrid_weight : 1.0 - r.hybrid_weight, w1 = 1.0 - w2;
+        std::vector<Vec3>& F = ctx.forcesRef();
+        for (size_t i = 0; i < F.size(); ++i)
+            F[i] += Vec3(w1 * fb1[3 * i] + w2 * fb2[3 * i], w1 * fb1[3 * i + 1] + w2 * fb2[3 * i + 1],
+                         w1 * fb1[3 * i + 2] + w2 * fb2[3 * i + 2]);
+    };
     if (sys.isPME()) {
         atm_result r;
         sys.evaluatePME(ctx, &p, includeForces, &r);  // dispersion correction inside U1 and U2
+        blendBonded(r);
         U1 = r.state1_energy;
         U2 = r.state2_energy;
         PerturbationEnergy = r.perturbation_energy;
@@ -444,6 +547,7 @@ double MI355XCalcATMMetaForceKernel::executeFused(ContextImpl& ctx, bool include
     atm_result r;
     check(atm_get_results(sys.handle(), 0, &r), "atm_get_results");
     if (includeForces) addForces(sys.handle(), sys.numAtoms(), ctx);
+    blendBonded(r);
     Vec3 b[3];
     ctx.getPeriodicBoxVectors(b[0], b[1], b[2]);
     const double edisp = sys.dispersionEnergy(b);  // present in both states: shifts e0 only
@@ -476,6 +580,7 @@ void MI355XCalcATMMetaForceKernel::copyParametersToContext(ContextImpl& ctx, con
             throw OpenMMException("MI355XCalcATMMetaForceKernel::copyParametersToContext: A particle index has changed");
         d[i] = Vec3(dx, dy, dz);
     }
+    displacements = d;
     if (hasVariableForces) sys.setDisplacements(d);
 }
 
@@ -587,46 +692,6 @@ extern "C" __attribute__((visibility("default"))) void registerPlatforms() {
 // Registers the ATMMetaForce kernel factory with the MI355X platform; errors are
 // ignored, as the reference's factories do (HipATMMetaForceKernelFactory.cpp:18-27).
 // ------------------------------------------------------------------ bonded kernels
-
-namespace {
-
-// Terms of one bonded force as the flat arrays of atm_bonded_set.
-struct BondedTerms {
-    std::vector<int32_t> bonds, angles, torsions;
-    std::vector<double> bprm, aprm, tprm;
-};
-
-void termsOf(const HarmonicBondForce& f, BondedTerms& t) {
-    for (int k = 0; k < f.getNumBonds(); ++k) {
-        int p1, p2;
-        double r0, kf;
-        f.getBondParameters(k, p1, p2, r0, kf);
-        t.bonds.insert(t.bonds.end(), {p1, p2});
-        t.bprm.insert(t.bprm.end(), {r0, kf});
-    }
-}
-
-void termsOf(const HarmonicAngleForce& f, BondedTerms& t) {
-    for (int k = 0; k < f.getNumAngles(); ++k) {
-        int p1, p2, p3;
-        double a0, kf;
-        f.getAngleParameters(k, p1, p2, p3, a0, kf);
-        t.angles.insert(t.angles.end(), {p1, p2, p3});
-        t.aprm.insert(t.aprm.end(), {a0, kf});
-    }
-}
-
-void termsOf(const PeriodicTorsionForce& f, BondedTerms& t) {
-    for (int k = 0; k < f.getNumTorsions(); ++k) {
-        int p1, p2, p3, p4, per;
-        double ph, kf;
-        f.getTorsionParameters(k, p1, p2, p3, p4, per, ph, kf);
-        t.torsions.insert(t.torsions.end(), {p1, p2, p3, p4});
-        t.tprm.insert(t.tprm.end(), {(double)per, ph, kf});
-    }
-}
-
-}  // namespace
 
 template <class F, class Base>
 MI355XCalcBondedKernel<F, Base>::~MI355XCalcBondedKernel() {

@@ -19,9 +19,15 @@ import temoa
 pytestmark = pytest.mark.gpu
 
 
-def build_abfe_context():
+def build_abfe_context(bonded_group=0):
+    """bonded_group: force group of the bonded forces (1 puts them into the ATM
+    variable group next to the NonbondedForce)."""
     s, box, x, ref40 = temoa.load()
     system = amber.create_system(s, box)
+    for i in range(system.getNumForces()):
+        fc = system.getForce(i)
+        if isinstance(fc, (mm.HarmonicBondForce, mm.HarmonicAngleForce, mm.PeriodicTorsionForce)):
+            fc.setForceGroup(bonded_group)
     utils = atm.ATMMetaForceUtils(system)
     rcpt, lig, posr = temoa.restraint_setup(s)
     utils.addVsiteRestraintForceCMCM(lig_cm_particles=list(lig), rcpt_cm_particles=list(rcpt),
