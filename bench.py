@@ -573,8 +573,14 @@ def main():
         log("[bench] autotuning")
         tuning = batch.autotune(repeats=5)
         batch.compute()
-    # in-cutoff pairs counted by the pair kernel itself (lane masks of its exact r < rc tests)
-    pairs = int(batch.list_stats()["pairs"].sum())
+    # in-cutoff pairs counted by the pair kernel itself (lane masks of its exact r < rc tests), over
+    # the coordinate sets it paired: with x + d evaluated as the difference from x (the default when
+    # few atoms are displaced) only the x sets have lists (max_entries > 0)
+    ls = batch.list_stats()
+    paired = ls["max_entries"] > 0
+    pairs = int(ls["pairs"][paired].sum())
+    nsets_paired = int(paired.sum())
+    state2_mode = "difference" if nsets_paired < len(ls["pairs"]) else "full"
 
     log(f"[bench] warmup {args.warmup}")
     for _ in range(args.warmup):
@@ -603,6 +609,24 @@ def main():
         batch.synchronize()
         fp32_eps = args.replicas * args.steps / (time.perf_counter() - t1)
 
+    full_eps = None
+    if state2_mode == "difference" and not args.no_fp32:
+        log("[bench] x + d binned and paired in full (full_state2)")
+        t = batch.tuning()
+        batch.set_tuning(t["column_scale"], t["pair_waves_per_block"], t["zbins_per_column"], t["pair_kernel"],
+                         t["list_capacity"], t["pair_split"], full_state2=True)
+        for _ in range(args.warmup):
+            batch.evaluate(forces=True, clear=True)
+        batch.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(args.steps):
+            batch.evaluate(forces=True, clear=True)
+        batch.synchronize()
+        full_eps = args.replicas * args.steps / (time.perf_counter() - t1)
+        batch.set_tuning(t["column_scale"], t["pair_waves_per_block"], t["zbins_per_column"], t["pair_kernel"],
+                         t["list_capacity"], t["pair_split"])
+        batch.compute()
+
     log("[bench] per-stage HIP-event timing")
     prof = batch.profile(reps=max(10, args.steps // 10))
     r0 = batch.result(0)
@@ -623,7 +647,9 @@ def main():
                    "kernel": kp.get("kernel"), "tuning": kp.get("tuning")}
     roofline = {"bound": "valu", "achieved": round(pair_tflops, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(pair_tflops / PEAK_FP32_TFLOPS, 4), "traffic": traffic,
-                "kernel": "k_pairs (LJ + reaction-field pair kernel, all 44 coordinate sets per launch)",
+                "kernel": f"k_pairs (LJ + reaction-field pair kernel, {nsets_paired} coordinate sets per launch: "
+                          + ("the x set of every replica; x + d is the difference at the displaced atoms, "
+                             "stage_ms.displaced_ms)" if state2_mode == "difference" else "x and x + d of every replica)"),
                 "per_launch_flop": FLOP_PER_PAIR * pairs, "pairs_per_launch": pairs,
                 "avg_launch_ms": round(prof["pairs_ms"], 4),
                 "valu_issue_frac": issue, "traffic_and_issue_source": tsource,
@@ -681,6 +707,10 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "pair_interactions_per_s": round(pairs * args.steps / dt, 1),
+        "pair_interactions_note": "in-cutoff pairs the pair kernel evaluates per second (the x sets when "
+                                  "x + d is the difference from x)",
+        "state2_mode": state2_mode,
+        "full_state2_evals_per_s": round(full_eps, 2) if full_eps else None,
         "plain_evals_per_s": round(plain_eps, 2) if plain_eps else None,
         "speedup_vs_plain": round(evals_per_s / plain_eps, 2) if plain_eps else None,
         "fp32_pair_mode_evals_per_s": round(fp32_eps, 2) if fp32_eps else None,

@@ -46,7 +46,7 @@
 extern "C" {
 #endif
 
-#define ATM_MI355X_ABI_VERSION 5
+#define ATM_MI355X_ABI_VERSION 6
 
 /* status codes */
 #define ATM_OK 0
@@ -132,7 +132,11 @@ typedef struct atm_tuning {
     int32_t pair_split;       /* waves sharing one group's list in the tile kernel (variant 0
                                  and FP32 pairs): 1, 2, 4 or 8; more, shorter waves shorten
                                  the end of the launch where few waves are left */
-    int32_t reserved_;        /* 0 */
+    int32_t full_state2;      /* 0: when every system displaces at most a quarter of its atoms,
+                                 x + d is evaluated as the difference from x (the pairs and
+                                 exception terms with a displaced atom at both states; the
+                                 other atoms do not move), so only x is binned and paired;
+                                 1: x + d is binned and paired in full like x */
 } atm_tuning;
 
 /* Library version string and ABI number. */
@@ -278,9 +282,11 @@ int atm_set_tuning(void* context, const atm_tuning* tuning);
 
 /* Per-stage device times (ms, averaged over `reps` batched evaluations launched
  * kernel by kernel with HIP events between stages on the context stream).
- * build = clear + binning + scan + scatter + cell sort + cluster boxes. */
+ * build = clear + binning + scan + scatter + cell sort + cluster boxes; pairs =
+ * the pair kernel alone; displaced = the exception terms and, when x + d is
+ * evaluated as the difference from x, the displaced-atom kernels. */
 typedef struct atm_stage_times {
-    double build_ms, pairlist_ms, pairs_ms, energy_ms, blend_ms, total_ms;
+    double build_ms, pairlist_ms, pairs_ms, displaced_ms, energy_ms, blend_ms, total_ms;
     int32_t reps;
 } atm_stage_times;
 int atm_profile(void* context, int32_t flags, int32_t reps, atm_stage_times* out);

@@ -70,12 +70,12 @@ class Tuning(ctypes.Structure):
     _fields_ = [("column_scale", ctypes.c_double), ("pair_waves_per_block", ctypes.c_int32),
                 ("zbins_per_column", ctypes.c_int32), ("best_ms", ctypes.c_double),
                 ("pair_kernel", ctypes.c_int32), ("list_capacity", ctypes.c_int32),
-                ("pair_split", ctypes.c_int32), ("reserved_", ctypes.c_int32)]
+                ("pair_split", ctypes.c_int32), ("full_state2", ctypes.c_int32)]
 
 
 class StageTimes(ctypes.Structure):
     _fields_ = [(n, ctypes.c_double) for n in
-                ("build_ms", "pairlist_ms", "pairs_ms", "energy_ms", "blend_ms", "total_ms")] + \
+                ("build_ms", "pairlist_ms", "pairs_ms", "displaced_ms", "energy_ms", "blend_ms", "total_ms")] + \
                [("reps", ctypes.c_int32)]
 
     def as_dict(self):

@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <iterator>
 #include <map>
 #include <memory>
 #include <stdexcept>
@@ -104,6 +105,15 @@ struct Context {
     std::vector<double> eoff;    // [2 S] energies of other variable-group forces per set (atm_set_energy_offsets)
     DevBuf dEoff;
     bool eoffDirty = false;
+
+    // state difference: displaced atoms (d != 0) per system and the exception terms whose
+    // two atoms are displaced differently (batch index); see update_displaced
+    std::vector<int> ligAtoms, lxidx;
+    std::vector<unsigned char> ligMark;
+    int maxLig = 0;
+    bool deltaOK = false;        // every system displaces at most a quarter of its atoms
+    bool lastDelta = false;      // the last batched evaluation used the state difference
+    DevBuf dLig, dLigMark, dLenergy, dLxidx;
 
     std::map<int, hipGraphExec_t> graphs;
 
@@ -341,6 +351,8 @@ void layout(Context& C) {
     C.totalAlist = aOff;
 }
 
+void update_displaced(Context& C);
+
 void alloc_layout(Context& C) {
     C.dropGraphs();
     const long long S2 = 2ll * C.nsys;
@@ -425,6 +437,73 @@ void alloc_layout(Context& C) {
     P.eoff = C.dEoff.as<double>();
     P.gq = P.gsig = P.geps = nullptr;
     upload_exceptions(C);
+    C.dropGraphs();  // new pointers in P (the displaced-atom arrays may be reallocated below)
+    update_displaced(C);
+}
+
+// The displaced atoms of every system and the exception terms between atoms displaced
+// differently, into the set descriptors (both sets of a system) and device arrays.
+// Called after every relayout and whenever displacements or exceptions change.
+void update_displaced(Context& C) {
+    const int oldMax = C.maxLig;
+    const bool oldOK = C.deltaOK, oldLx = C.lxidx.empty();
+    C.ligAtoms.clear();
+    C.lxidx.clear();
+    C.ligMark.assign(C.totalAtoms, 0);
+    C.maxLig = 0;
+    C.deltaOK = true;
+    auto isDisplaced = [](const float4& d) { return d.x != 0.f || d.y != 0.f || d.z != 0.f; };
+    for (int s = 0; s < C.nsys; ++s) {
+        const int off = C.atomOff[s], n = C.sys[s].num_atoms;
+        const int lb = (int)C.ligAtoms.size();
+        for (int i = 0; i < n; ++i)
+            if (isDisplaced(C.hDispl[off + i])) {
+                C.ligAtoms.push_back(i);
+                C.ligMark[off + i] = 1;
+            }
+        const int nl = (int)C.ligAtoms.size() - lb;
+        C.maxLig = std::max(C.maxLig, nl);
+        if (4ll * nl > n) C.deltaOK = false;
+        const int xb = (int)C.lxidx.size();
+        if (s < (int)C.exc.size()) {
+            const auto& X = C.exc[s];
+            for (size_t k = 0; k < X.pairs.size(); ++k) {
+                const float4 da = C.hDispl[off + X.pairs[k].x], db = C.hDispl[off + X.pairs[k].y];
+                if (da.x != db.x || da.y != db.y || da.z != db.z) C.lxidx.push_back(C.sets[2 * s].excOff + (int)k);
+            }
+        }
+        for (int st = 0; st < 2; ++st) {
+            SetDesc& D = C.sets[2 * s + st];
+            D.ligOff = lb; D.ligCount = nl;
+            D.lxOff = xb; D.lxCount = (int)C.lxidx.size() - xb;
+        }
+    }
+    const size_t nl = std::max<size_t>(C.ligAtoms.size(), 1), nx = std::max<size_t>(C.lxidx.size(), 1);
+    bool moved = false;  // a buffer was reallocated
+    auto fit = [&](DevBuf& b, size_t bytes) {
+        if (b.bytes < bytes) { b.alloc(bytes); moved = true; }
+    };
+    fit(C.dLig, sizeof(int) * nl);
+    fit(C.dLenergy, sizeof(double) * 2 * kDisplacedParts * nl);
+    fit(C.dLxidx, sizeof(int) * nx);
+    fit(C.dLigMark, (size_t)C.totalAtoms);
+    if (!C.ligAtoms.empty()) to_device(C, C.dLig.p, C.ligAtoms.data(), sizeof(int) * C.ligAtoms.size());
+    if (!C.lxidx.empty()) to_device(C, C.dLxidx.p, C.lxidx.data(), sizeof(int) * C.lxidx.size());
+    to_device(C, C.dLigMark.p, C.ligMark.data(), (size_t)C.totalAtoms);
+    to_device(C, C.dDesc.p, C.sets.data(), sizeof(SetDesc) * C.sets.size());
+    DevPtrs& P = C.P;
+    P.ligAtoms = C.dLig.as<int>();
+    P.ligMark = C.dLigMark.as<unsigned char>();
+    P.lenergy = C.dLenergy.as<double>();
+    P.lxidx = C.dLxidx.as<int>();
+    // the graphs hold the pointers, the displaced-atom kernel's grid (maxLig) and whether the
+    // displaced exception kernel runs
+    if (moved || C.maxLig != oldMax || C.deltaOK != oldOK || C.lxidx.empty() != oldLx) C.dropGraphs();
+}
+
+// The batched evaluation with these flags uses the state difference.
+bool use_delta(const Context& C, int flags) {
+    return C.deltaOK && !C.tuning.fullState2 && !(flags & ATM_EVAL_PARAM_GRADS);
 }
 
 // Pair kernel variant 5 (FP64 pairs only) reads the atom-level j lists.
@@ -473,36 +552,48 @@ void ensure_grad_buffers(Context& C) {
 
 // Batched evaluation of every set and system, kernel by kernel.  With `ev`
 // (profiling), an event is recorded after each stage: ev[0] before the first
-// kernel, then after build, pair list, pair kernel, energy and blend.
+// kernel, then after build, pair list, pair kernel, exception terms and displaced
+// atoms, energy and blend.
+// With the state difference (use_delta) only the state-0 sets are built and paired
+// (set stride 2), then the displaced-atom kernels evaluate what x + d changes.
 void enqueue_batched(Context& C, int flags, hipEvent_t* ev = nullptr) {
     const hipStream_t st = C.stream;
-    const DevPtrs& P = C.P;
-    const int S2 = 2 * C.nsys;
+    const bool delta = use_delta(C, flags);
+    DevPtrs P = C.P;
+    P.setStride = delta ? 2 : 1;
+    const int nsets = delta ? C.nsys : 2 * C.nsys;
+    const bool forces = (flags & ATM_EVAL_FORCES) != 0;
     if (ev) HIPCHK(hipEventRecord(ev[0], st));
     // with forces, the blend writes the cleared force itself
-    const bool clearHere = (flags & ATM_EVAL_CLEAR_FORCES) && !(flags & ATM_EVAL_FORCES);
-    HIPCHK(launch_zero_counts(P, 0, (int)C.totalCells, 0, S2, P.outF, clearHere ? 3 * C.totalAtoms : 0, st));
-    HIPCHK(launch_build(P, 0, S2, C.maxAtoms, C.maxCells, C.maxSlots, st));
+    const bool clearHere = (flags & ATM_EVAL_CLEAR_FORCES) && !forces;
+    HIPCHK(launch_zero_counts(P, 0, nsets, C.maxCells, P.outF, clearHere ? 3 * C.totalAtoms : 0, st));
+    HIPCHK(launch_build(P, 0, nsets, C.maxAtoms, C.maxCells, C.maxSlots, st));
     if (ev) HIPCHK(hipEventRecord(ev[1], st));
-    HIPCHK(launch_pairlist(P, 0, S2, C.maxGroups, 4, st));
+    HIPCHK(launch_pairlist(P, 0, nsets, C.maxGroups, 4, st));
     const bool grads = (flags & ATM_EVAL_PARAM_GRADS) != 0;
-    if (uses_atom_lists(C, flags) && !grads) HIPCHK(launch_alist(P, 0, S2, C.maxGroups, st));
+    if (uses_atom_lists(C, flags) && !grads) HIPCHK(launch_alist(P, 0, nsets, C.maxGroups, st));
     if (ev) HIPCHK(hipEventRecord(ev[2], st));
     if (grads) {
         // the backward mode of the tile kernel with unit weights: energies, pair counts
         // and dU/d(q', h, e') of every set in one pass over the pairs
-        HIPCHK(launch_backward(P, C.totalSlots, 0, S2, C.maxGroups, nullptr, C.tuning.pairSplit, st));
+        HIPCHK(launch_backward(P, C.totalSlots, 0, nsets, C.maxGroups, nullptr, C.tuning.pairSplit, st));
     } else {
-        HIPCHK(launch_pairs(P, 0, S2, C.maxGroups, C.tuning.pairWaves, (flags & ATM_EVAL_FORCES) != 0,
-                            (flags & ATM_EVAL_FP32_PAIRS) == 0, C.tuning.pairKernel, C.tuning.pairSplit, st));
+        HIPCHK(launch_pairs(P, 0, nsets, C.maxGroups, C.tuning.pairWaves, forces, (flags & ATM_EVAL_FP32_PAIRS) == 0,
+                            C.tuning.pairKernel, C.tuning.pairSplit, st));
     }
-    if (has_exception_terms(C)) HIPCHK(launch_exceptions(P, 0, S2, (flags & ATM_EVAL_FORCES) != 0, st));
     if (ev) HIPCHK(hipEventRecord(ev[3], st));
-    HIPCHK(launch_energy(P, 0, C.nsys, grads ? C.tuning.pairSplit : energy_stride(C, flags), st));
+    if (has_exception_terms(C)) HIPCHK(launch_exceptions(P, 0, nsets, forces, st));
+    if (delta) {
+        HIPCHK(launch_displaced(P, 0, C.nsys, C.maxLig, forces, st));
+        if (!C.lxidx.empty()) HIPCHK(launch_displaced_exceptions(P, 0, C.nsys, forces, st));
+    }
     if (ev) HIPCHK(hipEventRecord(ev[4], st));
-    if (flags & ATM_EVAL_FORCES)
-        HIPCHK(launch_blend(P, 0, C.nsys, C.maxAtoms, 0, 0.0, 0.0, (flags & ATM_EVAL_CLEAR_FORCES) != 0, st));
+    HIPCHK(launch_energy(P, 0, C.nsys, grads ? C.tuning.pairSplit : energy_stride(C, flags), st, delta));
     if (ev) HIPCHK(hipEventRecord(ev[5], st));
+    if (forces)
+        HIPCHK(launch_blend(P, 0, C.nsys, C.maxAtoms, 0, 0.0, 0.0, (flags & ATM_EVAL_CLEAR_FORCES) != 0, st, delta));
+    if (ev) HIPCHK(hipEventRecord(ev[6], st));
+    C.lastDelta = delta;
 }
 
 void eval_batched(Context& C, int flags) {
@@ -513,7 +604,10 @@ void eval_batched(Context& C, int flags) {
         enqueue_batched(C, key);
         return;
     }
-    auto it = C.graphs.find(key);
+    // graphs are kept per flag set and per state mode (the mode follows the displacements)
+    const int gkey = key | (use_delta(C, key) ? (1 << 24) : 0);
+    C.lastDelta = use_delta(C, key);
+    auto it = C.graphs.find(gkey);
     if (it == C.graphs.end()) {
         hipGraph_t g = nullptr;
         HIPCHK(hipStreamBeginCapture(C.stream, hipStreamCaptureModeThreadLocal));
@@ -530,7 +624,7 @@ void eval_batched(Context& C, int flags) {
         hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
         HIPCHK(e);
-        it = C.graphs.emplace(key, ex).first;
+        it = C.graphs.emplace(gkey, ex).first;
     }
     HIPCHK(hipGraphLaunch(it->second, C.stream));
 }
@@ -553,7 +647,7 @@ void eval_plain(Context& C, int flags) {
         for (int stt = 0; stt < 2; ++stt) {
             const int c = 2 * s + stt;
             const SetDesc& D = C.sets[c];
-            HIPCHK(launch_zero_counts(P, D.cellOff, D.cellOff + D.nx * D.ny * D.nz, c, c + 1, nullptr, 0, st));
+            HIPCHK(launch_zero_counts(P, c, 1, D.nx * D.ny * D.nz, nullptr, 0, st));
             HIPCHK(launch_build(P, c, 1, D.natoms, D.nx * D.ny * D.nz, D.slotCap, st));
             HIPCHK(launch_pairlist(P, c, 1, D.groupCap, 4, st));
             if (uses_atom_lists(C, flags)) HIPCHK(launch_alist(P, c, 1, D.groupCap, st));
@@ -586,7 +680,7 @@ void eval_state1(Context& C, int flags) {
         const SetDesc& D = C.sets[2 * s];
         const int c = 2 * s;
         if (flags & ATM_EVAL_CLEAR_FORCES) HIPCHK(launch_fill_zero(P.outF + 3ll * D.atomOff, 3ll * D.natoms, st));
-        HIPCHK(launch_zero_counts(P, D.cellOff, D.cellOff + D.nx * D.ny * D.nz, c, c + 1, nullptr, 0, st));
+        HIPCHK(launch_zero_counts(P, c, 1, D.nx * D.ny * D.nz, nullptr, 0, st));
         HIPCHK(launch_build(P, c, 1, D.natoms, D.nx * D.ny * D.nz, D.slotCap, st));
         HIPCHK(launch_pairlist(P, c, 1, D.groupCap, 4, st));
         if (uses_atom_lists(C, flags)) HIPCHK(launch_alist(P, c, 1, D.groupCap, st));
@@ -803,6 +897,7 @@ int atm_set_particles(void* context, int32_t sys, const double* charges, const d
                               C.stream));
         HIPCHK(hipMemcpyAsync(C.dDispl.as<float4>() + off, C.hDispl.data() + off, n * sizeof(float4), hipMemcpyHostToDevice, C.stream));
         HIPCHK(hipStreamSynchronize(C.stream));
+        if (displacements) update_displaced(C);
         C.haveParticles[sys] = 1;
     });
 }
@@ -883,6 +978,7 @@ int atm_set_displacements(void* context, int32_t sys, const double* d) {
             C.hDispl[off + i] = make_float4((float)d[3 * i], (float)d[3 * i + 1], (float)d[3 * i + 2], 0.f);
         HIPCHK(hipMemcpyAsync(C.dDispl.as<float4>() + off, C.hDispl.data() + off, n * sizeof(float4), hipMemcpyHostToDevice, C.stream));
         HIPCHK(hipStreamSynchronize(C.stream));
+        update_displaced(C);
     });
 }
 
@@ -984,6 +1080,7 @@ int atm_eval(void* context, int32_t flags) {
         HIPCHK(hipSetDevice(C.device));
         C.resultsValid = false;
         C.gradsReady = false;
+        C.lastDelta = false;
         if (flags & ATM_EVAL_STATE1_ONLY) {
             eval_state1(C, flags);
             C.resultsOnHost = true;
@@ -1069,23 +1166,48 @@ int atm_get_pairs(void* context, int32_t sys, int32_t state, int32_t* pairs, int
         if (capacity > 0 && !pairs) fail(ATM_ERR_INVALID_ARGUMENT, "null pair buffer");
         HIPCHK(hipSetDevice(C.device));
         fetch_results(C);
-        const int c = 2 * sys + state;
         const long long cap = std::max<long long>(capacity, 0);
-        DevBuf out, ctr;
-        out.alloc(sizeof(int2) * std::max<long long>(cap, 1));
-        ctr.alloc(sizeof(unsigned long long) * 4);
-        zero_on_stream(C, ctr.p, sizeof(unsigned long long) * 4);
-        HIPCHK(launch_pairs_dump(C.P, c, C.sets[c].groupCap, out.as<int2>(), cap, ctr.as<unsigned long long>(), C.stream));
-        unsigned long long h[4];
-        to_host(C, h, ctr.p, sizeof(h));
-        *count = (int64_t)h[0];
-        const long long nout = std::min<long long>((long long)h[0], cap);
-        if (nout > 0) {
-            std::vector<int2> v(nout);
-            to_host(C, v.data(), out.p, sizeof(int2) * nout);
-            std::sort(v.begin(), v.end(), [](const int2& a, const int2& b) { return a.x != b.x ? a.x < b.x : a.y < b.y; });
-            for (long long k = 0; k < nout; ++k) { pairs[2 * k] = v[k].x; pairs[2 * k + 1] = v[k].y; }
+        auto less = [](const int2& a, const int2& b) { return a.x != b.x ? a.x < b.x : a.y < b.y; };
+        // every pair a set's lists hold (the traversal of the tile kernel), or the pairs the
+        // displaced-atom kernel evaluates for `st`, sorted
+        auto dump = [&](bool displaced, int st) {
+            DevBuf ctr;
+            ctr.alloc(sizeof(unsigned long long) * 4);
+            unsigned long long h[4];
+            long long n = 0;
+            std::vector<int2> v;
+            for (int pass = 0; pass < 2; ++pass) {  // count, then fetch
+                DevBuf out;
+                out.alloc(sizeof(int2) * std::max<long long>(n, 1));
+                zero_on_stream(C, ctr.p, sizeof(unsigned long long) * 4);
+                if (displaced)
+                    HIPCHK(launch_displaced_dump(C.P, sys, st, C.sets[2 * sys].ligCount, out.as<int2>(), n,
+                                                 ctr.as<unsigned long long>(), C.stream));
+                else
+                    HIPCHK(launch_pairs_dump(C.P, 2 * sys + st, C.sets[2 * sys + st].groupCap, out.as<int2>(), n,
+                                             ctr.as<unsigned long long>(), C.stream));
+                to_host(C, h, ctr.p, sizeof(h));
+                if (pass == 0) { n = (long long)h[0]; continue; }
+                v.resize(std::min<long long>((long long)h[0], n));
+                if (!v.empty()) to_host(C, v.data(), out.p, sizeof(int2) * v.size());
+            }
+            std::sort(v.begin(), v.end(), less);
+            return v;
+        };
+        std::vector<int2> v;
+        if (state == 1 && C.lastDelta) {
+            // x + d was evaluated as the difference from x: the state-0 pairs without those of
+            // the displaced atoms at x, with those at x + d
+            const std::vector<int2> all = dump(false, 0), d0 = dump(true, 0), d1 = dump(true, 1);
+            std::vector<int2> keep;
+            std::set_difference(all.begin(), all.end(), d0.begin(), d0.end(), std::back_inserter(keep), less);
+            std::merge(keep.begin(), keep.end(), d1.begin(), d1.end(), std::back_inserter(v), less);
+        } else {
+            v = dump(false, state);
         }
+        *count = (int64_t)v.size();
+        const long long nout = std::min<long long>((long long)v.size(), cap);
+        for (long long k = 0; k < nout; ++k) { pairs[2 * k] = v[k].x; pairs[2 * k + 1] = v[k].y; }
     });
 }
 
@@ -1101,6 +1223,7 @@ int atm_get_stats(void* context, int64_t* pairs_in_cutoff, int64_t* pairs_evalua
         if (pairs_evaluated) ctr.alloc(sizeof(unsigned long long) * 4);
         for (int c = 0; c < 2 * C.nsys; ++c) {
             nin += (long long)rt[c].npairs;  // counted by the pair kernel of the evaluation
+            if (C.lastDelta && (c & 1)) continue;  // x + d was not binned (state difference)
             if (pairs_evaluated) {
                 zero_on_stream(C, ctr.p, sizeof(unsigned long long) * 4);
                 HIPCHK(launch_pairs_dump(C.P, c, C.sets[c].groupCap, nullptr, 0, ctr.as<unsigned long long>(), C.stream));
@@ -1156,7 +1279,7 @@ int atm_get_tuning(void* context, atm_tuning* out) {
         out->pair_kernel = C.tuning.pairKernel;
         out->list_capacity = C.tuning.listCapacity;
         out->pair_split = C.tuning.pairSplit;
-        out->reserved_ = 0;
+        out->full_state2 = C.tuning.fullState2;
     });
 }
 
@@ -1176,6 +1299,7 @@ int atm_set_tuning(void* context, const atm_tuning* t) {
         if (t->list_capacity < 0) fail(ATM_ERR_INVALID_ARGUMENT, "list_capacity must be >= 0");
         if (t->pair_split != 1 && t->pair_split != 2 && t->pair_split != 4 && t->pair_split != 8)
             fail(ATM_ERR_INVALID_ARGUMENT, "pair_split must be 1, 2, 4 or 8");
+        if (t->full_state2 != 0 && t->full_state2 != 1) fail(ATM_ERR_INVALID_ARGUMENT, "full_state2 must be 0 or 1");
         HIPCHK(hipSetDevice(C.device));
         HIPCHK(hipStreamSynchronize(C.stream));
         const bool relayout = t->column_scale != C.tuning.colScale || t->zbins_per_column != C.tuning.zbinsPerColumn ||
@@ -1186,6 +1310,7 @@ int atm_set_tuning(void* context, const atm_tuning* t) {
         C.tuning.pairWaves = t->pair_waves_per_block;
         C.tuning.pairKernel = t->pair_kernel;
         C.tuning.pairSplit = t->pair_split;
+        C.tuning.fullState2 = t->full_state2;
         if (t->list_capacity != C.tuning.listCapacity) {
             C.entryCapOverride.assign(2 * C.nsys, 0);
             C.alistCapOverride.assign(2 * C.nsys, 0);
@@ -1308,7 +1433,7 @@ int atm_autotune(void* context, int32_t repeats, atm_tuning* out) {
             out->pair_kernel = best.pairKernel;
             out->list_capacity = best.listCapacity;
             out->pair_split = best.pairSplit;
-            out->reserved_ = 0;
+            out->full_state2 = best.fullState2;
         }
     });
 }
@@ -1321,14 +1446,14 @@ int atm_profile(void* context, int32_t flags, int32_t reps, atm_stage_times* out
         require_ready(C);
         HIPCHK(hipSetDevice(C.device));
         flush_params(C);
-        hipEvent_t ev[6];
+        hipEvent_t ev[7];
         for (auto& e : ev) HIPCHK(hipEventCreate(&e));
-        double acc[5] = {0, 0, 0, 0, 0};
+        double acc[6] = {0, 0, 0, 0, 0, 0};
         const int key = flags & (ATM_EVAL_FORCES | ATM_EVAL_CLEAR_FORCES | ATM_EVAL_FP32_PAIRS);
         for (int r = 0; r < reps; ++r) {
             enqueue_batched(C, key, ev);
-            HIPCHK(hipEventSynchronize(ev[5]));
-            for (int k = 0; k < 5; ++k) {
+            HIPCHK(hipEventSynchronize(ev[6]));
+            for (int k = 0; k < 6; ++k) {
                 float ms = 0.f;
                 HIPCHK(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
                 acc[k] += ms;
@@ -1342,9 +1467,10 @@ int atm_profile(void* context, int32_t flags, int32_t reps, atm_stage_times* out
         out->build_ms = acc[0] / reps;
         out->pairlist_ms = acc[1] / reps;
         out->pairs_ms = acc[2] / reps;
-        out->energy_ms = acc[3] / reps;
-        out->blend_ms = acc[4] / reps;
-        out->total_ms = (acc[0] + acc[1] + acc[2] + acc[3] + acc[4]) / reps;
+        out->displaced_ms = acc[3] / reps;
+        out->energy_ms = acc[4] / reps;
+        out->blend_ms = acc[5] / reps;
+        out->total_ms = (acc[0] + acc[1] + acc[2] + acc[3] + acc[4] + acc[5]) / reps;
         out->reps = reps;
     });
 }
@@ -1371,7 +1497,19 @@ void backward_impl(Context& C, const double* grad_energy, double* d_charges, dou
     if (d_charges || d_sigmas || d_epsilons) {
         // after an evaluation with ATM_EVAL_PARAM_GRADS the accumulators hold dU/dp of
         // each set: only the weighted gather remains; otherwise one weighted pass
-        const bool fused = C.gradsReady;
+        bool fused = C.gradsReady;
+        if (!fused && C.lastDelta) {
+            // after a state-difference evaluation x + d has no lists of its own: evaluate
+            // with parameter gradients (both states in full, unit weights), then gather
+            // with the weights computed above
+            eval_batched(C, ATM_EVAL_PARAM_GRADS);
+            C.evaluated = true;
+            C.resultsValid = false;
+            C.resultsOnHost = false;
+            C.lastDelta = false;
+            fetch_results(C);  // list overflow: relays out and reports ATM_ERR_CAPACITY
+            C.gradsReady = fused = true;
+        }
         ensure_grad_buffers(C);
         if (C.dWset.bytes < sizeof(double) * 2 * S) C.dWset.alloc(sizeof(double) * 2 * S);
         HIPCHK(hipMemcpyAsync(C.dWset.p, wset.data(), sizeof(double) * 2 * S, hipMemcpyHostToDevice, C.stream));

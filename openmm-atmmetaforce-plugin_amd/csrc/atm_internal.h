@@ -20,6 +20,12 @@
 //     atoms); the pair kernels drop every excepted pair.  Exception terms with
 //     a nonzero charge product or epsilon are evaluated by a kernel of their own
 //     (one block per set) at the plain, non-periodic distance.
+//   * State difference: x + d differs from x only at the displaced atoms
+//     (d != 0).  When they are few, state 1 is not binned or paired: the
+//     state-0 set is, and a displaced-atom kernel evaluates every pair with a
+//     displaced atom at x and at x + d (neighbours from the state-0 cells), so
+//     U(x + d) = U(x) - E_d(x) + E_d(x + d) and F(x + d) = F(x) + dF, dF held in
+//     atom order in the state-1 set's fixed-point sums.
 #pragma once
 #include <stdint.h>
 
@@ -62,6 +68,10 @@ struct SetDesc {
     float fixMax;            // partial forces >= fixMax (kJ/mol/nm) go to the FP64 side sums
     int excCount;            // exception terms of this set's system (nonzero charge product or epsilon)
     int excOff;              // its first exception term in the batch arrays
+    int ligCount;            // displaced atoms (d != 0) of the system
+    int ligOff;              // their first entry in the batch's displaced-atom list
+    int lxCount;             // exception terms with a displaced atom
+    int lxOff;               // their first entry in the batch's list of such terms
     int pad_;
 };
 
@@ -74,6 +84,8 @@ struct SetRuntime {
     unsigned long long npairs;  // in-cutoff pairs the pair kernel evaluated (this evaluation)
     int maxAtoms;  // largest untruncated atom-level list of one cluster (this evaluation)
     int pad_;
+    unsigned long long lpairs[2];  // state-difference evaluation: in-cutoff pairs with a displaced
+                                   // atom at x ([0]) and at x + d ([1]) (state-1 set of a system)
 };
 
 // Per-system state of the last evaluation on the device (mirrors atm_result).

@@ -61,16 +61,22 @@ struct DevPtrs {
     const double4* xprm;   // (k_e q'q, sigma, 4 epsilon, 0)
     double* xenergy;       // [2 S] exception energy per set
     const double* eoff;    // [2 S] energies of other variable-group forces added to U per set (may be null)
+    // state difference (state-0 sets paired, displaced atoms evaluated at both states)
+    int setStride = 1;     // 1: every set; 2: the state-0 sets only (set c = setBegin + 2 k)
+    const int* ligAtoms;   // displaced atoms of every system (system-local, at SetDesc::ligOff)
+    const unsigned char* ligMark;  // per atom (batch order): 1 if displaced
+    double* lenergy;       // [2 x displaced atoms x kDisplacedParts]: pair energy of each at x and at x + d
+    const int* lxidx;      // exception terms (batch index) with a displaced atom, at SetDesc::lxOff
     // backward
     double* gq;            // [2 S sets x slots] gradient accumulators in slot order
     double* gsig;
     double* geps;
 };
 
-// Clears cell counters [cellBegin, cellEnd), the pair counters of sets [setBegin, setEnd)
-// and, when nf > 0, the doubles f[0, nf).
-hipError_t launch_zero_counts(const DevPtrs& P, int cellBegin, int cellEnd, int setBegin, int setEnd, double* f,
-                              long long nf, hipStream_t st);
+// Clears the cell counters and pair counters of sets setBegin + k P.setStride, k < nsets
+// (maxCells: the largest cell count among them) and, when nf > 0, the doubles f[0, nf).
+hipError_t launch_zero_counts(const DevPtrs& P, int setBegin, int nsets, int maxCells, double* f, long long nf,
+                              hipStream_t st);
 hipError_t launch_build(const DevPtrs& P, int setBegin, int nsets, int maxAtoms, int maxCells, int maxSlots,
                         hipStream_t st);
 hipError_t launch_pairlist(const DevPtrs& P, int setBegin, int nsets, int maxGroups, int listWaves, hipStream_t st);
@@ -99,7 +105,10 @@ inline int pair_energy_stride(int variant, bool fp64, int split) {
 // `forces` their forces added to the set's slot sums (call after the pair kernel
 // and before launch_energy / launch_set_energy, which add xenergy when excCount > 0).
 hipError_t launch_exceptions(const DevPtrs& P, int setBegin, int nsets, bool forces, hipStream_t st);
-hipError_t launch_energy(const DevPtrs& P, int sysBegin, int nsys, int estride, hipStream_t st);
+// Waves per (displaced atom, state) in the displaced-atom kernel (each its own energy slot).
+constexpr int kDisplacedParts = 1;
+// delta: the state-difference evaluation (state-0 sets paired, displaced atoms by launch_displaced)
+hipError_t launch_energy(const DevPtrs& P, int sysBegin, int nsys, int estride, hipStream_t st, bool delta = false);
 // withOffsets: add eoff[c] (an ATM evaluation; not a plain state-1 one)
 hipError_t launch_set_energy(const DevPtrs& P, int c, int estride, bool withOffsets, hipStream_t st);
 // Backward pass of the pair terms (the tile kernel's backward mode): adds wset[c] dE/d(q', h, e')
@@ -110,7 +119,16 @@ hipError_t launch_backward_pairs(const DevPtrs& P, int setBegin, int nsets, int 
                                  int split, hipStream_t st);
 // clear: write the blended force instead of adding it to outF.
 hipError_t launch_blend(const DevPtrs& P, int sysBegin, int nsys, int maxAtoms, int hostMode, double sp, double dir,
-                        bool clear, hipStream_t st);
+                        bool clear, hipStream_t st, bool delta = false);
+// State difference: the pairs with a displaced atom of systems [sysBegin, sysBegin + nsys) at
+// x and at x + d (after the state-0 sets' build; maxLig: most displaced atoms of one system),
+// and the exception terms with a displaced atom (their difference).
+hipError_t launch_displaced(const DevPtrs& P, int sysBegin, int nsys, int maxLig, bool forces, hipStream_t st);
+hipError_t launch_displaced_exceptions(const DevPtrs& P, int sysBegin, int nsys, bool forces, hipStream_t st);
+// The pairs the displaced-atom kernel evaluates for system `sys` at `state` (lower index first),
+// appended to out[ctr[0]++] while below cap.
+hipError_t launch_displaced_dump(const DevPtrs& P, int sys, int state, int nLig, int2* out, long long cap,
+                                 unsigned long long* ctr, hipStream_t st);
 hipError_t launch_fill_i64(long long* p, long long n, hipStream_t st);
 hipError_t launch_fill_zero(void* p, long long n8, hipStream_t st);
 hipError_t launch_pairs_dump(const DevPtrs& P, int c, int maxGroups, int2* out, long long cap,

@@ -243,21 +243,34 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
 
 // ---------------------------------------------------------------- binning
 
-__global__ void k_zero_counts(DevPtrs P, int cellBegin, int cellEnd, int setBegin, int setEnd, double* f,
-                              long long nf) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    for (int k = cellBegin + i; k < cellEnd; k += gridDim.x * blockDim.x) P.cellCount[k] = 0;
-    for (long long k = i; k < nf; k += (long long)gridDim.x * blockDim.x) f[k] = 0.0;
+__global__ void k_zero_counts(DevPtrs P, int setBegin, double* f, long long nf) {
+    // blockIdx.y: set setBegin + blockIdx.y * setStride (its cell counters and pair counters)
+    const int c = setBegin + (int)blockIdx.y * P.setStride;
+    const SetDesc& D = P.desc[c];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ncells = D.nx * D.ny * D.nz;
+    for (int k = i; k < ncells; k += gridDim.x * blockDim.x) P.cellCount[D.cellOff + k] = 0;
+    // outF (when the evaluation clears it without a blend): the blocks of the first set
+    if (blockIdx.y == 0)
+        for (long long k = i; k < nf; k += (long long)gridDim.x * blockDim.x) f[k] = 0.0;
     // the overflow flag stays set until the host has seen it (it relays out and clears it)
-    if (i < setEnd - setBegin) {
-        P.rt[setBegin + i].npairs = 0ull;
-        P.rt[setBegin + i].maxEntries = 0;
-        P.rt[setBegin + i].maxAtoms = 0;
+    if (i == 0) {
+        P.rt[c].npairs = 0ull;
+        P.rt[c].maxEntries = 0;
+        P.rt[c].maxAtoms = 0;
+        P.rt[c].lpairs[0] = 0ull;
+        P.rt[c].lpairs[1] = 0ull;
+        if (P.setStride == 2 && (c & 1) == 0) {
+            // state difference: the state-1 set's counters hold the derived state-1 values
+            P.rt[c + 1].npairs = 0ull;
+            P.rt[c + 1].maxEntries = 0;
+            P.rt[c + 1].maxAtoms = 0;
+        }
     }
 }
 
 __global__ void k_bin(DevPtrs P, int setBegin) {
-    const int c = setBegin + blockIdx.y;
+    const int c = setBegin + (int)blockIdx.y * P.setStride;
     const SetDesc& D = P.desc[c];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= D.natoms) return;
@@ -279,6 +292,21 @@ __global__ void k_bin(DevPtrs P, int setBegin) {
     P.bpos[k] = p;
     P.bcell[k] = cell;
     P.brank[k] = r;
+    if (P.setStride == 2 && (c & 1) == 0) {
+        // state difference: clear dF of atom i (the state-1 set's sums, by atom) and,
+        // after an evaluation that used them, its FP64 side sums and their mask word
+        // (the 64 atoms of a word are one wave's: set slot ranges start on multiples of 64)
+        const long long k2 = P.desc[c + 1].slotOff + i;
+        if (P.rt[c + 1].big) {
+            const unsigned long long m = P.sdMask[k2 >> 6];
+            if ((m >> (k2 & 63)) & 1ull) {
+                double* d = P.sd + 3 * k2;
+                d[0] = 0.0; d[1] = 0.0; d[2] = 0.0;
+            }
+            if ((k2 & 63) == 0 && m != 0ull) P.sdMask[k2 >> 6] = 0ull;
+        }
+        P.sf[k2] = 0; P.sf[P.sfn + k2] = 0; P.sf[2 * P.sfn + k2] = 0;
+    }
 }
 
 __device__ __forceinline__ int wave_incl_scan(int v) {
@@ -300,7 +328,7 @@ __device__ __forceinline__ int wave_incl_scan(int v) {
 // A cell's first slot is colStart[column] + cellStart[cell].
 __global__ void __launch_bounds__(1024) k_scan(DevPtrs P, int setBegin) {
     constexpr int NT = 1024, NW = NT / 64;
-    const int c = setBegin + blockIdx.x;
+    const int c = setBegin + (int)blockIdx.x * P.setStride;
     const SetDesc& D = P.desc[c];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int ncols = D.nx * D.ny, nz = D.nz;
@@ -365,7 +393,7 @@ __global__ void __launch_bounds__(1024) k_scan(DevPtrs P, int setBegin) {
 // Counting-sort scatter: each atom's (index, cell) into its provisional slot
 // (cell start + the atomic rank of k_bin, whose order is not deterministic).
 __global__ void k_scatter_idx(DevPtrs P, int setBegin) {
-    const int c = setBegin + blockIdx.y;
+    const int c = setBegin + (int)blockIdx.y * P.setStride;
     const SetDesc& D = P.desc[c];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= D.natoms) return;
@@ -449,7 +477,7 @@ __device__ __forceinline__ float oct_max(float v) {
 // reduce each cluster's bounding box over its 8 lanes.  Slot counts are
 // multiples of 8, so a cluster's lanes are all active or all gone.
 __global__ void __launch_bounds__(256) k_gather_bbox(DevPtrs P, int setBegin) {
-    const int c = setBegin + blockIdx.y;
+    const int c = setBegin + (int)blockIdx.y * P.setStride;
     const SetDesc& D = P.desc[c];
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     // FP64 side sums: after an evaluation that used them (k_pairlist then resets
@@ -513,11 +541,14 @@ __global__ void __launch_bounds__(256) k_gather_bbox(DevPtrs P, int setBegin) {
 // all 64 lanes test clusters, whatever the column occupancy.
 template <int W>
 __global__ void __launch_bounds__(64 * W) k_pairlist(DevPtrs P, int setBegin) {
-    const int c = setBegin + blockIdx.y;
+    const int c = setBegin + (int)blockIdx.y * P.setStride;
     const SetDesc D = P.desc[c];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int g = blockIdx.x * W + wave;
-    if (g == 0 && lane == 0) P.rt[c].big = 0;  // the side sums were cleared by k_gather_bbox
+    if (g == 0 && lane == 0) {
+        P.rt[c].big = 0;  // the side sums were cleared by k_gather_bbox
+        if (P.setStride == 2 && (c & 1) == 0) P.rt[c + 1].big = 0;  // dF's, by k_bin
+    }
     if (g >= P.rt[c].ngroups) return;
     const int4 gi = P.ginfo[D.groupOff + g];
     const int g0 = __builtin_amdgcn_readfirstlane(gi.x), ncl = __builtin_amdgcn_readfirstlane(gi.y);
@@ -783,7 +814,7 @@ template <int W, bool FORCES, bool F64, bool BWD, bool EXC>
 __global__ void __launch_bounds__(64 * W, 6) k_pairs(DevPtrs P, int setBegin, const double* wset, int split) {
     static_assert(!BWD || (F64 && !FORCES), "the backward mode is FP64 and computes no forces");
     constexpr bool JSUMS = FORCES || BWD;  // per-atom j sums flushed every kJFlush entries
-    const int c = setBegin + blockIdx.y;
+    const int c = setBegin + (int)blockIdx.y * P.setStride;
     const SetDesc D = P.desc[c];  // by value: lives in SGPRs for the whole kernel
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int jj = lane >> 3, ii = lane & 7;
@@ -1073,7 +1104,7 @@ constexpr uint32_t kNoAtom = 0xFFFFFFFFu;
 __global__ void __launch_bounds__(64) k_alist(DevPtrs P, int setBegin) {
     // one wave per (group, i-cluster t): lane = 8 e + jj tests atom jj of list
     // entry e of the current 8-entry window against t's bounding box
-    const int c = setBegin + blockIdx.y;
+    const int c = setBegin + (int)blockIdx.y * P.setStride;
     const SetDesc D = P.desc[c];
     const int g = blockIdx.x >> 3, t = blockIdx.x & 7, lane = threadIdx.x;
     if (g >= P.rt[c].ngroups) return;
@@ -1134,7 +1165,7 @@ __global__ void __launch_bounds__(64) k_alist(DevPtrs P, int setBegin) {
 // diagonal cluster i < j).
 template <int W, bool FORCES>
 __global__ void __launch_bounds__(64 * W) k_pairs_a(DevPtrs P, int setBegin) {
-    const int c = setBegin + blockIdx.y;
+    const int c = setBegin + (int)blockIdx.y * P.setStride;
     const SetDesc D = P.desc[c];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int jj = lane >> 3, ii = lane & 7;
@@ -1288,7 +1319,7 @@ __global__ void __launch_bounds__(64 * W) k_pairs_q(DevPtrs P, int setBegin) {
     constexpr int NJ = 2 * NH;
     constexpr int JS = NJ + 1;             // odd row stride of the j accumulators
     constexpr int QCAP = STEP ? 128 : 1024;  // pending pairs < 64 + 64 (STEP) or 64 + 8 * 64
-    const int c = setBegin + blockIdx.y;
+    const int c = setBegin + (int)blockIdx.y * P.setStride;
     const SetDesc D = P.desc[c];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int jj = lane >> 3, ii = lane & 7;
@@ -1599,7 +1630,7 @@ constexpr int kFiltEnergy = 0, kFiltForces = 1;
 template <int W, int MODE>
 __global__ void __launch_bounds__(64 * W, ATM_FILT_MINW) k_pairs_f(DevPtrs P, int setBegin) {
     constexpr bool FORCES = MODE == kFiltForces;
-    const int c = setBegin + blockIdx.y;
+    const int c = setBegin + (int)blockIdx.y * P.setStride;
     const SetDesc D = P.desc[c];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int gw = blockIdx.x * W + wave;
@@ -1789,13 +1820,27 @@ __device__ __forceinline__ void add_force64(const DevPtrs& P, int c, double fixM
     if (big) mark_side(P, c, slot);
 }
 
+// One exception term at the plain separation (dx, dy, dz) = a - b: its energy
+// k_e q'q / r + 4 eps' ((s'/r)^12 - (s'/r)^6) (q = (k_e q'q, sigma, 4 eps', 0)),
+// and fs, the force on a over the separation (F_a = fs (dx, dy, dz)).
+__device__ __forceinline__ double exception_term(const double4& q, double dx, double dy, double dz, double& fs) {
+    const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+    const double rinv = 1.0 / sqrt(r2);
+    const double r2inv = rinv * rinv;
+    const double s2 = q.y * q.y * r2inv, s6 = s2 * s2 * s2;
+    const double elj = q.z * fma(s6, s6, -s6);
+    const double ec = q.x * rinv;
+    fs = fma(q.z, fma(12.0 * s6, s6, -6.0 * s6), ec) * r2inv;
+    return elj + ec;
+}
+
 // One block per set: the exception terms k_e q'q / r + 4 eps' ((s'/r)^12 - (s'/r)^6)
 // of its system at the plain distance of the set's coordinates (x, or x + d for
 // state 1; no cutoff, no periodic image), as NonbondedForce computes its
 // exceptions by default (exceptions do not use periodic boundary conditions).
 // FP64 throughout; energy summed per thread in a fixed order, then a fixed tree.
 __global__ void __launch_bounds__(256) k_exceptions(DevPtrs P, int setBegin, int forces) {
-    const int c = setBegin + blockIdx.x;
+    const int c = setBegin + (int)blockIdx.x * P.setStride;
     const SetDesc& D = P.desc[c];
     __shared__ double red[256];
     const int n = D.excCount;
@@ -1815,15 +1860,9 @@ __global__ void __launch_bounds__(256) k_exceptions(DevPtrs P, int setBegin, int
             bx += (double)db.x; by += (double)db.y; bz += (double)db.z;
         }
         const double dx = ax - bx, dy = ay - by, dz = az - bz;
-        const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
-        const double rinv = 1.0 / sqrt(r2);
-        const double r2inv = rinv * rinv;
-        const double s2 = q.y * q.y * r2inv, s6 = s2 * s2 * s2;
-        const double elj = q.z * fma(s6, s6, -s6);
-        const double ec = q.x * rinv;
-        e += elj + ec;
+        double fs;
+        e += exception_term(q, dx, dy, dz, fs);
         if (forces) {
-            const double fs = fma(q.z, fma(12.0 * s6, s6, -6.0 * s6), ec) * r2inv;
             const long long si = D.slotOff + P.slotOf[D.setAtomOff + pr.x];
             const long long sj = D.slotOff + P.slotOf[D.setAtomOff + pr.y];
             add_force64(P, c, fixMax, si, fs * dx, fs * dy, fs * dz);
@@ -1839,15 +1878,283 @@ __global__ void __launch_bounds__(256) k_exceptions(DevPtrs P, int setBegin, int
     if (threadIdx.x == 0) P.xenergy[c] = red[0];
 }
 
+// ---------------------------------------------------------------- state difference
+//
+// When few atoms are displaced, x + d is not binned: every pair with a displaced
+// atom is evaluated at x and at x + d from the state-0 cells (the other atoms do
+// not move), U(x + d) = U(x) - E_d(x) + E_d(x + d) and F(x + d) = F(x) + dF.
+
+// Displaced atom a of a system at state st (0: x, 1: x + d), wrapped as k_bin does.
+__device__ __forceinline__ float4 displaced_position(const DevPtrs& P, const SetDesc& D, int a, int st) {
+    float4 p = P.posq[D.atomOff + a];
+    if (st) {
+        const float4 d = P.displ[D.atomOff + a];
+        p.x += d.x; p.y += d.y; p.z += d.z;  // exact on the 2^-20 grid
+    }
+    p.x = wrap_coord(p.x, D.L[0], D.invL[0]);
+    p.y = wrap_coord(p.y, D.L[1], D.invL[1]);
+    p.z = wrap_coord(p.z, D.L[2], D.invL[2]);
+    return p;
+}
+
+__device__ __forceinline__ bool same_displacement(const DevPtrs& P, const SetDesc& D, int a, int b) {
+    const float4 da = P.displ[D.atomOff + a], db = P.displ[D.atomOff + b];
+    return da.x == db.x && da.y == db.y && da.z == db.z;
+}
+
+// One wave per (displaced atom, state) of system sysBegin + blockIdx.y: its
+// in-cutoff pairs with the atoms that are not displaced (from the state-0 cells
+// within the cutoff of its position at that state) and with the displaced atoms
+// after it in the list whose displacement differs (pairs of atoms displaced alike
+// keep their separation, so they are the same at both states and are left to
+// the state-0 evaluation).  Exact FP64 r^2 < rc^2 and the FP64 pair function of
+// the state-0 tiles, exceptions dropped as there.  The wave's energy goes to
+// lenergy[2 (ligOff + k) + st], its pair count to rt[state-0 set].lpairs[st];
+// with FORCES, -F (state 0) or +F (state 1) goes to dF, the state-1 set's
+// fixed-point sums indexed by atom.  DUMP: the pairs (lower index first) are
+// written to out instead (state dumpState, no energy or force).
+template <bool FORCES, bool EXC, bool DUMP>
+__global__ void __launch_bounds__(256) k_displaced(DevPtrs P, int sysBegin, int dumpState, int2* out, long long cap,
+                                                   unsigned long long* ctr) {
+    const int s = sysBegin + blockIdx.y;
+    const int c0 = 2 * s, c1 = 2 * s + 1;
+    const SetDesc D = P.desc[c0];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int w = blockIdx.x * 4 + wave;
+    const int nL = D.ligCount;
+    // kDisplacedParts waves per (displaced atom, state): part q takes rounds q, q + parts, ...
+    // of the candidates (part 0 also the displaced partners)
+    int st, k, part;
+    if (DUMP) { st = dumpState; k = w / kDisplacedParts; part = w % kDisplacedParts; if (k >= nL) return; }
+    else {
+        if (w >= 2 * nL * kDisplacedParts) return;
+        part = w % kDisplacedParts;
+        st = (w / kDisplacedParts) & 1;
+        k = w / (2 * kDisplacedParts);
+    }
+    const int a = P.ligAtoms[D.ligOff + k];
+    const float4 p = displaced_position(P, D, a, st);
+    const double4 pa = P.prm64[D.atomOff + a];
+    const unsigned long long ma = EXC ? P.xmask[D.atomOff + a] : 0ull;
+    const double fixMax = P.desc[c1].fixMax;
+    const long long dOff = P.desc[c1].slotOff;  // dF of atom b at dOff + b
+    const double sgn = st ? 1.0 : -1.0;
+    double eacc = 0.0, fax = 0.0, fay = 0.0, faz = 0.0;
+    uint32_t npair = 0;
+    // one candidate: b at pb with FP64 parameters pbq; nb: slot or atom for dF
+    auto pair = [&](int b, float4 pb, double4 pbq) __attribute__((always_inline)) {
+        const float dx = min_image(p.x - pb.x, D.L[0], D.invL[0]);
+        const float dy = min_image(p.y - pb.y, D.L[1], D.invL[1]);
+        const float dz = min_image(p.z - pb.z, D.L[2], D.invL[2]);
+        const double r2 = fma((double)dx, (double)dx, fma((double)dy, (double)dy, (double)dz * (double)dz));
+        bool in = r2 < D.rc2;
+        if (EXC && in) in = !excluded_pair(P, D, a, b, ma);
+        if (!in) return;
+        ++npair;
+        if (DUMP) {
+            const unsigned long long at = atomicAdd(ctr, 1ull);
+            if ((long long)at < cap) out[at] = make_int2(min(a, b), max(a, b));
+            return;
+        }
+        const PairEF64 pe = pair_ef64(true, r2, pa.x + pbq.x, pa.y * pbq.y, pa.z * pbq.z, D.krf64, D.crf64);
+        eacc += pe.e;
+        if (FORCES) {
+            const double fx = pe.fs * dx, fy = pe.fs * dy, fz = pe.fs * dz;
+            fax += fx; fay += fy; faz += fz;
+            add_force64(P, c1, fixMax, dOff + b, -sgn * fx, -sgn * fy, -sgn * fz);
+        }
+    };
+    // The state-0 cells within reach.  One column per lane (columns within Rx + 1,
+    // Ry + 1: one more than the list search needs, FP32 binning may place an atom one
+    // column over); a column whose rectangle is farther than rc (+ 1e-3 nm) in x-y is
+    // dropped, the others take the z-bins within sqrt(rc^2 - rho^2) of the atom
+    // (+ one bin each side), as one or two slot ranges (two when the window wraps).
+    // The ranges are flattened by a wave prefix sum; each lane then takes 4 candidates
+    // per round (their loads issued together), found by binary search over the lanes'
+    // range starts in LDS.
+    __shared__ int sEx[4][64], sB0[4][64], sL0[4][64], sB1[4][64];
+    const int cx = (int)fminf(fmaxf(p.x * D.invCe[0], 0.f), (float)(D.nx - 1));
+    const int cy = (int)fminf(fmaxf(p.y * D.invCe[1], 0.f), (float)(D.ny - 1));
+    const int rx = min(D.Rx + 1, D.nx), ry = min(D.Ry + 1, D.ny);
+    const int nxs = min(2 * rx + 1, D.nx), nys = min(2 * ry + 1, D.ny);
+    const float lim = D.rc + 1e-3f, lim2 = lim * lim;
+    for (int c0 = 0; c0 < nxs * nys; c0 += 64) {
+        int b0 = 0, l0 = 0, b1 = 0, l1 = 0;
+        const int l = c0 + lane;
+        if (l < nxs * nys) {
+            const int ix = l / nys, iy = l - (l / nys) * nys;
+            int ccx = nxs == D.nx ? ix : cx - rx + ix;
+            ccx += ccx < 0 ? D.nx : (ccx >= D.nx ? -D.nx : 0);
+            int ccy = nys == D.ny ? iy : cy - ry + iy;
+            ccy += ccy < 0 ? D.ny : (ccy >= D.ny ? -D.ny : 0);
+            const float ex = fmaxf(fabsf(min_image(p.x - ((float)ccx + 0.5f) * D.ce[0], D.L[0], D.invL[0])) -
+                                       0.5f * D.ce[0], 0.f);
+            const float ey = fmaxf(fabsf(min_image(p.y - ((float)ccy + 0.5f) * D.ce[1], D.L[1], D.invL[1])) -
+                                       0.5f * D.ce[1], 0.f);
+            const float rho2 = ex * ex + ey * ey;
+            if (rho2 <= lim2) {
+                const int col = ccx * D.ny + ccy;
+                const int cs = P.colStart[D.colOff + col];
+                const int* ccs = P.cellStart + D.cellOff + (long long)col * D.nz;
+                const int* ccn = P.cellCount + D.cellOff + (long long)col * D.nz;
+                const float hz = sqrtf(lim2 - rho2);
+                int za0 = (int)floorf((p.z - hz) * D.invZe) - 1, za1 = (int)floorf((p.z + hz) * D.invZe) + 1;
+                int zb0 = 1, zb1 = 0;
+                if (za1 - za0 + 1 >= D.nz) { za0 = 0; za1 = D.nz - 1; }
+                else if (za0 < 0) { zb0 = za0 + D.nz; zb1 = D.nz - 1; za0 = 0; }
+                else if (za1 >= D.nz) { zb0 = 0; zb1 = za1 - D.nz; za1 = D.nz - 1; }
+                b0 = cs + ccs[za0];
+                l0 = ccs[za1] + ccn[za1] - ccs[za0];
+                if (zb0 <= zb1) {
+                    b1 = cs + ccs[zb0];
+                    l1 = ccs[zb1] + ccn[zb1] - ccs[zb0];
+                }
+            }
+        }
+        const int tot = l0 + l1;
+        const int incl = wave_incl_scan(tot);
+        const int T = __shfl(incl, 63, 64);
+        sEx[wave][lane] = incl - tot;
+        sB0[wave][lane] = b0; sL0[wave][lane] = l0; sB1[wave][lane] = b1;
+        wave_sync();
+        for (int t0 = 256 * part; t0 < T; t0 += 256 * kDisplacedParts) {
+            long long sl[4];
+            int bj[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int t = t0 + 64 * u + lane;
+                sl[u] = -1;
+                if (t < T) {
+                    int lo = 0;
+#pragma unroll
+                    for (int step = 32; step > 0; step >>= 1)
+                        if (sEx[wave][lo + step] <= t && lo + step < 64) lo += step;
+                    const int off = t - sEx[wave][lo];
+                    const int l0u = sL0[wave][lo];
+                    sl[u] = D.slotOff + (off < l0u ? sB0[wave][lo] + off : sB1[wave][lo] + (off - l0u));
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) bj[u] = sl[u] >= 0 ? P.sidx[sl[u]] : -1;
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (bj[u] >= 0 && P.ligMark[D.atomOff + bj[u]]) bj[u] = -1;
+            float4 pj[4];
+            double4 qj[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (bj[u] >= 0) { pj[u] = P.spos[sl[u]]; qj[u] = P.sprm64[sl[u]]; }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (bj[u] >= 0) pair(bj[u], pj[u], qj[u]);
+        }
+        wave_sync();  // the next chunk's columns overwrite the LDS ranges
+    }
+    // displaced partners after a in the list, displaced differently
+    for (int k0 = k + 1 + 64 * part; k0 < nL; k0 += 64 * kDisplacedParts) {
+        const int kk = k0 + lane;
+        if (kk < nL) {
+            const int b = P.ligAtoms[D.ligOff + kk];
+            if (!same_displacement(P, D, a, b)) pair(b, displaced_position(P, D, b, st), P.prm64[D.atomOff + b]);
+        }
+    }
+    const unsigned long long np = (unsigned long long)wave_sum_d((double)npair);
+    if (DUMP) return;
+    eacc = wave_sum_d(eacc);
+    if (FORCES) {
+        fax = wave_sum_d(fax); fay = wave_sum_d(fay); faz = wave_sum_d(faz);
+    }
+    if (lane == 0) {
+        P.lenergy[(2ll * (D.ligOff + k) + st) * kDisplacedParts + part] = eacc;
+        atomicAdd(&P.rt[c0].lpairs[st], np);
+        if (FORCES) add_force64(P, c1, fixMax, dOff + a, sgn * fax, sgn * fay, sgn * faz);
+    }
+}
+
+// One block per system: the exception terms with a displaced atom whose partner
+// is displaced differently, at x + d minus at x: energy into xenergy[state-1 set],
+// with forces their difference into dF.
+__global__ void __launch_bounds__(256) k_displaced_exceptions(DevPtrs P, int sysBegin, int forces) {
+    const int s = sysBegin + blockIdx.x;
+    const int c1 = 2 * s + 1;
+    const SetDesc& D = P.desc[2 * s];
+    __shared__ double red[256];
+    const float4* X = P.posq + D.atomOff;
+    const float4* Xd = P.displ + D.atomOff;
+    const double fixMax = P.desc[c1].fixMax;
+    const long long dOff = P.desc[c1].slotOff;
+    double e = 0.0;
+    for (int k = threadIdx.x; k < D.lxCount; k += 256) {
+        const int x = P.lxidx[D.lxOff + k];
+        const int2 pr = P.xpair[x];
+        const double4 q = P.xprm[x];
+        const float4 a = X[pr.x], b = X[pr.y], da = Xd[pr.x], db = Xd[pr.y];
+        const double dx0 = (double)a.x - (double)b.x, dy0 = (double)a.y - (double)b.y, dz0 = (double)a.z - (double)b.z;
+        const double dx1 = ((double)a.x + (double)da.x) - ((double)b.x + (double)db.x);
+        const double dy1 = ((double)a.y + (double)da.y) - ((double)b.y + (double)db.y);
+        const double dz1 = ((double)a.z + (double)da.z) - ((double)b.z + (double)db.z);
+        double fs0, fs1;
+        const double e0 = exception_term(q, dx0, dy0, dz0, fs0);
+        const double e1 = exception_term(q, dx1, dy1, dz1, fs1);
+        e += e1 - e0;
+        if (forces) {
+            add_force64(P, c1, fixMax, dOff + pr.x, fs1 * dx1 - fs0 * dx0, fs1 * dy1 - fs0 * dy0, fs1 * dz1 - fs0 * dz0);
+            add_force64(P, c1, fixMax, dOff + pr.y, fs0 * dx0 - fs1 * dx1, fs0 * dy0 - fs1 * dy1, fs0 * dz0 - fs1 * dz1);
+        }
+    }
+    red[threadIdx.x] = e;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) P.xenergy[c1] = red[0];
+}
+
 // ---------------------------------------------------------------- ATM combine
 
 // One block of 256 threads per system: fixed-order FP64 sums of the energy
 // slots of both states (estride: pair_energy_stride), then the scalar ATM combine.
-__global__ void __launch_bounds__(256) k_energy(DevPtrs P, int sysBegin, int estride) {
+__device__ __forceinline__ double block_sum_256(double v, double* red) {
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    const double r = red[0];
+    __syncthreads();
+    return r;
+}
+
+// delta: state difference (only the state-0 set was paired): U2 = U1 - E_d(x) +
+// E_d(x + d) + the displaced exception terms' difference, offsets per state; the
+// state-1 pair count is derived the same way.
+__global__ void __launch_bounds__(256) k_energy(DevPtrs P, int sysBegin, int estride, int delta) {
     const int s = sysBegin + blockIdx.x;
     __shared__ double red[256];
     double U[2];
-    for (int st = 0; st < 2; ++st) {
+    if (delta) {
+        const int c0 = 2 * s, c1 = 2 * s + 1;
+        const SetDesc& D = P.desc[c0];
+        const int ng = P.rt[c0].ngroups;
+        double acc = 0.0;
+        for (int g = threadIdx.x; g < ng; g += 256) acc += group_energy(P, D, g, estride);
+        const double e0 = block_sum_256(acc, red) + (D.excCount > 0 ? P.xenergy[c0] : 0.0);
+        double dd = 0.0;
+        for (int k = threadIdx.x; k < D.ligCount; k += 256) {
+            const double* e = P.lenergy + 2ll * kDisplacedParts * (D.ligOff + k);
+            double e0 = e[0], e1 = e[kDisplacedParts];
+            for (int q = 1; q < kDisplacedParts; ++q) { e0 += e[q]; e1 += e[kDisplacedParts + q]; }
+            dd += e1 - e0;
+        }
+        const double de = block_sum_256(dd, red) + (D.lxCount > 0 ? P.xenergy[c1] : 0.0);
+        U[0] = e0 + (P.eoff ? P.eoff[c0] : 0.0);
+        U[1] = (e0 + de) + (P.eoff ? P.eoff[c1] : 0.0);
+        if (threadIdx.x == 0)
+            P.rt[c1].npairs = P.rt[c0].npairs - P.rt[c0].lpairs[0] + P.rt[c0].lpairs[1];
+    }
+    for (int st = 0; st < 2 && !delta; ++st) {
         const int c = 2 * s + st;
         const SetDesc& D = P.desc[c];
         const int ng = P.rt[c].ngroups;
@@ -1893,7 +2200,9 @@ __global__ void __launch_bounds__(256) k_set_energy(DevPtrs P, int c, int estrid
 // sp and direction come from kernel arguments (the reference's HybridForce
 // receives sp as a scalar argument, CommonATMMetaForceKernels.cpp:199-201).
 // The FP64 side sums of a slot are read only where its mask bit is set.
-__global__ void k_blend(DevPtrs P, int sysBegin, int hostMode, double spHost, double dirHost, int clear) {
+// delta (state difference): F_sys (+)= F1 + w2 dF, dF read by atom from the
+// state-1 set's sums (F2 = F1 + dF, w1 + w2 = 1).
+__global__ void k_blend(DevPtrs P, int sysBegin, int hostMode, double spHost, double dirHost, int clear, int delta) {
     const int s = sysBegin + blockIdx.y;
     const SetDesc& D0 = P.desc[2 * s];
     const SetDesc& D1 = P.desc[2 * s + 1];
@@ -1920,6 +2229,19 @@ __global__ void k_blend(DevPtrs P, int sysBegin, int hostMode, double spHost, do
         F[2 * N + i] = oz + f1z;
         return;
     }
+    if (delta) {
+        const long long b = D1.slotOff + i;
+        const long long* fb = P.sf + b;
+        double gx = (double)fb[0] * inv, gy = (double)fb[P.sfn] * inv, gz = (double)fb[2 * P.sfn] * inv;
+        if (P.rt[2 * s + 1].big && ((P.sdMask[b >> 6] >> (b & 63)) & 1ull)) {
+            const double* d = P.sd + 3 * b;
+            gx += d[0]; gy += d[1]; gz += d[2];
+        }
+        F[i] = ox + (f1x + w2 * gx);
+        F[N + i] = oy + (f1y + w2 * gy);
+        F[2 * N + i] = oz + (f1z + w2 * gz);
+        return;
+    }
     const long long b = D1.slotOff + P.slotOf[D1.setAtomOff + i];
     const long long* fb = P.sf + b;
     double f2x = (double)fb[0] * inv, f2y = (double)fb[P.sfn] * inv, f2z = (double)fb[2 * P.sfn] * inv;
@@ -1940,12 +2262,11 @@ __global__ void k_fill_i64(long long* p, long long n) {
 
 static inline unsigned cdiv(long long a, long long b) { return (unsigned)((a + b - 1) / b); }
 
-hipError_t launch_zero_counts(const DevPtrs& P, int cellBegin, int cellEnd, int setBegin, int setEnd, double* f,
-                              long long nf, hipStream_t st) {
-    long long n = std::max<long long>(cellEnd - cellBegin, nf);
-    unsigned blocks = (unsigned)std::min<long long>(std::max<long long>(cdiv(n, 256), cdiv(setEnd - setBegin, 256)), 4096);
-    if (blocks == 0) blocks = 1;
-    k_zero_counts<<<blocks, 256, 0, st>>>(P, cellBegin, cellEnd, setBegin, setEnd, f, nf);
+hipError_t launch_zero_counts(const DevPtrs& P, int setBegin, int nsets, int maxCells, double* f, long long nf,
+                              hipStream_t st) {
+    const long long per = std::max<long long>(maxCells, cdiv(nf, std::max(nsets, 1)));
+    unsigned blocks = (unsigned)std::min<long long>(std::max<long long>(cdiv(per, 256), 1), 1024);
+    k_zero_counts<<<dim3(blocks, nsets), 256, 0, st>>>(P, setBegin, f, nf);
     return hipGetLastError();
 }
 
@@ -2042,8 +2363,8 @@ hipError_t launch_pairs(const DevPtrs& P, int setBegin, int nsets, int maxGroups
     return hipGetLastError();
 }
 
-hipError_t launch_energy(const DevPtrs& P, int sysBegin, int nsys, int estride, hipStream_t st) {
-    k_energy<<<nsys, 256, 0, st>>>(P, sysBegin, estride);
+hipError_t launch_energy(const DevPtrs& P, int sysBegin, int nsys, int estride, hipStream_t st, bool delta) {
+    k_energy<<<nsys, 256, 0, st>>>(P, sysBegin, estride, delta ? 1 : 0);
     return hipGetLastError();
 }
 
@@ -2066,8 +2387,38 @@ hipError_t launch_exceptions(const DevPtrs& P, int setBegin, int nsets, bool for
 }
 
 hipError_t launch_blend(const DevPtrs& P, int sysBegin, int nsys, int maxAtoms, int hostMode, double sp, double dir,
-                        bool clear, hipStream_t st) {
-    k_blend<<<dim3(cdiv(maxAtoms, 256), nsys), 256, 0, st>>>(P, sysBegin, hostMode, sp, dir, clear ? 1 : 0);
+                        bool clear, hipStream_t st, bool delta) {
+    k_blend<<<dim3(cdiv(maxAtoms, 256), nsys), 256, 0, st>>>(P, sysBegin, hostMode, sp, dir, clear ? 1 : 0,
+                                                             delta ? 1 : 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_displaced(const DevPtrs& P, int sysBegin, int nsys, int maxLig, bool forces, hipStream_t st) {
+    if (maxLig <= 0 || nsys <= 0) return hipSuccess;
+    const dim3 grid(cdiv(2ll * maxLig * kDisplacedParts, 4), nsys);
+    const bool exc = P.xmask != nullptr;
+    if (forces) {
+        if (exc) k_displaced<true, true, false><<<grid, 256, 0, st>>>(P, sysBegin, 0, nullptr, 0, nullptr);
+        else k_displaced<true, false, false><<<grid, 256, 0, st>>>(P, sysBegin, 0, nullptr, 0, nullptr);
+    } else {
+        if (exc) k_displaced<false, true, false><<<grid, 256, 0, st>>>(P, sysBegin, 0, nullptr, 0, nullptr);
+        else k_displaced<false, false, false><<<grid, 256, 0, st>>>(P, sysBegin, 0, nullptr, 0, nullptr);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_displaced_dump(const DevPtrs& P, int sys, int state, int nLig, int2* out, long long cap,
+                                 unsigned long long* ctr, hipStream_t st) {
+    if (nLig <= 0) return hipSuccess;
+    const dim3 grid(cdiv((long long)nLig * kDisplacedParts, 4), 1);
+    if (P.xmask) k_displaced<false, true, true><<<grid, 256, 0, st>>>(P, sys, state, out, cap, ctr);
+    else k_displaced<false, false, true><<<grid, 256, 0, st>>>(P, sys, state, out, cap, ctr);
+    return hipGetLastError();
+}
+
+hipError_t launch_displaced_exceptions(const DevPtrs& P, int sysBegin, int nsys, bool forces, hipStream_t st) {
+    if (nsys <= 0) return hipSuccess;
+    k_displaced_exceptions<<<nsys, 256, 0, st>>>(P, sysBegin, forces ? 1 : 0);
     return hipGetLastError();
 }
 

@@ -235,7 +235,7 @@ extern "C" int atm_eval_noalloc(const atm_system_desc* system, const atm_params*
         hipLaunchKernelGGL(k_noalloc_atoms, dim3((n + 255) / 256), dim3(256), 0, st, P, positions, charges, sigmas,
                            epsilons, displacements, n, std::sqrt(ATM_ONE_4PI_EPS0));
         HIPCHK(hipGetLastError());
-        HIPCHK(launch_zero_counts(P, 0, (int)cellOff, 0, 2, nullptr, 0, st));
+        HIPCHK(launch_zero_counts(P, 0, 2, maxCells, nullptr, 0, st));
         HIPCHK(launch_build(P, 0, 2, n, maxCells, maxSlots, st));
         HIPCHK(launch_pairlist(P, 0, 2, maxGroups, 4, st));
         HIPCHK(launch_pairs(P, 0, 2, maxGroups, tu.pairWaves, withForces, (flags & ATM_EVAL_FP32_PAIRS) == 0,

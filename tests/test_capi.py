@@ -29,7 +29,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_version_and_abi():
     lib = _capi.load()
-    assert lib.atm_abi_version() == 5
+    assert lib.atm_abi_version() == 6
     assert b"atm-mi355x" in lib.atm_version()
 
 

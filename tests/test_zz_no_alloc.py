@@ -4,7 +4,9 @@ atm_eval_noalloc runs atm_eval's kernels with the same default launch
 configuration on one system, with every intermediate buffer in a static device
 array instead of context allocations, so its results must equal a one-system
 context's bit for bit (the same cluster composition, list order and 64-bit
-fixed-point force sums).  Checked on two calls in a row, the second on a
+fixed-point force sums) when that context also bins and pairs x + d in full
+(full_state2); the context's default mode (x + d as the difference from x)
+agrees to rounding.  Checked on two calls in a row, the second on a
 smaller system (the scratch keeps the first call's contents, which the second
 must not see), on an accumulating call, on a call captured in a HIP graph and
 replayed, and on an invalid input.
@@ -62,8 +64,12 @@ class _Inputs:
                 int(self.status.item()))
 
 
-def _context(s):
+def _context(s, full_state2=True):
+    """atm_eval's results for s.  The no-allocation entry point bins and pairs x + d in
+    full, so the bit-for-bit comparison is with that mode (full_state2); the default
+    mode (x + d as the difference from x) agrees to rounding."""
     with ATMBatch([s]) as b:
+        b.set_tuning(full_state2=full_state2)
         b.compute()
         return b.result(0), b.forces(0)
 
@@ -82,6 +88,10 @@ def test_two_calls_match_atm_eval(gpu):
         for k in RESULT_KEYS:
             assert r[k] == r_ref[k], (n, k, r[k], r_ref[k])
         assert np.array_equal(F, F_ref), (n, float(np.abs(F - F_ref).max()))
+        r_d, F_d = _context(s, full_state2=False)
+        for k in ("energy", "state1_energy", "state2_energy"):
+            assert abs(r[k] - r_d[k]) <= 1e-11 * max(1.0, abs(r_d[k])), (n, k, r[k], r_d[k])
+        assert np.linalg.norm(F - F_d) / np.linalg.norm(F_d) < 1e-6
 
 
 def test_accumulates_and_energy_only(gpu):
