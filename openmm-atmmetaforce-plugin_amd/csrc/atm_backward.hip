@@ -24,17 +24,22 @@ __global__ void k_fill_f64(double* p, long long n) {
 
 // wset null: the accumulators already hold weighted sums; otherwise they hold
 // dU/dp of each set and wset[2 s], wset[2 s + 1] weight the two states here.
+// delta (x + d evaluated as the difference from x, wset given): the x + d set's
+// accumulators hold dU2/dp - dU1/dp by atom, so the weights are wa + wb on the x
+// set's and wb on the difference.
 __global__ void k_backward_gather(DevPtrs P, int sysBegin, double* dq, double* dsig, double* deps,
-                                  const double* wset) {
+                                  const double* wset, int delta) {
     const int s = sysBegin + blockIdx.y;
     const SetDesc& D0 = P.desc[2 * s];
     const SetDesc& D1 = P.desc[2 * s + 1];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= D0.natoms) return;
     const int a = D0.slotOff + P.slotOf[D0.setAtomOff + i];
-    const int b = D1.slotOff + P.slotOf[D1.setAtomOff + i];
+    const int b = delta ? D1.slotOff + i : D1.slotOff + P.slotOf[D1.setAtomOff + i];
     const int k = D0.atomOff + i;
-    const double wa = wset ? wset[2 * s] : 1.0, wb = wset ? wset[2 * s + 1] : 1.0;
+    double wa = wset ? wset[2 * s] : 1.0;
+    const double wb = wset ? wset[2 * s + 1] : 1.0;
+    if (delta) wa += wb;
     const double sqk = 11.787089759563214;  // sqrt(138.935456)
     const double ep = P.prm64[k].y;          // 2 sqrt(eps)
     if (dq) dq[k] = sqk * fma(wa, P.gq[a], wb * P.gq[b]);
@@ -100,8 +105,9 @@ hipError_t launch_backward(const DevPtrs& P, long long totalSlots, int setBegin,
 }
 
 hipError_t launch_backward_gather(const DevPtrs& P, int sysBegin, int nsys, int maxAtoms, double* dq, double* dsig,
-                                  double* deps, const double* wset, hipStream_t st) {
-    k_backward_gather<<<dim3(cdivb(maxAtoms, 256), nsys), 256, 0, st>>>(P, sysBegin, dq, dsig, deps, wset);
+                                  double* deps, const double* wset, hipStream_t st, bool delta) {
+    k_backward_gather<<<dim3(cdivb(maxAtoms, 256), nsys), 256, 0, st>>>(P, sysBegin, dq, dsig, deps, wset,
+                                                                      delta ? 1 : 0);
     return hipGetLastError();
 }
 

@@ -27,7 +27,7 @@ namespace atm {
 hipError_t launch_backward(const DevPtrs& P, long long totalSlots, int setBegin, int nsets, int maxGroups,
                            const double* wset, int split, hipStream_t st);
 hipError_t launch_backward_gather(const DevPtrs& P, int sysBegin, int nsys, int maxAtoms, double* dq, double* dsig,
-                                  double* deps, const double* wset, hipStream_t st);
+                                  double* deps, const double* wset, hipStream_t st, bool delta);
 hipError_t launch_set_positions(DevPtrs P, float4* posq, const float* xyz, long long n, hipStream_t st);
 hipError_t launch_check_particles(const double* q, const double* sig, const double* eps, long long n, int* bad,
                                   hipStream_t st);
@@ -503,7 +503,8 @@ void update_displaced(Context& C) {
 
 // The batched evaluation with these flags uses the state difference.
 bool use_delta(const Context& C, int flags) {
-    return C.deltaOK && !C.tuning.fullState2 && !(flags & ATM_EVAL_PARAM_GRADS);
+    (void)flags;
+    return C.deltaOK && !C.tuning.fullState2;
 }
 
 // Pair kernel variant 5 (FP64 pairs only) reads the atom-level j lists.
@@ -584,7 +585,7 @@ void enqueue_batched(Context& C, int flags, hipEvent_t* ev = nullptr) {
     if (ev) HIPCHK(hipEventRecord(ev[3], st));
     if (has_exception_terms(C)) HIPCHK(launch_exceptions(P, 0, nsets, forces, st));
     if (delta) {
-        HIPCHK(launch_displaced(P, 0, C.nsys, C.maxLig, forces, st));
+        HIPCHK(launch_displaced(P, 0, C.nsys, C.maxLig, forces, st, grads));
         if (!C.lxidx.empty()) HIPCHK(launch_displaced_exceptions(P, 0, C.nsys, forces, st));
     }
     if (ev) HIPCHK(hipEventRecord(ev[4], st));
@@ -1506,7 +1507,6 @@ void backward_impl(Context& C, const double* grad_energy, double* d_charges, dou
             C.evaluated = true;
             C.resultsValid = false;
             C.resultsOnHost = false;
-            C.lastDelta = false;
             fetch_results(C);  // list overflow: relays out and reports ATM_ERR_CAPACITY
             C.gradsReady = fused = true;
         }
@@ -1528,7 +1528,7 @@ void backward_impl(Context& C, const double* grad_energy, double* d_charges, dou
             oq = C.dDq.as<double>(); os = C.dDsig.as<double>(); oe = C.dDeps.as<double>();
         }
         HIPCHK(launch_backward_gather(C.P, 0, S, C.maxAtoms, oq, os, oe, fused ? C.dWset.as<double>() : nullptr,
-                                      C.stream));
+                                      C.stream, fused && C.lastDelta));
         HIPCHK(hipStreamSynchronize(C.stream));
         if (!deviceOut) {
             if (d_charges) to_host(C, d_charges, C.dDq.p, nb);

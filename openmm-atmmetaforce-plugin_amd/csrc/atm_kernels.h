@@ -123,7 +123,10 @@ hipError_t launch_blend(const DevPtrs& P, int sysBegin, int nsys, int maxAtoms, 
 // State difference: the pairs with a displaced atom of systems [sysBegin, sysBegin + nsys) at
 // x and at x + d (after the state-0 sets' build; maxLig: most displaced atoms of one system),
 // and the exception terms with a displaced atom (their difference).
-hipError_t launch_displaced(const DevPtrs& P, int sysBegin, int nsys, int maxLig, bool forces, hipStream_t st);
+// grads: the backward mode (energies, counts and the parameter derivatives of the displaced pairs,
+// x + d minus x, into gq/gsig/geps of the x + d sets indexed by atom; no forces)
+hipError_t launch_displaced(const DevPtrs& P, int sysBegin, int nsys, int maxLig, bool forces, hipStream_t st,
+                            bool grads = false);
 hipError_t launch_displaced_exceptions(const DevPtrs& P, int sysBegin, int nsys, bool forces, hipStream_t st);
 // The pairs the displaced-atom kernel evaluates for system `sys` at `state` (lower index first),
 // appended to out[ctr[0]++] while below cap.

@@ -1912,10 +1912,15 @@ __device__ __forceinline__ bool same_displacement(const DevPtrs& P, const SetDes
 // lenergy[2 (ligOff + k) + st], its pair count to rt[state-0 set].lpairs[st];
 // with FORCES, -F (state 0) or +F (state 1) goes to dF, the state-1 set's
 // fixed-point sums indexed by atom.  DUMP: the pairs (lower index first) are
-// written to out instead (state dumpState, no energy or force).
-template <bool FORCES, bool EXC, bool DUMP>
+// written to out instead (state dumpState, no energy or force).  BWD (no forces):
+// the derivatives of the pair energies with respect to the stored parameters
+// q' = q sqrt(k_e), h = sigma/2, e' = 2 sqrt(eps) of both atoms (the tile kernel's
+// backward formulas), -dE (state 0) or +dE (state 1), into the x + d set's
+// gradient accumulators indexed by atom.
+template <bool FORCES, bool EXC, bool DUMP, bool BWD = false>
 __global__ void __launch_bounds__(256) k_displaced(DevPtrs P, int sysBegin, int dumpState, int2* out, long long cap,
                                                    unsigned long long* ctr) {
+    static_assert(!(BWD && (FORCES || DUMP)), "the backward mode computes no forces");
     const int s = sysBegin + blockIdx.y;
     const int c0 = 2 * s, c1 = 2 * s + 1;
     const SetDesc D = P.desc[c0];
@@ -1940,6 +1945,7 @@ __global__ void __launch_bounds__(256) k_displaced(DevPtrs P, int sysBegin, int 
     const long long dOff = P.desc[c1].slotOff;  // dF of atom b at dOff + b
     const double sgn = st ? 1.0 : -1.0;
     double eacc = 0.0, fax = 0.0, fay = 0.0, faz = 0.0;
+    double gaq = 0.0, gah = 0.0, gae = 0.0;  // BWD: dE/d(q', h, e') of atom a
     uint32_t npair = 0;
     // one candidate: b at pb with FP64 parameters pbq; nb: slot or atom for dF
     auto pair = [&](int b, float4 pb, double4 pbq) __attribute__((always_inline)) {
@@ -1958,6 +1964,19 @@ __global__ void __launch_bounds__(256) k_displaced(DevPtrs P, int sysBegin, int 
         }
         const PairEF64 pe = pair_ef64(true, r2, pa.x + pbq.x, pa.y * pbq.y, pa.z * pbq.z, D.krf64, D.crf64);
         eacc += pe.e;
+        if (BWD) {
+            // phi = 1/r + k_rf r^2 - c_rf; with s = h_a + h_b, u = s / r^2, s2 = s u, s6 = s2^3:
+            // dE/dh = e'_a e'_b (12 s6 - 6) s2^2 u, dE/de'_a = e'_b (s6^2 - s6), dE/dq'_a = q'_b phi
+            const double r2inv = 1.0 / r2;
+            const double phi = fma(D.krf64, r2, sqrt(r2inv)) - D.crf64;
+            const double sg = pa.x + pbq.x, u = sg * r2inv, s2 = sg * u, s6 = s2 * s2 * s2;
+            const double dh = pa.y * pbq.y * fma(12.0, s6, -6.0) * s2 * s2 * u;
+            const double lj = fma(s6, s6, -s6);
+            gaq += pbq.z * phi; gah += dh; gae += pbq.y * lj;
+            atomicAdd(P.gq + dOff + b, sgn * pa.z * phi);
+            atomicAdd(P.gsig + dOff + b, sgn * dh);
+            atomicAdd(P.geps + dOff + b, sgn * pa.y * lj);
+        }
         if (FORCES) {
             const double fx = pe.fs * dx, fy = pe.fs * dy, fz = pe.fs * dz;
             fax += fx; fay += fy; faz += fz;
@@ -2063,6 +2082,14 @@ __global__ void __launch_bounds__(256) k_displaced(DevPtrs P, int sysBegin, int 
     eacc = wave_sum_d(eacc);
     if (FORCES) {
         fax = wave_sum_d(fax); fay = wave_sum_d(fay); faz = wave_sum_d(faz);
+    }
+    if (BWD) {
+        gaq = wave_sum_d(gaq); gah = wave_sum_d(gah); gae = wave_sum_d(gae);
+        if (lane == 0) {
+            atomicAdd(P.gq + dOff + a, sgn * gaq);
+            atomicAdd(P.gsig + dOff + a, sgn * gah);
+            atomicAdd(P.geps + dOff + a, sgn * gae);
+        }
     }
     if (lane == 0) {
         P.lenergy[(2ll * (D.ligOff + k) + st) * kDisplacedParts + part] = eacc;
@@ -2393,11 +2420,15 @@ hipError_t launch_blend(const DevPtrs& P, int sysBegin, int nsys, int maxAtoms, 
     return hipGetLastError();
 }
 
-hipError_t launch_displaced(const DevPtrs& P, int sysBegin, int nsys, int maxLig, bool forces, hipStream_t st) {
+hipError_t launch_displaced(const DevPtrs& P, int sysBegin, int nsys, int maxLig, bool forces, hipStream_t st,
+                            bool grads) {
     if (maxLig <= 0 || nsys <= 0) return hipSuccess;
     const dim3 grid(cdiv(2ll * maxLig * kDisplacedParts, 4), nsys);
     const bool exc = P.xmask != nullptr;
-    if (forces) {
+    if (grads) {
+        if (exc) k_displaced<false, true, false, true><<<grid, 256, 0, st>>>(P, sysBegin, 0, nullptr, 0, nullptr);
+        else k_displaced<false, false, false, true><<<grid, 256, 0, st>>>(P, sysBegin, 0, nullptr, 0, nullptr);
+    } else if (forces) {
         if (exc) k_displaced<true, true, false><<<grid, 256, 0, st>>>(P, sysBegin, 0, nullptr, 0, nullptr);
         else k_displaced<true, false, false><<<grid, 256, 0, st>>>(P, sysBegin, 0, nullptr, 0, nullptr);
     } else {
