@@ -68,6 +68,7 @@ struct Context {
     DevBuf dBpos, dBcell, dBrank, dSlotOf, dCellCount, dCellStart, dColStart, dColCount;
     DevBuf dSpos, dSprm, dSprm64, dSidx, dSslot, dSf, dSd, dSdMask, dBbc, dBbh, dGinfo, dGcount, dGenergy, dEntries, dSetEnergy;
     DevBuf dAlist, dAcount;
+    DevBuf dLptItems;  // the tile kernel's longest-first group order
     DevBuf dGq, dGsig, dGeps, dWset, dDq, dDsig, dDeps;
     DevBuf dFlag, dPosStage;  // validation counter; FP32 staging of host positions
     // exceptions (atm_set_exceptions): per system, near masks, far partner lists and
@@ -390,6 +391,7 @@ void alloc_layout(Context& C) {
         C.dAcount.release();
     }
     C.dSetEnergy.alloc(sizeof(double) * S2);
+    C.dLptItems.alloc(sizeof(int) * std::max<long long>(C.totalGroups, 1));
     C.dGq.release(); C.dGsig.release(); C.dGeps.release();
     C.gradsReady = false;
     zero_on_stream(C, C.dRt.p, sizeof(SetRuntime) * S2);
@@ -435,6 +437,7 @@ void alloc_layout(Context& C) {
     P.acount = C.dAcount.as<int>();
     P.setEnergy = C.dSetEnergy.as<double>();
     P.eoff = C.dEoff.as<double>();
+    P.lptItems = C.dLptItems.as<int>();
     P.gq = P.gsig = P.geps = nullptr;
     upload_exceptions(C);
     C.dropGraphs();  // new pointers in P (the displaced-atom arrays may be reallocated below)

@@ -67,6 +67,9 @@ struct DevPtrs {
     const unsigned char* ligMark;  // per atom (batch order): 1 if displaced
     double* lenergy;       // [2 x displaced atoms x kDisplacedParts]: pair energy of each at x and at x + d
     const int* lxidx;      // exception terms (batch index) with a displaced atom, at SetDesc::lxOff
+    // group order of the tile kernel, longest lists first per set (k_order, run by the pair
+    // launchers; the set's range at groupOff); null: launch order
+    int* lptItems;
     // backward
     double* gq;            // [2 S sets x slots] gradient accumulators in slot order
     double* gsig;
@@ -105,6 +108,9 @@ inline int pair_energy_stride(int variant, bool fp64, int split) {
 // `forces` their forces added to the set's slot sums (call after the pair kernel
 // and before launch_energy / launch_set_energy, which add xenergy when excCount > 0).
 hipError_t launch_exceptions(const DevPtrs& P, int setBegin, int nsets, bool forces, hipStream_t st);
+// Length buckets of the tile kernel's group order (entries / kLptShift, capped).
+constexpr int kLptBuckets = 64;
+constexpr int kLptShift = 4;
 // Waves per (displaced atom, state) in the displaced-atom kernel (each its own energy slot).
 constexpr int kDisplacedParts = 1;
 // delta: the state-difference evaluation (state-0 sets paired, displaced atoms by launch_displaced)

@@ -685,13 +685,63 @@ __global__ void __launch_bounds__(64 * W) k_pairlist(DevPtrs P, int setBegin) {
         }
     }
     if (lane == 0) {
-        P.gcount[D.groupOff + g] = min(nEnt, D.entryCap);
+        const int kept = min(nEnt, D.entryCap);
+        P.gcount[D.groupOff + g] = kept;
         atomicMax(&P.rt[c].maxEntries, nEnt);
         if (nEnt > D.entryCap) atomicOr(&P.rt[c].overflow, 1);
     }
 }
 
 // ---------------------------------------------------------------- pair kernel
+// The tile kernel's group order, longest lists first, per set: one block per set
+// setBegin + blockIdx.x setStride files its groups by list length (bucket
+// min(63, entries >> kLptShift)) with LDS counting and writes them from the
+// longest bucket down to lptItems[groupOff ...].  The tile kernel takes rank r as
+// the (r / nsets)-th longest group of set r mod nsets, so the launch starts with
+// every set's longest lists.  The order within a bucket follows LDS atomics,
+// which no result depends on (each group's sums are its own).
+__global__ void __launch_bounds__(512) k_order(DevPtrs P, int setBegin) {
+    constexpr int NT = 512, NI = 4;  // groups held per thread (more take a second sweep)
+    __shared__ int hist[kLptBuckets];
+    const int t = threadIdx.x;
+    const int c = setBegin + (int)blockIdx.x * P.setStride;
+    const int gOff = P.desc[c].groupOff, ng = P.rt[c].ngroups;
+    if (t < kLptBuckets) hist[t] = 0;
+    __syncthreads();
+    for (int g0 = 0; g0 < ng; g0 += NT * NI) {
+        int bk[NI];
+#pragma unroll
+        for (int u = 0; u < NI; ++u) {
+            const int g = g0 + u * NT + t;
+            bk[u] = g < ng ? min(kLptBuckets - 1, P.gcount[gOff + g] >> kLptShift) : -1;
+        }
+#pragma unroll
+        for (int u = 0; u < NI; ++u)
+            if (bk[u] >= 0) atomicAdd(hist + bk[u], 1);
+    }
+    __syncthreads();
+    if (t == 0) {
+        int acc = 0;
+        for (int b = kLptBuckets - 1; b >= 0; --b) {
+            const int n = hist[b];
+            hist[b] = acc;
+            acc += n;
+        }
+    }
+    __syncthreads();
+    for (int g0 = 0; g0 < ng; g0 += NT * NI) {
+        int bk[NI];
+#pragma unroll
+        for (int u = 0; u < NI; ++u) {
+            const int g = g0 + u * NT + t;
+            bk[u] = g < ng ? min(kLptBuckets - 1, P.gcount[gOff + g] >> kLptShift) : -1;
+        }
+#pragma unroll
+        for (int u = 0; u < NI; ++u)
+            if (bk[u] >= 0) P.lptItems[gOff + atomicAdd(hist + bk[u], 1)] = g0 + u * NT + t;
+    }
+}
+
 
 // Decode an entry and load this lane's j-atom (lane = 8 jj + ii).
 struct JAtom {
@@ -814,13 +864,27 @@ template <int W, bool FORCES, bool F64, bool BWD, bool EXC>
 __global__ void __launch_bounds__(64 * W, 6) k_pairs(DevPtrs P, int setBegin, const double* wset, int split) {
     static_assert(!BWD || (F64 && !FORCES), "the backward mode is FP64 and computes no forces");
     constexpr bool JSUMS = FORCES || BWD;  // per-atom j sums flushed every kJFlush entries
-    const int c = setBegin + (int)blockIdx.y * P.setStride;
-    const SetDesc D = P.desc[c];  // by value: lives in SGPRs for the whole kernel
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int jj = lane >> 3, ii = lane & 7;
-    const int part = blockIdx.x % split;
-    const int g = (blockIdx.x / split) * W + wave;
-    const bool active = g < P.rt[c].ngroups;
+    int c, g, part;
+    bool active;
+    if (P.lptItems) {
+        // longest lists first: this wave's rank f in launch order is part f % split of item
+        // f / split, the (item / nsets)-th longest group of set item mod nsets (k_order)
+        const int f = __builtin_amdgcn_readfirstlane((int)((blockIdx.y * gridDim.x + blockIdx.x) * W + wave));
+        const int item = f / split;
+        part = f - item * split;
+        const int nsets = gridDim.y, pos = item / nsets;
+        c = setBegin + (item - pos * nsets) * P.setStride;
+        active = pos < P.rt[c].ngroups;
+        g = active ? __builtin_amdgcn_readfirstlane(P.lptItems[P.desc[c].groupOff + pos]) : 0;
+    } else {
+        c = setBegin + (int)blockIdx.y * P.setStride;
+        part = blockIdx.x % split;
+        g = (blockIdx.x / split) * W + wave;
+        active = g < P.rt[c].ngroups;
+    }
+    const SetDesc D = P.desc[c];  // wave-uniform: lives in SGPRs for the whole kernel
     __shared__ float4 sPos[W][64];
     __shared__ float2 sPrm[W][F64 ? 1 : 64];
     __shared__ double4 sPrm64[W][F64 ? 64 : 1];
@@ -2369,11 +2433,13 @@ hipError_t launch_pairs(const DevPtrs& P, int setBegin, int nsets, int maxGroups
         return hipGetLastError();
     }
     if (split < 1 || split > 8) return hipErrorInvalidValue;
+    const DevPtrs& Pk = P;
+    if (P.lptItems) k_order<<<nsets, 512, 0, st>>>(P, setBegin);
 #define ATM_PAIRS_X(WV, F64, X)                                                                                   \
     if (forces) k_pairs<WV, true, F64, false, X><<<dim3(cdiv(maxGroups, WV) * split, nsets), 64 * WV, 0, st>>>(   \
-        P, setBegin, nullptr, split);                                                                             \
+        Pk, setBegin, nullptr, split);                                                                            \
     else k_pairs<WV, false, F64, false, X><<<dim3(cdiv(maxGroups, WV) * split, nsets), 64 * WV, 0, st>>>(        \
-        P, setBegin, nullptr, split);
+        Pk, setBegin, nullptr, split);
 #define ATM_PAIRS(WV, F64)                                \
     if (P.xmask) { ATM_PAIRS_X(WV, F64, true) } else { ATM_PAIRS_X(WV, F64, false) }
 #define ATM_PAIRS_W(F64)               \
@@ -2403,8 +2469,10 @@ hipError_t launch_set_energy(const DevPtrs& P, int c, int estride, bool withOffs
 hipError_t launch_backward_pairs(const DevPtrs& P, int setBegin, int nsets, int maxGroups, const double* wset,
                                  int split, hipStream_t st) {
     if (split < 1 || split > 8) return hipErrorInvalidValue;
-    if (P.xmask) k_pairs<1, false, true, true, true><<<dim3(maxGroups * split, nsets), 64, 0, st>>>(P, setBegin, wset, split);
-    else k_pairs<1, false, true, true, false><<<dim3(maxGroups * split, nsets), 64, 0, st>>>(P, setBegin, wset, split);
+    const DevPtrs& Pk = P;
+    if (P.lptItems) k_order<<<nsets, 512, 0, st>>>(P, setBegin);
+    if (P.xmask) k_pairs<1, false, true, true, true><<<dim3(maxGroups * split, nsets), 64, 0, st>>>(Pk, setBegin, wset, split);
+    else k_pairs<1, false, true, true, false><<<dim3(maxGroups * split, nsets), 64, 0, st>>>(Pk, setBegin, wset, split);
     return hipGetLastError();
 }
 
