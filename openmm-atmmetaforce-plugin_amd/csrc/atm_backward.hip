@@ -59,7 +59,8 @@ __global__ void k_set_positions(float4* posq, const float* xyz, long long n) {
 
 // Per-atom parameters of every atom of a context from device doubles, converted
 // as atm_set_particles converts host ones: posq.w = q sqrt(k_e), ljp = (sigma/2,
-// 2 sqrt(eps)) in FP32, prm64 = (sigma/2, 2 sqrt(eps), q sqrt(k_e), 0) in FP64.
+// 2 sqrt(eps)) in FP32, prm64 = (sigma/2, 2 sqrt(eps), q sqrt(k_e), w) in FP64 (w, 1 for a
+// displaced atom, is kept).
 // k_check_particles counts the entries atm_set_particles would reject.
 __global__ void k_check_particles(const double* q, const double* sig, const double* eps, long long n, int* bad) {
     const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
@@ -75,10 +76,21 @@ __global__ void k_set_particles(float4* posq, float2* ljp, double4* prm64, const
     const double qi = q[i], si = sig[i], ei = eps[i];
     posq[i].w = (float)(qi * sqk);
     ljp[i] = make_float2((float)(0.5 * si), (float)(2.0 * sqrt(ei)));
-    prm64[i] = make_double4(0.5 * si, 2.0 * sqrt(ei), qi * sqk, 0.0);
+    prm64[i] = make_double4(0.5 * si, 2.0 * sqrt(ei), qi * sqk, prm64[i].w);  // w: the displaced-atom mark
+}
+
+__global__ void k_set_marks(double4* prm64, const unsigned char* mark, long long n) {
+    const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (i < n) prm64[i].w = (double)mark[i];
 }
 
 static inline unsigned cdivb(long long a, long long b) { return (unsigned)((a + b - 1) / b); }
+
+hipError_t launch_set_marks(double4* prm64, const unsigned char* mark, long long n, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    k_set_marks<<<cdivb(n, 256), 256, 0, st>>>(prm64, mark, n);
+    return hipGetLastError();
+}
 
 hipError_t launch_check_particles(const double* q, const double* sig, const double* eps, long long n, int* bad,
                                   hipStream_t st) {

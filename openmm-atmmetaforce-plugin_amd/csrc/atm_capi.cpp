@@ -33,6 +33,7 @@ hipError_t launch_check_particles(const double* q, const double* sig, const doub
                                   hipStream_t st);
 hipError_t launch_set_particles(const DevPtrs& P, const double* q, const double* sig, const double* eps, long long n,
                                 double sqk, hipStream_t st);
+hipError_t launch_set_marks(double4* prm64, const unsigned char* mark, long long n, hipStream_t st);
 }
 
 namespace atm {
@@ -117,9 +118,17 @@ struct Context {
     DevBuf dLig, dLigMark, dLenergy, dLxidx;
 
     std::map<int, hipGraphExec_t> graphs;
+    // a second stream for work that runs beside the pair kernel (the displaced-atom kernels),
+    // forked from and joined to the context stream (graph branches when captured)
+    hipStream_t side = nullptr;
+    hipEvent_t evFork = nullptr, evJoin = nullptr;
 
     ~Context() {
         dropGraphs();
+        if (side) (void)hipStreamSynchronize(side);
+        if (evFork) (void)hipEventDestroy(evFork);
+        if (evJoin) (void)hipEventDestroy(evJoin);
+        if (side) (void)hipStreamDestroy(side);
         if (own) (void)hipStreamDestroy(own);
     }
     void dropGraphs() {
@@ -493,6 +502,20 @@ void update_displaced(Context& C) {
     if (!C.ligAtoms.empty()) to_device(C, C.dLig.p, C.ligAtoms.data(), sizeof(int) * C.ligAtoms.size());
     if (!C.lxidx.empty()) to_device(C, C.dLxidx.p, C.lxidx.data(), sizeof(int) * C.lxidx.size());
     to_device(C, C.dLigMark.p, C.ligMark.data(), (size_t)C.totalAtoms);
+    // the mark also rides in prm64.w (gathered into sprm64 with the parameters)
+    bool markMoved = false;
+    for (long long i = 0; i < C.totalAtoms; ++i)
+        if (C.hPrm64[i].w != (double)C.ligMark[i]) {
+            C.hPrm64[i].w = (double)C.ligMark[i];
+            markMoved = true;
+        }
+    if (markMoved) {
+        if (C.chargesOnDevice)
+            // the device parameters are newer than the host mirror: rewrite w only
+            HIPCHK(launch_set_marks(C.dPrm64.as<double4>(), C.dLigMark.as<unsigned char>(), C.totalAtoms, C.stream));
+        else
+            to_device(C, C.dPrm64.p, C.hPrm64.data(), sizeof(double4) * C.totalAtoms);
+    }
     to_device(C, C.dDesc.p, C.sets.data(), sizeof(SetDesc) * C.sets.size());
     DevPtrs& P = C.P;
     P.ligAtoms = C.dLig.as<int>();
@@ -567,14 +590,30 @@ void enqueue_batched(Context& C, int flags, hipEvent_t* ev = nullptr) {
     P.setStride = delta ? 2 : 1;
     const int nsets = delta ? C.nsys : 2 * C.nsys;
     const bool forces = (flags & ATM_EVAL_FORCES) != 0;
+    const bool grads = (flags & ATM_EVAL_PARAM_GRADS) != 0;
     if (ev) HIPCHK(hipEventRecord(ev[0], st));
     // with forces, the blend writes the cleared force itself
     const bool clearHere = (flags & ATM_EVAL_CLEAR_FORCES) && !forces;
     HIPCHK(launch_zero_counts(P, 0, nsets, C.maxCells, P.outF, clearHere ? 3 * C.totalAtoms : 0, st));
     HIPCHK(launch_build(P, 0, nsets, C.maxAtoms, C.maxCells, C.maxSlots, st));
     if (ev) HIPCHK(hipEventRecord(ev[1], st));
+    // the displaced-atom kernels need only the x sets' cells and clusters: without stage
+    // events they run on the side stream beside the pair list and pair kernel (not with
+    // parameter gradients, whose accumulators the main stream clears first)
+    const bool fork = delta && !ev && !grads;
+    if (fork) {
+        if (!C.side) {
+            HIPCHK(hipStreamCreateWithFlags(&C.side, hipStreamNonBlocking));
+            HIPCHK(hipEventCreateWithFlags(&C.evFork, hipEventDisableTiming));
+            HIPCHK(hipEventCreateWithFlags(&C.evJoin, hipEventDisableTiming));
+        }
+        HIPCHK(hipEventRecord(C.evFork, st));
+        HIPCHK(hipStreamWaitEvent(C.side, C.evFork, 0));
+        HIPCHK(launch_displaced(P, 0, C.nsys, C.maxLig, forces, C.side, grads));
+        if (!C.lxidx.empty()) HIPCHK(launch_displaced_exceptions(P, 0, C.nsys, forces, C.side));
+        HIPCHK(hipEventRecord(C.evJoin, C.side));
+    }
     HIPCHK(launch_pairlist(P, 0, nsets, C.maxGroups, 4, st));
-    const bool grads = (flags & ATM_EVAL_PARAM_GRADS) != 0;
     if (uses_atom_lists(C, flags) && !grads) HIPCHK(launch_alist(P, 0, nsets, C.maxGroups, st));
     if (ev) HIPCHK(hipEventRecord(ev[2], st));
     if (grads) {
@@ -587,10 +626,11 @@ void enqueue_batched(Context& C, int flags, hipEvent_t* ev = nullptr) {
     }
     if (ev) HIPCHK(hipEventRecord(ev[3], st));
     if (has_exception_terms(C)) HIPCHK(launch_exceptions(P, 0, nsets, forces, st));
-    if (delta) {
+    if (delta && !fork) {
         HIPCHK(launch_displaced(P, 0, C.nsys, C.maxLig, forces, st, grads));
         if (!C.lxidx.empty()) HIPCHK(launch_displaced_exceptions(P, 0, C.nsys, forces, st));
     }
+    if (fork) HIPCHK(hipStreamWaitEvent(st, C.evJoin, 0));
     if (ev) HIPCHK(hipEventRecord(ev[4], st));
     HIPCHK(launch_energy(P, 0, C.nsys, grads ? C.tuning.pairSplit : energy_stride(C, flags), st, delta));
     if (ev) HIPCHK(hipEventRecord(ev[5], st));
@@ -887,7 +927,8 @@ int atm_set_particles(void* context, int32_t sys, const double* charges, const d
                 fail(ATM_ERR_INVALID_ARGUMENT, "sigma and epsilon must be non-negative (atom " + std::to_string(i) + ")");
             C.hPosq[off + i].w = (float)(charges[i] * sqk);
             C.hLjp[off + i] = make_float2((float)(0.5 * sigmas[i]), (float)(2.0 * std::sqrt(epsilons[i])));
-            C.hPrm64[off + i] = make_double4(0.5 * sigmas[i], 2.0 * std::sqrt(epsilons[i]), charges[i] * sqk, 0.0);
+            C.hPrm64[off + i] = make_double4(0.5 * sigmas[i], 2.0 * std::sqrt(epsilons[i]), charges[i] * sqk,
+                                             C.ligMark.empty() ? 0.0 : (double)C.ligMark[off + i]);
             if (displacements)
                 C.hDispl[off + i] = make_float4((float)displacements[3 * i], (float)displacements[3 * i + 1],
                                                 (float)displacements[3 * i + 2], 0.f);

@@ -387,6 +387,9 @@ __global__ void __launch_bounds__(1024) k_scan(DevPtrs P, int setBegin) {
         P.rt[c].nslots = totS;
         P.rt[c].nclusters = totS >> 3;
         P.rt[c].ngroups = totG;
+        // state difference: dF's side sums were cleared by k_bin (the displaced-atom kernels,
+        // which may run beside the pair list, set the flag again)
+        if (P.setStride == 2 && (c & 1) == 0) P.rt[c + 1].big = 0;
     }
 }
 
@@ -545,10 +548,7 @@ __global__ void __launch_bounds__(64 * W) k_pairlist(DevPtrs P, int setBegin) {
     const SetDesc D = P.desc[c];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int g = blockIdx.x * W + wave;
-    if (g == 0 && lane == 0) {
-        P.rt[c].big = 0;  // the side sums were cleared by k_gather_bbox
-        if (P.setStride == 2 && (c & 1) == 0) P.rt[c + 1].big = 0;  // dF's, by k_bin
-    }
+    if (g == 0 && lane == 0) P.rt[c].big = 0;  // the side sums were cleared by k_gather_bbox
     if (g >= P.rt[c].ngroups) return;
     const int4 gi = P.ginfo[D.groupOff + g];
     const int g0 = __builtin_amdgcn_readfirstlane(gi.x), ncl = __builtin_amdgcn_readfirstlane(gi.y);
@@ -2047,15 +2047,17 @@ __global__ void __launch_bounds__(256) k_displaced(DevPtrs P, int sysBegin, int 
             add_force64(P, c1, fixMax, dOff + b, -sgn * fx, -sgn * fy, -sgn * fz);
         }
     };
-    // The state-0 cells within reach.  One column per lane (columns within Rx + 1,
+    // The state-0 clusters within reach.  One column per lane (columns within Rx + 1,
     // Ry + 1: one more than the list search needs, FP32 binning may place an atom one
     // column over); a column whose rectangle is farther than rc (+ 1e-3 nm) in x-y is
-    // dropped, the others take the z-bins within sqrt(rc^2 - rho^2) of the atom
-    // (+ one bin each side), as one or two slot ranges (two when the window wraps).
-    // The ranges are flattened by a wave prefix sum; each lane then takes 4 candidates
-    // per round (their loads issued together), found by binary search over the lanes'
-    // range starts in LDS.
-    __shared__ int sEx[4][64], sB0[4][64], sL0[4][64], sB1[4][64];
+    // dropped, the others take the clusters holding the z-bins within sqrt(rc^2 - rho^2)
+    // of the atom (+ one bin each side): one or two cluster ranges (two when the window
+    // wraps; the second starts after the first).  The ranges are flattened by a wave
+    // prefix sum; rounds of 64 clusters test their boxes against the atom (lane per
+    // cluster, binary search over the lanes' range starts in LDS), and the clusters that
+    // pass are taken 8 at a time, lane 8 k + m on atom m of the k-th.  An atom that is
+    // displaced carries 1 in sprm64.w (it is paired at its own state, below).
+    __shared__ int sEx[4][64], sB0[4][64], sL0[4][64], sB1[4][64], sKeep[4][64];
     const int cx = (int)fminf(fmaxf(p.x * D.invCe[0], 0.f), (float)(D.nx - 1));
     const int cy = (int)fminf(fmaxf(p.y * D.invCe[1], 0.f), (float)(D.ny - 1));
     const int rx = min(D.Rx + 1, D.nx), ry = min(D.Ry + 1, D.ny);
@@ -2086,11 +2088,21 @@ __global__ void __launch_bounds__(256) k_displaced(DevPtrs P, int sysBegin, int 
                 if (za1 - za0 + 1 >= D.nz) { za0 = 0; za1 = D.nz - 1; }
                 else if (za0 < 0) { zb0 = za0 + D.nz; zb1 = D.nz - 1; za0 = 0; }
                 else if (za1 >= D.nz) { zb0 = 0; zb1 = za1 - D.nz; za1 = D.nz - 1; }
-                b0 = cs + ccs[za0];
-                l0 = ccs[za1] + ccn[za1] - ccs[za0];
+                // slot ranges -> cluster ranges (clusters never straddle columns)
+                const int s0 = cs + ccs[za0], e0 = cs + ccs[za1] + ccn[za1];
+                b0 = s0 >> 3;
+                l0 = e0 > s0 ? ((e0 + 7) >> 3) - b0 : 0;
                 if (zb0 <= zb1) {
-                    b1 = cs + ccs[zb0];
-                    l1 = ccs[zb1] + ccn[zb1] - ccs[zb0];
+                    const int s1 = cs + ccs[zb0], e1 = cs + ccs[zb1] + ccn[zb1];
+                    b1 = s1 >> 3;
+                    l1 = e1 > s1 ? ((e1 + 7) >> 3) - b1 : 0;
+                    // the two ranges are disjoint in slots but may share the cluster where
+                    // they meet: that cluster stays with the range that comes first
+                    if (l0 > 0 && l1 > 0) {
+                        if (b1 >= b0) { const int e = b0 + l0; if (b1 < e) { l1 -= e - b1; b1 = e; } }
+                        else { const int e = b1 + l1; if (b0 < e) { l0 -= e - b0; b0 = e; } }
+                        l0 = max(l0, 0); l1 = max(l1, 0);
+                    }
                 }
             }
         }
@@ -2100,36 +2112,40 @@ __global__ void __launch_bounds__(256) k_displaced(DevPtrs P, int sysBegin, int 
         sEx[wave][lane] = incl - tot;
         sB0[wave][lane] = b0; sL0[wave][lane] = l0; sB1[wave][lane] = b1;
         wave_sync();
-        for (int t0 = 256 * part; t0 < T; t0 += 256 * kDisplacedParts) {
-            long long sl[4];
-            int bj[4];
+        for (int t0 = 0; t0 < T; t0 += 64) {
+            // this lane's cluster and its box test
+            const int t = t0 + lane;
+            bool keep = false;
+            int cl = 0;
+            if (t < T) {
+                int lo = 0;
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int t = t0 + 64 * u + lane;
-                sl[u] = -1;
-                if (t < T) {
-                    int lo = 0;
-#pragma unroll
-                    for (int step = 32; step > 0; step >>= 1)
-                        if (sEx[wave][lo + step] <= t && lo + step < 64) lo += step;
-                    const int off = t - sEx[wave][lo];
-                    const int l0u = sL0[wave][lo];
-                    sl[u] = D.slotOff + (off < l0u ? sB0[wave][lo] + off : sB1[wave][lo] + (off - l0u));
+                for (int step = 32; step > 0; step >>= 1)
+                    if (sEx[wave][lo + step] <= t) lo += step;
+                const int off = t - sEx[wave][lo];
+                const int l0u = sL0[wave][lo];
+                cl = off < l0u ? sB0[wave][lo] + off : sB1[wave][lo] + (off - l0u);
+                const float4 bc = P.bbc[D.clusterOff + cl], bh = P.bbh[D.clusterOff + cl];
+                const float dx = fmaxf(fabsf(min_image(p.x - bc.x, D.L[0], D.invL[0])) - bh.x, 0.f);
+                const float dy = fmaxf(fabsf(min_image(p.y - bc.y, D.L[1], D.invL[1])) - bh.y, 0.f);
+                const float dz = fmaxf(fabsf(min_image(p.z - bc.z, D.L[2], D.invL[2])) - bh.z, 0.f);
+                keep = dx * dx + dy * dy + dz * dz <= lim2;
+            }
+            const uint64_t km = __builtin_amdgcn_ballot_w64(keep);
+            const int nk = __popcll(km);
+            if (keep) sKeep[wave][__popcll(km & lanemask_lt())] = cl;
+            wave_sync();
+            for (int k0 = 0; k0 < nk; k0 += 8) {
+                const int kk = k0 + (lane >> 3);
+                if (kk < nk) {
+                    const long long sl = D.slotOff + 8ll * sKeep[wave][kk] + (lane & 7);
+                    const int bj = P.sidx[sl];
+                    const float4 pj = P.spos[sl];
+                    const double4 qj = P.sprm64[sl];
+                    if (bj >= 0 && qj.w == 0.0) pair(bj, pj, qj);
                 }
             }
-#pragma unroll
-            for (int u = 0; u < 4; ++u) bj[u] = sl[u] >= 0 ? P.sidx[sl[u]] : -1;
-#pragma unroll
-            for (int u = 0; u < 4; ++u)
-                if (bj[u] >= 0 && P.ligMark[D.atomOff + bj[u]]) bj[u] = -1;
-            float4 pj[4];
-            double4 qj[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u)
-                if (bj[u] >= 0) { pj[u] = P.spos[sl[u]]; qj[u] = P.sprm64[sl[u]]; }
-#pragma unroll
-            for (int u = 0; u < 4; ++u)
-                if (bj[u] >= 0) pair(bj[u], pj[u], qj[u]);
+            wave_sync();  // the next round overwrites the kept clusters
         }
         wave_sync();  // the next chunk's columns overwrite the LDS ranges
     }
