@@ -284,9 +284,7 @@ int atm_set_tuning(void* context, const atm_tuning* tuning);
  * kernel by kernel with HIP events between stages on the context stream).
  * build = clear + binning + scan + scatter + cell sort + cluster boxes; pairs =
  * the pair kernel alone; displaced = the exception terms and, when x + d is
- * evaluated as the difference from x, the displaced-atom kernels; with forces
- * the energies, the combine and the blend are one kernel, timed as blend_ms
- * (energy_ms is then ~0). */
+ * evaluated as the difference from x, the displaced-atom kernels. */
 typedef struct atm_stage_times {
     double build_ms, pairlist_ms, pairs_ms, displaced_ms, energy_ms, blend_ms, total_ms;
     int32_t reps;

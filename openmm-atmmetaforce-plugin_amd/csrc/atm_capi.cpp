@@ -632,15 +632,10 @@ void enqueue_batched(Context& C, int flags, hipEvent_t* ev = nullptr) {
     }
     if (fork) HIPCHK(hipStreamWaitEvent(st, C.evJoin, 0));
     if (ev) HIPCHK(hipEventRecord(ev[4], st));
-    const int estride = grads ? C.tuning.pairSplit : energy_stride(C, flags);
-    if (forces) {
-        // energies, combine and blend in one kernel (timed as the blend stage)
-        if (ev) HIPCHK(hipEventRecord(ev[5], st));
-        HIPCHK(launch_energy_blend(P, 0, C.nsys, C.maxAtoms, estride, (flags & ATM_EVAL_CLEAR_FORCES) != 0, st, delta));
-    } else {
-        HIPCHK(launch_energy(P, 0, C.nsys, estride, st, delta));
-        if (ev) HIPCHK(hipEventRecord(ev[5], st));
-    }
+    HIPCHK(launch_energy(P, 0, C.nsys, grads ? C.tuning.pairSplit : energy_stride(C, flags), st, delta));
+    if (ev) HIPCHK(hipEventRecord(ev[5], st));
+    if (forces)
+        HIPCHK(launch_blend(P, 0, C.nsys, C.maxAtoms, 0, 0.0, 0.0, (flags & ATM_EVAL_CLEAR_FORCES) != 0, st, delta));
     if (ev) HIPCHK(hipEventRecord(ev[6], st));
     C.lastDelta = delta;
 }

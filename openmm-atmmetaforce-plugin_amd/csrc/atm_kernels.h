@@ -126,9 +126,6 @@ hipError_t launch_backward_pairs(const DevPtrs& P, int setBegin, int nsets, int 
 // clear: write the blended force instead of adding it to outF.
 hipError_t launch_blend(const DevPtrs& P, int sysBegin, int nsys, int maxAtoms, int hostMode, double sp, double dir,
                         bool clear, hipStream_t st, bool delta = false);
-// Energies, combine and blend in one launch (the batched evaluation with forces).
-hipError_t launch_energy_blend(const DevPtrs& P, int sysBegin, int nsys, int maxAtoms, int estride, bool clear,
-                               hipStream_t st, bool delta = false);
 // State difference: the pairs with a displaced atom of systems [sysBegin, sysBegin + nsys) at
 // x and at x + d (after the state-0 sets' build; maxLig: most displaced atoms of one system),
 // and the exception terms with a displaced atom (their difference).

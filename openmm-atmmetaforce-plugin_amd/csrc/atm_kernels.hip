@@ -2234,13 +2234,13 @@ __device__ __forceinline__ double block_sum_256(double v, double* red) {
     return r;
 }
 
-// U1 and U2 of system s in every thread of the block: fixed-order FP64 sums of the
-// energy slots (estride: pair_energy_stride), exception energies and offsets.  delta:
-// state difference (only the state-0 set was paired): U2 = U1 - E_d(x) + E_d(x + d) +
-// the displaced exception terms' difference; with `counts` the state-1 pair count is
-// derived the same way.  Every block that calls this gets the same sums (same order).
-__device__ __forceinline__ void system_energies(const DevPtrs& P, int s, int estride, int delta, bool counts,
-                                                double* red, double U[2]) {
+// delta: state difference (only the state-0 set was paired): U2 = U1 - E_d(x) +
+// E_d(x + d) + the displaced exception terms' difference, offsets per state; the
+// state-1 pair count is derived the same way.
+__global__ void __launch_bounds__(256) k_energy(DevPtrs P, int sysBegin, int estride, int delta) {
+    const int s = sysBegin + blockIdx.x;
+    __shared__ double red[256];
+    double U[2];
     if (delta) {
         const int c0 = 2 * s, c1 = 2 * s + 1;
         const SetDesc& D = P.desc[c0];
@@ -2258,26 +2258,24 @@ __device__ __forceinline__ void system_energies(const DevPtrs& P, int s, int est
         const double de = block_sum_256(dd, red) + (D.lxCount > 0 ? P.xenergy[c1] : 0.0);
         U[0] = e0 + (P.eoff ? P.eoff[c0] : 0.0);
         U[1] = (e0 + de) + (P.eoff ? P.eoff[c1] : 0.0);
-        if (counts && threadIdx.x == 0)
+        if (threadIdx.x == 0)
             P.rt[c1].npairs = P.rt[c0].npairs - P.rt[c0].lpairs[0] + P.rt[c0].lpairs[1];
-        return;
     }
-    for (int st = 0; st < 2; ++st) {
+    for (int st = 0; st < 2 && !delta; ++st) {
         const int c = 2 * s + st;
         const SetDesc& D = P.desc[c];
         const int ng = P.rt[c].ngroups;
         double acc = 0.0;
         for (int g = threadIdx.x; g < ng; g += 256) acc += group_energy(P, D, g, estride);
-        U[st] = block_sum_256(acc, red) + (D.excCount > 0 ? P.xenergy[c] : 0.0) + (P.eoff ? P.eoff[c] : 0.0);
+        red[threadIdx.x] = acc;
+        __syncthreads();
+        for (int o = 128; o > 0; o >>= 1) {
+            if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+            __syncthreads();
+        }
+        U[st] = red[0] + (D.excCount > 0 ? P.xenergy[c] : 0.0) + (P.eoff ? P.eoff[c] : 0.0);
+        __syncthreads();
     }
-}
-
-// One block of 256 threads per system: the energies, then the scalar ATM combine.
-__global__ void __launch_bounds__(256) k_energy(DevPtrs P, int sysBegin, int estride, int delta) {
-    const int s = sysBegin + blockIdx.x;
-    __shared__ double red[256];
-    double U[2];
-    system_energies(P, s, estride, delta, true, red, U);
     if (threadIdx.x == 0) {
         SysResult r;
         atm_combine(P.params[s], U[0], U[1], r);
@@ -2311,12 +2309,15 @@ __global__ void __launch_bounds__(256) k_set_energy(DevPtrs P, int c, int estrid
 // The FP64 side sums of a slot are read only where its mask bit is set.
 // delta (state difference): F_sys (+)= F1 + w2 dF, dF read by atom from the
 // state-1 set's sums (F2 = F1 + dF, w1 + w2 = 1).
-__device__ __forceinline__ void blend_atom(const DevPtrs& P, int s, int i, int hostMode, double sp, double dir,
-                                           int clear, int delta) {
+__global__ void k_blend(DevPtrs P, int sysBegin, int hostMode, double spHost, double dirHost, int clear, int delta) {
+    const int s = sysBegin + blockIdx.y;
     const SetDesc& D0 = P.desc[2 * s];
     const SetDesc& D1 = P.desc[2 * s + 1];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int N = D0.natoms;
     if (i >= N) return;
+    const double sp = hostMode ? spHost : P.res[s].sp;
+    const double dir = hostMode ? dirHost : P.params[s].direction;
     const double w2 = dir > 0.0 ? sp : 1.0 - sp;
     const double w1 = dir > 0.0 ? 1.0 - sp : sp;
     const long long a = D0.slotOff + P.slotOf[D0.setAtomOff + i];
@@ -2358,30 +2359,6 @@ __device__ __forceinline__ void blend_atom(const DevPtrs& P, int s, int i, int h
     F[i] = ox + (w1 * f1x + w2 * f2x);
     F[N + i] = oy + (w1 * f1y + w2 * f2y);
     F[2 * N + i] = oz + (w1 * f1z + w2 * f2z);
-}
-
-__global__ void k_blend(DevPtrs P, int sysBegin, int hostMode, double spHost, double dirHost, int clear, int delta) {
-    const int s = sysBegin + blockIdx.y;
-    const double sp = hostMode ? spHost : P.res[s].sp;
-    const double dir = hostMode ? dirHost : P.params[s].direction;
-    blend_atom(P, s, blockIdx.x * blockDim.x + threadIdx.x, hostMode, sp, dir, clear, delta);
-}
-
-// The batched evaluation's last kernel with forces: every block of system
-// sysBegin + blockIdx.y sums the system's energies (system_energies: the same
-// fixed order in every block, so the same U1, U2 and weights), combines them,
-// and blends its 256 atoms; block 0 stores the results.  Replaces k_energy +
-// k_blend (one launch, and the blend needs no read of the stored weight).
-__global__ void __launch_bounds__(256) k_energy_blend(DevPtrs P, int sysBegin, int estride, int delta, int clear) {
-    const int s = sysBegin + blockIdx.y;
-    __shared__ double red[256];
-    double U[2];
-    system_energies(P, s, estride, delta, blockIdx.x == 0, red, U);
-    const AtmParamsDev prm = P.params[s];
-    SysResult r;
-    atm_combine(prm, U[0], U[1], r);
-    if (blockIdx.x == 0 && threadIdx.x == 0) P.res[s] = r;
-    blend_atom(P, s, blockIdx.x * blockDim.x + threadIdx.x, 0, r.sp, prm.direction, clear, delta);
 }
 
 __global__ void k_fill_i64(long long* p, long long n) {
@@ -2517,12 +2494,6 @@ hipError_t launch_backward_pairs(const DevPtrs& P, int setBegin, int nsets, int 
 
 hipError_t launch_exceptions(const DevPtrs& P, int setBegin, int nsets, bool forces, hipStream_t st) {
     k_exceptions<<<nsets, 256, 0, st>>>(P, setBegin, forces ? 1 : 0);
-    return hipGetLastError();
-}
-
-hipError_t launch_energy_blend(const DevPtrs& P, int sysBegin, int nsys, int maxAtoms, int estride, bool clear,
-                               hipStream_t st, bool delta) {
-    k_energy_blend<<<dim3(cdiv(maxAtoms, 256), nsys), 256, 0, st>>>(P, sysBegin, estride, delta ? 1 : 0, clear ? 1 : 0);
     return hipGetLastError();
 }
 

@@ -240,9 +240,9 @@ extern "C" int atm_eval_noalloc(const atm_system_desc* system, const atm_params*
         HIPCHK(launch_pairlist(P, 0, 2, maxGroups, 4, st));
         HIPCHK(launch_pairs(P, 0, 2, maxGroups, tu.pairWaves, withForces, (flags & ATM_EVAL_FP32_PAIRS) == 0,
                             tu.pairKernel, tu.pairSplit, st));
-        const int estride = pair_energy_stride(tu.pairKernel, (flags & ATM_EVAL_FP32_PAIRS) == 0, tu.pairSplit);
-        if (withForces) HIPCHK(launch_energy_blend(P, 0, 1, n, estride, true, st));
-        else HIPCHK(launch_energy(P, 0, 1, estride, st));
+        HIPCHK(launch_energy(P, 0, 1, pair_energy_stride(tu.pairKernel, (flags & ATM_EVAL_FP32_PAIRS) == 0, tu.pairSplit),
+                             st));
+        if (withForces) HIPCHK(launch_blend(P, 0, 1, n, 0, 0.0, 0.0, true, st));
         hipLaunchKernelGGL(k_noalloc_finish, dim3((std::max(n, 1) + 255) / 256), dim3(256), 0, st, P, n,
                            withForces ? 1 : 0, (flags & ATM_EVAL_CLEAR_FORCES) ? 1 : 0, forces, status);
         HIPCHK(hipGetLastError());
