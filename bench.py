@@ -344,7 +344,8 @@ def summarize_profiles(root: str, tuning_label: str, out_json: str) -> str:
                        "valu_insts_per_launch": round(valu) if valu else None,
                        "busy_cycles_per_xcd": round(grbm / 8.0) if grbm else None,
                        "valu_issue_frac": round(issue, 3) if issue else None,
-                       "workload": "22 replicas x 23,000 atoms, both states (44 sets) per launch"},
+                       "workload": "22 replicas x 23,000 atoms; the sets the pair kernel pairs per launch "
+                                   "(the 22 x sets, x + d as the difference at the displaced atoms)"},
            "k_blend": {"hbm_bytes_per_launch": round(sum(traffic.get("atm::k_blend", (0, 0, 0))[:2]))}}
     with open(out_json, "w") as fh:
         json.dump(rec, fh, indent=1)
@@ -635,13 +636,14 @@ def main():
     F0 = batch.forces(0)
 
     pair_tflops = FLOP_PER_PAIR * pairs / (prof["pairs_ms"] * 1e-3) / 1e12
-    traffic = issue = None
+    traffic = issue = valu_launch = None
     tsource = None
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tpath):  # HBM bytes and VALU issue rate per launch from the committed rocprofv3 --pmc summaries
         with open(tpath) as fh:
             kp = json.load(fh).get("k_pairs", {})
         traffic, issue = kp.get("hbm_bytes_per_launch"), kp.get("valu_issue_frac")
+        valu_launch = kp.get("valu_insts_per_launch")
         tsource = {"file": "profiles/traffic.json", "measured_with": "rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE, "
                    "SQ_INSTS_VALU, GRBM_GUI_ACTIVE (separate runs of bench.py, see profiles/k_pairs_22rep_23k.md)",
                    "kernel": kp.get("kernel"), "tuning": kp.get("tuning")}
@@ -652,7 +654,9 @@ def main():
                              "stage_ms.displaced_ms)" if state2_mode == "difference" else "x and x + d of every replica)"),
                 "per_launch_flop": FLOP_PER_PAIR * pairs, "pairs_per_launch": pairs,
                 "avg_launch_ms": round(prof["pairs_ms"], 4),
-                "valu_issue_frac": issue, "traffic_and_issue_source": tsource,
+                "valu_issue_frac": issue,
+                "valu_insts_per_pair": round(valu_launch / pairs, 3) if valu_launch else None,
+                "traffic_and_issue_source": tsource,
                 "note": "achieved counts 57 FLOP per in-cutoff pair only; the kernel issues VALU instructions "
                         "at valu_issue_frac of the chip's rate (SQ_INSTS_VALU over GRBM_GUI_ACTIVE, "
                         "profiles/k_pairs_22rep_23k.md): 8x8 cluster tiles hold 21 % in-cutoff pairs"}
