@@ -155,6 +155,9 @@ def config_report(steps=50, cpu_seconds=5.0):
     systems = W.uneven_batch(64)
     with ATMBatch(systems) as b:
         b.compute()
+        b.autotune(repeats=3)
+        b.compute()
+        tuning4 = b.tuning()
         for _ in range(5):
             b.evaluate(forces=True, clear=True)
         b.synchronize()
@@ -163,7 +166,8 @@ def config_report(steps=50, cpu_seconds=5.0):
             b.evaluate(forces=True, clear=True)
         b.synchronize()
         dt = time.perf_counter() - t0
-        pairs = int(b.list_stats()["pairs"].sum())
+        ls = b.list_stats()
+        pairs = int(ls["pairs"][ls["max_entries"] > 0].sum())  # the sets the pair kernel paired
         b.compute()
         worst = {"U1_rel": 0.0, "U2_rel": 0.0, "W_rel": 0.0, "force_rel_rms": 0.0}
         t_or = time.perf_counter()
@@ -176,7 +180,7 @@ def config_report(steps=50, cpu_seconds=5.0):
     out["config4"] = {"systems": len(systems), "atoms": sum(s.num_atoms for s in systems),
                       "atoms_min_max": [min(s.num_atoms for s in systems), max(s.num_atoms for s in systems)],
                       "system_evals_per_s": round(len(systems) * steps / dt, 1), "ms_per_batch": round(dt * 1e3 / steps, 4),
-                      "pair_interactions_per_s": round(pairs * steps / dt, 1),
+                      "pair_interactions_per_s": round(pairs * steps / dt, 1), "tuning": tuning4,
                       "max_error_vs_fp64_oracle_over_systems": worst}
 
     log("[bench] config5: forward + backward, 22 replicas")
@@ -515,7 +519,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--tuning", default=None,
-                    help="column_scale,pair_waves_per_block,zbins_per_column[,pair_kernel] (skips the autotuner)")
+                    help="column_scale,pair_waves_per_block,zbins_per_column[,pair_kernel[,pair_split[,pair_order]]] "
+                         "(skips the autotuner)")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--configs", action="store_true",
@@ -567,7 +572,7 @@ def main():
     if args.tuning:
         f = args.tuning.split(",")
         batch.set_tuning(float(f[0]), int(f[1]), int(f[2]), int(f[3]) if len(f) > 3 else 0,
-                         pair_split=int(f[4]) if len(f) > 4 else 1)
+                         pair_split=int(f[4]) if len(f) > 4 else 1, pair_order=int(f[5]) if len(f) > 5 else 2)
         batch.compute()
         tuning = batch.tuning()
     elif not args.no_autotune:
@@ -615,7 +620,7 @@ def main():
         log("[bench] x + d binned and paired in full (full_state2)")
         t = batch.tuning()
         batch.set_tuning(t["column_scale"], t["pair_waves_per_block"], t["zbins_per_column"], t["pair_kernel"],
-                         t["list_capacity"], t["pair_split"], full_state2=True)
+                         t["list_capacity"], t["pair_split"], full_state2=True, pair_order=t["pair_order"])
         for _ in range(args.warmup):
             batch.evaluate(forces=True, clear=True)
         batch.synchronize()
@@ -625,7 +630,7 @@ def main():
         batch.synchronize()
         full_eps = args.replicas * args.steps / (time.perf_counter() - t1)
         batch.set_tuning(t["column_scale"], t["pair_waves_per_block"], t["zbins_per_column"], t["pair_kernel"],
-                         t["list_capacity"], t["pair_split"])
+                         t["list_capacity"], t["pair_split"], pair_order=t["pair_order"])
         batch.compute()
 
     log("[bench] per-stage HIP-event timing")

@@ -137,6 +137,12 @@ typedef struct atm_tuning {
                                  exception terms with a displaced atom at both states; the
                                  other atoms do not move), so only x is binned and paired;
                                  1: x + d is binned and paired in full like x */
+    int32_t pair_order;       /* the tile kernel's group order: 0 launch order; 1 every set's
+                                 groups longest list first, interleaved over all sets; 2 the
+                                 same over windows of 8 sets (default); 3 each set longest
+                                 first, the sets one after the other.  The autotuner times
+                                 all four for the tile kernel */
+    int32_t reserved_;        /* 0 */
 } atm_tuning;
 
 /* Library version string and ABI number. */

@@ -70,7 +70,8 @@ class Tuning(ctypes.Structure):
     _fields_ = [("column_scale", ctypes.c_double), ("pair_waves_per_block", ctypes.c_int32),
                 ("zbins_per_column", ctypes.c_int32), ("best_ms", ctypes.c_double),
                 ("pair_kernel", ctypes.c_int32), ("list_capacity", ctypes.c_int32),
-                ("pair_split", ctypes.c_int32), ("full_state2", ctypes.c_int32)]
+                ("pair_split", ctypes.c_int32), ("full_state2", ctypes.c_int32),
+                ("pair_order", ctypes.c_int32), ("reserved_", ctypes.c_int32)]
 
 
 class StageTimes(ctypes.Structure):

@@ -293,20 +293,23 @@ class ATMBatch:
         return _tuning_dict(t)
 
     def set_tuning(self, column_scale=1.3, pair_waves_per_block=2, zbins_per_column=4, pair_kernel=0,
-                   list_capacity=0, pair_split=1, full_state2=False):
+                   list_capacity=0, pair_split=1, full_state2=False, pair_order=2):
         """pair_kernel 0: 8x8 cluster-pair tiles; 1-4: compacted in-cutoff pairs with (list entries
         per batch, copies of the j-force sums, drain after every test step) = (4, 2, yes), (4, 2, no),
         (4, 4, yes), (8, 2, yes); 5: atom-level j lists per i-cluster, 8 i x 8 j lanes per step;
         6: filtered atom queues.  list_capacity: list entries per group to start from (0: sized from
         the density).  pair_split: waves sharing one group's list in the tile kernel (1, 2, 4, 8).
         full_state2: bin and pair x + d in full instead of as the difference from x at the
-        displaced atoms (used when every system displaces at most a quarter of its atoms)."""
+        displaced atoms (used when every system displaces at most a quarter of its atoms).
+        pair_order: the tile kernel's group order (0 launch order, 1 longest lists first over all
+        sets, 2 the same over windows of 8 sets, 3 each set longest first, one set after another)."""
         t = C.Tuning(column_scale, pair_waves_per_block, zbins_per_column, 0.0, pair_kernel, list_capacity,
-                     pair_split, 1 if full_state2 else 0)
+                     pair_split, 1 if full_state2 else 0, pair_order, 0)
         C.check(self._lib.atm_set_tuning(self._h, ctypes.byref(t)))
 
 
 def _tuning_dict(t) -> dict:
     return {"column_scale": t.column_scale, "pair_waves_per_block": t.pair_waves_per_block,
             "zbins_per_column": t.zbins_per_column, "pair_kernel": t.pair_kernel, "best_ms": t.best_ms,
-            "list_capacity": t.list_capacity, "pair_split": t.pair_split, "full_state2": bool(t.full_state2)}
+            "list_capacity": t.list_capacity, "pair_split": t.pair_split, "full_state2": bool(t.full_state2),
+            "pair_order": t.pair_order}

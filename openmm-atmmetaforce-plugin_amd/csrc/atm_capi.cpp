@@ -447,6 +447,7 @@ void alloc_layout(Context& C) {
     P.setEnergy = C.dSetEnergy.as<double>();
     P.eoff = C.dEoff.as<double>();
     P.lptItems = C.dLptItems.as<int>();
+    P.lptMode = C.tuning.pairOrder;
     P.gq = P.gsig = P.geps = nullptr;
     upload_exceptions(C);
     C.dropGraphs();  // new pointers in P (the displaced-atom arrays may be reallocated below)
@@ -588,6 +589,7 @@ void enqueue_batched(Context& C, int flags, hipEvent_t* ev = nullptr) {
     const bool delta = use_delta(C, flags);
     DevPtrs P = C.P;
     P.setStride = delta ? 2 : 1;
+    P.lptMode = C.tuning.pairOrder;
     const int nsets = delta ? C.nsys : 2 * C.nsys;
     const bool forces = (flags & ATM_EVAL_FORCES) != 0;
     const bool grads = (flags & ATM_EVAL_PARAM_GRADS) != 0;
@@ -1325,6 +1327,8 @@ int atm_get_tuning(void* context, atm_tuning* out) {
         out->list_capacity = C.tuning.listCapacity;
         out->pair_split = C.tuning.pairSplit;
         out->full_state2 = C.tuning.fullState2;
+        out->pair_order = C.tuning.pairOrder;
+        out->reserved_ = 0;
     });
 }
 
@@ -1345,6 +1349,7 @@ int atm_set_tuning(void* context, const atm_tuning* t) {
         if (t->pair_split != 1 && t->pair_split != 2 && t->pair_split != 4 && t->pair_split != 8)
             fail(ATM_ERR_INVALID_ARGUMENT, "pair_split must be 1, 2, 4 or 8");
         if (t->full_state2 != 0 && t->full_state2 != 1) fail(ATM_ERR_INVALID_ARGUMENT, "full_state2 must be 0 or 1");
+        if (t->pair_order < 0 || t->pair_order > 3) fail(ATM_ERR_INVALID_ARGUMENT, "pair_order must be in [0, 3]");
         HIPCHK(hipSetDevice(C.device));
         HIPCHK(hipStreamSynchronize(C.stream));
         const bool relayout = t->column_scale != C.tuning.colScale || t->zbins_per_column != C.tuning.zbinsPerColumn ||
@@ -1356,6 +1361,8 @@ int atm_set_tuning(void* context, const atm_tuning* t) {
         C.tuning.pairKernel = t->pair_kernel;
         C.tuning.pairSplit = t->pair_split;
         C.tuning.fullState2 = t->full_state2;
+        C.tuning.pairOrder = t->pair_order;
+        C.P.lptMode = t->pair_order;
         if (t->list_capacity != C.tuning.listCapacity) {
             C.entryCapOverride.assign(2 * C.nsys, 0);
             C.alistCapOverride.assign(2 * C.nsys, 0);
@@ -1431,8 +1438,8 @@ int atm_autotune(void* context, int32_t repeats, atm_tuning* out) {
             layout(C);
             alloc_layout(C);
         };
-        // sweep, then re-time the four fastest with 4x the repeats (a single short
-        // timing of near-equal configurations picks by noise)
+        // sweep (at the current group order), then re-time the four fastest with 4x the
+        // repeats (a single short timing of near-equal configurations picks by noise)
         std::vector<std::pair<double, Tuning>> timed;
         for (double sc : scales) {
             for (int zp : zpcs) {
@@ -1450,14 +1457,19 @@ int atm_autotune(void* context, int32_t repeats, atm_tuning* out) {
                          [](const std::pair<double, Tuning>& a, const std::pair<double, Tuning>& b) { return a.first < b.first; });
         Tuning best = orig;  // kept if no configuration fits the list capacity
         double bestMs = 1e300;
+        // (the tile kernel's candidates with each of its four group orders)
         for (size_t i = 0; i < timed.size() && i < 4; ++i) {
             const Tuning t = timed[i].second;
             set_layout(t.colScale, t.zbinsPerColumn);
-            C.tuning = t;
-            const double per = time_config(4 * reps);
-            if (per < bestMs) {
-                bestMs = per;
-                best = t;
+            for (int order = 0; order <= 3; ++order) {
+                if (t.pairKernel != 0 && order != t.pairOrder) continue;
+                C.tuning = t;
+                C.tuning.pairOrder = t.pairKernel == 0 ? order : t.pairOrder;
+                const double per = time_config(4 * reps);
+                if (per < bestMs) {
+                    bestMs = per;
+                    best = C.tuning;
+                }
             }
         }
         (void)hipEventDestroy(e0);
@@ -1479,6 +1491,8 @@ int atm_autotune(void* context, int32_t repeats, atm_tuning* out) {
             out->list_capacity = best.listCapacity;
             out->pair_split = best.pairSplit;
             out->full_state2 = best.fullState2;
+            out->pair_order = best.pairOrder;
+            out->reserved_ = 0;
         }
     });
 }

@@ -70,6 +70,7 @@ struct DevPtrs {
     // group order of the tile kernel, longest lists first per set (k_order, run by the pair
     // launchers; the set's range at groupOff); null: launch order
     int* lptItems;
+    int lptMode;           // kOrder*: how the tile kernel's launch ranks map to the groups
     // backward
     double* gq;            // [2 S sets x slots] gradient accumulators in slot order
     double* gsig;
@@ -108,6 +109,11 @@ inline int pair_energy_stride(int variant, bool fp64, int split) {
 // `forces` their forces added to the set's slot sums (call after the pair kernel
 // and before launch_energy / launch_set_energy, which add xenergy when excCount > 0).
 hipError_t launch_exceptions(const DevPtrs& P, int setBegin, int nsets, bool forces, hipStream_t st);
+// The tile kernel's group order (atm_tuning.pair_order): launch order; every set's groups
+// longest list first, interleaved over all sets / over windows of kOrderWindowSets sets;
+// each set's groups longest first, the sets one after the other.
+constexpr int kOrderLaunch = 0, kOrderAllSets = 1, kOrderWindows = 2, kOrderSetMajor = 3;
+constexpr int kOrderWindowSets = 8;
 // Length buckets of the tile kernel's group order (entries / kLptShift, capped).
 constexpr int kLptBuckets = 64;
 constexpr int kLptShift = 4;

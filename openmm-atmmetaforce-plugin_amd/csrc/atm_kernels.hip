@@ -868,15 +868,29 @@ __global__ void __launch_bounds__(64 * W, 6) k_pairs(DevPtrs P, int setBegin, co
     const int jj = lane >> 3, ii = lane & 7;
     int c, g, part;
     bool active;
-    if (P.lptItems) {
-        // longest lists first: this wave's rank f in launch order is part f % split of item
-        // f / split, the (item / nsets)-th longest group of set item mod nsets (k_order)
-        const int f = __builtin_amdgcn_readfirstlane((int)((blockIdx.y * gridDim.x + blockIdx.x) * W + wave));
-        const int item = f / split;
-        part = f - item * split;
-        const int nsets = gridDim.y, pos = item / nsets;
-        c = setBegin + (item - pos * nsets) * P.setStride;
+    if (P.lptItems && P.lptMode == kOrderSetMajor) {
+        // each set's groups longest list first, the sets one after the other (blockIdx.y)
+        const int f = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * W + wave));
+        const int pos = f / split;
+        part = f - pos * split;
+        c = setBegin + (int)blockIdx.y * P.setStride;
         active = pos < P.rt[c].ngroups;
+        g = active ? __builtin_amdgcn_readfirstlane(P.lptItems[P.desc[c].groupOff + pos]) : 0;
+    } else if (P.lptItems && P.lptMode != kOrderLaunch) {
+        // longest lists first over windows of sets: this wave's rank f in launch order falls in
+        // window f / (K gx) of K sets (K = every set, or kOrderWindowSets; gx waves per set), where
+        // it is part (rank in the window) % split of item (rank in the window) / split: the
+        // (item / K')-th longest group of the window's set item mod K' (K' its sets; k_order)
+        const int gx = (int)gridDim.x * W;
+        const int K = P.lptMode == kOrderAllSets ? (int)gridDim.y : kOrderWindowSets;
+        const int f = __builtin_amdgcn_readfirstlane((int)((blockIdx.y * gridDim.x + blockIdx.x) * W + wave));
+        const int win = f / (K * gx), within = f - win * (K * gx);
+        const int kw = min(K, (int)gridDim.y - win * K);
+        const int item = within / split;
+        part = within - item * split;
+        const int pos = item / kw;
+        c = setBegin + (win * K + (item - pos * kw)) * P.setStride;
+        active = within < kw * gx && pos < P.rt[c].ngroups;
         g = active ? __builtin_amdgcn_readfirstlane(P.lptItems[P.desc[c].groupOff + pos]) : 0;
     } else {
         c = setBegin + (int)blockIdx.y * P.setStride;
@@ -2450,7 +2464,7 @@ hipError_t launch_pairs(const DevPtrs& P, int setBegin, int nsets, int maxGroups
     }
     if (split < 1 || split > 8) return hipErrorInvalidValue;
     const DevPtrs& Pk = P;
-    if (P.lptItems) k_order<<<nsets, 512, 0, st>>>(P, setBegin);
+    if (P.lptItems && P.lptMode != kOrderLaunch) k_order<<<nsets, 512, 0, st>>>(P, setBegin);
 #define ATM_PAIRS_X(WV, F64, X)                                                                                   \
     if (forces) k_pairs<WV, true, F64, false, X><<<dim3(cdiv(maxGroups, WV) * split, nsets), 64 * WV, 0, st>>>(   \
         Pk, setBegin, nullptr, split);                                                                            \
@@ -2486,7 +2500,7 @@ hipError_t launch_backward_pairs(const DevPtrs& P, int setBegin, int nsets, int 
                                  int split, hipStream_t st) {
     if (split < 1 || split > 8) return hipErrorInvalidValue;
     const DevPtrs& Pk = P;
-    if (P.lptItems) k_order<<<nsets, 512, 0, st>>>(P, setBegin);
+    if (P.lptItems && P.lptMode != kOrderLaunch) k_order<<<nsets, 512, 0, st>>>(P, setBegin);
     if (P.xmask) k_pairs<1, false, true, true, true><<<dim3(maxGroups * split, nsets), 64, 0, st>>>(Pk, setBegin, wset, split);
     else k_pairs<1, false, true, true, false><<<dim3(maxGroups * split, nsets), 64, 0, st>>>(Pk, setBegin, wset, split);
     return hipGetLastError();

@@ -16,6 +16,7 @@ struct Tuning {
     int listCapacity = 0;      // list entries per group to start from (0: sized from the density)
     int pairSplit = 1;         // waves sharing one group's list in the tile kernel (1, 2, 4 or 8)
     int fullState2 = 0;        // 1: bin and pair the x + d set of every system (no state difference)
+    int pairOrder = 2;         // the tile kernel's group order (kOrder* in atm_kernels.h)
 };
 
 // Throws AtmError(ATM_ERR_INVALID_ARGUMENT) for an unusable system description.

@@ -383,3 +383,28 @@ def test_fp32_pair_mode_batch(gpu):
             assert abs(r["state2_energy"] - ref["U2"]) <= b2
             assert abs(r["energy"] - ref["W"]) <= E_TOL * abs(ref["W"]) + abs(w1) * b1 + abs(w2) * b2
             assert frms(b.forces(k), Fref) < F_TOL
+
+
+def test_tile_kernel_group_orders_agree(gpu):
+    """The tile kernel's group orders (launch order; longest lists first over all sets,
+    over windows of 8 sets, or set by set) change only which wave runs which group:
+    energies, fixed-point forces and pair counts are identical, and match the oracle."""
+    systems = _variant_systems()
+    with ATMBatch(systems) as b:
+        ref = None
+        for order in (0, 1, 2, 3):
+            b.set_tuning(1.3, 2, 4, 0, pair_split=2, pair_order=order)
+            assert b.tuning()["pair_order"] == order
+            b.compute()
+            got = ([b.result(k) for k in range(len(systems))], [b.forces_fixed(k) for k in range(len(systems))],
+                   b.list_stats()["pairs"].copy())
+            if ref is None:
+                ref = got
+                counts = got[2]
+                for k, s in enumerate(systems):
+                    check_system(b, k, s, pairs=(k < 2), tag=f"order {order}", counts=counts)
+                continue
+            for k in range(len(systems)):
+                assert got[0][k] == ref[0][k], (order, k)
+                assert np.array_equal(got[1][k], ref[1][k]), (order, k)
+            assert np.array_equal(got[2], ref[2]), order
