@@ -30,7 +30,10 @@ METRIC = "ATM energy+force evals/s, 23k atoms: 22-replica batch vs plain 1-GPU"
 FLOP_PER_PAIR = 57            # FP32 FLOP per in-cutoff pair, FMA = 2
 PEAK_FP32_TFLOPS = 157.3      # MI355X FP32 vector peak (packed); 78.6 without packed FP32
 PEAK_HBM_GBS = 8000.0
-BLEND_BYTES_PER_ATOM = 96     # read F1, F2 (int64 x3 each), read + write F (int64 x3)
+# k_blend's algorithmic bytes per atom with F cleared (no read of the old F): the
+# atom's slot index (int32), F1 (int64 x3) and F2 or dF (int64 x3) read, F
+# (FP64 x3) written; the full mode reads a second slot index for F2
+BLEND_BYTES_PER_ATOM = {"difference": 76, "full": 80}
 
 
 def log(msg):
@@ -196,18 +199,25 @@ def config_report(steps=50, cpu_seconds=5.0):
         return q, sg, ep, sc
 
     with ATMBatch(reps) as b:
+        b.compute()
+        b.autotune(repeats=3)
         q, sg, ep, sc = tensors()
 
         def fwd_bwd():
             Wt = ATMEnergyFunction.apply(b, q, sg, ep, *sc)
             Wt.sum().backward()
+            for t in (q, sg, ep, *sc):
+                t.grad = None
             return Wt
 
-        fwd_bwd()
-        n = 10
+        for _ in range(3):
+            fwd_bwd()
+        torch.cuda.synchronize()
+        n = 20
         t0 = time.perf_counter()
         for _ in range(n):
             fwd_bwd()
+        torch.cuda.synchronize()
         fb_ms = (time.perf_counter() - t0) * 1e3 / n
     out["config5"] = {"replicas": 22, "ms_per_forward_backward": round(fb_ms, 3),
                       "note": "device tensors in and device gradients out; gradients checked against the FP64 "
@@ -350,7 +360,12 @@ def summarize_profiles(root: str, tuning_label: str, out_json: str) -> str:
                        "valu_issue_frac": round(issue, 3) if issue else None,
                        "workload": "22 replicas x 23,000 atoms; the sets the pair kernel pairs per launch "
                                    "(the 22 x sets, x + d as the difference at the displaced atoms)"},
-           "k_blend": {"hbm_bytes_per_launch": round(sum(traffic.get("atm::k_blend", (0, 0, 0))[:2]))}}
+           "k_blend": {"hbm_bytes_per_launch": round(sum(traffic.get("atm::k_blend", (0, 0, 0))[:2])),
+                       "hbm_read_bytes_per_launch": round(traffic.get("atm::k_blend", (0, 0, 0))[0]),
+                       "hbm_write_bytes_per_launch": round(traffic.get("atm::k_blend", (0, 0, 0))[1]),
+                       "trace_avg_us": next((round(float(r["AverageNs"]) / 1e3, 3)
+                                             for r in rows("trace", "*kernel_stats.csv")
+                                             if kname(r) == "atm::k_blend"), None)}}
     with open(out_json, "w") as fh:
         json.dump(rec, fh, indent=1)
     return "\n".join(out)
@@ -666,7 +681,13 @@ def main():
                         "at valu_issue_frac of the chip's rate (SQ_INSTS_VALU over GRBM_GUI_ACTIVE, "
                         "profiles/k_pairs_22rep_23k.md): 8x8 cluster tiles hold 21 % in-cutoff pairs"}
     total_atoms = sum(s.num_atoms for s in reps)
-    blend_gbs = BLEND_BYTES_PER_ATOM * total_atoms / (prof["blend_ms"] * 1e-3) / 1e9
+    blend_bytes = BLEND_BYTES_PER_ATOM[state2_mode] * total_atoms
+    blend_gbs = blend_bytes / (prof["blend_ms"] * 1e-3) / 1e9
+    blend_traffic = blend_trace_us = None
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            kb = json.load(fh).get("k_blend", {})
+        blend_traffic, blend_trace_us = kb.get("hbm_bytes_per_launch"), kb.get("trace_avg_us")
     backward = None
     if not args.no_backward:
         log("[bench] forward + backward through ATMEnergyFunction (device tensors)")
@@ -725,7 +746,15 @@ def main():
         "fp32_pair_mode_evals_per_s": round(fp32_eps, 2) if fp32_eps else None,
         "stage_ms": {k: round(v, 4) for k, v in prof.items() if k.endswith("_ms")},
         "blend_kernel": {"bound": "hbm", "achieved": round(blend_gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                         "frac": round(blend_gbs / PEAK_HBM_GBS, 4)},
+                         "frac": round(blend_gbs / PEAK_HBM_GBS, 4), "algorithmic_bytes": blend_bytes,
+                         "bytes_per_atom": BLEND_BYTES_PER_ATOM[state2_mode], "traffic": blend_traffic,
+                         "traffic_over_algorithmic": round(blend_traffic / blend_bytes, 3) if blend_traffic else None,
+                         "kernel_trace_us": blend_trace_us,
+                         "kernel_trace_frac": round(blend_bytes / (blend_trace_us * 1e-6) / 1e9 / PEAK_HBM_GBS, 4)
+                         if blend_trace_us else None,
+                         "note": "achieved: HIP events around the blend stage (stage_ms.blend_ms, launch gap "
+                                 "included); kernel_trace_us and traffic (FETCH_SIZE + WRITE_SIZE): rocprofv3 runs "
+                                 "of this bench, profiles/traffic.json"},
         "backward": backward,
         "companion_restraints": restraints,
         "replica_md": md,

@@ -2323,11 +2323,21 @@ __global__ void __launch_bounds__(256) k_set_energy(DevPtrs P, int c, int estrid
 // The FP64 side sums of a slot are read only where its mask bit is set.
 // delta (state difference): F_sys (+)= F1 + w2 dF, dF read by atom from the
 // state-1 set's sums (F2 = F1 + dF, w1 + w2 = 1).
-__global__ void k_blend(DevPtrs P, int sysBegin, int hostMode, double spHost, double dirHost, int clear, int delta) {
-    const int s = sysBegin + blockIdx.y;
+// Block order: the (system, atom block) list, gx blocks per system, is cut
+// into 8 contiguous eighths and launch block b takes item b / 8 of eighth
+// b mod 8.  Workgroups go to the 8 XCDs round-robin by launch index, so each
+// XCD's L2 gathers the F1 lines (slot order) of its own eighth of the systems
+// instead of every XCD fetching lines of every system.
+__global__ void k_blend(DevPtrs P, int sysBegin, int nsys, int gx, int hostMode, double spHost, double dirHost,
+                        int clear, int delta) {
+    const int per = (gx * nsys + 7) >> 3;
+    const int item = ((int)blockIdx.x & 7) * per + ((int)blockIdx.x >> 3);
+    if (item >= gx * nsys) return;
+    const int sys = item / gx;
+    const int s = sysBegin + sys;
     const SetDesc& D0 = P.desc[2 * s];
     const SetDesc& D1 = P.desc[2 * s + 1];
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = (item - sys * gx) * blockDim.x + threadIdx.x;
     const int N = D0.natoms;
     if (i >= N) return;
     const double sp = hostMode ? spHost : P.res[s].sp;
@@ -2513,8 +2523,10 @@ hipError_t launch_exceptions(const DevPtrs& P, int setBegin, int nsets, bool for
 
 hipError_t launch_blend(const DevPtrs& P, int sysBegin, int nsys, int maxAtoms, int hostMode, double sp, double dir,
                         bool clear, hipStream_t st, bool delta) {
-    k_blend<<<dim3(cdiv(maxAtoms, 256), nsys), 256, 0, st>>>(P, sysBegin, hostMode, sp, dir, clear ? 1 : 0,
-                                                             delta ? 1 : 0);
+    if (nsys <= 0 || maxAtoms <= 0) return hipSuccess;
+    const int gx = (int)cdiv(maxAtoms, 256);
+    const unsigned blocks = 8u * cdiv((long long)gx * nsys, 8);
+    k_blend<<<blocks, 256, 0, st>>>(P, sysBegin, nsys, gx, hostMode, sp, dir, clear ? 1 : 0, delta ? 1 : 0);
     return hipGetLastError();
 }
 
