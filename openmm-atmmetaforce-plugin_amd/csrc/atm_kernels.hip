@@ -332,7 +332,29 @@ __global__ void __launch_bounds__(1024) k_scan(DevPtrs P, int setBegin) {
     const SetDesc& D = P.desc[c];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int ncols = D.nx * D.ny, nz = D.nz;
-    for (int col = wave; col < ncols; col += NW) {
+    if (nz <= 64) {
+        // a column in one wave step: U columns' counts loaded before their scans,
+        // so U loads per wave are in flight instead of one
+        constexpr int U = 8;
+        for (int col0 = wave; col0 < ncols; col0 += U * NW) {
+            int v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int col = col0 + u * NW;
+                v[u] = (col < ncols && lane < nz) ? P.cellCount[D.cellOff + (size_t)col * nz + lane] : 0;
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int col = col0 + u * NW;
+                const int incl = wave_incl_scan(v[u]);  // lane 63: the column count (lanes >= nz add 0)
+                if (col < ncols) {
+                    if (lane < nz) P.cellStart[D.cellOff + (size_t)col * nz + lane] = incl - v[u];
+                    if (lane == 63) P.colCount[D.colOff + col] = incl;
+                }
+            }
+        }
+    }
+    for (int col = nz <= 64 ? ncols : wave; col < ncols; col += NW) {
         const int* cnt = P.cellCount + D.cellOff + (size_t)col * nz;
         int* st = P.cellStart + D.cellOff + (size_t)col * nz;
         int run = 0;
