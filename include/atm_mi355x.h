@@ -3,7 +3,8 @@
  * (libatm_mi355x.so).
  *
  * One atm_context_t owns a batch of independent ATM systems resident on one
- * GPU.  A system is an orthorhombic periodic box of N atoms with per-atom
+ * GPU.  A system is a periodic box (orthorhombic, or triclinic through
+ * atm_set_box_vectors) of N atoms with per-atom
  * charge, Lennard-Jones sigma/epsilon and an ATM displacement vector, plus the
  * nine ATM global parameters.  One evaluation computes, for every system,
  *
@@ -219,8 +220,19 @@ int atm_set_positions_device(void* context, const float* positions_dev);
  * used by the autograd entry point. */
 int atm_set_particles_device(void* context, const double* charges, const double* sigmas, const double* epsilons);
 
-/* New box edges for system `sys` (copyState copies box vectors, :130-133). */
+/* New box edges for system `sys`: an orthorhombic box (copyState copies the
+ * box vectors, ReferenceATMMetaForceKernels.cpp:126-129). */
 int atm_set_box(void* context, int32_t sys, const double* box3);
+
+/* Periodic box vectors a, b, c of system `sys` (vectors[9] = ax ay az bx by bz
+ * cx cy cz, nm), triclinic boxes included: the box vectors copyState copies into
+ * both inner contexts (ReferenceATMMetaForceKernels.cpp:126-129).  They must be in
+ * OpenMM's reduced form (ay = az = bz = 0, positive diagonal, ax >= 2|bx|,
+ * ax >= 2|cx|, by >= 2|cy|) with cutoff <= half of ax, by and cz.  A pair takes
+ * the image OpenMM's triclinic rule picks (shift by c to the nearest z, then by b
+ * to the nearest y, then by a to the nearest x).  A batch with a triclinic system
+ * runs the 8x8 tile kernel and bins x + d in full (no state difference). */
+int atm_set_box_vectors(void* context, int32_t sys, const double* vectors);
 
 /* The nine ATM global parameters of system `sys` (read by execute,
  * ReferenceATMMetaForceKernels.cpp:66-80). */

@@ -34,7 +34,7 @@ EXPORTED = (
     "atm_version", "atm_abi_version", "atm_last_error", "atm_device_count", "atm_create", "atm_destroy",
     "atm_set_particles", "atm_set_particles_device", "atm_set_exceptions", "atm_set_energy_offsets", "atm_set_displacements", "atm_set_positions",
     "atm_set_positions_device",
-    "atm_set_box", "atm_set_params", "atm_eval", "atm_synchronize", "atm_get_results", "atm_get_forces",
+    "atm_set_box", "atm_set_box_vectors", "atm_set_params", "atm_eval", "atm_synchronize", "atm_get_results", "atm_get_forces",
     "atm_get_forces_fixed", "atm_clear_forces", "atm_get_pairs", "atm_backward", "atm_backward_device",
     "atm_set_stream",
     "atm_autotune", "atm_get_tuning", "atm_set_tuning", "atm_get_stats", "atm_get_list_stats", "atm_profile",
@@ -137,6 +137,7 @@ def load(path: str | None = None):
         "atm_set_positions": [vp, i32, dp],
         "atm_set_positions_device": [vp, vp],
         "atm_set_box": [vp, i32, dp],
+        "atm_set_box_vectors": [vp, i32, dp],
         "atm_set_params": [vp, i32, ctypes.POINTER(Params)],
         "atm_eval": [vp, i32],
         "atm_synchronize": [vp],

@@ -14,6 +14,7 @@ from typing import Sequence
 import numpy as np
 
 from . import _capi as C
+from .workloads import box_vectors
 
 
 def _dptr(a):
@@ -60,6 +61,9 @@ class ATMBatch:
         self._host_positions = [None] * n
         self._boxes = [np.array(s.box, dtype=np.float64).reshape(3) for s in self.systems]
         for k, s in enumerate(self.systems):
+            tilt = getattr(s, "tilt", None)
+            if tilt is not None and np.any(np.asarray(tilt) != 0.0):
+                self.set_box_vectors(k, box_vectors(s.box, tilt))
             self.set_particles(k, s.charges, s.sigmas, s.epsilons, s.displacements)
             self.set_positions(k, s.positions)
             self.set_params(k, s.params)
@@ -139,9 +143,23 @@ class ATMBatch:
         self._pos_version += 1
 
     def set_box(self, k, box):
-        b = np.ascontiguousarray(box, dtype=np.float64).reshape(3)
+        """Box of system k: edges (3,) of an orthorhombic box, or vectors (3, 3) / (9,)
+        (rows a, b, c in OpenMM's reduced form; triclinic boxes included)."""
+        b = np.ascontiguousarray(box, dtype=np.float64).reshape(-1)
+        if b.size == 9:
+            self.set_box_vectors(k, b)
+            return
+        b = b.reshape(3)
         C.check(self._lib.atm_set_box(self._h, k, _dptr(b)))
         self._boxes[k] = b.copy()
+        self._pos_version += 1
+
+    def set_box_vectors(self, k, vectors):
+        """Periodic box vectors of system k (atm_set_box_vectors): rows a, b, c of a (3, 3)
+        array in OpenMM's reduced form, nm."""
+        v = np.ascontiguousarray(vectors, dtype=np.float64).reshape(9)
+        C.check(self._lib.atm_set_box_vectors(self._h, k, _dptr(v)))
+        self._boxes[k] = v.copy()
         self._pos_version += 1
 
     def position_snapshot(self):

@@ -73,7 +73,9 @@ class Exceptions:
 
 @dataclasses.dataclass
 class System:
-    """One ATM system: N atoms in an orthorhombic periodic box."""
+    """One ATM system: N atoms in a periodic box, orthorhombic (edges `box`) or, with
+    `tilt` = (b_x, c_x, c_y), triclinic with vectors a = (box[0], 0, 0),
+    b = (b_x, box[1], 0), c = (c_x, c_y, box[2]) (OpenMM's reduced form)."""
     positions: np.ndarray      # (N, 3) float64, nm
     charges: np.ndarray        # (N,) float64, e
     sigmas: np.ndarray         # (N,) float64, nm
@@ -86,6 +88,7 @@ class System:
     seed: int = 0
     num_ligand: int = 0
     exceptions: Exceptions | None = None
+    tilt: np.ndarray | None = None  # (3,) float64, nm: (b_x, c_x, c_y); None: orthorhombic
 
     @property
     def num_atoms(self) -> int:
@@ -185,6 +188,28 @@ def tiny_system(num_atoms: int, box: float, cutoff: float, num_ligand: int,
                 displacement, seed: int, params: ATMParams | None = None) -> System:
     """Small systems for exhaustive tests (same generator, any size)."""
     return lattice_system(num_atoms, box, cutoff, num_ligand, displacement, seed, params)
+
+
+def box_vectors(box, tilt=None) -> np.ndarray:
+    """Rows a, b, c (3, 3) of the box with edges `box` and tilts (b_x, c_x, c_y)."""
+    t = np.zeros(3) if tilt is None else np.asarray(tilt, dtype=np.float64)
+    return np.array([[box[0], 0.0, 0.0], [t[0], box[1], 0.0], [t[1], t[2], box[2]]], dtype=np.float64)
+
+
+def triclinic(system: System, sites=(5, -4, 7)) -> System:
+    """The same atoms in a triclinic box with the same edges and tilts
+    (b_x, c_x, c_y) = sites x the lattice spacing L/n, rounded to the grid.  The
+    brick [0, L)^3 holding the lattice is a unit cell of the triclinic lattice
+    too, and whole-site tilts map lattice sites onto lattice sites, so the atoms
+    still form a jittered simple-cubic lattice across every face of the box."""
+    n = int(math.ceil(system.num_atoms ** (1.0 / 3.0) - 1e-9))
+    while n ** 3 < system.num_atoms:
+        n += 1
+    a = float(system.box[0]) / n
+    tilt = quantize(np.asarray(sites, dtype=np.float64) * a)
+    if abs(tilt[0]) > 0.5 * system.box[0] or abs(tilt[1]) > 0.5 * system.box[0] or abs(tilt[2]) > 0.5 * system.box[1]:
+        raise ValueError("tilts beyond half an edge: not OpenMM's reduced form")
+    return dataclasses.replace(system, tilt=tilt)
 
 
 def with_molecules(system: System, size: int = 3, scale14: float = 0.5, coulomb14: float = 0.8333,

@@ -11,6 +11,7 @@
 #include <iterator>
 #include <map>
 #include <memory>
+#include <array>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -52,6 +53,7 @@ struct Context {
     hipStream_t stream = nullptr;
     int nsys = 0;
     std::vector<atm_system_desc> sys;
+    std::vector<std::array<double, 3>> tilt;  // per system (b_x, c_x, c_y): all zero = orthorhombic
     std::vector<int> atomOff;
     long long totalAtoms = 0;
     int maxAtoms = 0;
@@ -179,8 +181,9 @@ void validate_desc(const atm_system_desc& d, int s) {
 }
 
 // Geometry and capacities of one coordinate set.
-SetDesc make_set(const atm_system_desc& d, const Tuning& tu, int s, int state) {
+SetDesc make_set(const atm_system_desc& d, const Tuning& tu, int s, int state, const double* tilt) {
     SetDesc D{};
+    const bool tri = tilt && (tilt[0] != 0.0 || tilt[1] != 0.0 || tilt[2] != 0.0);
     const double L[3] = {d.box[0], d.box[1], d.box[2]};
     const double rc = d.cutoff;
     const double rho = d.num_atoms / (L[0] * L[1] * L[2]);
@@ -192,7 +195,8 @@ SetDesc make_set(const atm_system_desc& d, const Tuning& tu, int s, int state) {
     const int nz = std::max(1, std::min((int)std::floor(L[2] / zeT), 4096));
     const double rcl = rc * (1.0 + 1e-5) + 1e-4;  // list cutoff: conservative w.r.t. FP32 box rounding
     int Rx = (int)std::ceil(rcl / cex), Ry = (int)std::ceil(rcl / cey);
-    const int fullX = (2 * Rx + 1 > nx), fullY = (2 * Ry + 1 > ny);
+    // triclinic: the c and b images move x and y by tilts, so every column is a candidate
+    const int fullX = tri || (2 * Rx + 1 > nx), fullY = tri || (2 * Ry + 1 > ny);
     if (fullX) Rx = nx;
     if (fullY) Ry = ny;
     const int shiftOK = !fullX && !fullY && (Rx + 1) * cex < L[0] - rc && (Ry + 1) * cey < L[1] - rc;
@@ -215,6 +219,8 @@ SetDesc make_set(const atm_system_desc& d, const Tuning& tu, int s, int state) {
     D.crf64 = 3.0 * eps / ((2.0 * eps + 1.0) * rc);
     D.nx = nx; D.ny = ny; D.nz = nz;
     D.Rx = Rx; D.Ry = Ry; D.fullX = fullX; D.fullY = fullY; D.shiftOK = shiftOK;
+    D.tri = tri;
+    for (int k = 0; k < 3; ++k) D.tilt[k] = tri ? (float)tilt[k] : 0.f;
     D.sys = s; D.state = state;
     D.natoms = d.num_atoms;
     const int ncols = nx * ny;
@@ -318,7 +324,7 @@ void layout(Context& C) {
         const Context::SysExceptions& X = C.exc[s];
         for (int st = 0; st < 2; ++st) {
             const int c = 2 * s + st;
-            SetDesc D = make_set(C.sys[s], C.tuning, s, st);
+            SetDesc D = make_set(C.sys[s], C.tuning, s, st, C.tilt[s].data());
             if (C.entryCapOverride[c] > D.entryCap) D.entryCap = C.entryCapOverride[c];
             D.fixMax = fixed_point_limit(D, X.maxPerAtom);
             D.excOff = (int)xOff;
@@ -448,6 +454,8 @@ void alloc_layout(Context& C) {
     P.eoff = C.dEoff.as<double>();
     P.lptItems = C.dLptItems.as<int>();
     P.lptMode = C.tuning.pairOrder;
+    P.anyTri = 0;
+    for (const SetDesc& D : C.sets) P.anyTri |= D.tri;
     P.gq = P.gsig = P.geps = nullptr;
     upload_exceptions(C);
     C.dropGraphs();  // new pointers in P (the displaced-atom arrays may be reallocated below)
@@ -477,6 +485,7 @@ void update_displaced(Context& C) {
         const int nl = (int)C.ligAtoms.size() - lb;
         C.maxLig = std::max(C.maxLig, nl);
         if (4ll * nl > n) C.deltaOK = false;
+        if (C.sets[2 * s].tri) C.deltaOK = false;  // the displaced-atom kernel takes orthorhombic images only
         const int xb = (int)C.lxidx.size();
         if (s < (int)C.exc.size()) {
             const auto& X = C.exc[s];
@@ -867,6 +876,7 @@ int atm_create(const atm_system_desc* systems, int32_t num_systems, int32_t devi
         C->stream = C->own;
         C->nsys = num_systems;
         C->sys.assign(systems, systems + num_systems);
+        C->tilt.assign(num_systems, std::array<double, 3>{0.0, 0.0, 0.0});
         C->atomOff.resize(num_systems);
         long long off = 0;
         for (int s = 0; s < num_systems; ++s) {
@@ -1098,6 +1108,37 @@ int atm_set_box(void* context, int32_t sys, const double* box3) {
         HIPCHK(hipSetDevice(C.device));
         HIPCHK(hipStreamSynchronize(C.stream));
         C.sys[sys] = d;
+        C.tilt[sys] = {0.0, 0.0, 0.0};
+        layout(C);
+        alloc_layout(C);
+        C.evaluated = false;
+        C.resultsValid = false;
+    });
+}
+
+int atm_set_box_vectors(void* context, int32_t sys, const double* v) {
+    return guarded([&] {
+        Context& C = ctx_of(context);
+        check_sys(C, sys);
+        if (!v) fail(ATM_ERR_INVALID_ARGUMENT, "null box vectors");
+        for (int k = 0; k < 9; ++k)
+            if (!std::isfinite(v[k])) fail(ATM_ERR_INVALID_ARGUMENT, "box vectors must be finite");
+        // OpenMM's reduced form: a along x, b in the x-y plane, each vector's tilt at
+        // most half the edge of the ones before it
+        if (v[1] != 0.0 || v[2] != 0.0 || v[5] != 0.0)
+            fail(ATM_ERR_INVALID_ARGUMENT, "box vectors must be in reduced form (a = (ax, 0, 0), b = (bx, by, 0))");
+        if (!(v[0] > 0.0 && v[4] > 0.0 && v[8] > 0.0))
+            fail(ATM_ERR_INVALID_ARGUMENT, "box vectors must have positive diagonal components");
+        if (std::fabs(v[3]) > 0.5 * v[0] || std::fabs(v[6]) > 0.5 * v[0] || std::fabs(v[7]) > 0.5 * v[4])
+            fail(ATM_ERR_INVALID_ARGUMENT,
+                 "box vectors must be in reduced form (ax >= 2 |bx|, ax >= 2 |cx|, by >= 2 |cy|)");
+        atm_system_desc d = C.sys[sys];
+        d.box[0] = v[0]; d.box[1] = v[4]; d.box[2] = v[8];
+        validate_desc(d, sys);
+        HIPCHK(hipSetDevice(C.device));
+        HIPCHK(hipStreamSynchronize(C.stream));
+        C.sys[sys] = d;
+        C.tilt[sys] = {v[3], v[6], v[7]};
         layout(C);
         alloc_layout(C);
         C.evaluated = false;

@@ -72,7 +72,10 @@ struct SetDesc {
     int ligOff;              // their first entry in the batch's displaced-atom list
     int lxCount;             // exception terms with a displaced atom
     int lxOff;               // their first entry in the batch's list of such terms
-    int pad_;
+    int tri;                 // triclinic box: vectors a = (L0, 0, 0), b = (tilt0, L1, 0),
+    float tilt[3];           // c = (tilt1, tilt2, L2) (reduced form); every column is a
+                             // candidate (fullX, fullY) and pairs take the image of
+                             // OpenMM's triclinic rule (c, then b, then a)
 };
 
 struct SetRuntime {

@@ -71,6 +71,7 @@ struct DevPtrs {
     // launchers; the set's range at groupOff); null: launch order
     int* lptItems;
     int lptMode;           // kOrder*: how the tile kernel's launch ranks map to the groups
+    int anyTri = 0;        // some system has a triclinic box (SetDesc::tri)
     // backward
     double* gq;            // [2 S sets x slots] gradient accumulators in slot order
     double* gsig;
@@ -93,9 +94,9 @@ hipError_t launch_alist(const DevPtrs& P, int setBegin, int nsets, int maxGroups
 // 6: filtered atom queues (one wave per i-cluster, FP64 pairs only; FP32 pairs run variant 0)
 constexpr int kPairKernelFiltered = 6;
 constexpr int kPairKernelCount = 7;
-// A batch with exceptions runs the tile kernel whatever the variant asked for:
-// only it applies the exclusion masks.
-inline int effective_pair_variant(const DevPtrs& P, int variant) { return P.xmask ? 0 : variant; }
+// A batch with exceptions or a triclinic box runs the tile kernel whatever the
+// variant asked for: only it applies the exclusion masks and the triclinic images.
+inline int effective_pair_variant(const DevPtrs& P, int variant) { return (P.xmask || P.anyTri) ? 0 : variant; }
 // split (1-8): the tile kernel divides each group's list between that many waves.
 hipError_t launch_pairs(const DevPtrs& P, int setBegin, int nsets, int maxGroups, int waves, bool forces, bool fp64,
                         int variant, int split, hipStream_t st);

@@ -49,6 +49,81 @@ __device__ __forceinline__ float min_image(float d, float L, float invL) {
     return fmaf(-L, rintf(d * invL), d);
 }
 
+// Triclinic box in OpenMM's reduced form (a = (L0, 0, 0), b = (tilt0, L1, 0),
+// c = (tilt1, tilt2, L2)): wrap a position into the brick [0, L0) x [0, L1) x
+// [0, L2), which tiles space under the lattice -- by c to 0 <= z < L2, then by b
+// to 0 <= y < L1, then by a.  Each step is exact on the 2^-20 grid below 16 nm.
+__device__ __forceinline__ void wrap_tri(const SetDesc& D, float4& p) {
+    float n = floorf(p.z * D.invL[2]);
+    const float z = fmaf(-D.L[2], n, p.z);
+    n += z < 0.f ? -1.f : (z >= D.L[2] ? 1.f : 0.f);
+    p.z = fmaf(-D.L[2], n, p.z);
+    p.y = fmaf(-D.tilt[2], n, p.y);
+    p.x = fmaf(-D.tilt[1], n, p.x);
+    n = floorf(p.y * D.invL[1]);
+    const float y = fmaf(-D.L[1], n, p.y);
+    n += y < 0.f ? -1.f : (y >= D.L[1] ? 1.f : 0.f);
+    p.y = fmaf(-D.L[1], n, p.y);
+    p.x = fmaf(-D.tilt[0], n, p.x);
+    p.x = wrap_coord(p.x, D.L[0], D.invL[0]);
+}
+
+// The x and y parts of OpenMM's triclinic image of a pair difference (shift by c
+// to the nearest z, then by b to the nearest y, then by a to the nearest x); the
+// caller takes dz = min_image(dz, L2), which applies the same c shift to z.
+__device__ __forceinline__ void tri_image_xy(const SetDesc& D, float& dx, float& dy, float dz) {
+    const float nc = rintf(dz * D.invL[2]);
+    dy = fmaf(-D.tilt[2], nc, dy);
+    dx = fmaf(-D.tilt[1], nc, dx);
+    const float nb = rintf(dy * D.invL[1]);
+    dy = fmaf(-D.L[1], nb, dy);
+    dx = fmaf(-D.tilt[0], nb, dx);
+    dx = min_image(dx, D.L[0], D.invL[0]);
+}
+
+// The image a pair kernel takes for a difference vector: explicit column shifts
+// (shiftOK) leave x and y as they are; otherwise the orthorhombic minimum image or
+// the triclinic rule; z always to its nearest image.
+__device__ __forceinline__ void pair_image(const SetDesc& D, float& dx, float& dy, float& dz) {
+    if (!D.shiftOK) {
+        if (D.tri) {
+            tri_image_xy(D, dx, dy, dz);
+        } else {
+            dx = min_image(dx, D.L[0], D.invL[0]);
+            dy = min_image(dy, D.L[1], D.invL[1]);
+        }
+    }
+    dz = min_image(dz, D.L[2], D.invL[2]);
+}
+
+// Two boxes with centre difference d and summed half-extents h hold points closer
+// than sqrt(r2) under some lattice image n_a a + n_b b + n_c c of a triclinic box
+// (every image whose shifted box can reach is tested, so the pair kernel's image
+// of any pair inside the cutoff is among them).
+__device__ __forceinline__ bool tri_boxes_near(const SetDesc& D, float dx, float dy, float dz, float hx, float hy,
+                                               float hz, float r, float r2) {
+    const int c0 = (int)floorf((dz - hz - r) * D.invL[2]), c1 = (int)ceilf((dz + hz + r) * D.invL[2]);
+    for (int nc = c0; nc <= c1; ++nc) {
+        const float az = fmaxf(0.f, fabsf(fmaf(-D.L[2], (float)nc, dz)) - hz);
+        const float remz = r2 - az * az;
+        if (remz <= 0.f) continue;
+        const float dy1 = fmaf(-D.tilt[2], (float)nc, dy), dx1 = fmaf(-D.tilt[1], (float)nc, dx);
+        const int b0 = (int)floorf((dy1 - hy - r) * D.invL[1]), b1 = (int)ceilf((dy1 + hy + r) * D.invL[1]);
+        for (int nb = b0; nb <= b1; ++nb) {
+            const float ay = fmaxf(0.f, fabsf(fmaf(-D.L[1], (float)nb, dy1)) - hy);
+            const float remy = remz - ay * ay;
+            if (remy <= 0.f) continue;
+            const float dx2 = fmaf(-D.tilt[0], (float)nb, dx1);
+            const int a0 = (int)floorf((dx2 - hx - r) * D.invL[0]), a1 = (int)ceilf((dx2 + hx + r) * D.invL[0]);
+            for (int na = a0; na <= a1; ++na) {
+                const float ax = fmaxf(0.f, fabsf(fmaf(-D.L[0], (float)na, dx2)) - hx);
+                if (ax * ax < remy) return true;
+            }
+        }
+    }
+    return false;
+}
+
 // float -> 64-bit fixed point (round to nearest).  |f * scale| < 2^51 takes the
 // magic-number path (one FP64 FMA + a 64-bit subtract); larger values use the
 // general conversion.
@@ -279,9 +354,13 @@ __global__ void k_bin(DevPtrs P, int setBegin) {
         float4 d = P.displ[D.atomOff + i];
         p.x += d.x; p.y += d.y; p.z += d.z;  // exact on the 2^-20 grid
     }
-    p.x = wrap_coord(p.x, D.L[0], D.invL[0]);
-    p.y = wrap_coord(p.y, D.L[1], D.invL[1]);
-    p.z = wrap_coord(p.z, D.L[2], D.invL[2]);
+    if (D.tri) {
+        wrap_tri(D, p);
+    } else {
+        p.x = wrap_coord(p.x, D.L[0], D.invL[0]);
+        p.y = wrap_coord(p.y, D.L[1], D.invL[1]);
+        p.z = wrap_coord(p.z, D.L[2], D.invL[2]);
+    }
     // clamp in float before the conversion: every index stays inside the grid
     const int cx = (int)fminf(fmaxf(p.x * D.invCe[0], 0.f), (float)(D.nx - 1));
     const int cy = (int)fminf(fmaxf(p.y * D.invCe[1], 0.f), (float)(D.ny - 1));
@@ -564,7 +643,7 @@ __global__ void __launch_bounds__(256) k_gather_bbox(DevPtrs P, int setBegin) {
 // Candidate columns (up to 64 per pass, one per lane) are enumerated first; a
 // wave prefix sum then flattens their (column, cluster) candidates so that
 // all 64 lanes test clusters, whatever the column occupancy.
-template <int W>
+template <int W, bool TRI>
 __global__ void __launch_bounds__(64 * W) k_pairlist(DevPtrs P, int setBegin) {
     const int c = setBegin + (int)blockIdx.y * P.setStride;
     const SetDesc D = P.desc[c];
@@ -683,17 +762,27 @@ __global__ void __launch_bounds__(64 * W) k_pairlist(DevPtrs P, int setBegin) {
                 const float sx = D.shiftOK ? (sxc == 1 ? D.L[0] : (sxc == 2 ? -D.L[0] : 0.f)) : 0.f;
                 const float sy = D.shiftOK ? (syc == 1 ? D.L[1] : (syc == 2 ? -D.L[1] : 0.f)) : 0.f;
                 const float4 jc = P.bbc[D.clusterOff + J], jh = P.bbh[D.clusterOff + J];
-#pragma unroll
-                for (int q = 0; q < kGroupClusters; ++q) {
-                    if (q < ncl && g0 + q <= J) {
+                if (TRI && D.tri) {
+                    // triclinic: every lattice image the boxes can reach (one loop copy, not unrolled)
+                    for (int q = 0; q < ncl && g0 + q <= J; ++q) {
                         const float4 ic = sIc[wave][q], ih = sIh[wave][q];
-                        float dx = jc.x + sx - ic.x, dy = jc.y + sy - ic.y, dz = jc.z - ic.z;
-                        if (!D.shiftOK) { dx = min_image(dx, D.L[0], D.invL[0]); dy = min_image(dy, D.L[1], D.invL[1]); }
-                        dz = min_image(dz, D.L[2], D.invL[2]);
-                        const float ax = fmaxf(0.f, fabsf(dx) - ih.x - jh.x);
-                        const float ay = fmaxf(0.f, fabsf(dy) - ih.y - jh.y);
-                        const float az = fmaxf(0.f, fabsf(dz) - ih.z - jh.z);
-                        if (ax * ax + ay * ay + az * az < D.rcList2) mask |= 1u << q;
+                        if (tri_boxes_near(D, jc.x - ic.x, jc.y - ic.y, jc.z - ic.z, ih.x + jh.x, ih.y + jh.y,
+                                           ih.z + jh.z, rcl, D.rcList2))
+                            mask |= 1u << q;
+                    }
+                } else {
+#pragma unroll
+                    for (int q = 0; q < kGroupClusters; ++q) {
+                        if (q < ncl && g0 + q <= J) {
+                            const float4 ic = sIc[wave][q], ih = sIh[wave][q];
+                            float dx = jc.x + sx - ic.x, dy = jc.y + sy - ic.y, dz = jc.z - ic.z;
+                            if (!D.shiftOK) { dx = min_image(dx, D.L[0], D.invL[0]); dy = min_image(dy, D.L[1], D.invL[1]); }
+                            dz = min_image(dz, D.L[2], D.invL[2]);
+                            const float ax = fmaxf(0.f, fabsf(dx) - ih.x - jh.x);
+                            const float ay = fmaxf(0.f, fabsf(dy) - ih.y - jh.y);
+                            const float az = fmaxf(0.f, fabsf(dz) - ih.z - jh.z);
+                            if (ax * ax + ay * ay + az * az < D.rcList2) mask |= 1u << q;
+                        }
                     }
                 }
             }
@@ -874,9 +963,11 @@ __device__ __forceinline__ void flush_component_u(const DevPtrs& P, int c, float
 // With wset null (an evaluation that also differentiates) the weights are 1,
 // and the pair energies and counts are recorded as by the forward modes.
 //
-// EXC: the batch has exceptions; every in-cutoff pair is tested against the
-// i-atom's near mask (and, for far partners, its list) and excepted pairs are
-// dropped before the pair function.
+// EXC: the batch has exceptions or a triclinic box.  With exceptions (P.xmask)
+// every in-cutoff pair is tested against the i-atom's near mask (and, for far
+// partners, its list) and excepted pairs are dropped before the pair function;
+// sets with a triclinic box take OpenMM's triclinic image (the instantiations
+// without EXC carry neither).
 //
 // split: each group's list is divided into `split` consecutive parts, one per
 // wave (blockIdx.x = group block * split + part): shorter waves, so the launch
@@ -945,7 +1036,7 @@ __global__ void __launch_bounds__(64 * W, 6) k_pairs(DevPtrs P, int setBegin, co
             if (F64) qd = P.sprm64[s]; else q = P.sprm[s];
             ai = P.sidx[s];
             v = ai >= 0;
-            if (EXC && v) xm = P.xmask[D.atomOff + ai];
+            if (EXC && v && P.xmask) xm = P.xmask[D.atomOff + ai];  // no mask: no pair is excepted
         }
         if (EXC) {
             sAi[wave][lane] = ai;
@@ -1069,7 +1160,14 @@ __global__ void __launch_bounds__(64 * W, 6) k_pairs(DevPtrs P, int setBegin, co
             if (t + 1 < kGroupClusters) piN = sPos[wave][(t + 1) * kClusterSize + ii];
             if (imask & (1u << t)) {
                 float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-                if (!shiftOK) { dx = min_image(dx, Lx, invLx); dy = min_image(dy, Ly, invLy); }
+                if (!shiftOK) {
+                    if (EXC && D.tri) {
+                        tri_image_xy(D, dx, dy, dz);
+                    } else {
+                        dx = min_image(dx, Lx, invLx);
+                        dy = min_image(dy, Ly, invLy);
+                    }
+                }
                 dz = min_image(dz, Lz, invLz);
                 float fs;
                 if (F64) {
@@ -1683,8 +1781,7 @@ __global__ void __launch_bounds__(64) k_pairs_dump(DevPtrs P, int c, int2* out, 
             const float4 pi = P.spos[si];
             const int iidx = P.sidx[si];
             float dx = pi.x - ja.p.x, dy = pi.y - ja.p.y, dz = pi.z - ja.p.z;
-            if (!D.shiftOK) { dx = min_image(dx, D.L[0], D.invL[0]); dy = min_image(dy, D.L[1], D.invL[1]); }
-            dz = min_image(dz, D.L[2], D.invL[2]);
+            pair_image(D, dx, dy, dz);
             const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
             bool in = in_cutoff(D, dx, dy, dz, r2) && ja.valid && iidx >= 0;
             if (g0 + t == J) in = in && (ii < jj);
@@ -2434,11 +2531,14 @@ hipError_t launch_build(const DevPtrs& P, int setBegin, int nsets, int maxAtoms,
 }
 
 hipError_t launch_pairlist(const DevPtrs& P, int setBegin, int nsets, int maxGroups, int listWaves, hipStream_t st) {
-    switch (listWaves) {
-        case 1: k_pairlist<1><<<dim3(cdiv(maxGroups, 1), nsets), 64, 0, st>>>(P, setBegin); break;
-        case 2: k_pairlist<2><<<dim3(cdiv(maxGroups, 2), nsets), 128, 0, st>>>(P, setBegin); break;
-        default: k_pairlist<4><<<dim3(cdiv(maxGroups, 4), nsets), 256, 0, st>>>(P, setBegin); break;
+#define ATM_PAIRLIST(TRI)                                                                             \
+    switch (listWaves) {                                                                              \
+        case 1: k_pairlist<1, TRI><<<dim3(cdiv(maxGroups, 1), nsets), 64, 0, st>>>(P, setBegin); break;  \
+        case 2: k_pairlist<2, TRI><<<dim3(cdiv(maxGroups, 2), nsets), 128, 0, st>>>(P, setBegin); break; \
+        default: k_pairlist<4, TRI><<<dim3(cdiv(maxGroups, 4), nsets), 256, 0, st>>>(P, setBegin); break; \
     }
+    if (P.anyTri) { ATM_PAIRLIST(true) } else { ATM_PAIRLIST(false) }
+#undef ATM_PAIRLIST
     return hipGetLastError();
 }
 
@@ -2503,7 +2603,7 @@ hipError_t launch_pairs(const DevPtrs& P, int setBegin, int nsets, int maxGroups
     else k_pairs<WV, false, F64, false, X><<<dim3(cdiv(maxGroups, WV) * split, nsets), 64 * WV, 0, st>>>(        \
         Pk, setBegin, nullptr, split);
 #define ATM_PAIRS(WV, F64)                                \
-    if (P.xmask) { ATM_PAIRS_X(WV, F64, true) } else { ATM_PAIRS_X(WV, F64, false) }
+    if (P.xmask || P.anyTri) { ATM_PAIRS_X(WV, F64, true) } else { ATM_PAIRS_X(WV, F64, false) }
 #define ATM_PAIRS_W(F64)               \
     switch (waves) {                   \
         case 1: ATM_PAIRS(1, F64) break; \
@@ -2533,7 +2633,7 @@ hipError_t launch_backward_pairs(const DevPtrs& P, int setBegin, int nsets, int 
     if (split < 1 || split > 8) return hipErrorInvalidValue;
     const DevPtrs& Pk = P;
     if (P.lptItems && P.lptMode != kOrderLaunch) k_order<<<nsets, 512, 0, st>>>(P, setBegin);
-    if (P.xmask) k_pairs<1, false, true, true, true><<<dim3(maxGroups * split, nsets), 64, 0, st>>>(Pk, setBegin, wset, split);
+    if (P.xmask || P.anyTri) k_pairs<1, false, true, true, true><<<dim3(maxGroups * split, nsets), 64, 0, st>>>(Pk, setBegin, wset, split);
     else k_pairs<1, false, true, true, false><<<dim3(maxGroups * split, nsets), 64, 0, st>>>(Pk, setBegin, wset, split);
     return hipGetLastError();
 }

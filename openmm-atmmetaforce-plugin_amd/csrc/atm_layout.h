@@ -24,7 +24,9 @@ void validate_desc(const atm_system_desc& d, int s);
 
 // Geometry and capacities of coordinate set `state` of system `s`; the offsets
 // into the batch arrays are left zero (the caller places the set).
-SetDesc make_set(const atm_system_desc& d, const Tuning& tu, int s, int state);
+// tilt: (b_x, c_x, c_y) of a triclinic box in reduced form, or null / all zero for an
+// orthorhombic one.
+SetDesc make_set(const atm_system_desc& d, const Tuning& tu, int s, int state, const double* tilt = nullptr);
 
 // Largest partial force (kJ/mol/nm) that enters a set's 64-bit fixed-point sums
 // when each slot may also receive extraTerms exception terms.

@@ -110,10 +110,19 @@ void System::getDefaultPeriodicBoxVectors(Vec3& a, Vec3& b, Vec3& c) const {
     c = box[2];
 }
 
-void System::setDefaultPeriodicBoxVectors(const Vec3& a, const Vec3& b, const Vec3& c) {
-    if (a[1] != 0.0 || a[2] != 0.0 || b[0] != 0.0 || b[2] != 0.0 || c[0] != 0.0 || c[1] != 0.0)
-        throw OpenMMException("Only rectangular periodic boxes are supported");
+// OpenMM's reduced form of periodic box vectors: a along x, b in the x-y plane,
+// and each vector's tilt at most half the edge of the ones before it (triclinic
+// boxes included).
+static void checkReducedBox(const Vec3& a, const Vec3& b, const Vec3& c) {
+    if (a[1] != 0.0 || a[2] != 0.0) throw OpenMMException("First periodic box vector must be parallel to x.");
+    if (b[2] != 0.0) throw OpenMMException("Second periodic box vector must be in the x-y plane.");
     if (!(a[0] > 0 && b[1] > 0 && c[2] > 0)) throw OpenMMException("Periodic box vectors must be positive");
+    if (a[0] < 2 * std::fabs(b[0]) || a[0] < 2 * std::fabs(c[0]) || b[1] < 2 * std::fabs(c[1]))
+        throw OpenMMException("Periodic box vectors must be in reduced form.");
+}
+
+void System::setDefaultPeriodicBoxVectors(const Vec3& a, const Vec3& b, const Vec3& c) {
+    checkReducedBox(a, b, c);
     box[0] = a;
     box[1] = b;
     box[2] = c;
@@ -533,8 +542,7 @@ void ContextImpl::setParameter(const std::string& name, double value) {
 }
 
 void ContextImpl::setPeriodicBoxVectors(const Vec3& a, const Vec3& b, const Vec3& c) {
-    if (a[1] != 0.0 || a[2] != 0.0 || b[0] != 0.0 || b[2] != 0.0 || c[0] != 0.0 || c[1] != 0.0)
-        throw OpenMMException("Only rectangular periodic boxes are supported");
+    checkReducedBox(a, b, c);
     box[0] = a;
     box[1] = b;
     box[2] = c;

@@ -46,6 +46,7 @@ private:
     void* pme = nullptr;
     int n = 0;
     double box[3] = {0, 0, 0};
+    double tilt[3] = {0, 0, 0};        // (b_x, c_x, c_y) of a triclinic box
     double cutoff = 0, rfDielectric = 0;
     double dispersionCoefficient = 0;  // E_disp = coefficient / volume
     bool haveExceptions = false;       // exceptions were uploaded (atm_set_exceptions)
@@ -143,6 +144,7 @@ public:
 private:
     OpenMM::ContextImpl& context;
     void* set = nullptr;
+    bool periodic = false;  // the restraint distances take the (orthorhombic) minimum image
 };
 
 }  // namespace ATMMetaForcePlugin

@@ -112,11 +112,18 @@ public:
     }
 };
 
-// Box edges of an orthorhombic context.
+// Box edges of a context (the diagonal of its box vectors).
 void boxEdges(ContextImpl& ctx, double L[3]) {
     Vec3 a, b, c;
     ctx.getPeriodicBoxVectors(a, b, c);
     L[0] = a[0]; L[1] = b[1]; L[2] = c[2];
+}
+
+bool triclinic(const Vec3& b, const Vec3& c) { return b[0] != 0.0 || c[0] != 0.0 || c[1] != 0.0; }
+
+void requireOrthorhombic(const Vec3& b, const Vec3& c, const char* what) {
+    if (triclinic(b, c))
+        throw OpenMMException(std::string(what) + " supports orthorhombic periodic boxes only on the MI355X platform");
 }
 
 }  // namespace
@@ -205,6 +212,11 @@ void MI355XSystemContext::create(int device, const System& system, const Nonbond
     rfDielectric = d.rf_dielectric;
     for (int k = 0; k < 3; ++k) box[k] = d.box[k];
     check(atm_create(&d, 1, device, &ctx), "atm_create");
+    if (triclinic(b, c)) {
+        const double v[9] = {a[0], a[1], a[2], b[0], b[1], b[2], c[0], c[1], c[2]};
+        check(atm_set_box_vectors(ctx, 0, v), "atm_set_box_vectors");
+        tilt[0] = b[0]; tilt[1] = c[0]; tilt[2] = c[1];
+    }
     setParticles(nb, displacements);
     std::vector<double> zeros(3 * (size_t)n, 0.0);
     check(atm_set_positions(ctx, 0, zeros.data()), "atm_set_positions");
@@ -219,6 +231,7 @@ void MI355XSystemContext::createPME(int device, const System& system, const Nonb
         throw OpenMMException("NonbondedForce must have exactly as many particles as the System it belongs to.");
     Vec3 a, b, c;
     system.getDefaultPeriodicBoxVectors(a, b, c);
+    requireOrthorhombic(b, c, "NonbondedForce with PME");
     atm_pme_desc d{};
     d.num_atoms = n;
     double alpha;
@@ -255,6 +268,7 @@ void MI355XSystemContext::setPMEParticles(const NonbondedForce& nb, const std::v
 void MI355XSystemContext::evaluatePME(ContextImpl& context, const atm_params* params, bool includeForces, atm_result* r) {
     Vec3 a, b, c;
     context.getPeriodicBoxVectors(a, b, c);
+    requireOrthorhombic(b, c, "NonbondedForce with PME");
     const double L[3] = {a[0], b[1], c[2]};
     if (params) check(atm_pme_set_params(pme, params), "atm_pme_set_params");
     double* f = includeForces ? reinterpret_cast<double*>(context.forcesRef().data()) : nullptr;
@@ -324,12 +338,17 @@ double MI355XSystemContext::dispersionEnergy(const Vec3 b[3]) const {
 void MI355XSystemContext::syncState(ContextImpl& context) {
     Vec3 a, b, c;
     context.getPeriodicBoxVectors(a, b, c);
-    if (a[0] != box[0] || b[1] != box[1] || c[2] != box[2]) {
-        const double nb[3] = {a[0], b[1], c[2]};
-        check(atm_set_box(ctx, 0, nb), "atm_set_box");
-        box[0] = nb[0];
-        box[1] = nb[1];
-        box[2] = nb[2];
+    if (a[0] != box[0] || b[1] != box[1] || c[2] != box[2] || b[0] != tilt[0] || c[0] != tilt[1] ||
+        c[1] != tilt[2]) {
+        if (triclinic(b, c)) {
+            const double v[9] = {a[0], a[1], a[2], b[0], b[1], b[2], c[0], c[1], c[2]};
+            check(atm_set_box_vectors(ctx, 0, v), "atm_set_box_vectors");
+        } else {
+            const double nb[3] = {a[0], b[1], c[2]};
+            check(atm_set_box(ctx, 0, nb), "atm_set_box");
+        }
+        box[0] = a[0]; box[1] = b[1]; box[2] = c[2];
+        tilt[0] = b[0]; tilt[1] = c[0]; tilt[2] = c[1];
     }
     check(atm_set_positions(ctx, 0, reinterpret_cast<const double*>(context.positionsRef().data())), "atm_set_positions");
 }
@@ -669,14 +688,21 @@ MI355XCalcPositionRestraintForceKernel::~MI355XCalcPositionRestraintForceKernel(
 void MI355XCalcPositionRestraintForceKernel::initialize(const System& system, const PositionRestraintForce& force) {
     check(atm_restraints_create(MI355XPlatform::data(context).device, &set), "atm_restraints_create");
     setPosition(set, system, force);
+    periodic = force.getPeriodic();
 }
 
 double MI355XCalcPositionRestraintForceKernel::execute(ContextImpl& ctx, bool includeForces, bool includeEnergy) {
+    if (periodic) {
+        Vec3 a, b, c;
+        ctx.getPeriodicBoxVectors(a, b, c);
+        requireOrthorhombic(b, c, "A periodic PositionRestraintForce");
+    }
     return evalRestraints(set, ctx, includeForces, includeEnergy);
 }
 
 void MI355XCalcPositionRestraintForceKernel::copyParametersToContext(ContextImpl& ctx, const PositionRestraintForce& force) {
     setPosition(set, ctx.getSystem(), force);
+    periodic = force.getPeriodic();
 }
 
 // ------------------------------------------------------------------ plugin entry points

@@ -26,6 +26,9 @@
  *                        at the plain (non-periodic) distance, with no cutoff (OpenMM's documented
  *                        exception terms; q' = e' = 0 is an exclusion).  The build's convention for
  *                        the reaction-field method, the same as its PME path's.
+ *   triclinic boxes      oracle_pairs_tri, oracle_atm_eval_tri: pairs take OpenMM's triclinic image
+ *                        rule (tri_image; box vectors copied by copyState,
+ *                        ReferenceATMMetaForceKernels.cpp:126-129)
  *   oracle_atm_grad      analytic dW/d(q, sigma, epsilon) and dW/d(lambda1, lambda2, alpha, u0, w0)
  *                        (no reference counterpart: the reference has no gradients).
  *
@@ -103,6 +106,24 @@ void oracle_blend(int n, double sp, double direction, const double* F1, const do
 /* ------------------------------------------------------------------ pairs */
 
 static inline double mimg(double d, double L) { return d - L * floor(d / L + 0.5); }
+
+/* Image of a difference vector in a triclinic box in OpenMM's reduced form,
+ * a = (box[0], 0, 0), b = (tilt[0], box[1], 0), c = (tilt[1], tilt[2], box[2]):
+ * OpenMM's rule for triclinic boxes (ReferenceForce::getDeltaRPeriodic in OpenMM, external
+ * to the reference tree, reached through the inner contexts' NonbondedForce at
+ * ATMMetaForceImpl.cpp:113,116 with the box vectors copyState copies,
+ * ReferenceATMMetaForceKernels.cpp:126-129): subtract c floor(dz/cz + 1/2), then
+ * b floor(dy/by + 1/2), then a floor(dx/ax + 1/2). */
+static inline void tri_image(const double* box, const double* tilt, double* dx, double* dy, double* dz) {
+    double s = floor(*dz / box[2] + 0.5);
+    *dx -= tilt[1] * s;
+    *dy -= tilt[2] * s;
+    *dz -= box[2] * s;
+    s = floor(*dy / box[1] + 0.5);
+    *dx -= tilt[0] * s;
+    *dy -= box[1] * s;
+    *dx -= box[0] * floor(*dx / box[0] + 0.5);
+}
 
 typedef struct {
     double s, c;
@@ -182,11 +203,25 @@ static int excl_has(const excl_set* e, int i, int j) {
 }
 
 /* Visit every pair i<j with minimum-image r^2 < rc^2 that is not in `ex` exactly once, in a
- * fixed order.  brute != 0 forces the O(N^2) loop. */
-static long long visit_pairs(int n, const double* x, const double* box, double rc, int brute, const excl_set* ex,
-                             pair_fn fn, void* ctx) {
+ * fixed order.  brute != 0 forces the O(N^2) loop.  tilt (b_x, c_x, c_y; NULL: orthorhombic)
+ * makes the box triclinic: pairs take tri_image's image, in the O(N^2) loop. */
+static long long visit_pairs_tilt(int n, const double* x, const double* box, const double* tilt, double rc,
+                                  int brute, const excl_set* ex, pair_fn fn, void* ctx) {
     const double rc2 = rc * rc;
     long long cnt = 0;
+    if (tilt) {
+        for (int i = 0; i < n; ++i)
+            for (int j = i + 1; j < n; ++j) {
+                double dx = x[3 * i] - x[3 * j], dy = x[3 * i + 1] - x[3 * j + 1], dz = x[3 * i + 2] - x[3 * j + 2];
+                tri_image(box, tilt, &dx, &dy, &dz);
+                double r2 = dx * dx + dy * dy + dz * dz;
+                if (r2 < rc2 && !excl_has(ex, i, j)) {
+                    ++cnt;
+                    if (fn) fn(ctx, i, j, dx, dy, dz, r2);
+                }
+            }
+        return cnt;
+    }
     grid g;
     if (!brute && grid_build(&g, n, x, box, rc)) {
         /* for each atom i, scan the 27 neighbouring cells for j > i */
@@ -235,6 +270,11 @@ static long long visit_pairs(int n, const double* x, const double* box, double r
     return cnt;
 }
 
+static long long visit_pairs(int n, const double* x, const double* box, double rc, int brute, const excl_set* ex,
+                             pair_fn fn, void* ctx) {
+    return visit_pairs_tilt(n, x, box, NULL, rc, brute, ex, fn, ctx);
+}
+
 typedef struct {
     int* out;
     long long cap, k;
@@ -264,6 +304,17 @@ long long oracle_pairs_ex(int n, const double* x, const double* box, double rc, 
 
 long long oracle_pairs(int n, const double* x, const double* box, double rc, int brute, int* pairs, long long cap) {
     return oracle_pairs_ex(n, x, box, rc, brute, 0, NULL, pairs, cap);
+}
+
+/* oracle_pairs_ex in a triclinic box: edges box[3] = (a_x, b_y, c_z), tilt[3] = (b_x, c_x, c_y). */
+long long oracle_pairs_tri(int n, const double* x, const double* box, const double* tilt, double rc, int nexc,
+                           const int* exc, int* pairs, long long cap) {
+    pairs_ctx c = {pairs, pairs ? cap : 0, 0};
+    excl_set e;
+    excl_build(&e, nexc, exc);
+    long long r = visit_pairs_tilt(n, x, box, tilt, rc, 1, &e, pairs_cb, &c);
+    free(e.key);
+    return r;
 }
 
 typedef struct {
@@ -330,10 +381,10 @@ static void exception_terms(const double* x, int nexc, const int* exc, const dou
  * charge products, es: sigma, ee: epsilon).  parts (may be NULL, 4 doubles) receives
  * E_LJ and E_C of the cutoff pairs, the sum of |terms| over the cutoff pairs and the
  * exceptions, and the exception energy. */
-double oracle_nonbonded_ex(int n, const double* x, const double* q, const double* sig, const double* eps,
-                           const double* box, double rc, double eps_rf, int brute, int nexc, const int* exc,
-                           const double* qq, const double* es, const double* ee, double* F, long long* npairs,
-                           double* parts) {
+static double nonbonded_tilt(int n, const double* x, const double* q, const double* sig, const double* eps,
+                             const double* box, const double* tilt, double rc, double eps_rf, int brute, int nexc,
+                             const int* exc, const double* qq, const double* es, const double* ee, double* F,
+                             long long* npairs, double* parts) {
     nb_ctx c;
     memset(&c, 0, sizeof(c));
     c.q = q;
@@ -345,7 +396,7 @@ double oracle_nonbonded_ex(int n, const double* x, const double* q, const double
     if (F) memset(F, 0, sizeof(double) * 3 * n);
     excl_set e;
     excl_build(&e, nexc, exc);
-    long long np = visit_pairs(n, x, box, rc, brute, &e, nb_cb, &c);
+    long long np = visit_pairs_tilt(n, x, box, tilt, rc, brute, &e, nb_cb, &c);
     free(e.key);
     nsum Ex = {0.0, 0.0};
     exception_terms(x, nexc, exc, qq, es, ee, F, &Ex, &c.Eabs);
@@ -361,6 +412,13 @@ double oracle_nonbonded_ex(int n, const double* x, const double* q, const double
     return c.E.s + c.E.c;
 }
 
+double oracle_nonbonded_ex(int n, const double* x, const double* q, const double* sig, const double* eps,
+                           const double* box, double rc, double eps_rf, int brute, int nexc, const int* exc,
+                           const double* qq, const double* es, const double* ee, double* F, long long* npairs,
+                           double* parts) {
+    return nonbonded_tilt(n, x, q, sig, eps, box, NULL, rc, eps_rf, brute, nexc, exc, qq, es, ee, F, npairs, parts);
+}
+
 double oracle_nonbonded(int n, const double* x, const double* q, const double* sig, const double* eps,
                         const double* box, double rc, double eps_rf, int brute, double* F, long long* npairs,
                         double* parts) {
@@ -373,10 +431,10 @@ double oracle_nonbonded(int n, const double* x, const double* q, const double* s
 
 /* One ATM evaluation: U1,F1 at x; U2,F2 at x + d (entry i displaces atom i);
  * combine; F (n*3, overwritten with the ATM force; may be NULL). */
-int oracle_atm_eval_ex(int n, const double* x, const double* d, const double* q, const double* sig,
-                       const double* eps, const double* box, double rc, double eps_rf, int nexc, const int* exc,
-                       const double* qq, const double* es, const double* ee, const oracle_params* p, int brute,
-                       double* F, oracle_result* r) {
+static int atm_eval_tilt(int n, const double* x, const double* d, const double* q, const double* sig,
+                         const double* eps, const double* box, const double* tilt, double rc, double eps_rf,
+                         int nexc, const int* exc, const double* qq, const double* es, const double* ee,
+                         const oracle_params* p, int brute, double* F, oracle_result* r) {
     double* x2 = (double*)malloc(sizeof(double) * 3 * n);
     double* F1 = F ? (double*)malloc(sizeof(double) * 3 * n) : NULL;
     double* F2 = F ? (double*)malloc(sizeof(double) * 3 * n) : NULL;
@@ -384,8 +442,8 @@ int oracle_atm_eval_ex(int n, const double* x, const double* d, const double* q,
     for (int i = 0; i < 3 * n; ++i) x2[i] = x[i] + d[i];
     long long n1 = 0, n2 = 0;
     double parts1[4], parts2[4];
-    double U1 = oracle_nonbonded_ex(n, x, q, sig, eps, box, rc, eps_rf, brute, nexc, exc, qq, es, ee, F1, &n1, parts1);
-    double U2 = oracle_nonbonded_ex(n, x2, q, sig, eps, box, rc, eps_rf, brute, nexc, exc, qq, es, ee, F2, &n2, parts2);
+    double U1 = nonbonded_tilt(n, x, q, sig, eps, box, tilt, rc, eps_rf, brute, nexc, exc, qq, es, ee, F1, &n1, parts1);
+    double U2 = nonbonded_tilt(n, x2, q, sig, eps, box, tilt, rc, eps_rf, brute, nexc, exc, qq, es, ee, F2, &n2, parts2);
     oracle_combine(p, U1, U2, r);
     r->npairs1 = n1;
     r->npairs2 = n2;
@@ -399,6 +457,21 @@ int oracle_atm_eval_ex(int n, const double* x, const double* d, const double* q,
     free(F1);
     free(F2);
     return 0;
+}
+
+int oracle_atm_eval_ex(int n, const double* x, const double* d, const double* q, const double* sig,
+                       const double* eps, const double* box, double rc, double eps_rf, int nexc, const int* exc,
+                       const double* qq, const double* es, const double* ee, const oracle_params* p, int brute,
+                       double* F, oracle_result* r) {
+    return atm_eval_tilt(n, x, d, q, sig, eps, box, NULL, rc, eps_rf, nexc, exc, qq, es, ee, p, brute, F, r);
+}
+
+/* oracle_atm_eval_ex in a triclinic box (tilt as for oracle_pairs_tri; O(N^2) pair loop). */
+int oracle_atm_eval_tri(int n, const double* x, const double* d, const double* q, const double* sig,
+                        const double* eps, const double* box, const double* tilt, double rc, double eps_rf, int nexc,
+                        const int* exc, const double* qq, const double* es, const double* ee, const oracle_params* p,
+                        double* F, oracle_result* r) {
+    return atm_eval_tilt(n, x, d, q, sig, eps, box, tilt, rc, eps_rf, nexc, exc, qq, es, ee, p, 1, F, r);
 }
 
 int oracle_atm_eval(int n, const double* x, const double* d, const double* q, const double* sig, const double* eps,

@@ -69,6 +69,13 @@ def lib():
         L.oracle_atm_eval_ex.argtypes = [ctypes.c_int, dp, dp, dp, dp, dp, dp, ctypes.c_double, ctypes.c_double,
                                          ctypes.c_int, ip, dp, dp, dp, ctypes.POINTER(Params), ctypes.c_int, dp,
                                          ctypes.POINTER(Result)]
+        L.oracle_pairs_tri.restype = ctypes.c_longlong
+        L.oracle_pairs_tri.argtypes = [ctypes.c_int, dp, dp, dp, ctypes.c_double, ctypes.c_int, ip, ip,
+                                       ctypes.c_longlong]
+        L.oracle_atm_eval_tri.restype = ctypes.c_int
+        L.oracle_atm_eval_tri.argtypes = [ctypes.c_int, dp, dp, dp, dp, dp, dp, dp, ctypes.c_double, ctypes.c_double,
+                                          ctypes.c_int, ip, dp, dp, dp, ctypes.POINTER(Params), dp,
+                                          ctypes.POINTER(Result)]
         L.oracle_atm_grad_ex.argtypes = [ctypes.c_int, dp, dp, dp, dp, dp, dp, ctypes.c_double, ctypes.c_double,
                                          ctypes.c_int, ip, ctypes.POINTER(Params), ctypes.POINTER(Result), dp, dp, dp,
                                          dp]
@@ -125,15 +132,30 @@ def _exc(ex):
     return ex.count, pr.ctypes.data_as(ip), pq, ps, pe, (pr, qq, sg, ep)
 
 
-def pairs(x, box, rc, brute=False, exceptions=None) -> np.ndarray:
-    """Sorted (i, j) in-cutoff pairs, i < j, that are not exceptions, shape (P, 2)."""
+def _tilt(tilt):
+    """(array, pointer) of a triclinic box's (b_x, c_x, c_y), or None for an orthorhombic box."""
+    if tilt is None or not np.any(np.asarray(tilt, dtype=np.float64) != 0.0):
+        return None
+    return _d(tilt)
+
+
+def pairs(x, box, rc, brute=False, exceptions=None, tilt=None) -> np.ndarray:
+    """Sorted (i, j) in-cutoff pairs, i < j, that are not exceptions, shape (P, 2).  tilt:
+    (b_x, c_x, c_y) of a triclinic box with edges box = (a_x, b_y, c_z)."""
     x, px = _d(x)
     box, pb = _d(box)
     n = x.shape[0]
     ne, pex, _, _, _, keep = _exc(exceptions)
-    count = lib().oracle_pairs_ex(n, px, pb, rc, int(brute), ne, pex, None, 0)
-    out = np.zeros((max(count, 1), 2), dtype=np.int32)
-    lib().oracle_pairs_ex(n, px, pb, rc, int(brute), ne, pex, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), count)
+    t = _tilt(tilt)
+    ip = ctypes.POINTER(ctypes.c_int)
+    if t is not None:
+        count = lib().oracle_pairs_tri(n, px, pb, t[1], rc, ne, pex, None, 0)
+        out = np.zeros((max(count, 1), 2), dtype=np.int32)
+        lib().oracle_pairs_tri(n, px, pb, t[1], rc, ne, pex, out.ctypes.data_as(ip), count)
+    else:
+        count = lib().oracle_pairs_ex(n, px, pb, rc, int(brute), ne, pex, None, 0)
+        out = np.zeros((max(count, 1), 2), dtype=np.int32)
+        lib().oracle_pairs_ex(n, px, pb, rc, int(brute), ne, pex, out.ctypes.data_as(ip), count)
     out = out[:count]
     order = np.lexsort((out[:, 1], out[:, 0]))
     return out[order]
@@ -170,10 +192,15 @@ def atm_eval(system, params=None, brute=False, forces=True):
     F = np.zeros((n, 3)) if forces else None
     r = Result()
     ne, pex, pqq, psg, pep, keep = _exc(getattr(system, "exceptions", None))
-    rc = lib().oracle_atm_eval_ex(n, px, pd, pq, ps, pe, pb, system.cutoff, system.rf_dielectric, ne, pex, pqq, psg,
-                                  pep, ctypes.byref(p), int(brute),
-                                  F.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) if forces else None,
-                                  ctypes.byref(r))
+    pF = F.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) if forces else None
+    t = _tilt(getattr(system, "tilt", None))
+    if t is not None:
+        # triclinic box (system.tilt = (b_x, c_x, c_y)): O(N^2) pair loop
+        rc = lib().oracle_atm_eval_tri(n, px, pd, pq, ps, pe, pb, t[1], system.cutoff, system.rf_dielectric, ne,
+                                       pex, pqq, psg, pep, ctypes.byref(p), pF, ctypes.byref(r))
+    else:
+        rc = lib().oracle_atm_eval_ex(n, px, pd, pq, ps, pe, pb, system.cutoff, system.rf_dielectric, ne, pex, pqq,
+                                      psg, pep, ctypes.byref(p), int(brute), pF, ctypes.byref(r))
     if rc != 0:
         raise MemoryError("oracle_atm_eval failed")
     return r.as_dict(), F

@@ -106,3 +106,19 @@ def test_plugin_entry_points_and_platform():
     assert p.getPropertyDefaultValue("Precision") == "mixed"
     with pytest.raises(Exception, match="no registered Platform"):
         mm.Platform.getPlatformByName("CUDA")
+
+
+def test_box_vectors_reduced_form_checks():
+    """Triclinic default box vectors are accepted in OpenMM's reduced form and
+    rejected otherwise, with OpenMM's messages."""
+    from atmmetaforce import openmm as mm
+    s = mm.System()
+    s.setDefaultPeriodicBoxVectors((3.0, 0, 0), (1.5, 3.0, 0), (-1.5, 1.5, 3.0))
+    a, b, c = s.getDefaultPeriodicBoxVectors()
+    assert tuple(b) == (1.5, 3.0, 0.0) and tuple(c) == (-1.5, 1.5, 3.0)
+    for bad, msg in [(((3.0, 0.1, 0), (0, 3, 0), (0, 0, 3)), "parallel to x"),
+                     (((3.0, 0, 0), (0, 3, 0.2), (0, 0, 3)), "x-y plane"),
+                     (((3.0, 0, 0), (1.6, 3, 0), (0, 0, 3)), "reduced form"),
+                     (((3.0, 0, 0), (0, 3, 0), (0, 1.6, 3)), "reduced form")]:
+        with pytest.raises(Exception, match=msg):
+            s.setDefaultPeriodicBoxVectors(*bad)

@@ -101,7 +101,8 @@ def check_system(batch, k, system, pairs=True, tag="", strict=True, forces=True,
     if pairs:
         for state, x in ((0, system.positions), (1, system.positions + system.displacements)):
             pg = batch.pairs(k, state)
-            po = oracle.pairs(x, system.box, system.cutoff, exceptions=getattr(system, "exceptions", None))
+            po = oracle.pairs(x, system.box, system.cutoff, exceptions=getattr(system, "exceptions", None),
+                              tilt=getattr(system, "tilt", None))
             if pg.shape != po.shape or not np.array_equal(pg, po):
                 report_pair_mismatch(batch, k, state, x, system, pg, po)
             assert pg.shape == po.shape, f"state {state}: {pg.shape[0]} pairs vs oracle {po.shape[0]}"

@@ -1,0 +1,128 @@
+"""Triclinic boxes on the MI355X path against the FP64 oracle (through the C ABI).
+
+The reference copies the full box vectors into both inner contexts
+(platforms/reference/src/ReferenceATMMetaForceKernels.cpp:126-129); a triclinic
+box in OpenMM's reduced form reaches the library through atm_set_box_vectors.
+Same bar as the orthorhombic parity tests: pair sets identical to the oracle's
+(oracle_pairs_tri, pinned in tests/test_oracle_triclinic.py), U1, U2, W within
+1e-6 relative and the FP64 bound of the default path, forces within 1e-5
+relative RMS.  Positions and tilts sit on the 2^-20 nm grid, where every step of
+the triclinic wrap and image is exact in FP32, so the pair sets are integer
+objects both paths must reproduce.
+"""
+import numpy as np
+import pytest
+
+from atmmetaforce import ATMBatch, workloads as W
+from atmmetaforce import _capi as C
+from oracle import oracle
+
+from test_gpu_parity import check_system
+
+pytestmark = pytest.mark.gpu
+
+
+def tri_config1():
+    return W.triclinic(W.config1(), sites=(5, -4, 7))
+
+
+@pytest.mark.parametrize("plain", [False, True])
+def test_triclinic_3000_atoms(gpu, plain):
+    s = tri_config1()
+    with ATMBatch([s]) as b:
+        b.compute(plain=plain)
+        check_system(b, 0, s, tag=f"tri-3000 plain={plain}")
+
+
+def test_triclinic_with_exceptions_and_fp32_pairs(gpu):
+    s = W.triclinic(W.with_molecules(W.config1(seed=3), size=4, crosslink_every=7), sites=(-6, 3, -5))
+    with ATMBatch([s]) as b:
+        b.compute()
+        check_system(b, 0, s, tag="tri-exc")
+        b.compute(fp32_pairs=True)
+        check_system(b, 0, s, pairs=False, strict=False, fp32=True, tag="tri-exc fp32")
+
+
+def test_triclinic_23k_in_a_mixed_batch(gpu):
+    """A 23k-atom triclinic system between orthorhombic ones, autotuned: the batch
+    runs the tile kernel and bins x + d in full; every system still matches."""
+    big = W.triclinic(W.config2(), sites=(9, -7, 12))
+    ortho = W.tiny_system(3000, 3.1, 0.9, 24, (1.5, 1.5, 1.5), 8)
+    small = W.triclinic(W.tiny_system(1500, 2.9, 0.9, 16, (0.9, 0.8, 0.7), 9), sites=(-2, 5, -3))
+    systems = [ortho, big, small]
+    with ATMBatch(systems) as b:
+        b.autotune(repeats=1)
+        b.compute()
+        counts = b.list_stats()["pairs"]
+        for k, s in enumerate(systems):
+            check_system(b, k, s, tag="tri-mixed", counts=counts)
+
+
+def test_lattice_translations_give_identical_results(gpu):
+    """Moving atoms by lattice vectors of the triclinic box (a, b, c and sums) leaves
+    the wrapped positions, and so every result, bit for bit the same."""
+    s = tri_config1()
+    H = W.box_vectors(s.box, s.tilt)
+    k = np.random.default_rng(21).integers(-1, 2, size=(s.num_atoms, 3))
+    moved = W.System(**{**s.__dict__, "positions": W.quantize(s.positions + k @ H)})
+    out = []
+    for sys_ in (s, moved):
+        with ATMBatch([sys_]) as b:
+            b.compute()
+            out.append((b.result(0), b.forces_fixed(0)))
+    assert out[0][0] == out[1][0]
+    assert np.array_equal(out[0][1], out[1][1])
+
+
+def test_box_vector_switches(gpu):
+    """set_box_vectors on a context built orthorhombic, back to orthorhombic with
+    set_box, and the reduced-form checks."""
+    s = tri_config1()
+    flat = W.System(**{**s.__dict__, "tilt": None})
+    with ATMBatch([flat]) as b:
+        b.set_box_vectors(0, W.box_vectors(s.box, s.tilt))
+        b.compute()
+        check_system(b, 0, s, tag="tri-switch")
+        b.set_box(0, flat.box)
+        b.compute()
+        check_system(b, 0, flat, tag="ortho-again")
+        bad = W.box_vectors(s.box, s.tilt)
+        bad[1, 0] = 0.6 * s.box[0]  # |b_x| > a_x / 2
+        with pytest.raises(C.ATMError):
+            b.set_box_vectors(0, bad)
+        bad = W.box_vectors(s.box, s.tilt)
+        bad[0, 1] = 0.1            # a not along x
+        with pytest.raises(C.ATMError):
+            b.set_box_vectors(0, bad)
+
+
+def test_openmm_api_triclinic_context(gpu):
+    """ATMMetaForce over a CutoffPeriodic NonbondedForce (exceptions included) in a
+    System with triclinic default box vectors, then Context.setPeriodicBoxVectors
+    to another triclinic box, then an orthorhombic one."""
+    from test_boundary_gpu import build, frms, rel
+    import atmmetaforce as atm
+    from atmmetaforce import openmm as mm
+
+    s = W.triclinic(W.with_molecules(W.config1(seed=5), size=3), sites=(4, 6, -3))
+    system, f = build(s)
+    H = W.box_vectors(s.box, s.tilt)
+    system.setDefaultPeriodicBoxVectors(tuple(H[0]), tuple(H[1]), tuple(H[2]))
+    ctx = mm.Context(system, mm.VerletIntegrator(0.001), mm.Platform.getPlatformByName("MI355X"), {})
+    ctx.setPositions(s.positions)
+    for case in ("default", "other tilt", "orthorhombic"):
+        if case == "other tilt":
+            s = W.triclinic(W.System(**{**s.__dict__, "tilt": None}), sites=(-7, 2, 5))
+        elif case == "orthorhombic":
+            s = W.System(**{**s.__dict__, "tilt": None})
+        H = W.box_vectors(s.box, s.tilt)
+        if case != "default":
+            ctx.setPeriodicBoxVectors(tuple(H[0]), tuple(H[1]), tuple(H[2]))
+        ref, Fref = oracle.atm_eval(s)
+        st = ctx.getState(getEnergy=True, getForces=True, groups={2})
+        print(case, st.getPotentialEnergy(), ref["W"], flush=True)
+        assert rel(st.getPotentialEnergy(), ref["W"]) < 1e-6
+        assert frms(st.getForces(asNumpy=True), Fref) < 1e-5
+        assert rel(f.getPerturbationEnergy(ctx), ref["usc"]) < 1e-6
+    with pytest.raises(Exception, match="reduced form"):
+        ctx.setPeriodicBoxVectors((3.1, 0, 0), (1.6, 3.1, 0), (0, 0, 3.1))
