@@ -648,6 +648,28 @@ def main():
                          t["list_capacity"], t["pair_split"], pair_order=t["pair_order"])
         batch.compute()
 
+    tri_eps = tri_stage = None
+    if not args.no_legs:
+        # the same replicas in a triclinic box of the same edges (tilts (9, -7, 12) lattice
+        # spacings): tile kernel, every column a list candidate, x + d binned in full
+        log("[bench] triclinic boxes")
+        t = batch.tuning()
+        tri = ATMBatch([W.triclinic(r, sites=(9, -7, 12)) for r in reps])
+        tri.set_tuning(t["column_scale"], t["pair_waves_per_block"], t["zbins_per_column"], 0,
+                       pair_split=t["pair_split"], pair_order=t["pair_order"])
+        tri.compute()
+        for _ in range(args.warmup):
+            tri.evaluate(forces=True, clear=True)
+        tri.synchronize()
+        nt = max(10, args.steps // 4)
+        t1 = time.perf_counter()
+        for _ in range(nt):
+            tri.evaluate(forces=True, clear=True)
+        tri.synchronize()
+        tri_eps = args.replicas * nt / (time.perf_counter() - t1)
+        tri_stage = tri.profile(reps=10)
+        tri.close()
+
     log("[bench] per-stage HIP-event timing")
     prof = batch.profile(reps=max(10, args.steps // 10))
     r0 = batch.result(0)
@@ -741,6 +763,11 @@ def main():
                                   "x + d is the difference from x)",
         "state2_mode": state2_mode,
         "full_state2_evals_per_s": round(full_eps, 2) if full_eps else None,
+        "triclinic_evals_per_s": round(tri_eps, 2) if tri_eps else None,
+        "triclinic_stage_ms": {k: round(v, 4) for k, v in tri_stage.items() if k.endswith("_ms")} if tri_eps else None,
+        "triclinic_note": ("the same 22 replicas in triclinic boxes of the same edges (tilts b_x, c_x, c_y = "
+                           "9, -7, 12 lattice spacings): tile kernel, every column a list candidate, x + d "
+                           "binned in full") if tri_eps else None,
         "plain_evals_per_s": round(plain_eps, 2) if plain_eps else None,
         "speedup_vs_plain": round(evals_per_s / plain_eps, 2) if plain_eps else None,
         "fp32_pair_mode_evals_per_s": round(fp32_eps, 2) if fp32_eps else None,

@@ -124,6 +124,29 @@ __device__ __forceinline__ bool tri_boxes_near(const SetDesc& D, float dx, float
     return false;
 }
 
+// A column's x-y rectangle and a group's x-y box (centre difference d, summed
+// half-extents h) come closer than r in x-y under some image n_a a + n_b b +
+// n_c c of a triclinic box with |n_c| <= ncSpan (z is left to the cluster test).
+__device__ __forceinline__ bool tri_rect_near(const SetDesc& D, float dx, float dy, float hx, float hy, int ncSpan,
+                                              float r, float r2) {
+    for (int nc = -ncSpan; nc <= ncSpan; ++nc) {
+        const float dy1 = fmaf(-D.tilt[2], (float)nc, dy), dx1 = fmaf(-D.tilt[1], (float)nc, dx);
+        const int b0 = (int)floorf((dy1 - hy - r) * D.invL[1]), b1 = (int)ceilf((dy1 + hy + r) * D.invL[1]);
+        for (int nb = b0; nb <= b1; ++nb) {
+            const float ay = fmaxf(0.f, fabsf(fmaf(-D.L[1], (float)nb, dy1)) - hy);
+            const float remy = r2 - ay * ay;
+            if (remy <= 0.f) continue;
+            const float dx2 = fmaf(-D.tilt[0], (float)nb, dx1);
+            const int a0 = (int)floorf((dx2 - hx - r) * D.invL[0]), a1 = (int)ceilf((dx2 + hx + r) * D.invL[0]);
+            for (int na = a0; na <= a1; ++na) {
+                const float ax = fmaxf(0.f, fabsf(fmaf(-D.L[0], (float)na, dx2)) - hx);
+                if (ax * ax < remy) return true;
+            }
+        }
+    }
+    return false;
+}
+
 // float -> 64-bit fixed point (round to nearest).  |f * scale| < 2^51 takes the
 // magic-number path (one FP64 FMA + a 64-bit subtract); larger values use the
 // general conversion.
@@ -682,6 +705,23 @@ __global__ void __launch_bounds__(64 * W) k_pairlist(DevPtrs P, int setBegin) {
     const int bl = zall ? 0 : ((blo % D.nz) + D.nz) % D.nz;
     const int bh = zall ? D.nz - 1 : ((bhi % D.nz) + D.nz) % D.nz;
     const bool zwrap = !zall && bl > bh;       // two ranges: [bl, nz) and [0, bh]
+    // triclinic: every column is a candidate; a column whose x-y rectangle is beyond the
+    // list cutoff of the group's x-y box under every image is dropped before its
+    // clusters are tested.  A z-window inside [0, L2) admits only c-images n_c = 0.
+    float gcx = 0.f, gcy = 0.f, ghx = 0.f, ghy = 0.f;
+    const int ncSpan = (zwrap || zall) ? 1 : 0;
+    if (TRI && D.tri) {
+        float xlo = sIc[wave][0].x - sIh[wave][0].x, xhi = sIc[wave][0].x + sIh[wave][0].x;
+        float ylo = sIc[wave][0].y - sIh[wave][0].y, yhi = sIc[wave][0].y + sIh[wave][0].y;
+        for (int q = 1; q < ncl; ++q) {
+            xlo = fminf(xlo, sIc[wave][q].x - sIh[wave][q].x);
+            xhi = fmaxf(xhi, sIc[wave][q].x + sIh[wave][q].x);
+            ylo = fminf(ylo, sIc[wave][q].y - sIh[wave][q].y);
+            yhi = fmaxf(yhi, sIc[wave][q].y + sIh[wave][q].y);
+        }
+        gcx = 0.5f * (xlo + xhi); gcy = 0.5f * (ylo + yhi);
+        ghx = 0.5f * (xhi - xlo); ghy = 0.5f * (yhi - ylo);
+    }
     for (int cb = 0; cb < ncand; cb += 64) {
         // this lane's candidate column: up to two cluster ranges (a, b), shift codes
         int cnt_l = 0, c0_l = 0, codes_l = 0, cntb_l = 0, c0b_l = 0;
@@ -702,7 +742,14 @@ __global__ void __launch_bounds__(64 * W) k_pairlist(DevPtrs P, int setBegin) {
                 if (jy < 0) { jy += D.ny; syc = 2; } else if (jy >= D.ny) { jy -= D.ny; syc = 1; }
                 gapy = fmaxf(0.f, (float)(abs(oy) - 1)) * D.ce[1];
             }
-            if (gapx * gapx + gapy * gapy < D.rcList2) {
+            bool colNear = gapx * gapx + gapy * gapy < D.rcList2;
+            if (TRI && D.tri) {
+                // the column's rectangle, 1e-3 nm wider on each side than its edges (binning rounding)
+                const float hx = ghx + 0.5f * D.ce[0] + 1e-3f, hy = ghy + 0.5f * D.ce[1] + 1e-3f;
+                colNear = tri_rect_near(D, ((float)jx + 0.5f) * D.ce[0] - gcx, ((float)jy + 0.5f) * D.ce[1] - gcy,
+                                        hx, hy, ncSpan, rcl, D.rcList2);
+            }
+            if (colNear) {
                 const int colJ = jx * D.ny + jy;
                 const int cs = P.colStart[D.colOff + colJ], cnt = P.colCount[D.colOff + colJ];
                 const int* cst = P.cellStart + D.cellOff + (size_t)colJ * D.nz;
@@ -1184,7 +1231,7 @@ __global__ void __launch_bounds__(64 * W, 6) k_pairs(DevPtrs P, int setBegin, co
                     // exact: the squares of 2^-20-grid differences fit in 53 bits
                     const double r2 = fma((double)dx, (double)dx, fma((double)dy, (double)dy, (double)dz * (double)dz));
                     uint64_t inLanes = __builtin_amdgcn_ballot_w64(r2 < rc2) & candLanes;
-                    if (EXC) inLanes &= ~excepted(t, b.idx, inLanes);
+                    if (EXC && P.xmask) inLanes &= ~excepted(t, b.idx, inLanes);
                     npair += (uint32_t)__popcll(inLanes);
                     // lanes outside the cutoff are masked off (exec), not zeroed with selects
                     fs = 0.f;
@@ -1220,7 +1267,7 @@ __global__ void __launch_bounds__(64 * W, 6) k_pairs(DevPtrs P, int setBegin, co
                     const float2 qi = sPrm[wave][t * kClusterSize + ii];
                     const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
                     uint64_t inLanes = __builtin_amdgcn_ballot_w64(in_cutoff(D, dx, dy, dz, r2)) & okLanes;
-                    if (EXC && inLanes != 0ull) inLanes &= ~excepted(t, b.idx, inLanes);
+                    if (EXC && P.xmask && inLanes != 0ull) inLanes &= ~excepted(t, b.idx, inLanes);
                     if (inLanes == 0ull) continue;
                     npair += (uint32_t)__popcll(inLanes);
                     const bool in = __builtin_amdgcn_inverse_ballot_w64(inLanes);
