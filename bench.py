@@ -651,7 +651,7 @@ def main():
     tri_eps = tri_stage = None
     if not args.no_legs:
         # the same replicas in a triclinic box of the same edges (tilts (9, -7, 12) lattice
-        # spacings): tile kernel, every column a list candidate, x + d binned in full
+        # spacings): tile kernel, every column a list candidate
         log("[bench] triclinic boxes")
         t = batch.tuning()
         tri = ATMBatch([W.triclinic(r, sites=(9, -7, 12)) for r in reps])
@@ -766,8 +766,8 @@ def main():
         "triclinic_evals_per_s": round(tri_eps, 2) if tri_eps else None,
         "triclinic_stage_ms": {k: round(v, 4) for k, v in tri_stage.items() if k.endswith("_ms")} if tri_eps else None,
         "triclinic_note": ("the same 22 replicas in triclinic boxes of the same edges (tilts b_x, c_x, c_y = "
-                           "9, -7, 12 lattice spacings): tile kernel, every column a list candidate, x + d "
-                           "binned in full") if tri_eps else None,
+                           "9, -7, 12 lattice spacings): tile kernel, every column a list candidate, x + d as "
+                           "the difference from x at the displaced atoms") if tri_eps else None,
         "plain_evals_per_s": round(plain_eps, 2) if plain_eps else None,
         "speedup_vs_plain": round(evals_per_s / plain_eps, 2) if plain_eps else None,
         "fp32_pair_mode_evals_per_s": round(fp32_eps, 2) if fp32_eps else None,

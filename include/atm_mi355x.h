@@ -231,7 +231,7 @@ int atm_set_box(void* context, int32_t sys, const double* box3);
  * ax >= 2|cx|, by >= 2|cy|) with cutoff <= half of ax, by and cz.  A pair takes
  * the image OpenMM's triclinic rule picks (shift by c to the nearest z, then by b
  * to the nearest y, then by a to the nearest x).  A batch with a triclinic system
- * runs the 8x8 tile kernel and bins x + d in full (no state difference). */
+ * runs the 8x8 tile kernel. */
 int atm_set_box_vectors(void* context, int32_t sys, const double* vectors);
 
 /* The nine ATM global parameters of system `sys` (read by execute,

@@ -485,7 +485,6 @@ void update_displaced(Context& C) {
         const int nl = (int)C.ligAtoms.size() - lb;
         C.maxLig = std::max(C.maxLig, nl);
         if (4ll * nl > n) C.deltaOK = false;
-        if (C.sets[2 * s].tri) C.deltaOK = false;  // the displaced-atom kernel takes orthorhombic images only
         const int xb = (int)C.lxidx.size();
         if (s < (int)C.exc.size()) {
             const auto& X = C.exc[s];

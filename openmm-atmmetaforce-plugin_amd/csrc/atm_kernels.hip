@@ -2135,9 +2135,13 @@ __device__ __forceinline__ float4 displaced_position(const DevPtrs& P, const Set
         const float4 d = P.displ[D.atomOff + a];
         p.x += d.x; p.y += d.y; p.z += d.z;  // exact on the 2^-20 grid
     }
-    p.x = wrap_coord(p.x, D.L[0], D.invL[0]);
-    p.y = wrap_coord(p.y, D.L[1], D.invL[1]);
-    p.z = wrap_coord(p.z, D.L[2], D.invL[2]);
+    if (D.tri) {
+        wrap_tri(D, p);
+    } else {
+        p.x = wrap_coord(p.x, D.L[0], D.invL[0]);
+        p.y = wrap_coord(p.y, D.L[1], D.invL[1]);
+        p.z = wrap_coord(p.z, D.L[2], D.invL[2]);
+    }
     return p;
 }
 
@@ -2193,9 +2197,14 @@ __global__ void __launch_bounds__(256) k_displaced(DevPtrs P, int sysBegin, int 
     uint32_t npair = 0;
     // one candidate: b at pb with FP64 parameters pbq; nb: slot or atom for dF
     auto pair = [&](int b, float4 pb, double4 pbq) __attribute__((always_inline)) {
-        const float dx = min_image(p.x - pb.x, D.L[0], D.invL[0]);
-        const float dy = min_image(p.y - pb.y, D.L[1], D.invL[1]);
-        const float dz = min_image(p.z - pb.z, D.L[2], D.invL[2]);
+        float dx = p.x - pb.x, dy = p.y - pb.y, dz = p.z - pb.z;
+        if (D.tri) {
+            tri_image_xy(D, dx, dy, dz);  // OpenMM's triclinic image (z below)
+        } else {
+            dx = min_image(dx, D.L[0], D.invL[0]);
+            dy = min_image(dy, D.L[1], D.invL[1]);
+        }
+        dz = min_image(dz, D.L[2], D.invL[2]);
         const double r2 = fma((double)dx, (double)dx, fma((double)dy, (double)dy, (double)dz * (double)dz));
         bool in = r2 < D.rc2;
         if (EXC && in) in = !excluded_pair(P, D, a, b, ma);
@@ -2252,11 +2261,21 @@ __global__ void __launch_bounds__(256) k_displaced(DevPtrs P, int sysBegin, int 
             ccx += ccx < 0 ? D.nx : (ccx >= D.nx ? -D.nx : 0);
             int ccy = nys == D.ny ? iy : cy - ry + iy;
             ccy += ccy < 0 ? D.ny : (ccy >= D.ny ? -D.ny : 0);
-            const float ex = fmaxf(fabsf(min_image(p.x - ((float)ccx + 0.5f) * D.ce[0], D.L[0], D.invL[0])) -
-                                       0.5f * D.ce[0], 0.f);
-            const float ey = fmaxf(fabsf(min_image(p.y - ((float)ccy + 0.5f) * D.ce[1], D.L[1], D.invL[1])) -
-                                       0.5f * D.ce[1], 0.f);
-            const float rho2 = ex * ex + ey * ey;
+            float rho2;
+            if (D.tri) {
+                // triclinic (every column enumerated): the column's rectangle within lim of the atom
+                // under some lattice image (1e-3 nm of binning slack), z-window of the full lim
+                const bool nearCol = tri_rect_near(D, ((float)ccx + 0.5f) * D.ce[0] - p.x,
+                                                   ((float)ccy + 0.5f) * D.ce[1] - p.y, 0.5f * D.ce[0] + 1e-3f,
+                                                   0.5f * D.ce[1] + 1e-3f, 1, lim, lim2);
+                rho2 = nearCol ? 0.f : 2.f * lim2;
+            } else {
+                const float ex = fmaxf(fabsf(min_image(p.x - ((float)ccx + 0.5f) * D.ce[0], D.L[0], D.invL[0])) -
+                                           0.5f * D.ce[0], 0.f);
+                const float ey = fmaxf(fabsf(min_image(p.y - ((float)ccy + 0.5f) * D.ce[1], D.L[1], D.invL[1])) -
+                                           0.5f * D.ce[1], 0.f);
+                rho2 = ex * ex + ey * ey;
+            }
             if (rho2 <= lim2) {
                 const int col = ccx * D.ny + ccy;
                 const int cs = P.colStart[D.colOff + col];
@@ -2306,10 +2325,14 @@ __global__ void __launch_bounds__(256) k_displaced(DevPtrs P, int sysBegin, int 
                 const int l0u = sL0[wave][lo];
                 cl = off < l0u ? sB0[wave][lo] + off : sB1[wave][lo] + (off - l0u);
                 const float4 bc = P.bbc[D.clusterOff + cl], bh = P.bbh[D.clusterOff + cl];
-                const float dx = fmaxf(fabsf(min_image(p.x - bc.x, D.L[0], D.invL[0])) - bh.x, 0.f);
-                const float dy = fmaxf(fabsf(min_image(p.y - bc.y, D.L[1], D.invL[1])) - bh.y, 0.f);
-                const float dz = fmaxf(fabsf(min_image(p.z - bc.z, D.L[2], D.invL[2])) - bh.z, 0.f);
-                keep = dx * dx + dy * dy + dz * dz <= lim2;
+                if (D.tri) {
+                    keep = tri_boxes_near(D, bc.x - p.x, bc.y - p.y, bc.z - p.z, bh.x, bh.y, bh.z, lim, lim2);
+                } else {
+                    const float dx = fmaxf(fabsf(min_image(p.x - bc.x, D.L[0], D.invL[0])) - bh.x, 0.f);
+                    const float dy = fmaxf(fabsf(min_image(p.y - bc.y, D.L[1], D.invL[1])) - bh.y, 0.f);
+                    const float dz = fmaxf(fabsf(min_image(p.z - bc.z, D.L[2], D.invL[2])) - bh.z, 0.f);
+                    keep = dx * dx + dy * dy + dz * dz <= lim2;
+                }
             }
             const uint64_t km = __builtin_amdgcn_ballot_w64(keep);
             const int nk = __popcll(km);

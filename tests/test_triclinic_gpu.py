@@ -126,3 +126,23 @@ def test_openmm_api_triclinic_context(gpu):
         assert rel(f.getPerturbationEnergy(ctx), ref["usc"]) < 1e-6
     with pytest.raises(Exception, match="reduced form"):
         ctx.setPeriodicBoxVectors((3.1, 0, 0), (1.6, 3.1, 0), (0, 0, 3.1))
+
+
+def test_triclinic_state_difference_and_full_state2(gpu):
+    """x + d as the difference from x at the displaced atoms (the default: the
+    displaced-atom kernel sweeps the x cells over the triclinic images) and x + d
+    binned and paired in full both match the oracle, pair sets included."""
+    s = W.triclinic(W.with_molecules(W.config1(seed=6), size=3, crosslink_every=11), sites=(-5, -6, 4))
+    with ATMBatch([s]) as b:
+        b.compute()
+        ls = b.list_stats()
+        assert ls["max_entries"][1] == 0, "x + d should be the difference from x"
+        r_d, _ = check_system(b, 0, s, tag="tri difference")
+        t = b.tuning()
+        b.set_tuning(t["column_scale"], t["pair_waves_per_block"], t["zbins_per_column"], 0, t["list_capacity"],
+                     t["pair_split"], full_state2=True, pair_order=t["pair_order"])
+        b.compute()
+        assert b.list_stats()["max_entries"][1] > 0
+        r_f, _ = check_system(b, 0, s, tag="tri full")
+        for key in ("state1_energy", "state2_energy", "energy"):
+            assert abs(r_d[key] - r_f[key]) <= 1e-11 * abs(r_f[key])
