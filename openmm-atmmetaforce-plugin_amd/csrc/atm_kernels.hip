@@ -1109,7 +1109,7 @@ __global__ void __launch_bounds__(64 * W, 6) k_pairs(DevPtrs P, int setBegin, co
     const float Lx = D.L[0], invLx = D.invL[0], Ly = D.L[1], invLy = D.invL[1];
     const bool shiftOK = D.shiftOK;
     const double rc2 = D.rc2, krf64 = D.krf64, crf64 = D.crf64;
-    const float krf = D.krf, crf = D.crf, rc2hi = D.rc2hi, fixMax = D.fixMax;
+    const float krf = D.krf, crf = D.crf, fixMax = D.fixMax;
     const double wgt = BWD ? (wset ? wset[c] : 1.0) : 0.0;  // no weights: dU/dp of each set
     float fi[kGroupClusters][3];
 #pragma unroll
@@ -1218,19 +1218,17 @@ __global__ void __launch_bounds__(64 * W, 6) k_pairs(DevPtrs P, int setBegin, co
                 dz = min_image(dz, Lz, invLz);
                 float fs;
                 if (F64) {
-                    // FP32 r^2 with a margin: a superset of the in-cutoff pairs, decides the early-out.
                     // The lane masks stay in scalar registers (ballot / inverse ballot).
-                    const float r2f = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
                     // lanes whose i-atom (t, ii) is real: byte t of ivLanes repeated over the 8 jj
                     uint64_t iv = (ivLanes >> (t * kClusterSize)) & 0xFFull;
                     iv |= iv << 8; iv |= iv << 16; iv |= iv << 32;
                     const uint64_t okLanes = (t == tdiag ? upperLanes : ~0ull) & jvLanes & iv;
-                    const uint64_t candLanes = __builtin_amdgcn_ballot_w64(r2f < rc2hi) & okLanes;
-                    if (candLanes == 0ull) continue;  // no pair of this 8x8 step is inside
-                    const double4 id = sPrm64[wave][t * kClusterSize + ii];
-                    // exact: the squares of 2^-20-grid differences fit in 53 bits
+                    // exact: the squares of 2^-20-grid differences fit in 53 bits; decides the
+                    // in-cutoff lanes and the early-out of a step with none
                     const double r2 = fma((double)dx, (double)dx, fma((double)dy, (double)dy, (double)dz * (double)dz));
-                    uint64_t inLanes = __builtin_amdgcn_ballot_w64(r2 < rc2) & candLanes;
+                    uint64_t inLanes = __builtin_amdgcn_ballot_w64(r2 < rc2) & okLanes;
+                    if (inLanes == 0ull) continue;  // no pair of this 8x8 step is inside
+                    const double4 id = sPrm64[wave][t * kClusterSize + ii];
                     if (EXC && P.xmask) inLanes &= ~excepted(t, b.idx, inLanes);
                     npair += (uint32_t)__popcll(inLanes);
                     // lanes outside the cutoff are masked off (exec), not zeroed with selects
