@@ -28,6 +28,7 @@
 // + k_e q_i q_j (1/r + k_rf r^2 - c_rf), pairs with r < rc under minimum image).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <type_traits>
 #include "atm_internal.h"
 #include "atm_kernels.h"
 #include "atm_math.h"
@@ -1065,6 +1066,7 @@ __global__ void __launch_bounds__(64 * W, 6) k_pairs(DevPtrs P, int setBegin, co
     __shared__ int sAi[W][EXC ? 64 : 1];                   // i-atom indices (system-local, -1: dummy)
     __shared__ unsigned long long sXm[W][EXC ? 64 : 1];    // their near exception masks
     int g0 = 0, ncl = 0;
+    float zlo = 3.0e38f, zhi = -3.0e38f;  // z-range of the group's real i-atoms
     uint32_t ivm = 0;       // bit t: i-atom (t, ii) is a real atom
     uint64_t ivLanes = 0;   // bit 8 t + ii: the same, over the group's 64 slots
     if (active) {
@@ -1091,6 +1093,12 @@ __global__ void __launch_bounds__(64 * W, 6) k_pairs(DevPtrs P, int setBegin, co
         }
         sPos[wave][lane] = p;
         if (F64) sPrm64[wave][lane] = qd; else sPrm[wave][lane] = q;
+        zlo = v ? p.z : zlo;
+        zhi = v ? p.z : zhi;
+        for (int o = 32; o > 0; o >>= 1) {
+            zlo = fminf(zlo, __shfl_xor(zlo, o, 64));
+            zhi = fmaxf(zhi, __shfl_xor(zhi, o, 64));
+        }
         const uint64_t vb = __ballot(v);
         ivLanes = vb;
 #pragma unroll
@@ -1180,7 +1188,11 @@ __global__ void __launch_bounds__(64 * W, 6) k_pairs(DevPtrs P, int setBegin, co
         }
         wave_sync();  // read before the next entries overwrite them
     };
-    auto do_entry = [&](uint32_t ent, const JBuf& b) __attribute__((always_inline)) {
+    // zdir: every i-atom lies at least rc inside [0, L_z), so the in-cutoff image of any j-atom
+    // (wrapped into [0, L_z)) is the unshifted one in z: a j-atom within rc of an i-atom has
+    // z in [z_i - rc, z_i + rc], inside [0, L_z), where no other z-image of it lies; pairs
+    // outside the cutoff stay outside at the unshifted distance.  Such groups skip the z image.
+    auto do_entry = [&](uint32_t ent, const JBuf& b, auto zdir) __attribute__((always_inline)) {
         const int J = ent & 0xFFFFF;
         const uint32_t imask = (ent >> 20) & 0xFFu;
         const int js = D.slotOff + J * kClusterSize + jj;
@@ -1215,7 +1227,7 @@ __global__ void __launch_bounds__(64 * W, 6) k_pairs(DevPtrs P, int setBegin, co
                         dy = min_image(dy, Ly, invLy);
                     }
                 }
-                dz = min_image(dz, Lz, invLz);
+                if constexpr (!decltype(zdir)::value) dz = min_image(dz, Lz, invLz);
                 float fs;
                 if (F64) {
                     // The lane masks stay in scalar registers (ballot / inverse ballot).
@@ -1296,17 +1308,24 @@ __global__ void __launch_bounds__(64 * W, 6) k_pairs(DevPtrs P, int setBegin, co
     // list end the batch holds entry 0 (cluster 0, empty i-mask): no pair step
     // runs and its j sums are zero; the load after a batch's last pair wraps to
     // lane 0 and is not used.
-    for (int e0 = 0; e0 < ne; e0 += 64) {
-        const uint32_t ebuf = (e0 + lane < ne) ? E[e0 + lane] : 0u;
-        const int nb = min(64, ne - e0);
-        JBuf bufA = load_jbuf(__builtin_amdgcn_readlane(ebuf, 0)), bufB;
-        for (int k = 0; k < nb; k += 2) {
-            bufB = load_jbuf(__builtin_amdgcn_readlane(ebuf, k + 1));
-            do_entry(__builtin_amdgcn_readlane(ebuf, k), bufA);
-            bufA = load_jbuf(__builtin_amdgcn_readlane(ebuf, (k + 2) & 63));
-            do_entry(__builtin_amdgcn_readlane(ebuf, k + 1), bufB);
+    auto entries = [&](auto zdir) __attribute__((always_inline)) {
+        for (int e0 = 0; e0 < ne; e0 += 64) {
+            const uint32_t ebuf = (e0 + lane < ne) ? E[e0 + lane] : 0u;
+            const int nb = min(64, ne - e0);
+            JBuf bufA = load_jbuf(__builtin_amdgcn_readlane(ebuf, 0)), bufB;
+            for (int k = 0; k < nb; k += 2) {
+                bufB = load_jbuf(__builtin_amdgcn_readlane(ebuf, k + 1));
+                do_entry(__builtin_amdgcn_readlane(ebuf, k), bufA, zdir);
+                bufA = load_jbuf(__builtin_amdgcn_readlane(ebuf, (k + 2) & 63));
+                do_entry(__builtin_amdgcn_readlane(ebuf, k + 1), bufB, zdir);
+            }
         }
-    }
+    };
+    // explicit x/y shifts (shiftOK) and the i-atoms' z-range at least the cutoff (+ the list
+    // margin) inside the box in z; the FP32 D.rc carries its rounding margin
+    const float rcz = D.rc * (1.0f + 1e-5f) + 1e-4f;
+    const bool zDirect = shiftOK && __builtin_amdgcn_readfirstlane((int)(zlo - rcz >= 0.f && zhi + rcz < Lz)) != 0;
+    if (zDirect) entries(std::true_type{}); else entries(std::false_type{});
     if (JSUMS && (ecount & (kJFlush - 1)) != 0) flush_j(ecount & (kJFlush - 1));
     if (JSUMS) {
 #pragma unroll

@@ -146,3 +146,48 @@ def test_triclinic_state_difference_and_full_state2(gpu):
         r_f, _ = check_system(b, 0, s, tag="tri full")
         for key in ("state1_energy", "state2_energy", "energy"):
             assert abs(r_d[key] - r_f[key]) <= 1e-11 * abs(r_f[key])
+
+
+def test_triclinic_parameter_gradients_against_oracle_differences(gpu):
+    """dW/d(q, sigma, epsilon, lambda1) from the backward pass in a triclinic box
+    against central differences of the FP64 oracle's W (oracle_atm_eval_tri), on
+    ligand atoms and on atoms at the faces of the box, whose pairs take lattice
+    images with tilts."""
+    import torch
+    from atmmetaforce import ATMEnergyFunction
+
+    p = W.ATMParams(0.2, 0.45, 0.1, 110.0, 0.5, 200.0, 100.0, 0.0625, 1.0)
+    s = W.triclinic(W.tiny_system(800, 2.9, 0.9, 10, (0.9, 0.8, 0.7), 31, p), sites=(3, -2, 4))
+    dev = torch.device("cuda:0")
+    q = torch.tensor(s.charges, device=dev, requires_grad=True)
+    sg = torch.tensor(s.sigmas, device=dev, requires_grad=True)
+    ep = torch.tensor(s.epsilons, device=dev, requires_grad=True)
+    sc = [torch.tensor([getattr(p, n)], device=dev, dtype=torch.float64, requires_grad=True)
+          for n in ("lambda1", "lambda2", "alpha", "u0", "w0")]
+    with ATMBatch([s]) as b:
+        Wt = ATMEnergyFunction.apply(b, q, sg, ep, *sc)
+        Wt.sum().backward()
+    W0 = oracle.atm_eval(s, forces=False)[0]["W"]
+    assert Wt.item() == pytest.approx(W0, rel=1e-9)
+    z, x = s.positions[:, 2], s.positions[:, 0]
+    atoms = [0, 4, 9] + list(np.argsort(z)[-2:]) + list(np.argsort(x)[:2])
+
+    def w_at(**kw):
+        return oracle.atm_eval(W.System(**{**s.__dict__, **kw}), forces=False)[0]["W"]
+
+    for name, arr, grad, h in (("charges", s.charges, q.grad, 1e-5), ("sigmas", s.sigmas, sg.grad, 1e-6),
+                               ("epsilons", s.epsilons, ep.grad, 1e-5)):
+        g = grad.cpu().numpy()
+        scale = np.abs(g).max()
+        for a in atoms:
+            up, dn = arr.copy(), arr.copy()
+            up[a] += h
+            dn[a] -= h
+            fd = (w_at(**{name: up}) - w_at(**{name: dn})) / (2 * h)
+            print(f"{name}[{a}]: backward {g[a]:.8g} vs oracle differences {fd:.8g}", flush=True)
+            assert abs(g[a] - fd) <= 2e-5 * scale, (name, a)
+    h = 1e-6
+    import dataclasses
+    fd = (w_at(params=dataclasses.replace(p, lambda1=p.lambda1 + h))
+          - w_at(params=dataclasses.replace(p, lambda1=p.lambda1 - h))) / (2 * h)
+    assert sc[0].grad.item() == pytest.approx(fd, rel=1e-5, abs=1e-6)
