@@ -127,10 +127,12 @@ __device__ __forceinline__ bool tri_boxes_near(const SetDesc& D, float dx, float
 
 // A column's x-y rectangle and a group's x-y box (centre difference d, summed
 // half-extents h) come closer than r in x-y under some image n_a a + n_b b +
-// n_c c of a triclinic box with |n_c| <= ncSpan (z is left to the cluster test).
+// n_c c of a triclinic box with |n_c| <= ncSpan, n_c = 0 excluded when skip0
+// (z is left to the cluster test).
 __device__ __forceinline__ bool tri_rect_near(const SetDesc& D, float dx, float dy, float hx, float hy, int ncSpan,
-                                              float r, float r2) {
+                                              float r, float r2, bool skip0 = false) {
     for (int nc = -ncSpan; nc <= ncSpan; ++nc) {
+        if (skip0 && nc == 0) continue;
         const float dy1 = fmaf(-D.tilt[2], (float)nc, dy), dx1 = fmaf(-D.tilt[1], (float)nc, dx);
         const int b0 = (int)floorf((dy1 - hy - r) * D.invL[1]), b1 = (int)ceilf((dy1 + hy + r) * D.invL[1]);
         for (int nb = b0; nb <= b1; ++nb) {
@@ -744,11 +746,21 @@ __global__ void __launch_bounds__(64 * W) k_pairlist(DevPtrs P, int setBegin) {
                 gapy = fmaxf(0.f, (float)(abs(oy) - 1)) * D.ce[1];
             }
             bool colNear = gapx * gapx + gapy * gapy < D.rcList2;
+            // triclinic, window wrapped at one end: the bins of the window inside [0, L_z) pair at
+            // n_c = 0 only, the bins past the end (taken from the other end of the column) at
+            // n_c = +-1 only; each range is kept when the column is near under its own images
+            bool keepDirect = true, keepWrapped = true;
             if (TRI && D.tri) {
                 // the column's rectangle, 1e-3 nm wider on each side than its edges (binning rounding)
                 const float hx = ghx + 0.5f * D.ce[0] + 1e-3f, hy = ghy + 0.5f * D.ce[1] + 1e-3f;
-                colNear = tri_rect_near(D, ((float)jx + 0.5f) * D.ce[0] - gcx, ((float)jy + 0.5f) * D.ce[1] - gcy,
-                                        hx, hy, ncSpan, rcl, D.rcList2);
+                const float cdx = ((float)jx + 0.5f) * D.ce[0] - gcx, cdy = ((float)jy + 0.5f) * D.ce[1] - gcy;
+                if (zwrap) {
+                    keepDirect = tri_rect_near(D, cdx, cdy, hx, hy, 0, rcl, D.rcList2);
+                    keepWrapped = tri_rect_near(D, cdx, cdy, hx, hy, 1, rcl, D.rcList2, true);
+                    colNear = keepDirect || keepWrapped;
+                } else {
+                    colNear = tri_rect_near(D, cdx, cdy, hx, hy, ncSpan, rcl, D.rcList2);
+                }
             }
             if (colNear) {
                 const int colJ = jx * D.ny + jy;
@@ -772,10 +784,17 @@ __global__ void __launch_bounds__(64 * W) k_pairlist(DevPtrs P, int setBegin) {
                     } else {
                         range(bl, D.nz - 1, c0_l, cnt_l);
                         range(0, bh, c0b_l, cntb_l);
+                        // the window runs past the top (bhi >= nz: [bl, nz) is its direct part) or
+                        // below 0 ([0, bh] is); a range whose images are not near is dropped
+                        const bool topDirect = bhi >= D.nz;
+                        const bool keepTop = topDirect ? keepDirect : keepWrapped;
+                        const bool keepBottom = topDirect ? keepWrapped : keepDirect;
+                        if (!keepTop) cnt_l = 0;
+                        if (!keepBottom) cntb_l = 0;
                         // a cluster straddling the empty bins between the two windows
                         // would fall in both ranges: the top range starts after the bottom one
                         const int endb = c0b_l + cntb_l;
-                        if (cntb_l > 0 && c0_l < endb) {
+                        if (cntb_l > 0 && cnt_l > 0 && c0_l < endb) {
                             cnt_l = max(0, c0_l + cnt_l - endb);
                             c0_l = endb;
                         }

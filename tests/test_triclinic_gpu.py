@@ -191,3 +191,15 @@ def test_triclinic_parameter_gradients_against_oracle_differences(gpu):
     fd = (w_at(params=dataclasses.replace(p, lambda1=p.lambda1 + h))
           - w_at(params=dataclasses.replace(p, lambda1=p.lambda1 - h))) / (2 * h)
     assert sc[0].grad.item() == pytest.approx(fd, rel=1e-5, abs=1e-6)
+
+
+@pytest.mark.parametrize("variant,split", [(1, 1), (5, 1), (6, 1), (0, 4)])
+def test_triclinic_runs_the_tile_kernel_for_every_variant(gpu, variant, split):
+    """A tuning that asks for another pair-kernel variant (compacted pairs, atom
+    lists, filtered queues: orthorhombic images only) runs the tile kernel in a
+    batch with a triclinic system; list splits apply as usual."""
+    s = tri_config1()
+    with ATMBatch([s]) as b:
+        b.set_tuning(1.3, 2, 4, variant, pair_split=split)
+        b.compute()
+        check_system(b, 0, s, tag=f"tri variant {variant} split {split}")
