@@ -72,14 +72,19 @@ __device__ __forceinline__ void wrap_tri(const SetDesc& D, float4& p) {
 // The x and y parts of OpenMM's triclinic image of a pair difference (shift by c
 // to the nearest z, then by b to the nearest y, then by a to the nearest x); the
 // caller takes dz = min_image(dz, L2), which applies the same c shift to z.
-__device__ __forceinline__ void tri_image_xy(const SetDesc& D, float& dx, float& dy, float dz) {
-    const float nc = rintf(dz * D.invL[2]);
-    dy = fmaf(-D.tilt[2], nc, dy);
-    dx = fmaf(-D.tilt[1], nc, dx);
+// The b and a steps of the rule (what is left when the c image is n_c = 0).
+__device__ __forceinline__ void tri_image_ab(const SetDesc& D, float& dx, float& dy) {
     const float nb = rintf(dy * D.invL[1]);
     dy = fmaf(-D.L[1], nb, dy);
     dx = fmaf(-D.tilt[0], nb, dx);
     dx = min_image(dx, D.L[0], D.invL[0]);
+}
+
+__device__ __forceinline__ void tri_image_xy(const SetDesc& D, float& dx, float& dy, float dz) {
+    const float nc = rintf(dz * D.invL[2]);
+    dy = fmaf(-D.tilt[2], nc, dy);
+    dx = fmaf(-D.tilt[1], nc, dx);
+    tri_image_ab(D, dx, dy);
 }
 
 // The image a pair kernel takes for a difference vector: explicit column shifts
@@ -1210,7 +1215,8 @@ __global__ void __launch_bounds__(64 * W, 6) k_pairs(DevPtrs P, int setBegin, co
     // zdir: every i-atom lies at least rc inside [0, L_z), so the in-cutoff image of any j-atom
     // (wrapped into [0, L_z)) is the unshifted one in z: a j-atom within rc of an i-atom has
     // z in [z_i - rc, z_i + rc], inside [0, L_z), where no other z-image of it lies; pairs
-    // outside the cutoff stay outside at the unshifted distance.  Such groups skip the z image.
+    // outside the cutoff stay outside at the unshifted distance.  Such groups skip the z image
+    // (and, in a triclinic box, the c step of the image rule: n_c = 0 for every in-cutoff pair).
     auto do_entry = [&](uint32_t ent, const JBuf& b, auto zdir) __attribute__((always_inline)) {
         const int J = ent & 0xFFFFF;
         const uint32_t imask = (ent >> 20) & 0xFFu;
@@ -1240,7 +1246,9 @@ __global__ void __launch_bounds__(64 * W, 6) k_pairs(DevPtrs P, int setBegin, co
                 float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
                 if (!shiftOK) {
                     if (EXC && D.tri) {
-                        tri_image_xy(D, dx, dy, dz);
+                        // z-direct groups: the in-cutoff image has n_c = 0
+                        if constexpr (decltype(zdir)::value) tri_image_ab(D, dx, dy);
+                        else tri_image_xy(D, dx, dy, dz);
                     } else {
                         dx = min_image(dx, Lx, invLx);
                         dy = min_image(dy, Ly, invLy);
@@ -1340,10 +1348,12 @@ __global__ void __launch_bounds__(64 * W, 6) k_pairs(DevPtrs P, int setBegin, co
             }
         }
     };
-    // explicit x/y shifts (shiftOK) and the i-atoms' z-range at least the cutoff (+ the list
-    // margin) inside the box in z; the FP32 D.rc carries its rounding margin
+    // the i-atoms' z-range at least the cutoff (+ the list margin) inside the box in z; the
+    // FP32 D.rc carries its rounding margin.  Without EXC only sets with explicit x/y shifts
+    // take the z-direct loop (that loop then carries no x/y image code: fewer registers)
     const float rcz = D.rc * (1.0f + 1e-5f) + 1e-4f;
-    const bool zDirect = shiftOK && __builtin_amdgcn_readfirstlane((int)(zlo - rcz >= 0.f && zhi + rcz < Lz)) != 0;
+    const bool zDirect = (EXC || shiftOK) &&
+                         __builtin_amdgcn_readfirstlane((int)(zlo - rcz >= 0.f && zhi + rcz < Lz)) != 0;
     if (zDirect) entries(std::true_type{}); else entries(std::false_type{});
     if (JSUMS && (ecount & (kJFlush - 1)) != 0) flush_j(ecount & (kJFlush - 1));
     if (JSUMS) {
