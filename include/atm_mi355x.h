@@ -230,7 +230,8 @@ int atm_set_box(void* context, int32_t sys, const double* box3);
  * OpenMM's reduced form (ay = az = bz = 0, positive diagonal, ax >= 2|bx|,
  * ax >= 2|cx|, by >= 2|cy|) with cutoff <= half of ax, by and cz.  A pair takes
  * the image OpenMM's triclinic rule picks (shift by c to the nearest z, then by b
- * to the nearest y, then by a to the nearest x).  A batch with a triclinic system
+ * to the nearest y, then by a to the nearest x), in FP32 on the device: with the
+ * positions and the box vectors on the 2^-20 nm grid every shift is exact.  A batch with a triclinic system
  * runs the 8x8 tile kernel. */
 int atm_set_box_vectors(void* context, int32_t sys, const double* vectors);
 
